@@ -1,0 +1,234 @@
+"""Python surface of the device path, mirroring the reference's plugin interface.
+
+  EulerEqnsFvs          FiniteVolumeSolver + Engine state of one simulation on one GPU
+                        (src/finite_volume_solver.rs:9-17, euler_eqns_fvs.rs, engine.rs)
+  riemann_batch         RiemannFluxSolver::solve_for_flux over a batch (src/riemann_solver.rs:78-87)
+
+Every call goes through the C-ABI of include/hfv.h into lib/libhfv.so; there is no CPU fallback.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+
+EXACT, HLLC, PVRS, AIRS, TSRS, TRRS, LINEAR_ADVECTION = range(7)
+SOLVERS = {"Exact": EXACT, "HLLC": HLLC, "PVRS": PVRS, "AIRS": AIRS, "TSRS": TSRS, "TRRS": TRRS,
+           "LinearAdvection": LINEAR_ADVECTION}
+PERIODIC, REFLECTIVE, OPEN, VACUUM = range(4)
+BOUNDARIES = {"periodic": PERIODIC, "reflective": REFLECTIVE, "open": OPEN, "vacuum": VACUUM}
+FIXED, STEER, STEER_PAKMOR, FLUID = range(4)
+MOTIONS = {"fixed": FIXED, "steer": STEER, "steer_pakmor": STEER_PAKMOR, "fluid": FLUID}
+OPTIMAL_ORDER, GODUNOV = range(2)
+RUNNERS = {"OptimalOrder": OPTIMAL_ORDER, "Godunov": GODUNOV}
+FP64, FP32_FLUX = range(2)
+
+FIELD_WIDTH = {"prim": 5, "grad": 15, "gc": 3, "extrap": 5, "cons": 5, "fluxes": 5, "volume": 1, "loc": 3,
+               "centroid": 3, "v": 3, "v_rel": 3, "maor": 1, "msv": 1, "dt": 1}
+
+
+@dataclass
+class Config:
+    """Solver and engine configuration (the reference's YAML sections hydrodynamics, engine,
+    time_integration, space; src/bin/main.rs:253-606)."""
+    dim: int
+    box: tuple
+    gamma: float
+    cfl: float
+    solver: int = HLLC
+    boundary: int = REFLECTIVE
+    motion: int = FIXED
+    runner: int = OPTIMAL_ORDER
+    sync_all: bool = False
+    t_end: float = 1.0
+    dt_snap: float = 0.0
+    airs_threshold: float = 2.0
+    linadv_velocity: tuple = (0.0, 0.0, 0.0)
+    eos: int = 0
+    isothermal_u: float = 0.0
+    precision: int = FP64
+    extra: dict = field(default_factory=dict)
+
+    def to_c(self):
+        p = _lib.Params()
+        p.dim = self.dim
+        p.boundary = self.boundary
+        for a in range(3):
+            p.box[a] = float(self.box[a])
+            p.linadv_velocity[a] = float(self.linadv_velocity[a])
+        p.gamma = self.gamma
+        p.eos = self.eos
+        p.isothermal_u = self.isothermal_u
+        p.cfl = self.cfl
+        p.solver = self.solver
+        p.airs_threshold = self.airs_threshold
+        p.runner = self.runner
+        p.motion = self.motion
+        p.sync_all = int(self.sync_all)
+        p.t_end = self.t_end
+        p.dt_snap = self.dt_snap
+        p.precision = self.precision
+        return p
+
+
+def _dp(a):
+    return a.ctypes.data_as(_lib.DP)
+
+
+class EulerEqnsFvs:
+    """One simulation's device state: the FiniteVolumeSolver trait surface, the fused runner step and
+    the engine loop."""
+
+    def __init__(self, cfg: Config, mesh=None):
+        self.cfg = cfg
+        self.lib = _lib.hfv()
+        self._params = cfg.to_c()
+        self.ptr = C.c_void_p()
+        _lib.check(self.lib.hfv_create(C.byref(self._params), C.byref(self.ptr)))
+        self.n = 0
+        if mesh is not None:
+            self.set_mesh(mesh)
+
+    def close(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            self.lib.hfv_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    # ------------------------------------------------------------ setup
+    def set_mesh(self, mesh, update_mask=None):
+        m = mesh.arrays_struct()
+        mask = None
+        if update_mask is not None:
+            mask = np.ascontiguousarray(update_mask, dtype=np.uint8)
+        _lib.check(self.lib.hfv_set_mesh(self.ptr, C.byref(m),
+                                         mask.ctypes.data_as(_lib.U8P) if mask is not None else None))
+        self.n = mesh.n_cells
+        self.n_faces = mesh.n_faces
+
+    def init_from_ic(self, loc, mass, vel, u_int):
+        loc = np.ascontiguousarray(loc, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(vel, dtype=np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(mass, dtype=np.float64)
+        u_int = np.ascontiguousarray(u_int, dtype=np.float64)
+        _lib.check(self.lib.hfv_init_from_ic(self.ptr, loc.shape[0], _dp(loc), _dp(mass), _dp(vel), _dp(u_int)))
+
+    def get_state(self, fields=None):
+        n = self.n
+        fields = fields or list(FIELD_WIDTH) + ["timebin"]
+        out = {}
+        st = _lib.State()
+        for k in fields:
+            if k == "timebin":
+                out[k] = np.zeros(n, np.int8)
+                st.timebin = out[k].ctypes.data_as(_lib.I8P)
+            else:
+                out[k] = np.zeros(FIELD_WIDTH[k] * n)
+                setattr(st, k, _dp(out[k]))
+        _lib.check(self.lib.hfv_get_state(self.ptr, C.byref(st)))
+        return out
+
+    def set_state(self, fields):
+        st = _lib.State()
+        keep = {}
+        for k, v in fields.items():
+            if k == "timebin":
+                keep[k] = np.ascontiguousarray(v, dtype=np.int8)
+                st.timebin = keep[k].ctypes.data_as(_lib.I8P)
+            elif k in FIELD_WIDTH:
+                keep[k] = np.ascontiguousarray(v, dtype=np.float64)
+                assert keep[k].size == FIELD_WIDTH[k] * self.n, k
+                setattr(st, k, _dp(keep[k]))
+        _lib.check(self.lib.hfv_set_state(self.ptr, C.byref(st)))
+
+    # ------------------------------------------------------------ trait surface
+    @staticmethod
+    def timestep_info(ti_old, ti_current, time_base, ti_next=0):
+        t = _lib.TimestepInfo()
+        t.ti_old = ti_old
+        t.ti_current = ti_current
+        t.ti_next = ti_next
+        t.time_base = time_base
+        t.time_base_inv = 1.0 / time_base
+        return t
+
+    def predict(self, dt):
+        _lib.check(self.lib.hfv_predict(self.ptr, dt))
+
+    def compute_fluxes(self, info):
+        _lib.check(self.lib.hfv_compute_fluxes(self.ptr, C.byref(info)))
+
+    def get_fluxes(self):
+        out = np.zeros(7 * self.n_faces)
+        _lib.check(self.lib.hfv_get_fluxes(self.ptr, _dp(out)))
+        return out.reshape(-1, 7)
+
+    def apply_fluxes(self):
+        _lib.check(self.lib.hfv_apply_fluxes(self.ptr))
+
+    def convert_conserved_to_primitive(self):
+        _lib.check(self.lib.hfv_conserved_to_primitive(self.ptr))
+
+    def gradients(self):
+        _lib.check(self.lib.hfv_gradients(self.ptr))
+
+    def timesteps(self):
+        t = _lib.U64(0)
+        _lib.check(self.lib.hfv_timesteps(self.ptr, C.byref(t)))
+        return t.value
+
+    # ------------------------------------------------------------ step / engine
+    def step(self, info):
+        t = _lib.U64(0)
+        _lib.check(self.lib.hfv_step(self.ptr, C.byref(info), C.byref(t)))
+        return t.value
+
+    def run(self, max_steps=1 << 62):
+        done = _lib.I64(0)
+        _lib.check(self.lib.hfv_run(self.ptr, int(max_steps), C.byref(done)))
+        return done.value
+
+    def engine_info(self):
+        e = _lib.EngineInfo()
+        _lib.check(self.lib.hfv_get_engine_info(self.ptr, C.byref(e)))
+        d = {k: getattr(e, k) for k, _ in _lib.EngineInfo._fields_ if k != "ts"}
+        d.update({k: getattr(e.ts, k) for k, _ in _lib.TimestepInfo._fields_})
+        return d
+
+    def stats(self):
+        s = _lib.Stats()
+        _lib.check(self.lib.hfv_get_stats(self.ptr, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _lib.Stats._fields_}
+
+    def conservation(self):
+        out = np.zeros(5)
+        _lib.check(self.lib.hfv_conservation(self.ptr, _dp(out)))
+        return out
+
+    def synchronize(self):
+        _lib.check(self.lib.hfv_synchronize(self.ptr))
+
+
+def riemann_batch(cfg: Config, WL, WR, vface, nrm):
+    """RiemannFluxSolver::solve_for_flux for n face problems; arrays are [n,5] / [n,3]."""
+    WL = np.ascontiguousarray(np.asarray(WL, dtype=np.float64).T)
+    WR = np.ascontiguousarray(np.asarray(WR, dtype=np.float64).T)
+    vf = np.ascontiguousarray(np.asarray(vface, dtype=np.float64).T)
+    nr = np.ascontiguousarray(np.asarray(nrm, dtype=np.float64).T)
+    n = WL.shape[1]
+    out = np.zeros((5, n))
+    p = cfg.to_c()
+    _lib.check(_lib.hfv().hfv_riemann_batch(C.byref(p), n, _dp(WL), _dp(WR), _dp(vf), _dp(nr), _dp(out)))
+    return out.T.copy()
+
+
+def device_count():
+    n = C.c_int(0)
+    code = _lib.hfv().hfv_device_count(C.byref(n))
+    return n.value if code == 0 else 0

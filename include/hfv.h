@@ -1,0 +1,226 @@
+/*
+ * hfv — MI355X (gfx950) per-face finite-volume update for moving-mesh Voronoi hydrodynamics.
+ *
+ * C-ABI drop-in boundary for the hot path of the reference crate mvmm_hydro (yuyttenhove/hydro_rs):
+ * the FiniteVolumeSolver trait (src/finite_volume_solver.rs:9-17), the RiemannFluxSolver trait
+ * (src/riemann_solver.rs:78-87) and the runner functions that surround them
+ * (src/runner/hydrodynamics.rs:13-195, src/runner/hydrodynamics/optimal_order.rs:15-64).
+ *
+ * Conventions
+ *   - POD types only; no pointer is retained after a call returns (inputs are copied).
+ *   - Device state lives in library-owned HBM buffers behind an opaque context; one context owns one
+ *     HIP stream.  A context is not re-entrant; distinct contexts may be driven from distinct threads.
+ *   - Return value: HFV_OK (0) or a negative error code.  Negative codes HFV_ERR_NONFINITE ..
+ *     HFV_ERR_BRENT are the reference's hard assert!/panic! classes, raised by the device kernels;
+ *     the reference's soft events (eprintln! + reset) are counted in hfv_stats.
+ *   - Host arrays are structure-of-arrays, component-major: field[c * n + i].
+ */
+#ifndef HFV_H
+#define HFV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- error codes */
+#define HFV_OK 0
+#define HFV_ERR_NONFINITE (-1)    /* src/part.rs:180-182, 202-204, 249-260 */
+#define HFV_ERR_NEG_MASS (-2)     /* src/part.rs:220 */
+#define HFV_ERR_NEG_ENERGY (-3)   /* src/part.rs:221-231 */
+#define HFV_ERR_NEG_PRESSURE (-4) /* src/physical_quantities.rs:220-221, src/part.rs:246-247 */
+#define HFV_ERR_NEWTON (-5)       /* src/riemann_solver/exact.rs:272-274 */
+#define HFV_ERR_BRENT (-6)        /* src/riemann_solver/exact.rs:132-134 */
+#define HFV_ERR_UNSUPPORTED (-7)
+#define HFV_ERR_DRIFT (-8)        /* src/part.rs:118, src/utils.rs box_reflect containment */
+#define HFV_ERR_INVALID (-20)     /* bad argument / shape */
+#define HFV_ERR_HIP (-21)         /* HIP runtime error (hfv_last_error() has the text) */
+#define HFV_ERR_MESH (-22)        /* tessellation failed */
+#define HFV_ERR_NO_DEVICE (-23)
+
+/* ---------------------------------------------------------------- enums */
+/* Riemann solvers: src/riemann_solver.rs:28-44, src/bin/main.rs:625-642 */
+#define HFV_EXACT 0
+#define HFV_HLLC 1
+#define HFV_PVRS 2
+#define HFV_AIRS 3
+#define HFV_TSRS 4
+#define HFV_TRRS 5
+#define HFV_LINEAR_ADVECTION 6
+/* Boundary: src/space.rs:26-32 */
+#define HFV_PERIODIC 0
+#define HFV_REFLECTIVE 1
+#define HFV_OPEN 2
+#define HFV_VACUUM 3
+/* ParticleMotion: src/engine.rs:7-13 */
+#define HFV_FIXED 0
+#define HFV_STEER 1
+#define HFV_STEER_PAKMOR 2
+#define HFV_FLUID 3
+/* Runner: src/bin/main.rs:667-673 */
+#define HFV_OPTIMAL_ORDER 0
+#define HFV_GODUNOV 1
+/* Equation of state: src/gas_law.rs:57-60 */
+#define HFV_IDEAL 0
+#define HFV_ISOTHERMAL 1
+/* Arithmetic of the face reconstruction and Riemann solve. Accumulation, conserved->primitive and
+ * the exact solver's pressure iteration stay FP64 in both modes. */
+#define HFV_FP64 0
+#define HFV_FP32_FLUX 1
+
+/* ---------------------------------------------------------------- types */
+/* EulerEqnsFvs::new + GasLaw + engine configuration (src/finite_volume_solver/euler_eqns_fvs.rs:7-17,
+ * src/gas_law.rs:62-73, src/engine.rs:104-149, src/bin/main.rs:253-606). */
+typedef struct {
+  int dim;                 /* 1, 2 or 3 (Dimensionality) */
+  int boundary;            /* HFV_PERIODIC .. HFV_VACUUM */
+  double box[3];           /* box size (origin at 0) */
+  double gamma;            /* adiabatic index */
+  int eos;                 /* HFV_IDEAL / HFV_ISOTHERMAL */
+  double isothermal_u;     /* isothermal internal energy */
+  double cfl;              /* cfl_criterion */
+  int solver;              /* HFV_EXACT .. HFV_LINEAR_ADVECTION */
+  double airs_threshold;   /* AIRS p_max/p_min threshold */
+  double linadv_velocity[3];
+  int runner;              /* HFV_OPTIMAL_ORDER / HFV_GODUNOV */
+  int motion;              /* HFV_FIXED .. HFV_FLUID */
+  int sync_all;            /* time_integration:sync_timesteps */
+  double t_end;
+  double dt_snap;          /* dump sync points (drift only, no file output); <= 0: only t = 0 */
+  int precision;           /* HFV_FP64 / HFV_FP32_FLUX */
+} hfv_params;
+
+/* Face-indexed mesh (host arrays) as produced by the tessellator; replaces the VoronoiFace slice and
+ * cell->face connections of src/space.rs:76-77 (meshless_voronoi output). */
+typedef struct {
+  int64_t n_cells;
+  int64_t n_faces;
+  int64_t n_conn;             /* entries of conn[] */
+  const double* area;         /* [F] */
+  const double* normal;       /* [3F] xyz interleaved, unit, left -> right */
+  const double* centroid;     /* [3F] in the left cell's frame */
+  const double* shift;        /* [3F] periodic image offset of the right cell (0 when none) */
+  const int64_t* left;        /* [F] */
+  const int64_t* right;       /* [F], -1 for a wall face */
+  const double* volume;       /* [N] */
+  const double* cell_centroid;/* [3N] xyz interleaved */
+  const double* search_radius;/* [N] */
+  const int64_t* offset;      /* [N] face_connections_offset */
+  const int64_t* count;       /* [N] face_count */
+  const int64_t* conn;        /* [n_conn] cell -> face CSR (each cell's faces in accumulation order) */
+} hfv_mesh_arrays;
+
+/* Per-particle state of src/part.rs:10-39 in SoA form; host pointers, any may be NULL on set/get to
+ * skip that field.  Shapes: prim/extrap/cons/fluxes [5N], grad [15N] ((comp*3+axis)*N + i),
+ * gc/loc/centroid/v/v_rel [3N], scalars [N]. */
+typedef struct {
+  double* prim;
+  double* grad;
+  double* gc;       /* gradients_centroid */
+  double* extrap;   /* extrapolations */
+  double* cons;     /* conserved */
+  double* fluxes;   /* pending flux accumulator */
+  double* volume;
+  double* loc;
+  double* centroid;
+  double* v;
+  double* v_rel;
+  double* maor;     /* max_a_over_r */
+  double* msv;      /* max_signal_velocity */
+  double* dt;
+  int8_t* timebin;
+} hfv_state;
+
+/* TimestepInfo (src/engine.rs:33-81). */
+typedef struct {
+  uint64_t ti_old;
+  uint64_t ti_current;
+  uint64_t ti_next;
+  double time_base;
+  double time_base_inv;
+} hfv_timestep_info;
+
+/* Engine bookkeeping (src/engine.rs:84-101). */
+typedef struct {
+  hfv_timestep_info ts;
+  double t_current;
+  double t_end;
+  int64_t step_count;
+  int64_t cell_updates;    /* sum of active cells over all steps */
+  int64_t face_fluxes;     /* sum of gated-in faces over all steps */
+  int64_t last_n_active;
+  int64_t last_n_face_active;
+  int done;
+} hfv_engine_info;
+
+/* Soft-event counters (the reference eprintln!s these and continues). */
+typedef struct {
+  int64_t check_physical_resets; /* src/physical_quantities.rs:245-253 */
+  int64_t apply_flux_resets;     /* src/part.rs:184-191 */
+  int64_t gradient_fallbacks;    /* src/gradients.rs:48-53 */
+  int64_t brent_calls;           /* src/riemann_solver/exact.rs:279-284 */
+  int64_t error_code;            /* first hard error raised on the device (0 = none) */
+} hfv_stats;
+
+typedef struct hfv_ctx hfv_ctx;
+
+/* ---------------------------------------------------------------- lifecycle */
+const char* hfv_last_error(void);
+int hfv_device_count(int* n);
+/* EulerEqnsFvs::new (euler_eqns_fvs.rs:13-17) + Engine::new (engine.rs:104-149). */
+int hfv_create(const hfv_params* params, hfv_ctx** out);
+int hfv_destroy(hfv_ctx* ctx);
+/* Space::volume_calculation output (src/space.rs:236-263): copies the mesh to HBM. update_mask
+ * (nullable, [N]) selects the cells whose volume/centroid/search radius are replaced
+ * (Particle::update_geometry, src/part.rs:151-158); others only get new face offsets/counts. */
+int hfv_set_mesh(hfv_ctx* ctx, const hfv_mesh_arrays* mesh, const uint8_t* update_mask);
+/* Particle::from_ic + first_init (src/part.rs:200-217, src/space.rs:571-594): mesh must be set. */
+int hfv_init_from_ic(hfv_ctx* ctx, int64_t n, const double* loc, const double* mass, const double* vel,
+                     const double* u_int);
+int hfv_set_state(hfv_ctx* ctx, const hfv_state* st);
+int hfv_get_state(hfv_ctx* ctx, hfv_state* st);
+int hfv_get_stats(hfv_ctx* ctx, hfv_stats* out);
+int hfv_synchronize(hfv_ctx* ctx);
+
+/* ---------------------------------------------------------------- FiniteVolumeSolver trait */
+/* FiniteVolumeSolver::predict (src/finite_volume_solver.rs:10; euler_eqns_fvs.rs:20-24). */
+int hfv_predict(hfv_ctx* ctx, double dt);
+/* FiniteVolumeSolver::compute_fluxes (src/finite_volume_solver.rs:12; euler_eqns_fvs.rs:26-48).
+ * The active mask is bin_is_ending(timebin) at info->ti_current (optimal_order.rs:17).
+ * Flux records stay in HBM; hfv_get_fluxes copies them out as [7F]: dt*F (5), v_max, a_over_r. */
+int hfv_compute_fluxes(hfv_ctx* ctx, const hfv_timestep_info* info);
+int hfv_get_fluxes(hfv_ctx* ctx, double* out7f);
+/* runner apply_fluxes + kick2 (src/runner/hydrodynamics.rs:13-57). */
+int hfv_apply_fluxes(hfv_ctx* ctx);
+/* FiniteVolumeSolver::convert_conserved_to_primitive (src/finite_volume_solver.rs:14). */
+int hfv_conserved_to_primitive(hfv_ctx* ctx);
+/* runner gradient_estimate/limit/apply (src/runner/hydrodynamics.rs:67-143). */
+int hfv_gradients(hfv_ctx* ctx);
+/* FiniteVolumeSolver::compute_timesteps + timesteps_apply + timestep_limiter/sync +
+ * reset_extrapolations + kick1 (src/finite_volume_solver.rs:16; src/runner/hydrodynamics.rs:41-49,
+ * 145-195); writes ti_current + min(dti) to *ti_next. */
+int hfv_timesteps(hfv_ctx* ctx, uint64_t* ti_next);
+
+/* ---------------------------------------------------------------- fused step and engine */
+/* OptimalOrderRunner::step / GodunovHydroRunner::step without the tessellation
+ * (optimal_order.rs:15-64, godunov.rs:14-58).  For moving meshes in 1D the Voronoi rebuild runs on
+ * the device; 2D/3D moving meshes are rebuilt by the caller through hfv_set_mesh. */
+int hfv_step(hfv_ctx* ctx, const hfv_timestep_info* info, uint64_t* ti_next);
+/* Engine::run (src/engine.rs:152-192): advance until max_steps Step sync points ran or t_end. */
+int hfv_run(hfv_ctx* ctx, int64_t max_steps, int64_t* steps_done);
+int hfv_get_engine_info(hfv_ctx* ctx, hfv_engine_info* out);
+/* Space::status sums (src/space.rs:1248-1280): sum of conserved + pending fluxes per component,
+ * reduced on the device in a fixed order. */
+int hfv_conservation(hfv_ctx* ctx, double* out5);
+
+/* ---------------------------------------------------------------- unit entry point */
+/* RiemannFluxSolver::solve_for_flux (src/riemann_solver.rs:78-87) over a batch of host arrays:
+ * WL, WR [5n] primitive states (rho, vx, vy, vz, P), vface, nrm [3n], F [5n] out.  Component-major. */
+int hfv_riemann_batch(const hfv_params* params, int64_t n, const double* WL, const double* WR,
+                      const double* vface, const double* nrm, double* F);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,371 @@
+"""ctypes front-end of the CPU FP64 oracle (oracle/*.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline,
+never by the product package hydro_rs_amd.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+EXACT, HLLC, PVRS, AIRS, TSRS, TRRS, LINADV = range(7)
+VACUUM_SAMPLER = 100
+PERIODIC, REFLECTIVE, OPEN, VACUUM = range(4)
+FIXED, STEER, STEER_PAKMOR, FLUID = range(4)
+OPTIMAL_ORDER, GODUNOV = range(2)
+
+I64 = C.c_int64
+DP = C.POINTER(C.c_double)
+I64P = C.POINTER(C.c_int64)
+I8P = C.POINTER(C.c_int8)
+
+
+def build(force=False):
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.check_call(["make", "-s", "-C", _HERE])
+    return _LIB_PATH
+
+
+class Riemann(C.Structure):
+    _fields_ = [("kind", C.c_int), ("threshold", C.c_double), ("velocity", C.c_double * 3)]
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int), ("boundary", C.c_int), ("box", C.c_double * 3), ("gamma", C.c_double),
+        ("eos", C.c_int), ("isothermal_u", C.c_double), ("cfl", C.c_double), ("riemann", Riemann),
+        ("runner", C.c_int), ("motion", C.c_int), ("sync_all", C.c_int), ("t_end", C.c_double),
+        ("dt_snap", C.c_double),
+    ]
+
+
+class Mesh(C.Structure):
+    _fields_ = [
+        ("n_faces", I64), ("area", DP), ("normal", DP), ("centroid", DP), ("left", I64P), ("right", I64P),
+        ("shift", DP), ("n_conn", I64), ("conn", I64P), ("offset", I64P), ("count", I64P), ("volume", DP),
+        ("cell_centroid", DP), ("search_radius", DP),
+    ]
+
+
+class Counters(C.Structure):
+    _fields_ = [("check_physical_resets", I64), ("apply_flux_resets", I64), ("gradient_fallbacks", I64),
+                ("brent_calls", I64)]
+
+
+class EngineInfo(C.Structure):
+    _fields_ = [("ti_old", C.c_uint64), ("ti_current", C.c_uint64), ("ti_next", C.c_uint64),
+                ("time_base", C.c_double), ("time_base_inv", C.c_double), ("t_current", C.c_double),
+                ("t_end", C.c_double), ("step_count", I64), ("error", C.c_int), ("last_n_active", I64),
+                ("last_n_face_active", I64)]
+
+
+class Fields(C.Structure):
+    _fields_ = [(k, DP) for k in ("prim", "grad", "gc", "extrap", "cons", "fluxes", "volume", "loc", "centroid",
+                                  "v", "v_rel", "maor", "msv", "dt")] + [("timebin", I8P)]
+
+
+FIELD_SHAPES = {"prim": 5, "grad": 15, "gc": 3, "extrap": 5, "cons": 5, "fluxes": 5, "volume": 1, "loc": 3,
+                "centroid": 3, "v": 3, "v_rel": 3, "maor": 1, "msv": 1, "dt": 1}
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build()
+        L = C.CDLL(_LIB_PATH)
+        L.ora_solve_for_flux.argtypes = [C.POINTER(Riemann), C.c_double, DP, DP, DP, DP, DP]
+        L.ora_sample.argtypes = [C.c_int, C.c_double, C.c_double, DP, DP, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, DP, DP]
+        L.ora_exact_star.argtypes = [C.c_double, DP, DP, C.c_double, C.c_double, C.c_double, C.c_double, DP]
+        L.ora_pairwise_limiter.argtypes = [DP, DP, DP, C.c_double, DP]
+        L.ora_pairwise_limiter.restype = None
+        L.ora_sound_speed.argtypes = [C.c_double, C.c_double, C.c_double]
+        L.ora_sound_speed.restype = C.c_double
+        L.ora_get_integer_timestep.argtypes = [C.c_int8]
+        L.ora_get_integer_timestep.restype = C.c_uint64
+        L.ora_get_time_bin.argtypes = [C.c_uint64]
+        L.ora_get_time_bin.restype = C.c_int8
+        L.ora_get_integer_time_end.argtypes = [C.c_uint64, C.c_int8]
+        L.ora_get_integer_time_end.restype = C.c_uint64
+        L.ora_get_max_active_bin.argtypes = [C.c_uint64]
+        L.ora_get_max_active_bin.restype = C.c_int8
+        L.ora_make_integer_timestep.argtypes = [C.c_double, C.c_int8, C.c_int8, C.c_uint64, C.c_double]
+        L.ora_make_integer_timestep.restype = C.c_uint64
+        L.ora_saturating_f64_to_u64.argtypes = [C.c_double]
+        L.ora_saturating_f64_to_u64.restype = C.c_uint64
+        L.ora_sim_new.argtypes = [C.POINTER(Params), I64, DP, DP, DP, DP, C.POINTER(Mesh)]
+        L.ora_sim_new.restype = C.c_void_p
+        L.ora_sim_free.argtypes = [C.c_void_p]
+        L.ora_sim_free.restype = None
+        L.ora_sim_set_threads.argtypes = [C.c_int]
+        L.ora_sim_advance.argtypes = [C.c_void_p, I64]
+        L.ora_sim_advance.restype = I64
+        L.ora_sim_runner_step.argtypes = [C.c_void_p, C.POINTER(C.c_uint64)]
+        L.ora_sim_engine_info.argtypes = [C.c_void_p, C.POINTER(EngineInfo)]
+        L.ora_sim_engine_info.restype = None
+        L.ora_sim_set_engine_info.argtypes = [C.c_void_p, C.c_uint64, C.c_uint64, C.c_uint64]
+        L.ora_sim_set_engine_info.restype = None
+        L.ora_sim_counters.argtypes = [C.c_void_p, C.POINTER(Counters)]
+        L.ora_sim_counters.restype = None
+        L.ora_sim_get_fields.argtypes = [C.c_void_p, C.POINTER(Fields)]
+        L.ora_sim_get_fields.restype = None
+        L.ora_sim_set_fields.argtypes = [C.c_void_p, C.POINTER(Fields)]
+        L.ora_sim_set_fields.restype = None
+        L.ora_sim_n_cells.argtypes = [C.c_void_p]
+        L.ora_sim_n_cells.restype = I64
+        L.ora_sim_n_faces.argtypes = [C.c_void_p]
+        L.ora_sim_n_faces.restype = I64
+        L.ora_sim_get_mesh.argtypes = [C.c_void_p, DP, DP, DP, I64P, I64P, DP, I64P, I64P, I64P]
+        L.ora_sim_get_mesh.restype = None
+        L.ora_phase_predict.argtypes = [C.c_void_p, C.c_double]
+        L.ora_phase_predict.restype = None
+        L.ora_phase_compute_fluxes.argtypes = [C.c_void_p, C.POINTER(C.c_uint8), DP]
+        L.ora_phase_compute_fluxes.restype = None
+        L.ora_sim_conservation.argtypes = [C.c_void_p, DP]
+        L.ora_sim_conservation.restype = None
+        L.ora_voronoi_1d.argtypes = [I64, DP, C.c_double, C.c_int, DP, DP, DP, I64P, I64P, DP, I64P, I64P, I64P,
+                                     DP, DP, DP, I64P]
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    return a.ctypes.data_as(DP)
+
+
+def _i64p(a):
+    return a.ctypes.data_as(I64P)
+
+
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    if n is not None:
+        assert a.size == n
+    return a
+
+
+def riemann(kind, threshold=2.0, velocity=(0.0, 0.0, 0.0)):
+    r = Riemann()
+    r.kind = kind
+    r.threshold = threshold
+    for i in range(3):
+        r.velocity[i] = velocity[i]
+    return r
+
+
+def solve_for_flux(kind, gamma, left, right, iface_v, n_unit, threshold=2.0, velocity=(0.0, 0.0, 0.0)):
+    rs = riemann(kind, threshold, velocity)
+    out = np.zeros(5)
+    e = lib().ora_solve_for_flux(C.byref(rs), gamma, _dp(_f64(left, 5)), _dp(_f64(right, 5)),
+                                 _dp(_f64(iface_v, 3)), _dp(_f64(n_unit, 3)), _dp(out))
+    if e:
+        raise RuntimeError(f"oracle error {e}")
+    return out
+
+
+def sample(kind, gamma, left, right, v_l, v_r, a_l, a_r, n_unit, threshold=2.0):
+    out = np.zeros(5)
+    e = lib().ora_sample(kind, threshold, gamma, _dp(_f64(left, 5)), _dp(_f64(right, 5)), v_l, v_r, a_l, a_r,
+                         _dp(_f64(n_unit, 3)), _dp(out))
+    if e:
+        raise RuntimeError(f"oracle error {e}")
+    return out
+
+
+def exact_star(gamma, left, right, v_l, v_r, a_l, a_r):
+    out = np.zeros(4)
+    e = lib().ora_exact_star(gamma, _dp(_f64(left, 5)), _dp(_f64(right, 5)), v_l, v_r, a_l, a_r, _dp(out))
+    return e, out
+
+
+def pairwise_limiter(pl, pr, pdash, dx_fac):
+    out = np.zeros(5)
+    lib().ora_pairwise_limiter(_dp(_f64(pl, 5)), _dp(_f64(pr, 5)), _dp(_f64(pdash, 5)), dx_fac, _dp(out))
+    return out
+
+
+def sound_speed(gamma, p, rho_inv):
+    return lib().ora_sound_speed(gamma, p, rho_inv)
+
+
+def voronoi_1d(x, box_x, periodic=False):
+    x = _f64(x)
+    n = x.size
+    nf = n if periodic else n + 1
+    m = {
+        "area": np.zeros(nf), "normal": np.zeros(3 * nf), "centroid": np.zeros(3 * nf),
+        "left": np.zeros(nf, np.int64), "right": np.zeros(nf, np.int64), "shift": np.zeros(3 * nf),
+        "conn": np.zeros(2 * n, np.int64), "offset": np.zeros(n, np.int64), "count": np.zeros(n, np.int64),
+        "volume": np.zeros(n), "cell_centroid": np.zeros(3 * n), "search_radius": np.zeros(n),
+    }
+    nfo = I64(0)
+    e = lib().ora_voronoi_1d(n, _dp(x), box_x, int(periodic), _dp(m["area"]), _dp(m["normal"]),
+                             _dp(m["centroid"]), _i64p(m["left"]), _i64p(m["right"]), _dp(m["shift"]),
+                             _i64p(m["conn"]), _i64p(m["offset"]), _i64p(m["count"]), _dp(m["volume"]),
+                             _dp(m["cell_centroid"]), _dp(m["search_radius"]), C.byref(nfo))
+    if e:
+        raise RuntimeError(f"oracle 1D voronoi error {e}")
+    m["n_faces"] = int(nfo.value)
+    return m
+
+
+def make_params(dim, boundary, box, gamma, cfl, solver, threshold=2.0, runner=OPTIMAL_ORDER, motion=FIXED,
+                sync_all=False, t_end=1.0, dt_snap=0.0, eos=0, isothermal_u=0.0, velocity=(0.0, 0.0, 0.0)):
+    p = Params()
+    p.dim = dim
+    p.boundary = boundary
+    for i in range(3):
+        p.box[i] = box[i]
+    p.gamma = gamma
+    p.eos = eos
+    p.isothermal_u = isothermal_u
+    p.cfl = cfl
+    p.riemann = riemann(solver, threshold, velocity)
+    p.runner = runner
+    p.motion = motion
+    p.sync_all = int(sync_all)
+    p.t_end = t_end
+    p.dt_snap = dt_snap
+    return p
+
+
+class Sim:
+    """The oracle's Space + Engine for one simulation."""
+
+    def __init__(self, params, loc, mass, vel, u_int, mesh):
+        self.lib = lib()
+        n = len(mass)
+        self.n = n
+        self._keep = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in mesh.items()}
+        mk = self._keep
+        m = Mesh()
+        m.n_faces = int(mk["n_faces"])
+        for k in ("area", "normal", "centroid", "shift", "volume", "cell_centroid", "search_radius"):
+            mk[k] = np.ascontiguousarray(mk[k], dtype=np.float64)
+            setattr(m, k, _dp(mk[k]))
+        for k in ("left", "right", "conn", "offset", "count"):
+            mk[k] = np.ascontiguousarray(mk[k], dtype=np.int64)
+            setattr(m, k, _i64p(mk[k]))
+        m.n_conn = int(mk["conn"].size)
+        self.params = params
+        self.ptr = self.lib.ora_sim_new(C.byref(params), n, _dp(_f64(loc, 3 * n)), _dp(_f64(mass, n)),
+                                        _dp(_f64(vel, 3 * n)), _dp(_f64(u_int, n)), C.byref(m))
+
+    def __del__(self):
+        if getattr(self, "ptr", None):
+            self.lib.ora_sim_free(self.ptr)
+            self.ptr = None
+
+    def advance(self, max_steps):
+        r = self.lib.ora_sim_advance(self.ptr, max_steps)
+        if r < 0:
+            raise RuntimeError(f"oracle error {r}")
+        return r
+
+    def runner_step(self):
+        t = C.c_uint64(0)
+        e = self.lib.ora_sim_runner_step(self.ptr, C.byref(t))
+        if e:
+            raise RuntimeError(f"oracle error {e}")
+        return t.value
+
+    def info(self):
+        inf = EngineInfo()
+        self.lib.ora_sim_engine_info(self.ptr, C.byref(inf))
+        return {k: getattr(inf, k) for k, _ in EngineInfo._fields_}
+
+    def set_engine_info(self, ti_old, ti_current, ti_next):
+        self.lib.ora_sim_set_engine_info(self.ptr, ti_old, ti_current, ti_next)
+
+    def counters(self):
+        c = Counters()
+        self.lib.ora_sim_counters(self.ptr, C.byref(c))
+        return {k: getattr(c, k) for k, _ in Counters._fields_}
+
+    def get_fields(self):
+        n = self.n
+        out = {k: np.zeros(w * n) for k, w in FIELD_SHAPES.items()}
+        out["timebin"] = np.zeros(n, np.int8)
+        f = Fields()
+        for k in FIELD_SHAPES:
+            setattr(f, k, _dp(out[k]))
+        f.timebin = out["timebin"].ctypes.data_as(I8P)
+        self.lib.ora_sim_get_fields(self.ptr, C.byref(f))
+        return out
+
+    def set_fields(self, fields):
+        f = Fields()
+        keep = {}
+        for k in FIELD_SHAPES:
+            if k in fields:
+                keep[k] = np.ascontiguousarray(fields[k], dtype=np.float64)
+                setattr(f, k, _dp(keep[k]))
+        if "timebin" in fields:
+            keep["timebin"] = np.ascontiguousarray(fields["timebin"], dtype=np.int8)
+            f.timebin = keep["timebin"].ctypes.data_as(I8P)
+        self.lib.ora_sim_set_fields(self.ptr, C.byref(f))
+
+    def n_faces(self):
+        return self.lib.ora_sim_n_faces(self.ptr)
+
+    def get_mesh(self):
+        nf = self.n_faces()
+        n = self.n
+        m = {"area": np.zeros(nf), "normal": np.zeros(3 * nf), "centroid": np.zeros(3 * nf),
+             "left": np.zeros(nf, np.int64), "right": np.zeros(nf, np.int64), "shift": np.zeros(3 * nf),
+             "conn": np.zeros(2 * nf, np.int64), "offset": np.zeros(n, np.int64), "count": np.zeros(n, np.int64)}
+        self.lib.ora_sim_get_mesh(self.ptr, _dp(m["area"]), _dp(m["normal"]), _dp(m["centroid"]),
+                                  _i64p(m["left"]), _i64p(m["right"]), _dp(m["shift"]), _i64p(m["conn"]),
+                                  _i64p(m["offset"]), _i64p(m["count"]))
+        m["n_faces"] = nf
+        return m
+
+    def predict(self, dt):
+        self.lib.ora_phase_predict(self.ptr, dt)
+
+    def compute_fluxes(self, active):
+        act = np.ascontiguousarray(active, dtype=np.uint8)
+        out = np.zeros(7 * self.n_faces())
+        self.lib.ora_phase_compute_fluxes(self.ptr, act.ctypes.data_as(C.POINTER(C.c_uint8)), _dp(out))
+        return out.reshape(-1, 7)
+
+    def conservation(self):
+        out = np.zeros(5)
+        self.lib.ora_sim_conservation(self.ptr, _dp(out))
+        return out
+
+
+def set_threads(n):
+    return lib().ora_sim_set_threads(n)
+
+
+# timeline
+def get_integer_timestep(b):
+    return lib().ora_get_integer_timestep(b)
+
+
+def get_time_bin(dti):
+    return lib().ora_get_time_bin(dti)
+
+
+def get_integer_time_end(ti, b):
+    return lib().ora_get_integer_time_end(ti, b)
+
+
+def get_max_active_bin(ti):
+    return lib().ora_get_max_active_bin(ti)
+
+
+def make_integer_timestep(dt, old_bin, min_ngb_bin, ti_current, time_base_inv):
+    return lib().ora_make_integer_timestep(dt, old_bin, min_ngb_bin, ti_current, time_base_inv)
+
+
+def saturating_cast(x):
+    return lib().ora_saturating_f64_to_u64(x)

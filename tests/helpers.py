@@ -1,0 +1,63 @@
+"""Shared test helpers: build the oracle simulation for a product Problem, and compare states."""
+import numpy as np
+
+from oracle import ora
+
+FLOAT_FIELDS = ["prim", "grad", "gc", "extrap", "cons", "fluxes", "volume", "loc", "centroid", "v", "v_rel",
+                "maor", "msv", "dt"]
+
+
+def oracle_sim(prob):
+    c = prob.cfg
+    p = ora.make_params(c.dim, c.boundary, c.box, c.gamma, c.cfl, c.solver, threshold=c.airs_threshold,
+                        runner=c.runner, motion=c.motion, sync_all=c.sync_all, t_end=c.t_end, dt_snap=c.dt_snap,
+                        eos=c.eos, isothermal_u=c.isothermal_u, velocity=c.linadv_velocity)
+    return ora.Sim(p, prob.loc.ravel(), prob.mass, prob.vel.ravel(), prob.u_int, prob.mesh.as_dict())
+
+
+def rel_err(a, b, floor=1e-300):
+    """max |a-b| / max(|b|) over an array (norm-relative)."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if a.size == 0:
+        return 0.0
+    both_nan = np.isnan(a) & np.isnan(b)
+    d = np.where(both_nan, 0.0, np.abs(a - b))
+    scale = max(np.max(np.abs(np.where(np.isfinite(b), b, 0.0))), floor)
+    return float(np.max(d) / scale)
+
+
+def per_component_rel(a, b, width, n):
+    """Worst norm-relative error over the components of an SoA field."""
+    a = np.asarray(a).reshape(width, n)
+    b = np.asarray(b).reshape(width, n)
+    return max(rel_err(a[k], b[k]) for k in range(width))
+
+
+def compare_states(dev, ref, n, tol=1e-12, fields=None, ignore=()):
+    """Return {field: worst relative error}; timebins compared exactly (reported as 0/1)."""
+    from hydro_rs_amd.fvs import FIELD_WIDTH
+
+    out = {}
+    for k in fields or FLOAT_FIELDS:
+        if k in ignore:
+            continue
+        out[k] = per_component_rel(dev[k], ref[k], FIELD_WIDTH[k], n)
+    out["timebin_mismatch"] = int(np.sum(dev["timebin"] != ref["timebin"]))
+    return out
+
+
+def face_flux_rel(dev_rec, ref_rec):
+    """Per-face norm-relative error of the flux records [F,7] (components 0..4 scaled by the face's flux
+    magnitude, v_max and a_over_r by themselves); returns the worst face."""
+    dev_rec = np.asarray(dev_rec)
+    ref_rec = np.asarray(ref_rec)
+    fmag = np.max(np.abs(ref_rec[:, :5]), axis=1)
+    d = np.max(np.abs(dev_rec[:, :5] - ref_rec[:, :5]), axis=1)
+    with np.errstate(invalid="ignore", divide="ignore"):
+        r5 = np.where(fmag > 0, d / np.where(fmag > 0, fmag, 1.0), d)
+        r6 = np.abs(dev_rec[:, 5] - ref_rec[:, 5]) / np.maximum(np.abs(ref_rec[:, 5]), 1e-300)
+        r7 = np.abs(dev_rec[:, 6] - ref_rec[:, 6]) / np.maximum(np.abs(ref_rec[:, 6]), 1e-300)
+    r6 = np.where(ref_rec[:, 5] == dev_rec[:, 5], 0.0, r6)
+    r7 = np.where(ref_rec[:, 6] == dev_rec[:, 6], 0.0, r7)
+    return float(max(np.max(r5, initial=0.0), np.max(r6, initial=0.0), np.max(r7, initial=0.0)))

@@ -1,0 +1,231 @@
+"""Parity of the HIP path with the CPU oracle, through the C-ABI.
+
+FP64 path: per-face flux records and every field after one step within 1e-12 (norm-relative), integer
+timebins and ti_next bit-exact; whole runs within 1e-9 (pow in the device libm may differ from glibc in
+the last bit, and the step sequence compounds it).  FP32-flux path: L1 density error < 1e-5 vs FP64.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from hydro_rs_amd import fvs, ics
+from hydro_rs_amd.mesh import VoronoiMesh
+from tests.helpers import compare_states, face_flux_rel, oracle_sim, rel_err
+
+pytestmark = pytest.mark.gpu
+
+TOL_STEP = 1e-12
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _device():
+    if fvs.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+def mesh_from_oracle(sim, fields, dim, box, periodic):
+    m = sim.get_mesh()
+    mesh = object.__new__(VoronoiMesh)
+    mesh.dim, mesh.box, mesh.periodic = dim, np.asarray(box, float), periodic
+    mesh.n_cells = sim.n
+    mesh.n_faces = m["n_faces"]
+    mesh.area = m["area"]
+    mesh.normal = m["normal"].reshape(-1, 3)
+    mesh.centroid = m["centroid"].reshape(-1, 3)
+    mesh.shift = m["shift"].reshape(-1, 3)
+    mesh.left, mesh.right = m["left"], m["right"]
+    mesh.offset, mesh.count = m["offset"], m["count"]
+    mesh.conn = m["conn"][: int(m["count"].sum())] if m["conn"].size > m["count"].sum() else m["conn"]
+    n = sim.n
+    mesh.volume = fields["volume"].copy()
+    mesh.cell_centroid = fields["centroid"].reshape(3, n).T.copy()
+    mesh.search_radius = np.zeros(n)
+    return mesh
+
+
+CASES = {
+    "sod1d": lambda: ics.sod_1d(n=200, t_end=0.2),
+    "sedov2d": lambda: ics.sedov_2d(side=32, t_end=0.05),
+    "kh3d": lambda: ics.kelvin_helmholtz_3d(side=10, t_end=0.5),
+    "tube2d_reflective_exact": lambda: _tube2d(),
+}
+
+
+def _tube2d():
+    from hydro_rs_amd.fvs import EXACT, REFLECTIVE
+
+    rng = np.random.default_rng(11)
+    loc = ics.jittered_grid(2, 24, (1.0, 1.0, 1.0), 0.05, rng)
+    left = loc[:, 0] < 0.5
+    rho = np.where(left, 1.0, 0.25)
+    P = np.where(left, 1.0, 0.2)
+    vel = np.zeros_like(loc)
+    vel[:, 1] = 0.1
+    cfg = fvs.Config(dim=2, box=(1.0, 1.0, 1.0), gamma=1.4, cfl=0.3, solver=EXACT, boundary=REFLECTIVE,
+                     motion=fvs.FIXED, t_end=0.1)
+    return ics._finish("tube2d", cfg, loc, rho, vel, P)
+
+
+def _synced(prob, k):
+    """Oracle advanced k engine steps, and a device sim holding the same state and mesh."""
+    osim = oracle_sim(prob)
+    if k:
+        osim.advance(k)
+    f = osim.get_fields()
+    inf = osim.info()
+    dev = fvs.EulerEqnsFvs(prob.cfg)
+    c = prob.cfg
+    dev.set_mesh(mesh_from_oracle(osim, f, c.dim, c.box, c.boundary == fvs.PERIODIC))
+    dev.set_state(f)
+    return osim, dev, inf
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("k", [0, 7])
+def test_face_flux_parity(ora, case, k):
+    say(f"[face-flux] case={case} k={k}")
+    prob = CASES[case]()
+    osim, dev, inf = _synced(prob, k)
+    ti_old, ti_cur = inf["ti_old"], inf["ti_next"]
+    tb = inf["time_base"]
+    osim.set_engine_info(ti_old, ti_cur, 0)
+    mab = ora.get_max_active_bin(ti_cur)
+    f = osim.get_fields()
+    active = (f["timebin"] <= mab).astype(np.uint8)
+    ref = osim.compute_fluxes(active)
+    info = dev.timestep_info(ti_old, ti_cur, tb)
+    dev.compute_fluxes(info)
+    got = dev.get_fluxes()
+    err = face_flux_rel(got, ref)
+    say(f"  faces={len(ref)} active_cells={int(active.sum())} worst face rel err={err:.3e}")
+    assert err <= TOL_STEP
+
+
+@pytest.mark.parametrize("case", list(CASES))
+@pytest.mark.parametrize("k", [0, 1, 9])
+def test_step_parity(ora, case, k):
+    say(f"[step] case={case} k={k}")
+    prob = CASES[case]()
+    osim, dev, inf = _synced(prob, k)
+    ti_old, ti_cur = inf["ti_old"], inf["ti_next"]
+    tb = inf["time_base"]
+    osim.set_engine_info(ti_old, ti_cur, 0)
+    t_ref = osim.runner_step()
+    t_dev = dev.step(dev.timestep_info(ti_old, ti_cur, tb))
+    ref = osim.get_fields()
+    got = dev.get_state()
+    errs = compare_states(got, ref, prob.n)
+    say(f"  ti_next ref={t_ref} dev={t_dev} errs={ {k2: f'{v:.1e}' for k2, v in errs.items()} }")
+    assert t_dev == t_ref
+    assert errs.pop("timebin_mismatch") == 0
+    assert max(errs.values()) <= TOL_STEP, errs
+    st = dev.stats()
+    assert st["error_code"] == 0
+
+
+@pytest.mark.parametrize("case,steps", [("sod1d", 1 << 40), ("sedov2d", 400), ("tube2d_reflective_exact", 1 << 40)])
+def test_full_run_parity(ora, case, steps):
+    say(f"[run] case={case}")
+    prob = CASES[case]()
+    osim = oracle_sim(prob)
+    n_ref = osim.advance(steps)
+    dev = ics.build_device_sim(prob)
+    n_dev = dev.run(steps)
+    ref = osim.get_fields()
+    got = dev.get_state()
+    errs = compare_states(got, ref, prob.n)
+    say(f"  steps ref={n_ref} dev={n_dev} errs={ {k2: f'{v:.1e}' for k2, v in errs.items()} }")
+    assert n_dev == n_ref
+    assert errs.pop("timebin_mismatch") == 0
+    assert errs["prim"] <= 1e-9 and errs["cons"] <= 1e-9, errs
+    ei = dev.engine_info()
+    oi = osim.info()
+    assert ei["ti_current"] == oi["ti_current"]
+
+
+def test_conservation_on_device():
+    say("[conservation] sedov2d side=48")
+    prob = ics.sedov_2d(side=48, t_end=0.05)
+    dev = ics.build_device_sim(prob)
+    c0 = dev.conservation()
+    dev.run(60)
+    c1 = dev.conservation()
+    assert abs(c1[0] - c0[0]) <= 1e-13 * abs(c0[0])
+    assert abs(c1[4] - c0[4]) <= 1e-12 * abs(c0[4])
+    assert np.all(np.abs(c1[1:4] - c0[1:4]) <= 1e-13)
+
+
+@pytest.mark.parametrize("solver", ["Exact", "HLLC", "PVRS", "AIRS", "TSRS", "TRRS", "LinearAdvection"])
+def test_riemann_batch_parity(ora, solver):
+    say(f"[riemann] solver={solver}")
+    rng = np.random.default_rng(hash(solver) % 1000)
+    n = 4000
+    WL = np.column_stack([10 ** rng.uniform(-1, 1, n), rng.uniform(-1, 1, (n, 3)), 10 ** rng.uniform(-2, 1, n)])
+    WR = np.column_stack([10 ** rng.uniform(-1, 1, n), rng.uniform(-1, 1, (n, 3)), 10 ** rng.uniform(-2, 1, n)])
+    # some vacuum and vacuum-generating pairs
+    WL[:40] = 0.0
+    WR[40:80] = 0.0
+    WL[80:120, 1] = -8.0
+    WR[80:120, 1] = 8.0
+    vf = rng.uniform(-0.5, 0.5, (n, 3))
+    nr = rng.normal(size=(n, 3))
+    nr /= np.linalg.norm(nr, axis=1, keepdims=True)
+    cfg = fvs.Config(dim=3, box=(1, 1, 1), gamma=5.0 / 3.0, cfl=0.3, solver=fvs.SOLVERS[solver],
+                     linadv_velocity=(0.3, -0.2, 0.1))
+    got = fvs.riemann_batch(cfg, WL, WR, vf, nr)
+    ref = np.array([ora.solve_for_flux(fvs.SOLVERS[solver], 5.0 / 3.0, WL[i], WR[i], vf[i], nr[i],
+                                       threshold=2.0, velocity=(0.3, -0.2, 0.1)) for i in range(n)])
+    finite = np.all(np.isfinite(ref), axis=1)
+    assert np.array_equal(finite, np.all(np.isfinite(got), axis=1))
+    scale = np.maximum(np.max(np.abs(ref[finite]), axis=1), 1e-300)
+    err = np.max(np.abs(got[finite] - ref[finite]), axis=1) / scale
+    say(f"  worst rel err={err.max():.3e}, exact-equal fraction={np.mean(got[finite] == ref[finite]):.4f}")
+    assert err.max() <= TOL_STEP
+
+
+def test_fp32_flux_l1_density():
+    """FP32 reconstruction/solve with FP64 accumulation stays within L1(rho) < 1e-5 of the FP64 path."""
+    say("[fp32] sedov2d side=64, 200 steps")
+    p64 = ics.sedov_2d(side=64, t_end=0.05)
+    p32 = ics.sedov_2d(side=64, t_end=0.05, precision=fvs.FP32_FLUX)
+    d64 = ics.build_device_sim(p64)
+    d32 = ics.build_device_sim(p32)
+    s64 = d64.run(200)
+    s32 = d32.run(200)
+    r64 = d64.get_state(["prim"])["prim"][: p64.n]
+    r32 = d32.get_state(["prim"])["prim"][: p64.n]
+    l1 = float(np.mean(np.abs(r32 - r64)))
+    say(f"  steps {s64}/{s32}, L1(rho) = {l1:.3e}")
+    assert l1 < 1e-5
+
+
+def test_trait_phases_equal_fused_step():
+    """predict/compute_fluxes/apply/c2p/gradients/timesteps through the trait entry points produce the
+    fused hfv_step result bit for bit."""
+    say("[phases] sedov2d side=24")
+    prob = ics.sedov_2d(side=24, t_end=0.05)
+    a = ics.build_device_sim(prob)
+    b = ics.build_device_sim(prob)
+    tb = prob.cfg.t_end / float(1 << 57)
+    ti_old, ti_cur = 0, 0
+    for _ in range(5):
+        info = a.timestep_info(ti_old, ti_cur, tb)
+        t1 = a.step(info)
+        b.predict((ti_cur - ti_old) * tb)
+        b.compute_fluxes(info)
+        b.apply_fluxes()
+        b.convert_conserved_to_primitive()
+        b.gradients()
+        t2 = b.timesteps()
+        assert t1 == t2
+        sa, sb = a.get_state(), b.get_state()
+        for k in sa:
+            assert np.array_equal(sa[k], sb[k]), k
+        ti_old, ti_cur = ti_cur, t1
+    assert rel_err(sa["prim"], sb["prim"]) == 0.0
