@@ -71,6 +71,13 @@ class Stats(C.Structure):
                 ("brent_calls", I64), ("error_code", I64)]
 
 
+class KernelTimes(C.Structure):
+    _fields_ = [("launches", I64 * 8), ("ms", C.c_double * 8)]
+
+
+KERNEL_SLOTS = ["mask_predict", "mesh1d", "flux", "apply_c2p", "grad_timestep", "wakeup", "limit_kick", "drift"]
+
+
 # Every symbol include/hfv.h and include/hfv_host.h declare, with its ctypes signature.
 HFV_SYMBOLS = {
     "hfv_last_error": (C.c_char_p, []),
@@ -95,6 +102,8 @@ HFV_SYMBOLS = {
     "hfv_get_engine_info": (C.c_int, [C.c_void_p, C.POINTER(EngineInfo)]),
     "hfv_conservation": (C.c_int, [C.c_void_p, DP]),
     "hfv_riemann_batch": (C.c_int, [C.POINTER(Params), I64, DP, DP, DP, DP, DP]),
+    "hfv_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfv_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTimes)]),
 }
 HOST_SYMBOLS = {
     "hfv_voronoi_build": (C.c_int, [C.c_int, DP, C.c_int, I64, DP, C.c_int, C.POINTER(C.c_void_p)]),

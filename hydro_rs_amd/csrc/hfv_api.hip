@@ -124,7 +124,11 @@ struct hfv_ctx {
   hfv_timestep_info last_info{};
   int64_t soft[4] = {0, 0, 0, 0};
   int err = 0;
-  int mesh_periodic1d = 0;
+  // kernel timing (hfv_profile_enable)
+  bool prof = false;
+  hipEvent_t ev0[HFV_NUM_KERNEL_SLOTS]{}, ev1[HFV_NUM_KERNEL_SLOTS]{};
+  bool pending[HFV_NUM_KERNEL_SLOTS]{};
+  hfv_kernel_times times{};
 };
 
 namespace {
@@ -210,12 +214,39 @@ int sync_and_check(hfv_ctx* c) {
 
 int8_t max_active_bin(uint64_t ti) { return get_max_active_bin(ti); }
 
+enum { KS_PREDICT = 0, KS_MESH1D, KS_FLUX, KS_APPLY, KS_GRAD, KS_WAKEUP, KS_LIMIT, KS_DRIFT };
+
+inline void prof_begin(hfv_ctx* c, int slot) {
+  if (c->prof) (void)hipEventRecord(c->ev0[slot], c->stream);
+}
+inline void prof_end(hfv_ctx* c, int slot) {
+  if (c->prof) {
+    (void)hipEventRecord(c->ev1[slot], c->stream);
+    c->pending[slot] = true;
+  }
+}
+// Called after the stream was synchronized.
+void prof_collect(hfv_ctx* c) {
+  if (!c->prof) return;
+  for (int k = 0; k < HFV_NUM_KERNEL_SLOTS; ++k)
+    if (c->pending[k]) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, c->ev0[k], c->ev1[k]) == hipSuccess) {
+        c->times.ms[k] += ms;
+        c->times.launches[k] += 1;
+      }
+      c->pending[k] = false;
+    }
+}
+
 int launch_flux(hfv_ctx* c) {
   if (c->nf == 0) return HFV_OK;
+  prof_begin(c, KS_FLUX);
   if (c->p.precision == HFV_FP32_FLUX)
     hipLaunchKernelGGL(k_flux<float>, dim3(grid_for(c->nf)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt);
   else
     hipLaunchKernelGGL(k_flux<double>, dim3(grid_for(c->nf)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt);
+  prof_end(c, KS_FLUX);
   HIP_OK(hipGetLastError());
   return HFV_OK;
 }
@@ -223,8 +254,13 @@ int launch_flux(hfv_ctx* c) {
 int mesh_update_1d(hfv_ctx* c) {
   if (c->p.dim != 1) return set_err(HFV_ERR_UNSUPPORTED, "device mesh rebuild is 1D only; rebuild 2D/3D meshes on the host");
   int periodic = c->p.boundary == HFV_PERIODIC;
+  // the kernels assume the 1D face layout of the tessellator: face k between cells k-1 and k
+  if (c->nf != (periodic ? c->n : c->n + 1))
+    return set_err(HFV_ERR_INVALID, "1D moving mesh needs the 1D tessellator's face layout");
+  prof_begin(c, KS_MESH1D);
   hipLaunchKernelGGL(k_voronoi1d_faces, dim3(grid_for(c->nf)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, periodic, c->dcnt);
   hipLaunchKernelGGL(k_voronoi1d_cells, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, periodic);
+  prof_end(c, KS_MESH1D);
   HIP_OK(hipGetLastError());
   return HFV_OK;
 }
@@ -233,18 +269,27 @@ int mesh_update_1d(hfv_ctx* c) {
 int launch_step_tail(hfv_ctx* c, const hfv_timestep_info* info) {
   int e = launch_flux(c);
   if (e) return e;
+  prof_begin(c, KS_APPLY);
   hipLaunchKernelGGL(k_apply_c2p, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt, 3);
+  prof_end(c, KS_APPLY);
+  prof_begin(c, KS_GRAD);
   hipLaunchKernelGGL(k_grad_timestep, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt,
                      info->ti_current, info->time_base, info->time_base_inv, 1, 1);
+  prof_end(c, KS_GRAD);
+  prof_begin(c, KS_WAKEUP);
   hipLaunchKernelGGL(k_wakeup, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->dcnt, c->p.sync_all);
+  prof_end(c, KS_WAKEUP);
+  prof_begin(c, KS_LIMIT);
   hipLaunchKernelGGL(k_limit_kick, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->cfg, c->dcnt,
                      c->p.sync_all, info->ti_current, info->time_base);
+  prof_end(c, KS_LIMIT);
   HIP_OK(hipGetLastError());
   return HFV_OK;
 }
 
 int finish_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {
   int e = sync_and_check(c);
+  prof_collect(c);
   if (e) return e;
   uint64_t m = c->hcnt->ti_min;
   if (c->hcnt->n_active == 0 || m == ~0ull) return set_err(HFV_ERR_INVALID, "no active cell in this step");
@@ -320,6 +365,10 @@ int hfv_destroy(hfv_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   free_list(c->cell_bufs);
   free_list(c->face_bufs);
+  for (int k = 0; k < HFV_NUM_KERNEL_SLOTS; ++k) {
+    if (c->ev0[k]) (void)hipEventDestroy(c->ev0[k]);
+    if (c->ev1[k]) (void)hipEventDestroy(c->ev1[k]);
+  }
   if (c->dcnt) (void)hipFree(c->dcnt);
   if (c->hcnt) (void)hipHostFree(c->hcnt);
   (void)hipStreamDestroy(c->stream);
@@ -596,8 +645,10 @@ int hfv_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {
   c->last_info = *info;
   double dt_pred = (double)(info->ti_current - info->ti_old) * info->time_base;
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, c->stream, c->dcnt);
+  prof_begin(c, KS_PREDICT);
   hipLaunchKernelGGL(k_mask_predict, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->cfg, dt_pred,
                      max_active_bin(info->ti_current), 1, c->dcnt);
+  prof_end(c, KS_PREDICT);
   if (c->p.motion != HFV_FIXED) {
     e = mesh_update_1d(c);
     if (e) return e;
@@ -621,7 +672,9 @@ int hfv_run(hfv_ctx* c, int64_t max_steps, int64_t* steps_done) {
     if (dti > 0) {
       double dt = (double)dti * c->ts.time_base;
       if (c->p.motion != HFV_FIXED) {  // with fixed motion v == 0 and the drift is the identity
+        prof_begin(c, KS_DRIFT);
         hipLaunchKernelGGL(k_drift, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->cfg, dt, c->dcnt);
+        prof_end(c, KS_DRIFT);
         HIP_OK(hipGetLastError());
       }
       c->ts.ti_current += dti;
@@ -673,6 +726,31 @@ int hfv_conservation(hfv_ctx* c, double* out5) {
   HIP_OK(hipMemcpyAsync(out5, d, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   free_list(tmp);
+  return HFV_OK;
+}
+
+int hfv_profile_enable(hfv_ctx* c, int enable) {
+  if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
+  HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  if (enable) {
+    for (int k = 0; k < HFV_NUM_KERNEL_SLOTS; ++k) {
+      if (!c->ev0[k]) HIP_OK(hipEventCreate(&c->ev0[k]));
+      if (!c->ev1[k]) HIP_OK(hipEventCreate(&c->ev1[k]));
+      c->pending[k] = false;
+    }
+    c->times = hfv_kernel_times{};
+  }
+  c->prof = enable != 0;
+  return HFV_OK;
+}
+
+int hfv_profile_read(hfv_ctx* c, hfv_kernel_times* out) {
+  if (!c || !out) return set_err(HFV_ERR_INVALID, "NULL argument");
+  HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  prof_collect(c);
+  *out = c->times;
   return HFV_OK;
 }
 

@@ -211,6 +211,14 @@ class EulerEqnsFvs:
         _lib.check(self.lib.hfv_conservation(self.ptr, _dp(out)))
         return out
 
+    def profile(self, enable=True):
+        _lib.check(self.lib.hfv_profile_enable(self.ptr, int(bool(enable))))
+
+    def kernel_times(self):
+        t = _lib.KernelTimes()
+        _lib.check(self.lib.hfv_profile_read(self.ptr, C.byref(t)))
+        return {name: {"launches": t.launches[k], "ms": t.ms[k]} for k, name in enumerate(_lib.KERNEL_SLOTS)}
+
     def synchronize(self):
         _lib.check(self.lib.hfv_synchronize(self.ptr))
 

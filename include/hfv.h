@@ -214,6 +214,20 @@ int hfv_get_engine_info(hfv_ctx* ctx, hfv_engine_info* out);
  * reduced on the device in a fixed order. */
 int hfv_conservation(hfv_ctx* ctx, double* out5);
 
+/* ---------------------------------------------------------------- measurement */
+/* Device time of the step's kernels, from HIP events recorded on the context's stream around each
+ * launch (no counterpart in the reference, which has no device).  Slots: 0 mask+predict, 1 1D mesh
+ * rebuild, 2 face flux, 3 accumulate+c2p, 4 gradients+timestep, 5 limiter wakeup, 6 limiter+kick1,
+ * 7 drift. */
+#define HFV_NUM_KERNEL_SLOTS 8
+typedef struct {
+  int64_t launches[HFV_NUM_KERNEL_SLOTS];
+  double ms[HFV_NUM_KERNEL_SLOTS];
+} hfv_kernel_times;
+/* enable != 0 starts recording (and clears the accumulators); 0 stops. */
+int hfv_profile_enable(hfv_ctx* ctx, int enable);
+int hfv_profile_read(hfv_ctx* ctx, hfv_kernel_times* out);
+
 /* ---------------------------------------------------------------- unit entry point */
 /* RiemannFluxSolver::solve_for_flux (src/riemann_solver.rs:78-87) over a batch of host arrays:
  * WL, WR [5n] primitive states (rho, vx, vy, vz, P), vface, nrm [3n], F [5n] out.  Component-major. */
