@@ -1,10 +1,14 @@
-// C-ABI of the MI355X per-face finite-volume path (include/hfv.h) and the host engine loop.
+// C-ABI of the MI355X per-face finite-volume path (include/hfv.h).
+//
+// A context is a batch of slots (hfv_create makes a batch of one).  Device-resident engines advance
+// in HBM; the host only enqueues launch sequences (replayed from a captured HIP graph) and reads the
+// engines back every few steps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
-#include <deque>
 #include <string>
 #include <vector>
 
@@ -22,23 +26,34 @@ int set_err(int code, const char* what) {
   return code;
 }
 
-#define HIP_OK(expr)                                                                          \
-  do {                                                                                        \
-    hipError_t _e = (expr);                                                                   \
-    if (_e != hipSuccess) {                                                                   \
-      char _b[512];                                                                           \
-      snprintf(_b, sizeof(_b), "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, \
-               __LINE__);                                                                     \
-      return set_err(HFV_ERR_HIP, _b);                                                        \
-    }                                                                                         \
+#define HIP_OK(expr)                                                                               \
+  do {                                                                                             \
+    hipError_t _e = (expr);                                                                        \
+    if (_e != hipSuccess) {                                                                        \
+      char _b[512];                                                                                \
+      snprintf(_b, sizeof(_b), "%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, __LINE__); \
+      return set_err(HFV_ERR_HIP, _b);                                                             \
+    }                                                                                              \
+  } while (0)
+
+#define TRY(expr)            \
+  do {                       \
+    int _r = (expr);         \
+    if (_r != HFV_OK) return _r; \
   } while (0)
 
 constexpr int BLOCK = 256;
-inline unsigned grid_for(int64_t n) { return (unsigned)((n + BLOCK - 1) / BLOCK); }
+constexpr int64_t LIST_GRID_CAP = 4096;  // blocks of a list kernel (grid-stride beyond)
+inline unsigned grid_full(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK); }
+inline unsigned grid_list(int64_t bound) {
+  int64_t g = (bound + BLOCK - 1) / BLOCK;
+  g = std::min<int64_t>(std::max<int64_t>(g, 8), LIST_GRID_CAP);
+  return (unsigned)((g + 7) / 8 * 8);  // multiple of 8 for the XCD-aware block order
+}
 
 template <class T>
 Gas<T> make_gas(double gamma, int eos, double iso_u) {
-  // AdiabaticIndex (gas_law.rs:1-54), evaluated in double exactly as the reference does
+  // AdiabaticIndex (gas_law.rs:1-54), evaluated in double with the reference's expressions
   double gi = 1. / gamma;
   double odgm1 = 1. / (gamma - 1.);
   double odgp1 = 1. / (gamma + 1.);
@@ -62,11 +77,13 @@ Gas<T> make_gas(double gamma, int eos, double iso_u) {
 
 Cfg make_cfg(const hfv_params& p) {
   Cfg c;
+  std::memset(&c, 0, sizeof(c));
   c.dim = p.dim;
   c.boundary = p.boundary;
   c.motion = p.motion;
   c.runner = p.runner;
   c.sync_all = p.sync_all;
+  c.precision = p.precision;
   for (int a = 0; a < 3; ++a) c.box[a] = p.box[a];
   c.cfl = p.cfl;
   c.radius3_fac = 0.25 * 3. * 0.318309886183790671537767526745028724;
@@ -91,120 +108,213 @@ int check_params(const hfv_params* p) {
   return HFV_OK;
 }
 
-template <class T>
-T* dalloc(int64_t count, std::vector<void*>& owned) {
-  void* p = nullptr;
-  if (count <= 0) count = 1;
-  if (hipMalloc(&p, sizeof(T) * (size_t)count) != hipSuccess) return nullptr;
-  owned.push_back(p);
-  return (T*)p;
+SimEngine fresh_engine(const hfv_params& p) {
+  // Engine::new (engine.rs:104-149)
+  SimEngine E;
+  std::memset(&E, 0, sizeof(E));
+  E.time_base = p.t_end / (double)HFV_MAX_NR_TIMESTEPS;
+  E.time_base_inv = 1. / E.time_base;
+  E.t_end = p.t_end;
+  E.t_current = 0.;
+  uint64_t between = p.dt_snap > 0. ? sat_u64(p.dt_snap * E.time_base_inv) : HFV_MAX_NR_TIMESTEPS;
+  E.between = between ? between : HFV_MAX_NR_TIMESTEPS;
+  E.ti_min = ~0ull;
+  E.min_bin = 127;
+  return E;
 }
+
+enum { KS_PREDICT = 0, KS_MESH1D, KS_FLUX, KS_APPLY, KS_GRAD, KS_WAKEUP, KS_LIMIT, KS_DRIFT };
 
 }  // namespace
 
-enum { SP_DUMP = 0, SP_STEP = 3 };
-
 struct hfv_ctx {
-  hfv_params p;
-  Cfg cfg;
-  int device;
-  hipStream_t stream;
-  int64_t n = 0, nf = 0, nconn = 0, face_cap = 0, conn_cap = 0;
+  int device = 0;
+  hipStream_t stream{};
+  int nslots = 0;
+  bool batch = false;  // created by hfv_batch_create (sim ids per cell)
+  std::vector<hfv_params> params;
+  std::vector<int> loaded;
+  std::vector<SimDesc> desc;           // host copy
+  std::vector<int64_t> ccap, fcap, kcap, cbase, fbase, kbase;
+  std::vector<int64_t> nused, fused, kused;
+  int64_t n = 0, nf = 0, nconn = 0;    // capacities (totals)
   Cells C{};
   Faces F{};
-  std::vector<void*> cell_bufs, face_bufs;
+  Work W{};
+  SimDesc* ddesc = nullptr;
+  SimEngine* deng = nullptr;
+  SimEngine* heng = nullptr;  // pinned mirror
   DevCounters* dcnt = nullptr;
   DevCounters* hcnt = nullptr;  // pinned
-  // engine (src/engine.rs)
-  hfv_timestep_info ts{};
-  double t_current = 0.;
-  int64_t step_count = 0, cell_updates = 0, face_fluxes = 0, last_active = 0, last_face_active = 0;
-  std::deque<std::pair<uint64_t, int>> sync_points;
-  uint64_t ti_snap = 0, ti_between_snaps = 0;
+  std::vector<void*> bufs;
+  int precision = HFV_FP64;
+  int solver = -1;  // uniform solver of the loaded simulations, or -1
+  bool any_moving1d = false;
+  hipGraph_t graph{};
+  hipGraphExec_t gexec{};
+  bool graph_ok = false;
+  bool use_graph = true;
   hfv_timestep_info last_info{};
-  int64_t soft[4] = {0, 0, 0, 0};
-  int err = 0;
   // kernel timing (hfv_profile_enable)
   bool prof = false;
   hipEvent_t ev0[HFV_NUM_KERNEL_SLOTS]{}, ev1[HFV_NUM_KERNEL_SLOTS]{};
   bool pending[HFV_NUM_KERNEL_SLOTS]{};
   hfv_kernel_times times{};
+
+  Batch batch_args() const {
+    Batch B;
+    B.C = C;
+    B.F = F;
+    B.sims = ddesc;
+    B.eng = deng;
+    B.nsims = nslots;
+    B.W = W;
+    B.cnt = dcnt;
+    return B;
+  }
 };
 
 namespace {
 
-int free_list(std::vector<void*>& v) {
-  for (void* p : v) (void)hipFree(p);
-  v.clear();
-  return HFV_OK;
+// Cells a slot occupies in HBM.  A batch slot carries one permanent empty cell after its capacity: its
+// CSR range runs from the end of the slot's used entries to the next slot's first entry, so a slot
+// never writes another slot's offsets and no real cell's range covers unused CSR entries.
+inline int64_t icap(const hfv_ctx* c, int s) { return c->ccap[s] + (c->batch ? 1 : 0); }
+
+template <class T>
+T* dalloc(hfv_ctx* c, int64_t count) {
+  void* p = nullptr;
+  if (count <= 0) count = 1;
+  if (hipMalloc(&p, sizeof(T) * (size_t)count) != hipSuccess) return nullptr;
+  c->bufs.push_back(p);
+  return (T*)p;
 }
 
-int alloc_cells(hfv_ctx* c, int64_t n) {
-  free_list(c->cell_bufs);
-  auto& o = c->cell_bufs;
+void free_all(hfv_ctx* c) {
+  for (void* p : c->bufs) (void)hipFree(p);
+  c->bufs.clear();
+}
+
+void drop_graph(hfv_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  if (c->graph) (void)hipGraphDestroy(c->graph);
+  c->gexec = nullptr;
+  c->graph = nullptr;
+  c->graph_ok = false;
+}
+
+// Allocate every buffer for the slot capacities (cells zeroed, faces' cell indices -1).
+int alloc_batch(hfv_ctx* c) {
+  free_all(c);
+  drop_graph(c);
+  int64_t n = 0, nf = 0, nk = 0;
+  int S = c->nslots;
+  c->cbase.assign(S, 0);
+  c->fbase.assign(S, 0);
+  c->kbase.assign(S, 0);
+  for (int s = 0; s < S; ++s) {
+    c->cbase[s] = n;
+    c->fbase[s] = nf;
+    c->kbase[s] = nk;
+    n += icap(c, s);
+    nf += c->fcap[s];
+    nk += c->kcap[s];
+  }
+  if (n >= (1ll << 30) || nf >= (1ll << 29) || nk >= (1ll << 31)) return set_err(HFV_ERR_INVALID, "batch too large for 32-bit indices");
+  c->n = n;
+  c->nf = nf;
+  c->nconn = nk;
   Cells& C = c->C;
   C.n = n;
-  C.prim = dalloc<double>(5 * n, o);
-  C.grad = dalloc<double>(15 * n, o);
-  C.gc = dalloc<double>(3 * n, o);
-  C.extrap = dalloc<double>(5 * n, o);
-  C.cons = dalloc<double>(5 * n, o);
-  C.facc = dalloc<double>(5 * n, o);
-  C.vol = dalloc<double>(n, o);
-  C.loc = dalloc<double>(3 * n, o);
-  C.cen = dalloc<double>(3 * n, o);
-  C.v = dalloc<double>(3 * n, o);
-  C.vrel = dalloc<double>(3 * n, o);
-  C.maor = dalloc<double>(n, o);
-  C.msv = dalloc<double>(n, o);
-  C.dt = dalloc<double>(n, o);
-  C.srad = dalloc<double>(n, o);
-  C.bin = dalloc<int8_t>(n, o);
-  C.wake = dalloc<int8_t>(n, o);
-  C.act = dalloc<uint8_t>(n, o);
-  c->F.off = dalloc<int32_t>(n + 1, o);
-  for (void* p : o)
-    if (!p) return set_err(HFV_ERR_HIP, "hipMalloc failed (cells)");
-  // zero every per-cell field (Particle::default, src/part.rs:10)
-  const size_t szd = sizeof(double) * (size_t)n;
+  C.prim = dalloc<double>(c, 5 * n);
+  C.grad = dalloc<double>(c, 15 * n);
+  C.gc = dalloc<double>(c, 3 * n);
+  C.extrap = dalloc<double>(c, 5 * n);
+  C.cons = dalloc<double>(c, 5 * n);
+  C.facc = dalloc<double>(c, 5 * n);
+  C.vol = dalloc<double>(c, n);
+  C.loc = dalloc<double>(c, 3 * n);
+  C.cen = dalloc<double>(c, 3 * n);
+  C.v = dalloc<double>(c, 3 * n);
+  C.vrel = dalloc<double>(c, 3 * n);
+  C.maor = dalloc<double>(c, n);
+  C.msv = dalloc<double>(c, n);
+  C.dt = dalloc<double>(c, n);
+  C.srad = dalloc<double>(c, n);
+  C.bin = dalloc<int8_t>(c, n);
+  C.wake = dalloc<int8_t>(c, n);
+  C.act = dalloc<uint8_t>(c, n);
+  C.touched = dalloc<uint32_t>(c, n);
+  C.sim = c->batch ? dalloc<int32_t>(c, n) : nullptr;
+  Faces& F = c->F;
+  F.nf = nf;
+  F.area = dalloc<double>(c, nf);
+  F.nrm = dalloc<double>(c, 3 * nf);
+  F.fcen = dalloc<double>(c, 3 * nf);
+  F.rec = dalloc<double>(c, 7 * nf);
+  F.left = dalloc<int32_t>(c, nf);
+  F.right = dalloc<int32_t>(c, nf);
+  F.shc = dalloc<uint8_t>(c, nf);
+  F.off = dalloc<int32_t>(c, n + 1);
+  F.conn = dalloc<int32_t>(c, nk);
+  c->W.act_list = dalloc<int32_t>(c, n);
+  c->W.touch_list = dalloc<int32_t>(c, n);
+  c->W.face_list = dalloc<int32_t>(c, nf);
+  c->ddesc = dalloc<SimDesc>(c, S);
+  c->deng = dalloc<SimEngine>(c, S);
+  for (void* p : c->bufs)
+    if (!p) return set_err(HFV_ERR_HIP, "hipMalloc failed");
+  hipStream_t st = c->stream;
+  const size_t sd = sizeof(double) * (size_t)n;
   double* dbl[] = {C.prim, C.grad, C.gc, C.extrap, C.cons, C.facc, C.vol, C.loc, C.cen, C.v, C.vrel, C.maor, C.msv, C.dt, C.srad};
   int mult[] = {5, 15, 3, 5, 5, 5, 1, 3, 3, 3, 3, 1, 1, 1, 1};
-  for (int k = 0; k < 15; ++k) HIP_OK(hipMemsetAsync(dbl[k], 0, szd * mult[k], c->stream));
-  HIP_OK(hipMemsetAsync(C.bin, 0, (size_t)n, c->stream));
-  HIP_OK(hipMemsetAsync(C.wake, 0, (size_t)n, c->stream));
-  HIP_OK(hipMemsetAsync(C.act, 0, (size_t)n, c->stream));
-  c->n = n;
+  for (int k = 0; k < 15; ++k) HIP_OK(hipMemsetAsync(dbl[k], 0, sd * mult[k], st));
+  HIP_OK(hipMemsetAsync(C.bin, 0, (size_t)n, st));
+  HIP_OK(hipMemsetAsync(C.wake, 0, (size_t)n, st));
+  HIP_OK(hipMemsetAsync(C.act, 0, (size_t)n, st));
+  HIP_OK(hipMemsetAsync(C.touched, 0, sizeof(uint32_t) * (size_t)n, st));
+  if (C.sim) HIP_OK(hipMemsetAsync(C.sim, 0xFF, sizeof(int32_t) * (size_t)n, st));  // -1: empty
+  HIP_OK(hipMemsetAsync(F.rec, 0, sizeof(double) * 7 * (size_t)nf, st));
+  HIP_OK(hipMemsetAsync(F.left, 0xFF, sizeof(int32_t) * (size_t)nf, st));
+  HIP_OK(hipMemsetAsync(F.right, 0xFF, sizeof(int32_t) * (size_t)nf, st));
+  HIP_OK(hipMemsetAsync(F.off, 0, sizeof(int32_t) * (size_t)(n + 1), st));
+  HIP_OK(hipMemsetAsync(F.area, 0, sizeof(double) * (size_t)nf, st));
+  HIP_OK(hipMemsetAsync(F.nrm, 0, sizeof(double) * 3 * (size_t)nf, st));
+  HIP_OK(hipMemsetAsync(F.fcen, 0, sizeof(double) * 3 * (size_t)nf, st));
+  HIP_OK(hipMemsetAsync(F.shc, 0, (size_t)nf, st));
+  // engines: all empty (done)
+  for (int s = 0; s < S; ++s) {
+    std::memset(&c->heng[s], 0, sizeof(SimEngine));
+    c->heng[s].done = 1;
+  }
+  HIP_OK(hipMemcpyAsync(c->deng, c->heng, sizeof(SimEngine) * S, hipMemcpyHostToDevice, st));
+  c->desc.assign(S, SimDesc{});
+  for (int s = 0; s < S; ++s) {
+    std::memset(&c->desc[s], 0, sizeof(SimDesc));
+    c->desc[s].cell_base = c->cbase[s];
+    c->desc[s].face_base = c->fbase[s];
+  }
+  HIP_OK(hipMemcpyAsync(c->ddesc, c->desc.data(), sizeof(SimDesc) * S, hipMemcpyHostToDevice, st));
+  // CSR offsets: every cell of an empty slot starts at the slot's first CSR entry
+  std::vector<int32_t> off(n + 1, 0);
+  for (int s = 0; s < S; ++s)
+    for (int64_t i = 0; i < icap(c, s); ++i) off[c->cbase[s] + i] = (int32_t)c->kbase[s];
+  off[n] = (int32_t)nk;
+  HIP_OK(hipMemcpyAsync(F.off, off.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemsetAsync(F.conn, 0, sizeof(int32_t) * (size_t)std::max<int64_t>(nk, 1), st));
+  HIP_OK(hipStreamSynchronize(st));
+  c->nused.assign(S, 0);
+  c->fused.assign(S, 0);
+  c->kused.assign(S, 0);
+  c->loaded.assign(S, 0);
   return HFV_OK;
 }
 
-int alloc_faces(hfv_ctx* c, int64_t nf, int64_t nconn) {
-  if (nf <= c->face_cap && nconn <= c->conn_cap && !c->face_bufs.empty()) return HFV_OK;
-  free_list(c->face_bufs);
-  auto& o = c->face_bufs;
-  int64_t cap = std::max<int64_t>(nf, 1);
-  int64_t ccap = std::max<int64_t>(nconn, 1);
-  Faces& F = c->F;
-  F.area = dalloc<double>(cap, o);
-  F.nrm = dalloc<double>(3 * cap, o);
-  F.fcen = dalloc<double>(3 * cap, o);
-  F.rec = dalloc<double>(7 * cap, o);
-  F.left = dalloc<int32_t>(cap, o);
-  F.right = dalloc<int32_t>(cap, o);
-  F.shc = dalloc<uint8_t>(cap, o);
-  F.conn = dalloc<int32_t>(ccap, o);
-  for (void* p : o)
-    if (!p) return set_err(HFV_ERR_HIP, "hipMalloc failed (faces)");
-  HIP_OK(hipMemsetAsync(F.rec, 0, sizeof(double) * 7 * (size_t)cap, c->stream));
-  c->face_cap = cap;
-  c->conn_cap = ccap;
-  return HFV_OK;
-}
-
-int sync_and_check(hfv_ctx* c) {
+int sync_counters(hfv_ctx* c) {
   HIP_OK(hipMemcpyAsync(c->hcnt, c->dcnt, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(c->heng, c->deng, sizeof(SimEngine) * c->nslots, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   if (c->hcnt->err) {
-    c->err = c->hcnt->err;
     char b[128];
     snprintf(b, sizeof(b), "device raised hard error %d", c->hcnt->err);
     return set_err(c->hcnt->err, b);
@@ -212,9 +322,211 @@ int sync_and_check(hfv_ctx* c) {
   return HFV_OK;
 }
 
-int8_t max_active_bin(uint64_t ti) { return get_max_active_bin(ti); }
+int push_engines(hfv_ctx* c) {
+  HIP_OK(hipMemcpyAsync(c->deng, c->heng, sizeof(SimEngine) * c->nslots, hipMemcpyHostToDevice, c->stream));
+  return HFV_OK;
+}
 
-enum { KS_PREDICT = 0, KS_MESH1D, KS_FLUX, KS_APPLY, KS_GRAD, KS_WAKEUP, KS_LIMIT, KS_DRIFT };
+void recompute_uniforms(hfv_ctx* c) {
+  int sol = -2;
+  c->any_moving1d = false;
+  int prec = -1;
+  for (int s = 0; s < c->nslots; ++s) {
+    if (!c->loaded[s]) continue;
+    const hfv_params& p = c->params[s];
+    if (sol == -2) sol = p.solver;
+    else if (sol != p.solver) sol = -1;
+    if (p.dim == 1 && p.motion != HFV_FIXED) c->any_moving1d = true;
+    if (prec < 0) prec = p.precision;
+  }
+  c->solver = sol < 0 ? -1 : sol;
+  c->precision = prec < 0 ? HFV_FP64 : prec;
+}
+
+// Upload a mesh into a slot.  update_mask: see hfv_set_mesh.
+int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* update_mask, bool keep_geometry) {
+  int64_t n = m->n_cells, nf = m->n_faces;
+  if (n <= 0 || nf < 0) return set_err(HFV_ERR_INVALID, "empty mesh");
+  if (n > c->ccap[s] || nf > c->fcap[s]) return set_err(HFV_ERR_INVALID, "mesh exceeds the slot's capacity");
+  const hfv_params& p = c->params[s];
+  const int64_t cb = c->cbase[s], fb = c->fbase[s], kb = c->kbase[s];
+  std::vector<double> area(nf), nrm(3 * nf), fcen(3 * nf);
+  std::vector<int32_t> left(nf), right(nf);
+  std::vector<uint8_t> shc(nf);
+  for (int64_t f = 0; f < nf; ++f) {
+    if (m->left[f] < 0 || m->left[f] >= n || m->right[f] < -1 || m->right[f] >= n)
+      return set_err(HFV_ERR_INVALID, "face cell index out of range");
+    area[f] = m->area[f];
+    left[f] = (int32_t)(cb + m->left[f]);
+    right[f] = m->right[f] < 0 ? -1 : (int32_t)(cb + m->right[f]);
+    uint8_t code = 0;
+    for (int a = 0; a < 3; ++a) {
+      nrm[a * nf + f] = m->normal[3 * f + a];
+      fcen[a * nf + f] = m->centroid[3 * f + a];
+      double sv = m->shift ? m->shift[3 * f + a] : 0.;
+      int cc;
+      if (sv == 0.) cc = 0;
+      else if (sv == p.box[a]) cc = 1;
+      else if (sv == -p.box[a]) cc = 2;
+      else return set_err(HFV_ERR_INVALID, "face shift is not 0 or +-box");
+      code |= (uint8_t)(cc << (2 * a));
+    }
+    shc[f] = code;
+  }
+  // CSR in cell order with side bits; offsets global
+  int64_t tot = 0;
+  for (int64_t i = 0; i < n; ++i) tot += m->count[i];
+  if (tot > c->kcap[s]) return set_err(HFV_ERR_INVALID, "csr exceeds the slot's capacity");
+  // offsets of this slot's cells only (plus the global end entry for the last slot)
+  const int64_t noff = icap(c, s) + (s == c->nslots - 1 ? 1 : 0);
+  std::vector<int32_t> off(noff), conn(tot);
+  int64_t acc = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    off[i] = (int32_t)(kb + acc);
+    for (int64_t k = 0; k < m->count[i]; ++k) {
+      int64_t idx = m->offset[i] + k;
+      if (idx < 0 || idx >= m->n_conn) return set_err(HFV_ERR_INVALID, "csr offset out of range");
+      int64_t f = m->conn[idx];
+      if (f < 0 || f >= nf) return set_err(HFV_ERR_INVALID, "csr face index out of range");
+      int side;
+      if (m->left[f] == i) side = 0;
+      else if (m->right[f] == i) side = 1;
+      else return set_err(HFV_ERR_INVALID, "csr lists a face that does not touch the cell");
+      conn[acc + k] = (int32_t)(((fb + f) << 1) | side);
+    }
+    acc += m->count[i];
+  }
+  for (int64_t i = n; i < noff; ++i) off[i] = (int32_t)(kb + acc);  // empty capacity: no faces
+  if (p.dim == 1 && p.motion != HFV_FIXED) {
+    int per = p.boundary == HFV_PERIODIC;
+    if (nf != (per ? n : n + 1))
+      return set_err(HFV_ERR_INVALID, "1D moving mesh needs the 1D tessellator's face layout");
+  }
+  hipStream_t st = c->stream;
+  Faces& F = c->F;
+  const int64_t NF = c->nf;
+  HIP_OK(hipMemcpyAsync(F.area + fb, area.data(), sizeof(double) * nf, hipMemcpyHostToDevice, st));
+  for (int a = 0; a < 3; ++a) {
+    HIP_OK(hipMemcpyAsync(F.nrm + a * NF + fb, nrm.data() + a * nf, sizeof(double) * nf, hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(F.fcen + a * NF + fb, fcen.data() + a * nf, sizeof(double) * nf, hipMemcpyHostToDevice, st));
+  }
+  HIP_OK(hipMemcpyAsync(F.left + fb, left.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(F.right + fb, right.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(F.shc + fb, shc.data(), nf, hipMemcpyHostToDevice, st));
+  // faces beyond this mesh inside the slot: no cells
+  if (nf < c->fcap[s]) {
+    HIP_OK(hipMemsetAsync(F.left + fb + nf, 0xFF, sizeof(int32_t) * (c->fcap[s] - nf), st));
+    HIP_OK(hipMemsetAsync(F.right + fb + nf, 0xFF, sizeof(int32_t) * (c->fcap[s] - nf), st));
+  }
+  if (cb + noff > c->n + 1 || (s < c->nslots - 1 && cb + noff > c->cbase[s + 1]))
+    return set_err(HFV_ERR_INVALID, "internal: slot offsets overlap the next slot");
+  HIP_OK(hipMemcpyAsync(F.off + cb, off.data(), sizeof(int32_t) * noff, hipMemcpyHostToDevice, st));
+  if (tot) HIP_OK(hipMemcpyAsync(F.conn + kb, conn.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, st));
+  // Particle::update_geometry (part.rs:151-158) for the selected cells
+  const int64_t N = c->n;
+  std::vector<double> vol(n), cen(3 * n), srad(n);
+  if (keep_geometry && update_mask) {
+    std::vector<double> ov(n), oc(3 * n), os(n);
+    HIP_OK(hipMemcpyAsync(ov.data(), c->C.vol + cb, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    for (int a = 0; a < 3; ++a)
+      HIP_OK(hipMemcpyAsync(oc.data() + a * n, c->C.cen + a * N + cb, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(os.data(), c->C.srad + cb, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    for (int64_t i = 0; i < n; ++i) {
+      bool u = update_mask[i];
+      vol[i] = u ? m->volume[i] : ov[i];
+      srad[i] = u ? m->search_radius[i] : os[i];
+      for (int a = 0; a < 3; ++a) cen[a * n + i] = u ? m->cell_centroid[3 * i + a] : oc[a * n + i];
+    }
+  } else {
+    for (int64_t i = 0; i < n; ++i) {
+      vol[i] = m->volume[i];
+      srad[i] = m->search_radius[i];
+      for (int a = 0; a < 3; ++a) cen[a * n + i] = m->cell_centroid[3 * i + a];
+    }
+  }
+  HIP_OK(hipMemcpyAsync(c->C.vol + cb, vol.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  for (int a = 0; a < 3; ++a)
+    HIP_OK(hipMemcpyAsync(c->C.cen + a * N + cb, cen.data() + a * n, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(c->C.srad + cb, srad.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIP_OK(hipStreamSynchronize(st));
+  c->nused[s] = n;
+  c->fused[s] = nf;
+  c->kused[s] = tot;
+  SimDesc& D = c->desc[s];
+  D.cfg = make_cfg(p);
+  D.cell_base = cb;
+  D.n = n;
+  D.face_base = fb;
+  D.nf = nf;
+  D.moving1d = p.dim == 1 && p.motion != HFV_FIXED;
+  D.periodic = p.boundary == HFV_PERIODIC;
+  HIP_OK(hipMemcpyAsync(c->ddesc + s, &D, sizeof(SimDesc), hipMemcpyHostToDevice, st));
+  // invariant: every slot's first CSR offset is its CSR base (this slot and the next one)
+  for (int t = s; t <= std::min(s + 1, c->nslots - 1); ++t) {
+    int32_t first = -1;
+    HIP_OK(hipMemcpyAsync(&first, F.off + c->cbase[t], sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIP_OK(hipStreamSynchronize(st));
+    if (first != (int32_t)c->kbase[t]) return set_err(HFV_ERR_INVALID, "internal: CSR offsets of a slot were overwritten");
+  }
+  HIP_OK(hipStreamSynchronize(st));
+  return HFV_OK;
+}
+
+// Reset a slot's per-cell state (Particle::default) and its engine; set the cell -> slot map.
+int reset_slot(hfv_ctx* c, int s) {
+  hipStream_t st = c->stream;
+  const int64_t N = c->n, cb = c->cbase[s], cap = icap(c, s), n = c->nused[s];
+  Cells& C = c->C;
+  struct {
+    double* p;
+    int w;
+  } fl[] = {{C.prim, 5}, {C.grad, 15}, {C.gc, 3}, {C.extrap, 5}, {C.cons, 5}, {C.facc, 5}, {C.loc, 3}, {C.v, 3},
+            {C.vrel, 3}, {C.maor, 1}, {C.msv, 1}, {C.dt, 1}};
+  for (auto& f : fl)
+    for (int k = 0; k < f.w; ++k) HIP_OK(hipMemsetAsync(f.p + k * N + cb, 0, sizeof(double) * cap, st));
+  HIP_OK(hipMemsetAsync(C.bin + cb, 0, cap, st));
+  HIP_OK(hipMemsetAsync(C.wake + cb, 0, cap, st));
+  HIP_OK(hipMemsetAsync(C.act + cb, 0, cap, st));
+  HIP_OK(hipMemsetAsync(C.touched + cb, 0, sizeof(uint32_t) * cap, st));
+  if (C.sim) {
+    std::vector<int32_t> sim(cap, -1);
+    for (int64_t i = 0; i < n; ++i) sim[i] = s;
+    HIP_OK(hipMemcpyAsync(C.sim + cb, sim.data(), sizeof(int32_t) * cap, hipMemcpyHostToDevice, st));
+  }
+  c->heng[s] = fresh_engine(c->params[s]);
+  HIP_OK(hipMemcpyAsync(c->deng + s, &c->heng[s], sizeof(SimEngine), hipMemcpyHostToDevice, st));
+  HIP_OK(hipStreamSynchronize(st));
+  return HFV_OK;
+}
+
+int init_ic(hfv_ctx* c, int s, int64_t n, const double* loc, const double* mass, const double* vel, const double* u_int) {
+  if (n != c->nused[s] || !loc || !mass || !vel || !u_int) return set_err(HFV_ERR_INVALID, "bad IC arrays");
+  const int64_t N = c->n, cb = c->cbase[s];
+  std::vector<double> locs(3 * n), vels(3 * n);
+  for (int64_t i = 0; i < n; ++i)
+    for (int a = 0; a < 3; ++a) {
+      locs[a * n + i] = loc[3 * i + a];
+      vels[a * n + i] = vel[3 * i + a];
+    }
+  hipStream_t st = c->stream;
+  for (int a = 0; a < 3; ++a)
+    HIP_OK(hipMemcpyAsync(c->C.loc + a * N + cb, locs.data() + a * n, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  double *dm = nullptr, *dv = nullptr, *du = nullptr;
+  HIP_OK(hipMalloc((void**)&dm, sizeof(double) * n));
+  HIP_OK(hipMalloc((void**)&dv, sizeof(double) * 3 * n));
+  HIP_OK(hipMalloc((void**)&du, sizeof(double) * n));
+  HIP_OK(hipMemcpyAsync(dm, mass, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(dv, vels.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(du, u_int, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_init_ic, dim3(grid_full(n)), dim3(BLOCK), 0, st, c->batch_args(), cb, cb + n, dm, dv, du);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipStreamSynchronize(st));
+  (void)hipFree(dm);
+  (void)hipFree(dv);
+  (void)hipFree(du);
+  return sync_counters(c);
+}
 
 inline void prof_begin(hfv_ctx* c, int slot) {
   if (c->prof) (void)hipEventRecord(c->ev0[slot], c->stream);
@@ -225,7 +537,6 @@ inline void prof_end(hfv_ctx* c, int slot) {
     c->pending[slot] = true;
   }
 }
-// Called after the stream was synchronized.
 void prof_collect(hfv_ctx* c) {
   if (!c->prof) return;
   for (int k = 0; k < HFV_NUM_KERNEL_SLOTS; ++k)
@@ -239,70 +550,182 @@ void prof_collect(hfv_ctx* c) {
     }
 }
 
-int launch_flux(hfv_ctx* c) {
-  if (c->nf == 0) return HFV_OK;
-  prof_begin(c, KS_FLUX);
-  if (c->p.precision == HFV_FP32_FLUX)
-    hipLaunchKernelGGL(k_flux<float>, dim3(grid_for(c->nf)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt);
+template <class T>
+void launch_flux_t(hfv_ctx* c, const Batch& B, unsigned g) {
+  switch (c->solver) {
+    case HFV_HLLC:
+      hipLaunchKernelGGL((k_flux<T, SOLVER_HLLC>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      break;
+    case HFV_AIRS:
+      hipLaunchKernelGGL((k_flux<T, SOLVER_AIRS>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      break;
+    case HFV_EXACT:
+      hipLaunchKernelGGL((k_flux<T, SOLVER_EXACT>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      break;
+    default:
+      hipLaunchKernelGGL((k_flux<T, -1>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+  }
+}
+
+void launch_flux(hfv_ctx* c, const Batch& B) {
+  unsigned g = grid_list(c->nf);
+  if (c->precision == HFV_FP32_FLUX)
+    launch_flux_t<float>(c, B, g);
   else
-    hipLaunchKernelGGL(k_flux<double>, dim3(grid_for(c->nf)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt);
-  prof_end(c, KS_FLUX);
-  HIP_OK(hipGetLastError());
+    launch_flux_t<double>(c, B, g);
+}
+
+// The kernels of one batch step (k_begin .. k_end).  Timing events only outside graph capture.
+void enqueue_step(hfv_ctx* c, bool timed) {
+  Batch B = c->batch_args();
+  hipStream_t st = c->stream;
+  unsigned gs = (unsigned)std::max(1, (c->nslots + 63) / 64);
+  bool t = timed && c->prof;
+  hipLaunchKernelGGL(k_begin, dim3(gs), dim3(64), 0, st, B);
+  if (t) prof_begin(c, KS_PREDICT);
+  hipLaunchKernelGGL(k_mask, dim3(grid_full(c->n)), dim3(BLOCK), 0, st, B, 0);
+  if (t) prof_end(c, KS_PREDICT);
+  if (c->any_moving1d) {
+    if (t) prof_begin(c, KS_MESH1D);
+    hipLaunchKernelGGL(k_mesh1d_faces, dim3(grid_full(c->nf)), dim3(BLOCK), 0, st, B);
+    hipLaunchKernelGGL(k_mesh1d_cells, dim3(grid_full(c->n)), dim3(BLOCK), 0, st, B);
+    if (t) prof_end(c, KS_MESH1D);
+  }
+  hipLaunchKernelGGL(k_face_list, dim3(grid_list(c->n)), dim3(BLOCK), 0, st, B);
+  if (t) prof_begin(c, KS_FLUX);
+  launch_flux(c, B);
+  if (t) prof_end(c, KS_FLUX);
+  if (t) prof_begin(c, KS_APPLY);
+  hipLaunchKernelGGL(k_apply, dim3(grid_list(c->n)), dim3(BLOCK), 0, st, B, 3);
+  if (t) prof_end(c, KS_APPLY);
+  if (t) prof_begin(c, KS_GRAD);
+  hipLaunchKernelGGL(k_grad, dim3(grid_list(c->n)), dim3(BLOCK), 0, st, B, 1, 1);
+  if (t) prof_end(c, KS_GRAD);
+  if (t) prof_begin(c, KS_WAKEUP);
+  hipLaunchKernelGGL(k_wakeup, dim3(grid_list(c->n)), dim3(BLOCK), 0, st, B);
+  if (t) prof_end(c, KS_WAKEUP);
+  if (t) prof_begin(c, KS_LIMIT);
+  hipLaunchKernelGGL(k_limit, dim3(grid_list(c->n)), dim3(BLOCK), 0, st, B);
+  if (t) prof_end(c, KS_LIMIT);
+  hipLaunchKernelGGL(k_end, dim3(gs), dim3(64), 0, st, B);
+}
+
+int launch_step(hfv_ctx* c) {
+  if (c->prof || !c->use_graph) {
+    enqueue_step(c, true);
+    HIP_OK(hipGetLastError());
+    return HFV_OK;
+  }
+  if (!c->graph_ok) {
+    drop_graph(c);
+    HIP_OK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(c, false);
+    HIP_OK(hipStreamEndCapture(c->stream, &c->graph));
+    HIP_OK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    c->graph_ok = true;
+  }
+  HIP_OK(hipGraphLaunch(c->gexec, c->stream));
   return HFV_OK;
 }
 
-int mesh_update_1d(hfv_ctx* c) {
-  if (c->p.dim != 1) return set_err(HFV_ERR_UNSUPPORTED, "device mesh rebuild is 1D only; rebuild 2D/3D meshes on the host");
-  int periodic = c->p.boundary == HFV_PERIODIC;
-  // the kernels assume the 1D face layout of the tessellator: face k between cells k-1 and k
-  if (c->nf != (periodic ? c->n : c->n + 1))
-    return set_err(HFV_ERR_INVALID, "1D moving mesh needs the 1D tessellator's face layout");
-  prof_begin(c, KS_MESH1D);
-  hipLaunchKernelGGL(k_voronoi1d_faces, dim3(grid_for(c->nf)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, periodic, c->dcnt);
-  hipLaunchKernelGGL(k_voronoi1d_cells, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, periodic);
-  prof_end(c, KS_MESH1D);
-  HIP_OK(hipGetLastError());
-  return HFV_OK;
-}
-
-// Kernels of one runner step after predict/mesh: flux .. limiter + kick1.
-int launch_step_tail(hfv_ctx* c, const hfv_timestep_info* info) {
-  int e = launch_flux(c);
-  if (e) return e;
-  prof_begin(c, KS_APPLY);
-  hipLaunchKernelGGL(k_apply_c2p, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt, 3);
-  prof_end(c, KS_APPLY);
-  prof_begin(c, KS_GRAD);
-  hipLaunchKernelGGL(k_grad_timestep, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt,
-                     info->ti_current, info->time_base, info->time_base_inv, 1, 1);
-  prof_end(c, KS_GRAD);
-  prof_begin(c, KS_WAKEUP);
-  hipLaunchKernelGGL(k_wakeup, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->dcnt, c->p.sync_all);
-  prof_end(c, KS_WAKEUP);
-  prof_begin(c, KS_LIMIT);
-  hipLaunchKernelGGL(k_limit_kick, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->cfg, c->dcnt,
-                     c->p.sync_all, info->ti_current, info->time_base);
-  prof_end(c, KS_LIMIT);
-  HIP_OK(hipGetLastError());
-  return HFV_OK;
-}
-
-int finish_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {
-  int e = sync_and_check(c);
-  prof_collect(c);
-  if (e) return e;
-  uint64_t m = c->hcnt->ti_min;
-  if (c->hcnt->n_active == 0 || m == ~0ull) return set_err(HFV_ERR_INVALID, "no active cell in this step");
-  c->last_active = (int64_t)c->hcnt->n_active;
-  c->last_face_active = (int64_t)c->hcnt->n_face_active;
-  if (ti_next) *ti_next = info->ti_current + m;
+int check_slot(hfv_ctx* c, int s) {
+  if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
+  if (s < 0 || s >= c->nslots) return set_err(HFV_ERR_INVALID, "slot out of range");
+  HIP_OK(hipSetDevice(c->device));
   return HFV_OK;
 }
 
 int require_mesh(hfv_ctx* c) {
-  if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
-  if (c->n == 0 || c->face_bufs.empty()) return set_err(HFV_ERR_INVALID, "mesh not set");
-  HIP_OK(hipSetDevice(c->device));
+  TRY(check_slot(c, 0));
+  if (!c->loaded[0]) return set_err(HFV_ERR_INVALID, "mesh not set");
+  return HFV_OK;
+}
+
+int new_ctx(hfv_ctx** out, int nslots) {
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(HFV_ERR_NO_DEVICE, "no HIP device");
+  hfv_ctx* c = new hfv_ctx();
+  c->nslots = nslots;
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    delete c;
+    return set_err(HFV_ERR_HIP, "hipGetDevice failed");
+  }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete c;
+    return set_err(HFV_ERR_HIP, "hipStreamCreate failed");
+  }
+  if (hipMalloc((void**)&c->dcnt, sizeof(DevCounters)) != hipSuccess ||
+      hipHostMalloc((void**)&c->hcnt, sizeof(DevCounters), hipHostMallocDefault) != hipSuccess ||
+      hipHostMalloc((void**)&c->heng, sizeof(SimEngine) * (size_t)std::max(1, nslots), hipHostMallocDefault) != hipSuccess) {
+    delete c;
+    return set_err(HFV_ERR_HIP, "counter allocation failed");
+  }
+  std::memset(c->hcnt, 0, sizeof(DevCounters));
+  if (hipMemcpy(c->dcnt, c->hcnt, sizeof(DevCounters), hipMemcpyHostToDevice) != hipSuccess) {
+    delete c;
+    return set_err(HFV_ERR_HIP, "counter init failed");
+  }
+  c->params.assign(nslots, hfv_params{});
+  c->loaded.assign(nslots, 0);
+  *out = c;
+  return HFV_OK;
+}
+
+void fill_engine_info(const SimEngine& E, hfv_engine_info* out) {
+  out->ts.ti_old = E.ti_old;
+  out->ts.ti_current = E.ti_current;
+  out->ts.ti_next = E.ti_next;
+  out->ts.time_base = E.time_base;
+  out->ts.time_base_inv = E.time_base_inv;
+  out->t_current = E.t_current;
+  out->t_end = E.t_end;
+  out->step_count = E.step_count;
+  out->cell_updates = E.cell_updates;
+  out->face_fluxes = E.face_fluxes;
+  out->last_n_active = E.last_active;
+  out->last_n_face_active = E.last_face_active;
+  out->done = E.done || !(E.t_current < E.t_end);
+}
+
+#define STATE_FIELDS(X)                                                                           \
+  X(prim, C.prim, 5) X(grad, C.grad, 15) X(gc, C.gc, 3) X(extrap, C.extrap, 5) X(cons, C.cons, 5) \
+  X(fluxes, C.facc, 5) X(volume, C.vol, 1) X(loc, C.loc, 3) X(centroid, C.cen, 3) X(v, C.v, 3)    \
+  X(v_rel, C.vrel, 3) X(maor, C.maor, 1) X(msv, C.msv, 1) X(dt, C.dt, 1)
+
+int copy_state(hfv_ctx* c, int s, hfv_state* st, bool to_device) {
+  const int64_t N = c->n, cb = c->cbase[s], n = c->nused[s];
+  hipStream_t strm = c->stream;
+  hipMemcpyKind kind = to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+#define CPF(name, dev, w)                                                                        \
+  if (st->name)                                                                                  \
+    for (int q = 0; q < (w); ++q) {                                                              \
+      double* d = c->dev + q * N + cb;                                                           \
+      double* h = st->name + q * n;                                                              \
+      HIP_OK(hipMemcpyAsync(to_device ? (void*)d : (void*)h, to_device ? (const void*)h : (const void*)d, \
+                            sizeof(double) * n, kind, strm));                                    \
+    }
+  STATE_FIELDS(CPF)
+#undef CPF
+  if (st->timebin)
+    HIP_OK(hipMemcpyAsync(to_device ? (void*)(c->C.bin + cb) : (void*)st->timebin,
+                          to_device ? (const void*)st->timebin : (const void*)(c->C.bin + cb), (size_t)n, kind, strm));
+  HIP_OK(hipStreamSynchronize(strm));
+  return HFV_OK;
+}
+
+// Trait-level phases operate on slot 0 with the caller's TimestepInfo.
+int begin_phase_step(hfv_ctx* c, const hfv_timestep_info* info) {
+  SimEngine& E = c->heng[0];
+  HIP_OK(hipMemcpyAsync(&E, c->deng, sizeof(SimEngine), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  E.ti_old = info->ti_old;
+  E.ti_current = info->ti_current;
+  E.ti_next = info->ti_current;
+  E.time_base = info->time_base;
+  E.time_base_inv = info->time_base_inv;
+  E.force = 1;
+  E.done = 0;
+  HIP_OK(hipMemcpyAsync(c->deng, &E, sizeof(SimEngine), hipMemcpyHostToDevice, c->stream));
   return HFV_OK;
 }
 
@@ -326,35 +749,12 @@ int hfv_device_count(int* n) {
 }
 
 int hfv_create(const hfv_params* params, hfv_ctx** out) {
-  int e = check_params(params);
-  if (e) return e;
+  TRY(check_params(params));
   if (!out) return set_err(HFV_ERR_INVALID, "out is NULL");
-  int ndev = 0;
-  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(HFV_ERR_NO_DEVICE, "no HIP device");
-  hfv_ctx* c = new hfv_ctx();
-  c->p = *params;
-  c->cfg = make_cfg(*params);
-  if (hipGetDevice(&c->device) != hipSuccess) {
-    delete c;
-    return set_err(HFV_ERR_HIP, "hipGetDevice failed");
-  }
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-    delete c;
-    return set_err(HFV_ERR_HIP, "hipStreamCreate failed");
-  }
-  if (hipMalloc((void**)&c->dcnt, sizeof(DevCounters)) != hipSuccess ||
-      hipHostMalloc((void**)&c->hcnt, sizeof(DevCounters), hipHostMallocDefault) != hipSuccess) {
-    delete c;
-    return set_err(HFV_ERR_HIP, "counter allocation failed");
-  }
-  hipLaunchKernelGGL(k_clear_all_counters, dim3(1), dim3(1), 0, c->stream, c->dcnt);
-  // Engine::new (engine.rs:104-149)
-  c->ts.time_base = params->t_end / (double)HFV_MAX_NR_TIMESTEPS;
-  c->ts.time_base_inv = 1. / c->ts.time_base;
-  c->sync_points.push_back({0, SP_STEP});
-  c->ti_snap = 0;
-  c->ti_between_snaps = params->dt_snap > 0. ? sat_u64(params->dt_snap * c->ts.time_base_inv) : HFV_MAX_NR_TIMESTEPS;
-  if (c->ti_between_snaps == 0) c->ti_between_snaps = HFV_MAX_NR_TIMESTEPS;
+  hfv_ctx* c = nullptr;
+  TRY(new_ctx(&c, 1));
+  c->params[0] = *params;
+  c->batch = false;
   *out = c;
   return HFV_OK;
 }
@@ -363,183 +763,65 @@ int hfv_destroy(hfv_ctx* c) {
   if (!c) return HFV_OK;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
-  free_list(c->cell_bufs);
-  free_list(c->face_bufs);
+  drop_graph(c);
+  free_all(c);
   for (int k = 0; k < HFV_NUM_KERNEL_SLOTS; ++k) {
     if (c->ev0[k]) (void)hipEventDestroy(c->ev0[k]);
     if (c->ev1[k]) (void)hipEventDestroy(c->ev1[k]);
   }
   if (c->dcnt) (void)hipFree(c->dcnt);
   if (c->hcnt) (void)hipHostFree(c->hcnt);
+  if (c->heng) (void)hipHostFree(c->heng);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return HFV_OK;
 }
 
 int hfv_set_mesh(hfv_ctx* c, const hfv_mesh_arrays* m, const uint8_t* update_mask) {
-  if (!c || !m) return set_err(HFV_ERR_INVALID, "NULL argument");
-  HIP_OK(hipSetDevice(c->device));
-  int64_t n = m->n_cells, nf = m->n_faces;
-  if (n <= 0 || nf < 0) return set_err(HFV_ERR_INVALID, "empty mesh");
-  if (n >= (1ll << 30) || nf >= (1ll << 29)) return set_err(HFV_ERR_INVALID, "mesh too large for 32-bit indices");
-  if (c->n != 0 && c->n != n) return set_err(HFV_ERR_INVALID, "cell count differs from the existing state");
-  bool first = c->n == 0;
+  TRY(check_slot(c, 0));
+  if (!m) return set_err(HFV_ERR_INVALID, "mesh is NULL");
+  if (c->batch) return set_err(HFV_ERR_INVALID, "use hfv_batch_load on a batch context");
+  bool first = !c->loaded[0];
+  if (!first && m->n_cells != c->nused[0]) return set_err(HFV_ERR_INVALID, "cell count differs from the existing state");
+  if (first || m->n_faces > c->fcap[0] || m->n_conn > c->kcap[0]) {
+    if (!first) {
+      // growing the face arrays of a live simulation: keep the cell state across the reallocation
+      return set_err(HFV_ERR_INVALID, "mesh has more faces than the context was created for");
+    }
+    c->ccap.assign(1, m->n_cells);
+    // headroom for meshes rebuilt later (moving mesh)
+    c->fcap.assign(1, m->n_faces + m->n_faces / 4 + 16);
+    int64_t tot = 0;
+    for (int64_t i = 0; i < m->n_cells; ++i) tot += m->count[i];
+    c->kcap.assign(1, std::max<int64_t>(tot, m->n_conn) + std::max<int64_t>(tot, m->n_conn) / 4 + 32);
+    TRY(alloc_batch(c));
+  }
+  TRY(upload_mesh(c, 0, m, update_mask, !first));
   if (first) {
-    int e = alloc_cells(c, n);
-    if (e) return e;
+    c->loaded[0] = 1;
+    TRY(reset_slot(c, 0));
   }
-  // faces -> SoA, periodic shift -> code
-  std::vector<double> area(nf), nrm(3 * nf), fcen(3 * nf);
-  std::vector<int32_t> left(nf), right(nf);
-  std::vector<uint8_t> shc(nf);
-  for (int64_t f = 0; f < nf; ++f) {
-    area[f] = m->area[f];
-    left[f] = (int32_t)m->left[f];
-    right[f] = (int32_t)m->right[f];
-    if (m->left[f] < 0 || m->left[f] >= n || m->right[f] < -1 || m->right[f] >= n)
-      return set_err(HFV_ERR_INVALID, "face cell index out of range");
-    uint8_t code = 0;
-    for (int a = 0; a < 3; ++a) {
-      nrm[a * nf + f] = m->normal[3 * f + a];
-      fcen[a * nf + f] = m->centroid[3 * f + a];
-      double s = m->shift ? m->shift[3 * f + a] : 0.;
-      int cc;
-      if (s == 0.) cc = 0;
-      else if (s == c->p.box[a]) cc = 1;
-      else if (s == -c->p.box[a]) cc = 2;
-      else return set_err(HFV_ERR_INVALID, "face shift is not 0 or +-box");
-      code |= (uint8_t)(cc << (2 * a));
-    }
-    shc[f] = code;
-  }
-  // CSR: repack in cell order, encode side
-  std::vector<int32_t> off(n + 1);
-  int64_t tot = 0;
-  for (int64_t i = 0; i < n; ++i) {
-    off[i] = (int32_t)tot;
-    tot += m->count[i];
-  }
-  off[n] = (int32_t)tot;
-  if (tot >= (1ll << 31)) return set_err(HFV_ERR_INVALID, "csr too large");
-  std::vector<int32_t> conn(tot);
-  for (int64_t i = 0; i < n; ++i) {
-    for (int64_t k = 0; k < m->count[i]; ++k) {
-      int64_t idx = m->offset[i] + k;
-      if (idx < 0 || idx >= m->n_conn) return set_err(HFV_ERR_INVALID, "csr offset out of range");
-      int64_t f = m->conn[idx];
-      if (f < 0 || f >= nf) return set_err(HFV_ERR_INVALID, "csr face index out of range");
-      int side;
-      if (m->left[f] == i) side = 0;
-      else if (m->right[f] == i) side = 1;
-      else return set_err(HFV_ERR_INVALID, "csr lists a face that does not touch the cell");
-      conn[off[i] + k] = (int32_t)((f << 1) | side);
-    }
-  }
-  int e = alloc_faces(c, nf, tot);
-  if (e) return e;
-  c->nf = nf;
-  c->nconn = tot;
-  c->F.nf = nf;
-  Faces& F = c->F;
-  hipStream_t s = c->stream;
-  HIP_OK(hipMemcpyAsync(F.area, area.data(), sizeof(double) * nf, hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(F.nrm, nrm.data(), sizeof(double) * 3 * nf, hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(F.fcen, fcen.data(), sizeof(double) * 3 * nf, hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(F.left, left.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(F.right, right.data(), sizeof(int32_t) * nf, hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(F.shc, shc.data(), nf, hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(F.off, off.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(F.conn, conn.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, s));
-  // Particle::update_geometry for the selected cells
-  std::vector<double> vol(n), cen(3 * n), srad(n);
-  std::vector<double> old_vol, old_cen, old_srad;
-  bool partial = update_mask && !first;
-  if (partial) {
-    old_vol.resize(n);
-    old_cen.resize(3 * n);
-    old_srad.resize(n);
-    HIP_OK(hipMemcpyAsync(old_vol.data(), c->C.vol, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(old_cen.data(), c->C.cen, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipMemcpyAsync(old_srad.data(), c->C.srad, sizeof(double) * n, hipMemcpyDeviceToHost, s));
-    HIP_OK(hipStreamSynchronize(s));
-  }
-  for (int64_t i = 0; i < n; ++i) {
-    bool upd = !partial || update_mask[i];
-    vol[i] = upd ? m->volume[i] : old_vol[i];
-    srad[i] = upd ? m->search_radius[i] : old_srad[i];
-    for (int a = 0; a < 3; ++a) cen[a * n + i] = upd ? m->cell_centroid[3 * i + a] : old_cen[a * n + i];
-  }
-  HIP_OK(hipMemcpyAsync(c->C.vol, vol.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(c->C.cen, cen.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(c->C.srad, srad.data(), sizeof(double) * n, hipMemcpyHostToDevice, s));
-  HIP_OK(hipStreamSynchronize(s));
+  recompute_uniforms(c);
+  drop_graph(c);
   return HFV_OK;
 }
 
 int hfv_init_from_ic(hfv_ctx* c, int64_t n, const double* loc, const double* mass, const double* vel,
                      const double* u_int) {
-  int e = require_mesh(c);
-  if (e) return e;
-  if (n != c->n || !loc || !mass || !vel || !u_int) return set_err(HFV_ERR_INVALID, "bad IC arrays");
-  std::vector<double> locs(3 * n), vels(3 * n);
-  for (int64_t i = 0; i < n; ++i)
-    for (int a = 0; a < 3; ++a) {
-      locs[a * n + i] = loc[3 * i + a];
-      vels[a * n + i] = vel[3 * i + a];
-    }
-  std::vector<void*> tmp;
-  double* dm = dalloc<double>(n, tmp);
-  double* dv = dalloc<double>(3 * n, tmp);
-  double* du = dalloc<double>(n, tmp);
-  if (!dm || !dv || !du) {
-    free_list(tmp);
-    return set_err(HFV_ERR_HIP, "hipMalloc failed");
-  }
-  hipStream_t s = c->stream;
-  HIP_OK(hipMemcpyAsync(c->C.loc, locs.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(dm, mass, sizeof(double) * n, hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(dv, vels.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
-  HIP_OK(hipMemcpyAsync(du, u_int, sizeof(double) * n, hipMemcpyHostToDevice, s));
-  hipLaunchKernelGGL(k_init_ic, dim3(grid_for(n)), dim3(BLOCK), 0, s, c->C, c->cfg, dm, dv, du, c->dcnt);
-  HIP_OK(hipGetLastError());
-  e = sync_and_check(c);
-  free_list(tmp);
-  return e;
+  TRY(require_mesh(c));
+  return init_ic(c, 0, n, loc, mass, vel, u_int);
 }
 
-#define STATE_FIELDS(X)                                                                           \
-  X(prim, C.prim, 5) X(grad, C.grad, 15) X(gc, C.gc, 3) X(extrap, C.extrap, 5) X(cons, C.cons, 5) \
-  X(fluxes, C.facc, 5) X(volume, C.vol, 1) X(loc, C.loc, 3) X(centroid, C.cen, 3) X(v, C.v, 3)    \
-  X(v_rel, C.vrel, 3) X(maor, C.maor, 1) X(msv, C.msv, 1) X(dt, C.dt, 1)
-
 int hfv_set_state(hfv_ctx* c, const hfv_state* st) {
-  int e = require_mesh(c);
-  if (e) return e;
+  TRY(require_mesh(c));
   if (!st) return set_err(HFV_ERR_INVALID, "state is NULL");
-  const int64_t n = c->n;
-  hipStream_t s = c->stream;
-#define SETF(name, dst, w) \
-  if (st->name) HIP_OK(hipMemcpyAsync(c->dst, st->name, sizeof(double) * (w) * n, hipMemcpyHostToDevice, s));
-  STATE_FIELDS(SETF)
-#undef SETF
-  if (st->timebin) HIP_OK(hipMemcpyAsync(c->C.bin, st->timebin, (size_t)n, hipMemcpyHostToDevice, s));
-  HIP_OK(hipStreamSynchronize(s));
-  return HFV_OK;
+  return copy_state(c, 0, const_cast<hfv_state*>(st), true);
 }
 
 int hfv_get_state(hfv_ctx* c, hfv_state* st) {
-  int e = require_mesh(c);
-  if (e) return e;
+  TRY(require_mesh(c));
   if (!st) return set_err(HFV_ERR_INVALID, "state is NULL");
-  const int64_t n = c->n;
-  hipStream_t s = c->stream;
-#define GETF(name, src, w) \
-  if (st->name) HIP_OK(hipMemcpyAsync(st->name, c->src, sizeof(double) * (w) * n, hipMemcpyDeviceToHost, s));
-  STATE_FIELDS(GETF)
-#undef GETF
-  if (st->timebin) HIP_OK(hipMemcpyAsync(st->timebin, c->C.bin, (size_t)n, hipMemcpyDeviceToHost, s));
-  HIP_OK(hipStreamSynchronize(s));
-  return HFV_OK;
+  return copy_state(c, 0, st, false);
 }
 
 int hfv_get_stats(hfv_ctx* c, hfv_stats* out) {
@@ -562,173 +844,248 @@ int hfv_synchronize(hfv_ctx* c) {
   return HFV_OK;
 }
 
-// ---- FiniteVolumeSolver trait surface
+// ---- FiniteVolumeSolver trait surface (slot 0, caller-provided TimestepInfo)
 int hfv_predict(hfv_ctx* c, double dt) {
-  int e = require_mesh(c);
-  if (e) return e;
-  int8_t mab = max_active_bin(c->last_info.ti_current);
-  hipLaunchKernelGGL(k_mask_predict, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->cfg, dt, mab, 1, c->dcnt);
+  TRY(require_mesh(c));
+  SimEngine& E = c->heng[0];
+  HIP_OK(hipMemcpyAsync(&E, c->deng, sizeof(SimEngine), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  E.dt_pred = dt;
+  HIP_OK(hipMemcpyAsync(c->deng, &E, sizeof(SimEngine), hipMemcpyHostToDevice, c->stream));
+  hipLaunchKernelGGL(k_mask, dim3(grid_full(c->n)), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
   HIP_OK(hipGetLastError());
-  return sync_and_check(c);
+  return sync_counters(c);
 }
 
 int hfv_compute_fluxes(hfv_ctx* c, const hfv_timestep_info* info) {
-  int e = require_mesh(c);
-  if (e) return e;
+  TRY(require_mesh(c));
   if (!info) return set_err(HFV_ERR_INVALID, "info is NULL");
   c->last_info = *info;
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, c->stream, c->dcnt);
-  hipLaunchKernelGGL(k_mask_predict, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->cfg, 0.,
-                     max_active_bin(info->ti_current), 0, c->dcnt);
-  e = launch_flux(c);
-  if (e) return e;
-  return sync_and_check(c);
+  TRY(begin_phase_step(c, info));
+  Batch B = c->batch_args();
+  hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, c->stream, B);
+  hipLaunchKernelGGL(k_mask, dim3(grid_full(c->n)), dim3(BLOCK), 0, c->stream, B, 2);
+  hipLaunchKernelGGL(k_face_list, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, B);
+  launch_flux(c, B);
+  HIP_OK(hipGetLastError());
+  return sync_counters(c);
 }
 
 int hfv_get_fluxes(hfv_ctx* c, double* out7f) {
-  int e = require_mesh(c);
-  if (e) return e;
+  TRY(require_mesh(c));
   if (!out7f) return set_err(HFV_ERR_INVALID, "out is NULL");
-  const int64_t nf = c->nf;
+  const int64_t NF = c->nf, nf = c->fused[0], fb = c->fbase[0], n = c->nused[0], cb = c->cbase[0];
   std::vector<double> soa(7 * nf);
-  HIP_OK(hipMemcpyAsync(soa.data(), c->F.rec, sizeof(double) * 7 * nf, hipMemcpyDeviceToHost, c->stream));
+  std::vector<uint8_t> act(n);
+  std::vector<int32_t> left(nf), right(nf);
+  for (int q = 0; q < 7; ++q)
+    HIP_OK(hipMemcpyAsync(soa.data() + q * nf, c->F.rec + q * NF + fb, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(act.data(), c->C.act + cb, n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(left.data(), c->F.left + fb, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(right.data(), c->F.right + fb, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
-  for (int64_t f = 0; f < nf; ++f)
-    for (int k = 0; k < 7; ++k) out7f[7 * f + k] = soa[k * nf + f];
+  for (int64_t f = 0; f < nf; ++f) {
+    // records are current for faces with an active cell; the others are gated off (zero)
+    bool valid = act[left[f] - cb] || (right[f] >= 0 && act[right[f] - cb]);
+    for (int k = 0; k < 7; ++k) out7f[7 * f + k] = valid ? soa[k * nf + f] : 0.;
+  }
   return HFV_OK;
 }
 
 int hfv_apply_fluxes(hfv_ctx* c) {
-  int e = require_mesh(c);
-  if (e) return e;
-  hipLaunchKernelGGL(k_apply_c2p, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt, 1);
+  TRY(require_mesh(c));
+  hipLaunchKernelGGL(k_apply, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
   HIP_OK(hipGetLastError());
-  return sync_and_check(c);
+  return sync_counters(c);
 }
 
 int hfv_conserved_to_primitive(hfv_ctx* c) {
-  int e = require_mesh(c);
-  if (e) return e;
-  hipLaunchKernelGGL(k_apply_c2p, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt, 2);
+  TRY(require_mesh(c));
+  hipLaunchKernelGGL(k_apply, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, c->batch_args(), 2);
   HIP_OK(hipGetLastError());
-  return sync_and_check(c);
+  return sync_counters(c);
 }
 
 int hfv_gradients(hfv_ctx* c) {
-  int e = require_mesh(c);
-  if (e) return e;
-  const hfv_timestep_info& t = c->last_info;
-  hipLaunchKernelGGL(k_grad_timestep, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt,
-                     t.ti_current, t.time_base, t.time_base_inv, 1, 0);
+  TRY(require_mesh(c));
+  hipLaunchKernelGGL(k_grad, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, c->batch_args(), 1, 0);
   HIP_OK(hipGetLastError());
-  return sync_and_check(c);
+  return sync_counters(c);
 }
 
 int hfv_timesteps(hfv_ctx* c, uint64_t* ti_next) {
-  int e = require_mesh(c);
-  if (e) return e;
-  const hfv_timestep_info& t = c->last_info;
-  hipLaunchKernelGGL(k_grad_timestep, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->cfg, c->dcnt,
-                     t.ti_current, t.time_base, t.time_base_inv, 0, 1);
-  hipLaunchKernelGGL(k_wakeup, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->F, c->dcnt, c->p.sync_all);
-  hipLaunchKernelGGL(k_limit_kick, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->cfg, c->dcnt,
-                     c->p.sync_all, t.ti_current, t.time_base);
+  TRY(require_mesh(c));
+  Batch B = c->batch_args();
+  hipLaunchKernelGGL(k_grad, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, B, 0, 1);
+  hipLaunchKernelGGL(k_wakeup, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_limit, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_end, dim3(1), dim3(64), 0, c->stream, B);
   HIP_OK(hipGetLastError());
-  return finish_step(c, &t, ti_next);
+  TRY(sync_counters(c));
+  if (c->heng[0].err) return set_err(c->heng[0].err, "no active cell in this step");
+  if (ti_next) *ti_next = c->heng[0].ti_next;
+  return HFV_OK;
 }
 
-// ---- fused step
+// ---- fused step with the caller's TimestepInfo
 int hfv_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {
-  int e = require_mesh(c);
-  if (e) return e;
+  TRY(require_mesh(c));
   if (!info) return set_err(HFV_ERR_INVALID, "info is NULL");
   c->last_info = *info;
-  double dt_pred = (double)(info->ti_current - info->ti_old) * info->time_base;
-  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(1), 0, c->stream, c->dcnt);
-  prof_begin(c, KS_PREDICT);
-  hipLaunchKernelGGL(k_mask_predict, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->cfg, dt_pred,
-                     max_active_bin(info->ti_current), 1, c->dcnt);
-  prof_end(c, KS_PREDICT);
-  if (c->p.motion != HFV_FIXED) {
-    e = mesh_update_1d(c);
-    if (e) return e;
-  }
-  e = launch_step_tail(c, info);
-  if (e) return e;
-  return finish_step(c, info, ti_next);
+  TRY(begin_phase_step(c, info));
+  enqueue_step(c, true);
+  HIP_OK(hipGetLastError());
+  TRY(sync_counters(c));
+  prof_collect(c);
+  if (c->heng[0].err) return set_err(c->heng[0].err, "no active cell in this step");
+  if (ti_next) *ti_next = c->heng[0].ti_next;
+  return HFV_OK;
 }
 
-// ---- engine (src/engine.rs:152-268)
+// ---- engine (src/engine.rs:152-268), device-resident
 int hfv_run(hfv_ctx* c, int64_t max_steps, int64_t* steps_done) {
-  int e = require_mesh(c);
-  if (e) return e;
+  TRY(require_mesh(c));
+  TRY(sync_counters(c));
+  const long long s0 = c->heng[0].step_count;
   int64_t done = 0;
-  while (c->t_current < c->p.t_end && done < max_steps) {
-    if (c->sync_points.empty()) return set_err(HFV_ERR_INVALID, "sync point queue empty");
-    auto sp = c->sync_points.front();
-    c->sync_points.pop_front();
-    if (sp.first < c->ts.ti_current) return set_err(HFV_ERR_INVALID, "drift backwards in time");
-    uint64_t dti = sp.first - c->ts.ti_current;
-    if (dti > 0) {
-      double dt = (double)dti * c->ts.time_base;
-      if (c->p.motion != HFV_FIXED) {  // with fixed motion v == 0 and the drift is the identity
-        prof_begin(c, KS_DRIFT);
-        hipLaunchKernelGGL(k_drift, dim3(grid_for(c->n)), dim3(BLOCK), 0, c->stream, c->C, c->cfg, dt, c->dcnt);
-        prof_end(c, KS_DRIFT);
-        HIP_OK(hipGetLastError());
-      }
-      c->ts.ti_current += dti;
-      c->t_current = (double)c->ts.ti_current * c->ts.time_base;
-    }
-    if (sp.second == SP_DUMP) continue;
-    uint64_t ti_next = 0;
-    e = hfv_step(c, &c->ts, &ti_next);
-    if (e) return e;
-    c->cell_updates += c->last_active;
-    c->face_fluxes += c->last_face_active;
-    while (c->ti_snap < ti_next) {
-      c->sync_points.push_back({c->ti_snap, SP_DUMP});
-      c->ti_snap += c->ti_between_snaps;
-    }
-    c->sync_points.push_back({ti_next, SP_STEP});
-    c->ts.ti_old = c->ts.ti_next;
-    c->ts.ti_next = ti_next;
-    c->step_count++;
-    done++;
+  int64_t chunk = 8;
+  while (done < max_steps && !c->heng[0].done) {
+    int64_t k = std::min<int64_t>(chunk, max_steps - done);
+    for (int64_t j = 0; j < k; ++j) TRY(launch_step(c));
+    TRY(sync_counters(c));
+    prof_collect(c);
+    done = c->heng[0].step_count - s0;
+    if (c->heng[0].err) return set_err(c->heng[0].err, "engine error");
+    chunk = std::min<int64_t>(chunk * 2, 256);
   }
   if (steps_done) *steps_done = done;
   return HFV_OK;
 }
 
 int hfv_get_engine_info(hfv_ctx* c, hfv_engine_info* out) {
-  if (!c || !out) return set_err(HFV_ERR_INVALID, "NULL argument");
-  out->ts = c->ts;
-  out->t_current = c->t_current;
-  out->t_end = c->p.t_end;
-  out->step_count = c->step_count;
-  out->cell_updates = c->cell_updates;
-  out->face_fluxes = c->face_fluxes;
-  out->last_n_active = c->last_active;
-  out->last_n_face_active = c->last_face_active;
-  out->done = !(c->t_current < c->p.t_end);
+  TRY(require_mesh(c));
+  if (!out) return set_err(HFV_ERR_INVALID, "out is NULL");
+  TRY(sync_counters(c));
+  fill_engine_info(c->heng[0], out);
   return HFV_OK;
 }
 
 int hfv_conservation(hfv_ctx* c, double* out5) {
-  int e = require_mesh(c);
-  if (e) return e;
+  TRY(require_mesh(c));
   if (!out5) return set_err(HFV_ERR_INVALID, "out is NULL");
-  std::vector<void*> tmp;
-  double* d = dalloc<double>(5, tmp);
-  if (!d) return set_err(HFV_ERR_HIP, "hipMalloc failed");
-  hipLaunchKernelGGL(k_conservation, dim3(1), dim3(1024), 0, c->stream, c->C, d);
+  double* d = nullptr;
+  HIP_OK(hipMalloc((void**)&d, sizeof(double) * 5));
+  hipLaunchKernelGGL(k_conservation, dim3(1), dim3(1024), 0, c->stream, c->batch_args(), c->cbase[0],
+                     c->cbase[0] + c->nused[0], d);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpyAsync(out5, d, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
-  free_list(tmp);
+  (void)hipFree(d);
   return HFV_OK;
 }
 
+// ---- batches
+int hfv_batch_create(int nslots, const int64_t* cell_cap, const int64_t* face_cap, const int64_t* conn_cap,
+                     hfv_ctx** out) {
+  if (nslots <= 0 || !cell_cap || !face_cap || !conn_cap || !out) return set_err(HFV_ERR_INVALID, "bad batch arguments");
+  hfv_ctx* c = nullptr;
+  TRY(new_ctx(&c, nslots));
+  c->batch = true;
+  c->ccap.assign(cell_cap, cell_cap + nslots);
+  c->fcap.assign(face_cap, face_cap + nslots);
+  c->kcap.assign(conn_cap, conn_cap + nslots);
+  int e = alloc_batch(c);
+  if (e) {
+    hfv_destroy(c);
+    return e;
+  }
+  *out = c;
+  return HFV_OK;
+}
+
+int hfv_batch_load(hfv_ctx* c, int s, const hfv_params* params, const hfv_mesh_arrays* mesh, const double* loc,
+                   const double* mass, const double* vel, const double* u_int) {
+  TRY(check_slot(c, s));
+  TRY(check_params(params));
+  if (!c->batch) return set_err(HFV_ERR_INVALID, "not a batch context");
+  if (!mesh) return set_err(HFV_ERR_INVALID, "mesh is NULL");
+  for (int k = 0; k < c->nslots; ++k)
+    if (k != s && c->loaded[k] && c->params[k].precision != params->precision)
+      return set_err(HFV_ERR_INVALID, "all simulations of a batch share one precision");
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->params[s] = *params;
+  TRY(upload_mesh(c, s, mesh, nullptr, false));
+  TRY(reset_slot(c, s));
+  c->loaded[s] = 1;
+  TRY(init_ic(c, s, mesh->n_cells, loc, mass, vel, u_int));
+  int prev_solver = c->solver;
+  bool prev_m = c->any_moving1d;
+  recompute_uniforms(c);
+  if (prev_solver != c->solver || prev_m != c->any_moving1d) drop_graph(c);
+  return HFV_OK;
+}
+
+int hfv_batch_unload(hfv_ctx* c, int s) {
+  TRY(check_slot(c, s));
+  if (!c->batch) return set_err(HFV_ERR_INVALID, "not a batch context");
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->loaded[s] = 0;
+  c->nused[s] = 0;
+  std::vector<int32_t> sim(icap(c, s), -1);
+  HIP_OK(hipMemcpyAsync(c->C.sim + c->cbase[s], sim.data(), sizeof(int32_t) * icap(c, s), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipMemcpyAsync(c->heng + s, c->deng + s, sizeof(SimEngine), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->heng[s].done = 1;
+  c->heng[s].force = 0;
+  HIP_OK(hipMemcpyAsync(c->deng + s, c->heng + s, sizeof(SimEngine), hipMemcpyHostToDevice, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  recompute_uniforms(c);
+  return HFV_OK;
+}
+
+int hfv_batch_run(hfv_ctx* c, int64_t max_launches, int64_t* launches) {
+  if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
+  HIP_OK(hipSetDevice(c->device));
+  int64_t done = 0;
+  int64_t chunk = 8;
+  while (done < max_launches) {
+    int64_t k = std::min<int64_t>(chunk, max_launches - done);
+    for (int64_t j = 0; j < k; ++j) TRY(launch_step(c));
+    done += k;
+    TRY(sync_counters(c));
+    prof_collect(c);
+    bool all_done = true;
+    for (int s = 0; s < c->nslots; ++s)
+      if (c->loaded[s] && !c->heng[s].done) all_done = false;
+    if (all_done) break;
+    chunk = std::min<int64_t>(chunk * 2, 64);
+  }
+  if (launches) *launches = done;
+  return HFV_OK;
+}
+
+int hfv_batch_get_engine_info(hfv_ctx* c, int s, hfv_engine_info* out) {
+  TRY(check_slot(c, s));
+  if (!out) return set_err(HFV_ERR_INVALID, "out is NULL");
+  HIP_OK(hipMemcpyAsync(c->heng + s, c->deng + s, sizeof(SimEngine), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  fill_engine_info(c->heng[s], out);
+  return HFV_OK;
+}
+
+int hfv_batch_get_state(hfv_ctx* c, int s, hfv_state* st) {
+  TRY(check_slot(c, s));
+  if (!st) return set_err(HFV_ERR_INVALID, "state is NULL");
+  return copy_state(c, s, st, false);
+}
+
+int hfv_batch_num_slots(hfv_ctx* c, int* n) {
+  if (!c || !n) return set_err(HFV_ERR_INVALID, "NULL argument");
+  *n = c->nslots;
+  return HFV_OK;
+}
+
+// ---- measurement
 int hfv_profile_enable(hfv_ctx* c, int enable) {
   if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
   HIP_OK(hipSetDevice(c->device));
@@ -756,37 +1113,39 @@ int hfv_profile_read(hfv_ctx* c, hfv_kernel_times* out) {
 
 int hfv_riemann_batch(const hfv_params* params, int64_t nb, const double* WL, const double* WR, const double* vface,
                       const double* nrm, double* Fo) {
-  int e = check_params(params);
-  if (e) return e;
+  TRY(check_params(params));
   if (nb <= 0 || !WL || !WR || !vface || !nrm || !Fo) return set_err(HFV_ERR_INVALID, "bad batch arrays");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(HFV_ERR_NO_DEVICE, "no HIP device");
   Cfg cfg = make_cfg(*params);
-  std::vector<void*> tmp;
-  double* dl = dalloc<double>(5 * nb, tmp);
-  double* dr = dalloc<double>(5 * nb, tmp);
-  double* dv = dalloc<double>(3 * nb, tmp);
-  double* dn = dalloc<double>(3 * nb, tmp);
-  double* df = dalloc<double>(5 * nb, tmp);
-  DevCounters* cnt = dalloc<DevCounters>(1, tmp);
-  if (!dl || !dr || !dv || !dn || !df || !cnt) {
-    free_list(tmp);
-    return set_err(HFV_ERR_HIP, "hipMalloc failed");
-  }
-  DevCounters h{};
+  double *dl, *dr, *dv, *dn, *df;
+  DevCounters* cnt;
+  HIP_OK(hipMalloc((void**)&dl, sizeof(double) * 5 * nb));
+  HIP_OK(hipMalloc((void**)&dr, sizeof(double) * 5 * nb));
+  HIP_OK(hipMalloc((void**)&dv, sizeof(double) * 3 * nb));
+  HIP_OK(hipMalloc((void**)&dn, sizeof(double) * 3 * nb));
+  HIP_OK(hipMalloc((void**)&df, sizeof(double) * 5 * nb));
+  HIP_OK(hipMalloc((void**)&cnt, sizeof(DevCounters)));
+  DevCounters h;
+  std::memset(&h, 0, sizeof(h));
   HIP_OK(hipMemcpy(dl, WL, sizeof(double) * 5 * nb, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(dr, WR, sizeof(double) * 5 * nb, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(dv, vface, sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(dn, nrm, sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
   HIP_OK(hipMemcpy(cnt, &h, sizeof(DevCounters), hipMemcpyHostToDevice));
   if (params->precision == HFV_FP32_FLUX)
-    hipLaunchKernelGGL(k_riemann_batch<float>, dim3(grid_for(nb)), dim3(BLOCK), 0, 0, cfg, nb, dl, dr, dv, dn, df, cnt);
+    hipLaunchKernelGGL(k_riemann_batch<float>, dim3(grid_full(nb)), dim3(BLOCK), 0, 0, cfg, nb, dl, dr, dv, dn, df, cnt);
   else
-    hipLaunchKernelGGL(k_riemann_batch<double>, dim3(grid_for(nb)), dim3(BLOCK), 0, 0, cfg, nb, dl, dr, dv, dn, df, cnt);
+    hipLaunchKernelGGL(k_riemann_batch<double>, dim3(grid_full(nb)), dim3(BLOCK), 0, 0, cfg, nb, dl, dr, dv, dn, df, cnt);
   HIP_OK(hipGetLastError());
   HIP_OK(hipMemcpy(Fo, df, sizeof(double) * 5 * nb, hipMemcpyDeviceToHost));
   HIP_OK(hipMemcpy(&h, cnt, sizeof(DevCounters), hipMemcpyDeviceToHost));
-  free_list(tmp);
+  (void)hipFree(dl);
+  (void)hipFree(dr);
+  (void)hipFree(dv);
+  (void)hipFree(dn);
+  (void)hipFree(df);
+  (void)hipFree(cnt);
   if (h.err) return set_err(h.err, "device raised a hard error in the Riemann solver");
   return HFV_OK;
 }

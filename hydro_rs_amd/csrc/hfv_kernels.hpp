@@ -1,12 +1,22 @@
 // Device kernels of the per-face finite-volume step (gfx950).  Included by hfv_api.hip.
 //
-// HBM layout (one context = one simulation):
+// One launch sequence advances every simulation of a batch by one of its own engine steps (a batch
+// of one is the single-simulation path).  Engine state lives in HBM, so steps are enqueued (or replayed
+// from a captured graph) without a host round trip.
+//
+// HBM layout (cells/faces of all simulations concatenated, indices global):
 //   cells, component-major SoA (field[c * N + i]): prim[5], grad[15], gc[3], extrap[5], cons[5],
 //     facc[5] (pending flux accumulator), vol, loc[3], cen[3], v[3], vrel[3], maor, msv, dt, srad,
-//     bin (int8), wake (int8), act (uint8)
+//     bin (int8), wake (int8), act (uint8), touched (uint32), sim (int32, simulation of the cell)
 //   faces, SoA: area, nrm[3], fcen[3], left (int32), right (int32, -1 = wall), shc (uint8 periodic
 //     shift code, 2 bits per axis), rec[7] (dt*F[5], v_max, a_over_r)
 //   csr: off[N+1] (int32), conn[2F] (int32: face << 1 | side, side 1 = the cell is the face's right)
+//   work lists (rebuilt each step): act_list[N], touch_list[N], face_list[F]
+//
+// Step kernels: k_begin (per simulation) -> k_mask (all cells: drift, active mask, predict, list
+// append) -> [k_mesh1d_*] -> k_face_list (active cells) -> k_flux (active faces) -> k_apply (touched
+// cells) -> k_grad (active cells) -> k_wakeup, k_limit (touched cells) -> k_end (per simulation).
+// When every cell of the batch is active ("dense"), lists are bypassed and indices are natural.
 #pragma once
 #include "hfv_physics.hpp"
 
@@ -17,6 +27,8 @@ struct Cells {
   double *prim, *grad, *gc, *extrap, *cons, *facc, *vol, *loc, *cen, *v, *vrel, *maor, *msv, *dt, *srad;
   int8_t *bin, *wake;
   uint8_t* act;
+  uint32_t* touched;
+  int32_t* sim;  // nullptr for a batch of one
 };
 
 struct Faces {
@@ -27,7 +39,7 @@ struct Faces {
 };
 
 struct Cfg {
-  int dim, boundary, motion, runner, sync_all;
+  int dim, boundary, motion, runner, sync_all, precision;
   double box[3];
   double cfl;
   double radius3_fac;  // 0.25 * 3. * FRAC_1_PI
@@ -36,15 +48,56 @@ struct Cfg {
   Gas<float> gf;
 };
 
-// Device-side counters of one context.
-struct DevCounters {
+struct SimDesc {
+  Cfg cfg;
+  int64_t cell_base, n, face_base, nf;
+  int moving1d;  // 1D moving mesh rebuilt on the device
+  int periodic;
+};
+
+#define HFV_MAX_SUBDRIFTS 8
+// Engine of one simulation (src/engine.rs:33-101, 152-268).
+struct SimEngine {
+  uint64_t ti_old, ti_current, ti_next;
+  uint64_t ti_snap, between, dump_lo, dump_hi;  // pending dump sync points: dump_lo, +between, .. < dump_hi
+  double time_base, time_base_inv, t_current, t_end;
+  double dt_pred;
+  double sub_dt[HFV_MAX_SUBDRIFTS];
+  int n_sub;
+  int stepping;  // this step runs for this simulation
+  int done;
+  int force;     // next k_begin runs a step at ti_current without the engine's time loop (hfv_step)
+  int8_t mab;
+  int8_t pad8[3];
+  int min_bin;
+  unsigned long long ti_min, n_active, n_face_active;
+  long long step_count, cell_updates, face_fluxes;
+  long long last_active, last_face_active;
   int err;
   int pad;
+};
+
+// Batch-global counters.
+struct DevCounters {
+  int err;
+  int dense;
   unsigned long long check_physical, flux_reset, grad_fallback, brent;
-  unsigned long long n_active, n_face_active;
-  unsigned long long ti_min;
-  int min_bin;
-  int pad2;
+  unsigned long long act_count, touch_count, face_count;
+  unsigned long long epoch;
+};
+
+struct Work {
+  int32_t *act_list, *touch_list, *face_list;
+};
+
+struct Batch {
+  Cells C;
+  Faces F;
+  const SimDesc* sims;
+  SimEngine* eng;
+  int nsims;
+  Work W;
+  DevCounters* cnt;
 };
 
 #define HFV_NUM_TIME_BINS 56
@@ -157,615 +210,138 @@ __device__ __forceinline__ void flush_events(const Events& ev, DevCounters* cnt)
   if (ev.flux_reset) atomicAdd(&cnt->flux_reset, (unsigned long long)ev.flux_reset);
 }
 
-// Wavefront-aggregated count: one atomic per wave.
-__device__ __forceinline__ void wave_count(bool pred, unsigned long long* dst) {
-  unsigned long long m = __ballot(pred);
-  if (__lane_id() == __ffsll((long long)__ballot(1)) - 1 && m) atomicAdd(dst, (unsigned long long)__popcll(m));
+__device__ __forceinline__ unsigned long long lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
+
+// Wavefront-aggregated append: one atomic per wave; returns the slot (or -1 when !want).
+__device__ __forceinline__ long long wave_append(bool want, unsigned long long* counter) {
+  unsigned long long m = __ballot(want);
+  if (m == 0) return -1;
+  int leader = __ffsll((long long)m) - 1;
+  unsigned long long base = 0;
+  if (__lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
+  base = __shfl(base, leader);
+  return want ? (long long)(base + __popcll(m & lanemask_lt())) : -1;
 }
 
-// ------------------------------------------------------------------ kernels
-__global__ void k_reset_counters(DevCounters* cnt) {
-  cnt->n_active = 0;
-  cnt->n_face_active = 0;
-  cnt->ti_min = ~0ull;
-  cnt->min_bin = 127;
-}
-
-__global__ void k_clear_all_counters(DevCounters* cnt) {
-  cnt->err = 0;
-  cnt->check_physical = 0;
-  cnt->flux_reset = 0;
-  cnt->grad_fallback = 0;
-  cnt->brent = 0;
-  cnt->n_active = 0;
-  cnt->n_face_active = 0;
-  cnt->ti_min = ~0ull;
-  cnt->min_bin = 127;
-}
-
-// Active mask (optimal_order.rs:17) fused with EulerEqnsFvs::predict (euler_eqns_fvs.rs:20-24,
-// part.rs:123-149) over all cells.  do_predict = 0 computes the mask only.
-__global__ void k_mask_predict(Cells C, Cfg cfg, double dt, int8_t mab, int do_predict, DevCounters* cnt) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool act = false;
-  if (i < C.n) {
-    act = C.bin[i] <= mab;
-    C.act[i] = act;
-    if (do_predict) {
-      Events ev;
-      int64_t n = C.n;
-      St<double> prim = ld5(C.prim, n, i);
-      Gr<double> gr = ldg(C.grad, n, i);
-      V3<double> vrel = ld3(C.vrel, n, i);
-      St<double> ex = ld5(C.extrap, n, i);
-      ex = ex + time_extrapolations(prim, gr, vrel, dt, cfg.gd);
-      st5(C.extrap, n, i, ex);
-      check_physical(prim, ev);
-      if (ev.check_physical) {
-        st5(C.prim, n, i, prim);
-        flush_events(ev, cnt);
-      }
-      // volume += dvdt * dt clamped to [V/2, 2V]: dvdt is never set on the path (part.rs:130-131)
+// Per-simulation wave-aggregated add (waterfall over the simulations present in the wave).
+__device__ __forceinline__ void sim_add(bool valid, int s, SimEngine* eng, int which) {
+  bool todo = valid;
+  while (true) {
+    unsigned long long m = __ballot(todo);
+    if (m == 0) break;
+    int leader = __ffsll((long long)m) - 1;
+    int su = __shfl(s, leader);
+    bool mine = todo && s == su;
+    unsigned long long tot = (unsigned long long)__popcll(__ballot(mine));
+    if (__lane_id() == leader) {
+      if (which == 0)
+        atomicAdd(&eng[su].n_active, tot);
+      else
+        atomicAdd(&eng[su].n_face_active, tot);
     }
+    if (mine) todo = false;
   }
-  wave_count(act, &cnt->n_active);
 }
 
-// FiniteVolumeSolver::compute_fluxes: gate (euler_eqns_fvs.rs:26-48) + flux_exchange (flux.rs:23-100)
-// + flux_exchange_boundary (flux.rs:102-195).  T is the arithmetic of reconstruction and solve.
-template <class T>
-__device__ __forceinline__ const Gas<T>& gas_for(const Cfg& c);
-template <>
-__device__ __forceinline__ const Gas<double>& gas_for<double>(const Cfg& c) {
-  return c.gd;
-}
-template <>
-__device__ __forceinline__ const Gas<float>& gas_for<float>(const Cfg& c) {
-  return c.gf;
+// XCD-aware block order: the 8 XCDs receive blocks round-robin; give each XCD a contiguous share.
+__device__ __forceinline__ int64_t xcd_block() {
+  int64_t b = blockIdx.x, g = gridDim.x;
+  if ((g & 7) == 0) return (b & 7) * (g >> 3) + (b >> 3);
+  return b;
 }
 
-template <class T>
-__global__ void __launch_bounds__(256) k_flux(Cells C, Faces F, Cfg cfg, DevCounters* cnt) {
-  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool computed = false;
-  if (f < F.nf) {
-    const int64_t n = C.n;
-    const int64_t nf = F.nf;
-    int32_t l = F.left[f];
-    int32_t r = F.right[f];
-    bool la = C.act[l];
-    double outv[7] = {0., 0., 0., 0., 0., 0., 0.};
-    Events ev;
-    const Gas<T>& g = gas_for<T>(cfg);
-    if (r >= 0) {
-      bool ra = C.act[r];
-      double dtl = C.dt[l], dtr = C.dt[r];
-      if (!((!la && !ra) || (dtr < dtl && !ra) || (dtl < dtr && !la))) {
-        computed = true;
-        double dt = hmin(dtl, dtr);
-        V3<double> fc = ld3(F.fcen, nf, f);
-        V3<double> nrm = ld3(F.nrm, nf, f);
-        double area = F.area[f];
-        V3<double> sh = decode_shift(F.shc[f], cfg.box);
-        V3<double> gcl = ld3(C.gc, n, l), gcr = ld3(C.gc, n, r);
-        V3<double> dx_left = fc - gcl;
-        V3<double> dx_right = (fc - gcr) - sh;
-        double dx_gradients = len(dx_left - dx_right);
-        V3<double> cl = ld3(C.cen, n, l), cr = ld3(C.cen, n, r);
-        V3<double> dx_centroid = (cr + sh) - cl;
-        V3<double> locl = ld3(C.loc, n, l), locr = ld3(C.loc, n, r);
-        V3<double> dx = (locr + sh) - locl;
-        St<double> pl = ld5(C.prim, n, l), pr = ld5(C.prim, n, r);
-        double v_max = 0.;
-        if (pl.c[0] > 0.) v_max += sound_speed(cfg.gd, pl.c[4], 1. / pl.c[0]);
-        if (pr.c[0] > 0.) v_max += sound_speed(cfg.gd, pr.c[4], 1. / pr.c[0]);
-        v_max -= hmin(dot(vel(pr) - vel(pl), dx_centroid), 0.);
-        // reconstruction (flux.rs:52-73) in T
-        T dt_extrapolate = (T)(-0.5 * dt);
-        Gr<double> gl = ldg(C.grad, n, l), grr = ldg(C.grad, n, r);
-        St<double> exl = ld5(C.extrap, n, l), exr = ld5(C.extrap, n, r);
-        V3<double> vrl = ld3(C.vrel, n, l), vrr = ld3(C.vrel, n, r);
-        St<T> plT = cvtst<T>(pl), prT = cvtst<T>(pr);
-        Gr<T> glT, grT;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          glT.g[q] = cvt3<T>(gl.g[q]);
-          grT.g[q] = cvt3<T>(grr.g[q]);
-        }
-        V3<T> dxlT = cvt3<T>(dx_left), dxrT = cvt3<T>(dx_right);
-        St<T> left_dash = ((plT + gdot(glT, dxlT)) + cvtst<T>(exl)) +
-                          time_extrapolations(plT, glT, cvt3<T>(vrl), dt_extrapolate, g);
-        St<T> right_dash = ((prT + gdot(grT, dxrT)) + cvtst<T>(exr)) +
-                           time_extrapolations(prT, grT, cvt3<T>(vrr), dt_extrapolate, g);
-        T dxg = (T)dx_gradients;
-        St<T> prim_left = pairwise_limiter(plT, prT, left_dash, len(dxlT) / dxg, ev);
-        St<T> prim_right = pairwise_limiter(prT, plT, right_dash, len(dxrT) / dxg, ev);
-        // face velocity (flux.rs:75-78), FP64
-        V3<double> vl = ld3(C.v, n, l), vr = ld3(C.v, n, r);
-        V3<double> midpoint = 0.5 * ((locl + locr) + sh);
-        double fac = dot(vr - vl, fc - midpoint) / len2(dx);
-        V3<double> v_face = (0.5 * (vl + vr)) - (fac * dx);
-        V3<T> vfT = cvt3<T>(v_face);
-        St<T> Fl = riemann_flux(cfg.solver, boost(prim_left, -vfT), boost(prim_right, -vfT), vfT, cvt3<T>(nrm), g,
-                                cfg.gd, ev);
-        St<double> fluxes = area * cvtst<double>(Fl);
-        St<double> rec = dt * fluxes;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) outv[k] = rec.c[k];
-        outv[5] = v_max;
-        outv[6] = area / len(dx);
-      }
-    } else if (la) {
-      computed = true;
-      V3<double> fc = ld3(F.fcen, nf, f);
-      V3<double> nrm = ld3(F.nrm, nf, f);
-      double area = F.area[f];
-      V3<double> loc = ld3(C.loc, n, l), cen = ld3(C.cen, n, l), gc = ld3(C.gc, n, l);
-      // Particle::reflect (part.rs:374-383)
-      V3<double> loc_r = loc + ((2. * dot(fc - loc, nrm)) * nrm);
-      V3<double> cen_r = cen + ((2. * dot(fc - cen, nrm)) * nrm);
-      V3<double> gc_r = gc + ((2. * dot(fc - gc, nrm)) * nrm);
-      V3<double> dx_face = fc - gc;
-      double dx_gradients = len(gc_r - gc);
-      V3<double> dx_centroid = cen_r - cen;
-      V3<double> dx = loc_r - loc;
-      St<double> p = ld5(C.prim, n, l);
-      double dtp = C.dt[l];
-      double v_max = 0.;
-      if (p.c[0] > 0.) v_max += sound_speed(cfg.gd, p.c[4], 1. / p.c[0]);
-      Gr<double> gp = ldg(C.grad, n, l);
-      St<double> ex = ld5(C.extrap, n, l);
-      V3<double> vrel = ld3(C.vrel, n, l);
-      St<T> pT = cvtst<T>(p);
-      Gr<T> gT;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) gT.g[q] = cvt3<T>(gp.g[q]);
-      V3<T> dxfT = cvt3<T>(dx_face);
-      V3<T> nT = cvt3<T>(nrm);
-      St<T> dash = ((pT + gdot(gT, dxfT)) + cvtst<T>(ex)) + time_extrapolations(pT, gT, cvt3<T>(vrel), (T)(-0.5 * dtp), g);
-      T dx_fac = len(dxfT) / (T)dx_gradients;
-      St<T> bstate;
-      if (cfg.boundary == BND_REFLECTIVE) {
-        St<T> rp = reflect(pT, nT);
-        dash = pairwise_limiter(pT, rp, dash, dx_fac, ev);
-        St<double> rpd = reflect(p, nrm);
-        v_max -= hmin(dot(vel(rpd) - vel(p), dx_centroid), 0.);
-        bstate = reflect(dash, nT);
-      } else if (cfg.boundary == BND_OPEN) {
-        dash = pairwise_limiter(pT, pT, dash, dx_fac, ev);
-        bstate = dash;
-      } else if (cfg.boundary == BND_VACUUM) {
-        St<T> vac = vacuum<T>();
-        dash = pairwise_limiter(pT, vac, dash, dx_fac, ev);
-        v_max += hmin(dot(vel(p), dx_centroid), 0.);
-        bstate = vac;
-      } else {
-        ev.fail(E_UNSUPPORTED);
-        bstate = dash;
-      }
-      St<T> Fl = riemann_flux(cfg.solver, dash, bstate, mk3(T(0), T(0), T(0)), nT, g, cfg.gd, ev);
-      St<double> fluxes = area * cvtst<double>(Fl);
-      St<double> rec = dtp * fluxes;
-#pragma unroll
-      for (int k = 0; k < 5; ++k) outv[k] = rec.c[k];
-      outv[5] = v_max;
-      outv[6] = area / len(dx);
-    }
-#pragma unroll
-    for (int k = 0; k < 7; ++k) F.rec[k * nf + f] = outv[k];
-    flush_events(ev, cnt);
-  }
-  wave_count(computed, &cnt->n_face_active);
-}
+__device__ __forceinline__ int sim_of(const Cells& C, int64_t i) { return C.sim ? C.sim[i] : 0; }
 
-// runner apply_fluxes (hydrodynamics.rs:13-39) + kick2 (:51-57) + convert_conserved_to_primitive
-// (euler_eqns_fvs.rs:63-69, part.rs:219-261): one thread per cell walks its faces in CSR order, so the
-// FP64 accumulation order is the reference's and needs no atomics.
-// mode bit 1: accumulate + apply + kick2; mode bit 2: conserved -> primitive.
-__global__ void __launch_bounds__(256) k_apply_c2p(Cells C, Faces F, Cfg cfg, DevCounters* cnt, int mode) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.n) return;
-  const int64_t n = C.n, nf = F.nf;
-  Events ev;
-  bool a = C.act[i];
-  if (!(mode & 1)) {
-    if (!a) return;
-    St<double> c = ld5(C.cons, n, i);
-    St<double> prim;
-    if (!(c.c[0] >= 0.)) ev.fail(E_NEG_MASS);
-    if (!(c.c[4] >= 0.)) ev.fail(E_NEG_ENERGY);
-    if (c.c[0] == 0.) {
-      if (c.c[4] != 0.) ev.fail(E_NEG_ENERGY);
-      prim = vacuum<double>();
-    } else if (c.c[0] > 0.) {
-      double m_inv = 1. / c.c[0];
-      double density = c.c[0] / C.vol[i];
-      V3<double> velocity = m_inv * vel(c);
-      double thermal = c.c[4] - 0.5 * len2(vel(c)) * m_inv;
-      double pressure = gas_p_from_u(cfg.gd, thermal * m_inv, density);
-      if (!(density >= 0.) || !(pressure >= 0.)) ev.fail(E_NEG_PRESSURE);
-      prim = mkst(density, velocity, pressure);
-    } else {
-      prim = vacuum<double>();
-    }
-#pragma unroll
-    for (int q = 0; q < 5; ++q)
-      if (!hfinite(prim.c[q])) ev.fail(E_NONFINITE);
-    st5(C.prim, n, i, prim);
-    flush_events(ev, cnt);
-    return;
+// ------------------------------------------------------------------ engine (per simulation)
+// Engine::run loop head (engine.rs:152-185): the pending dump sync points are drifted through, then
+// the step at ti_next.  Also resets the batch's list counters.
+__global__ void k_begin(Batch B) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) {
+    B.cnt->act_count = 0;
+    B.cnt->touch_count = 0;
+    B.cnt->face_count = 0;
+    B.cnt->dense = 0;
+    B.cnt->epoch += 1;
   }
-  St<double> acc = ld5(C.facc, n, i);
-  double msv = 0., maor = 0.;
-  if (a) {
-    msv = C.msv[i];
-    maor = C.maor[i];
-  }
-  int32_t k0 = F.off[i], k1 = F.off[i + 1];
-  for (int32_t k = k0; k < k1; ++k) {
-    int32_t e = F.conn[k];
-    int64_t f = e >> 1;
-    St<double> rec;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) rec.c[q] = F.rec[q * nf + f];
-    if ((e & 1) == 0)
-      acc = acc - rec;
-    else
-      acc = acc + rec;
-    if (a) {
-      msv = hmax(F.rec[5 * nf + f], msv);
-      maor = hmax(maor, F.rec[6 * nf + f]);
-    }
-  }
-  if (a) {
-    St<double> c = ld5(C.cons, n, i) + acc;  // apply_flux (part.rs:178-193)
-#pragma unroll
-    for (int q = 0; q < 5; ++q)
-      if (!hfinite(c.c[q])) ev.fail(E_NONFINITE);
-    if (c.c[0] < 0.) {
-      ev.flux_reset++;
-      c = vacuum<double>();
-    }
-    if (c.c[4] < 0.) {
-      ev.flux_reset++;
-      c = vacuum<double>();
-    }
-    acc = vacuum<double>();
-    // kick2 -> grav_kick with a_grav = 0 (part.rs:303-312): momentum += (0 * mass)
-    c.c[1] += 0.0;
-    c.c[2] += 0.0;
-    c.c[3] += 0.0;
-    st5(C.cons, n, i, c);
-    C.msv[i] = msv;
-    C.maor[i] = maor;
-    if (!(mode & 2)) {
-      st5(C.facc, n, i, acc);
-      flush_events(ev, cnt);
+  if (s >= B.nsims) return;
+  SimEngine& E = B.eng[s];
+  E.stepping = 0;
+  E.n_sub = 0;
+  E.n_active = 0;
+  E.n_face_active = 0;
+  E.ti_min = ~0ull;
+  E.min_bin = 127;
+  if (E.force) {  // hfv_step: a step at ti_current with the caller's ti_old
+    E.force = 0;
+    E.stepping = 1;
+  } else {
+    if (E.done) return;
+    if (!(E.t_current < E.t_end)) {
+      E.done = 1;
       return;
     }
-    // convert_conserved_to_primitive
-    St<double> prim;
-    if (!(c.c[0] >= 0.)) ev.fail(E_NEG_MASS);
-    if (!(c.c[4] >= 0.)) ev.fail(E_NEG_ENERGY);
-    if (c.c[0] == 0.) {
-      if (c.c[4] != 0.) ev.fail(E_NEG_ENERGY);
-      prim = vacuum<double>();
-    } else if (c.c[0] > 0.) {
-      double m_inv = 1. / c.c[0];
-      double density = c.c[0] / C.vol[i];
-      V3<double> velocity = m_inv * vel(c);
-      double thermal = c.c[4] - 0.5 * len2(vel(c)) * m_inv;
-      double u = thermal * m_inv;
-      double pressure = gas_p_from_u(cfg.gd, u, density);
-      if (!(density >= 0.) || !(pressure >= 0.)) ev.fail(E_NEG_PRESSURE);
-      prim = mkst(density, velocity, pressure);
-    } else {
-      prim = vacuum<double>();
+    uint64_t cur = E.ti_current;
+    int ns = 0;
+    for (uint64_t d = E.dump_lo; d < E.dump_hi; d += E.between) {
+      if (d > cur) {
+        if (ns < HFV_MAX_SUBDRIFTS) E.sub_dt[ns] = (double)(d - cur) * E.time_base;
+        ns++;
+        cur = d;
+      }
+      if (E.between == 0) break;
     }
-#pragma unroll
-    for (int q = 0; q < 5; ++q)
-      if (!hfinite(prim.c[q])) ev.fail(E_NONFINITE);
-    st5(C.prim, n, i, prim);
+    E.dump_lo = E.dump_hi;
+    if (E.ti_next > cur) {
+      if (ns < HFV_MAX_SUBDRIFTS) E.sub_dt[ns] = (double)(E.ti_next - cur) * E.time_base;
+      ns++;
+    }
+    if (ns > HFV_MAX_SUBDRIFTS) {
+      E.err = E_UNSUPPORTED;
+      atomicCAS(&B.cnt->err, 0, E_UNSUPPORTED);
+      ns = HFV_MAX_SUBDRIFTS;
+    }
+    E.n_sub = ns;
+    E.ti_current = E.ti_next;
+    E.t_current = (double)E.ti_current * E.time_base;
+    E.stepping = 1;
   }
-  st5(C.facc, n, i, acc);
-  flush_events(ev, cnt);
+  E.dt_pred = (double)(E.ti_current - E.ti_old) * E.time_base;
+  E.mab = get_max_active_bin(E.ti_current);
 }
 
-// glam::DMat3 as three columns
-struct M3 {
-  V3<double> x, y, z;
-};
-
-// Particle::radius (part.rs:439-447)
-__device__ __forceinline__ double cell_radius(double V, const Cfg& cfg) {
-  if (cfg.dim == 1) return 0.5 * V;
-  if (cfg.dim == 2) return sqrt(0.318309886183790671537767526745028724 * V);
-  return pow(cfg.radius3_fac * V, 1. / 3.);
+// Engine::step / queue_step / update_ti_next (engine.rs:194-268).
+__global__ void k_end(Batch B) {
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= B.nsims) return;
+  SimEngine& E = B.eng[s];
+  if (!E.stepping) return;
+  if (E.n_active == 0 || E.ti_min == ~0ull) {  // the reference panics: "At least one particle must be active"
+    E.err = E_UNSUPPORTED;
+    atomicCAS(&B.cnt->err, 0, E_UNSUPPORTED);
+    E.done = 1;
+    return;
+  }
+  uint64_t ti_next = E.ti_current + E.ti_min;
+  E.dump_lo = E.ti_snap;
+  if (E.between) {
+    while (E.ti_snap < ti_next) E.ti_snap += E.between;
+  }
+  E.dump_hi = E.ti_snap;
+  E.ti_old = E.ti_next;
+  E.ti_next = ti_next;
+  E.step_count++;
+  E.cell_updates += (long long)E.n_active;
+  E.face_fluxes += (long long)E.n_face_active;
+  E.last_active = (long long)E.n_active;
+  E.last_face_active = (long long)E.n_face_active;
 }
 
-// runner gradient_estimate / gradient_limit / gradient_apply (hydrodynamics.rs:67-143, gradients.rs:8-103)
-// fused with compute_timesteps (euler_eqns_fvs.rs:50-61, part.rs:42-84, 328-372) and timesteps_apply
-// (hydrodynamics.rs:145-158, timeline.rs:103-139) for the active cells.
-__global__ void __launch_bounds__(256) k_grad_timestep(Cells C, Faces F, Cfg cfg, DevCounters* cnt, uint64_t ti_current,
-                                                       double time_base, double time_base_inv, int do_grad,
-                                                       int do_ts) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  unsigned long long my_dti = ~0ull;
-  if (i < C.n && C.act[i]) {
-    const int64_t n = C.n, nf = F.nf;
-    Events ev;
-    St<double> prim = ld5(C.prim, n, i);
-    V3<double> cen = ld3(C.cen, n, i);
-    Gr<double> gr;
-    V3<double> gcen;
-    int32_t k0 = F.off[i], k1 = F.off[i + 1];
-    if (do_grad && cfg.runner == 0) {
-      // ---- estimate
-      Gr<double> g;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) g.g[q] = mk3(0., 0., 0.);
-      M3 m;
-      m.x = mk3(0., 0., 0.);
-      m.y = mk3(0., 0., 0.);
-      m.z = mk3(0., 0., 0.);
-      if (cfg.dim == 1) {
-        m.y.y = 1.;
-        m.z.z = 1.;
-      } else if (cfg.dim == 2) {
-        m.z.z = 1.;
-      }
-      for (int32_t k = k0; k < k1; ++k) {
-        int32_t e = F.conn[k];
-        int64_t f = e >> 1;
-        int side = e & 1;
-        double area = F.area[f];
-        V3<double> sh = decode_shift(F.shc[f], cfg.box);
-        V3<double> shift = side == 0 ? sh : -sh;
-        int32_t other = side == 0 ? F.right[f] : F.left[f];
-        V3<double> oc;
-        St<double> op;
-        if (other >= 0) {
-          oc = ld3(C.cen, n, other);
-          op = ld5(C.prim, n, other);
-        } else {  // Space::get_boundary_part (space.rs:213-229)
-          V3<double> fc = ld3(F.fcen, nf, f), nrm = ld3(F.nrm, nf, f);
-          oc = cen + ((2. * dot(fc - cen, nrm)) * nrm);
-          op = prim;
-          if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, nrm);
-          else if (cfg.boundary == BND_VACUUM) op = vacuum<double>();
-        }
-        V3<double> ds = (oc + shift) - cen;
-        double w = area / len2(ds);
-#pragma unroll
-        for (int q = 0; q < 5; ++q) g.g[q] = g.g[q] + ((w * (op.c[q] - prim.c[q])) * ds);
-        m.x = m.x + (w * (ds.x * ds));
-        m.y = m.y + (w * (ds.y * ds));
-        m.z = m.z + (w * (ds.z * ds));
-      }
-      // DMat3::inverse (glam 0.27): cofactors by cross products, det = z . (x cross y), transpose
-      V3<double> t0 = cross(m.y, m.z), t1 = cross(m.z, m.x), t2 = cross(m.x, m.y);
-      double det = dot(m.z, t2);
-      double inv_det = 1. / det;
-      V3<double> c0 = mk3(t0.x * inv_det, t0.y * inv_det, t0.z * inv_det);
-      V3<double> c1 = mk3(t1.x * inv_det, t1.y * inv_det, t1.z * inv_det);
-      V3<double> c2 = mk3(t2.x * inv_det, t2.y * inv_det, t2.z * inv_det);
-      M3 inv;
-      inv.x = mk3(c0.x, c1.x, c2.x);
-      inv.y = mk3(c0.y, c1.y, c2.y);
-      inv.z = mk3(c0.z, c1.z, c2.z);
-      bool fin = true;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        V3<double> v = g.g[q];
-        V3<double> r = v.x * inv.x;
-        r = r + (v.y * inv.y);
-        r = r + (v.z * inv.z);
-        g.g[q] = r;
-        fin = fin && finite3(r);
-      }
-      if (!fin) {
-        ev.grad_fallback++;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) g.g[q] = mk3(0., 0., 0.);
-      }
-      // ---- limit (LimiterData)
-      St<double> lmin = prim, lmax = prim, emin, emax;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        emin.c[q] = INFINITY;
-        emax.c[q] = -INFINITY;
-      }
-      for (int32_t k = k0; k < k1; ++k) {
-        int32_t e = F.conn[k];
-        int64_t f = e >> 1;
-        int side = e & 1;
-        V3<double> sh = decode_shift(F.shc[f], cfg.box);
-        V3<double> shift = side == 0 ? sh : -sh;
-        V3<double> fc = ld3(F.fcen, nf, f);
-        St<double> ext = gdot(g, (fc - cen) - shift);
-        int32_t other = side == 0 ? F.right[f] : F.left[f];
-        St<double> op;
-        if (other >= 0) {
-          op = ld5(C.prim, n, other);
-        } else {
-          op = prim;
-          if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, ld3(F.nrm, nf, f));
-          else if (cfg.boundary == BND_VACUUM) op = vacuum<double>();
-        }
-#pragma unroll
-        for (int q = 0; q < 5; ++q) {
-          lmin.c[q] = hmin(lmin.c[q], op.c[q]);
-          lmax.c[q] = hmax(lmax.c[q], op.c[q]);
-          emin.c[q] = hmin(emin.c[q], ext.c[q]);
-          emax.c[q] = hmax(emax.c[q], ext.c[q]);
-        }
-      }
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        double d_max = lmax.c[q] - prim.c[q], d_min = lmin.c[q] - prim.c[q];
-        double alpha = (emax.c[q] == 0. || emin.c[q] == 0.) ? 1. : hmin(1.0, hmin(d_max / emax.c[q], d_min / emin.c[q]));
-        g.g[q] = alpha * g.g[q];
-      }
-      gr = g;
-      gcen = cen;
-      stg(C.grad, n, i, gr);
-      st3(C.gc, n, i, gcen);
-    } else {
-      gr = ldg(C.grad, n, i);
-      gcen = ld3(C.gc, n, i);
-    }
-    // ---- Particle::timestep (part.rs:42-84)
-    St<double> c = ld5(C.cons, n, i);
-    double dt_req = INFINITY;
-    if (do_ts) {
-    if (c.c[0] == 0.) {
-      st3(C.v, n, i, mk3(0., 0., 0.));
-      dt_req = INFINITY;
-    } else {
-      double m_inv = 1. / c.c[0];
-      V3<double> loc = ld3(C.loc, n, i);
-      V3<double> d_v = vel(gdot(gr, loc - gcen));
-      V3<double> fluid_v = (m_inv * vel(c)) + d_v;
-      double uint_ = (c.c[4] - 0.5 * dot(vel(c), vel(prim))) / c.c[0];
-      double V = C.vol[i];
-      double cs = sound_speed(cfg.gd, gas_p_from_u(cfg.gd, uint_, prim.c[0]), V * m_inv);
-      V3<double> v;
-      switch (cfg.motion) {
-        case MOT_FIXED:
-          v = mk3(0., 0., 0.);
-          break;
-        case MOT_FLUID:
-          v = fluid_v;
-          break;
-        case MOT_STEER: {
-          V3<double> d = cen - loc;
-          double abs_d = len(d);
-          double eta_r = 0.25 * cell_radius(V, cfg);
-          if (abs_d > 0.9 * eta_r) {
-            double fac = 0.99 * cs / abs_d;
-            if (abs_d < 1.1 * eta_r) fac *= 5. * (abs_d - 0.9 * eta_r) / eta_r;
-            v = fluid_v + (fac * d);
-          } else {
-            v = fluid_v;
-          }
-          break;
-        }
-        default: {  // STEER_PAKMOR
-          double alpha = 2. / (double)cfg.dim * C.maor[i];
-          double beta = 2.25;
-          double eta = hmax(0., hmin(0.5, 2. / beta * (alpha - 0.75 * beta)));
-          V3<double> dd = cen - loc;
-          double lr = 1. / len(dd);
-          V3<double> d = lr * dd;
-          V3<double> curl = mk3(gr.g[3].y - gr.g[2].z, gr.g[1].z - gr.g[3].x, gr.g[2].x - gr.g[1].y);
-          double v_reg = cell_radius(V, cfg) * len(curl);
-          v = fluid_v + ((eta * hmax(cs, v_reg)) * d);
-          break;
-        }
-      }
-      if (!finite3(v)) ev.fail(E_NONFINITE);
-      V3<double> v_rel = fluid_v - v;
-      st3(C.v, n, i, v);
-      st3(C.vrel, n, i, v_rel);
-      C.maor[i] = 0.;
-      double v_max = hmax(C.msv[i], len(v_rel) + cs);
-      C.msv[i] = 0.;
-      dt_req = v_max > 0. ? cfg.cfl * cell_radius(V, cfg) / v_max : INFINITY;
-    }
-    // ---- timesteps_apply
-    int8_t bin = C.bin[i];
-    uint64_t dti = make_integer_timestep(dt_req, bin, HFV_NUM_TIME_BINS, ti_current, time_base_inv);
-    C.dt[i] = (double)dti * time_base;
-    C.bin[i] = get_time_bin(dti);
-    my_dti = dti;
-    }
-    flush_events(ev, cnt);
-  }
-  // wave-level min, then one atomic per wave
-  for (int off = 32; off > 0; off >>= 1) {
-    unsigned long long o = __shfl_xor(my_dti, off);
-    my_dti = o < my_dti ? o : my_dti;
-  }
-  if (__lane_id() == 0 && my_dti != ~0ull) atomicMin(&cnt->ti_min, my_dti);
-}
-
-// timestep_limiter, first sweep (hydrodynamics.rs:165-179): wakeup from active neighbours' new bins.
-__global__ void __launch_bounds__(256) k_wakeup(Cells C, Faces F, DevCounters* cnt, int sync_all) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  int mybin = 127;
-  if (i < C.n) {
-    int8_t w = C.bin[i];
-    mybin = w;
-    if (!sync_all) {
-      int32_t k0 = F.off[i], k1 = F.off[i + 1];
-      for (int32_t k = k0; k < k1; ++k) {
-        int32_t e = F.conn[k];
-        int64_t f = e >> 1;
-        int32_t other = (e & 1) == 0 ? F.right[f] : F.left[f];
-        if (other < 0) continue;
-        if (!C.act[other]) continue;
-        int8_t cand = (int8_t)(C.bin[other] + 2);
-        if (cand < w) w = cand;
-      }
-      C.wake[i] = w;
-    }
-  }
-  if (sync_all) {  // timestep_sync (hydrodynamics.rs:186-191): min bin over all cells
-    for (int off = 32; off > 0; off >>= 1) {
-      int o = __shfl_xor(mybin, off);
-      mybin = o < mybin ? o : mybin;
-    }
-    if (__lane_id() == 0 && mybin != 127) atomicMin(&cnt->min_bin, mybin);
-  }
-}
-
-// Particle::timestep_limit (part.rs:263-297) + reset_extrapolations (hydrodynamics.rs:193-195) +
-// kick1 (hydro_kick1 part.rs:87-104, grav_kick with a_grav = 0).
-__global__ void __launch_bounds__(256) k_limit_kick(Cells C, Cfg cfg, DevCounters* cnt, int sync_all,
-                                                    uint64_t ti_current, double time_base) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.n) return;
-  const int64_t n = C.n;
-  int8_t wakeup = sync_all ? (int8_t)cnt->min_bin : C.wake[i];
-  int8_t bin = C.bin[i];
-  if (wakeup < bin) {
-    double dt = C.dt[i];
-    if (bin <= get_max_active_bin(ti_current)) {
-      bin = wakeup;
-      dt = (double)get_integer_timestep(bin) * time_base;
-    } else {
-      uint64_t ti_end_old = get_integer_time_end(ti_current, bin);
-      dt -= (double)(ti_end_old - ti_current) * time_base;
-      bin = wakeup;
-      uint64_t dti_new = get_integer_timestep(bin);
-      uint64_t ti_end = get_integer_time_end(ti_current, bin);
-      dt += (ti_end == ti_current) ? (double)dti_new * time_base : (double)(ti_end - ti_current) * time_base;
-    }
-    C.bin[i] = bin;
-    C.dt[i] = dt;
-  }
-  if (C.act[i]) {
-    st5(C.extrap, n, i, vacuum<double>());
-    if (cfg.motion != MOT_FIXED) {
-      double rho = C.prim[i];
-      if (rho > 0.) {
-        double kick_fac = 0.5 * C.dt[i] / rho;
-        for (int a = 0; a < cfg.dim; ++a) C.v[a * n + i] -= kick_fac * C.grad[(12 + a) * n + i];
-      }
-    }
-    C.cons[1 * n + i] += 0.0;
-    C.cons[2 * n + i] += 0.0;
-    C.cons[3 * n + i] += 0.0;
-  }
-}
-
-// Space::drift (space.rs:833-854) with Particle::drift / box_wrap / box_reflect (part.rs:107-119, 449-469).
-__global__ void k_drift(Cells C, Cfg cfg, double dt, DevCounters* cnt) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.n) return;
+// ------------------------------------------------------------------ drift (space.rs:833-854)
+// Particle::drift + box_wrap / box_reflect (part.rs:107-119, 449-469) for one sub-interval.
+__device__ void drift_cell(const Cells& C, const Cfg& cfg, int64_t i, double dt, Events& ev) {
   const int64_t n = C.n;
   const int dim = cfg.dim;
-  Events ev;
   double loc[3], cen[3], gc[3];
   for (int a = 0; a < 3; ++a) {
     loc[a] = C.loc[a * n + i];
@@ -790,9 +366,9 @@ __global__ void k_drift(Cells C, Cfg cfg, double dt, DevCounters* cnt) {
         while (loc[a] >= cfg.box[a] && ++guard < 128) loc[a] -= cfg.box[a];
       }
       for (int a = 0; a < 3; ++a) {
-        double s = loc[a] - old[a];
-        gc[a] += s;
-        cen[a] += s;
+        double sh = loc[a] - old[a];
+        gc[a] += sh;
+        cen[a] += sh;
       }
     } else if (cfg.boundary == BND_REFLECTIVE) {
       double old[3] = {loc[0], loc[1], loc[2]};
@@ -806,8 +382,7 @@ __global__ void k_drift(Cells C, Cfg cfg, double dt, DevCounters* cnt) {
       V3<double> sh = mk3(loc[0] - old[0], loc[1] - old[1], loc[2] - old[2]);
       if (len2(sh) > 0.) {
         V3<double> nn = (1. / len(sh)) * sh;
-        St<double> p = ld5(C.prim, n, i);
-        st5(C.prim, n, i, reflect(p, nn));
+        st5(C.prim, n, i, reflect(ld5(C.prim, n, i), nn));
         // Particle::reflect_gradients (part.rs:390-429), including its use of the already reflected
         // density gradient for the pressure gradient
         Gr<double> g = ldg(C.grad, n, i);
@@ -832,59 +407,765 @@ __global__ void k_drift(Cells C, Cfg cfg, double dt, DevCounters* cnt) {
     C.cen[a * n + i] = cen[a];
     C.gc[a * n + i] = gc[a];
   }
-  flush_events(ev, cnt);
 }
 
-// 1D Voronoi rebuild on the device for moving meshes (Space::volume_calculation, space.rs:236-263, 1D
-// semantics pinned by test_init_1d, space.rs:1453-1491): generators stay sorted, so the face topology is
-// fixed and only face positions, volumes and centroids change.  Interior face k sits between cells
-// k-1 and k; non-periodic meshes have walls at faces 0 and N; periodic meshes wrap at face 0.
-__global__ void k_voronoi1d_faces(Cells C, Faces F, Cfg cfg, int periodic, DevCounters* cnt) {
-  int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= F.nf) return;
-  const int64_t n = C.n;
+// ------------------------------------------------------------------ k_mask
+// mode 0: step (drift + mask + predict + lists); 1: predict only (dt from eng[0].dt_pred);
+// 2: mask + lists only.  Active mask: optimal_order.rs:17; predict: euler_eqns_fvs.rs:20-24,
+// part.rs:123-149.
+__global__ void __launch_bounds__(256) k_mask(Batch B, int mode) {
+  const Cells& C = B.C;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  bool valid = i < C.n;
+  int s = valid ? sim_of(C, i) : 0;
+  if (valid && s < 0) {  // empty slot capacity
+    if (mode != 1) C.act[i] = 0;
+    valid = false;
+    s = 0;
+  }
+  bool act = false, touch = false;
+  Events ev;
+  if (valid) {
+    const SimEngine& E = B.eng[s];
+    const Cfg& cfg = B.sims[s].cfg;
+    const int64_t n = C.n;
+    if (!E.stepping && mode != 1) {
+      C.act[i] = 0;
+      C.touched[i] = 0;
+    } else {
+      if (mode == 0 && E.n_sub > 0 && cfg.motion != MOT_FIXED)
+        for (int k = 0; k < E.n_sub; ++k) drift_cell(C, cfg, i, E.sub_dt[k], ev);
+      if (mode != 1) {
+        act = C.bin[i] <= E.mab;
+        C.act[i] = act;
+        touch = act || cfg.sync_all;
+        C.touched[i] = touch ? 1u : 0u;  // every flag rewritten each step
+      }
+      if (mode != 2) {
+        St<double> prim = ld5(C.prim, n, i);
+        Gr<double> gr = ldg(C.grad, n, i);
+        V3<double> vrel = ld3(C.vrel, n, i);
+        St<double> ex = ld5(C.extrap, n, i);
+        ex = ex + time_extrapolations(prim, gr, vrel, E.dt_pred, cfg.gd);
+        st5(C.extrap, n, i, ex);
+        check_physical(prim, ev);
+        if (ev.check_physical) st5(C.prim, n, i, prim);
+        // volume += dvdt * dt clamped to [V/2, 2V]: dvdt is never set on the path (part.rs:130-131)
+      }
+    }
+  }
+  long long pa = wave_append(act, &B.cnt->act_count);
+  if (pa >= 0) B.W.act_list[pa] = (int32_t)i;
+  long long pt = wave_append(touch, &B.cnt->touch_count);
+  if (pt >= 0) B.W.touch_list[pt] = (int32_t)i;
+  sim_add(valid && act, s, B.eng, 0);
+  if (valid) flush_events(ev, B.cnt);
+}
+
+// ------------------------------------------------------------------ 1D moving mesh on the device
+// Space::volume_calculation for 1D (space.rs:236-263; semantics pinned by test_init_1d,
+// space.rs:1453-1491): generators stay sorted, so the face topology is fixed; face k of a simulation
+// sits between its cells k-1 and k; walls are faces 0 and n (non periodic); periodic meshes wrap at 0.
+__global__ void k_mesh1d_faces(Batch B) {
+  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= B.F.nf) return;
+  int32_t lf = B.F.left[f];
+  if (lf < 0) return;  // spare face capacity
+  int s = sim_of(B.C, lf);
+  if (s < 0) return;
+  const SimDesc& D = B.sims[s];
+  if (!D.moving1d || !B.eng[s].stepping) return;
+  int64_t k = f - D.face_base, n = D.n, cb = D.cell_base;
   double x;
   if (k >= 1 && k < n) {
-    double a = C.loc[k - 1], b = C.loc[k];
-    if (!(a < b)) atomicCAS(&cnt->err, 0, E_UNSUPPORTED);
+    double a = B.C.loc[cb + k - 1], b = B.C.loc[cb + k];
+    if (!(a < b)) atomicCAS(&B.cnt->err, 0, E_UNSUPPORTED);
     x = 0.5 * (a + b);
-  } else if (!periodic) {
-    x = (k == 0) ? 0. : cfg.box[0];
+  } else if (!D.periodic) {
+    x = (k == 0) ? 0. : D.cfg.box[0];
   } else {
-    x = 0.5 * (C.loc[n - 1] + (C.loc[0] + cfg.box[0]));
+    x = 0.5 * (B.C.loc[cb + n - 1] + (B.C.loc[cb] + D.cfg.box[0]));
   }
-  F.fcen[k] = x;
+  B.F.fcen[f] = x;
 }
-__global__ void k_voronoi1d_cells(Cells C, Faces F, Cfg cfg, int periodic) {
+__global__ void k_mesh1d_cells(Batch B) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.n || !C.act[i]) return;
-  const int64_t n = C.n;
+  if (i >= B.C.n || !B.C.act[i]) return;
+  int s = sim_of(B.C, i);
+  const SimDesc& D = B.sims[s];
+  if (!D.moving1d) return;
+  const int64_t N = B.C.n;
+  int64_t k = i - D.cell_base, n = D.n, fb = D.face_base;
   double xl, xr;
-  if (!periodic) {
-    xl = F.fcen[i];
-    xr = F.fcen[i + 1];
+  if (!D.periodic) {
+    xl = B.F.fcen[fb + k];
+    xr = B.F.fcen[fb + k + 1];
   } else {
-    xl = (i == 0) ? F.fcen[0] - cfg.box[0] : F.fcen[i];
-    xr = (i == n - 1) ? F.fcen[0] : F.fcen[i + 1];
+    xl = (k == 0) ? B.F.fcen[fb] - D.cfg.box[0] : B.F.fcen[fb + k];
+    xr = (k == n - 1) ? B.F.fcen[fb] : B.F.fcen[fb + k + 1];
   }
   double V = xr - xl;
-  C.vol[i] = V;
-  C.cen[i] = 0.5 * (xl + xr);
-  C.cen[n + i] = 0.;
-  C.cen[2 * n + i] = 0.;
-  C.srad[i] = 1.5 * V;
+  B.C.vol[i] = V;
+  B.C.cen[i] = 0.5 * (xl + xr);
+  B.C.cen[N + i] = 0.;
+  B.C.cen[2 * N + i] = 0.;
+  B.C.srad[i] = 1.5 * V;
 }
 
-// Particle::from_ic + first_init (part.rs:200-217)
-__global__ void k_init_ic(Cells C, Cfg cfg, const double* mass, const double* vel3, const double* u_int,
-                          DevCounters* cnt) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= C.n) return;
+__device__ __forceinline__ bool is_dense(const Batch& B) { return B.cnt->act_count == (unsigned long long)B.C.n; }
+
+// ------------------------------------------------------------------ k_face_list
+// Every face adjacent to an active cell, once: owned by its lower-index active cell.  Inactive
+// neighbours of active cells join the touched list (they accumulate fluxes, hydrodynamics.rs:13-39).
+__global__ void __launch_bounds__(256) k_face_list(Batch B) {
+  if (is_dense(B)) return;
+  const int64_t count = (int64_t)B.cnt->act_count;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = xcd_block() * blockDim.x; base < count; base += stride) {
+    int64_t k = base + threadIdx.x;
+    bool valid = k < count;
+    int32_t i = valid ? B.W.act_list[k] : 0;
+    int32_t k0 = valid ? B.F.off[i] : 0, k1 = valid ? B.F.off[i + 1] : 0;
+    for (int32_t e = k0;; ++e) {
+      bool more = e < k1;
+      if (__ballot(more) == 0) break;
+      bool own = false, newt = false;
+      int32_t f = 0, other = -1;
+      if (more) {
+        int32_t c = B.F.conn[e];
+        f = c >> 1;
+        other = (c & 1) == 0 ? B.F.right[f] : B.F.left[f];
+        bool oa = other >= 0 && B.C.act[other];
+        own = other < 0 || !oa || i < other;
+        if (other >= 0 && !oa) newt = atomicExch(&B.C.touched[other], 1u) == 0u;
+      }
+      long long pf = wave_append(own, &B.cnt->face_count);
+      if (pf >= 0) B.W.face_list[pf] = f;
+      long long pt = wave_append(newt, &B.cnt->touch_count);
+      if (pt >= 0) B.W.touch_list[pt] = other;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ k_flux
+// FiniteVolumeSolver::compute_fluxes: gate (euler_eqns_fvs.rs:26-48) + flux_exchange (flux.rs:23-100)
+// + flux_exchange_boundary (flux.rs:102-195).  T: arithmetic of reconstruction and solve; SOLVER >= 0
+// specialises the kernel for one Riemann solver (-1: runtime switch).
+template <class T>
+__device__ __forceinline__ const Gas<T>& gas_for(const Cfg& c);
+template <>
+__device__ __forceinline__ const Gas<double>& gas_for<double>(const Cfg& c) {
+  return c.gd;
+}
+template <>
+__device__ __forceinline__ const Gas<float>& gas_for<float>(const Cfg& c) {
+  return c.gf;
+}
+
+template <class T, int SOLVER>
+__device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_t f, double* outv, Events& ev) {
+  const Cells& C = B.C;
+  const Faces& F = B.F;
   const int64_t n = C.n;
+  const int64_t nf = F.nf;
+  int32_t l = F.left[f];
+  int32_t r = F.right[f];
+  bool la = C.act[l];
+  const Gas<T>& g = gas_for<T>(cfg);
+  SolverCfg sc = cfg.solver;
+  if (SOLVER >= 0) sc.kind = SOLVER;
+#pragma unroll
+  for (int k = 0; k < 7; ++k) outv[k] = 0.;
+  if (r >= 0) {
+    bool ra = C.act[r];
+    double dtl = C.dt[l], dtr = C.dt[r];
+    if ((!la && !ra) || (dtr < dtl && !ra) || (dtl < dtr && !la)) return false;
+    double dt = hmin(dtl, dtr);
+    V3<double> fc = ld3(F.fcen, nf, f);
+    V3<double> nrm = ld3(F.nrm, nf, f);
+    double area = F.area[f];
+    V3<double> sh = decode_shift(F.shc[f], cfg.box);
+    V3<double> gcl = ld3(C.gc, n, l), gcr = ld3(C.gc, n, r);
+    V3<double> dx_left = fc - gcl;
+    V3<double> dx_right = (fc - gcr) - sh;
+    double dx_gradients = len(dx_left - dx_right);
+    V3<double> cl = ld3(C.cen, n, l), cr = ld3(C.cen, n, r);
+    V3<double> dx_centroid = (cr + sh) - cl;
+    V3<double> locl = ld3(C.loc, n, l), locr = ld3(C.loc, n, r);
+    V3<double> dx = (locr + sh) - locl;
+    St<double> pl = ld5(C.prim, n, l), pr = ld5(C.prim, n, r);
+    double v_max = 0.;
+    if (pl.c[0] > 0.) v_max += sound_speed(cfg.gd, pl.c[4], 1. / pl.c[0]);
+    if (pr.c[0] > 0.) v_max += sound_speed(cfg.gd, pr.c[4], 1. / pr.c[0]);
+    v_max -= hmin(dot(vel(pr) - vel(pl), dx_centroid), 0.);
+    // reconstruction (flux.rs:52-73) in T
+    T dt_extrapolate = (T)(-0.5 * dt);
+    Gr<double> gl = ldg(C.grad, n, l), grr = ldg(C.grad, n, r);
+    St<double> exl = ld5(C.extrap, n, l), exr = ld5(C.extrap, n, r);
+    V3<double> vrl = ld3(C.vrel, n, l), vrr = ld3(C.vrel, n, r);
+    St<T> plT = cvtst<T>(pl), prT = cvtst<T>(pr);
+    Gr<T> glT, grT;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      glT.g[q] = cvt3<T>(gl.g[q]);
+      grT.g[q] = cvt3<T>(grr.g[q]);
+    }
+    V3<T> dxlT = cvt3<T>(dx_left), dxrT = cvt3<T>(dx_right);
+    St<T> left_dash = ((plT + gdot(glT, dxlT)) + cvtst<T>(exl)) + time_extrapolations(plT, glT, cvt3<T>(vrl), dt_extrapolate, g);
+    St<T> right_dash = ((prT + gdot(grT, dxrT)) + cvtst<T>(exr)) + time_extrapolations(prT, grT, cvt3<T>(vrr), dt_extrapolate, g);
+    T dxg = (T)dx_gradients;
+    St<T> prim_left = pairwise_limiter(plT, prT, left_dash, len(dxlT) / dxg, ev);
+    St<T> prim_right = pairwise_limiter(prT, plT, right_dash, len(dxrT) / dxg, ev);
+    // face velocity (flux.rs:75-78), FP64
+    V3<double> vl = ld3(C.v, n, l), vr = ld3(C.v, n, r);
+    V3<double> midpoint = 0.5 * ((locl + locr) + sh);
+    double fac = dot(vr - vl, fc - midpoint) / len2(dx);
+    V3<double> v_face = (0.5 * (vl + vr)) - (fac * dx);
+    V3<T> vfT = cvt3<T>(v_face);
+    St<T> Fl = riemann_flux(sc, boost(prim_left, -vfT), boost(prim_right, -vfT), vfT, cvt3<T>(nrm), g, cfg.gd, ev);
+    St<double> fluxes = area * cvtst<double>(Fl);
+    St<double> rec = dt * fluxes;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) outv[k] = rec.c[k];
+    outv[5] = v_max;
+    outv[6] = area / len(dx);
+    return true;
+  }
+  if (!la) return false;
+  V3<double> fc = ld3(F.fcen, nf, f);
+  V3<double> nrm = ld3(F.nrm, nf, f);
+  double area = F.area[f];
+  V3<double> loc = ld3(C.loc, n, l), cen = ld3(C.cen, n, l), gc = ld3(C.gc, n, l);
+  // Particle::reflect (part.rs:374-383)
+  V3<double> loc_r = loc + ((2. * dot(fc - loc, nrm)) * nrm);
+  V3<double> cen_r = cen + ((2. * dot(fc - cen, nrm)) * nrm);
+  V3<double> gc_r = gc + ((2. * dot(fc - gc, nrm)) * nrm);
+  V3<double> dx_face = fc - gc;
+  double dx_gradients = len(gc_r - gc);
+  V3<double> dx_centroid = cen_r - cen;
+  V3<double> dx = loc_r - loc;
+  St<double> p = ld5(C.prim, n, l);
+  double dtp = C.dt[l];
+  double v_max = 0.;
+  if (p.c[0] > 0.) v_max += sound_speed(cfg.gd, p.c[4], 1. / p.c[0]);
+  Gr<double> gp = ldg(C.grad, n, l);
+  St<double> ex = ld5(C.extrap, n, l);
+  V3<double> vrel = ld3(C.vrel, n, l);
+  St<T> pT = cvtst<T>(p);
+  Gr<T> gT;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) gT.g[q] = cvt3<T>(gp.g[q]);
+  V3<T> dxfT = cvt3<T>(dx_face);
+  V3<T> nT = cvt3<T>(nrm);
+  St<T> dash = ((pT + gdot(gT, dxfT)) + cvtst<T>(ex)) + time_extrapolations(pT, gT, cvt3<T>(vrel), (T)(-0.5 * dtp), g);
+  T dx_fac = len(dxfT) / (T)dx_gradients;
+  St<T> bstate;
+  if (cfg.boundary == BND_REFLECTIVE) {
+    St<T> rp = reflect(pT, nT);
+    dash = pairwise_limiter(pT, rp, dash, dx_fac, ev);
+    St<double> rpd = reflect(p, nrm);
+    v_max -= hmin(dot(vel(rpd) - vel(p), dx_centroid), 0.);
+    bstate = reflect(dash, nT);
+  } else if (cfg.boundary == BND_OPEN) {
+    dash = pairwise_limiter(pT, pT, dash, dx_fac, ev);
+    bstate = dash;
+  } else if (cfg.boundary == BND_VACUUM) {
+    St<T> vac = vacuum<T>();
+    dash = pairwise_limiter(pT, vac, dash, dx_fac, ev);
+    v_max += hmin(dot(vel(p), dx_centroid), 0.);
+    bstate = vac;
+  } else {
+    ev.fail(E_UNSUPPORTED);
+    bstate = dash;
+  }
+  St<T> Fl = riemann_flux(sc, dash, bstate, mk3(T(0), T(0), T(0)), nT, g, cfg.gd, ev);
+  St<double> fluxes = area * cvtst<double>(Fl);
+  St<double> rec = dtp * fluxes;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) outv[k] = rec.c[k];
+  outv[5] = v_max;
+  outv[6] = area / len(dx);
+  return true;
+}
+
+template <class T, int SOLVER>
+__global__ void __launch_bounds__(256) k_flux(Batch B) {
+  const bool dense = is_dense(B);
+  const int64_t count = dense ? B.F.nf : (int64_t)B.cnt->face_count;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t nf = B.F.nf;
+  for (int64_t base = xcd_block() * blockDim.x; base < count; base += stride) {
+    int64_t k = base + threadIdx.x;
+    bool valid = k < count;
+    int64_t f = valid ? (dense ? k : (int64_t)B.W.face_list[k]) : 0;
+    int32_t lf = valid ? B.F.left[f] : 0;
+    if (lf < 0) valid = false;  // spare face capacity (dense mode walks every face slot)
+    int s = (valid && B.C.sim) ? B.C.sim[lf] : 0;
+    Events ev;
+    bool computed = false;
+    bool todo = valid;
+    while (true) {  // waterfall over the simulations present in the wave: uniform config loads
+      unsigned long long m = __ballot(todo);
+      if (m == 0) break;
+      int su = __builtin_amdgcn_readfirstlane(__shfl(s, __ffsll((long long)m) - 1));
+      if (todo && s == su) {
+        double outv[7];
+        computed = face_flux<T, SOLVER>(B, B.sims[su].cfg, f, outv, ev);
+#pragma unroll
+        for (int q = 0; q < 7; ++q) B.F.rec[q * nf + f] = outv[q];
+        todo = false;
+      }
+    }
+    sim_add(valid && computed, s, B.eng, 1);
+    if (valid) flush_events(ev, B.cnt);
+  }
+}
+
+// ------------------------------------------------------------------ k_apply
+// runner apply_fluxes (hydrodynamics.rs:13-39) + kick2 (:51-57) + convert_conserved_to_primitive
+// (euler_eqns_fvs.rs:63-69, part.rs:219-261).  One thread per touched cell walks its faces in CSR order,
+// so the FP64 accumulation order is the reference's and needs no atomics.  A face record is current
+// when one of its cells is active (inactive pairs are gated off and contribute zero).
+// mode bit 1: accumulate + apply + kick2; mode bit 2: conserved -> primitive.
+__device__ __forceinline__ St<double> c2p(const St<double>& c, double vol, const Gas<double>& g, Events& ev) {
+  St<double> prim;
+  if (!(c.c[0] >= 0.)) ev.fail(E_NEG_MASS);
+  if (!(c.c[4] >= 0.)) ev.fail(E_NEG_ENERGY);
+  if (c.c[0] == 0.) {
+    if (c.c[4] != 0.) ev.fail(E_NEG_ENERGY);
+    prim = vacuum<double>();
+  } else if (c.c[0] > 0.) {
+    double m_inv = 1. / c.c[0];
+    double density = c.c[0] / vol;
+    V3<double> velocity = m_inv * vel(c);
+    double thermal = c.c[4] - 0.5 * len2(vel(c)) * m_inv;
+    double u = thermal * m_inv;
+    double pressure = gas_p_from_u(g, u, density);
+    if (!(density >= 0.) || !(pressure >= 0.)) ev.fail(E_NEG_PRESSURE);
+    prim = mkst(density, velocity, pressure);
+  } else {
+    prim = vacuum<double>();
+  }
+#pragma unroll
+  for (int q = 0; q < 5; ++q)
+    if (!hfinite(prim.c[q])) ev.fail(E_NONFINITE);
+  return prim;
+}
+
+__global__ void __launch_bounds__(256) k_apply(Batch B, int mode) {
+  const Cells& C = B.C;
+  const Faces& F = B.F;
+  const bool dense = is_dense(B);
+  const int64_t count = dense ? C.n : (int64_t)B.cnt->touch_count;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t n = C.n, nf = F.nf;
+  for (int64_t k = xcd_block() * blockDim.x + threadIdx.x; k < count; k += stride) {
+    int64_t i = dense ? k : (int64_t)B.W.touch_list[k];
+    bool a = C.act[i];
+    Events ev;
+    const Cfg& cfg = B.sims[sim_of(C, i)].cfg;
+    if (!(mode & 1)) {
+      if (a) st5(C.prim, n, i, c2p(ld5(C.cons, n, i), C.vol[i], cfg.gd, ev));
+      flush_events(ev, B.cnt);
+      continue;
+    }
+    St<double> acc = ld5(C.facc, n, i);
+    double msv = 0., maor = 0.;
+    if (a) {
+      msv = C.msv[i];
+      maor = C.maor[i];
+    }
+    int32_t k0 = F.off[i], k1 = F.off[i + 1];
+    for (int32_t e = k0; e < k1; ++e) {
+      int32_t c = F.conn[e];
+      int64_t f = c >> 1;
+      int side = c & 1;
+      if (!a) {
+        int32_t other = side == 0 ? F.right[f] : F.left[f];
+        if (other < 0 || !C.act[other]) continue;
+      }
+      St<double> rec;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) rec.c[q] = F.rec[q * nf + f];
+      if (side == 0)
+        acc = acc - rec;
+      else
+        acc = acc + rec;
+      if (a) {
+        msv = hmax(F.rec[5 * nf + f], msv);
+        maor = hmax(maor, F.rec[6 * nf + f]);
+      }
+    }
+    if (a) {
+      St<double> c = ld5(C.cons, n, i) + acc;  // apply_flux (part.rs:178-193)
+#pragma unroll
+      for (int q = 0; q < 5; ++q)
+        if (!hfinite(c.c[q])) ev.fail(E_NONFINITE);
+      if (c.c[0] < 0.) {
+        ev.flux_reset++;
+        c = vacuum<double>();
+      }
+      if (c.c[4] < 0.) {
+        ev.flux_reset++;
+        c = vacuum<double>();
+      }
+      acc = vacuum<double>();
+      // kick2 -> grav_kick with a_grav = 0 (part.rs:303-312): momentum += (0 * mass)
+      c.c[1] += 0.0;
+      c.c[2] += 0.0;
+      c.c[3] += 0.0;
+      st5(C.cons, n, i, c);
+      C.msv[i] = msv;
+      C.maor[i] = maor;
+      if (mode & 2) st5(C.prim, n, i, c2p(c, C.vol[i], cfg.gd, ev));
+    }
+    st5(C.facc, n, i, acc);
+    flush_events(ev, B.cnt);
+  }
+}
+
+// glam::DMat3 as three columns
+struct M3 {
+  V3<double> x, y, z;
+};
+
+// Particle::radius (part.rs:439-447)
+__device__ __forceinline__ double cell_radius(double V, const Cfg& cfg) {
+  if (cfg.dim == 1) return 0.5 * V;
+  if (cfg.dim == 2) return sqrt(0.318309886183790671537767526745028724 * V);
+  return pow(cfg.radius3_fac * V, 1. / 3.);
+}
+
+// ------------------------------------------------------------------ k_grad
+// gradient_estimate / gradient_limit / gradient_apply (hydrodynamics.rs:67-143, gradients.rs:8-103)
+// fused with compute_timesteps (euler_eqns_fvs.rs:50-61, part.rs:42-84, 328-372) and timesteps_apply
+// (hydrodynamics.rs:145-158, timeline.rs:103-139) for the active cells.
+__device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const SimEngine& E, int64_t i, int do_grad,
+                                          int do_ts, unsigned long long* dti_out, Events& ev) {
+  const Cells& C = B.C;
+  const Faces& F = B.F;
+  const int64_t n = C.n, nf = F.nf;
+  St<double> prim = ld5(C.prim, n, i);
+  V3<double> cen = ld3(C.cen, n, i);
+  Gr<double> gr;
+  V3<double> gcen;
+  int32_t k0 = F.off[i], k1 = F.off[i + 1];
+  if (do_grad && cfg.runner == 0) {
+    // ---- estimate
+    Gr<double> g;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) g.g[q] = mk3(0., 0., 0.);
+    M3 m;
+    m.x = mk3(0., 0., 0.);
+    m.y = mk3(0., 0., 0.);
+    m.z = mk3(0., 0., 0.);
+    if (cfg.dim == 1) {
+      m.y.y = 1.;
+      m.z.z = 1.;
+    } else if (cfg.dim == 2) {
+      m.z.z = 1.;
+    }
+    for (int32_t k = k0; k < k1; ++k) {
+      int32_t e = F.conn[k];
+      int64_t f = e >> 1;
+      int side = e & 1;
+      double area = F.area[f];
+      V3<double> sh = decode_shift(F.shc[f], cfg.box);
+      V3<double> shift = side == 0 ? sh : -sh;
+      int32_t other = side == 0 ? F.right[f] : F.left[f];
+      V3<double> oc;
+      St<double> op;
+      if (other >= 0) {
+        oc = ld3(C.cen, n, other);
+        op = ld5(C.prim, n, other);
+      } else {  // Space::get_boundary_part (space.rs:213-229)
+        V3<double> fc = ld3(F.fcen, nf, f), nrm = ld3(F.nrm, nf, f);
+        oc = cen + ((2. * dot(fc - cen, nrm)) * nrm);
+        op = prim;
+        if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, nrm);
+        else if (cfg.boundary == BND_VACUUM) op = vacuum<double>();
+      }
+      V3<double> ds = (oc + shift) - cen;
+      double w = area / len2(ds);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) g.g[q] = g.g[q] + ((w * (op.c[q] - prim.c[q])) * ds);
+      m.x = m.x + (w * (ds.x * ds));
+      m.y = m.y + (w * (ds.y * ds));
+      m.z = m.z + (w * (ds.z * ds));
+    }
+    // DMat3::inverse (glam 0.27): cofactors by cross products, det = z . (x cross y), transpose
+    V3<double> t0 = cross(m.y, m.z), t1 = cross(m.z, m.x), t2 = cross(m.x, m.y);
+    double det = dot(m.z, t2);
+    double inv_det = 1. / det;
+    V3<double> c0 = mk3(t0.x * inv_det, t0.y * inv_det, t0.z * inv_det);
+    V3<double> c1 = mk3(t1.x * inv_det, t1.y * inv_det, t1.z * inv_det);
+    V3<double> c2 = mk3(t2.x * inv_det, t2.y * inv_det, t2.z * inv_det);
+    M3 inv;
+    inv.x = mk3(c0.x, c1.x, c2.x);
+    inv.y = mk3(c0.y, c1.y, c2.y);
+    inv.z = mk3(c0.z, c1.z, c2.z);
+    bool fin = true;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      V3<double> v = g.g[q];
+      V3<double> r = v.x * inv.x;
+      r = r + (v.y * inv.y);
+      r = r + (v.z * inv.z);
+      g.g[q] = r;
+      fin = fin && finite3(r);
+    }
+    if (!fin) {
+      ev.grad_fallback++;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) g.g[q] = mk3(0., 0., 0.);
+    }
+    // ---- limit (LimiterData)
+    St<double> lmin = prim, lmax = prim, emin, emax;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      emin.c[q] = INFINITY;
+      emax.c[q] = -INFINITY;
+    }
+    for (int32_t k = k0; k < k1; ++k) {
+      int32_t e = F.conn[k];
+      int64_t f = e >> 1;
+      int side = e & 1;
+      V3<double> sh = decode_shift(F.shc[f], cfg.box);
+      V3<double> shift = side == 0 ? sh : -sh;
+      V3<double> fc = ld3(F.fcen, nf, f);
+      St<double> ext = gdot(g, (fc - cen) - shift);
+      int32_t other = side == 0 ? F.right[f] : F.left[f];
+      St<double> op;
+      if (other >= 0) {
+        op = ld5(C.prim, n, other);
+      } else {
+        op = prim;
+        if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, ld3(F.nrm, nf, f));
+        else if (cfg.boundary == BND_VACUUM) op = vacuum<double>();
+      }
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        lmin.c[q] = hmin(lmin.c[q], op.c[q]);
+        lmax.c[q] = hmax(lmax.c[q], op.c[q]);
+        emin.c[q] = hmin(emin.c[q], ext.c[q]);
+        emax.c[q] = hmax(emax.c[q], ext.c[q]);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      double d_max = lmax.c[q] - prim.c[q], d_min = lmin.c[q] - prim.c[q];
+      double alpha = (emax.c[q] == 0. || emin.c[q] == 0.) ? 1. : hmin(1.0, hmin(d_max / emax.c[q], d_min / emin.c[q]));
+      g.g[q] = alpha * g.g[q];
+    }
+    gr = g;
+    gcen = cen;
+    stg(C.grad, n, i, gr);
+    st3(C.gc, n, i, gcen);
+  } else {
+    gr = ldg(C.grad, n, i);
+    gcen = ld3(C.gc, n, i);
+  }
+  if (!do_ts) return;
+  // ---- Particle::timestep (part.rs:42-84)
+  St<double> c = ld5(C.cons, n, i);
+  double dt_req;
+  if (c.c[0] == 0.) {
+    st3(C.v, n, i, mk3(0., 0., 0.));
+    dt_req = INFINITY;
+  } else {
+    double m_inv = 1. / c.c[0];
+    V3<double> loc = ld3(C.loc, n, i);
+    V3<double> d_v = vel(gdot(gr, loc - gcen));
+    V3<double> fluid_v = (m_inv * vel(c)) + d_v;
+    double uint_ = (c.c[4] - 0.5 * dot(vel(c), vel(prim))) / c.c[0];
+    double V = C.vol[i];
+    double cs = sound_speed(cfg.gd, gas_p_from_u(cfg.gd, uint_, prim.c[0]), V * m_inv);
+    V3<double> v;
+    switch (cfg.motion) {
+      case MOT_FIXED:
+        v = mk3(0., 0., 0.);
+        break;
+      case MOT_FLUID:
+        v = fluid_v;
+        break;
+      case MOT_STEER: {
+        V3<double> d = cen - loc;
+        double abs_d = len(d);
+        double eta_r = 0.25 * cell_radius(V, cfg);
+        if (abs_d > 0.9 * eta_r) {
+          double fac = 0.99 * cs / abs_d;
+          if (abs_d < 1.1 * eta_r) fac *= 5. * (abs_d - 0.9 * eta_r) / eta_r;
+          v = fluid_v + (fac * d);
+        } else {
+          v = fluid_v;
+        }
+        break;
+      }
+      default: {  // STEER_PAKMOR
+        double alpha = 2. / (double)cfg.dim * C.maor[i];
+        double beta = 2.25;
+        double eta = hmax(0., hmin(0.5, 2. / beta * (alpha - 0.75 * beta)));
+        V3<double> dd = cen - loc;
+        double lr = 1. / len(dd);
+        V3<double> d = lr * dd;
+        V3<double> curl = mk3(gr.g[3].y - gr.g[2].z, gr.g[1].z - gr.g[3].x, gr.g[2].x - gr.g[1].y);
+        double v_reg = cell_radius(V, cfg) * len(curl);
+        v = fluid_v + ((eta * hmax(cs, v_reg)) * d);
+        break;
+      }
+    }
+    if (!finite3(v)) ev.fail(E_NONFINITE);
+    V3<double> v_rel = fluid_v - v;
+    st3(C.v, n, i, v);
+    st3(C.vrel, n, i, v_rel);
+    C.maor[i] = 0.;
+    double v_max = hmax(C.msv[i], len(v_rel) + cs);
+    C.msv[i] = 0.;
+    dt_req = v_max > 0. ? cfg.cfl * cell_radius(V, cfg) / v_max : INFINITY;
+  }
+  // ---- timesteps_apply
+  int8_t bin = C.bin[i];
+  uint64_t dti = make_integer_timestep(dt_req, bin, HFV_NUM_TIME_BINS, E.ti_current, E.time_base_inv);
+  C.dt[i] = (double)dti * E.time_base;
+  C.bin[i] = get_time_bin(dti);
+  *dti_out = dti;
+}
+
+__global__ void __launch_bounds__(256) k_grad(Batch B, int do_grad, int do_ts) {
+  const bool dense = is_dense(B);
+  const int64_t count = dense ? B.C.n : (int64_t)B.cnt->act_count;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t base = xcd_block() * blockDim.x; base < count; base += stride) {
+    int64_t k = base + threadIdx.x;
+    bool valid = k < count;
+    int64_t i = valid ? (dense ? k : (int64_t)B.W.act_list[k]) : 0;
+    int s = valid ? sim_of(B.C, i) : 0;
+    unsigned long long my = ~0ull;
+    Events ev;
+    bool todo = valid;
+    while (true) {
+      unsigned long long m = __ballot(todo);
+      if (m == 0) break;
+      int su = __builtin_amdgcn_readfirstlane(__shfl(s, __ffsll((long long)m) - 1));
+      if (todo && s == su) {
+        grad_cell(B, B.sims[su].cfg, B.eng[su], i, do_grad, do_ts, &my, ev);
+        todo = false;
+      }
+    }
+    // per-simulation minimum of dti (waterfall + wave reduction, one atomic per simulation per wave)
+    todo = valid && my != ~0ull;
+    while (true) {
+      unsigned long long m = __ballot(todo);
+      if (m == 0) break;
+      int leader = __ffsll((long long)m) - 1;
+      int su = __shfl(s, leader);
+      bool mine = todo && s == su;
+      unsigned long long v = mine ? my : ~0ull;
+      for (int off = 32; off > 0; off >>= 1) {
+        unsigned long long o = __shfl_xor(v, off);
+        v = o < v ? o : v;
+      }
+      if (__lane_id() == leader) atomicMin(&B.eng[su].ti_min, v);
+      if (mine) todo = false;
+    }
+    if (valid) flush_events(ev, B.cnt);
+  }
+}
+
+// ------------------------------------------------------------------ limiter
+// timestep_limiter first sweep (hydrodynamics.rs:165-179): wakeup from active neighbours' new bins;
+// timestep_sync (hydrodynamics.rs:186-191) reduces the minimum bin per simulation instead.
+__global__ void __launch_bounds__(256) k_wakeup(Batch B) {
+  const Cells& C = B.C;
+  const Faces& F = B.F;
+  const bool dense = is_dense(B);
+  const int64_t count = dense ? C.n : (int64_t)B.cnt->touch_count;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t k = xcd_block() * blockDim.x + threadIdx.x; k < count; k += stride) {
+    int64_t i = dense ? k : (int64_t)B.W.touch_list[k];
+    int s = sim_of(C, i);
+    int8_t w = C.bin[i];
+    if (B.sims[s].cfg.sync_all) {
+      atomicMin(&B.eng[s].min_bin, (int)w);
+      continue;
+    }
+    int32_t k0 = F.off[i], k1 = F.off[i + 1];
+    for (int32_t e = k0; e < k1; ++e) {
+      int32_t c = F.conn[e];
+      int64_t f = c >> 1;
+      int32_t other = (c & 1) == 0 ? F.right[f] : F.left[f];
+      if (other < 0) continue;
+      if (!C.act[other]) continue;
+      int8_t cand = (int8_t)(C.bin[other] + 2);
+      if (cand < w) w = cand;
+    }
+    C.wake[i] = w;
+  }
+}
+
+// Particle::timestep_limit (part.rs:263-297) + reset_extrapolations (hydrodynamics.rs:193-195) +
+// kick1 (hydro_kick1 part.rs:87-104, grav_kick with a_grav = 0).  Clears the touched flags.
+__global__ void __launch_bounds__(256) k_limit(Batch B) {
+  const Cells& C = B.C;
+  const bool dense = is_dense(B);
+  const int64_t count = dense ? C.n : (int64_t)B.cnt->touch_count;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t n = C.n;
+  for (int64_t k = xcd_block() * blockDim.x + threadIdx.x; k < count; k += stride) {
+    int64_t i = dense ? k : (int64_t)B.W.touch_list[k];
+    int s = sim_of(C, i);
+    const Cfg& cfg = B.sims[s].cfg;
+    const SimEngine& E = B.eng[s];
+    C.touched[i] = 0;
+    int8_t wakeup = cfg.sync_all ? (int8_t)E.min_bin : C.wake[i];
+    int8_t bin = C.bin[i];
+    if (wakeup < bin) {
+      double dt = C.dt[i];
+      uint64_t ti_current = E.ti_current;
+      double time_base = E.time_base;
+      if (bin <= get_max_active_bin(ti_current)) {
+        bin = wakeup;
+        dt = (double)get_integer_timestep(bin) * time_base;
+      } else {
+        uint64_t ti_end_old = get_integer_time_end(ti_current, bin);
+        dt -= (double)(ti_end_old - ti_current) * time_base;
+        bin = wakeup;
+        uint64_t dti_new = get_integer_timestep(bin);
+        uint64_t ti_end = get_integer_time_end(ti_current, bin);
+        dt += (ti_end == ti_current) ? (double)dti_new * time_base : (double)(ti_end - ti_current) * time_base;
+      }
+      C.bin[i] = bin;
+      C.dt[i] = dt;
+    }
+    if (C.act[i]) {
+      st5(C.extrap, n, i, vacuum<double>());
+      if (cfg.motion != MOT_FIXED) {
+        double rho = C.prim[i];
+        if (rho > 0.) {
+          double kick_fac = 0.5 * C.dt[i] / rho;
+          for (int a = 0; a < cfg.dim; ++a) C.v[a * n + i] -= kick_fac * C.grad[(12 + a) * n + i];
+        }
+      }
+      C.cons[1 * n + i] += 0.0;
+      C.cons[2 * n + i] += 0.0;
+      C.cons[3 * n + i] += 0.0;
+    }
+  }
+}
+
+// Particle::from_ic + first_init (part.rs:200-217) for cells [lo, hi); vel3 is component-major [3m].
+__global__ void k_init_ic(Batch B, int64_t lo, int64_t hi, const double* mass, const double* vel3,
+                          const double* u_int) {
+  int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t i = lo + j;
+  if (i >= hi) return;
+  const Cells& C = B.C;
+  const int64_t n = C.n, m_ = hi - lo;
+  const Cfg& cfg = B.sims[sim_of(C, i)].cfg;
   Events ev;
-  double m = mass[i];
-  V3<double> v = ld3(vel3, n, i);
-  St<double> c = mkst(m, m * v, m * u_int[i] + 0.5 * m * len2(v));
+  double m = mass[j];
+  V3<double> v = mk3(vel3[j], vel3[m_ + j], vel3[2 * m_ + j]);
+  St<double> c = mkst(m, m * v, m * u_int[j] + 0.5 * m * len2(v));
   St<double> prim;
   if (c.c[0] > 0.) {
     double m_inv = 1. / c.c[0];
@@ -899,16 +1180,16 @@ __global__ void k_init_ic(Cells C, Cfg cfg, const double* mass, const double* ve
   }
   st5(C.cons, n, i, c);
   st5(C.prim, n, i, prim);
-  flush_events(ev, cnt);
+  flush_events(ev, B.cnt);
 }
 
-// Space::status sums (space.rs:1248-1280): one block, fixed-order tree reduction per component.
-__global__ void __launch_bounds__(1024) k_conservation(Cells C, double* out5) {
+// Space::status sums (space.rs:1248-1280) over cells [lo, hi): one block, fixed-order tree reduction.
+__global__ void __launch_bounds__(1024) k_conservation(Batch B, int64_t lo, int64_t hi, double* out5) {
   __shared__ double sh[1024];
-  const int64_t n = C.n;
+  const int64_t n = B.C.n;
   for (int q = 0; q < 5; ++q) {
     double s = 0.;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) s += C.cons[q * n + i] + C.facc[q * n + i];
+    for (int64_t i = lo + threadIdx.x; i < hi; i += blockDim.x) s += B.C.cons[q * n + i] + B.C.facc[q * n + i];
     sh[threadIdx.x] = s;
     __syncthreads();
     for (int w = blockDim.x / 2; w > 0; w >>= 1) {

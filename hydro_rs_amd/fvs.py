@@ -223,6 +223,106 @@ class EulerEqnsFvs:
         _lib.check(self.lib.hfv_synchronize(self.ptr))
 
 
+def _engine_dict(e):
+    d = {k: getattr(e, k) for k, _ in _lib.EngineInfo._fields_ if k != "ts"}
+    d.update({k: getattr(e.ts, k) for k, _ in _lib.TimestepInfo._fields_})
+    return d
+
+
+class Batch:
+    """Many independent simulations in one device context (one launch sequence steps them all).
+
+    Slots have fixed HBM capacities (cells, faces, CSR entries); load() puts a simulation into a slot
+    and can be called again on a finished slot (continuous batching)."""
+
+    def __init__(self, cell_caps, face_caps, conn_caps):
+        self.lib = _lib.hfv()
+        cc = np.ascontiguousarray(cell_caps, dtype=np.int64)
+        fc = np.ascontiguousarray(face_caps, dtype=np.int64)
+        kc = np.ascontiguousarray(conn_caps, dtype=np.int64)
+        self.nslots = cc.size
+        self.caps = (cc.copy(), fc.copy(), kc.copy())
+        self.ptr = C.c_void_p()
+        _lib.check(self.lib.hfv_batch_create(self.nslots, cc.ctypes.data_as(_lib.I64P), fc.ctypes.data_as(_lib.I64P),
+                                             kc.ctypes.data_as(_lib.I64P), C.byref(self.ptr)))
+        self.n_cells = [0] * self.nslots
+
+    @classmethod
+    def for_problems(cls, probs, headroom=1.0):
+        caps = [(p.n, p.mesh.n_faces, p.mesh.conn.size) for p in probs]
+        b = cls(*[np.ceil(np.array(c) * headroom).astype(np.int64) for c in zip(*caps)])
+        for s, p in enumerate(probs):
+            b.load(s, p)
+        return b
+
+    def close(self):
+        if getattr(self, "ptr", None) and self.ptr.value:
+            self.lib.hfv_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def load(self, slot, prob):
+        m = prob.mesh.arrays_struct()
+        p = prob.cfg.to_c()
+        loc = np.ascontiguousarray(prob.loc, dtype=np.float64).reshape(-1, 3)
+        vel = np.ascontiguousarray(prob.vel, dtype=np.float64).reshape(-1, 3)
+        mass = np.ascontiguousarray(prob.mass, dtype=np.float64)
+        u = np.ascontiguousarray(prob.u_int, dtype=np.float64)
+        _lib.check(self.lib.hfv_batch_load(self.ptr, slot, C.byref(p), C.byref(m), _dp(loc), _dp(mass), _dp(vel),
+                                           _dp(u)))
+        self.n_cells[slot] = prob.n
+
+    def unload(self, slot):
+        _lib.check(self.lib.hfv_batch_unload(self.ptr, slot))
+        self.n_cells[slot] = 0
+
+    def run(self, max_launches=1 << 62):
+        k = _lib.I64(0)
+        _lib.check(self.lib.hfv_batch_run(self.ptr, int(max_launches), C.byref(k)))
+        return k.value
+
+    def engine_info(self, slot):
+        e = _lib.EngineInfo()
+        _lib.check(self.lib.hfv_batch_get_engine_info(self.ptr, slot, C.byref(e)))
+        return _engine_dict(e)
+
+    def get_state(self, slot, fields=None):
+        n = self.n_cells[slot]
+        fields = fields or list(FIELD_WIDTH) + ["timebin"]
+        out = {}
+        st = _lib.State()
+        for k in fields:
+            if k == "timebin":
+                out[k] = np.zeros(n, np.int8)
+                st.timebin = out[k].ctypes.data_as(_lib.I8P)
+            else:
+                out[k] = np.zeros(FIELD_WIDTH[k] * n)
+                setattr(st, k, _dp(out[k]))
+        _lib.check(self.lib.hfv_batch_get_state(self.ptr, slot, C.byref(st)))
+        return out
+
+    def stats(self):
+        s = _lib.Stats()
+        _lib.check(self.lib.hfv_get_stats(self.ptr, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in _lib.Stats._fields_}
+
+    def profile(self, enable=True):
+        _lib.check(self.lib.hfv_profile_enable(self.ptr, int(bool(enable))))
+
+    def kernel_times(self):
+        t = _lib.KernelTimes()
+        _lib.check(self.lib.hfv_profile_read(self.ptr, C.byref(t)))
+        return {name: {"launches": t.launches[k], "ms": t.ms[k]} for k, name in enumerate(_lib.KERNEL_SLOTS)}
+
+    def synchronize(self):
+        _lib.check(self.lib.hfv_synchronize(self.ptr))
+
+
 def riemann_batch(cfg: Config, WL, WR, vface, nrm):
     """RiemannFluxSolver::solve_for_flux for n face problems; arrays are [n,5] / [n,3]."""
     WL = np.ascontiguousarray(np.asarray(WL, dtype=np.float64).T)

@@ -214,6 +214,31 @@ int hfv_get_engine_info(hfv_ctx* ctx, hfv_engine_info* out);
  * reduced on the device in a fixed order. */
 int hfv_conservation(hfv_ctx* ctx, double* out5);
 
+/* ---------------------------------------------------------------- batched ensembles and serving */
+/* Many independent simulations in one context.  No counterpart in the reference, which runs one
+ * simulation per process (src/bin/main.rs:615-736); every simulation keeps the reference's semantics
+ * (its own Engine, TimestepInfo and timebins), and one launch sequence, captured in a HIP graph,
+ * advances every unfinished simulation by one of its own engine steps.
+ *
+ * A batch is a set of slots; slot s owns cell_cap[s] cells, face_cap[s] faces and conn_cap[s] CSR
+ * entries of HBM.  A slot holds at most one simulation at a time; hfv_batch_load replaces it, so a
+ * serving loop refills slots as simulations finish. */
+int hfv_batch_create(int nslots, const int64_t* cell_cap, const int64_t* face_cap, const int64_t* conn_cap,
+                     hfv_ctx** out);
+/* Load a simulation into a slot: Space::initialize + first_init (src/space.rs:88-211) from the mesh and
+ * the IC arrays (loc, vel [3n] xyz interleaved; mass, u_int [n]); resets the slot's engine (Engine::new,
+ * src/engine.rs:104-149).  Fails with HFV_ERR_INVALID if the mesh exceeds the slot's capacity. */
+int hfv_batch_load(hfv_ctx* ctx, int slot, const hfv_params* params, const hfv_mesh_arrays* mesh,
+                   const double* loc, const double* mass, const double* vel, const double* u_int);
+/* Mark a slot empty (its cells stop participating). */
+int hfv_batch_unload(hfv_ctx* ctx, int slot);
+/* Enqueue up to max_launches batch steps; stops early (checked every few launches) when every loaded
+ * simulation has finished.  *launches receives the number of batch steps enqueued. */
+int hfv_batch_run(hfv_ctx* ctx, int64_t max_launches, int64_t* launches);
+int hfv_batch_get_engine_info(hfv_ctx* ctx, int slot, hfv_engine_info* out);
+int hfv_batch_get_state(hfv_ctx* ctx, int slot, hfv_state* st);
+int hfv_batch_num_slots(hfv_ctx* ctx, int* n);
+
 /* ---------------------------------------------------------------- measurement */
 /* Device time of the step's kernels, from HIP events recorded on the context's stream around each
  * launch (no counterpart in the reference, which has no device).  Slots: 0 mask+predict, 1 1D mesh
