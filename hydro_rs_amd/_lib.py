@@ -63,7 +63,7 @@ class TimestepInfo(C.Structure):
 class EngineInfo(C.Structure):
     _fields_ = [("ts", TimestepInfo), ("t_current", C.c_double), ("t_end", C.c_double), ("step_count", I64),
                 ("cell_updates", I64), ("face_fluxes", I64), ("last_n_active", I64),
-                ("last_n_face_active", I64), ("done", C.c_int)]
+                ("last_n_face_active", I64), ("done", C.c_int), ("error_code", C.c_int)]
 
 
 class Stats(C.Structure):

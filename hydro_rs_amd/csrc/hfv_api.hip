@@ -157,8 +157,12 @@ struct hfv_ctx {
   hfv_timestep_info last_info{};
   // kernel timing (hfv_profile_enable)
   bool prof = false;
-  hipEvent_t ev0[HFV_NUM_KERNEL_SLOTS]{}, ev1[HFV_NUM_KERNEL_SLOTS]{};
-  bool pending[HFV_NUM_KERNEL_SLOTS]{};
+  // a ring of event pairs per kernel slot: every timed step records into its own ring entry, and an
+  // entry is read back (waiting for it if needed) before it is recorded again
+  static constexpr int PROF_RING = 64;
+  hipEvent_t ev0[PROF_RING][HFV_NUM_KERNEL_SLOTS]{}, ev1[PROF_RING][HFV_NUM_KERNEL_SLOTS]{};
+  bool pending[PROF_RING][HFV_NUM_KERNEL_SLOTS]{};
+  int ring = 0;
   hfv_kernel_times times{};
 
   Batch batch_args() const {
@@ -314,7 +318,8 @@ int sync_counters(hfv_ctx* c) {
   HIP_OK(hipMemcpyAsync(c->hcnt, c->dcnt, sizeof(DevCounters), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipMemcpyAsync(c->heng, c->deng, sizeof(SimEngine) * c->nslots, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
-  if (c->hcnt->err) {
+  // A batch keeps going when one simulation fails: its error is in that slot's engine info.
+  if (c->hcnt->err && !c->batch) {
     char b[128];
     snprintf(b, sizeof(b), "device raised hard error %d", c->hcnt->err);
     return set_err(c->hcnt->err, b);
@@ -529,25 +534,37 @@ int init_ic(hfv_ctx* c, int s, int64_t n, const double* loc, const double* mass,
 }
 
 inline void prof_begin(hfv_ctx* c, int slot) {
-  if (c->prof) (void)hipEventRecord(c->ev0[slot], c->stream);
+  if (c->prof) (void)hipEventRecord(c->ev0[c->ring][slot], c->stream);
 }
 inline void prof_end(hfv_ctx* c, int slot) {
   if (c->prof) {
-    (void)hipEventRecord(c->ev1[slot], c->stream);
-    c->pending[slot] = true;
+    (void)hipEventRecord(c->ev1[c->ring][slot], c->stream);
+    c->pending[c->ring][slot] = true;
   }
 }
-void prof_collect(hfv_ctx* c) {
-  if (!c->prof) return;
+// Adds the elapsed time of ring entry r's recorded pairs; `wait` blocks until they have completed.
+void prof_take(hfv_ctx* c, int r, bool wait) {
   for (int k = 0; k < HFV_NUM_KERNEL_SLOTS; ++k)
-    if (c->pending[k]) {
+    if (c->pending[r][k]) {
+      if (wait) (void)hipEventSynchronize(c->ev1[r][k]);
       float ms = 0.f;
-      if (hipEventElapsedTime(&ms, c->ev0[k], c->ev1[k]) == hipSuccess) {
+      hipError_t e = hipEventElapsedTime(&ms, c->ev0[r][k], c->ev1[r][k]);
+      if (e == hipErrorNotReady) continue;  // read at a later collect
+      if (e == hipSuccess) {
         c->times.ms[k] += ms;
         c->times.launches[k] += 1;
       }
-      c->pending[k] = false;
+      c->pending[r][k] = false;
     }
+}
+void prof_collect(hfv_ctx* c) {
+  if (!c->prof) return;
+  for (int r = 0; r < hfv_ctx::PROF_RING; ++r) prof_take(c, r, true);
+}
+// Moves to the next ring entry before a timed step, reading it back first if it is still pending.
+void prof_next(hfv_ctx* c) {
+  c->ring = (c->ring + 1) % hfv_ctx::PROF_RING;
+  prof_take(c, c->ring, true);
 }
 
 template <class T>
@@ -581,6 +598,7 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   hipStream_t st = c->stream;
   unsigned gs = (unsigned)std::max(1, (c->nslots + 63) / 64);
   bool t = timed && c->prof;
+  if (t) prof_next(c);
   hipLaunchKernelGGL(k_begin, dim3(gs), dim3(64), 0, st, B);
   if (t) prof_begin(c, KS_PREDICT);
   hipLaunchKernelGGL(k_mask, dim3(grid_full(c->n)), dim3(BLOCK), 0, st, B, 0);
@@ -685,6 +703,7 @@ void fill_engine_info(const SimEngine& E, hfv_engine_info* out) {
   out->last_n_active = E.last_active;
   out->last_n_face_active = E.last_face_active;
   out->done = E.done || !(E.t_current < E.t_end);
+  out->error_code = E.err;
 }
 
 #define STATE_FIELDS(X)                                                                           \
@@ -765,10 +784,11 @@ int hfv_destroy(hfv_ctx* c) {
   (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
   free_all(c);
-  for (int k = 0; k < HFV_NUM_KERNEL_SLOTS; ++k) {
-    if (c->ev0[k]) (void)hipEventDestroy(c->ev0[k]);
-    if (c->ev1[k]) (void)hipEventDestroy(c->ev1[k]);
-  }
+  for (int r = 0; r < hfv_ctx::PROF_RING; ++r)
+    for (int k = 0; k < HFV_NUM_KERNEL_SLOTS; ++k) {
+      if (c->ev0[r][k]) (void)hipEventDestroy(c->ev0[r][k]);
+      if (c->ev1[r][k]) (void)hipEventDestroy(c->ev1[r][k]);
+    }
   if (c->dcnt) (void)hipFree(c->dcnt);
   if (c->hcnt) (void)hipHostFree(c->hcnt);
   if (c->heng) (void)hipHostFree(c->heng);
@@ -1091,11 +1111,13 @@ int hfv_profile_enable(hfv_ctx* c, int enable) {
   HIP_OK(hipSetDevice(c->device));
   HIP_OK(hipStreamSynchronize(c->stream));
   if (enable) {
-    for (int k = 0; k < HFV_NUM_KERNEL_SLOTS; ++k) {
-      if (!c->ev0[k]) HIP_OK(hipEventCreate(&c->ev0[k]));
-      if (!c->ev1[k]) HIP_OK(hipEventCreate(&c->ev1[k]));
-      c->pending[k] = false;
-    }
+    for (int r = 0; r < hfv_ctx::PROF_RING; ++r)
+      for (int k = 0; k < HFV_NUM_KERNEL_SLOTS; ++k) {
+        if (!c->ev0[r][k]) HIP_OK(hipEventCreate(&c->ev0[r][k]));
+        if (!c->ev1[r][k]) HIP_OK(hipEventCreate(&c->ev1[r][k]));
+        c->pending[r][k] = false;
+      }
+    c->ring = 0;
     c->times = hfv_kernel_times{};
   }
   c->prof = enable != 0;

@@ -202,8 +202,13 @@ __device__ __forceinline__ V3<double> decode_shift(uint8_t code, const double* b
   return mk3(s[0], s[1], s[2]);
 }
 
-__device__ __forceinline__ void flush_events(const Events& ev, DevCounters* cnt) {
-  if (ev.err) atomicCAS(&cnt->err, 0, ev.err);
+// Hard errors are recorded for the batch and for the simulation that raised them; a batch keeps
+// stepping its other simulations (the failed one stops at its next k_begin).
+__device__ __forceinline__ void flush_events(const Events& ev, DevCounters* cnt, SimEngine* eng = nullptr, int s = -1) {
+  if (ev.err) {
+    atomicCAS(&cnt->err, 0, ev.err);
+    if (eng && s >= 0) atomicCAS(&eng[s].err, 0, ev.err);
+  }
   if (ev.check_physical) atomicAdd(&cnt->check_physical, (unsigned long long)ev.check_physical);
   if (ev.brent) atomicAdd(&cnt->brent, (unsigned long long)ev.brent);
   if (ev.grad_fallback) atomicAdd(&cnt->grad_fallback, (unsigned long long)ev.grad_fallback);
@@ -272,6 +277,11 @@ __global__ void k_begin(Batch B) {
   E.n_face_active = 0;
   E.ti_min = ~0ull;
   E.min_bin = 127;
+  if (E.err) {  // a hard error stops this simulation only; the batch's other simulations go on
+    E.force = 0;
+    E.done = 1;
+    return;
+  }
   if (E.force) {  // hfv_step: a step at ti_current with the caller's ti_old
     E.force = 0;
     E.stepping = 1;
@@ -459,7 +469,7 @@ __global__ void __launch_bounds__(256) k_mask(Batch B, int mode) {
   long long pt = wave_append(touch, &B.cnt->touch_count);
   if (pt >= 0) B.W.touch_list[pt] = (int32_t)i;
   sim_add(valid && act, s, B.eng, 0);
-  if (valid) flush_events(ev, B.cnt);
+  if (valid) flush_events(ev, B.cnt, B.eng, s);
 }
 
 // ------------------------------------------------------------------ 1D moving mesh on the device
@@ -479,7 +489,10 @@ __global__ void k_mesh1d_faces(Batch B) {
   double x;
   if (k >= 1 && k < n) {
     double a = B.C.loc[cb + k - 1], b = B.C.loc[cb + k];
-    if (!(a < b)) atomicCAS(&B.cnt->err, 0, E_UNSUPPORTED);
+    if (!(a < b)) {
+      atomicCAS(&B.cnt->err, 0, E_UNSUPPORTED);
+      atomicCAS(&B.eng[s].err, 0, E_UNSUPPORTED);
+    }
     x = 0.5 * (a + b);
   } else if (!D.periodic) {
     x = (k == 0) ? 0. : D.cfg.box[0];
@@ -717,7 +730,7 @@ __global__ void __launch_bounds__(256) k_flux(Batch B) {
       }
     }
     sim_add(valid && computed, s, B.eng, 1);
-    if (valid) flush_events(ev, B.cnt);
+    if (valid) flush_events(ev, B.cnt, B.eng, s);
   }
 }
 
@@ -763,10 +776,11 @@ __global__ void __launch_bounds__(256) k_apply(Batch B, int mode) {
     int64_t i = dense ? k : (int64_t)B.W.touch_list[k];
     bool a = C.act[i];
     Events ev;
-    const Cfg& cfg = B.sims[sim_of(C, i)].cfg;
+    const int s = sim_of(C, i);
+    const Cfg& cfg = B.sims[s].cfg;
     if (!(mode & 1)) {
       if (a) st5(C.prim, n, i, c2p(ld5(C.cons, n, i), C.vol[i], cfg.gd, ev));
-      flush_events(ev, B.cnt);
+      flush_events(ev, B.cnt, B.eng, s);
       continue;
     }
     St<double> acc = ld5(C.facc, n, i);
@@ -820,7 +834,7 @@ __global__ void __launch_bounds__(256) k_apply(Batch B, int mode) {
       if (mode & 2) st5(C.prim, n, i, c2p(c, C.vol[i], cfg.gd, ev));
     }
     st5(C.facc, n, i, acc);
-    flush_events(ev, B.cnt);
+    flush_events(ev, B.cnt, B.eng, s);
   }
 }
 
@@ -1068,7 +1082,7 @@ __global__ void __launch_bounds__(256) k_grad(Batch B, int do_grad, int do_ts) {
       if (__lane_id() == leader) atomicMin(&B.eng[su].ti_min, v);
       if (mine) todo = false;
     }
-    if (valid) flush_events(ev, B.cnt);
+    if (valid) flush_events(ev, B.cnt, B.eng, s);
   }
 }
 
@@ -1180,7 +1194,7 @@ __global__ void k_init_ic(Batch B, int64_t lo, int64_t hi, const double* mass, c
   }
   st5(C.cons, n, i, c);
   st5(C.prim, n, i, prim);
-  flush_events(ev, B.cnt);
+  flush_events(ev, B.cnt, B.eng, sim_of(C, i));
 }
 
 // Space::status sums (space.rs:1248-1280) over cells [lo, hi): one block, fixed-order tree reduction.

@@ -62,7 +62,7 @@ def sod_1d(n=1000, seed=1, t_end=0.2, motion=FLUID, **kw):
     return _finish("sod1d", cfg, loc, rho, vel, P)
 
 
-def sedov_2d(side=256, seed=2, t_end=0.05, solver=HLLC, threads=0, **kw):
+def sedov_2d(side=256, seed=2, t_end=0.05, solver=HLLC, threads=0, gamma=5.0 / 3.0, **kw):
     """2D Sedov blast on the unit periodic box: rho 1, P 1e-6, E = 1 shared by the cells with r < 2 dx."""
     rng = np.random.default_rng(seed)
     box = (1.0, 1.0, 1.0)
@@ -72,7 +72,7 @@ def sedov_2d(side=256, seed=2, t_end=0.05, solver=HLLC, threads=0, **kw):
     rho = np.ones(n)
     P = np.full(n, 1e-6)
     vel = np.zeros((n, 3))
-    cfg = Config(dim=2, box=box, gamma=5.0 / 3.0, cfl=0.3, solver=solver, boundary=PERIODIC, motion=FIXED,
+    cfg = Config(dim=2, box=box, gamma=gamma, cfl=0.3, solver=solver, boundary=PERIODIC, motion=FIXED,
                  t_end=t_end, **kw)
     prob = _finish(f"sedov2d_{side}", cfg, loc, rho, vel, P, threads)
     r = np.linalg.norm(loc[:, :2] - 0.5, axis=1)

@@ -152,6 +152,8 @@ typedef struct {
   int64_t last_n_active;
   int64_t last_n_face_active;
   int done;
+  int error_code;          /* the simulation's first hard error (HFV_ERR_*), 0 if none; a batch slot
+                              that fails stops (done = 1) while the other slots go on */
 } hfv_engine_info;
 
 /* Soft-event counters (the reference eprintln!s these and continues). */

@@ -94,3 +94,24 @@ def test_batch_fp32_flux_matches_fp64_l1():
         l1 = float(np.mean(np.abs(r32 - r64)))
         say(f"  slot {s}: L1(rho) fp32 vs fp64 = {l1:.2e}")
         assert l1 < 1e-5
+
+
+def test_failed_member_stops_alone():
+    """A hard error in one simulation (here: more dump sync points between two steps than the engine
+    drifts through, HFV_ERR_UNSUPPORTED) stops that slot only; the other slots run to the end."""
+    good = [ics.riemann_tube_1d(300, 1.0, 1.0, 0.3, 0.2, 0.1, -0.2, 5.0 / 3.0, t_end=0.02),
+            ics.sedov_2d(side=16, t_end=0.005)]
+    bad = ics.riemann_tube_1d(300, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, t_end=0.02, dt_snap=1e-6)
+    probs = [good[0], bad, good[1]]
+    b = fvs.Batch.for_problems(probs)
+    b.run(1 << 40)
+    ei = b.engine_info(1)
+    say(f"  failed slot: error_code={ei['error_code']} after {ei['step_count']} steps")
+    assert ei["done"] and ei["error_code"] == -7  # HFV_ERR_UNSUPPORTED
+    for s in (0, 2):
+        assert b.engine_info(s)["error_code"] == 0
+        alone, ai = _alone(probs[s], 1 << 40)
+        assert b.engine_info(s)["step_count"] == ai["step_count"]
+        got = b.get_state(s)
+        for k in alone:
+            assert np.array_equal(got[k], alone[k]), (s, k)
