@@ -1,18 +1,26 @@
-"""Benchmark of the MI355X finite-volume path.
+"""Benchmark of the MI355X finite-volume path (one GPU).
 
-python bench.py [--gpus 1] [--steps K] [--warmup W] [--side 256] [--cpu-seconds S] [--no-cpu]
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|ensemble|sedov|kh3d|serve] ...
 
-Workload: 2D Sedov blast, 256x256 generators on the unit periodic box (jitter +-0.05 dx), gamma 5/3,
-HLLC, hierarchical timebins, fixed mesh.  One step is one OptimalOrderRunner::step (one engine sync
-point) over the device-resident state; W untimed steps, then K timed steps bracketed by device
-synchronisation.  value = cell updates (active cells summed over the K steps) / wall time.
+Headline (value): the batched ensemble -- 512 independent 1D/2D Riemann-problem simulations (cell
+counts log-uniform in [200, 65536], HLLC, FP32 flux / FP64 accumulate) in one device context.  One
+step is one batch launch: every unfinished member advances by one engine step (one
+OptimalOrderRunner::step with its own hierarchical timebins).  W untimed launches, then K timed
+launches bracketed by device synchronisation; value = cell updates (active cells summed over members
+and launches) / wall time of the K launches.
 
 Printed JSON line: the contract keys plus
-  roofline     dominant kernel (by device time in the timed region, from HIP events on the context's
-               stream): algorithmic bytes per launch / average launch time vs 8 TB/s
-  cpu_baseline the oracle (C restatement of the reference path, OpenMP at the reference's rayon sites)
-               on the host cores, over a bounded sample of the same steps
-  fp32         the FP32-flux / FP64-accumulate path on the same steps, with L1(rho) vs FP64
+  speedup_vs_plain_fp64  value / the plain single-simulation FP64 device path on the same members and
+                         step windows (members 0, 16, 32, ... each run alone, timed the same way)
+  l1_gate                L1(rho) of the FP32-flux batch vs the FP64 batch, over members that reached
+                         t_end in both runs (all members of the window otherwise at equal time)
+  roofline               dominant kernel of the timed launches (HIP events on the context's stream):
+                         algorithmic bytes per launch / average launch time vs 8 TB/s
+  cpu_baseline           the oracle (C restatement of the reference path, OpenMP at the reference's
+                         rayon sites) on the host cores, over a bounded sample of the same members
+  sedov2d / kh3d / serve the 2D Sedov blast (plain FP64 and FP32), the 3D Kelvin-Helmholtz layer
+                         (plain FP64 with its roofline, and FP32) and the served request stream
+                         (p50/p99 job latency, cell updates/s, L1 gate)
 """
 import argparse
 import json
@@ -28,15 +36,14 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-updates/s vs plain FP64 GPU path; p50/p99 job latency at L1 err<1e-5"
 
-# Algorithmic HBM bytes per unit of work, FP64 layout (one read of every operand a phase needs and one
-# write of every output, ideal caching within a phase):
-#   flux kernel: per active cell 329 B (prim 40, grad 120, gc 24, extrap 40, centroid 24, loc 24, v 24,
-#                v_rel 24, dt 8, active flag 1) + per computed face 121 B (area 8, normal 24, centroid 24,
-#                left/right 8, shift code 1, flux record 56)
-#   accumulate+c2p: per cell 89 B (flag 1, accumulator r/w 80, CSR offset 8) + per active cell 208 B
-#                (conserved r/w 80, msv/maor r/w 32, volume 8, prim write 40, + 48 spare of the record's
-#                v_max/a_over_r), per face record read 56 B x 2 sides + CSR entry 4 B x 2
-#   gradients+timestep: per active cell 600 B, per face of an active cell 60 B
+# Algorithmic HBM bytes per unit of work (device state is FP64 in both precisions; the FP32 path
+# converts in registers), one read of every operand a kernel needs and one write of every output:
+#   flux: per active cell 329 B (prim 40, grad 120, gc 24, extrap 40, centroid 24, loc 24, v 24, v_rel 24,
+#         dt 8, flag 1) + per computed face 121 B (area 8, normal 24, centroid 24, left/right 8, shift 1,
+#         flux record 56)
+#   apply_c2p: per cell 89 B (flag 1, accumulator r/w 80, CSR offset 8) + per active cell 208 B + per
+#         face 120 B (record 56 x 2 sides, CSR entries 4 x 2)
+#   grad_timestep: per active cell 600 B + per face of an active cell 120 B
 ALGO = {
     "flux": lambda n_act, f_act, n_all: 329 * n_act + 121 * f_act,
     "apply_c2p": lambda n_act, f_act, n_all: 89 * n_all + 208 * n_act + 120 * f_act,
@@ -51,7 +58,128 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def run_device(prob, warmup, steps, profile=True):
+def roofline(kt, launches_timed, cell_updates, face_fluxes, n_all):
+    """Dominant kernel (most device time) of a timed region; bytes from ALGO at the region's mean
+    active counts per launch."""
+    name, rec = max(((k, v) for k, v in kt.items() if v["launches"] > 0), key=lambda kv: kv[1]["ms"])
+    launches = rec["launches"]
+    avg_s = rec["ms"] / 1e3 / launches
+    steps = max(launches_timed, 1)
+    n_act = cell_updates / steps
+    f_act = face_fluxes / steps
+    algo = ALGO.get(name, lambda *_: 0)(n_act, f_act, n_all)
+    achieved = algo / avg_s / 1e9
+    return {
+        "kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "avg_launch_us": round(avg_s * 1e6, 3),
+        "algorithmic_bytes_per_launch": int(algo), "launches": launches,
+        "per_kernel_ms": {k: round(v["ms"], 3) for k, v in kt.items() if v["launches"]},
+    }
+
+
+def _l1(a, b):
+    return float(np.mean(np.abs(np.asarray(a) - np.asarray(b))))
+
+
+# ------------------------------------------------------------------ ensemble (headline)
+def bench_ensemble(args):
+    from hydro_rs_amd import ensemble, fvs, ics
+
+    t = time.perf_counter()
+    probs = ensemble.ensemble_problems(count=args.members, seed=4, precision=fvs.FP32_FLUX)
+    n_all = sum(p.n for p in probs)
+    log(f"[ensemble] {len(probs)} members, {n_all} cells, generated in {time.perf_counter() - t:.1f}s")
+    b = fvs.Batch.for_problems(probs)
+    b.run(args.warmup)
+    b.synchronize()
+    i0 = [b.engine_info(s) for s in range(len(probs))]
+    b.profile(True)
+    t0 = time.perf_counter()
+    launches = b.run(args.steps)
+    b.synchronize()
+    secs = time.perf_counter() - t0
+    i1 = [b.engine_info(s) for s in range(len(probs))]
+    kt = b.kernel_times()
+    cu = sum(a["cell_updates"] - z["cell_updates"] for a, z in zip(i1, i0))
+    ff = sum(a["face_fluxes"] - z["face_fluxes"] for a, z in zip(i1, i0))
+    value = cu / secs
+    log(f"[ensemble] fp32: {launches} launches in {secs:.3f}s -> {value:.4e} cell-updates/s")
+    res = {"launches": launches, "seconds": secs, "cell_updates": cu, "face_fluxes": ff, "value": value,
+           "n_all": n_all, "members_done": sum(1 for i in i1 if i["done"]),
+           "members_failed": [s for s, i in enumerate(i1) if i["error_code"]],
+           "roofline": roofline(kt, launches, cu, ff, n_all)}
+    # the same window in FP64 (batched) for the L1 gate
+    p64 = [ics.with_precision(p, fvs.FP64) for p in probs]
+    b64 = fvs.Batch.for_problems(p64)
+    t1 = time.perf_counter()
+    b64.run(args.warmup + launches)
+    b64.synchronize()
+    res["fp64_batched_value"] = sum(b64.engine_info(s)["cell_updates"] for s in range(len(p64))) / (time.perf_counter() - t1)
+    l1s, same_t = [], 0
+    for s, p in enumerate(probs):
+        e32, e64 = i1[s], b64.engine_info(s)
+        if e32["error_code"] or e64["error_code"]:
+            continue
+        if e32["ti_current"] != e64["ti_current"]:
+            continue
+        same_t += 1
+        r32 = b.get_state(s, ["prim"])["prim"][: p.n]
+        r64 = b64.get_state(s, ["prim"])["prim"][: p.n]
+        l1s.append(_l1(r32, r64))
+    res["l1_gate"] = {"members_compared": same_t, "max_l1_rho": max(l1s) if l1s else None,
+                      "mean_l1_rho": float(np.mean(l1s)) if l1s else None,
+                      "pass": bool(l1s) and max(l1s) < 1e-5}
+    log(f"[ensemble] L1 gate: {res['l1_gate']}")
+    b64.close()
+    b.close()
+    # plain single-simulation FP64 path on a sample of members, same windows
+    idx = list(range(0, len(probs), args.plain_stride))
+    st = ensemble.run_plain([p64[i] for i in idx], max_steps=args.steps, warmup=args.warmup)
+    res["plain_fp64"] = {"value": st["cell_updates"] / st["seconds"], "members": idx, "steps": st["steps"],
+                         "seconds": st["seconds"], "failed": st["failed"]}
+    res["speedup_vs_plain_fp64"] = value / res["plain_fp64"]["value"]
+    log(f"[ensemble] plain fp64 sample: {res['plain_fp64']['value']:.4e} cell-updates/s -> "
+        f"speedup {res['speedup_vs_plain_fp64']:.1f}x")
+    return res, probs
+
+
+def cpu_baseline_ensemble(probs, stride, warmup, budget_s):
+    """The oracle over members 0, stride, 2*stride, ... (FP64, as the reference), each for up to
+    budget/len steps' worth of time after `warmup` untimed steps."""
+    from oracle import ora  # checker / CPU baseline only
+    from tests.helpers import oracle_sim
+    from hydro_rs_amd import fvs, ics
+
+    threads = ora.set_threads(0)
+    idx = list(range(0, len(probs), stride))
+    per = budget_s / len(idx)
+    cu = 0
+    secs = 0.0
+    steps = 0
+    for i in idx:
+        sim = oracle_sim(ics.with_precision(probs[i], fvs.FP64))
+        try:
+            sim.advance(warmup)
+        except RuntimeError:
+            continue
+        t0 = time.perf_counter()
+        while time.perf_counter() - t0 < per:
+            try:
+                if sim.advance(1) == 0:
+                    break
+            except RuntimeError:
+                break
+            cu += sim.info()["last_n_active"]
+            steps += 1
+        secs += time.perf_counter() - t0
+    return {"value": cu / secs if secs > 0 else 0.0, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"members {idx[0]}..{idx[-1]} step {len(idx)} of the ensemble, each for {per:.2f}s after "
+                      f"{warmup} untimed steps: {steps} steps, {cu} cell updates in {secs:.1f}s, "
+                      f"OpenMP threads={threads}"}
+
+
+# ------------------------------------------------------------------ single simulations
+def run_device(prob, warmup, steps, profile=False):
     from hydro_rs_amd import ics
 
     dev = ics.build_device_sim(prob)
@@ -68,60 +196,114 @@ def run_device(prob, warmup, steps, profile=True):
     kt = dev.kernel_times() if profile else None
     return dev, {
         "steps": done, "seconds": t1 - t0, "cell_updates": e1["cell_updates"] - e0["cell_updates"],
-        "face_fluxes": e1["face_fluxes"] - e0["face_fluxes"], "kernel_times": kt, "t_end_reached": e1["done"],
-        "step0": e0["step_count"],
+        "face_fluxes": e1["face_fluxes"] - e0["face_fluxes"], "kernel_times": kt, "done": e1["done"],
+        "ti_current": e1["ti_current"],
     }
 
 
-def roofline(stats, n_all):
-    kt = stats["kernel_times"]
-    name, rec = max(((k, v) for k, v in kt.items() if v["launches"] > 0), key=lambda kv: kv[1]["ms"])
-    launches = rec["launches"]
-    avg_s = rec["ms"] / 1e3 / launches
-    steps = max(stats["steps"], 1)
-    n_act = stats["cell_updates"] / steps
-    f_act = stats["face_fluxes"] / steps
-    algo = ALGO.get(name, lambda *_: 0)(n_act, f_act, n_all)
-    achieved = algo / avg_s / 1e9
-    return {
-        "kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "avg_launch_us": round(avg_s * 1e6, 3),
-        "algorithmic_bytes_per_launch": int(algo), "launches": launches,
-        "per_kernel_ms": {k: round(v["ms"], 3) for k, v in kt.items() if v["launches"]},
-    }
+def bench_sedov(args):
+    from hydro_rs_amd import fvs, ics
+
+    prob = ics.sedov_2d(side=args.sedov_side, t_end=0.05)
+    log(f"[sedov] {prob.n} cells, {prob.mesh.n_faces} faces; FP64 to t_end")
+    d64, s64 = run_device(prob, 0, 1 << 40)
+    d32, s32 = run_device(ics.with_precision(prob, fvs.FP32_FLUX), 0, 1 << 40)
+    l1 = _l1(d32.get_state(["prim"])["prim"][: prob.n], d64.get_state(["prim"])["prim"][: prob.n])
+    out = {"workload": f"sedov2d_{args.sedov_side}x{args.sedov_side}_periodic_hllc_t0.05", "cells": prob.n,
+           "faces": prob.mesh.n_faces}
+    for k, s in (("fp64", s64), ("fp32_flux", s32)):
+        out[k] = {"cell_updates_per_s": s["cell_updates"] / s["seconds"],
+                  "face_fluxes_per_s": s["face_fluxes"] / s["seconds"], "steps": s["steps"],
+                  "us_per_step": 1e6 * s["seconds"] / max(s["steps"], 1), "reached_t_end": bool(s["done"])}
+    out["l1_rho_fp32_vs_fp64_at_t_end"] = l1
+    log(f"[sedov] {out}")
+    d64.close()
+    d32.close()
+    return out, prob
 
 
-def cpu_baseline(prob, warmup, budget_s):
-    from oracle import ora  # checker / CPU baseline only
-    from tests.helpers import oracle_sim
+def bench_kh3d(args):
+    from hydro_rs_amd import fvs, ics
 
-    threads = ora.set_threads(0)
-    sim = oracle_sim(prob)
-    sim.advance(warmup)
-    i0 = sim.info()
-    cu = 0
-    t0 = time.perf_counter()
-    steps = 0
-    while time.perf_counter() - t0 < budget_s:
-        if sim.advance(1) == 0:
-            break
-        cu += sim.info()["last_n_active"]
-        steps += 1
-    dt = time.perf_counter() - t0
-    return {"value": cu / dt if dt > 0 else 0.0, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"{steps} steps of the same run after step {i0['step_count']} ({cu} cell updates, "
-                      f"{dt:.1f} s, OpenMP threads={threads})"}
+    t = time.perf_counter()
+    prob = ics.kelvin_helmholtz_3d(side=args.kh_side, t_end=1.0, threads=args.threads)
+    log(f"[kh3d] {prob.n} cells, {prob.mesh.n_faces} faces, tessellated in {time.perf_counter() - t:.1f}s")
+    out = {"workload": f"kh3d_{args.kh_side}^3_periodic_airs", "cells": prob.n, "faces": prob.mesh.n_faces,
+           "warmup": args.kh_warmup, "steps": args.kh_steps}
+    devs = {}
+    for k, prec in (("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)):
+        d, s = run_device(ics.with_precision(prob, prec), args.kh_warmup, args.kh_steps, profile=True)
+        rl = roofline(s["kernel_times"], s["steps"], s["cell_updates"], s["face_fluxes"], prob.n)
+        n_act = s["cell_updates"] / max(s["steps"], 1)
+        f_act = s["face_fluxes"] / max(s["steps"], 1)
+        step_bytes = 192 * prob.n + 1247 * n_act + 256 * f_act
+        out[k] = {"cell_updates_per_s": s["cell_updates"] / s["seconds"],
+                  "face_fluxes_per_s": s["face_fluxes"] / s["seconds"],
+                  "ms_per_step": 1e3 * s["seconds"] / max(s["steps"], 1),
+                  "step_roofline_frac": step_bytes / (s["seconds"] / max(s["steps"], 1)) / (HBM_PEAK_GBS * 1e9),
+                  "roofline": rl, "ti_current": s["ti_current"]}
+        devs[k] = d
+        log(f"[kh3d] {k}: {out[k]['ms_per_step']:.2f} ms/step, {out[k]['cell_updates_per_s']:.3e} cell-updates/s, "
+            f"dominant {rl['kernel']} {rl['achieved']} GB/s")
+    if out["fp64"]["ti_current"] == out["fp32_flux"]["ti_current"]:
+        out["l1_rho_fp32_vs_fp64"] = _l1(devs["fp32_flux"].get_state(["prim"])["prim"][: prob.n],
+                                         devs["fp64"].get_state(["prim"])["prim"][: prob.n])
+    for d in devs.values():
+        d.close()
+    return out
+
+
+def bench_serve(args):
+    from hydro_rs_amd import fvs, ics, serve
+
+    jobs = serve.served_jobs(count=args.serve_jobs, rate=args.serve_rate, seed=5, side_min=32,
+                             side_max=args.serve_side_max, sedov_t_end=args.serve_sedov_t_end,
+                             vortex_t_end=args.serve_vortex_t_end, threads=args.threads)
+    log(f"[serve] {len(jobs)} jobs, rate {args.serve_rate}/s, {sum(j.prob.n for j in jobs)} cells in total")
+    srv = serve.Server(hbm_budget_bytes=int(args.serve_hbm_gb * 2 ** 30), launches_per_poll=args.serve_poll)
+    done = srv.serve(jobs, fetch=("prim",), progress=lambda t: log(f"[serve] {t}"))
+    summ = serve.latency_summary(done, srv.wall)
+    # L1 gate: jobs 0, k, 2k, ... rerun alone in FP64
+    l1s = {}
+    for j in done[:: max(1, len(done) // args.serve_l1_jobs)]:
+        d = ics.build_device_sim(ics.with_precision(j.prob, fvs.FP64))
+        d.run()
+        ref = d.get_state(["prim"])["prim"][: j.prob.n]
+        d.close()
+        l1s[j.jid] = _l1(j.result["prim"][: j.prob.n], ref)
+    summ.update({"rate_jobs_per_s": args.serve_rate, "hbm_budget_gb": args.serve_hbm_gb,
+                 "slots": len(srv.caps[0]), "l1_rho_vs_fp64": {"jobs": len(l1s), "max": max(l1s.values()),
+                                                               "pass": max(l1s.values()) < 1e-5},
+                 "jobs_desc": f"Poisson arrivals, 50/50 Sedov (t_end {args.serve_sedov_t_end}) / isentropic vortex "
+                              f"(t_end {args.serve_vortex_t_end}), side log-uniform in [32, {args.serve_side_max}], "
+                              "gamma 1.4, FP32 flux"})
+    log(f"[serve] {summ}")
+    return summ
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2000)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--side", type=int, default=256)
+    ap.add_argument("--steps", type=int, default=1000, help="timed batch launches of the ensemble")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="all", choices=["all", "ensemble", "sedov", "kh3d", "serve"])
+    ap.add_argument("--members", type=int, default=512)
+    ap.add_argument("--plain-stride", type=int, default=16)
+    ap.add_argument("--sedov-side", type=int, default=256)
+    ap.add_argument("--kh-side", type=int, default=128)
+    ap.add_argument("--kh-steps", type=int, default=20)
+    ap.add_argument("--kh-warmup", type=int, default=3)
+    ap.add_argument("--serve-jobs", type=int, default=48)
+    ap.add_argument("--serve-rate", type=float, default=6.0)
+    ap.add_argument("--serve-side-max", type=int, default=512)
+    ap.add_argument("--serve-sedov-t-end", type=float, default=0.02)
+    ap.add_argument("--serve-vortex-t-end", type=float, default=0.5)
+    ap.add_argument("--serve-hbm-gb", type=float, default=24.0)
+    ap.add_argument("--serve-poll", type=int, default=8)
+    ap.add_argument("--serve-l1-jobs", type=int, default=8)
+    ap.add_argument("--threads", type=int, default=0, help="host tessellation threads (0: all)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp32", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain device copy of this many MiB (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -137,46 +319,59 @@ def main():
         print(json.dumps({"copy_probe_bytes_read": nbytes, "copy_probe_bytes_written": nbytes, "launches": 5}))
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (each job runs on one GPU, "
-              "see DESIGN.md for how the path shards)", file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported: every simulation and every "
+              "batch runs on one GPU here (the multi-GPU layout is design only)", file=sys.stderr)
         sys.exit(2)
 
-    from hydro_rs_amd import fvs, ics
+    from hydro_rs_amd import fvs
 
     if fvs.device_count() < 1:
         raise SystemExit("no HIP device visible")
-    log(f"[bench] building sedov2d side={args.side}")
-    prob = ics.sedov_2d(side=args.side, t_end=0.05)
-    n = prob.n
-    log(f"[bench] N={n} F={prob.mesh.n_faces}; FP64 warmup={args.warmup} steps={args.steps}")
-    dev, s64 = run_device(prob, args.warmup, args.steps)
-    rl = roofline(s64, n)
-    value = s64["cell_updates"] / s64["seconds"]
-    log(f"[bench] fp64: {s64['steps']} steps in {s64['seconds']:.3f}s -> {value:.4e} cell-updates/s; "
-        f"dominant {rl['kernel']} {rl['achieved']} GB/s")
-    out = {
-        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1, "steps": s64["steps"],
-        "warmup": args.warmup, "ms_per_step": 1e3 * s64["seconds"] / max(s64["steps"], 1),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"sedov2d_{args.side}x{args.side}_periodic_hllc_fp64", "cells": n,
-                   "faces": prob.mesh.n_faces, "gamma": 5.0 / 3.0, "t_end": 0.05, "timebins": "hierarchical",
-                   "parallelism": "single GPU"},
-        "face_fluxes_per_s": s64["face_fluxes"] / s64["seconds"],
-        "cell_updates": s64["cell_updates"],
-        "roofline": rl,
-    }
-    if not args.no_fp32:
-        p32 = ics.sedov_2d(side=args.side, t_end=0.05, precision=fvs.FP32_FLUX)
-        d32, s32 = run_device(p32, args.warmup, args.steps, profile=False)
-        r64 = dev.get_state(["prim"])["prim"][:n]
-        r32 = d32.get_state(["prim"])["prim"][:n]
-        out["fp32_flux"] = {"value": s32["cell_updates"] / s32["seconds"], "unit": "cell-updates/s",
-                            "steps": s32["steps"], "l1_rho_vs_fp64": float(np.mean(np.abs(r32 - r64)))}
-        log(f"[bench] fp32: {out['fp32_flux']}")
-    if not args.no_cpu:
-        log(f"[bench] cpu baseline (oracle), budget {args.cpu_seconds}s")
-        out["cpu_baseline"] = cpu_baseline(prob, args.warmup, args.cpu_seconds)
-        log(f"[bench] cpu: {out['cpu_baseline']}")
+    w = args.workload
+    out = {"metric": METRIC, "unit": "cell-updates/s", "n_gpus": 1, "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "data": "synthetic"}
+    if w in ("all", "ensemble"):
+        ens, probs = bench_ensemble(args)
+        out.update({
+            "value": ens["value"], "steps": ens["launches"], "warmup": args.warmup,
+            "ms_per_step": 1e3 * ens["seconds"] / max(ens["launches"], 1), "dtype": "f32 flux / f64 accumulate",
+            "config": {"workload": f"ensemble{args.members}_riemann_1d2d_hllc_fp32flux", "members": args.members,
+                       "cells": ens["n_all"], "cell_range": [200, 65536], "t_end": 0.05,
+                       "step": "one batch launch = one engine step of every unfinished member",
+                       "parallelism": "single GPU, one batch context"},
+            "face_fluxes_per_s": ens["face_fluxes"] / ens["seconds"],
+            "speedup_vs_plain_fp64": ens["speedup_vs_plain_fp64"], "plain_fp64": ens["plain_fp64"],
+            "fp64_batched_value": ens["fp64_batched_value"], "l1_gate": ens["l1_gate"],
+            "members_done": ens["members_done"], "members_failed": ens["members_failed"],
+            "roofline": ens["roofline"],
+        })
+        if not args.no_cpu:
+            log(f"[bench] cpu baseline (oracle), budget {args.cpu_seconds}s")
+            out["cpu_baseline"] = cpu_baseline_ensemble(probs, args.plain_stride, args.warmup, args.cpu_seconds)
+            log(f"[bench] cpu: {out['cpu_baseline']}")
+        del probs
+    if w in ("all", "sedov"):
+        out["sedov2d"], _ = bench_sedov(args)
+    if w in ("all", "kh3d"):
+        out["kh3d"] = bench_kh3d(args)
+    if w in ("all", "serve"):
+        out["serve"] = bench_serve(args)
+    if "value" not in out:  # a single secondary workload: its own headline numbers
+        if w == "sedov":
+            s = out["sedov2d"]["fp64"]
+            out.update({"value": s["cell_updates_per_s"], "steps": s["steps"], "warmup": 0,
+                        "ms_per_step": s["us_per_step"] / 1e3, "dtype": "f64",
+                        "config": {"workload": out["sedov2d"]["workload"]}})
+        elif w == "kh3d":
+            s = out["kh3d"]["fp64"]
+            out.update({"value": s["cell_updates_per_s"], "steps": args.kh_steps, "warmup": args.kh_warmup,
+                        "ms_per_step": s["ms_per_step"], "dtype": "f64", "roofline": s["roofline"],
+                        "config": {"workload": out["kh3d"]["workload"]}})
+        else:
+            s = out["serve"]
+            out.update({"value": s["cell_updates_per_s"], "steps": s["jobs"], "warmup": 0,
+                        "ms_per_step": 1e3 * s["wall_s"] / max(s["jobs"], 1), "dtype": "f32 flux / f64 accumulate",
+                        "config": {"workload": "served_stream", "jobs": s["jobs"]}})
     print(json.dumps(out), flush=True)
 
 

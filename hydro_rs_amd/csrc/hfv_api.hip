@@ -42,14 +42,9 @@ int set_err(int code, const char* what) {
     if (_r != HFV_OK) return _r; \
   } while (0)
 
-constexpr int BLOCK = 256;
-constexpr int64_t LIST_GRID_CAP = 4096;  // blocks of a list kernel (grid-stride beyond)
+constexpr int BLOCK = HFV_BLOCK;
 inline unsigned grid_full(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + BLOCK - 1) / BLOCK); }
-inline unsigned grid_list(int64_t bound) {
-  int64_t g = (bound + BLOCK - 1) / BLOCK;
-  g = std::min<int64_t>(std::max<int64_t>(g, 8), LIST_GRID_CAP);
-  return (unsigned)((g + 7) / 8 * 8);  // multiple of 8 for the XCD-aware block order
-}
+inline int64_t chunks_of(int64_t items) { return std::max<int64_t>(1, (items + BLOCK - 1) / BLOCK); }
 
 template <class T>
 Gas<T> make_gas(double gamma, int eos, double iso_u) {
@@ -140,7 +135,13 @@ struct hfv_ctx {
   int64_t n = 0, nf = 0, nconn = 0;    // capacities (totals)
   Cells C{};
   Faces F{};
-  Work W{};
+  // block tables of the dense cell / face kernels (see hfv_kernels.hpp): chunk ranges per slot, the
+  // table (position b of the grid holds chunk perm(b)), its host copy and the per-chunk partials
+  std::vector<int64_t> cchunk0, fchunk0;
+  int64_t ncchunk = 0, nfchunk = 0, ncpos = 0, nfpos = 0;
+  std::vector<BlockJob> hcjobs, hfjobs;
+  BlockJob *dcjobs = nullptr, *dfjobs = nullptr;
+  Partials P{};
   SimDesc* ddesc = nullptr;
   SimEngine* deng = nullptr;
   SimEngine* heng = nullptr;  // pinned mirror
@@ -150,6 +151,7 @@ struct hfv_ctx {
   int precision = HFV_FP64;
   int solver = -1;  // uniform solver of the loaded simulations, or -1
   bool any_moving1d = false;
+  bool any_moving = false;
   hipGraph_t graph{};
   hipGraphExec_t gexec{};
   bool graph_ok = false;
@@ -172,7 +174,9 @@ struct hfv_ctx {
     B.sims = ddesc;
     B.eng = deng;
     B.nsims = nslots;
-    B.W = W;
+    B.cjobs = dcjobs;
+    B.fjobs = dfjobs;
+    B.P = P;
     B.cnt = dcnt;
     return B;
   }
@@ -207,6 +211,43 @@ void drop_graph(hfv_ctx* c) {
   c->graph_ok = false;
 }
 
+// Rebuild the block tables from the loaded slots (their used cells and faces) and upload them with
+// the simulation descriptors.  Chunk k sits at grid position (k % Q) * 8 + k / Q, Q = positions / 8:
+// the XCD that runs position b (b % 8) gets the contiguous chunks [xcd * Q, (xcd + 1) * Q).
+int write_jobs(hfv_ctx* c) {
+  const int S = c->nslots;
+  c->hcjobs.assign(c->ncpos, BlockJob{-1, 0, 0, 0});
+  c->hfjobs.assign(c->nfpos, BlockJob{-1, 0, 0, 0});
+  const int64_t qc = c->ncpos / 8, qf = c->nfpos / 8;
+  for (int s = 0; s < S; ++s) {
+    SimDesc& D = c->desc[s];
+    const int64_t nc = c->loaded[s] ? (c->nused[s] + BLOCK - 1) / BLOCK : 0;
+    const int64_t nfc = c->loaded[s] ? (c->fused[s] + BLOCK - 1) / BLOCK : 0;
+    D.cchunk0 = (int32_t)c->cchunk0[s];
+    D.cchunk1 = (int32_t)(c->cchunk0[s] + nc);
+    D.fchunk0 = (int32_t)c->fchunk0[s];
+    D.fchunk1 = (int32_t)(c->fchunk0[s] + nfc);
+    for (int64_t j = 0; j < nc; ++j) {
+      const int64_t k = c->cchunk0[s] + j;
+      const int64_t lo = c->cbase[s] + j * BLOCK;
+      const int64_t hi = std::min<int64_t>(lo + BLOCK, c->cbase[s] + c->nused[s]);
+      c->hcjobs[(k % qc) * 8 + k / qc] = BlockJob{s, (int32_t)lo, (int32_t)hi, (int32_t)k};
+    }
+    for (int64_t j = 0; j < nfc; ++j) {
+      const int64_t k = c->fchunk0[s] + j;
+      const int64_t lo = c->fbase[s] + j * BLOCK;
+      const int64_t hi = std::min<int64_t>(lo + BLOCK, c->fbase[s] + c->fused[s]);
+      c->hfjobs[(k % qf) * 8 + k / qf] = BlockJob{s, (int32_t)lo, (int32_t)hi, (int32_t)k};
+    }
+  }
+  hipStream_t st = c->stream;
+  HIP_OK(hipMemcpyAsync(c->dcjobs, c->hcjobs.data(), sizeof(BlockJob) * c->ncpos, hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(c->dfjobs, c->hfjobs.data(), sizeof(BlockJob) * c->nfpos, hipMemcpyHostToDevice, st));
+  HIP_OK(hipMemcpyAsync(c->ddesc, c->desc.data(), sizeof(SimDesc) * S, hipMemcpyHostToDevice, st));
+  HIP_OK(hipStreamSynchronize(st));
+  return HFV_OK;
+}
+
 // Allocate every buffer for the slot capacities (cells zeroed, faces' cell indices -1).
 int alloc_batch(hfv_ctx* c) {
   free_all(c);
@@ -224,7 +265,7 @@ int alloc_batch(hfv_ctx* c) {
     nf += c->fcap[s];
     nk += c->kcap[s];
   }
-  if (n >= (1ll << 30) || nf >= (1ll << 29) || nk >= (1ll << 31)) return set_err(HFV_ERR_INVALID, "batch too large for 32-bit indices");
+  if (n >= (1ll << 29) || nf >= (1ll << 29) || nk >= (1ll << 30)) return set_err(HFV_ERR_INVALID, "batch too large for 32-bit indices");
   c->n = n;
   c->nf = nf;
   c->nconn = nk;
@@ -245,11 +286,11 @@ int alloc_batch(hfv_ctx* c) {
   C.msv = dalloc<double>(c, n);
   C.dt = dalloc<double>(c, n);
   C.srad = dalloc<double>(c, n);
+  C.tex = dalloc<double>(c, 5 * n);
+  C.cs = dalloc<double>(c, n);
   C.bin = dalloc<int8_t>(c, n);
   C.wake = dalloc<int8_t>(c, n);
   C.act = dalloc<uint8_t>(c, n);
-  C.touched = dalloc<uint32_t>(c, n);
-  C.sim = c->batch ? dalloc<int32_t>(c, n) : nullptr;
   Faces& F = c->F;
   F.nf = nf;
   F.area = dalloc<double>(c, nf);
@@ -261,23 +302,42 @@ int alloc_batch(hfv_ctx* c) {
   F.shc = dalloc<uint8_t>(c, nf);
   F.off = dalloc<int32_t>(c, n + 1);
   F.conn = dalloc<int32_t>(c, nk);
-  c->W.act_list = dalloc<int32_t>(c, n);
-  c->W.touch_list = dalloc<int32_t>(c, n);
-  c->W.face_list = dalloc<int32_t>(c, nf);
+  F.cur = dalloc<uint8_t>(c, nf);
+  // chunks: a slot's cell capacity and face capacity in blocks of BLOCK items
+  c->cchunk0.assign(S, 0);
+  c->fchunk0.assign(S, 0);
+  int64_t ncc = 0, nfc = 0;
+  for (int s = 0; s < S; ++s) {
+    c->cchunk0[s] = ncc;
+    c->fchunk0[s] = nfc;
+    ncc += chunks_of(c->ccap[s]);
+    nfc += chunks_of(c->fcap[s]);
+  }
+  c->ncchunk = ncc;
+  c->nfchunk = nfc;
+  c->ncpos = (ncc + 7) / 8 * 8;
+  c->nfpos = (nfc + 7) / 8 * 8;
+  c->dcjobs = dalloc<BlockJob>(c, c->ncpos);
+  c->dfjobs = dalloc<BlockJob>(c, c->nfpos);
+  c->P.act = dalloc<uint32_t>(c, ncc);
+  c->P.face = dalloc<uint32_t>(c, nfc);
+  c->P.ti = dalloc<uint64_t>(c, ncc);
   c->ddesc = dalloc<SimDesc>(c, S);
   c->deng = dalloc<SimEngine>(c, S);
   for (void* p : c->bufs)
     if (!p) return set_err(HFV_ERR_HIP, "hipMalloc failed");
   hipStream_t st = c->stream;
   const size_t sd = sizeof(double) * (size_t)n;
-  double* dbl[] = {C.prim, C.grad, C.gc, C.extrap, C.cons, C.facc, C.vol, C.loc, C.cen, C.v, C.vrel, C.maor, C.msv, C.dt, C.srad};
-  int mult[] = {5, 15, 3, 5, 5, 5, 1, 3, 3, 3, 3, 1, 1, 1, 1};
-  for (int k = 0; k < 15; ++k) HIP_OK(hipMemsetAsync(dbl[k], 0, sd * mult[k], st));
+  double* dbl[] = {C.prim, C.grad, C.gc, C.extrap, C.cons, C.facc, C.vol, C.loc, C.cen, C.v, C.vrel, C.maor, C.msv, C.dt, C.srad, C.tex, C.cs};
+  int mult[] = {5, 15, 3, 5, 5, 5, 1, 3, 3, 3, 3, 1, 1, 1, 1, 5, 1};
+  for (int k = 0; k < 17; ++k) HIP_OK(hipMemsetAsync(dbl[k], 0, sd * mult[k], st));
   HIP_OK(hipMemsetAsync(C.bin, 0, (size_t)n, st));
   HIP_OK(hipMemsetAsync(C.wake, 0, (size_t)n, st));
   HIP_OK(hipMemsetAsync(C.act, 0, (size_t)n, st));
-  HIP_OK(hipMemsetAsync(C.touched, 0, sizeof(uint32_t) * (size_t)n, st));
-  if (C.sim) HIP_OK(hipMemsetAsync(C.sim, 0xFF, sizeof(int32_t) * (size_t)n, st));  // -1: empty
+  HIP_OK(hipMemsetAsync(F.cur, 0, (size_t)nf, st));
+  HIP_OK(hipMemsetAsync(c->P.act, 0, sizeof(uint32_t) * (size_t)ncc, st));
+  HIP_OK(hipMemsetAsync(c->P.face, 0, sizeof(uint32_t) * (size_t)nfc, st));
+  HIP_OK(hipMemsetAsync(c->P.ti, 0xFF, sizeof(uint64_t) * (size_t)ncc, st));
   HIP_OK(hipMemsetAsync(F.rec, 0, sizeof(double) * 7 * (size_t)nf, st));
   HIP_OK(hipMemsetAsync(F.left, 0xFF, sizeof(int32_t) * (size_t)nf, st));
   HIP_OK(hipMemsetAsync(F.right, 0xFF, sizeof(int32_t) * (size_t)nf, st));
@@ -311,7 +371,7 @@ int alloc_batch(hfv_ctx* c) {
   c->fused.assign(S, 0);
   c->kused.assign(S, 0);
   c->loaded.assign(S, 0);
-  return HFV_OK;
+  return write_jobs(c);
 }
 
 int sync_counters(hfv_ctx* c) {
@@ -335,6 +395,7 @@ int push_engines(hfv_ctx* c) {
 void recompute_uniforms(hfv_ctx* c) {
   int sol = -2;
   c->any_moving1d = false;
+  c->any_moving = false;
   int prec = -1;
   for (int s = 0; s < c->nslots; ++s) {
     if (!c->loaded[s]) continue;
@@ -342,6 +403,7 @@ void recompute_uniforms(hfv_ctx* c) {
     if (sol == -2) sol = p.solver;
     else if (sol != p.solver) sol = -1;
     if (p.dim == 1 && p.motion != HFV_FIXED) c->any_moving1d = true;
+    if (p.motion != HFV_FIXED) c->any_moving = true;
     if (prec < 0) prec = p.precision;
   }
   c->solver = sol < 0 ? -1 : sol;
@@ -466,7 +528,7 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
   D.nf = nf;
   D.moving1d = p.dim == 1 && p.motion != HFV_FIXED;
   D.periodic = p.boundary == HFV_PERIODIC;
-  HIP_OK(hipMemcpyAsync(c->ddesc + s, &D, sizeof(SimDesc), hipMemcpyHostToDevice, st));
+  if (c->loaded[s]) TRY(write_jobs(c));  // a live slot's mesh changed (moving mesh)
   // invariant: every slot's first CSR offset is its CSR base (this slot and the next one)
   for (int t = s; t <= std::min(s + 1, c->nslots - 1); ++t) {
     int32_t first = -1;
@@ -487,18 +549,13 @@ int reset_slot(hfv_ctx* c, int s) {
     double* p;
     int w;
   } fl[] = {{C.prim, 5}, {C.grad, 15}, {C.gc, 3}, {C.extrap, 5}, {C.cons, 5}, {C.facc, 5}, {C.loc, 3}, {C.v, 3},
-            {C.vrel, 3}, {C.maor, 1}, {C.msv, 1}, {C.dt, 1}};
+            {C.vrel, 3}, {C.maor, 1}, {C.msv, 1}, {C.dt, 1}, {C.tex, 5}, {C.cs, 1}};
   for (auto& f : fl)
     for (int k = 0; k < f.w; ++k) HIP_OK(hipMemsetAsync(f.p + k * N + cb, 0, sizeof(double) * cap, st));
   HIP_OK(hipMemsetAsync(C.bin + cb, 0, cap, st));
   HIP_OK(hipMemsetAsync(C.wake + cb, 0, cap, st));
   HIP_OK(hipMemsetAsync(C.act + cb, 0, cap, st));
-  HIP_OK(hipMemsetAsync(C.touched + cb, 0, sizeof(uint32_t) * cap, st));
-  if (C.sim) {
-    std::vector<int32_t> sim(cap, -1);
-    for (int64_t i = 0; i < n; ++i) sim[i] = s;
-    HIP_OK(hipMemcpyAsync(C.sim + cb, sim.data(), sizeof(int32_t) * cap, hipMemcpyHostToDevice, st));
-  }
+  (void)n;
   c->heng[s] = fresh_engine(c->params[s]);
   HIP_OK(hipMemcpyAsync(c->deng + s, &c->heng[s], sizeof(SimEngine), hipMemcpyHostToDevice, st));
   HIP_OK(hipStreamSynchronize(st));
@@ -524,7 +581,7 @@ int init_ic(hfv_ctx* c, int s, int64_t n, const double* loc, const double* mass,
   HIP_OK(hipMemcpyAsync(dm, mass, sizeof(double) * n, hipMemcpyHostToDevice, st));
   HIP_OK(hipMemcpyAsync(dv, vels.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
   HIP_OK(hipMemcpyAsync(du, u_int, sizeof(double) * n, hipMemcpyHostToDevice, st));
-  hipLaunchKernelGGL(k_init_ic, dim3(grid_full(n)), dim3(BLOCK), 0, st, c->batch_args(), cb, cb + n, dm, dv, du);
+  hipLaunchKernelGGL(k_init_ic, dim3(grid_full(n)), dim3(BLOCK), 0, st, c->batch_args(), s, cb, cb + n, dm, dv, du);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(st));
   (void)hipFree(dm);
@@ -585,7 +642,7 @@ void launch_flux_t(hfv_ctx* c, const Batch& B, unsigned g) {
 }
 
 void launch_flux(hfv_ctx* c, const Batch& B) {
-  unsigned g = grid_list(c->nf);
+  unsigned g = (unsigned)c->nfpos;
   if (c->precision == HFV_FP32_FLUX)
     launch_flux_t<float>(c, B, g);
   else
@@ -596,36 +653,41 @@ void launch_flux(hfv_ctx* c, const Batch& B) {
 void enqueue_step(hfv_ctx* c, bool timed) {
   Batch B = c->batch_args();
   hipStream_t st = c->stream;
-  unsigned gs = (unsigned)std::max(1, (c->nslots + 63) / 64);
-  bool t = timed && c->prof;
+  const unsigned gs = (unsigned)std::max(1, (c->nslots + 63) / 64);
+  const dim3 gc((unsigned)c->ncpos), gf((unsigned)c->nfpos), gb(BLOCK);
+  const bool t = timed && c->prof;
   if (t) prof_next(c);
   hipLaunchKernelGGL(k_begin, dim3(gs), dim3(64), 0, st, B);
+  if (c->any_moving) {
+    if (t) prof_begin(c, KS_DRIFT);
+    hipLaunchKernelGGL(k_drift, gc, gb, 0, st, B);
+    if (t) prof_end(c, KS_DRIFT);
+  }
   if (t) prof_begin(c, KS_PREDICT);
-  hipLaunchKernelGGL(k_mask, dim3(grid_full(c->n)), dim3(BLOCK), 0, st, B, 0);
+  hipLaunchKernelGGL(k_mask, gc, gb, 0, st, B, 0);
   if (t) prof_end(c, KS_PREDICT);
   if (c->any_moving1d) {
     if (t) prof_begin(c, KS_MESH1D);
-    hipLaunchKernelGGL(k_mesh1d_faces, dim3(grid_full(c->nf)), dim3(BLOCK), 0, st, B);
-    hipLaunchKernelGGL(k_mesh1d_cells, dim3(grid_full(c->n)), dim3(BLOCK), 0, st, B);
+    hipLaunchKernelGGL(k_mesh1d_faces, gf, gb, 0, st, B);
+    hipLaunchKernelGGL(k_mesh1d_cells, gc, gb, 0, st, B);
     if (t) prof_end(c, KS_MESH1D);
   }
-  hipLaunchKernelGGL(k_face_list, dim3(grid_list(c->n)), dim3(BLOCK), 0, st, B);
   if (t) prof_begin(c, KS_FLUX);
   launch_flux(c, B);
   if (t) prof_end(c, KS_FLUX);
   if (t) prof_begin(c, KS_APPLY);
-  hipLaunchKernelGGL(k_apply, dim3(grid_list(c->n)), dim3(BLOCK), 0, st, B, 3);
+  hipLaunchKernelGGL(k_apply, gc, gb, 0, st, B, 3);
   if (t) prof_end(c, KS_APPLY);
   if (t) prof_begin(c, KS_GRAD);
-  hipLaunchKernelGGL(k_grad, dim3(grid_list(c->n)), dim3(BLOCK), 0, st, B, 1, 1);
+  hipLaunchKernelGGL(k_grad, gc, gb, 0, st, B, 1, 1);
   if (t) prof_end(c, KS_GRAD);
   if (t) prof_begin(c, KS_WAKEUP);
-  hipLaunchKernelGGL(k_wakeup, dim3(grid_list(c->n)), dim3(BLOCK), 0, st, B);
+  hipLaunchKernelGGL(k_wakeup, gc, gb, 0, st, B);
   if (t) prof_end(c, KS_WAKEUP);
   if (t) prof_begin(c, KS_LIMIT);
-  hipLaunchKernelGGL(k_limit, dim3(grid_list(c->n)), dim3(BLOCK), 0, st, B);
+  hipLaunchKernelGGL(k_limit, gc, gb, 0, st, B);
   if (t) prof_end(c, KS_LIMIT);
-  hipLaunchKernelGGL(k_end, dim3(gs), dim3(64), 0, st, B);
+  hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), gb, 0, st, B);
 }
 
 int launch_step(hfv_ctx* c) {
@@ -819,6 +881,7 @@ int hfv_set_mesh(hfv_ctx* c, const hfv_mesh_arrays* m, const uint8_t* update_mas
   TRY(upload_mesh(c, 0, m, update_mask, !first));
   if (first) {
     c->loaded[0] = 1;
+    TRY(write_jobs(c));
     TRY(reset_slot(c, 0));
   }
   recompute_uniforms(c);
@@ -872,7 +935,7 @@ int hfv_predict(hfv_ctx* c, double dt) {
   HIP_OK(hipStreamSynchronize(c->stream));
   E.dt_pred = dt;
   HIP_OK(hipMemcpyAsync(c->deng, &E, sizeof(SimEngine), hipMemcpyHostToDevice, c->stream));
-  hipLaunchKernelGGL(k_mask, dim3(grid_full(c->n)), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
+  hipLaunchKernelGGL(k_mask, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
   HIP_OK(hipGetLastError());
   return sync_counters(c);
 }
@@ -884,8 +947,7 @@ int hfv_compute_fluxes(hfv_ctx* c, const hfv_timestep_info* info) {
   TRY(begin_phase_step(c, info));
   Batch B = c->batch_args();
   hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, c->stream, B);
-  hipLaunchKernelGGL(k_mask, dim3(grid_full(c->n)), dim3(BLOCK), 0, c->stream, B, 2);
-  hipLaunchKernelGGL(k_face_list, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_mask, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B, 2);
   launch_flux(c, B);
   HIP_OK(hipGetLastError());
   return sync_counters(c);
@@ -914,21 +976,21 @@ int hfv_get_fluxes(hfv_ctx* c, double* out7f) {
 
 int hfv_apply_fluxes(hfv_ctx* c) {
   TRY(require_mesh(c));
-  hipLaunchKernelGGL(k_apply, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
+  hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
   HIP_OK(hipGetLastError());
   return sync_counters(c);
 }
 
 int hfv_conserved_to_primitive(hfv_ctx* c) {
   TRY(require_mesh(c));
-  hipLaunchKernelGGL(k_apply, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, c->batch_args(), 2);
+  hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 2);
   HIP_OK(hipGetLastError());
   return sync_counters(c);
 }
 
 int hfv_gradients(hfv_ctx* c) {
   TRY(require_mesh(c));
-  hipLaunchKernelGGL(k_grad, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, c->batch_args(), 1, 0);
+  hipLaunchKernelGGL(k_grad, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 1, 0);
   HIP_OK(hipGetLastError());
   return sync_counters(c);
 }
@@ -936,13 +998,13 @@ int hfv_gradients(hfv_ctx* c) {
 int hfv_timesteps(hfv_ctx* c, uint64_t* ti_next) {
   TRY(require_mesh(c));
   Batch B = c->batch_args();
-  hipLaunchKernelGGL(k_grad, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, B, 0, 1);
-  hipLaunchKernelGGL(k_wakeup, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, B);
-  hipLaunchKernelGGL(k_limit, dim3(grid_list(c->n)), dim3(BLOCK), 0, c->stream, B);
-  hipLaunchKernelGGL(k_end, dim3(1), dim3(64), 0, c->stream, B);
+  hipLaunchKernelGGL(k_grad, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B, 0, 1);
+  hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_limit, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(BLOCK), 0, c->stream, B);
   HIP_OK(hipGetLastError());
   TRY(sync_counters(c));
-  if (c->heng[0].err) return set_err(c->heng[0].err, "no active cell in this step");
+  if (c->heng[0].err) return set_err(c->heng[0].err, "engine error in this step");
   if (ti_next) *ti_next = c->heng[0].ti_next;
   return HFV_OK;
 }
@@ -957,7 +1019,7 @@ int hfv_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {
   HIP_OK(hipGetLastError());
   TRY(sync_counters(c));
   prof_collect(c);
-  if (c->heng[0].err) return set_err(c->heng[0].err, "no active cell in this step");
+  if (c->heng[0].err) return set_err(c->heng[0].err, "engine error in this step");
   if (ti_next) *ti_next = c->heng[0].ti_next;
   return HFV_OK;
 }
@@ -985,8 +1047,11 @@ int hfv_run(hfv_ctx* c, int64_t max_steps, int64_t* steps_done) {
 int hfv_get_engine_info(hfv_ctx* c, hfv_engine_info* out) {
   TRY(require_mesh(c));
   if (!out) return set_err(HFV_ERR_INVALID, "out is NULL");
-  TRY(sync_counters(c));
+  // readable after a hard error too (error_code then holds it)
+  HIP_OK(hipMemcpyAsync(c->heng, c->deng, sizeof(SimEngine), hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
   fill_engine_info(c->heng[0], out);
+  if (!out->error_code) out->error_code = c->hcnt->err;
   return HFV_OK;
 }
 
@@ -1037,6 +1102,7 @@ int hfv_batch_load(hfv_ctx* c, int s, const hfv_params* params, const hfv_mesh_a
   TRY(upload_mesh(c, s, mesh, nullptr, false));
   TRY(reset_slot(c, s));
   c->loaded[s] = 1;
+  TRY(write_jobs(c));
   TRY(init_ic(c, s, mesh->n_cells, loc, mass, vel, u_int));
   int prev_solver = c->solver;
   bool prev_m = c->any_moving1d;
@@ -1051,8 +1117,7 @@ int hfv_batch_unload(hfv_ctx* c, int s) {
   HIP_OK(hipStreamSynchronize(c->stream));
   c->loaded[s] = 0;
   c->nused[s] = 0;
-  std::vector<int32_t> sim(icap(c, s), -1);
-  HIP_OK(hipMemcpyAsync(c->C.sim + c->cbase[s], sim.data(), sizeof(int32_t) * icap(c, s), hipMemcpyHostToDevice, c->stream));
+  TRY(write_jobs(c));
   HIP_OK(hipMemcpyAsync(c->heng + s, c->deng + s, sizeof(SimEngine), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
   c->heng[s].done = 1;
@@ -1137,6 +1202,7 @@ int hfv_riemann_batch(const hfv_params* params, int64_t nb, const double* WL, co
                       const double* nrm, double* Fo) {
   TRY(check_params(params));
   if (nb <= 0 || !WL || !WR || !vface || !nrm || !Fo) return set_err(HFV_ERR_INVALID, "bad batch arrays");
+  if (nb >= (1ll << 29)) return set_err(HFV_ERR_INVALID, "batch too large for 32-bit indices");
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(HFV_ERR_NO_DEVICE, "no HIP device");
   Cfg cfg = make_cfg(*params);

@@ -7,16 +7,21 @@
 // HBM layout (cells/faces of all simulations concatenated, indices global):
 //   cells, component-major SoA (field[c * N + i]): prim[5], grad[15], gc[3], extrap[5], cons[5],
 //     facc[5] (pending flux accumulator), vol, loc[3], cen[3], v[3], vrel[3], maor, msv, dt, srad,
-//     bin (int8), wake (int8), act (uint8), touched (uint32), sim (int32, simulation of the cell)
+//     tex[5] (time-extrapolation rate of this step's primitives), cs (their sound speed, 0 for vacuum),
+//     bin (int8), wake (int8), act (uint8)
 //   faces, SoA: area, nrm[3], fcen[3], left (int32), right (int32, -1 = wall), shc (uint8 periodic
-//     shift code, 2 bits per axis), rec[7] (dt*F[5], v_max, a_over_r)
+//     shift code, 2 bits per axis), rec[7] (dt*F[5], v_max, a_over_r), cur (uint8: record of this step)
 //   csr: off[N+1] (int32), conn[2F] (int32: face << 1 | side, side 1 = the cell is the face's right)
-//   work lists (rebuilt each step): act_list[N], touch_list[N], face_list[F]
+//   block tables: one {sim, lo, hi, chunk} per thread block of the cell kernels and of the face
+//     kernels.  A block only holds items of one simulation, so configuration and engine loads are
+//     uniform (scalar) and a finished simulation's blocks exit at once.  Blocks are ordered so that
+//     the 8 XCDs (round-robin dispatch) each receive a contiguous range of chunks (L2 locality of the
+//     neighbour gathers).  Per-chunk partial counts replace global atomics; k_end reduces them.
 //
-// Step kernels: k_begin (per simulation) -> k_mask (all cells: drift, active mask, predict, list
-// append) -> [k_mesh1d_*] -> k_face_list (active cells) -> k_flux (active faces) -> k_apply (touched
-// cells) -> k_grad (active cells) -> k_wakeup, k_limit (touched cells) -> k_end (per simulation).
-// When every cell of the batch is active ("dense"), lists are bypassed and indices are natural.
+// Step kernels: k_begin (per simulation) -> [k_drift] -> k_mask (cells: active flag, predict) ->
+// [k_mesh1d_*] -> k_flux (faces: gate + flux) -> k_apply (cells: accumulate in CSR order, apply, c2p)
+// -> k_grad (active cells: gradients, limiter, timestep) -> k_wakeup, k_limit (cells) -> k_end (per
+// simulation: partial sums, next sync point).
 #pragma once
 #include "hfv_physics.hpp"
 
@@ -25,17 +30,16 @@ namespace hfv {
 struct Cells {
   int64_t n;
   double *prim, *grad, *gc, *extrap, *cons, *facc, *vol, *loc, *cen, *v, *vrel, *maor, *msv, *dt, *srad;
+  double *tex, *cs;
   int8_t *bin, *wake;
   uint8_t* act;
-  uint32_t* touched;
-  int32_t* sim;  // nullptr for a batch of one
 };
 
 struct Faces {
   int64_t nf;
   double *area, *nrm, *fcen, *rec;
   int32_t *left, *right, *off, *conn;
-  uint8_t* shc;
+  uint8_t *shc, *cur;
 };
 
 struct Cfg {
@@ -51,8 +55,14 @@ struct Cfg {
 struct SimDesc {
   Cfg cfg;
   int64_t cell_base, n, face_base, nf;
+  int32_t cchunk0, cchunk1, fchunk0, fchunk1;  // chunk ranges of the cell / face block tables
   int moving1d;  // 1D moving mesh rebuilt on the device
   int periodic;
+};
+
+// Thread block -> work: simulation (-1: nothing), items [lo, hi), chunk (index of its partials).
+struct BlockJob {
+  int32_t sim, lo, hi, chunk;
 };
 
 #define HFV_MAX_SUBDRIFTS 8
@@ -77,17 +87,18 @@ struct SimEngine {
   int pad;
 };
 
-// Batch-global counters.
+// Batch-global counters (soft events; the first hard error of any simulation).
 struct DevCounters {
   int err;
-  int dense;
+  int pad;
   unsigned long long check_physical, flux_reset, grad_fallback, brent;
-  unsigned long long act_count, touch_count, face_count;
-  unsigned long long epoch;
 };
 
-struct Work {
-  int32_t *act_list, *touch_list, *face_list;
+// Per-chunk partials written by the dense kernels (one entry per block-table chunk).
+struct Partials {
+  uint32_t* act;    // active cells of a cell chunk (k_mask)
+  uint32_t* face;   // computed faces of a face chunk (k_flux)
+  uint64_t* ti;     // minimum new dti of a cell chunk's active cells (k_grad)
 };
 
 struct Batch {
@@ -96,12 +107,15 @@ struct Batch {
   const SimDesc* sims;
   SimEngine* eng;
   int nsims;
-  Work W;
+  const BlockJob* cjobs;  // cell kernels: grid = number of entries
+  const BlockJob* fjobs;  // face kernels
+  Partials P;
   DevCounters* cnt;
 };
 
 #define HFV_NUM_TIME_BINS 56
 #define HFV_MAX_NR_TIMESTEPS (1ull << 57)
+#define HFV_BLOCK 256
 
 // ------------------------------------------------------------------ timeline (src/timeline.rs)
 __device__ __host__ __forceinline__ uint64_t get_integer_timestep(int bin) {
@@ -158,8 +172,14 @@ __device__ __host__ __forceinline__ uint64_t make_integer_timestep(double dt, in
 }
 
 // ------------------------------------------------------------------ helpers
+// Cell and face indices are below 2^29 (checked at allocation), so gathers use a 32-bit byte offset
+// from a uniform base: one VGPR per address instead of a 64-bit pair per field.
+__device__ __forceinline__ double ldo(const double* a, uint32_t i) {
+  return *(const double*)((const char*)a + (uint64_t)(i * 8u));
+}
 __device__ __forceinline__ V3<double> ld3(const double* a, int64_t n, int64_t i) {
-  return mk3(a[i], a[n + i], a[2 * n + i]);
+  uint32_t u = (uint32_t)i;
+  return mk3(ldo(a, u), ldo(a + n, u), ldo(a + 2 * n, u));
 }
 __device__ __forceinline__ void st3(double* a, int64_t n, int64_t i, V3<double> v) {
   a[i] = v.x;
@@ -168,8 +188,9 @@ __device__ __forceinline__ void st3(double* a, int64_t n, int64_t i, V3<double> 
 }
 __device__ __forceinline__ St<double> ld5(const double* a, int64_t n, int64_t i) {
   St<double> s;
+  uint32_t u = (uint32_t)i;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) s.c[k] = a[k * n + i];
+  for (int k = 0; k < 5; ++k) s.c[k] = ldo(a + k * n, u);
   return s;
 }
 __device__ __forceinline__ void st5(double* a, int64_t n, int64_t i, const St<double>& s) {
@@ -178,8 +199,10 @@ __device__ __forceinline__ void st5(double* a, int64_t n, int64_t i, const St<do
 }
 __device__ __forceinline__ Gr<double> ldg(const double* a, int64_t n, int64_t i) {
   Gr<double> g;
+  uint32_t u = (uint32_t)i;
 #pragma unroll
-  for (int k = 0; k < 5; ++k) g.g[k] = mk3(a[(3 * k) * n + i], a[(3 * k + 1) * n + i], a[(3 * k + 2) * n + i]);
+  for (int k = 0; k < 5; ++k)
+    g.g[k] = mk3(ldo(a + (3 * k) * n, u), ldo(a + (3 * k + 1) * n, u), ldo(a + (3 * k + 2) * n, u));
   return g;
 }
 __device__ __forceinline__ void stg(double* a, int64_t n, int64_t i, const Gr<double>& g) {
@@ -202,6 +225,11 @@ __device__ __forceinline__ V3<double> decode_shift(uint8_t code, const double* b
   return mk3(s[0], s[1], s[2]);
 }
 
+// Keeps the compiler from moving memory operations across this point: the flux and gradient kernels
+// issue their gathers stage by stage, so that a stage's loads are not all in flight (and holding
+// registers) at once.
+#define HFV_STAGE() asm volatile("" ::: "memory")
+
 // Hard errors are recorded for the batch and for the simulation that raised them; a batch keeps
 // stepping its other simulations (the failed one stops at its next k_begin).
 __device__ __forceinline__ void flush_events(const Events& ev, DevCounters* cnt, SimEngine* eng = nullptr, int s = -1) {
@@ -215,60 +243,52 @@ __device__ __forceinline__ void flush_events(const Events& ev, DevCounters* cnt,
   if (ev.flux_reset) atomicAdd(&cnt->flux_reset, (unsigned long long)ev.flux_reset);
 }
 
-__device__ __forceinline__ unsigned long long lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
-
-// Wavefront-aggregated append: one atomic per wave; returns the slot (or -1 when !want).
-__device__ __forceinline__ long long wave_append(bool want, unsigned long long* counter) {
-  unsigned long long m = __ballot(want);
-  if (m == 0) return -1;
-  int leader = __ffsll((long long)m) - 1;
-  unsigned long long base = 0;
-  if (__lane_id() == leader) base = atomicAdd(counter, (unsigned long long)__popcll(m));
-  base = __shfl(base, leader);
-  return want ? (long long)(base + __popcll(m & lanemask_lt())) : -1;
+// Block-wide reductions (256 threads = 4 waves); the result is valid in thread 0.
+__device__ __forceinline__ uint32_t block_sum_u32(uint32_t v) {
+  __shared__ uint32_t red_u32[HFV_BLOCK / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if (__lane_id() == 0) red_u32[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint32_t t = 0;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red_u32[w];
+  return t;
 }
-
-// Per-simulation wave-aggregated add (waterfall over the simulations present in the wave).
-__device__ __forceinline__ void sim_add(bool valid, int s, SimEngine* eng, int which) {
-  bool todo = valid;
-  while (true) {
-    unsigned long long m = __ballot(todo);
-    if (m == 0) break;
-    int leader = __ffsll((long long)m) - 1;
-    int su = __shfl(s, leader);
-    bool mine = todo && s == su;
-    unsigned long long tot = (unsigned long long)__popcll(__ballot(mine));
-    if (__lane_id() == leader) {
-      if (which == 0)
-        atomicAdd(&eng[su].n_active, tot);
-      else
-        atomicAdd(&eng[su].n_face_active, tot);
-    }
-    if (mine) todo = false;
+__device__ __forceinline__ uint64_t block_min_u64(uint64_t v) {
+  __shared__ uint64_t red_u64[HFV_BLOCK / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    uint64_t o = __shfl_xor(v, off);
+    v = o < v ? o : v;
   }
+  if (__lane_id() == 0) red_u64[threadIdx.x >> 6] = v;
+  __syncthreads();
+  uint64_t t = ~0ull;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t = red_u64[w] < t ? red_u64[w] : t;
+  return t;
 }
-
-// XCD-aware block order: the 8 XCDs receive blocks round-robin; give each XCD a contiguous share.
-__device__ __forceinline__ int64_t xcd_block() {
-  int64_t b = blockIdx.x, g = gridDim.x;
-  if ((g & 7) == 0) return (b & 7) * (g >> 3) + (b >> 3);
-  return b;
+__device__ __forceinline__ int block_min_i32(int v) {
+  __shared__ int red_i32[HFV_BLOCK / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    int o = __shfl_xor(v, off);
+    v = o < v ? o : v;
+  }
+  if (__lane_id() == 0) red_i32[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int t = 0x7fffffff;
+  if (threadIdx.x == 0)
+    for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t = red_i32[w] < t ? red_i32[w] : t;
+  return t;
 }
-
-__device__ __forceinline__ int sim_of(const Cells& C, int64_t i) { return C.sim ? C.sim[i] : 0; }
 
 // ------------------------------------------------------------------ engine (per simulation)
 // Engine::run loop head (engine.rs:152-185): the pending dump sync points are drifted through, then
-// the step at ti_next.  Also resets the batch's list counters.
+// the step at ti_next.
 __global__ void k_begin(Batch B) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s == 0) {
-    B.cnt->act_count = 0;
-    B.cnt->touch_count = 0;
-    B.cnt->face_count = 0;
-    B.cnt->dense = 0;
-    B.cnt->epoch += 1;
-  }
   if (s >= B.nsims) return;
   SimEngine& E = B.eng[s];
   E.stepping = 0;
@@ -320,12 +340,34 @@ __global__ void k_begin(Batch B) {
   E.mab = get_max_active_bin(E.ti_current);
 }
 
-// Engine::step / queue_step / update_ti_next (engine.rs:194-268).
-__global__ void k_end(Batch B) {
-  int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= B.nsims) return;
+// Engine::step / queue_step / update_ti_next (engine.rs:194-268).  One block per simulation: sums the
+// chunk partials of this step, then one thread moves the engine to its next sync point.
+__global__ void __launch_bounds__(HFV_BLOCK) k_end(Batch B) {
+  const int s = blockIdx.x;
   SimEngine& E = B.eng[s];
   if (!E.stepping) return;
+  const SimDesc& D = B.sims[s];
+  uint32_t na = 0, nfc = 0;
+  uint64_t tmin = ~0ull;
+  for (int k = D.cchunk0 + (int)threadIdx.x; k < D.cchunk1; k += blockDim.x) {
+    na += B.P.act[k];
+    uint64_t t = B.P.ti[k];
+    tmin = t < tmin ? t : tmin;
+  }
+  for (int k = D.fchunk0 + (int)threadIdx.x; k < D.fchunk1; k += blockDim.x) nfc += B.P.face[k];
+  na = block_sum_u32(na);
+  __syncthreads();
+  nfc = block_sum_u32(nfc);
+  __syncthreads();
+  tmin = block_min_u64(tmin);
+  if (threadIdx.x != 0) return;
+  E.n_active = na;
+  E.n_face_active = nfc;
+  E.ti_min = tmin;
+  if (E.err) {  // the reference panics inside this step: it does not complete
+    E.done = 1;
+    return;
+  }
   if (E.n_active == 0 || E.ti_min == ~0ull) {  // the reference panics: "At least one particle must be active"
     E.err = E_UNSUPPORTED;
     atomicCAS(&B.cnt->err, 0, E_UNSUPPORTED);
@@ -419,56 +461,62 @@ __device__ void drift_cell(const Cells& C, const Cfg& cfg, int64_t i, double dt,
   }
 }
 
+// Engine::run's drift through the step's sync points (moving meshes only).
+__global__ void __launch_bounds__(HFV_BLOCK) k_drift(Batch B) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const SimEngine& E = B.eng[J.sim];
+  const Cfg& cfg = B.sims[J.sim].cfg;
+  if (!E.stepping || E.n_sub == 0 || cfg.motion == MOT_FIXED) return;
+  const int64_t i = (int64_t)J.lo + threadIdx.x;
+  if (i >= J.hi) return;
+  Events ev;
+  for (int k = 0; k < E.n_sub; ++k) drift_cell(B.C, cfg, i, E.sub_dt[k], ev);
+  flush_events(ev, B.cnt, B.eng, J.sim);
+}
+
 // ------------------------------------------------------------------ k_mask
-// mode 0: step (drift + mask + predict + lists); 1: predict only (dt from eng[0].dt_pred);
-// 2: mask + lists only.  Active mask: optimal_order.rs:17; predict: euler_eqns_fvs.rs:20-24,
-// part.rs:123-149.
-__global__ void __launch_bounds__(256) k_mask(Batch B, int mode) {
+// Active mask (optimal_order.rs:17) and predict (euler_eqns_fvs.rs:20-24, part.rs:121-149) of every
+// cell; also the flux phase's per-cell inputs tex (time-extrapolation rate) and cs (sound speed).
+// mode 0: step; 1: predict only (dt from the engine's dt_pred); 2: mask only.
+__global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const int s = J.sim;
+  const SimEngine& E = B.eng[s];
+  if (mode != 1 && !E.stepping) return;
+  const Cfg& cfg = B.sims[s].cfg;
   const Cells& C = B.C;
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  bool valid = i < C.n;
-  int s = valid ? sim_of(C, i) : 0;
-  if (valid && s < 0) {  // empty slot capacity
-    if (mode != 1) C.act[i] = 0;
-    valid = false;
-    s = 0;
-  }
-  bool act = false, touch = false;
+  const int64_t n = C.n;
+  const int64_t i = (int64_t)J.lo + threadIdx.x;
+  const bool valid = i < J.hi;
+  bool act = false;
   Events ev;
   if (valid) {
-    const SimEngine& E = B.eng[s];
-    const Cfg& cfg = B.sims[s].cfg;
-    const int64_t n = C.n;
-    if (!E.stepping && mode != 1) {
-      C.act[i] = 0;
-      C.touched[i] = 0;
-    } else {
-      if (mode == 0 && E.n_sub > 0 && cfg.motion != MOT_FIXED)
-        for (int k = 0; k < E.n_sub; ++k) drift_cell(C, cfg, i, E.sub_dt[k], ev);
-      if (mode != 1) {
-        act = C.bin[i] <= E.mab;
-        C.act[i] = act;
-        touch = act || cfg.sync_all;
-        C.touched[i] = touch ? 1u : 0u;  // every flag rewritten each step
-      }
-      if (mode != 2) {
-        St<double> prim = ld5(C.prim, n, i);
-        Gr<double> gr = ldg(C.grad, n, i);
-        V3<double> vrel = ld3(C.vrel, n, i);
-        St<double> ex = ld5(C.extrap, n, i);
-        ex = ex + time_extrapolations(prim, gr, vrel, E.dt_pred, cfg.gd);
-        st5(C.extrap, n, i, ex);
-        check_physical(prim, ev);
-        if (ev.check_physical) st5(C.prim, n, i, prim);
-        // volume += dvdt * dt clamped to [V/2, 2V]: dvdt is never set on the path (part.rs:130-131)
-      }
+    if (mode != 1) {
+      act = C.bin[i] <= E.mab;
+      C.act[i] = act;
     }
+    St<double> prim = ld5(C.prim, n, i);
+    Gr<double> gr = ldg(C.grad, n, i);
+    V3<double> vrel = ld3(C.vrel, n, i);
+    St<double> rate = vacuum<double>();
+    if (prim.c[0] > 0.) rate = tex_rate(prim, gr, vrel, cfg.gd);
+    if (mode != 2) {
+      // extrapolations += time_extrapolations(dt) (its vacuum branch adds +0)
+      St<double> te = prim.c[0] > 0. ? (-E.dt_pred) * rate : vacuum<double>();
+      st5(C.extrap, n, i, ld5(C.extrap, n, i) + te);
+      check_physical(prim, ev);
+      if (ev.check_physical) st5(C.prim, n, i, prim);
+      // volume += dvdt * dt clamped to [V/2, 2V]: dvdt is never set on the path (part.rs:130-131)
+    }
+    st5(C.tex, n, i, rate);
+    C.cs[i] = prim.c[0] > 0. ? sound_speed(cfg.gd, prim.c[4], 1. / prim.c[0]) : 0.;
   }
-  long long pa = wave_append(act, &B.cnt->act_count);
-  if (pa >= 0) B.W.act_list[pa] = (int32_t)i;
-  long long pt = wave_append(touch, &B.cnt->touch_count);
-  if (pt >= 0) B.W.touch_list[pt] = (int32_t)i;
-  sim_add(valid && act, s, B.eng, 0);
+  if (mode != 1) {
+    uint32_t cnt = block_sum_u32(act ? 1u : 0u);
+    if (threadIdx.x == 0) B.P.act[J.chunk] = cnt;
+  }
   if (valid) flush_events(ev, B.cnt, B.eng, s);
 }
 
@@ -476,15 +524,14 @@ __global__ void __launch_bounds__(256) k_mask(Batch B, int mode) {
 // Space::volume_calculation for 1D (space.rs:236-263; semantics pinned by test_init_1d,
 // space.rs:1453-1491): generators stay sorted, so the face topology is fixed; face k of a simulation
 // sits between its cells k-1 and k; walls are faces 0 and n (non periodic); periodic meshes wrap at 0.
-__global__ void k_mesh1d_faces(Batch B) {
-  int64_t f = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (f >= B.F.nf) return;
-  int32_t lf = B.F.left[f];
-  if (lf < 0) return;  // spare face capacity
-  int s = sim_of(B.C, lf);
-  if (s < 0) return;
+__global__ void __launch_bounds__(HFV_BLOCK) k_mesh1d_faces(Batch B) {
+  const BlockJob J = B.fjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const int s = J.sim;
   const SimDesc& D = B.sims[s];
   if (!D.moving1d || !B.eng[s].stepping) return;
+  const int64_t f = (int64_t)J.lo + threadIdx.x;
+  if (f >= J.hi) return;
   int64_t k = f - D.face_base, n = D.n, cb = D.cell_base;
   double x;
   if (k >= 1 && k < n) {
@@ -501,12 +548,13 @@ __global__ void k_mesh1d_faces(Batch B) {
   }
   B.F.fcen[f] = x;
 }
-__global__ void k_mesh1d_cells(Batch B) {
-  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= B.C.n || !B.C.act[i]) return;
-  int s = sim_of(B.C, i);
-  const SimDesc& D = B.sims[s];
-  if (!D.moving1d) return;
+__global__ void __launch_bounds__(HFV_BLOCK) k_mesh1d_cells(Batch B) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const SimDesc& D = B.sims[J.sim];
+  if (!D.moving1d || !B.eng[J.sim].stepping) return;
+  const int64_t i = (int64_t)J.lo + threadIdx.x;
+  if (i >= J.hi || !B.C.act[i]) return;
   const int64_t N = B.C.n;
   int64_t k = i - D.cell_base, n = D.n, fb = D.face_base;
   double xl, xr;
@@ -525,41 +573,6 @@ __global__ void k_mesh1d_cells(Batch B) {
   B.C.srad[i] = 1.5 * V;
 }
 
-__device__ __forceinline__ bool is_dense(const Batch& B) { return B.cnt->act_count == (unsigned long long)B.C.n; }
-
-// ------------------------------------------------------------------ k_face_list
-// Every face adjacent to an active cell, once: owned by its lower-index active cell.  Inactive
-// neighbours of active cells join the touched list (they accumulate fluxes, hydrodynamics.rs:13-39).
-__global__ void __launch_bounds__(256) k_face_list(Batch B) {
-  if (is_dense(B)) return;
-  const int64_t count = (int64_t)B.cnt->act_count;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t base = xcd_block() * blockDim.x; base < count; base += stride) {
-    int64_t k = base + threadIdx.x;
-    bool valid = k < count;
-    int32_t i = valid ? B.W.act_list[k] : 0;
-    int32_t k0 = valid ? B.F.off[i] : 0, k1 = valid ? B.F.off[i + 1] : 0;
-    for (int32_t e = k0;; ++e) {
-      bool more = e < k1;
-      if (__ballot(more) == 0) break;
-      bool own = false, newt = false;
-      int32_t f = 0, other = -1;
-      if (more) {
-        int32_t c = B.F.conn[e];
-        f = c >> 1;
-        other = (c & 1) == 0 ? B.F.right[f] : B.F.left[f];
-        bool oa = other >= 0 && B.C.act[other];
-        own = other < 0 || !oa || i < other;
-        if (other >= 0 && !oa) newt = atomicExch(&B.C.touched[other], 1u) == 0u;
-      }
-      long long pf = wave_append(own, &B.cnt->face_count);
-      if (pf >= 0) B.W.face_list[pf] = f;
-      long long pt = wave_append(newt, &B.cnt->touch_count);
-      if (pt >= 0) B.W.touch_list[pt] = other;
-    }
-  }
-}
-
 // ------------------------------------------------------------------ k_flux
 // FiniteVolumeSolver::compute_fluxes: gate (euler_eqns_fvs.rs:26-48) + flux_exchange (flux.rs:23-100)
 // + flux_exchange_boundary (flux.rs:102-195).  T: arithmetic of reconstruction and solve; SOLVER >= 0
@@ -575,103 +588,116 @@ __device__ __forceinline__ const Gas<float>& gas_for<float>(const Cfg& c) {
   return c.gf;
 }
 
+// One side's reconstructed state at the face before limiting (flux.rs:52-73):
+// ((W + grad . dx) + extrapolations) + time_extrapolations(-dt/2), in T.
+template <class T>
+__device__ __forceinline__ St<T> face_side(const Cells& C, int64_t n, int32_t c, const St<T>& pT, V3<T> dxT, T dte) {
+  Gr<double> g = ldg(C.grad, n, c);
+  St<double> ex = ld5(C.extrap, n, c);
+  St<double> rate = ld5(C.tex, n, c);
+  St<T> out;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) out.c[q] = (pT.c[q] + dot(cvt3<T>(g.g[q]), dxT)) + (T)ex.c[q];
+  if (pT.c[0] > T(0)) out = out + (-dte) * cvtst<T>(rate);
+  else out = out + vacuum<T>();
+  return out;
+}
+
 template <class T, int SOLVER>
-__device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_t f, double* outv, Events& ev) {
+__device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_t f, int32_t l, int32_t r,
+                                          double* outv, Events& ev) {
   const Cells& C = B.C;
   const Faces& F = B.F;
   const int64_t n = C.n;
   const int64_t nf = F.nf;
-  int32_t l = F.left[f];
-  int32_t r = F.right[f];
-  bool la = C.act[l];
   const Gas<T>& g = gas_for<T>(cfg);
   SolverCfg sc = cfg.solver;
   if (SOLVER >= 0) sc.kind = SOLVER;
 #pragma unroll
   for (int k = 0; k < 7; ++k) outv[k] = 0.;
+  const bool la = C.act[l];
+  const double dtl = C.dt[l];
   if (r >= 0) {
-    bool ra = C.act[r];
-    double dtl = C.dt[l], dtr = C.dt[r];
+    // ---- gate
+    const bool ra = C.act[r];
+    const double dtr = C.dt[r];
     if ((!la && !ra) || (dtr < dtl && !ra) || (dtl < dtr && !la)) return false;
-    double dt = hmin(dtl, dtr);
-    V3<double> fc = ld3(F.fcen, nf, f);
-    V3<double> nrm = ld3(F.nrm, nf, f);
-    double area = F.area[f];
-    V3<double> sh = decode_shift(F.shc[f], cfg.box);
-    V3<double> gcl = ld3(C.gc, n, l), gcr = ld3(C.gc, n, r);
-    V3<double> dx_left = fc - gcl;
-    V3<double> dx_right = (fc - gcr) - sh;
-    double dx_gradients = len(dx_left - dx_right);
-    V3<double> cl = ld3(C.cen, n, l), cr = ld3(C.cen, n, r);
-    V3<double> dx_centroid = (cr + sh) - cl;
-    V3<double> locl = ld3(C.loc, n, l), locr = ld3(C.loc, n, r);
-    V3<double> dx = (locr + sh) - locl;
-    St<double> pl = ld5(C.prim, n, l), pr = ld5(C.prim, n, r);
+    const double dt = hmin(dtl, dtr);
+    HFV_STAGE();
+    // ---- geometry and the two reconstructed sides
+    const V3<double> fc = ld3(F.fcen, nf, f);
+    const V3<double> sh = decode_shift(F.shc[f], cfg.box);
+    const T dte = (T)(-0.5 * dt);
+    const V3<double> dx_left = fc - ld3(C.gc, n, l);
+    const St<T> plT = cvtst<T>(ld5(C.prim, n, l));
+    const St<T> left_dash = face_side<T>(C, n, l, plT, cvt3<T>(dx_left), dte);
+    HFV_STAGE();
+    const V3<double> dx_right = (fc - ld3(C.gc, n, r)) - sh;
+    const St<T> prT = cvtst<T>(ld5(C.prim, n, r));
+    const St<T> right_dash = face_side<T>(C, n, r, prT, cvt3<T>(dx_right), dte);
+    HFV_STAGE();
+    // ---- pairwise limiter (gradients.rs:105-152)
+    const T dxg = (T)len(dx_left - dx_right);
+    const St<T> prim_left = pairwise_limiter(plT, prT, left_dash, len(cvt3<T>(dx_left)) / dxg, ev);
+    const St<T> prim_right = pairwise_limiter(prT, plT, right_dash, len(cvt3<T>(dx_right)) / dxg, ev);
+    HFV_STAGE();
+    // ---- signal speed (flux.rs:43-50), face velocity (flux.rs:75-78), a_over_r: FP64
     double v_max = 0.;
-    if (pl.c[0] > 0.) v_max += sound_speed(cfg.gd, pl.c[4], 1. / pl.c[0]);
-    if (pr.c[0] > 0.) v_max += sound_speed(cfg.gd, pr.c[4], 1. / pr.c[0]);
-    v_max -= hmin(dot(vel(pr) - vel(pl), dx_centroid), 0.);
-    // reconstruction (flux.rs:52-73) in T
-    T dt_extrapolate = (T)(-0.5 * dt);
-    Gr<double> gl = ldg(C.grad, n, l), grr = ldg(C.grad, n, r);
-    St<double> exl = ld5(C.extrap, n, l), exr = ld5(C.extrap, n, r);
-    V3<double> vrl = ld3(C.vrel, n, l), vrr = ld3(C.vrel, n, r);
-    St<T> plT = cvtst<T>(pl), prT = cvtst<T>(pr);
-    Gr<T> glT, grT;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      glT.g[q] = cvt3<T>(gl.g[q]);
-      grT.g[q] = cvt3<T>(grr.g[q]);
+    {
+      const V3<double> dx_centroid = (ld3(C.cen, n, r) + sh) - ld3(C.cen, n, l);
+      const double rho_l = C.prim[l], rho_r = C.prim[r];
+      if (rho_l > 0.) v_max += C.cs[l];
+      if (rho_r > 0.) v_max += C.cs[r];
+      const V3<double> vpl = mk3(C.prim[n + l], C.prim[2 * n + l], C.prim[3 * n + l]);
+      const V3<double> vpr = mk3(C.prim[n + r], C.prim[2 * n + r], C.prim[3 * n + r]);
+      v_max -= hmin(dot(vpr - vpl, dx_centroid), 0.);
     }
-    V3<T> dxlT = cvt3<T>(dx_left), dxrT = cvt3<T>(dx_right);
-    St<T> left_dash = ((plT + gdot(glT, dxlT)) + cvtst<T>(exl)) + time_extrapolations(plT, glT, cvt3<T>(vrl), dt_extrapolate, g);
-    St<T> right_dash = ((prT + gdot(grT, dxrT)) + cvtst<T>(exr)) + time_extrapolations(prT, grT, cvt3<T>(vrr), dt_extrapolate, g);
-    T dxg = (T)dx_gradients;
-    St<T> prim_left = pairwise_limiter(plT, prT, left_dash, len(dxlT) / dxg, ev);
-    St<T> prim_right = pairwise_limiter(prT, plT, right_dash, len(dxrT) / dxg, ev);
-    // face velocity (flux.rs:75-78), FP64
-    V3<double> vl = ld3(C.v, n, l), vr = ld3(C.v, n, r);
-    V3<double> midpoint = 0.5 * ((locl + locr) + sh);
-    double fac = dot(vr - vl, fc - midpoint) / len2(dx);
-    V3<double> v_face = (0.5 * (vl + vr)) - (fac * dx);
-    V3<T> vfT = cvt3<T>(v_face);
+    HFV_STAGE();
+    V3<T> vfT;
+    double len_dx;
+    {
+      const V3<double> locl = ld3(C.loc, n, l), locr = ld3(C.loc, n, r);
+      const V3<double> dx = (locr + sh) - locl;
+      const V3<double> vl = ld3(C.v, n, l), vr = ld3(C.v, n, r);
+      const V3<double> midpoint = 0.5 * ((locl + locr) + sh);
+      const double fac = dot(vr - vl, fc - midpoint) / len2(dx);
+      vfT = cvt3<T>((0.5 * (vl + vr)) - (fac * dx));
+      len_dx = len(dx);
+    }
+    HFV_STAGE();
+    // ---- Riemann problem in the face frame
+    const V3<double> nrm = ld3(F.nrm, nf, f);
+    const double area = F.area[f];
     St<T> Fl = riemann_flux(sc, boost(prim_left, -vfT), boost(prim_right, -vfT), vfT, cvt3<T>(nrm), g, cfg.gd, ev);
-    St<double> fluxes = area * cvtst<double>(Fl);
-    St<double> rec = dt * fluxes;
+    St<double> rec = dt * (area * cvtst<double>(Fl));
 #pragma unroll
     for (int k = 0; k < 5; ++k) outv[k] = rec.c[k];
     outv[5] = v_max;
-    outv[6] = area / len(dx);
+    outv[6] = area / len_dx;
     return true;
   }
   if (!la) return false;
-  V3<double> fc = ld3(F.fcen, nf, f);
-  V3<double> nrm = ld3(F.nrm, nf, f);
-  double area = F.area[f];
-  V3<double> loc = ld3(C.loc, n, l), cen = ld3(C.cen, n, l), gc = ld3(C.gc, n, l);
+  const V3<double> fc = ld3(F.fcen, nf, f);
+  const V3<double> nrm = ld3(F.nrm, nf, f);
+  const double area = F.area[f];
+  const V3<double> loc = ld3(C.loc, n, l), cen = ld3(C.cen, n, l), gc = ld3(C.gc, n, l);
   // Particle::reflect (part.rs:374-383)
-  V3<double> loc_r = loc + ((2. * dot(fc - loc, nrm)) * nrm);
-  V3<double> cen_r = cen + ((2. * dot(fc - cen, nrm)) * nrm);
-  V3<double> gc_r = gc + ((2. * dot(fc - gc, nrm)) * nrm);
-  V3<double> dx_face = fc - gc;
-  double dx_gradients = len(gc_r - gc);
-  V3<double> dx_centroid = cen_r - cen;
-  V3<double> dx = loc_r - loc;
-  St<double> p = ld5(C.prim, n, l);
-  double dtp = C.dt[l];
+  const V3<double> loc_r = loc + ((2. * dot(fc - loc, nrm)) * nrm);
+  const V3<double> cen_r = cen + ((2. * dot(fc - cen, nrm)) * nrm);
+  const V3<double> gc_r = gc + ((2. * dot(fc - gc, nrm)) * nrm);
+  const V3<double> dx_face = fc - gc;
+  const double dx_gradients = len(gc_r - gc);
+  const V3<double> dx_centroid = cen_r - cen;
+  const V3<double> dx = loc_r - loc;
+  const St<double> p = ld5(C.prim, n, l);
+  const double dtp = dtl;
   double v_max = 0.;
-  if (p.c[0] > 0.) v_max += sound_speed(cfg.gd, p.c[4], 1. / p.c[0]);
-  Gr<double> gp = ldg(C.grad, n, l);
-  St<double> ex = ld5(C.extrap, n, l);
-  V3<double> vrel = ld3(C.vrel, n, l);
-  St<T> pT = cvtst<T>(p);
-  Gr<T> gT;
-#pragma unroll
-  for (int q = 0; q < 5; ++q) gT.g[q] = cvt3<T>(gp.g[q]);
-  V3<T> dxfT = cvt3<T>(dx_face);
-  V3<T> nT = cvt3<T>(nrm);
-  St<T> dash = ((pT + gdot(gT, dxfT)) + cvtst<T>(ex)) + time_extrapolations(pT, gT, cvt3<T>(vrel), (T)(-0.5 * dtp), g);
-  T dx_fac = len(dxfT) / (T)dx_gradients;
+  if (p.c[0] > 0.) v_max += C.cs[l];
+  const St<T> pT = cvtst<T>(p);
+  const V3<T> dxfT = cvt3<T>(dx_face);
+  const V3<T> nT = cvt3<T>(nrm);
+  St<T> dash = face_side<T>(C, n, l, pT, dxfT, (T)(-0.5 * dtp));
+  const T dx_fac = len(dxfT) / (T)dx_gradients;
   St<T> bstate;
   if (cfg.boundary == BND_REFLECTIVE) {
     St<T> rp = reflect(pT, nT);
@@ -701,44 +727,39 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
   return true;
 }
 
+// Faces with an active cell get their record (zero when gated off) and cur = 1; the others cur = 0.
 template <class T, int SOLVER>
-__global__ void __launch_bounds__(256) k_flux(Batch B) {
-  const bool dense = is_dense(B);
-  const int64_t count = dense ? B.F.nf : (int64_t)B.cnt->face_count;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+__global__ void __launch_bounds__(HFV_BLOCK) k_flux(Batch B) {
+  const BlockJob J = B.fjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const int s = J.sim;
+  if (!B.eng[s].stepping) return;
+  const Cfg& cfg = B.sims[s].cfg;
   const int64_t nf = B.F.nf;
-  for (int64_t base = xcd_block() * blockDim.x; base < count; base += stride) {
-    int64_t k = base + threadIdx.x;
-    bool valid = k < count;
-    int64_t f = valid ? (dense ? k : (int64_t)B.W.face_list[k]) : 0;
-    int32_t lf = valid ? B.F.left[f] : 0;
-    if (lf < 0) valid = false;  // spare face capacity (dense mode walks every face slot)
-    int s = (valid && B.C.sim) ? B.C.sim[lf] : 0;
-    Events ev;
-    bool computed = false;
-    bool todo = valid;
-    while (true) {  // waterfall over the simulations present in the wave: uniform config loads
-      unsigned long long m = __ballot(todo);
-      if (m == 0) break;
-      int su = __builtin_amdgcn_readfirstlane(__shfl(s, __ffsll((long long)m) - 1));
-      if (todo && s == su) {
-        double outv[7];
-        computed = face_flux<T, SOLVER>(B, B.sims[su].cfg, f, outv, ev);
+  const int64_t f = (int64_t)J.lo + threadIdx.x;
+  bool computed = false;
+  Events ev;
+  if (f < J.hi) {
+    const int32_t l = B.F.left[f], r = B.F.right[f];
+    const bool cur = B.C.act[l] || (r >= 0 && B.C.act[r]);
+    B.F.cur[f] = cur ? 1 : 0;
+    if (cur) {
+      double outv[7];
+      computed = face_flux<T, SOLVER>(B, cfg, f, l, r, outv, ev);
 #pragma unroll
-        for (int q = 0; q < 7; ++q) B.F.rec[q * nf + f] = outv[q];
-        todo = false;
-      }
+      for (int q = 0; q < 7; ++q) B.F.rec[q * nf + f] = outv[q];
     }
-    sim_add(valid && computed, s, B.eng, 1);
-    if (valid) flush_events(ev, B.cnt, B.eng, s);
   }
+  uint32_t cnt = block_sum_u32(computed ? 1u : 0u);
+  if (threadIdx.x == 0) B.P.face[J.chunk] = cnt;
+  flush_events(ev, B.cnt, B.eng, s);
 }
 
 // ------------------------------------------------------------------ k_apply
 // runner apply_fluxes (hydrodynamics.rs:13-39) + kick2 (:51-57) + convert_conserved_to_primitive
-// (euler_eqns_fvs.rs:63-69, part.rs:219-261).  One thread per touched cell walks its faces in CSR order,
-// so the FP64 accumulation order is the reference's and needs no atomics.  A face record is current
-// when one of its cells is active (inactive pairs are gated off and contribute zero).
+// (euler_eqns_fvs.rs:63-69, part.rs:219-261).  One thread per cell walks its faces in CSR order, so the
+// FP64 accumulation order is the reference's and needs no atomics.  Faces without an active cell are
+// gated off in the reference (zero records); they are skipped here.
 // mode bit 1: accumulate + apply + kick2; mode bit 2: conserved -> primitive.
 __device__ __forceinline__ St<double> c2p(const St<double>& c, double vol, const Gas<double>& g, Events& ev) {
   St<double> prim;
@@ -765,77 +786,81 @@ __device__ __forceinline__ St<double> c2p(const St<double>& c, double vol, const
   return prim;
 }
 
-__global__ void __launch_bounds__(256) k_apply(Batch B, int mode) {
+
+__global__ void __launch_bounds__(HFV_BLOCK) k_apply(Batch B, int mode) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const int s = J.sim;
+  if (!B.eng[s].stepping) return;
+  const int64_t i = (int64_t)J.lo + threadIdx.x;
+  if (i >= J.hi) return;
   const Cells& C = B.C;
   const Faces& F = B.F;
-  const bool dense = is_dense(B);
-  const int64_t count = dense ? C.n : (int64_t)B.cnt->touch_count;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   const int64_t n = C.n, nf = F.nf;
-  for (int64_t k = xcd_block() * blockDim.x + threadIdx.x; k < count; k += stride) {
-    int64_t i = dense ? k : (int64_t)B.W.touch_list[k];
-    bool a = C.act[i];
-    Events ev;
-    const int s = sim_of(C, i);
-    const Cfg& cfg = B.sims[s].cfg;
-    if (!(mode & 1)) {
-      if (a) st5(C.prim, n, i, c2p(ld5(C.cons, n, i), C.vol[i], cfg.gd, ev));
-      flush_events(ev, B.cnt, B.eng, s);
-      continue;
-    }
-    St<double> acc = ld5(C.facc, n, i);
-    double msv = 0., maor = 0.;
-    if (a) {
-      msv = C.msv[i];
-      maor = C.maor[i];
-    }
-    int32_t k0 = F.off[i], k1 = F.off[i + 1];
-    for (int32_t e = k0; e < k1; ++e) {
-      int32_t c = F.conn[e];
-      int64_t f = c >> 1;
-      int side = c & 1;
-      if (!a) {
-        int32_t other = side == 0 ? F.right[f] : F.left[f];
-        if (other < 0 || !C.act[other]) continue;
-      }
-      St<double> rec;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) rec.c[q] = F.rec[q * nf + f];
-      if (side == 0)
-        acc = acc - rec;
-      else
-        acc = acc + rec;
-      if (a) {
-        msv = hmax(F.rec[5 * nf + f], msv);
-        maor = hmax(maor, F.rec[6 * nf + f]);
-      }
-    }
-    if (a) {
-      St<double> c = ld5(C.cons, n, i) + acc;  // apply_flux (part.rs:178-193)
-#pragma unroll
-      for (int q = 0; q < 5; ++q)
-        if (!hfinite(c.c[q])) ev.fail(E_NONFINITE);
-      if (c.c[0] < 0.) {
-        ev.flux_reset++;
-        c = vacuum<double>();
-      }
-      if (c.c[4] < 0.) {
-        ev.flux_reset++;
-        c = vacuum<double>();
-      }
-      acc = vacuum<double>();
-      // kick2 -> grav_kick with a_grav = 0 (part.rs:303-312): momentum += (0 * mass)
-      c.c[1] += 0.0;
-      c.c[2] += 0.0;
-      c.c[3] += 0.0;
-      st5(C.cons, n, i, c);
-      C.msv[i] = msv;
-      C.maor[i] = maor;
-      if (mode & 2) st5(C.prim, n, i, c2p(c, C.vol[i], cfg.gd, ev));
-    }
-    st5(C.facc, n, i, acc);
+  const Cfg& cfg = B.sims[s].cfg;
+  const bool a = C.act[i];
+  Events ev;
+  if (!(mode & 1)) {
+    if (a) st5(C.prim, n, i, c2p(ld5(C.cons, n, i), C.vol[i], cfg.gd, ev));
     flush_events(ev, B.cnt, B.eng, s);
+    return;
   }
+  St<double> acc = vacuum<double>();
+  bool have = false;
+  double msv = 0., maor = 0.;
+  if (a) {
+    acc = ld5(C.facc, n, i);
+    have = true;
+    msv = C.msv[i];
+    maor = C.maor[i];
+  }
+  const int32_t k0 = F.off[i], k1 = F.off[i + 1];
+  for (int32_t e = k0; e < k1; ++e) {
+    const int32_t c = F.conn[e];
+    const uint32_t f = (uint32_t)(c >> 1);
+    if (!F.cur[f]) continue;
+    if (!have) {
+      acc = ld5(C.facc, n, i);
+      have = true;
+    }
+    St<double> rec;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) rec.c[q] = ldo(F.rec + q * nf, f);
+    if ((c & 1) == 0)
+      acc = acc - rec;
+    else
+      acc = acc + rec;
+    if (a) {
+      msv = hmax(ldo(F.rec + 5 * nf, f), msv);
+      maor = hmax(maor, ldo(F.rec + 6 * nf, f));
+    }
+  }
+  if (!have) return;
+  if (a) {
+    St<double> c = ld5(C.cons, n, i) + acc;  // apply_flux (part.rs:178-193)
+#pragma unroll
+    for (int q = 0; q < 5; ++q)
+      if (!hfinite(c.c[q])) ev.fail(E_NONFINITE);
+    if (c.c[0] < 0.) {
+      ev.flux_reset++;
+      c = vacuum<double>();
+    }
+    if (c.c[4] < 0.) {
+      ev.flux_reset++;
+      c = vacuum<double>();
+    }
+    acc = vacuum<double>();
+    // kick2 -> grav_kick with a_grav = 0 (part.rs:303-312): momentum += (0 * mass)
+    c.c[1] += 0.0;
+    c.c[2] += 0.0;
+    c.c[3] += 0.0;
+    st5(C.cons, n, i, c);
+    C.msv[i] = msv;
+    C.maor[i] = maor;
+    if (mode & 2) st5(C.prim, n, i, c2p(c, C.vol[i], cfg.gd, ev));
+  }
+  st5(C.facc, n, i, acc);
+  flush_events(ev, B.cnt, B.eng, s);
 }
 
 // glam::DMat3 as three columns
@@ -850,22 +875,34 @@ __device__ __forceinline__ double cell_radius(double V, const Cfg& cfg) {
   return pow(cfg.radius3_fac * V, 1. / 3.);
 }
 
+// Space::get_boundary_part (space.rs:213-229): the mirrored neighbour of a wall face.
+__device__ __forceinline__ void boundary_neighbour(const Faces& F, const Cfg& cfg, int64_t f, const V3<double>& cen,
+                                                   const St<double>& prim, V3<double>& oc, St<double>& op) {
+  const int64_t nf = F.nf;
+  V3<double> fc = ld3(F.fcen, nf, f), nrm = ld3(F.nrm, nf, f);
+  oc = cen + ((2. * dot(fc - cen, nrm)) * nrm);
+  op = prim;
+  if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, nrm);
+  else if (cfg.boundary == BND_VACUUM) op = vacuum<double>();
+}
+
 // ------------------------------------------------------------------ k_grad
 // gradient_estimate / gradient_limit / gradient_apply (hydrodynamics.rs:67-143, gradients.rs:8-103)
 // fused with compute_timesteps (euler_eqns_fvs.rs:50-61, part.rs:42-84, 328-372) and timesteps_apply
-// (hydrodynamics.rs:145-158, timeline.rs:103-139) for the active cells.
+// (hydrodynamics.rs:145-158, timeline.rs:103-139) for an active cell.  The limiter's neighbour min/max
+// is gathered in the estimate sweep (same values, same order); the second sweep only extrapolates.
 __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const SimEngine& E, int64_t i, int do_grad,
-                                          int do_ts, unsigned long long* dti_out, Events& ev) {
+                                          int do_ts, uint64_t* dti_out, Events& ev) {
   const Cells& C = B.C;
   const Faces& F = B.F;
   const int64_t n = C.n, nf = F.nf;
-  St<double> prim = ld5(C.prim, n, i);
-  V3<double> cen = ld3(C.cen, n, i);
+  const St<double> prim = ld5(C.prim, n, i);
+  const V3<double> cen = ld3(C.cen, n, i);
   Gr<double> gr;
   V3<double> gcen;
-  int32_t k0 = F.off[i], k1 = F.off[i + 1];
+  const int32_t k0 = F.off[i], k1 = F.off[i + 1];
   if (do_grad && cfg.runner == 0) {
-    // ---- estimate
+    // ---- estimate (GradientData) + neighbour extrema (LimiterData)
     Gr<double> g;
 #pragma unroll
     for (int q = 0; q < 5; ++q) g.g[q] = mk3(0., 0., 0.);
@@ -879,88 +916,82 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
     } else if (cfg.dim == 2) {
       m.z.z = 1.;
     }
+    St<double> lmin = prim, lmax = prim;
     for (int32_t k = k0; k < k1; ++k) {
-      int32_t e = F.conn[k];
-      int64_t f = e >> 1;
-      int side = e & 1;
-      double area = F.area[f];
-      V3<double> sh = decode_shift(F.shc[f], cfg.box);
-      V3<double> shift = side == 0 ? sh : -sh;
-      int32_t other = side == 0 ? F.right[f] : F.left[f];
+      const int32_t e = F.conn[k];
+      const uint32_t f = (uint32_t)(e >> 1);
+      const int side = e & 1;
+      const double area = ldo(F.area, f);
+      const V3<double> sh = decode_shift(F.shc[f], cfg.box);
+      const V3<double> shift = side == 0 ? sh : -sh;
+      const int32_t other = side == 0 ? F.right[f] : F.left[f];
       V3<double> oc;
       St<double> op;
       if (other >= 0) {
         oc = ld3(C.cen, n, other);
         op = ld5(C.prim, n, other);
-      } else {  // Space::get_boundary_part (space.rs:213-229)
-        V3<double> fc = ld3(F.fcen, nf, f), nrm = ld3(F.nrm, nf, f);
-        oc = cen + ((2. * dot(fc - cen, nrm)) * nrm);
-        op = prim;
-        if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, nrm);
-        else if (cfg.boundary == BND_VACUUM) op = vacuum<double>();
+      } else {
+        boundary_neighbour(F, cfg, f, cen, prim, oc, op);
       }
-      V3<double> ds = (oc + shift) - cen;
-      double w = area / len2(ds);
+      const V3<double> ds = (oc + shift) - cen;
+      const double w = area / len2(ds);
 #pragma unroll
       for (int q = 0; q < 5; ++q) g.g[q] = g.g[q] + ((w * (op.c[q] - prim.c[q])) * ds);
       m.x = m.x + (w * (ds.x * ds));
       m.y = m.y + (w * (ds.y * ds));
       m.z = m.z + (w * (ds.z * ds));
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        lmin.c[q] = hmin(lmin.c[q], op.c[q]);
+        lmax.c[q] = hmax(lmax.c[q], op.c[q]);
+      }
     }
+    HFV_STAGE();
     // DMat3::inverse (glam 0.27): cofactors by cross products, det = z . (x cross y), transpose
-    V3<double> t0 = cross(m.y, m.z), t1 = cross(m.z, m.x), t2 = cross(m.x, m.y);
-    double det = dot(m.z, t2);
-    double inv_det = 1. / det;
-    V3<double> c0 = mk3(t0.x * inv_det, t0.y * inv_det, t0.z * inv_det);
-    V3<double> c1 = mk3(t1.x * inv_det, t1.y * inv_det, t1.z * inv_det);
-    V3<double> c2 = mk3(t2.x * inv_det, t2.y * inv_det, t2.z * inv_det);
-    M3 inv;
-    inv.x = mk3(c0.x, c1.x, c2.x);
-    inv.y = mk3(c0.y, c1.y, c2.y);
-    inv.z = mk3(c0.z, c1.z, c2.z);
-    bool fin = true;
+    {
+      V3<double> t0 = cross(m.y, m.z), t1 = cross(m.z, m.x), t2 = cross(m.x, m.y);
+      double det = dot(m.z, t2);
+      double inv_det = 1. / det;
+      V3<double> c0 = mk3(t0.x * inv_det, t0.y * inv_det, t0.z * inv_det);
+      V3<double> c1 = mk3(t1.x * inv_det, t1.y * inv_det, t1.z * inv_det);
+      V3<double> c2 = mk3(t2.x * inv_det, t2.y * inv_det, t2.z * inv_det);
+      M3 inv;
+      inv.x = mk3(c0.x, c1.x, c2.x);
+      inv.y = mk3(c0.y, c1.y, c2.y);
+      inv.z = mk3(c0.z, c1.z, c2.z);
+      bool fin = true;
 #pragma unroll
-    for (int q = 0; q < 5; ++q) {
-      V3<double> v = g.g[q];
-      V3<double> r = v.x * inv.x;
-      r = r + (v.y * inv.y);
-      r = r + (v.z * inv.z);
-      g.g[q] = r;
-      fin = fin && finite3(r);
-    }
-    if (!fin) {
-      ev.grad_fallback++;
+      for (int q = 0; q < 5; ++q) {
+        V3<double> v = g.g[q];
+        V3<double> r = v.x * inv.x;
+        r = r + (v.y * inv.y);
+        r = r + (v.z * inv.z);
+        g.g[q] = r;
+        fin = fin && finite3(r);
+      }
+      if (!fin) {
+        ev.grad_fallback++;
 #pragma unroll
-      for (int q = 0; q < 5; ++q) g.g[q] = mk3(0., 0., 0.);
+        for (int q = 0; q < 5; ++q) g.g[q] = mk3(0., 0., 0.);
+      }
     }
-    // ---- limit (LimiterData)
-    St<double> lmin = prim, lmax = prim, emin, emax;
+    HFV_STAGE();
+    // ---- limit: extrema of the extrapolations to the faces
+    St<double> emin, emax;
 #pragma unroll
     for (int q = 0; q < 5; ++q) {
       emin.c[q] = INFINITY;
       emax.c[q] = -INFINITY;
     }
     for (int32_t k = k0; k < k1; ++k) {
-      int32_t e = F.conn[k];
-      int64_t f = e >> 1;
-      int side = e & 1;
-      V3<double> sh = decode_shift(F.shc[f], cfg.box);
-      V3<double> shift = side == 0 ? sh : -sh;
-      V3<double> fc = ld3(F.fcen, nf, f);
-      St<double> ext = gdot(g, (fc - cen) - shift);
-      int32_t other = side == 0 ? F.right[f] : F.left[f];
-      St<double> op;
-      if (other >= 0) {
-        op = ld5(C.prim, n, other);
-      } else {
-        op = prim;
-        if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, ld3(F.nrm, nf, f));
-        else if (cfg.boundary == BND_VACUUM) op = vacuum<double>();
-      }
+      const int32_t e = F.conn[k];
+      const uint32_t f = (uint32_t)(e >> 1);
+      const V3<double> sh = decode_shift(F.shc[f], cfg.box);
+      const V3<double> shift = (e & 1) == 0 ? sh : -sh;
+      const V3<double> fc = ld3(F.fcen, nf, f);
+      const St<double> ext = gdot(g, (fc - cen) - shift);
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
-        lmin.c[q] = hmin(lmin.c[q], op.c[q]);
-        lmax.c[q] = hmax(lmax.c[q], op.c[q]);
         emin.c[q] = hmin(emin.c[q], ext.c[q]);
         emax.c[q] = hmax(emax.c[q], ext.c[q]);
       }
@@ -980,8 +1011,9 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
     gcen = ld3(C.gc, n, i);
   }
   if (!do_ts) return;
+  HFV_STAGE();
   // ---- Particle::timestep (part.rs:42-84)
-  St<double> c = ld5(C.cons, n, i);
+  const St<double> c = ld5(C.cons, n, i);
   double dt_req;
   if (c.c[0] == 0.) {
     st3(C.v, n, i, mk3(0., 0., 0.));
@@ -1045,137 +1077,115 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
   *dti_out = dti;
 }
 
-__global__ void __launch_bounds__(256) k_grad(Batch B, int do_grad, int do_ts) {
-  const bool dense = is_dense(B);
-  const int64_t count = dense ? B.C.n : (int64_t)B.cnt->act_count;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t base = xcd_block() * blockDim.x; base < count; base += stride) {
-    int64_t k = base + threadIdx.x;
-    bool valid = k < count;
-    int64_t i = valid ? (dense ? k : (int64_t)B.W.act_list[k]) : 0;
-    int s = valid ? sim_of(B.C, i) : 0;
-    unsigned long long my = ~0ull;
-    Events ev;
-    bool todo = valid;
-    while (true) {
-      unsigned long long m = __ballot(todo);
-      if (m == 0) break;
-      int su = __builtin_amdgcn_readfirstlane(__shfl(s, __ffsll((long long)m) - 1));
-      if (todo && s == su) {
-        grad_cell(B, B.sims[su].cfg, B.eng[su], i, do_grad, do_ts, &my, ev);
-        todo = false;
-      }
-    }
-    // per-simulation minimum of dti (waterfall + wave reduction, one atomic per simulation per wave)
-    todo = valid && my != ~0ull;
-    while (true) {
-      unsigned long long m = __ballot(todo);
-      if (m == 0) break;
-      int leader = __ffsll((long long)m) - 1;
-      int su = __shfl(s, leader);
-      bool mine = todo && s == su;
-      unsigned long long v = mine ? my : ~0ull;
-      for (int off = 32; off > 0; off >>= 1) {
-        unsigned long long o = __shfl_xor(v, off);
-        v = o < v ? o : v;
-      }
-      if (__lane_id() == leader) atomicMin(&B.eng[su].ti_min, v);
-      if (mine) todo = false;
-    }
-    if (valid) flush_events(ev, B.cnt, B.eng, s);
+__global__ void __launch_bounds__(HFV_BLOCK) k_grad(Batch B, int do_grad, int do_ts) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const int s = J.sim;
+  const SimEngine& E = B.eng[s];
+  if (!E.stepping) return;
+  const Cfg& cfg = B.sims[s].cfg;
+  const int64_t i = (int64_t)J.lo + threadIdx.x;
+  const bool a = i < J.hi && B.C.act[i];
+  uint64_t my = ~0ull;
+  Events ev;
+  if (a) grad_cell(B, cfg, E, i, do_grad, do_ts, &my, ev);
+  if (do_ts) {
+    uint64_t t = block_min_u64(my);
+    if (threadIdx.x == 0) B.P.ti[J.chunk] = t;
   }
+  flush_events(ev, B.cnt, B.eng, s);
 }
 
 // ------------------------------------------------------------------ limiter
 // timestep_limiter first sweep (hydrodynamics.rs:165-179): wakeup from active neighbours' new bins;
 // timestep_sync (hydrodynamics.rs:186-191) reduces the minimum bin per simulation instead.
-__global__ void __launch_bounds__(256) k_wakeup(Batch B) {
+__global__ void __launch_bounds__(HFV_BLOCK) k_wakeup(Batch B) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const int s = J.sim;
+  if (!B.eng[s].stepping) return;
   const Cells& C = B.C;
   const Faces& F = B.F;
-  const bool dense = is_dense(B);
-  const int64_t count = dense ? C.n : (int64_t)B.cnt->touch_count;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-  for (int64_t k = xcd_block() * blockDim.x + threadIdx.x; k < count; k += stride) {
-    int64_t i = dense ? k : (int64_t)B.W.touch_list[k];
-    int s = sim_of(C, i);
-    int8_t w = C.bin[i];
-    if (B.sims[s].cfg.sync_all) {
-      atomicMin(&B.eng[s].min_bin, (int)w);
-      continue;
-    }
-    int32_t k0 = F.off[i], k1 = F.off[i + 1];
-    for (int32_t e = k0; e < k1; ++e) {
-      int32_t c = F.conn[e];
-      int64_t f = c >> 1;
-      int32_t other = (c & 1) == 0 ? F.right[f] : F.left[f];
-      if (other < 0) continue;
-      if (!C.act[other]) continue;
-      int8_t cand = (int8_t)(C.bin[other] + 2);
-      if (cand < w) w = cand;
-    }
-    C.wake[i] = w;
+  const int64_t i = (int64_t)J.lo + threadIdx.x;
+  const bool valid = i < J.hi;
+  if (B.sims[s].cfg.sync_all) {
+    int m = block_min_i32(valid ? (int)C.bin[i] : 0x7fffffff);
+    if (threadIdx.x == 0) atomicMin(&B.eng[s].min_bin, m);
+    return;
   }
+  if (!valid) return;
+  int8_t w = C.bin[i];
+  const int32_t k0 = F.off[i], k1 = F.off[i + 1];
+  for (int32_t e = k0; e < k1; ++e) {
+    const int32_t c = F.conn[e];
+    const uint32_t f = (uint32_t)(c >> 1);
+    const int32_t other = (c & 1) == 0 ? F.right[f] : F.left[f];
+    if (other < 0) continue;
+    if (!C.act[other]) continue;
+    int8_t cand = (int8_t)(C.bin[other] + 2);
+    if (cand < w) w = cand;
+  }
+  C.wake[i] = w;
 }
 
 // Particle::timestep_limit (part.rs:263-297) + reset_extrapolations (hydrodynamics.rs:193-195) +
-// kick1 (hydro_kick1 part.rs:87-104, grav_kick with a_grav = 0).  Clears the touched flags.
-__global__ void __launch_bounds__(256) k_limit(Batch B) {
+// kick1 (hydro_kick1 part.rs:87-104, grav_kick with a_grav = 0).
+__global__ void __launch_bounds__(HFV_BLOCK) k_limit(Batch B) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const int s = J.sim;
+  const SimEngine& E = B.eng[s];
+  if (!E.stepping) return;
+  const int64_t i = (int64_t)J.lo + threadIdx.x;
+  if (i >= J.hi) return;
   const Cells& C = B.C;
-  const bool dense = is_dense(B);
-  const int64_t count = dense ? C.n : (int64_t)B.cnt->touch_count;
-  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const Cfg& cfg = B.sims[s].cfg;
   const int64_t n = C.n;
-  for (int64_t k = xcd_block() * blockDim.x + threadIdx.x; k < count; k += stride) {
-    int64_t i = dense ? k : (int64_t)B.W.touch_list[k];
-    int s = sim_of(C, i);
-    const Cfg& cfg = B.sims[s].cfg;
-    const SimEngine& E = B.eng[s];
-    C.touched[i] = 0;
-    int8_t wakeup = cfg.sync_all ? (int8_t)E.min_bin : C.wake[i];
-    int8_t bin = C.bin[i];
-    if (wakeup < bin) {
-      double dt = C.dt[i];
-      uint64_t ti_current = E.ti_current;
-      double time_base = E.time_base;
-      if (bin <= get_max_active_bin(ti_current)) {
-        bin = wakeup;
-        dt = (double)get_integer_timestep(bin) * time_base;
-      } else {
-        uint64_t ti_end_old = get_integer_time_end(ti_current, bin);
-        dt -= (double)(ti_end_old - ti_current) * time_base;
-        bin = wakeup;
-        uint64_t dti_new = get_integer_timestep(bin);
-        uint64_t ti_end = get_integer_time_end(ti_current, bin);
-        dt += (ti_end == ti_current) ? (double)dti_new * time_base : (double)(ti_end - ti_current) * time_base;
-      }
-      C.bin[i] = bin;
-      C.dt[i] = dt;
+  int8_t wakeup = cfg.sync_all ? (int8_t)E.min_bin : C.wake[i];
+  int8_t bin = C.bin[i];
+  if (wakeup < bin) {
+    double dt = C.dt[i];
+    uint64_t ti_current = E.ti_current;
+    double time_base = E.time_base;
+    if (bin <= get_max_active_bin(ti_current)) {
+      bin = wakeup;
+      dt = (double)get_integer_timestep(bin) * time_base;
+    } else {
+      uint64_t ti_end_old = get_integer_time_end(ti_current, bin);
+      dt -= (double)(ti_end_old - ti_current) * time_base;
+      bin = wakeup;
+      uint64_t dti_new = get_integer_timestep(bin);
+      uint64_t ti_end = get_integer_time_end(ti_current, bin);
+      dt += (ti_end == ti_current) ? (double)dti_new * time_base : (double)(ti_end - ti_current) * time_base;
     }
-    if (C.act[i]) {
-      st5(C.extrap, n, i, vacuum<double>());
-      if (cfg.motion != MOT_FIXED) {
-        double rho = C.prim[i];
-        if (rho > 0.) {
-          double kick_fac = 0.5 * C.dt[i] / rho;
-          for (int a = 0; a < cfg.dim; ++a) C.v[a * n + i] -= kick_fac * C.grad[(12 + a) * n + i];
-        }
+    C.bin[i] = bin;
+    C.dt[i] = dt;
+  }
+  if (C.act[i]) {
+    st5(C.extrap, n, i, vacuum<double>());
+    if (cfg.motion != MOT_FIXED) {
+      double rho = C.prim[i];
+      if (rho > 0.) {
+        double kick_fac = 0.5 * C.dt[i] / rho;
+        for (int a = 0; a < cfg.dim; ++a) C.v[a * n + i] -= kick_fac * C.grad[(12 + a) * n + i];
       }
-      C.cons[1 * n + i] += 0.0;
-      C.cons[2 * n + i] += 0.0;
-      C.cons[3 * n + i] += 0.0;
     }
+    C.cons[1 * n + i] += 0.0;
+    C.cons[2 * n + i] += 0.0;
+    C.cons[3 * n + i] += 0.0;
   }
 }
 
-// Particle::from_ic + first_init (part.rs:200-217) for cells [lo, hi); vel3 is component-major [3m].
-__global__ void k_init_ic(Batch B, int64_t lo, int64_t hi, const double* mass, const double* vel3,
+// Particle::from_ic + first_init (part.rs:200-217) for cells [lo, hi) of simulation s; vel3 is
+// component-major [3m].
+__global__ void k_init_ic(Batch B, int s, int64_t lo, int64_t hi, const double* mass, const double* vel3,
                           const double* u_int) {
   int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int64_t i = lo + j;
   if (i >= hi) return;
   const Cells& C = B.C;
   const int64_t n = C.n, m_ = hi - lo;
-  const Cfg& cfg = B.sims[sim_of(C, i)].cfg;
+  const Cfg& cfg = B.sims[s].cfg;
   Events ev;
   double m = mass[j];
   V3<double> v = mk3(vel3[j], vel3[m_ + j], vel3[2 * m_ + j]);
@@ -1194,7 +1204,7 @@ __global__ void k_init_ic(Batch B, int64_t lo, int64_t hi, const double* mass, c
   }
   st5(C.cons, n, i, c);
   st5(C.prim, n, i, prim);
-  flush_events(ev, B.cnt, B.eng, sim_of(C, i));
+  flush_events(ev, B.cnt, B.eng, s);
 }
 
 // Space::status sums (space.rs:1248-1280) over cells [lo, hi): one block, fixed-order tree reduction.

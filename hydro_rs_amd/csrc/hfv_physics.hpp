@@ -702,18 +702,20 @@ __device__ __forceinline__ St<T> pairwise_limiter(const St<T>& pl, const St<T>& 
 }
 
 // ------------------------------------------------------------------ part.rs:134-149
+// The state that time_extrapolations scales by -dt (valid for rho > 0).
+template <class T>
+__device__ __forceinline__ St<T> tex_rate(const St<T>& prim, const Gr<T>& gr, V3<T> v_rel, const Gas<T>& g) {
+  T rho = prim.c[0];
+  T rho_inv = T(1) / rho;
+  T P = prim.c[4];
+  T dv = div_v(gr);
+  return mkst(rho * dv + dot(v_rel, gr.g[0]), (dv * v_rel) + (rho_inv * gr.g[4]),
+              g.gamma * P * dv + dot(v_rel, gr.g[4]));
+}
 template <class T>
 __device__ __forceinline__ St<T> time_extrapolations(const St<T>& prim, const Gr<T>& gr, V3<T> v_rel, T dt,
                                                      const Gas<T>& g) {
-  T rho = prim.c[0];
-  if (rho > T(0)) {
-    T rho_inv = T(1) / rho;
-    T P = prim.c[4];
-    T dv = div_v(gr);
-    St<T> s = mkst(rho * dv + dot(v_rel, gr.g[0]), (dv * v_rel) + (rho_inv * gr.g[4]),
-                   g.gamma * P * dv + dot(v_rel, gr.g[4]));
-    return (-dt) * s;
-  }
+  if (prim.c[0] > T(0)) return (-dt) * tex_rate(prim, gr, v_rel, g);
   return vacuum<T>();
 }
 

@@ -68,17 +68,33 @@ def run_ensemble(probs, max_launches=1 << 40, profile=False):
     }
 
 
-def run_plain(probs):
-    """Each problem alone through the single-simulation path (FP64 unless the problem says otherwise)."""
+def run_plain(probs, max_steps=1 << 40, warmup=0):
+    """Each problem alone through the single-simulation path (FP64 unless the problem says otherwise):
+    `warmup` untimed steps, then up to `max_steps` timed steps.  A simulation that raises a hard error
+    stops there (its completed steps count)."""
     cu = 0
+    steps = 0
     secs = 0.0
-    for p in probs:
+    failed = []
+    for k, p in enumerate(probs):
         d = ics.build_device_sim(p)
-        d.synchronize()
-        t0 = time.perf_counter()
-        d.run()
-        d.synchronize()
-        secs += time.perf_counter() - t0
-        cu += d.engine_info()["cell_updates"]
-        d.close()
-    return {"seconds": secs, "cell_updates": cu}
+        try:
+            if warmup:
+                d.run(warmup)
+            d.synchronize()
+            e0 = d.engine_info()
+            t0 = time.perf_counter()
+            try:
+                d.run(max_steps)
+            except fvs.HfvError as e:
+                failed.append((k, e.code))
+            d.synchronize()
+            secs += time.perf_counter() - t0
+            e1 = d.engine_info()
+            cu += e1["cell_updates"] - e0["cell_updates"]
+            steps += e1["step_count"] - e0["step_count"]
+        except fvs.HfvError as e:
+            failed.append((k, e.code))
+        finally:
+            d.close()
+    return {"seconds": secs, "cell_updates": cu, "steps": steps, "failed": failed}

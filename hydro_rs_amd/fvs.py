@@ -13,6 +13,8 @@ import numpy as np
 
 from . import _lib
 
+HfvError = _lib.HfvError
+
 EXACT, HLLC, PVRS, AIRS, TSRS, TRRS, LINEAR_ADVECTION = range(7)
 SOLVERS = {"Exact": EXACT, "HLLC": HLLC, "PVRS": PVRS, "AIRS": AIRS, "TSRS": TSRS, "TRRS": TRRS,
            "LinearAdvection": LINEAR_ADVECTION}

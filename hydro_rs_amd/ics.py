@@ -144,3 +144,10 @@ def build_device_sim(prob: Problem):
     sim = EulerEqnsFvs(prob.cfg, prob.mesh)
     sim.init_from_ic(prob.loc, prob.mass, prob.vel, prob.u_int)
     return sim
+
+
+def with_precision(prob: Problem, precision: int) -> Problem:
+    """The same problem (shared mesh and initial arrays) with another flux precision."""
+    from dataclasses import replace
+
+    return replace(prob, cfg=replace(prob.cfg, precision=precision))

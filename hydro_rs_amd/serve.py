@@ -103,8 +103,9 @@ class Server:
         self.resident[best] = job
         return True
 
-    def serve(self, jobs, fetch=("prim",), clock=time.perf_counter):
-        """Run the request loop over `jobs` (sorted by arrival); returns the jobs with their timings."""
+    def serve(self, jobs, fetch=("prim",), clock=time.perf_counter, progress=None, progress_every=5.0):
+        """Run the request loop over `jobs` (sorted by arrival); returns the jobs with their timings.
+        `progress(text)` is called about every `progress_every` seconds with the loop's counts."""
         for j in jobs:
             if not any(self._fits(s, j.prob) for s in range(len(self.caps[0]))):
                 raise ValueError(f"job {j.jid} ({j.prob.n} cells) fits no slot of this server")
@@ -113,8 +114,13 @@ class Server:
         t0 = clock()
         done = []
         pi = 0
+        last = 0.0
         while len(done) < len(jobs):
             now = clock() - t0
+            if progress and now - last >= progress_every:
+                last = now
+                progress(f"t={now:.1f}s done={len(done)} resident={len(self.resident)} queued={len(queue)} "
+                         f"launches={self.launches}")
             while pi < len(pending) and pending[pi].arrival <= now:
                 queue.append(pending[pi])
                 pi += 1

@@ -115,3 +115,30 @@ def test_failed_member_stops_alone():
         got = b.get_state(s)
         for k in alone:
             assert np.array_equal(got[k], alone[k]), (s, k)
+
+
+def test_hard_error_matches_oracle_step(ora):
+    """A member of the 512-simulation ensemble whose rarefaction drives a cell to negative pressure: the
+    reference's c2p assert (physical_quantities.rs:220-221) fires; the device path raises the same
+    error in the same step as the oracle, alone and as a batch slot, and the batch's other slot finishes."""
+    from hydro_rs_amd import ensemble
+
+    probs = ensemble.ensemble_problems(count=512, seed=4, precision=fvs.FP64)
+    p = probs[470]  # 1D, 1283 cells
+    ref = oracle_sim(p)
+    with pytest.raises(RuntimeError) as oe:
+        ref.advance(1 << 40)
+    ora_code = int(str(oe.value).split()[-1])
+    ri = ref.info()
+    say(f"  oracle: error after {ri['step_count']} completed steps, t={ri['t_current']}")
+    d = ics.build_device_sim(p)
+    with pytest.raises(fvs.HfvError) as ex:
+        d.run(1 << 40)
+    assert ex.value.code == ora_code
+    di = d.engine_info()
+    assert di["step_count"] == ri["step_count"]
+    b = fvs.Batch.for_problems([p, probs[0]])
+    b.run(1 << 40)
+    bi = b.engine_info(0)
+    assert bi["error_code"] == ex.value.code and bi["step_count"] == ri["step_count"]
+    assert b.engine_info(1)["done"] and b.engine_info(1)["error_code"] == 0
