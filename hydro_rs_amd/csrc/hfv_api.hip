@@ -142,6 +142,9 @@ struct hfv_ctx {
   std::vector<BlockJob> hcjobs, hfjobs;
   BlockJob *dcjobs = nullptr, *dfjobs = nullptr;
   Partials P{};
+  int2* fall = nullptr;      // exact-solver fallback queue (AIRS)
+  bool pack = false;         // packed flux records (hfv_set_flux_variant)
+  double* fp = nullptr;      // their storage, allocated on first use
   SimDesc* ddesc = nullptr;
   SimEngine* deng = nullptr;
   SimEngine* heng = nullptr;  // pinned mirror
@@ -178,6 +181,8 @@ struct hfv_ctx {
     B.fjobs = dfjobs;
     B.P = P;
     B.cnt = dcnt;
+    B.fall = fall;
+    B.pack = pack ? 1 : 0;
     return B;
   }
 };
@@ -288,6 +293,7 @@ int alloc_batch(hfv_ctx* c) {
   C.srad = dalloc<double>(c, n);
   C.tex = dalloc<double>(c, 5 * n);
   C.cs = dalloc<double>(c, n);
+  C.pc = dalloc<double>(c, 8 * n);
   C.bin = dalloc<int8_t>(c, n);
   C.wake = dalloc<int8_t>(c, n);
   C.act = dalloc<uint8_t>(c, n);
@@ -296,7 +302,7 @@ int alloc_batch(hfv_ctx* c) {
   F.area = dalloc<double>(c, nf);
   F.nrm = dalloc<double>(c, 3 * nf);
   F.fcen = dalloc<double>(c, 3 * nf);
-  F.rec = dalloc<double>(c, 7 * nf);
+  F.rec = dalloc<double>(c, 8 * nf);
   F.left = dalloc<int32_t>(c, nf);
   F.right = dalloc<int32_t>(c, nf);
   F.shc = dalloc<uint8_t>(c, nf);
@@ -322,6 +328,10 @@ int alloc_batch(hfv_ctx* c) {
   c->P.act = dalloc<uint32_t>(c, ncc);
   c->P.face = dalloc<uint32_t>(c, nfc);
   c->P.ti = dalloc<uint64_t>(c, ncc);
+  c->fall = dalloc<int2>(c, nf);
+  c->fp = nullptr;
+  c->C.fp = nullptr;
+  c->pack = false;
   c->ddesc = dalloc<SimDesc>(c, S);
   c->deng = dalloc<SimEngine>(c, S);
   for (void* p : c->bufs)
@@ -331,6 +341,7 @@ int alloc_batch(hfv_ctx* c) {
   double* dbl[] = {C.prim, C.grad, C.gc, C.extrap, C.cons, C.facc, C.vol, C.loc, C.cen, C.v, C.vrel, C.maor, C.msv, C.dt, C.srad, C.tex, C.cs};
   int mult[] = {5, 15, 3, 5, 5, 5, 1, 3, 3, 3, 3, 1, 1, 1, 1, 5, 1};
   for (int k = 0; k < 17; ++k) HIP_OK(hipMemsetAsync(dbl[k], 0, sd * mult[k], st));
+  HIP_OK(hipMemsetAsync(C.pc, 0, sd * 8, st));
   HIP_OK(hipMemsetAsync(C.bin, 0, (size_t)n, st));
   HIP_OK(hipMemsetAsync(C.wake, 0, (size_t)n, st));
   HIP_OK(hipMemsetAsync(C.act, 0, (size_t)n, st));
@@ -338,7 +349,7 @@ int alloc_batch(hfv_ctx* c) {
   HIP_OK(hipMemsetAsync(c->P.act, 0, sizeof(uint32_t) * (size_t)ncc, st));
   HIP_OK(hipMemsetAsync(c->P.face, 0, sizeof(uint32_t) * (size_t)nfc, st));
   HIP_OK(hipMemsetAsync(c->P.ti, 0xFF, sizeof(uint64_t) * (size_t)ncc, st));
-  HIP_OK(hipMemsetAsync(F.rec, 0, sizeof(double) * 7 * (size_t)nf, st));
+  HIP_OK(hipMemsetAsync(F.rec, 0, sizeof(double) * 8 * (size_t)nf, st));
   HIP_OK(hipMemsetAsync(F.left, 0xFF, sizeof(int32_t) * (size_t)nf, st));
   HIP_OK(hipMemsetAsync(F.right, 0xFF, sizeof(int32_t) * (size_t)nf, st));
   HIP_OK(hipMemsetAsync(F.off, 0, sizeof(int32_t) * (size_t)(n + 1), st));
@@ -624,29 +635,37 @@ void prof_next(hfv_ctx* c) {
   prof_take(c, c->ring, true);
 }
 
-template <class T>
-void launch_flux_t(hfv_ctx* c, const Batch& B, unsigned g) {
+template <class T, bool PACKED>
+void launch_flux_tp(hfv_ctx* c, const Batch& B, unsigned g) {
   switch (c->solver) {
     case HFV_HLLC:
-      hipLaunchKernelGGL((k_flux<T, SOLVER_HLLC>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, SOLVER_HLLC, PACKED>), dim3(g), dim3(BLOCK), 0, c->stream, B);
       break;
     case HFV_AIRS:
-      hipLaunchKernelGGL((k_flux<T, SOLVER_AIRS>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, SOLVER_AIRS, PACKED>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux_fallback<T, PACKED>), dim3(256), dim3(BLOCK), 0, c->stream, B);
       break;
     case HFV_EXACT:
-      hipLaunchKernelGGL((k_flux<T, SOLVER_EXACT>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, SOLVER_EXACT, PACKED>), dim3(g), dim3(BLOCK), 0, c->stream, B);
       break;
     default:
-      hipLaunchKernelGGL((k_flux<T, -1>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, -1, PACKED>), dim3(g), dim3(BLOCK), 0, c->stream, B);
   }
 }
 
 void launch_flux(hfv_ctx* c, const Batch& B) {
   unsigned g = (unsigned)c->nfpos;
-  if (c->precision == HFV_FP32_FLUX)
-    launch_flux_t<float>(c, B, g);
-  else
-    launch_flux_t<double>(c, B, g);
+  if (c->precision == HFV_FP32_FLUX) {
+    if (c->pack)
+      launch_flux_tp<float, true>(c, B, g);
+    else
+      launch_flux_tp<float, false>(c, B, g);
+  } else {
+    if (c->pack)
+      launch_flux_tp<double, true>(c, B, g);
+    else
+      launch_flux_tp<double, false>(c, B, g);
+  }
 }
 
 // The kernels of one batch step (k_begin .. k_end).  Timing events only outside graph capture.
@@ -676,7 +695,7 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   launch_flux(c, B);
   if (t) prof_end(c, KS_FLUX);
   if (t) prof_begin(c, KS_APPLY);
-  hipLaunchKernelGGL(k_apply, gc, gb, 0, st, B, 3);
+  hipLaunchKernelGGL(k_apply, gc, gb, 0, st, B, 7);
   if (t) prof_end(c, KS_APPLY);
   if (t) prof_begin(c, KS_GRAD);
   hipLaunchKernelGGL(k_grad, gc, gb, 0, st, B, 1, 1);
@@ -957,11 +976,11 @@ int hfv_get_fluxes(hfv_ctx* c, double* out7f) {
   TRY(require_mesh(c));
   if (!out7f) return set_err(HFV_ERR_INVALID, "out is NULL");
   const int64_t NF = c->nf, nf = c->fused[0], fb = c->fbase[0], n = c->nused[0], cb = c->cbase[0];
-  std::vector<double> soa(7 * nf);
+  std::vector<double> aos(8 * nf);
   std::vector<uint8_t> act(n);
   std::vector<int32_t> left(nf), right(nf);
-  for (int q = 0; q < 7; ++q)
-    HIP_OK(hipMemcpyAsync(soa.data() + q * nf, c->F.rec + q * NF + fb, sizeof(double) * nf, hipMemcpyDeviceToHost, c->stream));
+  (void)NF;
+  HIP_OK(hipMemcpyAsync(aos.data(), c->F.rec + 8 * fb, sizeof(double) * 8 * nf, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipMemcpyAsync(act.data(), c->C.act + cb, n, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipMemcpyAsync(left.data(), c->F.left + fb, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipMemcpyAsync(right.data(), c->F.right + fb, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, c->stream));
@@ -969,7 +988,7 @@ int hfv_get_fluxes(hfv_ctx* c, double* out7f) {
   for (int64_t f = 0; f < nf; ++f) {
     // records are current for faces with an active cell; the others are gated off (zero)
     bool valid = act[left[f] - cb] || (right[f] >= 0 && act[right[f] - cb]);
-    for (int k = 0; k < 7; ++k) out7f[7 * f + k] = valid ? soa[k * nf + f] : 0.;
+    for (int k = 0; k < 7; ++k) out7f[7 * f + k] = valid ? aos[8 * f + k] : 0.;
   }
   return HFV_OK;
 }
@@ -990,6 +1009,7 @@ int hfv_conserved_to_primitive(hfv_ctx* c) {
 
 int hfv_gradients(hfv_ctx* c) {
   TRY(require_mesh(c));
+  hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 4);
   hipLaunchKernelGGL(k_grad, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 1, 0);
   HIP_OK(hipGetLastError());
   return sync_counters(c);
@@ -1167,6 +1187,30 @@ int hfv_batch_get_state(hfv_ctx* c, int s, hfv_state* st) {
 int hfv_batch_num_slots(hfv_ctx* c, int* n) {
   if (!c || !n) return set_err(HFV_ERR_INVALID, "NULL argument");
   *n = c->nslots;
+  return HFV_OK;
+}
+
+// ---- kernel variants
+int hfv_set_flux_variant(hfv_ctx* c, int packed) {
+  if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
+  if (packed != 0 && packed != 1) return set_err(HFV_ERR_INVALID, "variant must be 0 or 1");
+  HIP_OK(hipSetDevice(c->device));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  if (packed && !c->fp) {
+    c->fp = dalloc<double>(c, HFV_FP_STRIDE * c->n);
+    if (!c->fp) return set_err(HFV_ERR_HIP, "hipMalloc failed for the flux records");
+    HIP_OK(hipMemsetAsync(c->fp, 0, sizeof(double) * HFV_FP_STRIDE * (size_t)c->n, c->stream));
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->C.fp = c->fp;
+  }
+  if (c->pack != (packed != 0)) drop_graph(c);
+  c->pack = packed != 0;
+  return HFV_OK;
+}
+
+int hfv_get_flux_variant(hfv_ctx* c, int* packed) {
+  if (!c || !packed) return set_err(HFV_ERR_INVALID, "NULL argument");
+  *packed = c->pack ? 1 : 0;
   return HFV_OK;
 }
 

@@ -9,8 +9,11 @@
 //     facc[5] (pending flux accumulator), vol, loc[3], cen[3], v[3], vrel[3], maor, msv, dt, srad,
 //     tex[5] (time-extrapolation rate of this step's primitives), cs (their sound speed, 0 for vacuum),
 //     bin (int8), wake (int8), act (uint8)
+//     pc[N][8] (AoS: prim[5], cen[3] of every cell after this step's c2p, one 64-byte line per cell
+//     for the gradient sweeps' neighbour gathers)
 //   faces, SoA: area, nrm[3], fcen[3], left (int32), right (int32, -1 = wall), shc (uint8 periodic
-//     shift code, 2 bits per axis), rec[7] (dt*F[5], v_max, a_over_r), cur (uint8: record of this step)
+//     shift code, 2 bits per axis), cur (uint8: record of this step); rec[F][8] (AoS, one 64-byte line
+//     per face: dt*F[5], v_max, a_over_r, unused)
 //   csr: off[N+1] (int32), conn[2F] (int32: face << 1 | side, side 1 = the cell is the face's right)
 //   block tables: one {sim, lo, hi, chunk} per thread block of the cell kernels and of the face
 //     kernels.  A block only holds items of one simulation, so configuration and engine loads are
@@ -30,7 +33,8 @@ namespace hfv {
 struct Cells {
   int64_t n;
   double *prim, *grad, *gc, *extrap, *cons, *facc, *vol, *loc, *cen, *v, *vrel, *maor, *msv, *dt, *srad;
-  double *tex, *cs;
+  double *tex, *cs, *pc;
+  double* fp;  // flux records [N][48] (nullptr unless the packed flux variant is in use)
   int8_t *bin, *wake;
   uint8_t* act;
 };
@@ -90,7 +94,7 @@ struct SimEngine {
 // Batch-global counters (soft events; the first hard error of any simulation).
 struct DevCounters {
   int err;
-  int pad;
+  unsigned int nfall;  // faces queued for the exact-solver fallback this step
   unsigned long long check_physical, flux_reset, grad_fallback, brent;
 };
 
@@ -111,11 +115,14 @@ struct Batch {
   const BlockJob* fjobs;  // face kernels
   Partials P;
   DevCounters* cnt;
+  int2* fall;  // (face, simulation) queued for k_flux_fallback
+  int pack;    // k_mask writes the flux records C.fp
 };
 
 #define HFV_NUM_TIME_BINS 56
 #define HFV_MAX_NR_TIMESTEPS (1ull << 57)
 #define HFV_BLOCK 256
+#define HFV_FP_STRIDE 48  // doubles per cell of the packed flux records
 
 // ------------------------------------------------------------------ timeline (src/timeline.rs)
 __device__ __host__ __forceinline__ uint64_t get_integer_timestep(int bin) {
@@ -289,6 +296,7 @@ __device__ __forceinline__ int block_min_i32(int v) {
 // the step at ti_next.
 __global__ void k_begin(Batch B) {
   int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0) B.cnt->nfall = 0;
   if (s >= B.nsims) return;
   SimEngine& E = B.eng[s];
   E.stepping = 0;
@@ -502,16 +510,48 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
     V3<double> vrel = ld3(C.vrel, n, i);
     St<double> rate = vacuum<double>();
     if (prim.c[0] > 0.) rate = tex_rate(prim, gr, vrel, cfg.gd);
+    St<double> ex = ld5(C.extrap, n, i);
     if (mode != 2) {
       // extrapolations += time_extrapolations(dt) (its vacuum branch adds +0)
       St<double> te = prim.c[0] > 0. ? (-E.dt_pred) * rate : vacuum<double>();
-      st5(C.extrap, n, i, ld5(C.extrap, n, i) + te);
+      ex = ex + te;
+      st5(C.extrap, n, i, ex);
       check_physical(prim, ev);
       if (ev.check_physical) st5(C.prim, n, i, prim);
       // volume += dvdt * dt clamped to [V/2, 2V]: dvdt is never set on the path (part.rs:130-131)
     }
-    st5(C.tex, n, i, rate);
-    C.cs[i] = prim.c[0] > 0. ? sound_speed(cfg.gd, prim.c[4], 1. / prim.c[0]) : 0.;
+    const double cs = prim.c[0] > 0. ? sound_speed(cfg.gd, prim.c[4], 1. / prim.c[0]) : 0.;
+    if (B.pack && mode != 1) {
+      double2* d = (double2*)(C.fp + HFV_FP_STRIDE * (uint64_t)i);
+      const V3<double> gc = ld3(C.gc, n, i), cen = ld3(C.cen, n, i), loc = ld3(C.loc, n, i), v = ld3(C.v, n, i);
+      d[0] = make_double2(act ? 1. : 0., C.dt[i]);
+      d[1] = make_double2(prim.c[0], prim.c[1]);
+      d[2] = make_double2(prim.c[2], prim.c[3]);
+      d[3] = make_double2(prim.c[4], cs);
+      d[4] = make_double2(gr.g[0].x, gr.g[0].y);
+      d[5] = make_double2(gr.g[0].z, gr.g[1].x);
+      d[6] = make_double2(gr.g[1].y, gr.g[1].z);
+      d[7] = make_double2(gr.g[2].x, gr.g[2].y);
+      d[8] = make_double2(gr.g[2].z, gr.g[3].x);
+      d[9] = make_double2(gr.g[3].y, gr.g[3].z);
+      d[10] = make_double2(gr.g[4].x, gr.g[4].y);
+      d[11] = make_double2(gr.g[4].z, ex.c[0]);
+      d[12] = make_double2(ex.c[1], ex.c[2]);
+      d[13] = make_double2(ex.c[3], ex.c[4]);
+      d[14] = make_double2(rate.c[0], rate.c[1]);
+      d[15] = make_double2(rate.c[2], rate.c[3]);
+      d[16] = make_double2(rate.c[4], gc.x);
+      d[17] = make_double2(gc.y, gc.z);
+      d[18] = make_double2(cen.x, cen.y);
+      d[19] = make_double2(cen.z, loc.x);
+      d[20] = make_double2(loc.y, loc.z);
+      d[21] = make_double2(v.x, v.y);
+      d[22] = make_double2(v.z, 0.);
+      d[23] = make_double2(0., 0.);
+    } else {
+      st5(C.tex, n, i, rate);
+      C.cs[i] = cs;
+    }
   }
   if (mode != 1) {
     uint32_t cnt = block_sum_u32(act ? 1u : 0u);
@@ -571,6 +611,12 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mesh1d_cells(Batch B) {
   B.C.cen[N + i] = 0.;
   B.C.cen[2 * N + i] = 0.;
   B.C.srad[i] = 1.5 * V;
+  if (B.pack) {
+    double* P = B.C.fp + HFV_FP_STRIDE * (uint64_t)i;
+    P[36] = 0.5 * (xl + xr);
+    P[37] = 0.;
+    P[38] = 0.;
+  }
 }
 
 // ------------------------------------------------------------------ k_flux
@@ -588,39 +634,82 @@ __device__ __forceinline__ const Gas<float>& gas_for<float>(const Cfg& c) {
   return c.gf;
 }
 
+// Per-cell inputs of the flux phase, read either from the SoA fields (SoaCell) or from the cell's
+// 384-byte flux record fp[48] written by k_mask (PackCell: three 128-byte lines per cell instead of
+// one line per field).  Record layout: 0 act, 1 dt, 2-6 prim, 7 cs, 8-22 grad, 23-27 extrap,
+// 28-32 tex, 33-35 gc, 36-38 cen, 39-41 loc, 42-44 v, 45-47 unused.
+struct SoaCell {
+  const Cells* C;
+  uint32_t i;
+  __device__ __forceinline__ SoaCell(const Cells& c, int32_t idx) : C(&c), i((uint32_t)idx) {}
+  __device__ __forceinline__ double dt() const { return ldo(C->dt, i); }
+  __device__ __forceinline__ St<double> prim() const { return ld5(C->prim, C->n, i); }
+  __device__ __forceinline__ double rho() const { return ldo(C->prim, i); }
+  __device__ __forceinline__ V3<double> pvel() const { return mk3(ldo(C->prim + C->n, i), ldo(C->prim + 2 * C->n, i), ldo(C->prim + 3 * C->n, i)); }
+  __device__ __forceinline__ double cs() const { return ldo(C->cs, i); }
+  __device__ __forceinline__ V3<double> grad(int q) const {
+    return mk3(ldo(C->grad + (3 * q) * C->n, i), ldo(C->grad + (3 * q + 1) * C->n, i), ldo(C->grad + (3 * q + 2) * C->n, i));
+  }
+  __device__ __forceinline__ double ex(int q) const { return ldo(C->extrap + q * C->n, i); }
+  __device__ __forceinline__ double tex(int q) const { return ldo(C->tex + q * C->n, i); }
+  __device__ __forceinline__ V3<double> gc() const { return ld3(C->gc, C->n, i); }
+  __device__ __forceinline__ V3<double> cen() const { return ld3(C->cen, C->n, i); }
+  __device__ __forceinline__ V3<double> loc() const { return ld3(C->loc, C->n, i); }
+  __device__ __forceinline__ V3<double> v() const { return ld3(C->v, C->n, i); }
+};
+struct PackCell {
+  const double* P;
+  __device__ __forceinline__ PackCell(const Cells& c, int32_t idx)
+      : P((const double*)__builtin_assume_aligned(c.fp + HFV_FP_STRIDE * (uint64_t)(uint32_t)idx, 128)) {}
+  __device__ __forceinline__ double dt() const { return P[1]; }
+  __device__ __forceinline__ St<double> prim() const { return mkst(P[2], mk3(P[3], P[4], P[5]), P[6]); }
+  __device__ __forceinline__ double rho() const { return P[2]; }
+  __device__ __forceinline__ V3<double> pvel() const { return mk3(P[3], P[4], P[5]); }
+  __device__ __forceinline__ double cs() const { return P[7]; }
+  __device__ __forceinline__ V3<double> grad(int q) const { return mk3(P[8 + 3 * q], P[9 + 3 * q], P[10 + 3 * q]); }
+  __device__ __forceinline__ double ex(int q) const { return P[23 + q]; }
+  __device__ __forceinline__ double tex(int q) const { return P[28 + q]; }
+  __device__ __forceinline__ V3<double> gc() const { return mk3(P[33], P[34], P[35]); }
+  __device__ __forceinline__ V3<double> cen() const { return mk3(P[36], P[37], P[38]); }
+  __device__ __forceinline__ V3<double> loc() const { return mk3(P[39], P[40], P[41]); }
+  __device__ __forceinline__ V3<double> v() const { return mk3(P[42], P[43], P[44]); }
+};
+
 // One side's reconstructed state at the face before limiting (flux.rs:52-73):
 // ((W + grad . dx) + extrapolations) + time_extrapolations(-dt/2), in T.
-template <class T>
-__device__ __forceinline__ St<T> face_side(const Cells& C, int64_t n, int32_t c, const St<T>& pT, V3<T> dxT, T dte) {
-  Gr<double> g = ldg(C.grad, n, c);
-  St<double> ex = ld5(C.extrap, n, c);
-  St<double> rate = ld5(C.tex, n, c);
+template <class T, class Acc>
+__device__ __forceinline__ St<T> face_side(const Acc& X, const St<T>& pT, V3<T> dxT, T dte) {
   St<T> out;
 #pragma unroll
-  for (int q = 0; q < 5; ++q) out.c[q] = (pT.c[q] + dot(cvt3<T>(g.g[q]), dxT)) + (T)ex.c[q];
-  if (pT.c[0] > T(0)) out = out + (-dte) * cvtst<T>(rate);
-  else out = out + vacuum<T>();
+  for (int q = 0; q < 5; ++q) out.c[q] = (pT.c[q] + dot(cvt3<T>(X.grad(q)), dxT)) + (T)X.ex(q);
+  if (pT.c[0] > T(0)) {
+    St<T> rate;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) rate.c[q] = (T)X.tex(q);
+    out = out + (-dte) * rate;
+  } else {
+    out = out + vacuum<T>();
+  }
   return out;
 }
 
-template <class T, int SOLVER>
-__device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_t f, int32_t l, int32_t r,
-                                          double* outv, Events& ev) {
-  const Cells& C = B.C;
+// The face's record (dt*F, v_max, a_over_r) when computed; false when gated off (record zero).
+template <class T, int SOLVER, bool DEFER, class Acc>
+__device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_t f, int32_t l, int32_t r, bool la,
+                                          bool ra, double* outv, Events& ev) {
   const Faces& F = B.F;
-  const int64_t n = C.n;
   const int64_t nf = F.nf;
   const Gas<T>& g = gas_for<T>(cfg);
   SolverCfg sc = cfg.solver;
   if (SOLVER >= 0) sc.kind = SOLVER;
 #pragma unroll
   for (int k = 0; k < 7; ++k) outv[k] = 0.;
-  const bool la = C.act[l];
-  const double dtl = C.dt[l];
+  const Acc L(B.C, l);
+  const double dtl = L.dt();
   if (r >= 0) {
+    const Acc R(B.C, r);
     // ---- gate
-    const bool ra = C.act[r];
-    const double dtr = C.dt[r];
+    const double dtr = R.dt();
     if ((!la && !ra) || (dtr < dtl && !ra) || (dtl < dtr && !la)) return false;
     const double dt = hmin(dtl, dtr);
     HFV_STAGE();
@@ -628,13 +717,13 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
     const V3<double> fc = ld3(F.fcen, nf, f);
     const V3<double> sh = decode_shift(F.shc[f], cfg.box);
     const T dte = (T)(-0.5 * dt);
-    const V3<double> dx_left = fc - ld3(C.gc, n, l);
-    const St<T> plT = cvtst<T>(ld5(C.prim, n, l));
-    const St<T> left_dash = face_side<T>(C, n, l, plT, cvt3<T>(dx_left), dte);
+    const V3<double> dx_left = fc - L.gc();
+    const St<T> plT = cvtst<T>(L.prim());
+    const St<T> left_dash = face_side<T>(L, plT, cvt3<T>(dx_left), dte);
     HFV_STAGE();
-    const V3<double> dx_right = (fc - ld3(C.gc, n, r)) - sh;
-    const St<T> prT = cvtst<T>(ld5(C.prim, n, r));
-    const St<T> right_dash = face_side<T>(C, n, r, prT, cvt3<T>(dx_right), dte);
+    const V3<double> dx_right = (fc - R.gc()) - sh;
+    const St<T> prT = cvtst<T>(R.prim());
+    const St<T> right_dash = face_side<T>(R, prT, cvt3<T>(dx_right), dte);
     HFV_STAGE();
     // ---- pairwise limiter (gradients.rs:105-152)
     const T dxg = (T)len(dx_left - dx_right);
@@ -644,21 +733,18 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
     // ---- signal speed (flux.rs:43-50), face velocity (flux.rs:75-78), a_over_r: FP64
     double v_max = 0.;
     {
-      const V3<double> dx_centroid = (ld3(C.cen, n, r) + sh) - ld3(C.cen, n, l);
-      const double rho_l = C.prim[l], rho_r = C.prim[r];
-      if (rho_l > 0.) v_max += C.cs[l];
-      if (rho_r > 0.) v_max += C.cs[r];
-      const V3<double> vpl = mk3(C.prim[n + l], C.prim[2 * n + l], C.prim[3 * n + l]);
-      const V3<double> vpr = mk3(C.prim[n + r], C.prim[2 * n + r], C.prim[3 * n + r]);
-      v_max -= hmin(dot(vpr - vpl, dx_centroid), 0.);
+      const V3<double> dx_centroid = (R.cen() + sh) - L.cen();
+      if (L.rho() > 0.) v_max += L.cs();
+      if (R.rho() > 0.) v_max += R.cs();
+      v_max -= hmin(dot(R.pvel() - L.pvel(), dx_centroid), 0.);
     }
     HFV_STAGE();
     V3<T> vfT;
     double len_dx;
     {
-      const V3<double> locl = ld3(C.loc, n, l), locr = ld3(C.loc, n, r);
+      const V3<double> locl = L.loc(), locr = R.loc();
       const V3<double> dx = (locr + sh) - locl;
-      const V3<double> vl = ld3(C.v, n, l), vr = ld3(C.v, n, r);
+      const V3<double> vl = L.v(), vr = R.v();
       const V3<double> midpoint = 0.5 * ((locl + locr) + sh);
       const double fac = dot(vr - vl, fc - midpoint) / len2(dx);
       vfT = cvt3<T>((0.5 * (vl + vr)) - (fac * dx));
@@ -668,7 +754,8 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
     // ---- Riemann problem in the face frame
     const V3<double> nrm = ld3(F.nrm, nf, f);
     const double area = F.area[f];
-    St<T> Fl = riemann_flux(sc, boost(prim_left, -vfT), boost(prim_right, -vfT), vfT, cvt3<T>(nrm), g, cfg.gd, ev);
+    St<T> Fl = riemann_flux<T, DEFER>(sc, boost(prim_left, -vfT), boost(prim_right, -vfT), vfT, cvt3<T>(nrm), g,
+                                      cfg.gd, ev);
     St<double> rec = dt * (area * cvtst<double>(Fl));
 #pragma unroll
     for (int k = 0; k < 5; ++k) outv[k] = rec.c[k];
@@ -680,7 +767,7 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
   const V3<double> fc = ld3(F.fcen, nf, f);
   const V3<double> nrm = ld3(F.nrm, nf, f);
   const double area = F.area[f];
-  const V3<double> loc = ld3(C.loc, n, l), cen = ld3(C.cen, n, l), gc = ld3(C.gc, n, l);
+  const V3<double> loc = L.loc(), cen = L.cen(), gc = L.gc();
   // Particle::reflect (part.rs:374-383)
   const V3<double> loc_r = loc + ((2. * dot(fc - loc, nrm)) * nrm);
   const V3<double> cen_r = cen + ((2. * dot(fc - cen, nrm)) * nrm);
@@ -689,14 +776,14 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
   const double dx_gradients = len(gc_r - gc);
   const V3<double> dx_centroid = cen_r - cen;
   const V3<double> dx = loc_r - loc;
-  const St<double> p = ld5(C.prim, n, l);
+  const St<double> p = L.prim();
   const double dtp = dtl;
   double v_max = 0.;
-  if (p.c[0] > 0.) v_max += C.cs[l];
+  if (p.c[0] > 0.) v_max += L.cs();
   const St<T> pT = cvtst<T>(p);
   const V3<T> dxfT = cvt3<T>(dx_face);
   const V3<T> nT = cvt3<T>(nrm);
-  St<T> dash = face_side<T>(C, n, l, pT, dxfT, (T)(-0.5 * dtp));
+  St<T> dash = face_side<T>(L, pT, dxfT, (T)(-0.5 * dtp));
   const T dx_fac = len(dxfT) / (T)dx_gradients;
   St<T> bstate;
   if (cfg.boundary == BND_REFLECTIVE) {
@@ -717,7 +804,7 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
     ev.fail(E_UNSUPPORTED);
     bstate = dash;
   }
-  St<T> Fl = riemann_flux(sc, dash, bstate, mk3(T(0), T(0), T(0)), nT, g, cfg.gd, ev);
+  St<T> Fl = riemann_flux<T, DEFER>(sc, dash, bstate, mk3(T(0), T(0), T(0)), nT, g, cfg.gd, ev);
   St<double> fluxes = area * cvtst<double>(Fl);
   St<double> rec = dtp * fluxes;
 #pragma unroll
@@ -727,32 +814,74 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
   return true;
 }
 
+__device__ __forceinline__ void store_rec(double* rec, int64_t f, const double* outv) {
+  double2* dst = (double2*)(rec + 8 * (uint64_t)f);
+#pragma unroll
+  for (int q = 0; q < 3; ++q) dst[q] = make_double2(outv[2 * q], outv[2 * q + 1]);
+  dst[3] = make_double2(outv[6], 0.);
+}
+
 // Faces with an active cell get their record (zero when gated off) and cur = 1; the others cur = 0.
-template <class T, int SOLVER>
+// With SOLVER == AIRS, faces that need the exact solver are queued for k_flux_fallback.
+template <class T, int SOLVER, bool PACKED>
 __global__ void __launch_bounds__(HFV_BLOCK) k_flux(Batch B) {
   const BlockJob J = B.fjobs[blockIdx.x];
   if (J.sim < 0) return;
   const int s = J.sim;
   if (!B.eng[s].stepping) return;
   const Cfg& cfg = B.sims[s].cfg;
-  const int64_t nf = B.F.nf;
   const int64_t f = (int64_t)J.lo + threadIdx.x;
+  constexpr bool DEFER = SOLVER == SOLVER_AIRS;
   bool computed = false;
   Events ev;
   if (f < J.hi) {
     const int32_t l = B.F.left[f], r = B.F.right[f];
-    const bool cur = B.C.act[l] || (r >= 0 && B.C.act[r]);
+    const bool la = B.C.act[l], ra = r >= 0 && B.C.act[r];
+    const bool cur = la || ra;
     B.F.cur[f] = cur ? 1 : 0;
     if (cur) {
       double outv[7];
-      computed = face_flux<T, SOLVER>(B, cfg, f, l, r, outv, ev);
-#pragma unroll
-      for (int q = 0; q < 7; ++q) B.F.rec[q * nf + f] = outv[q];
+      if (PACKED)
+        computed = face_flux<T, SOLVER, DEFER, PackCell>(B, cfg, f, l, r, la, ra, outv, ev);
+      else
+        computed = face_flux<T, SOLVER, DEFER, SoaCell>(B, cfg, f, l, r, la, ra, outv, ev);
+      if (!ev.deferred) store_rec(B.F.rec, f, outv);
     }
+  }
+  // deferred faces: wave-aggregated append (rare)
+  const unsigned long long m = __ballot(ev.deferred != 0);
+  if (m) {
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned int base = 0;
+    if (__lane_id() == leader) base = atomicAdd(&B.cnt->nfall, (unsigned int)__popcll(m));
+    base = __shfl(base, leader);
+    if (ev.deferred) B.fall[base + __popcll(m & ((1ull << __lane_id()) - 1ull))] = make_int2((int)f, s);
   }
   uint32_t cnt = block_sum_u32(computed ? 1u : 0u);
   if (threadIdx.x == 0) B.P.face[J.chunk] = cnt;
+  ev.deferred = 0;
   flush_events(ev, B.cnt, B.eng, s);
+}
+
+// The queued faces of this step with the full AIRS solver (exact fallback), grid-stride.
+template <class T, bool PACKED>
+__global__ void __launch_bounds__(HFV_BLOCK) k_flux_fallback(Batch B) {
+  const unsigned int count = B.cnt->nfall;
+  for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
+    const int2 e = B.fall[k];
+    const int64_t f = e.x;
+    const int s = e.y;
+    const int32_t l = B.F.left[f], r = B.F.right[f];
+    const bool la = B.C.act[l], ra = r >= 0 && B.C.act[r];
+    Events ev;
+    double outv[7];
+    if (PACKED)
+      face_flux<T, SOLVER_AIRS, false, PackCell>(B, B.sims[s].cfg, f, l, r, la, ra, outv, ev);
+    else
+      face_flux<T, SOLVER_AIRS, false, SoaCell>(B, B.sims[s].cfg, f, l, r, la, ra, outv, ev);
+    store_rec(B.F.rec, f, outv);
+    flush_events(ev, B.cnt, B.eng, s);
+  }
 }
 
 // ------------------------------------------------------------------ k_apply
@@ -787,6 +916,27 @@ __device__ __forceinline__ St<double> c2p(const St<double>& c, double vol, const
 }
 
 
+// One 64-byte line of the pc pack: prim[5], cen[3].
+__device__ __forceinline__ void store_pc(double* pc, int64_t i, const St<double>& p, const V3<double>& c) {
+  double2* d = (double2*)(pc + 8 * (uint64_t)i);
+  d[0] = make_double2(p.c[0], p.c[1]);
+  d[1] = make_double2(p.c[2], p.c[3]);
+  d[2] = make_double2(p.c[4], c.x);
+  d[3] = make_double2(c.y, c.z);
+}
+__device__ __forceinline__ void load_pc(const double* pc, uint32_t i, St<double>& p, V3<double>& c) {
+  const double2* d = (const double2*)(pc + 8 * (uint64_t)i);
+  double2 a = d[0], b = d[1], e = d[2], g = d[3];
+  p.c[0] = a.x;
+  p.c[1] = a.y;
+  p.c[2] = b.x;
+  p.c[3] = b.y;
+  p.c[4] = e.x;
+  c = mk3(e.y, g.x, g.y);
+}
+
+// mode bit 1: accumulate + apply + kick2; bit 2: conserved -> primitive; bit 4: write the pc pack of
+// every cell (after bits 1 and 2).
 __global__ void __launch_bounds__(HFV_BLOCK) k_apply(Batch B, int mode) {
   const BlockJob J = B.cjobs[blockIdx.x];
   if (J.sim < 0) return;
@@ -796,70 +946,79 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_apply(Batch B, int mode) {
   if (i >= J.hi) return;
   const Cells& C = B.C;
   const Faces& F = B.F;
-  const int64_t n = C.n, nf = F.nf;
+  const int64_t n = C.n;
   const Cfg& cfg = B.sims[s].cfg;
   const bool a = C.act[i];
   Events ev;
-  if (!(mode & 1)) {
-    if (a) st5(C.prim, n, i, c2p(ld5(C.cons, n, i), C.vol[i], cfg.gd, ev));
-    flush_events(ev, B.cnt, B.eng, s);
-    return;
-  }
-  St<double> acc = vacuum<double>();
-  bool have = false;
-  double msv = 0., maor = 0.;
-  if (a) {
-    acc = ld5(C.facc, n, i);
-    have = true;
-    msv = C.msv[i];
-    maor = C.maor[i];
-  }
-  const int32_t k0 = F.off[i], k1 = F.off[i + 1];
-  for (int32_t e = k0; e < k1; ++e) {
-    const int32_t c = F.conn[e];
-    const uint32_t f = (uint32_t)(c >> 1);
-    if (!F.cur[f]) continue;
-    if (!have) {
+  bool have_prim = false;
+  St<double> prim;
+  if (mode & 1) {
+    St<double> acc = vacuum<double>();
+    bool have = false;
+    double msv = 0., maor = 0.;
+    if (a) {
       acc = ld5(C.facc, n, i);
       have = true;
+      msv = C.msv[i];
+      maor = C.maor[i];
     }
-    St<double> rec;
+    const int32_t k0 = F.off[i], k1 = F.off[i + 1];
+    for (int32_t e = k0; e < k1; ++e) {
+      const int32_t c = F.conn[e];
+      const uint32_t f = (uint32_t)(c >> 1);
+      if (!F.cur[f]) continue;
+      if (!have) {
+        acc = ld5(C.facc, n, i);
+        have = true;
+      }
+      const double2* r2 = (const double2*)(F.rec + 8 * (uint64_t)f);
+      const double2 r01 = r2[0], r23 = r2[1], r45 = r2[2], r67 = r2[3];
+      const St<double> rec = mkst(r01.x, mk3(r01.y, r23.x, r23.y), r45.x);
+      if ((c & 1) == 0)
+        acc = acc - rec;
+      else
+        acc = acc + rec;
+      if (a) {
+        msv = hmax(r45.y, msv);
+        maor = hmax(maor, r67.x);
+      }
+    }
+    if (have) {
+      if (a) {
+        St<double> cc = ld5(C.cons, n, i) + acc;  // apply_flux (part.rs:178-193)
 #pragma unroll
-    for (int q = 0; q < 5; ++q) rec.c[q] = ldo(F.rec + q * nf, f);
-    if ((c & 1) == 0)
-      acc = acc - rec;
-    else
-      acc = acc + rec;
-    if (a) {
-      msv = hmax(ldo(F.rec + 5 * nf, f), msv);
-      maor = hmax(maor, ldo(F.rec + 6 * nf, f));
+        for (int q = 0; q < 5; ++q)
+          if (!hfinite(cc.c[q])) ev.fail(E_NONFINITE);
+        if (cc.c[0] < 0.) {
+          ev.flux_reset++;
+          cc = vacuum<double>();
+        }
+        if (cc.c[4] < 0.) {
+          ev.flux_reset++;
+          cc = vacuum<double>();
+        }
+        acc = vacuum<double>();
+        // kick2 -> grav_kick with a_grav = 0 (part.rs:303-312): momentum += (0 * mass)
+        cc.c[1] += 0.0;
+        cc.c[2] += 0.0;
+        cc.c[3] += 0.0;
+        st5(C.cons, n, i, cc);
+        C.msv[i] = msv;
+        C.maor[i] = maor;
+        if (mode & 2) {
+          prim = c2p(cc, C.vol[i], cfg.gd, ev);
+          st5(C.prim, n, i, prim);
+          have_prim = true;
+        }
+      }
+      st5(C.facc, n, i, acc);
     }
+  } else if ((mode & 2) && a) {
+    prim = c2p(ld5(C.cons, n, i), C.vol[i], cfg.gd, ev);
+    st5(C.prim, n, i, prim);
+    have_prim = true;
   }
-  if (!have) return;
-  if (a) {
-    St<double> c = ld5(C.cons, n, i) + acc;  // apply_flux (part.rs:178-193)
-#pragma unroll
-    for (int q = 0; q < 5; ++q)
-      if (!hfinite(c.c[q])) ev.fail(E_NONFINITE);
-    if (c.c[0] < 0.) {
-      ev.flux_reset++;
-      c = vacuum<double>();
-    }
-    if (c.c[4] < 0.) {
-      ev.flux_reset++;
-      c = vacuum<double>();
-    }
-    acc = vacuum<double>();
-    // kick2 -> grav_kick with a_grav = 0 (part.rs:303-312): momentum += (0 * mass)
-    c.c[1] += 0.0;
-    c.c[2] += 0.0;
-    c.c[3] += 0.0;
-    st5(C.cons, n, i, c);
-    C.msv[i] = msv;
-    C.maor[i] = maor;
-    if (mode & 2) st5(C.prim, n, i, c2p(c, C.vol[i], cfg.gd, ev));
-  }
-  st5(C.facc, n, i, acc);
+  if (mode & 4) store_pc(C.pc, i, have_prim ? prim : ld5(C.prim, n, i), ld3(C.cen, n, i));
   flush_events(ev, B.cnt, B.eng, s);
 }
 
@@ -928,8 +1087,7 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
       V3<double> oc;
       St<double> op;
       if (other >= 0) {
-        oc = ld3(C.cen, n, other);
-        op = ld5(C.prim, n, other);
+        load_pc(C.pc, (uint32_t)other, op, oc);
       } else {
         boundary_neighbour(F, cfg, f, cen, prim, oc, op);
       }

@@ -36,7 +36,8 @@ struct Events {
   int brent;
   int grad_fallback;
   int flux_reset;
-  __device__ __host__ Events() : err(0), check_physical(0), brent(0), grad_fallback(0), flux_reset(0) {}
+  int deferred;  // the face needs the exact solver and was left to the fallback pass
+  __device__ __host__ Events() : err(0), check_physical(0), brent(0), grad_fallback(0), flux_reset(0), deferred(0) {}
   __device__ __host__ void fail(int e) {
     if (!err) err = e;
   }
@@ -597,7 +598,10 @@ __device__ __forceinline__ Star<T> star_for(int kind, double threshold, const St
 
 // RiemannFluxSolver::solve_for_flux.  For the FP32-flux path (T = float) the exact solver's pressure
 // iteration runs in FP64 (gd = double constants) and its star state is rounded back to float.
-template <class T>
+// DEFER (AIRS only): a face whose pressure ratio calls for the exact solver is not solved here
+// (ev.deferred is set); the caller hands it to a separate pass, so that this code path carries no
+// exact-solver registers.
+template <class T, bool DEFER = false>
 __device__ St<T> riemann_flux(const SolverCfg& sc, const St<T>& left, const St<T>& right, V3<T> iv, V3<T> n,
                               const Gas<T>& g, const Gas<double>& gd, Events& ev) {
   if (sc.kind == SOLVER_LINADV) {  // linear_advection.rs:17-33
@@ -653,6 +657,12 @@ __device__ St<T> riemann_flux(const SolverCfg& sc, const St<T>& left, const St<T
   St<T> w;
   if (is_vacuum(left, right, a_l, a_r, v_r_m_v_l, g)) {
     w = vacuum_sample(left, right, v_l, v_r, a_l, a_r, n, g);
+  } else if (DEFER) {
+    if (!(hmax(left.c[4], right.c[4]) / hmin(left.c[4], right.c[4]) < (T)sc.threshold)) {
+      ev.deferred = 1;
+      return vacuum<T>();
+    }
+    w = sample_from_star(pvrs_star(left, right, v_l, v_r, a_l, a_r), left, right, v_l, v_r, a_l, a_r, n, g);
   } else {
     Star<T> star;
     bool need_fp64 = (sizeof(T) != sizeof(double)) &&

@@ -224,6 +224,15 @@ class EulerEqnsFvs:
     def synchronize(self):
         _lib.check(self.lib.hfv_synchronize(self.ptr))
 
+    def set_flux_variant(self, packed):
+        """0: flux kernel gathers SoA fields; 1: packed per-cell flux records (bit-identical results)."""
+        _lib.check(self.lib.hfv_set_flux_variant(self.ptr, int(packed)))
+
+    def flux_variant(self):
+        v = C.c_int(0)
+        _lib.check(self.lib.hfv_get_flux_variant(self.ptr, C.byref(v)))
+        return v.value
+
 
 def _engine_dict(e):
     d = {k: getattr(e, k) for k, _ in _lib.EngineInfo._fields_ if k != "ts"}
@@ -323,6 +332,15 @@ class Batch:
 
     def synchronize(self):
         _lib.check(self.lib.hfv_synchronize(self.ptr))
+
+    def set_flux_variant(self, packed):
+        """0: flux kernel gathers SoA fields; 1: packed per-cell flux records (bit-identical results)."""
+        _lib.check(self.lib.hfv_set_flux_variant(self.ptr, int(packed)))
+
+    def flux_variant(self):
+        v = C.c_int(0)
+        _lib.check(self.lib.hfv_get_flux_variant(self.ptr, C.byref(v)))
+        return v.value
 
 
 def riemann_batch(cfg: Config, WL, WR, vface, nrm):

@@ -39,10 +39,12 @@ def _alone(prob, steps):
     return d.get_state(), d.engine_info()
 
 
-def test_batch_equals_standalone():
+@pytest.mark.parametrize("packed", [0, 1])
+def test_batch_equals_standalone(packed):
     probs = _mix()
-    say(f"[batch] {len(probs)} simulations: {[p.name for p in probs]}")
+    say(f"[batch] {len(probs)} simulations: {[p.name for p in probs]}, flux variant {packed}")
     b = fvs.Batch.for_problems(probs, headroom=1.25)
+    b.set_flux_variant(packed)
     launches = b.run(1 << 40)
     say(f"  batch launches until all done: {launches}")
     for s, p in enumerate(probs):
