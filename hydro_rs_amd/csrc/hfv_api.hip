@@ -308,6 +308,7 @@ int alloc_batch(hfv_ctx* c) {
   F.shc = dalloc<uint8_t>(c, nf);
   F.off = dalloc<int32_t>(c, n + 1);
   F.conn = dalloc<int32_t>(c, nk);
+  F.nbr = dalloc<int32_t>(c, nk);
   F.cur = dalloc<uint8_t>(c, nf);
   // chunks: a slot's cell capacity and face capacity in blocks of BLOCK items
   c->cchunk0.assign(S, 0);
@@ -377,6 +378,7 @@ int alloc_batch(hfv_ctx* c) {
   off[n] = (int32_t)nk;
   HIP_OK(hipMemcpyAsync(F.off, off.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
   HIP_OK(hipMemsetAsync(F.conn, 0, sizeof(int32_t) * (size_t)std::max<int64_t>(nk, 1), st));
+  HIP_OK(hipMemsetAsync(F.nbr, 0xFF, sizeof(int32_t) * (size_t)std::max<int64_t>(nk, 1), st));
   HIP_OK(hipStreamSynchronize(st));
   c->nused.assign(S, 0);
   c->fused.assign(S, 0);
@@ -457,7 +459,7 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
   if (tot > c->kcap[s]) return set_err(HFV_ERR_INVALID, "csr exceeds the slot's capacity");
   // offsets of this slot's cells only (plus the global end entry for the last slot)
   const int64_t noff = icap(c, s) + (s == c->nslots - 1 ? 1 : 0);
-  std::vector<int32_t> off(noff), conn(tot);
+  std::vector<int32_t> off(noff), conn(tot), nbr(tot);
   int64_t acc = 0;
   for (int64_t i = 0; i < n; ++i) {
     off[i] = (int32_t)(kb + acc);
@@ -471,6 +473,8 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
       else if (m->right[f] == i) side = 1;
       else return set_err(HFV_ERR_INVALID, "csr lists a face that does not touch the cell");
       conn[acc + k] = (int32_t)(((fb + f) << 1) | side);
+      const int64_t other = side == 0 ? m->right[f] : m->left[f];
+      nbr[acc + k] = other < 0 ? -1 : (int32_t)(cb + other);
     }
     acc += m->count[i];
   }
@@ -500,6 +504,7 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
     return set_err(HFV_ERR_INVALID, "internal: slot offsets overlap the next slot");
   HIP_OK(hipMemcpyAsync(F.off + cb, off.data(), sizeof(int32_t) * noff, hipMemcpyHostToDevice, st));
   if (tot) HIP_OK(hipMemcpyAsync(F.conn + kb, conn.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, st));
+  if (tot) HIP_OK(hipMemcpyAsync(F.nbr + kb, nbr.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, st));
   // Particle::update_geometry (part.rs:151-158) for the selected cells
   const int64_t N = c->n;
   std::vector<double> vol(n), cen(3 * n), srad(n);
@@ -706,7 +711,7 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   if (t) prof_begin(c, KS_LIMIT);
   hipLaunchKernelGGL(k_limit, gc, gb, 0, st, B);
   if (t) prof_end(c, KS_LIMIT);
-  hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), gb, 0, st, B);
+  hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, st, B);
 }
 
 int launch_step(hfv_ctx* c) {
@@ -1021,7 +1026,7 @@ int hfv_timesteps(hfv_ctx* c, uint64_t* ti_next) {
   hipLaunchKernelGGL(k_grad, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B, 0, 1);
   hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B);
   hipLaunchKernelGGL(k_limit, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B);
-  hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, c->stream, B);
   HIP_OK(hipGetLastError());
   TRY(sync_counters(c));
   if (c->heng[0].err) return set_err(c->heng[0].err, "engine error in this step");

@@ -14,7 +14,8 @@
 //   faces, SoA: area, nrm[3], fcen[3], left (int32), right (int32, -1 = wall), shc (uint8 periodic
 //     shift code, 2 bits per axis), cur (uint8: record of this step); rec[F][8] (AoS, one 64-byte line
 //     per face: dt*F[5], v_max, a_over_r, unused)
-//   csr: off[N+1] (int32), conn[2F] (int32: face << 1 | side, side 1 = the cell is the face's right)
+//   csr: off[N+1] (int32), conn[2F] (int32: face << 1 | side, side 1 = the cell is the face's right),
+//     nbr[2F] (int32: the face's other cell, -1 for a wall)
 //   block tables: one {sim, lo, hi, chunk} per thread block of the cell kernels and of the face
 //     kernels.  A block only holds items of one simulation, so configuration and engine loads are
 //     uniform (scalar) and a finished simulation's blocks exit at once.  Blocks are ordered so that
@@ -43,6 +44,7 @@ struct Faces {
   int64_t nf;
   double *area, *nrm, *fcen, *rec;
   int32_t *left, *right, *off, *conn;
+  int32_t* nbr;  // per CSR entry: the face's other cell (-1: wall)
   uint8_t *shc, *cur;
 };
 
@@ -236,6 +238,25 @@ __device__ __forceinline__ V3<double> decode_shift(uint8_t code, const double* b
 // issue their gathers stage by stage, so that a stage's loads are not all in flight (and holding
 // registers) at once.
 #define HFV_STAGE() asm volatile("" ::: "memory")
+// The flux kernel's stage barriers can be compiled out (HFV_FLUX_STAGES 0) to let the scheduler overlap
+// the stages' gathers at the cost of registers.
+#ifndef HFV_FLUX_STAGES
+#define HFV_FLUX_STAGES 1
+#endif
+#if HFV_FLUX_STAGES
+#define HFV_FSTAGE() HFV_STAGE()
+#else
+#define HFV_FSTAGE() \
+  do {               \
+  } while (0)
+#endif
+// Minimum waves per SIMD the compiler must fit the flux / gradient kernels into (0: its choice).
+#ifndef HFV_FLUX_WAVES
+#define HFV_FLUX_WAVES 0
+#endif
+#ifndef HFV_GRAD_WAVES
+#define HFV_GRAD_WAVES 0
+#endif
 
 // Hard errors are recorded for the batch and for the simulation that raised them; a batch keeps
 // stepping its other simulations (the failed one stops at its next k_begin).
@@ -348,27 +369,67 @@ __global__ void k_begin(Batch B) {
   E.mab = get_max_active_bin(E.ti_current);
 }
 
-// Engine::step / queue_step / update_ti_next (engine.rs:194-268).  One block per simulation: sums the
-// chunk partials of this step, then one thread moves the engine to its next sync point.
-__global__ void __launch_bounds__(HFV_BLOCK) k_end(Batch B) {
+// Engine::step / queue_step / update_ti_next (engine.rs:194-268).  One block (HFV_END_BLOCK threads) per
+// simulation sums the chunk partials of this step (eight loads in flight per thread), then one thread
+// moves the engine to its next sync point.
+#define HFV_END_BLOCK 1024
+__global__ void __launch_bounds__(HFV_END_BLOCK) k_end(Batch B) {
+  __shared__ uint32_t sa[HFV_END_BLOCK / 64], sf[HFV_END_BLOCK / 64];
+  __shared__ uint64_t st[HFV_END_BLOCK / 64];
   const int s = blockIdx.x;
   SimEngine& E = B.eng[s];
   if (!E.stepping) return;
   const SimDesc& D = B.sims[s];
+  const int T = blockDim.x;
   uint32_t na = 0, nfc = 0;
   uint64_t tmin = ~0ull;
-  for (int k = D.cchunk0 + (int)threadIdx.x; k < D.cchunk1; k += blockDim.x) {
-    na += B.P.act[k];
-    uint64_t t = B.P.ti[k];
-    tmin = t < tmin ? t : tmin;
+  for (int k0 = D.cchunk0 + (int)threadIdx.x; k0 < D.cchunk1; k0 += 8 * T) {
+    uint32_t a[8];
+    uint64_t t[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 + u * T;
+      a[u] = k < D.cchunk1 ? B.P.act[k] : 0u;
+      t[u] = k < D.cchunk1 ? B.P.ti[k] : ~0ull;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      na += a[u];
+      tmin = t[u] < tmin ? t[u] : tmin;
+    }
   }
-  for (int k = D.fchunk0 + (int)threadIdx.x; k < D.fchunk1; k += blockDim.x) nfc += B.P.face[k];
-  na = block_sum_u32(na);
+  for (int k0 = D.fchunk0 + (int)threadIdx.x; k0 < D.fchunk1; k0 += 8 * T) {
+    uint32_t a[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int k = k0 + u * T;
+      a[u] = k < D.fchunk1 ? B.P.face[k] : 0u;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) nfc += a[u];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    na += __shfl_xor(na, off);
+    nfc += __shfl_xor(nfc, off);
+    uint64_t o = __shfl_xor(tmin, off);
+    tmin = o < tmin ? o : tmin;
+  }
+  if (__lane_id() == 0) {
+    sa[threadIdx.x >> 6] = na;
+    sf[threadIdx.x >> 6] = nfc;
+    st[threadIdx.x >> 6] = tmin;
+  }
   __syncthreads();
-  nfc = block_sum_u32(nfc);
-  __syncthreads();
-  tmin = block_min_u64(tmin);
   if (threadIdx.x != 0) return;
+  na = 0;
+  nfc = 0;
+  tmin = ~0ull;
+  for (int w = 0; w < (T >> 6); ++w) {
+    na += sa[w];
+    nfc += sf[w];
+    tmin = st[w] < tmin ? st[w] : tmin;
+  }
   E.n_active = na;
   E.n_face_active = nfc;
   E.ti_min = tmin;
@@ -712,7 +773,7 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
     const double dtr = R.dt();
     if ((!la && !ra) || (dtr < dtl && !ra) || (dtl < dtr && !la)) return false;
     const double dt = hmin(dtl, dtr);
-    HFV_STAGE();
+    HFV_FSTAGE();
     // ---- geometry and the two reconstructed sides
     const V3<double> fc = ld3(F.fcen, nf, f);
     const V3<double> sh = decode_shift(F.shc[f], cfg.box);
@@ -720,16 +781,16 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
     const V3<double> dx_left = fc - L.gc();
     const St<T> plT = cvtst<T>(L.prim());
     const St<T> left_dash = face_side<T>(L, plT, cvt3<T>(dx_left), dte);
-    HFV_STAGE();
+    HFV_FSTAGE();
     const V3<double> dx_right = (fc - R.gc()) - sh;
     const St<T> prT = cvtst<T>(R.prim());
     const St<T> right_dash = face_side<T>(R, prT, cvt3<T>(dx_right), dte);
-    HFV_STAGE();
+    HFV_FSTAGE();
     // ---- pairwise limiter (gradients.rs:105-152)
     const T dxg = (T)len(dx_left - dx_right);
     const St<T> prim_left = pairwise_limiter(plT, prT, left_dash, len(cvt3<T>(dx_left)) / dxg, ev);
     const St<T> prim_right = pairwise_limiter(prT, plT, right_dash, len(cvt3<T>(dx_right)) / dxg, ev);
-    HFV_STAGE();
+    HFV_FSTAGE();
     // ---- signal speed (flux.rs:43-50), face velocity (flux.rs:75-78), a_over_r: FP64
     double v_max = 0.;
     {
@@ -738,7 +799,7 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
       if (R.rho() > 0.) v_max += R.cs();
       v_max -= hmin(dot(R.pvel() - L.pvel(), dx_centroid), 0.);
     }
-    HFV_STAGE();
+    HFV_FSTAGE();
     V3<T> vfT;
     double len_dx;
     {
@@ -750,7 +811,7 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
       vfT = cvt3<T>((0.5 * (vl + vr)) - (fac * dx));
       len_dx = len(dx);
     }
-    HFV_STAGE();
+    HFV_FSTAGE();
     // ---- Riemann problem in the face frame
     const V3<double> nrm = ld3(F.nrm, nf, f);
     const double area = F.area[f];
@@ -824,7 +885,7 @@ __device__ __forceinline__ void store_rec(double* rec, int64_t f, const double* 
 // Faces with an active cell get their record (zero when gated off) and cur = 1; the others cur = 0.
 // With SOLVER == AIRS, faces that need the exact solver are queued for k_flux_fallback.
 template <class T, int SOLVER, bool PACKED>
-__global__ void __launch_bounds__(HFV_BLOCK) k_flux(Batch B) {
+__global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   const BlockJob J = B.fjobs[blockIdx.x];
   if (J.sim < 0) return;
   const int s = J.sim;
@@ -832,20 +893,38 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_flux(Batch B) {
   const Cfg& cfg = B.sims[s].cfg;
   const int64_t f = (int64_t)J.lo + threadIdx.x;
   constexpr bool DEFER = SOLVER == SOLVER_AIRS;
+  __shared__ double2 recs[HFV_BLOCK * 4];  // the block's records, written out as whole lines
   bool computed = false;
   Events ev;
+  double outv[7];
+#pragma unroll
+  for (int k = 0; k < 7; ++k) outv[k] = 0.;
   if (f < J.hi) {
     const int32_t l = B.F.left[f], r = B.F.right[f];
     const bool la = B.C.act[l], ra = r >= 0 && B.C.act[r];
     const bool cur = la || ra;
     B.F.cur[f] = cur ? 1 : 0;
     if (cur) {
-      double outv[7];
       if (PACKED)
         computed = face_flux<T, SOLVER, DEFER, PackCell>(B, cfg, f, l, r, la, ra, outv, ev);
       else
         computed = face_flux<T, SOLVER, DEFER, SoaCell>(B, cfg, f, l, r, la, ra, outv, ev);
-      if (!ev.deferred) store_rec(B.F.rec, f, outv);
+    }
+  }
+  // faces without an active cell and deferred faces get a zero record here (the first is never read,
+  // the second is overwritten by k_flux_fallback)
+  recs[4 * threadIdx.x + 0] = make_double2(outv[0], outv[1]);
+  recs[4 * threadIdx.x + 1] = make_double2(outv[2], outv[3]);
+  recs[4 * threadIdx.x + 2] = make_double2(outv[4], outv[5]);
+  recs[4 * threadIdx.x + 3] = make_double2(outv[6], 0.);
+  __syncthreads();
+  {
+    double2* dst = (double2*)(B.F.rec + 8 * (uint64_t)J.lo);
+    const int nvalid = 4 * (J.hi - J.lo);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int c = q * HFV_BLOCK + (int)threadIdx.x;
+      if (c < nvalid) dst[c] = recs[c];
     }
   }
   // deferred faces: wave-aggregated append (rare)
@@ -1083,7 +1162,7 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
       const double area = ldo(F.area, f);
       const V3<double> sh = decode_shift(F.shc[f], cfg.box);
       const V3<double> shift = side == 0 ? sh : -sh;
-      const int32_t other = side == 0 ? F.right[f] : F.left[f];
+      const int32_t other = F.nbr[k];
       V3<double> oc;
       St<double> op;
       if (other >= 0) {
@@ -1235,7 +1314,7 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
   *dti_out = dti;
 }
 
-__global__ void __launch_bounds__(HFV_BLOCK) k_grad(Batch B, int do_grad, int do_ts) {
+__global__ void __launch_bounds__(HFV_BLOCK, HFV_GRAD_WAVES) k_grad(Batch B, int do_grad, int do_ts) {
   const BlockJob J = B.cjobs[blockIdx.x];
   if (J.sim < 0) return;
   const int s = J.sim;
@@ -1275,9 +1354,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_wakeup(Batch B) {
   int8_t w = C.bin[i];
   const int32_t k0 = F.off[i], k1 = F.off[i + 1];
   for (int32_t e = k0; e < k1; ++e) {
-    const int32_t c = F.conn[e];
-    const uint32_t f = (uint32_t)(c >> 1);
-    const int32_t other = (c & 1) == 0 ? F.right[f] : F.left[f];
+    const int32_t other = F.nbr[e];
     if (other < 0) continue;
     if (!C.act[other]) continue;
     int8_t cand = (int8_t)(C.bin[other] + 2);

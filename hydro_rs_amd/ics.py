@@ -28,12 +28,27 @@ class Problem:
         return self.mass.size
 
 
-def jittered_grid(dim, side, box, jitter, rng):
-    """Generators at cell centres of a side^dim grid, each coordinate jittered by U(-jitter, jitter)*dx."""
+def morton_order(idx):
+    """Permutation sorting integer grid coordinates [m, dim] by their interleaved-bit (Z-order) code."""
+    idx = np.asarray(idx, dtype=np.uint64)
+    dim = idx.shape[1]
+    code = np.zeros(idx.shape[0], dtype=np.uint64)
+    for b in range(21):
+        for a in range(dim):
+            code |= ((idx[:, a] >> np.uint64(b)) & np.uint64(1)) << np.uint64(dim * b + (dim - 1 - a))
+    return np.argsort(code, kind="stable")
+
+
+def jittered_grid(dim, side, box, jitter, rng, order="grid"):
+    """Generators at cell centres of a side^dim grid, each coordinate jittered by U(-jitter, jitter)*dx.
+    order "grid": row-major (x slowest); "morton": the same generators in Z-order, so that cells close
+    in space are close in index (neighbour gathers stay within a small index window)."""
     box = np.asarray(box, dtype=np.float64)
     idx = np.stack(np.meshgrid(*[np.arange(side)] * dim, indexing="ij"), -1).reshape(-1, dim)
     dx = box[:dim] / side
     pos = (idx + 0.5 + rng.uniform(-jitter, jitter, idx.shape)) * dx
+    if order == "morton":
+        pos = pos[morton_order(idx)]
     loc = np.zeros((pos.shape[0], 3))
     loc[:, :dim] = pos
     return loc
@@ -84,11 +99,11 @@ def sedov_2d(side=256, seed=2, t_end=0.05, solver=HLLC, threads=0, gamma=5.0 / 3
     return prob
 
 
-def kelvin_helmholtz_3d(side=128, seed=3, t_end=1.0, solver=AIRS, threads=0, **kw):
+def kelvin_helmholtz_3d(side=128, seed=3, t_end=1.0, solver=AIRS, threads=0, order="morton", **kw):
     """3D Kelvin-Helmholtz shear layer in the unit periodic box (gamma 5/3, P 2.5)."""
     rng = np.random.default_rng(seed)
     box = (1.0, 1.0, 1.0)
-    loc = jittered_grid(3, side, box, 0.1, rng)
+    loc = jittered_grid(3, side, box, 0.1, rng, order=order)
     n = loc.shape[0]
     band = np.abs(loc[:, 1] - 0.5) < 0.25
     rho = np.where(band, 2.0, 1.0)
