@@ -81,6 +81,96 @@ def _l1(a, b):
     return float(np.mean(np.abs(np.asarray(a) - np.asarray(b))))
 
 
+# ------------------------------------------------------------------ rocprofv3 summaries
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def _counter_rows(d):
+    import csv
+    import glob
+
+    rows = []
+    for path in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+        rows += list(csv.DictReader(open(path)))
+    return rows
+
+
+def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, label, command, copy_mib):
+    """profiles/<label>_pmc.md: HBM bytes per dispatch of every kernel from a FETCH_SIZE run and a
+    WRITE_SIZE run (KiB per dispatch), each scaled by what the counter reads for a plain device copy of
+    copy_mib MiB (true bytes / counted bytes), plus a TRAFFIC_JSON line that bench.py reads back."""
+    def per_kernel(d, counter):
+        acc = {}
+        for r in _counter_rows(d):
+            if r["Counter_Name"] != counter:
+                continue
+            acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]) * 1024.0)
+        return acc
+
+    true_bytes = copy_mib * 2 ** 20
+    cf = per_kernel(copy_fetch_dir, "FETCH_SIZE")
+    cw = per_kernel(copy_write_dir, "WRITE_SIZE")
+    big = lambda acc: max(v for vals in acc.values() for v in vals)  # the copy dispatch of copy_mib MiB
+    f_scale = true_bytes / big(cf)
+    w_scale = true_bytes / big(cw)
+    fetch = per_kernel(fetch_dir, "FETCH_SIZE")
+    write = per_kernel(write_dir, "WRITE_SIZE")
+    table = {}
+    for k in fetch:
+        if k not in write or not k.startswith(("void hfv::", "hfv::")):
+            continue
+        fb = float(np.mean(fetch[k])) * f_scale
+        wb = float(np.mean(write[k])) * w_scale
+        table[k] = {"dispatches": len(fetch[k]), "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb}
+    lines = [f"# HBM traffic per dispatch: {label}", "", "Commands (from /tmp on the GPU box, R = repository root):", ""]
+    lines += [f"    {c}" for c in command]
+    lines += ["", f"Calibration: a {copy_mib} MiB device copy reads FETCH_SIZE x {f_scale:.3f} and WRITE_SIZE x "
+                  f"{w_scale:.3f} of its true bytes; every kernel's counters below are scaled by these factors.", "",
+              "| kernel | dispatches | HBM read MB | HBM write MB | total MB |", "|---|---|---|---|---|"]
+    for k, v in sorted(table.items(), key=lambda kv: -kv[1]["traffic_bytes"]):
+        lines.append(f"| `{k}` | {v['dispatches']} | {v['fetch_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} | "
+                     f"{v['traffic_bytes'] / 1e6:.1f} |")
+    lines += ["", "TRAFFIC_JSON " + json.dumps({"label": label, "kernels": table})]
+    out = os.path.join(PROFILES, f"{label}_pmc.md")
+    open(out, "w").write("\n".join(lines) + "\n")
+    return out
+
+
+def summarize_trace(stats_csv, label, command, note=""):
+    """profiles/<label>_kernel_stats.md from rocprofv3 --kernel-trace --stats (kernel_stats.csv)."""
+    import csv
+
+    rows = list(csv.DictReader(open(stats_csv)))
+    lines = [f"# Kernel stats: {label}", "", "Command (from /tmp on the GPU box, R = repository root):", "",
+             f"    {command}", "", "| kernel | calls | total us | avg us | min us | max us | % |",
+             "|---|---|---|---|---|---|---|"]
+    for r in rows:
+        lines.append(f"| `{r['Name']}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e3:.1f} | "
+                     f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
+                     f"{float(r['Percentage']):.2f} |")
+    if note:
+        lines += ["", note]
+    out = os.path.join(PROFILES, f"{label}_kernel_stats.md")
+    open(out, "w").write("\n".join(lines) + "\n")
+    return out
+
+
+def traffic_for(label, kernel_name):
+    """HBM bytes per dispatch of kernel_name from profiles/<label>_pmc.md (None when not profiled)."""
+    path = os.path.join(PROFILES, f"{label}_pmc.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("TRAFFIC_JSON "):
+            rec = json.loads(line[len("TRAFFIC_JSON "):]).get("kernels", {}).get(kernel_name)
+            return rec["traffic_bytes"] if rec else None
+    return None
+
+
+def flux_kernel_name(fp32, solver, packed=False):
+    return f"void hfv::k_flux<{'float' if fp32 else 'double'}, {solver}, {'true' if packed else 'false'}>(hfv::Batch)"
+
+
 # ------------------------------------------------------------------ ensemble (headline)
 def bench_ensemble(args):
     from hydro_rs_amd import ensemble, fvs, ics
@@ -108,6 +198,9 @@ def bench_ensemble(args):
            "n_all": n_all, "members_done": sum(1 for i in i1 if i["done"]),
            "members_failed": [s for s, i in enumerate(i1) if i["error_code"]],
            "roofline": roofline(kt, launches, cu, ff, n_all)}
+    if res["roofline"]["kernel"] == "flux":
+        res["roofline"]["kernel_name"] = flux_kernel_name(True, fvs.HLLC, b.flux_variant() == 1)
+        res["roofline"]["traffic"] = traffic_for("ensemble512", res["roofline"]["kernel_name"])
     # the same window in FP64 (batched) for the L1 gate
     p64 = [ics.with_precision(p, fvs.FP64) for p in probs]
     b64 = fvs.Batch.for_problems(p64)
@@ -234,6 +327,9 @@ def bench_kh3d(args):
     for k, prec in (("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)):
         d, s = run_device(ics.with_precision(prob, prec), args.kh_warmup, args.kh_steps, profile=True)
         rl = roofline(s["kernel_times"], s["steps"], s["cell_updates"], s["face_fluxes"], prob.n)
+        if rl["kernel"] == "flux":
+            rl["kernel_name"] = flux_kernel_name(prec == fvs.FP32_FLUX, fvs.AIRS)
+            rl["traffic"] = traffic_for(f"kh3d_{args.kh_side}", rl["kernel_name"])
         n_act = s["cell_updates"] / max(s["steps"], 1)
         f_act = s["face_fluxes"] / max(s["steps"], 1)
         step_bytes = 192 * prob.n + 1247 * n_act + 256 * f_act
@@ -306,7 +402,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=float, default=0.0,
                     help="only run a plain device copy of this many MiB (PMC byte-counter calibration)")
+    ap.add_argument("--summarize-pmc", nargs=5, metavar=("FETCH_DIR", "WRITE_DIR", "COPY_FETCH_DIR", "COPY_WRITE_DIR", "LABEL"),
+                    help="write profiles/LABEL_pmc.md from rocprofv3 --pmc output folders (no GPU needed)")
+    ap.add_argument("--summarize-trace", nargs=2, metavar=("KERNEL_STATS_CSV", "LABEL"),
+                    help="write profiles/LABEL_kernel_stats.md from rocprofv3 --stats output (no GPU needed)")
+    ap.add_argument("--command", action="append", default=[], help="command line(s) recorded in a summary")
+    ap.add_argument("--note", default="")
     args = ap.parse_args()
+    if args.summarize_pmc:
+        fd, wd, cfd, cwd, label = args.summarize_pmc
+        print(summarize_pmc(fd, wd, cfd, cwd, label, args.command, args.copy_probe or 1024.0))
+        return
+    if args.summarize_trace:
+        print(summarize_trace(args.summarize_trace[0], args.summarize_trace[1], " ".join(args.command), args.note))
+        return
     if args.copy_probe > 0:
         import torch
 

@@ -755,6 +755,8 @@ __device__ __forceinline__ St<T> face_side(const Acc& X, const St<T>& pT, V3<T> 
 }
 
 // The face's record (dt*F, v_max, a_over_r) when computed; false when gated off (record zero).
+// Interior and wall faces build the face-frame problem (states, face velocity, normal, dt, area) in
+// their own branch and share one Riemann call, which is inlined (no argument traffic through scratch).
 template <class T, int SOLVER, bool DEFER, class Acc>
 __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_t f, int32_t l, int32_t r, bool la,
                                           bool ra, double* outv, Events& ev) {
@@ -767,12 +769,15 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
   for (int k = 0; k < 7; ++k) outv[k] = 0.;
   const Acc L(B.C, l);
   const double dtl = L.dt();
+  St<T> WL, WR;  // face-frame states
+  V3<T> vfT;     // face velocity
+  double dt, v_max = 0., len_dx;
   if (r >= 0) {
     const Acc R(B.C, r);
     // ---- gate
     const double dtr = R.dt();
     if ((!la && !ra) || (dtr < dtl && !ra) || (dtl < dtr && !la)) return false;
-    const double dt = hmin(dtl, dtr);
+    dt = hmin(dtl, dtr);
     HFV_FSTAGE();
     // ---- geometry and the two reconstructed sides
     const V3<double> fc = ld3(F.fcen, nf, f);
@@ -788,11 +793,10 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
     HFV_FSTAGE();
     // ---- pairwise limiter (gradients.rs:105-152)
     const T dxg = (T)len(dx_left - dx_right);
-    const St<T> prim_left = pairwise_limiter(plT, prT, left_dash, len(cvt3<T>(dx_left)) / dxg, ev);
-    const St<T> prim_right = pairwise_limiter(prT, plT, right_dash, len(cvt3<T>(dx_right)) / dxg, ev);
+    WL = pairwise_limiter(plT, prT, left_dash, len(cvt3<T>(dx_left)) / dxg, ev);
+    WR = pairwise_limiter(prT, plT, right_dash, len(cvt3<T>(dx_right)) / dxg, ev);
     HFV_FSTAGE();
     // ---- signal speed (flux.rs:43-50), face velocity (flux.rs:75-78), a_over_r: FP64
-    double v_max = 0.;
     {
       const V3<double> dx_centroid = (R.cen() + sh) - L.cen();
       if (L.rho() > 0.) v_max += L.cs();
@@ -800,8 +804,6 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
       v_max -= hmin(dot(R.pvel() - L.pvel(), dx_centroid), 0.);
     }
     HFV_FSTAGE();
-    V3<T> vfT;
-    double len_dx;
     {
       const V3<double> locl = L.loc(), locr = R.loc();
       const V3<double> dx = (locr + sh) - locl;
@@ -812,66 +814,60 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
       len_dx = len(dx);
     }
     HFV_FSTAGE();
-    // ---- Riemann problem in the face frame
+    WL = boost(WL, -vfT);
+    WR = boost(WR, -vfT);
+  } else {
+    // ---- wall face (flux_exchange_boundary, flux.rs:102-195)
+    if (!la) return false;
+    const V3<double> fc = ld3(F.fcen, nf, f);
     const V3<double> nrm = ld3(F.nrm, nf, f);
-    const double area = F.area[f];
-    St<T> Fl = riemann_flux<T, DEFER>(sc, boost(prim_left, -vfT), boost(prim_right, -vfT), vfT, cvt3<T>(nrm), g,
-                                      cfg.gd, ev);
-    St<double> rec = dt * (area * cvtst<double>(Fl));
-#pragma unroll
-    for (int k = 0; k < 5; ++k) outv[k] = rec.c[k];
-    outv[5] = v_max;
-    outv[6] = area / len_dx;
-    return true;
+    const V3<double> loc = L.loc(), cen = L.cen(), gc = L.gc();
+    // Particle::reflect (part.rs:374-383)
+    const V3<double> loc_r = loc + ((2. * dot(fc - loc, nrm)) * nrm);
+    const V3<double> cen_r = cen + ((2. * dot(fc - cen, nrm)) * nrm);
+    const V3<double> gc_r = gc + ((2. * dot(fc - gc, nrm)) * nrm);
+    const V3<double> dx_face = fc - gc;
+    const double dx_gradients = len(gc_r - gc);
+    const V3<double> dx_centroid = cen_r - cen;
+    len_dx = len(loc_r - loc);
+    const St<double> p = L.prim();
+    dt = dtl;
+    if (p.c[0] > 0.) v_max += L.cs();
+    const St<T> pT = cvtst<T>(p);
+    const V3<T> dxfT = cvt3<T>(dx_face);
+    const V3<T> nT = cvt3<T>(nrm);
+    St<T> dash = face_side<T>(L, pT, dxfT, (T)(-0.5 * dt));
+    const T dx_fac = len(dxfT) / (T)dx_gradients;
+    if (cfg.boundary == BND_REFLECTIVE) {
+      St<T> rp = reflect(pT, nT);
+      dash = pairwise_limiter(pT, rp, dash, dx_fac, ev);
+      St<double> rpd = reflect(p, nrm);
+      v_max -= hmin(dot(vel(rpd) - vel(p), dx_centroid), 0.);
+      WR = reflect(dash, nT);
+    } else if (cfg.boundary == BND_OPEN) {
+      dash = pairwise_limiter(pT, pT, dash, dx_fac, ev);
+      WR = dash;
+    } else if (cfg.boundary == BND_VACUUM) {
+      St<T> vac = vacuum<T>();
+      dash = pairwise_limiter(pT, vac, dash, dx_fac, ev);
+      v_max += hmin(dot(vel(p), dx_centroid), 0.);
+      WR = vac;
+    } else {
+      ev.fail(E_UNSUPPORTED);
+      WR = dash;
+    }
+    WL = dash;
+    vfT = mk3(T(0), T(0), T(0));
   }
-  if (!la) return false;
-  const V3<double> fc = ld3(F.fcen, nf, f);
+  // ---- Riemann problem in the face frame (the wall problem has no boost: v_face = 0)
   const V3<double> nrm = ld3(F.nrm, nf, f);
   const double area = F.area[f];
-  const V3<double> loc = L.loc(), cen = L.cen(), gc = L.gc();
-  // Particle::reflect (part.rs:374-383)
-  const V3<double> loc_r = loc + ((2. * dot(fc - loc, nrm)) * nrm);
-  const V3<double> cen_r = cen + ((2. * dot(fc - cen, nrm)) * nrm);
-  const V3<double> gc_r = gc + ((2. * dot(fc - gc, nrm)) * nrm);
-  const V3<double> dx_face = fc - gc;
-  const double dx_gradients = len(gc_r - gc);
-  const V3<double> dx_centroid = cen_r - cen;
-  const V3<double> dx = loc_r - loc;
-  const St<double> p = L.prim();
-  const double dtp = dtl;
-  double v_max = 0.;
-  if (p.c[0] > 0.) v_max += L.cs();
-  const St<T> pT = cvtst<T>(p);
-  const V3<T> dxfT = cvt3<T>(dx_face);
-  const V3<T> nT = cvt3<T>(nrm);
-  St<T> dash = face_side<T>(L, pT, dxfT, (T)(-0.5 * dtp));
-  const T dx_fac = len(dxfT) / (T)dx_gradients;
-  St<T> bstate;
-  if (cfg.boundary == BND_REFLECTIVE) {
-    St<T> rp = reflect(pT, nT);
-    dash = pairwise_limiter(pT, rp, dash, dx_fac, ev);
-    St<double> rpd = reflect(p, nrm);
-    v_max -= hmin(dot(vel(rpd) - vel(p), dx_centroid), 0.);
-    bstate = reflect(dash, nT);
-  } else if (cfg.boundary == BND_OPEN) {
-    dash = pairwise_limiter(pT, pT, dash, dx_fac, ev);
-    bstate = dash;
-  } else if (cfg.boundary == BND_VACUUM) {
-    St<T> vac = vacuum<T>();
-    dash = pairwise_limiter(pT, vac, dash, dx_fac, ev);
-    v_max += hmin(dot(vel(p), dx_centroid), 0.);
-    bstate = vac;
-  } else {
-    ev.fail(E_UNSUPPORTED);
-    bstate = dash;
-  }
-  St<T> Fl = riemann_flux<T, DEFER>(sc, dash, bstate, mk3(T(0), T(0), T(0)), nT, g, cfg.gd, ev);
-  St<double> fluxes = area * cvtst<double>(Fl);
-  St<double> rec = dtp * fluxes;
+  const St<T> Fl = riemann_flux<T, DEFER>(sc, WL, WR, vfT, cvt3<T>(nrm), g, cfg.gd, ev);
+  const St<double> rec = dt * (area * cvtst<double>(Fl));
 #pragma unroll
   for (int k = 0; k < 5; ++k) outv[k] = rec.c[k];
   outv[5] = v_max;
-  outv[6] = area / len(dx);
+  outv[6] = area / len_dx;
   return true;
 }
 
@@ -894,7 +890,9 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   const int64_t f = (int64_t)J.lo + threadIdx.x;
   constexpr bool DEFER = SOLVER == SOLVER_AIRS;
   __shared__ double2 recs[HFV_BLOCK * 4];  // the block's records, written out as whole lines
+  __shared__ uint8_t curs[HFV_BLOCK];
   bool computed = false;
+  bool cur = false;
   Events ev;
   double outv[7];
 #pragma unroll
@@ -902,7 +900,7 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   if (f < J.hi) {
     const int32_t l = B.F.left[f], r = B.F.right[f];
     const bool la = B.C.act[l], ra = r >= 0 && B.C.act[r];
-    const bool cur = la || ra;
+    cur = la || ra;
     B.F.cur[f] = cur ? 1 : 0;
     if (cur) {
       if (PACKED)
@@ -911,8 +909,8 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
         computed = face_flux<T, SOLVER, DEFER, SoaCell>(B, cfg, f, l, r, la, ra, outv, ev);
     }
   }
-  // faces without an active cell and deferred faces get a zero record here (the first is never read,
-  // the second is overwritten by k_flux_fallback)
+  // records of faces with an active cell (deferred ones zero here, overwritten by k_flux_fallback)
+  curs[threadIdx.x] = cur ? 1 : 0;
   recs[4 * threadIdx.x + 0] = make_double2(outv[0], outv[1]);
   recs[4 * threadIdx.x + 1] = make_double2(outv[2], outv[3]);
   recs[4 * threadIdx.x + 2] = make_double2(outv[4], outv[5]);
@@ -924,7 +922,7 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int c = q * HFV_BLOCK + (int)threadIdx.x;
-      if (c < nvalid) dst[c] = recs[c];
+      if (c < nvalid && curs[c >> 2]) dst[c] = recs[c];
     }
   }
   // deferred faces: wave-aggregated append (rare)

@@ -283,7 +283,7 @@ __device__ __forceinline__ St<T> sample_middle_state(T rho, T u, T p, const St<T
   return mkst(rho, vel(s) + ((u - v) * n), p);
 }
 template <class T>
-__device__ __noinline__ St<T> sample_from_star(const Star<T>& star, const St<T>& left, const St<T>& right, T v_l,
+__device__ __forceinline__ St<T> sample_from_star(const Star<T>& star, const St<T>& left, const St<T>& right, T v_l,
                                                T v_r, T a_l, T a_r, V3<T> n, const Gas<T>& g) {
   if (star.u < T(0)) {
     if (star.p > right.c[4]) {  // right shock
@@ -321,8 +321,9 @@ __device__ __forceinline__ bool is_vacuum(const St<T>& left, const St<T>& right,
   return false;
 }
 template <class T>
-__device__ __noinline__ St<T> vacuum_sample(const St<T>& left, const St<T>& right, T v_l, T v_r, T a_l, T a_r,
-                                            V3<T> n, const Gas<T>& g) {
+// Out of line (rare path); states by value so that a call passes them in registers, not via scratch.
+__device__ __noinline__ St<T> vacuum_sample(St<T> left, St<T> right, T v_l, T v_r, T a_l, T a_r, V3<T> n,
+                                            const Gas<T>& g) {
   bool right_vacuum, left_vacuum;
   T s_l = T(0), s_r = T(0);
   if (right.c[0] == T(0)) {
@@ -407,8 +408,8 @@ __device__ __forceinline__ T ex_guess_p(const St<T>& l, const St<T>& r, T v_l, T
   return hmax(p_guess, T(1e-8));
 }
 template <class T>
-__device__ __noinline__ T ex_solve_brent(T lower, T upper, T low_f, T up_f, T tol, const St<T>& l, const St<T>& r,
-                                         T v_l, T v_r, T a_l, T a_r, const Gas<T>& g, Events& ev) {
+__device__ __noinline__ T ex_solve_brent(T lower, T upper, T low_f, T up_f, T tol, St<T> l, St<T> r, T v_l, T v_r,
+                                         T a_l, T a_r, const Gas<T>& g, Events& ev) {
   T a = lower, b = upper, c, d = T(INFINITY);
   T fa = low_f, fb = up_f, fc;
   T s, fs;
@@ -481,8 +482,8 @@ __device__ __forceinline__ Star<T> star_from_pstar(T pstar, const St<T>& l, cons
 }
 // exact.rs:242-290
 template <class T>
-__device__ __noinline__ Star<T> exact_star(const St<T>& l, const St<T>& r, T v_l, T v_r, T a_l, T a_r,
-                                           const Gas<T>& g, Events& ev) {
+__device__ __noinline__ Star<T> exact_star(St<T> l, St<T> r, T v_l, T v_r, T a_l, T a_r, const Gas<T>& g,
+                                           Events& ev) {
   T p = T(0);
   T p_guess = ex_guess_p(l, r, v_l, v_r, a_l, a_r, g);
   T fp = ex_f(p, l, r, v_l, v_r, a_l, a_r, g);
@@ -602,7 +603,7 @@ __device__ __forceinline__ Star<T> star_for(int kind, double threshold, const St
 // (ev.deferred is set); the caller hands it to a separate pass, so that this code path carries no
 // exact-solver registers.
 template <class T, bool DEFER = false>
-__device__ St<T> riemann_flux(const SolverCfg& sc, const St<T>& left, const St<T>& right, V3<T> iv, V3<T> n,
+__device__ __forceinline__ St<T> riemann_flux(const SolverCfg& sc, const St<T>& left, const St<T>& right, V3<T> iv, V3<T> n,
                               const Gas<T>& g, const Gas<double>& gd, Events& ev) {
   if (sc.kind == SOLVER_LINADV) {  // linear_advection.rs:17-33
     V3<T> velo = mk3((T)sc.linv[0], (T)sc.linv[1], (T)sc.linv[2]);
