@@ -167,8 +167,9 @@ def traffic_for(label, kernel_name):
     return None
 
 
-def flux_kernel_name(fp32, solver, packed=False):
-    return f"void hfv::k_flux<{'float' if fp32 else 'double'}, {solver}, {'true' if packed else 'false'}>(hfv::Batch)"
+def flux_kernel_name(fp32, solver, static_geometry=True):
+    return (f"void hfv::k_flux<{'float' if fp32 else 'double'}, {solver}, "
+            f"{'true' if static_geometry else 'false'}>(hfv::Batch)")
 
 
 # ------------------------------------------------------------------ ensemble (headline)
@@ -199,7 +200,7 @@ def bench_ensemble(args):
            "members_failed": [s for s, i in enumerate(i1) if i["error_code"]],
            "roofline": roofline(kt, launches, cu, ff, n_all)}
     if res["roofline"]["kernel"] == "flux":
-        res["roofline"]["kernel_name"] = flux_kernel_name(True, fvs.HLLC, b.flux_variant() == 1)
+        res["roofline"]["kernel_name"] = flux_kernel_name(True, fvs.HLLC, True)
         res["roofline"]["traffic"] = traffic_for("ensemble512", res["roofline"]["kernel_name"])
     # the same window in FP64 (batched) for the L1 gate
     p64 = [ics.with_precision(p, fvs.FP64) for p in probs]
@@ -328,7 +329,7 @@ def bench_kh3d(args):
         d, s = run_device(ics.with_precision(prob, prec), args.kh_warmup, args.kh_steps, profile=True)
         rl = roofline(s["kernel_times"], s["steps"], s["cell_updates"], s["face_fluxes"], prob.n)
         if rl["kernel"] == "flux":
-            rl["kernel_name"] = flux_kernel_name(prec == fvs.FP32_FLUX, fvs.AIRS)
+            rl["kernel_name"] = flux_kernel_name(prec == fvs.FP32_FLUX, fvs.AIRS, True)
             rl["traffic"] = traffic_for(f"kh3d_{args.kh_side}", rl["kernel_name"])
         n_act = s["cell_updates"] / max(s["steps"], 1)
         f_act = s["face_fluxes"] / max(s["steps"], 1)

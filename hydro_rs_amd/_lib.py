@@ -109,8 +109,8 @@ HFV_SYMBOLS = {
     "hfv_batch_get_engine_info": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(EngineInfo)]),
     "hfv_batch_get_state": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(State)]),
     "hfv_batch_num_slots": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
-    "hfv_set_flux_variant": (C.c_int, [C.c_void_p, C.c_int]),
-    "hfv_get_flux_variant": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
+    "hfv_set_geometry_mode": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfv_get_geometry_mode": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "hfv_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "hfv_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTimes)]),
 }

@@ -143,8 +143,9 @@ struct hfv_ctx {
   BlockJob *dcjobs = nullptr, *dfjobs = nullptr;
   Partials P{};
   int2* fall = nullptr;      // exact-solver fallback queue (AIRS)
-  bool pack = false;         // packed flux records (hfv_set_flux_variant)
-  double* fp = nullptr;      // their storage, allocated on first use
+  bool use_static = true;    // precomputed geometry for fixed meshes (hfv_set_geometry_mode)
+  std::vector<int> v_zero;   // per slot: every particle velocity is +0 (fixed meshes keep it so)
+  bool any_static = false, any_dynamic = false, any_static_grad = false, any_dynamic_grad = false;
   SimDesc* ddesc = nullptr;
   SimEngine* deng = nullptr;
   SimEngine* heng = nullptr;  // pinned mirror
@@ -182,7 +183,6 @@ struct hfv_ctx {
     B.P = P;
     B.cnt = dcnt;
     B.fall = fall;
-    B.pack = pack ? 1 : 0;
     return B;
   }
 };
@@ -330,9 +330,10 @@ int alloc_batch(hfv_ctx* c) {
   c->P.face = dalloc<uint32_t>(c, nfc);
   c->P.ti = dalloc<uint64_t>(c, ncc);
   c->fall = dalloc<int2>(c, nf);
-  c->fp = nullptr;
-  c->C.fp = nullptr;
-  c->pack = false;
+  C.minv = dalloc<double>(c, 9 * n);
+  F.eg = dalloc<double>(c, 8 * nk);
+  F.fs = dalloc<double>(c, 4 * nf);
+  c->v_zero.assign(S, 1);
   c->ddesc = dalloc<SimDesc>(c, S);
   c->deng = dalloc<SimEngine>(c, S);
   for (void* p : c->bufs)
@@ -421,6 +422,14 @@ void recompute_uniforms(hfv_ctx* c) {
   }
   c->solver = sol < 0 ? -1 : sol;
   c->precision = prec < 0 ? HFV_FP64 : prec;
+  c->any_static = c->any_dynamic = c->any_static_grad = c->any_dynamic_grad = false;
+  for (int s = 0; s < c->nslots; ++s) {
+    if (!c->loaded[s]) continue;
+    if (c->desc[s].static_geom) c->any_static = true;
+    else c->any_dynamic = true;
+    if (c->desc[s].static_grad) c->any_static_grad = true;
+    else c->any_dynamic_grad = true;
+  }
 }
 
 // Upload a mesh into a slot.  update_mask: see hfv_set_mesh.
@@ -572,9 +581,35 @@ int reset_slot(hfv_ctx* c, int s) {
   HIP_OK(hipMemsetAsync(C.wake + cb, 0, cap, st));
   HIP_OK(hipMemsetAsync(C.act + cb, 0, cap, st));
   (void)n;
+  c->v_zero[s] = 1;
   c->heng[s] = fresh_engine(c->params[s]);
   HIP_OK(hipMemcpyAsync(c->deng + s, &c->heng[s], sizeof(SimEngine), hipMemcpyHostToDevice, st));
   HIP_OK(hipStreamSynchronize(st));
+  return HFV_OK;
+}
+
+// Fixed meshes whose particle velocities are all +0 take the static-geometry kernels: their geometric
+// terms are evaluated here once (k_geom_*), after every change of mesh, generators or centroids.
+int refresh_geometry(hfv_ctx* c, int s) {
+  const hfv_params& p = c->params[s];
+  const bool st = c->use_static && p.motion == HFV_FIXED && c->v_zero[s] && c->nused[s] > 0;
+  c->desc[s].static_geom = st ? 1 : 0;
+  c->desc[s].static_grad = st && p.dim == 3 ? 1 : 0;
+  HIP_OK(hipMemcpyAsync(c->ddesc + s, &c->desc[s], sizeof(SimDesc), hipMemcpyHostToDevice, c->stream));
+  if (st) {
+    const int64_t cb = c->cbase[s], n = c->nused[s], fb = c->fbase[s], nf = c->fused[s];
+    Batch B = c->batch_args();
+    if (c->desc[s].static_grad)
+      hipLaunchKernelGGL(k_geom_cells, dim3(grid_full(n)), dim3(BLOCK), 0, c->stream, B, s, cb, cb + n);
+    if (nf > 0) hipLaunchKernelGGL(k_geom_faces, dim3(grid_full(nf)), dim3(BLOCK), 0, c->stream, B, s, fb, fb + nf);
+    HIP_OK(hipGetLastError());
+  }
+  HIP_OK(hipStreamSynchronize(c->stream));
+  const bool prev[4] = {c->any_static, c->any_dynamic, c->any_static_grad, c->any_dynamic_grad};
+  recompute_uniforms(c);
+  if (prev[0] != c->any_static || prev[1] != c->any_dynamic || prev[2] != c->any_static_grad ||
+      prev[3] != c->any_dynamic_grad)
+    drop_graph(c);
   return HFV_OK;
 }
 
@@ -603,7 +638,8 @@ int init_ic(hfv_ctx* c, int s, int64_t n, const double* loc, const double* mass,
   (void)hipFree(dm);
   (void)hipFree(dv);
   (void)hipFree(du);
-  return sync_counters(c);
+  TRY(sync_counters(c));
+  return refresh_geometry(c, s);
 }
 
 inline void prof_begin(hfv_ctx* c, int slot) {
@@ -640,37 +676,46 @@ void prof_next(hfv_ctx* c) {
   prof_take(c, c->ring, true);
 }
 
-template <class T, bool PACKED>
-void launch_flux_tp(hfv_ctx* c, const Batch& B, unsigned g) {
+template <class T, bool STATIC>
+void launch_flux_ts(hfv_ctx* c, const Batch& B, unsigned g) {
   switch (c->solver) {
     case HFV_HLLC:
-      hipLaunchKernelGGL((k_flux<T, SOLVER_HLLC, PACKED>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, SOLVER_HLLC, STATIC>), dim3(g), dim3(BLOCK), 0, c->stream, B);
       break;
     case HFV_AIRS:
-      hipLaunchKernelGGL((k_flux<T, SOLVER_AIRS, PACKED>), dim3(g), dim3(BLOCK), 0, c->stream, B);
-      hipLaunchKernelGGL((k_flux_fallback<T, PACKED>), dim3(256), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, SOLVER_AIRS, STATIC>), dim3(g), dim3(BLOCK), 0, c->stream, B);
       break;
     case HFV_EXACT:
-      hipLaunchKernelGGL((k_flux<T, SOLVER_EXACT, PACKED>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, SOLVER_EXACT, STATIC>), dim3(g), dim3(BLOCK), 0, c->stream, B);
       break;
     default:
-      hipLaunchKernelGGL((k_flux<T, -1, PACKED>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, -1, STATIC>), dim3(g), dim3(BLOCK), 0, c->stream, B);
   }
 }
 
+// Flux kernels of the step: the static-geometry and the general variant each take the blocks of their
+// simulations; AIRS queues exact-solver faces for the fallback pass.
 void launch_flux(hfv_ctx* c, const Batch& B) {
-  unsigned g = (unsigned)c->nfpos;
-  if (c->precision == HFV_FP32_FLUX) {
-    if (c->pack)
-      launch_flux_tp<float, true>(c, B, g);
-    else
-      launch_flux_tp<float, false>(c, B, g);
-  } else {
-    if (c->pack)
-      launch_flux_tp<double, true>(c, B, g);
-    else
-      launch_flux_tp<double, false>(c, B, g);
+  const unsigned g = (unsigned)c->nfpos;
+  const bool f32 = c->precision == HFV_FP32_FLUX;
+  if (c->any_static) {
+    if (f32) launch_flux_ts<float, true>(c, B, g);
+    else launch_flux_ts<double, true>(c, B, g);
   }
+  if (c->any_dynamic) {
+    if (f32) launch_flux_ts<float, false>(c, B, g);
+    else launch_flux_ts<double, false>(c, B, g);
+  }
+  if (c->solver == HFV_AIRS) {
+    if (f32) hipLaunchKernelGGL(k_flux_fallback<float>, dim3(256), dim3(BLOCK), 0, c->stream, B);
+    else hipLaunchKernelGGL(k_flux_fallback<double>, dim3(256), dim3(BLOCK), 0, c->stream, B);
+  }
+}
+
+void launch_grad(hfv_ctx* c, const Batch& B, int do_grad, int do_ts) {
+  const dim3 g((unsigned)c->ncpos), b(BLOCK);
+  if (c->any_static_grad) hipLaunchKernelGGL(k_grad<true>, g, b, 0, c->stream, B, do_grad, do_ts);
+  if (c->any_dynamic_grad) hipLaunchKernelGGL(k_grad<false>, g, b, 0, c->stream, B, do_grad, do_ts);
 }
 
 // The kernels of one batch step (k_begin .. k_end).  Timing events only outside graph capture.
@@ -703,7 +748,7 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   hipLaunchKernelGGL(k_apply, gc, gb, 0, st, B, 7);
   if (t) prof_end(c, KS_APPLY);
   if (t) prof_begin(c, KS_GRAD);
-  hipLaunchKernelGGL(k_grad, gc, gb, 0, st, B, 1, 1);
+  launch_grad(c, B, 1, 1);
   if (t) prof_end(c, KS_GRAD);
   if (t) prof_begin(c, KS_WAKEUP);
   hipLaunchKernelGGL(k_wakeup, gc, gb, 0, st, B);
@@ -815,6 +860,18 @@ int copy_state(hfv_ctx* c, int s, hfv_state* st, bool to_device) {
     HIP_OK(hipMemcpyAsync(to_device ? (void*)(c->C.bin + cb) : (void*)st->timebin,
                           to_device ? (const void*)st->timebin : (const void*)(c->C.bin + cb), (size_t)n, kind, strm));
   HIP_OK(hipStreamSynchronize(strm));
+  if (to_device) {
+    if (st->v) {  // the static geometry needs every particle velocity to be exactly +0
+      int z = 1;
+      for (int64_t k = 0; k < 3 * n && z; ++k) {
+        uint64_t bits;
+        std::memcpy(&bits, st->v + k, sizeof(bits));
+        z = bits == 0;
+      }
+      c->v_zero[s] = z;
+    }
+    if (st->loc || st->centroid || st->v || st->prim) TRY(refresh_geometry(c, s));
+  }
   return HFV_OK;
 }
 
@@ -907,6 +964,8 @@ int hfv_set_mesh(hfv_ctx* c, const hfv_mesh_arrays* m, const uint8_t* update_mas
     c->loaded[0] = 1;
     TRY(write_jobs(c));
     TRY(reset_slot(c, 0));
+  } else {
+    TRY(refresh_geometry(c, 0));
   }
   recompute_uniforms(c);
   drop_graph(c);
@@ -1015,7 +1074,7 @@ int hfv_conserved_to_primitive(hfv_ctx* c) {
 int hfv_gradients(hfv_ctx* c) {
   TRY(require_mesh(c));
   hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 4);
-  hipLaunchKernelGGL(k_grad, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 1, 0);
+  launch_grad(c, c->batch_args(), 1, 0);
   HIP_OK(hipGetLastError());
   return sync_counters(c);
 }
@@ -1023,7 +1082,7 @@ int hfv_gradients(hfv_ctx* c) {
 int hfv_timesteps(hfv_ctx* c, uint64_t* ti_next) {
   TRY(require_mesh(c));
   Batch B = c->batch_args();
-  hipLaunchKernelGGL(k_grad, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B, 0, 1);
+  launch_grad(c, B, 0, 1);
   hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B);
   hipLaunchKernelGGL(k_limit, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B);
   hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, c->stream, B);
@@ -1195,27 +1254,22 @@ int hfv_batch_num_slots(hfv_ctx* c, int* n) {
   return HFV_OK;
 }
 
-// ---- kernel variants
-int hfv_set_flux_variant(hfv_ctx* c, int packed) {
+// ---- geometry mode
+int hfv_set_geometry_mode(hfv_ctx* c, int use_static) {
   if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
-  if (packed != 0 && packed != 1) return set_err(HFV_ERR_INVALID, "variant must be 0 or 1");
+  if (use_static != 0 && use_static != 1) return set_err(HFV_ERR_INVALID, "mode must be 0 or 1");
   HIP_OK(hipSetDevice(c->device));
   HIP_OK(hipStreamSynchronize(c->stream));
-  if (packed && !c->fp) {
-    c->fp = dalloc<double>(c, HFV_FP_STRIDE * c->n);
-    if (!c->fp) return set_err(HFV_ERR_HIP, "hipMalloc failed for the flux records");
-    HIP_OK(hipMemsetAsync(c->fp, 0, sizeof(double) * HFV_FP_STRIDE * (size_t)c->n, c->stream));
-    HIP_OK(hipStreamSynchronize(c->stream));
-    c->C.fp = c->fp;
-  }
-  if (c->pack != (packed != 0)) drop_graph(c);
-  c->pack = packed != 0;
+  c->use_static = use_static != 0;
+  for (int s = 0; s < c->nslots; ++s)
+    if (c->loaded[s]) TRY(refresh_geometry(c, s));
   return HFV_OK;
 }
 
-int hfv_get_flux_variant(hfv_ctx* c, int* packed) {
-  if (!c || !packed) return set_err(HFV_ERR_INVALID, "NULL argument");
-  *packed = c->pack ? 1 : 0;
+int hfv_get_geometry_mode(hfv_ctx* c, int slot, int* is_static) {
+  TRY(check_slot(c, slot));
+  if (!is_static) return set_err(HFV_ERR_INVALID, "NULL argument");
+  *is_static = c->loaded[slot] ? c->desc[slot].static_geom : 0;
   return HFV_OK;
 }
 

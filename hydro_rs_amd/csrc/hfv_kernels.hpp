@@ -35,7 +35,7 @@ struct Cells {
   int64_t n;
   double *prim, *grad, *gc, *extrap, *cons, *facc, *vol, *loc, *cen, *v, *vrel, *maor, *msv, *dt, *srad;
   double *tex, *cs, *pc;
-  double* fp;  // flux records [N][48] (nullptr unless the packed flux variant is in use)
+  double* minv;  // [9][N] inverse of the gradient matrix (static geometry only)
   int8_t *bin, *wake;
   uint8_t* act;
 };
@@ -46,6 +46,8 @@ struct Faces {
   int32_t *left, *right, *off, *conn;
   int32_t* nbr;  // per CSR entry: the face's other cell (-1: wall)
   uint8_t *shc, *cur;
+  double* eg;    // [2F][8] per CSR entry: w, ds[3], dxf[3] (static gradient geometry only)
+  double* fs;    // [F][4] per face: dx_centroid[3], a_over_r (static geometry only)
 };
 
 struct Cfg {
@@ -64,6 +66,8 @@ struct SimDesc {
   int32_t cchunk0, cchunk1, fchunk0, fchunk1;  // chunk ranges of the cell / face block tables
   int moving1d;  // 1D moving mesh rebuilt on the device
   int periodic;
+  int static_geom;  // fixed mesh with zero particle velocities: face geometry precomputed (k_geom_faces)
+  int static_grad;  // ... and gradient geometry too (k_geom_cells; chosen for 3D meshes)
 };
 
 // Thread block -> work: simulation (-1: nothing), items [lo, hi), chunk (index of its partials).
@@ -118,13 +122,11 @@ struct Batch {
   Partials P;
   DevCounters* cnt;
   int2* fall;  // (face, simulation) queued for k_flux_fallback
-  int pack;    // k_mask writes the flux records C.fp
 };
 
 #define HFV_NUM_TIME_BINS 56
 #define HFV_MAX_NR_TIMESTEPS (1ull << 57)
 #define HFV_BLOCK 256
-#define HFV_FP_STRIDE 48  // doubles per cell of the packed flux records
 
 // ------------------------------------------------------------------ timeline (src/timeline.rs)
 __device__ __host__ __forceinline__ uint64_t get_integer_timestep(int bin) {
@@ -582,37 +584,9 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
       // volume += dvdt * dt clamped to [V/2, 2V]: dvdt is never set on the path (part.rs:130-131)
     }
     const double cs = prim.c[0] > 0. ? sound_speed(cfg.gd, prim.c[4], 1. / prim.c[0]) : 0.;
-    if (B.pack && mode != 1) {
-      double2* d = (double2*)(C.fp + HFV_FP_STRIDE * (uint64_t)i);
-      const V3<double> gc = ld3(C.gc, n, i), cen = ld3(C.cen, n, i), loc = ld3(C.loc, n, i), v = ld3(C.v, n, i);
-      d[0] = make_double2(act ? 1. : 0., C.dt[i]);
-      d[1] = make_double2(prim.c[0], prim.c[1]);
-      d[2] = make_double2(prim.c[2], prim.c[3]);
-      d[3] = make_double2(prim.c[4], cs);
-      d[4] = make_double2(gr.g[0].x, gr.g[0].y);
-      d[5] = make_double2(gr.g[0].z, gr.g[1].x);
-      d[6] = make_double2(gr.g[1].y, gr.g[1].z);
-      d[7] = make_double2(gr.g[2].x, gr.g[2].y);
-      d[8] = make_double2(gr.g[2].z, gr.g[3].x);
-      d[9] = make_double2(gr.g[3].y, gr.g[3].z);
-      d[10] = make_double2(gr.g[4].x, gr.g[4].y);
-      d[11] = make_double2(gr.g[4].z, ex.c[0]);
-      d[12] = make_double2(ex.c[1], ex.c[2]);
-      d[13] = make_double2(ex.c[3], ex.c[4]);
-      d[14] = make_double2(rate.c[0], rate.c[1]);
-      d[15] = make_double2(rate.c[2], rate.c[3]);
-      d[16] = make_double2(rate.c[4], gc.x);
-      d[17] = make_double2(gc.y, gc.z);
-      d[18] = make_double2(cen.x, cen.y);
-      d[19] = make_double2(cen.z, loc.x);
-      d[20] = make_double2(loc.y, loc.z);
-      d[21] = make_double2(v.x, v.y);
-      d[22] = make_double2(v.z, 0.);
-      d[23] = make_double2(0., 0.);
-    } else {
-      st5(C.tex, n, i, rate);
-      C.cs[i] = cs;
-    }
+    st5(C.tex, n, i, rate);
+    C.cs[i] = cs;
+
   }
   if (mode != 1) {
     uint32_t cnt = block_sum_u32(act ? 1u : 0u);
@@ -672,12 +646,6 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mesh1d_cells(Batch B) {
   B.C.cen[N + i] = 0.;
   B.C.cen[2 * N + i] = 0.;
   B.C.srad[i] = 1.5 * V;
-  if (B.pack) {
-    double* P = B.C.fp + HFV_FP_STRIDE * (uint64_t)i;
-    P[36] = 0.5 * (xl + xr);
-    P[37] = 0.;
-    P[38] = 0.;
-  }
 }
 
 // ------------------------------------------------------------------ k_flux
@@ -695,10 +663,7 @@ __device__ __forceinline__ const Gas<float>& gas_for<float>(const Cfg& c) {
   return c.gf;
 }
 
-// Per-cell inputs of the flux phase, read either from the SoA fields (SoaCell) or from the cell's
-// 384-byte flux record fp[48] written by k_mask (PackCell: three 128-byte lines per cell instead of
-// one line per field).  Record layout: 0 act, 1 dt, 2-6 prim, 7 cs, 8-22 grad, 23-27 extrap,
-// 28-32 tex, 33-35 gc, 36-38 cen, 39-41 loc, 42-44 v, 45-47 unused.
+// Per-cell inputs of the flux phase (SoA fields).
 struct SoaCell {
   const Cells* C;
   uint32_t i;
@@ -718,24 +683,6 @@ struct SoaCell {
   __device__ __forceinline__ V3<double> loc() const { return ld3(C->loc, C->n, i); }
   __device__ __forceinline__ V3<double> v() const { return ld3(C->v, C->n, i); }
 };
-struct PackCell {
-  const double* P;
-  __device__ __forceinline__ PackCell(const Cells& c, int32_t idx)
-      : P((const double*)__builtin_assume_aligned(c.fp + HFV_FP_STRIDE * (uint64_t)(uint32_t)idx, 128)) {}
-  __device__ __forceinline__ double dt() const { return P[1]; }
-  __device__ __forceinline__ St<double> prim() const { return mkst(P[2], mk3(P[3], P[4], P[5]), P[6]); }
-  __device__ __forceinline__ double rho() const { return P[2]; }
-  __device__ __forceinline__ V3<double> pvel() const { return mk3(P[3], P[4], P[5]); }
-  __device__ __forceinline__ double cs() const { return P[7]; }
-  __device__ __forceinline__ V3<double> grad(int q) const { return mk3(P[8 + 3 * q], P[9 + 3 * q], P[10 + 3 * q]); }
-  __device__ __forceinline__ double ex(int q) const { return P[23 + q]; }
-  __device__ __forceinline__ double tex(int q) const { return P[28 + q]; }
-  __device__ __forceinline__ V3<double> gc() const { return mk3(P[33], P[34], P[35]); }
-  __device__ __forceinline__ V3<double> cen() const { return mk3(P[36], P[37], P[38]); }
-  __device__ __forceinline__ V3<double> loc() const { return mk3(P[39], P[40], P[41]); }
-  __device__ __forceinline__ V3<double> v() const { return mk3(P[42], P[43], P[44]); }
-};
-
 // One side's reconstructed state at the face before limiting (flux.rs:52-73):
 // ((W + grad . dx) + extrapolations) + time_extrapolations(-dt/2), in T.
 template <class T, class Acc>
@@ -757,7 +704,10 @@ __device__ __forceinline__ St<T> face_side(const Acc& X, const St<T>& pT, V3<T> 
 // The face's record (dt*F, v_max, a_over_r) when computed; false when gated off (record zero).
 // Interior and wall faces build the face-frame problem (states, face velocity, normal, dt, area) in
 // their own branch and share one Riemann call, which is inlined (no argument traffic through scratch).
-template <class T, int SOLVER, bool DEFER, class Acc>
+// STATIC (fixed mesh, zero particle velocities): the centroid offset and a_over_r come from the face's
+// precomputed line (same expressions, evaluated once by k_geom_faces) and the face velocity is the
+// value the formula yields for v = 0, namely +0 in every component.
+template <class T, int SOLVER, bool DEFER, bool STATIC, class Acc>
 __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_t f, int32_t l, int32_t r, bool la,
                                           bool ra, double* outv, Events& ev) {
   const Faces& F = B.F;
@@ -771,7 +721,7 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
   const double dtl = L.dt();
   St<T> WL, WR;  // face-frame states
   V3<T> vfT;     // face velocity
-  double dt, v_max = 0., len_dx;
+  double dt, v_max = 0., len_dx = 1., a_over_r = 0.;
   if (r >= 0) {
     const Acc R(B.C, r);
     // ---- gate
@@ -797,21 +747,30 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
     WR = pairwise_limiter(prT, plT, right_dash, len(cvt3<T>(dx_right)) / dxg, ev);
     HFV_FSTAGE();
     // ---- signal speed (flux.rs:43-50), face velocity (flux.rs:75-78), a_over_r: FP64
-    {
-      const V3<double> dx_centroid = (R.cen() + sh) - L.cen();
+    if (STATIC) {
+      const double4 st = *(const double4*)(F.fs + 4 * (uint64_t)f);
       if (L.rho() > 0.) v_max += L.cs();
       if (R.rho() > 0.) v_max += R.cs();
-      v_max -= hmin(dot(R.pvel() - L.pvel(), dx_centroid), 0.);
-    }
-    HFV_FSTAGE();
-    {
-      const V3<double> locl = L.loc(), locr = R.loc();
-      const V3<double> dx = (locr + sh) - locl;
-      const V3<double> vl = L.v(), vr = R.v();
-      const V3<double> midpoint = 0.5 * ((locl + locr) + sh);
-      const double fac = dot(vr - vl, fc - midpoint) / len2(dx);
-      vfT = cvt3<T>((0.5 * (vl + vr)) - (fac * dx));
-      len_dx = len(dx);
+      v_max -= hmin(dot(R.pvel() - L.pvel(), mk3(st.x, st.y, st.z)), 0.);
+      a_over_r = st.w;
+      vfT = mk3(T(0), T(0), T(0));
+    } else {
+      {
+        const V3<double> dx_centroid = (R.cen() + sh) - L.cen();
+        if (L.rho() > 0.) v_max += L.cs();
+        if (R.rho() > 0.) v_max += R.cs();
+        v_max -= hmin(dot(R.pvel() - L.pvel(), dx_centroid), 0.);
+      }
+      HFV_FSTAGE();
+      {
+        const V3<double> locl = L.loc(), locr = R.loc();
+        const V3<double> dx = (locr + sh) - locl;
+        const V3<double> vl = L.v(), vr = R.v();
+        const V3<double> midpoint = 0.5 * ((locl + locr) + sh);
+        const double fac = dot(vr - vl, fc - midpoint) / len2(dx);
+        vfT = cvt3<T>((0.5 * (vl + vr)) - (fac * dx));
+        len_dx = len(dx);
+      }
     }
     HFV_FSTAGE();
     WL = boost(WL, -vfT);
@@ -821,15 +780,23 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
     if (!la) return false;
     const V3<double> fc = ld3(F.fcen, nf, f);
     const V3<double> nrm = ld3(F.nrm, nf, f);
-    const V3<double> loc = L.loc(), cen = L.cen(), gc = L.gc();
+    const V3<double> gc = L.gc();
     // Particle::reflect (part.rs:374-383)
-    const V3<double> loc_r = loc + ((2. * dot(fc - loc, nrm)) * nrm);
-    const V3<double> cen_r = cen + ((2. * dot(fc - cen, nrm)) * nrm);
     const V3<double> gc_r = gc + ((2. * dot(fc - gc, nrm)) * nrm);
     const V3<double> dx_face = fc - gc;
     const double dx_gradients = len(gc_r - gc);
-    const V3<double> dx_centroid = cen_r - cen;
-    len_dx = len(loc_r - loc);
+    V3<double> dx_centroid;
+    if (STATIC) {
+      const double4 st = *(const double4*)(F.fs + 4 * (uint64_t)f);
+      dx_centroid = mk3(st.x, st.y, st.z);
+      a_over_r = st.w;
+    } else {
+      const V3<double> loc = L.loc(), cen = L.cen();
+      const V3<double> loc_r = loc + ((2. * dot(fc - loc, nrm)) * nrm);
+      const V3<double> cen_r = cen + ((2. * dot(fc - cen, nrm)) * nrm);
+      dx_centroid = cen_r - cen;
+      len_dx = len(loc_r - loc);
+    }
     const St<double> p = L.prim();
     dt = dtl;
     if (p.c[0] > 0.) v_max += L.cs();
@@ -867,7 +834,7 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
 #pragma unroll
   for (int k = 0; k < 5; ++k) outv[k] = rec.c[k];
   outv[5] = v_max;
-  outv[6] = area / len_dx;
+  outv[6] = STATIC ? a_over_r : area / len_dx;
   return true;
 }
 
@@ -880,12 +847,12 @@ __device__ __forceinline__ void store_rec(double* rec, int64_t f, const double* 
 
 // Faces with an active cell get their record (zero when gated off) and cur = 1; the others cur = 0.
 // With SOLVER == AIRS, faces that need the exact solver are queued for k_flux_fallback.
-template <class T, int SOLVER, bool PACKED>
+template <class T, int SOLVER, bool STATIC>
 __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   const BlockJob J = B.fjobs[blockIdx.x];
   if (J.sim < 0) return;
   const int s = J.sim;
-  if (!B.eng[s].stepping) return;
+  if (!B.eng[s].stepping || B.sims[s].static_geom != (STATIC ? 1 : 0)) return;
   const Cfg& cfg = B.sims[s].cfg;
   const int64_t f = (int64_t)J.lo + threadIdx.x;
   constexpr bool DEFER = SOLVER == SOLVER_AIRS;
@@ -903,10 +870,7 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
     cur = la || ra;
     B.F.cur[f] = cur ? 1 : 0;
     if (cur) {
-      if (PACKED)
-        computed = face_flux<T, SOLVER, DEFER, PackCell>(B, cfg, f, l, r, la, ra, outv, ev);
-      else
-        computed = face_flux<T, SOLVER, DEFER, SoaCell>(B, cfg, f, l, r, la, ra, outv, ev);
+      computed = face_flux<T, SOLVER, DEFER, STATIC, SoaCell>(B, cfg, f, l, r, la, ra, outv, ev);
     }
   }
   // records of faces with an active cell (deferred ones zero here, overwritten by k_flux_fallback)
@@ -941,7 +905,7 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
 }
 
 // The queued faces of this step with the full AIRS solver (exact fallback), grid-stride.
-template <class T, bool PACKED>
+template <class T>
 __global__ void __launch_bounds__(HFV_BLOCK) k_flux_fallback(Batch B) {
   const unsigned int count = B.cnt->nfall;
   for (unsigned int k = blockIdx.x * blockDim.x + threadIdx.x; k < count; k += gridDim.x * blockDim.x) {
@@ -952,10 +916,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_flux_fallback(Batch B) {
     const bool la = B.C.act[l], ra = r >= 0 && B.C.act[r];
     Events ev;
     double outv[7];
-    if (PACKED)
-      face_flux<T, SOLVER_AIRS, false, PackCell>(B, B.sims[s].cfg, f, l, r, la, ra, outv, ev);
-    else
-      face_flux<T, SOLVER_AIRS, false, SoaCell>(B, B.sims[s].cfg, f, l, r, la, ra, outv, ev);
+    face_flux<T, SOLVER_AIRS, false, false, SoaCell>(B, B.sims[s].cfg, f, l, r, la, ra, outv, ev);
     store_rec(B.F.rec, f, outv);
     flush_events(ev, B.cnt, B.eng, s);
   }
@@ -1111,6 +1072,22 @@ __device__ __forceinline__ double cell_radius(double V, const Cfg& cfg) {
   return pow(cfg.radius3_fac * V, 1. / 3.);
 }
 
+// DMat3::inverse (glam 0.27): cofactors by cross products, det = z . (x cross y), x 1/det, transposed;
+// returned as the three columns of the inverse.
+__device__ __forceinline__ M3 inverse_cols(const M3& m) {
+  V3<double> t0 = cross(m.y, m.z), t1 = cross(m.z, m.x), t2 = cross(m.x, m.y);
+  double det = dot(m.z, t2);
+  double inv_det = 1. / det;
+  V3<double> c0 = mk3(t0.x * inv_det, t0.y * inv_det, t0.z * inv_det);
+  V3<double> c1 = mk3(t1.x * inv_det, t1.y * inv_det, t1.z * inv_det);
+  V3<double> c2 = mk3(t2.x * inv_det, t2.y * inv_det, t2.z * inv_det);
+  M3 inv;
+  inv.x = mk3(c0.x, c1.x, c2.x);
+  inv.y = mk3(c0.y, c1.y, c2.y);
+  inv.z = mk3(c0.z, c1.z, c2.z);
+  return inv;
+}
+
 // Space::get_boundary_part (space.rs:213-229): the mirrored neighbour of a wall face.
 __device__ __forceinline__ void boundary_neighbour(const Faces& F, const Cfg& cfg, int64_t f, const V3<double>& cen,
                                                    const St<double>& prim, V3<double>& oc, St<double>& op) {
@@ -1127,6 +1104,10 @@ __device__ __forceinline__ void boundary_neighbour(const Faces& F, const Cfg& cf
 // fused with compute_timesteps (euler_eqns_fvs.rs:50-61, part.rs:42-84, 328-372) and timesteps_apply
 // (hydrodynamics.rs:145-158, timeline.rs:103-139) for an active cell.  The limiter's neighbour min/max
 // is gathered in the estimate sweep (same values, same order); the second sweep only extrapolates.
+// STATIC (fixed 3D mesh): the per-entry weight w, offset ds and face offset dxf, and the inverse of the
+// gradient matrix, are the values k_geom_cells computed once with the same expressions and order.  (On
+// 1D/2D meshes recomputing them reads fewer bytes than the precomputed lines, so those stay dynamic.)
+template <bool STATIC>
 __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const SimEngine& E, int64_t i, int do_grad,
                                           int do_ts, uint64_t* dti_out, Events& ev) {
   const Cells& C = B.C;
@@ -1154,27 +1135,41 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
     }
     St<double> lmin = prim, lmax = prim;
     for (int32_t k = k0; k < k1; ++k) {
-      const int32_t e = F.conn[k];
-      const uint32_t f = (uint32_t)(e >> 1);
-      const int side = e & 1;
-      const double area = ldo(F.area, f);
-      const V3<double> sh = decode_shift(F.shc[f], cfg.box);
-      const V3<double> shift = side == 0 ? sh : -sh;
       const int32_t other = F.nbr[k];
-      V3<double> oc;
+      V3<double> ds;
+      double w;
       St<double> op;
-      if (other >= 0) {
-        load_pc(C.pc, (uint32_t)other, op, oc);
+      if (STATIC) {
+        const double4 a = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k))[0];
+        w = a.x;
+        ds = mk3(a.y, a.z, a.w);
+        if (other >= 0) {
+          V3<double> oc;
+          load_pc(C.pc, (uint32_t)other, op, oc);
+        } else {
+          op = prim;  // Space::get_boundary_part (space.rs:213-229)
+          if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, ld3(F.nrm, nf, (uint32_t)(F.conn[k] >> 1)));
+          else if (cfg.boundary == BND_VACUUM) op = vacuum<double>();
+        }
       } else {
-        boundary_neighbour(F, cfg, f, cen, prim, oc, op);
+        const int32_t e = F.conn[k];
+        const uint32_t f = (uint32_t)(e >> 1);
+        const V3<double> sh = decode_shift(F.shc[f], cfg.box);
+        const V3<double> shift = (e & 1) == 0 ? sh : -sh;
+        V3<double> oc;
+        if (other >= 0) {
+          load_pc(C.pc, (uint32_t)other, op, oc);
+        } else {
+          boundary_neighbour(F, cfg, f, cen, prim, oc, op);
+        }
+        ds = (oc + shift) - cen;
+        w = ldo(F.area, f) / len2(ds);
+        m.x = m.x + (w * (ds.x * ds));
+        m.y = m.y + (w * (ds.y * ds));
+        m.z = m.z + (w * (ds.z * ds));
       }
-      const V3<double> ds = (oc + shift) - cen;
-      const double w = area / len2(ds);
 #pragma unroll
       for (int q = 0; q < 5; ++q) g.g[q] = g.g[q] + ((w * (op.c[q] - prim.c[q])) * ds);
-      m.x = m.x + (w * (ds.x * ds));
-      m.y = m.y + (w * (ds.y * ds));
-      m.z = m.z + (w * (ds.z * ds));
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
         lmin.c[q] = hmin(lmin.c[q], op.c[q]);
@@ -1184,16 +1179,14 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
     HFV_STAGE();
     // DMat3::inverse (glam 0.27): cofactors by cross products, det = z . (x cross y), transpose
     {
-      V3<double> t0 = cross(m.y, m.z), t1 = cross(m.z, m.x), t2 = cross(m.x, m.y);
-      double det = dot(m.z, t2);
-      double inv_det = 1. / det;
-      V3<double> c0 = mk3(t0.x * inv_det, t0.y * inv_det, t0.z * inv_det);
-      V3<double> c1 = mk3(t1.x * inv_det, t1.y * inv_det, t1.z * inv_det);
-      V3<double> c2 = mk3(t2.x * inv_det, t2.y * inv_det, t2.z * inv_det);
       M3 inv;
-      inv.x = mk3(c0.x, c1.x, c2.x);
-      inv.y = mk3(c0.y, c1.y, c2.y);
-      inv.z = mk3(c0.z, c1.z, c2.z);
+      if (STATIC) {
+        inv.x = mk3(ldo(C.minv, (uint32_t)i), ldo(C.minv + n, (uint32_t)i), ldo(C.minv + 2 * n, (uint32_t)i));
+        inv.y = mk3(ldo(C.minv + 3 * n, (uint32_t)i), ldo(C.minv + 4 * n, (uint32_t)i), ldo(C.minv + 5 * n, (uint32_t)i));
+        inv.z = mk3(ldo(C.minv + 6 * n, (uint32_t)i), ldo(C.minv + 7 * n, (uint32_t)i), ldo(C.minv + 8 * n, (uint32_t)i));
+      } else {
+        inv = inverse_cols(m);
+      }
       bool fin = true;
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
@@ -1219,12 +1212,18 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
       emax.c[q] = -INFINITY;
     }
     for (int32_t k = k0; k < k1; ++k) {
-      const int32_t e = F.conn[k];
-      const uint32_t f = (uint32_t)(e >> 1);
-      const V3<double> sh = decode_shift(F.shc[f], cfg.box);
-      const V3<double> shift = (e & 1) == 0 ? sh : -sh;
-      const V3<double> fc = ld3(F.fcen, nf, f);
-      const St<double> ext = gdot(g, (fc - cen) - shift);
+      V3<double> dxf;
+      if (STATIC) {
+        const double4 b = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k))[1];
+        dxf = mk3(b.x, b.y, b.z);
+      } else {
+        const int32_t e = F.conn[k];
+        const uint32_t f = (uint32_t)(e >> 1);
+        const V3<double> sh = decode_shift(F.shc[f], cfg.box);
+        const V3<double> shift = (e & 1) == 0 ? sh : -sh;
+        dxf = (ld3(F.fcen, nf, f) - cen) - shift;
+      }
+      const St<double> ext = gdot(g, dxf);
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
         emin.c[q] = hmin(emin.c[q], ext.c[q]);
@@ -1312,23 +1311,98 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
   *dti_out = dti;
 }
 
+template <bool STATIC>
 __global__ void __launch_bounds__(HFV_BLOCK, HFV_GRAD_WAVES) k_grad(Batch B, int do_grad, int do_ts) {
   const BlockJob J = B.cjobs[blockIdx.x];
   if (J.sim < 0) return;
   const int s = J.sim;
   const SimEngine& E = B.eng[s];
-  if (!E.stepping) return;
+  if (!E.stepping || B.sims[s].static_grad != (STATIC ? 1 : 0)) return;
   const Cfg& cfg = B.sims[s].cfg;
   const int64_t i = (int64_t)J.lo + threadIdx.x;
   const bool a = i < J.hi && B.C.act[i];
   uint64_t my = ~0ull;
   Events ev;
-  if (a) grad_cell(B, cfg, E, i, do_grad, do_ts, &my, ev);
+  if (a) grad_cell<STATIC>(B, cfg, E, i, do_grad, do_ts, &my, ev);
   if (do_ts) {
     uint64_t t = block_min_u64(my);
     if (threadIdx.x == 0) B.P.ti[J.chunk] = t;
   }
   flush_events(ev, B.cnt, B.eng, s);
+}
+
+// ------------------------------------------------------------------ static geometry
+// For a fixed mesh the geometric terms of the gradient sweeps and of the face signal speed depend only
+// on the mesh and the (unchanging) generator positions: they are evaluated once, with the expressions
+// of the per-step code, for cells [lo, hi) / faces [flo, fhi) of simulation s.
+__global__ void __launch_bounds__(HFV_BLOCK) k_geom_cells(Batch B, int s, int64_t lo, int64_t hi) {
+  const int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= hi) return;
+  const Cells& C = B.C;
+  const Faces& F = B.F;
+  const Cfg& cfg = B.sims[s].cfg;
+  const int64_t n = C.n, nf = F.nf;
+  const V3<double> cen = ld3(C.cen, n, i);
+  const St<double> prim = ld5(C.prim, n, i);
+  M3 m;
+  m.x = mk3(0., 0., 0.);
+  m.y = mk3(0., 0., 0.);
+  m.z = mk3(0., 0., 0.);
+  if (cfg.dim == 1) {
+    m.y.y = 1.;
+    m.z.z = 1.;
+  } else if (cfg.dim == 2) {
+    m.z.z = 1.;
+  }
+  const int32_t k0 = F.off[i], k1 = F.off[i + 1];
+  for (int32_t k = k0; k < k1; ++k) {
+    const int32_t e = F.conn[k];
+    const uint32_t f = (uint32_t)(e >> 1);
+    const V3<double> sh = decode_shift(F.shc[f], cfg.box);
+    const V3<double> shift = (e & 1) == 0 ? sh : -sh;
+    const int32_t other = F.nbr[k];
+    V3<double> oc;
+    St<double> op;
+    if (other >= 0) oc = ld3(C.cen, n, other);
+    else boundary_neighbour(F, cfg, f, cen, prim, oc, op);
+    const V3<double> ds = (oc + shift) - cen;
+    const double w = ldo(F.area, f) / len2(ds);
+    m.x = m.x + (w * (ds.x * ds));
+    m.y = m.y + (w * (ds.y * ds));
+    m.z = m.z + (w * (ds.z * ds));
+    const V3<double> dxf = (ld3(F.fcen, nf, f) - cen) - shift;
+    double4* eg = (double4*)(F.eg + 8 * (uint64_t)(uint32_t)k);
+    eg[0] = make_double4(w, ds.x, ds.y, ds.z);
+    eg[1] = make_double4(dxf.x, dxf.y, dxf.z, 0.);
+  }
+  const M3 inv = inverse_cols(m);
+  const double v[9] = {inv.x.x, inv.x.y, inv.x.z, inv.y.x, inv.y.y, inv.y.z, inv.z.x, inv.z.y, inv.z.z};
+#pragma unroll
+  for (int q = 0; q < 9; ++q) C.minv[q * n + i] = v[q];
+}
+__global__ void __launch_bounds__(HFV_BLOCK) k_geom_faces(Batch B, int s, int64_t flo, int64_t fhi) {
+  const int64_t f = flo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (f >= fhi) return;
+  const Cells& C = B.C;
+  const Faces& F = B.F;
+  const Cfg& cfg = B.sims[s].cfg;
+  const int64_t n = C.n, nf = F.nf;
+  const int32_t l = F.left[f], r = F.right[f];
+  if (l < 0) return;
+  V3<double> dx_centroid, dx;
+  if (r >= 0) {
+    const V3<double> sh = decode_shift(F.shc[f], cfg.box);
+    dx_centroid = (ld3(C.cen, n, r) + sh) - ld3(C.cen, n, l);
+    dx = (ld3(C.loc, n, r) + sh) - ld3(C.loc, n, l);
+  } else {
+    const V3<double> fc = ld3(F.fcen, nf, f), nrm = ld3(F.nrm, nf, f);
+    const V3<double> loc = ld3(C.loc, n, l), cen = ld3(C.cen, n, l);
+    const V3<double> loc_r = loc + ((2. * dot(fc - loc, nrm)) * nrm);
+    const V3<double> cen_r = cen + ((2. * dot(fc - cen, nrm)) * nrm);
+    dx_centroid = cen_r - cen;
+    dx = loc_r - loc;
+  }
+  *(double4*)(F.fs + 4 * (uint64_t)f) = make_double4(dx_centroid.x, dx_centroid.y, dx_centroid.z, F.area[f] / len(dx));
 }
 
 // ------------------------------------------------------------------ limiter

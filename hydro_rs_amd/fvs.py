@@ -224,14 +224,14 @@ class EulerEqnsFvs:
     def synchronize(self):
         _lib.check(self.lib.hfv_synchronize(self.ptr))
 
-    def set_flux_variant(self, packed):
-        """0: flux kernel gathers SoA fields; 1: packed per-cell flux records (bit-identical results)."""
-        _lib.check(self.lib.hfv_set_flux_variant(self.ptr, int(packed)))
+    def set_geometry_mode(self, use_static):
+        """1 (default): fixed meshes use precomputed geometry; 0: recompute it every step.  Bit-identical."""
+        _lib.check(self.lib.hfv_set_geometry_mode(self.ptr, int(use_static)))
 
-    def flux_variant(self):
+    def geometry_static(self, slot=0):
         v = C.c_int(0)
-        _lib.check(self.lib.hfv_get_flux_variant(self.ptr, C.byref(v)))
-        return v.value
+        _lib.check(self.lib.hfv_get_geometry_mode(self.ptr, int(slot), C.byref(v)))
+        return bool(v.value)
 
 
 def _engine_dict(e):
@@ -333,14 +333,14 @@ class Batch:
     def synchronize(self):
         _lib.check(self.lib.hfv_synchronize(self.ptr))
 
-    def set_flux_variant(self, packed):
-        """0: flux kernel gathers SoA fields; 1: packed per-cell flux records (bit-identical results)."""
-        _lib.check(self.lib.hfv_set_flux_variant(self.ptr, int(packed)))
+    def set_geometry_mode(self, use_static):
+        """1 (default): fixed meshes use precomputed geometry; 0: recompute it every step.  Bit-identical."""
+        _lib.check(self.lib.hfv_set_geometry_mode(self.ptr, int(use_static)))
 
-    def flux_variant(self):
+    def geometry_static(self, slot=0):
         v = C.c_int(0)
-        _lib.check(self.lib.hfv_get_flux_variant(self.ptr, C.byref(v)))
-        return v.value
+        _lib.check(self.lib.hfv_get_geometry_mode(self.ptr, int(slot), C.byref(v)))
+        return bool(v.value)
 
 
 def riemann_batch(cfg: Config, WL, WR, vface, nrm):

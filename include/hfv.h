@@ -241,13 +241,14 @@ int hfv_batch_get_engine_info(hfv_ctx* ctx, int slot, hfv_engine_info* out);
 int hfv_batch_get_state(hfv_ctx* ctx, int slot, hfv_state* st);
 int hfv_batch_num_slots(hfv_ctx* ctx, int* n);
 
-/* ---------------------------------------------------------------- kernel variants */
-/* Face-flux gather variant (no counterpart in the reference, which has no device): 0 gathers each
- * cell's inputs from the SoA fields; 1 has the predict kernel write a 384-byte flux record per cell
- * (three cache lines) that the flux kernel gathers instead (384 more bytes of HBM per cell).  Results
- * are bit-identical; only speed differs.  hfv_autotune picks one by timing both. */
-int hfv_set_flux_variant(hfv_ctx* ctx, int packed);
-int hfv_get_flux_variant(hfv_ctx* ctx, int* packed);
+/* ---------------------------------------------------------------- geometry mode */
+/* Fixed meshes (HFV_FIXED motion) whose particle velocities are all +0 use precomputed geometry: the
+ * gradient weights and offsets per CSR entry, the inverse gradient matrix per cell, and the centroid
+ * offset and a_over_r per face, evaluated once with the per-step expressions (no counterpart in the
+ * reference, which recomputes them every step).  Results are bit-identical to use_static = 0.  Default
+ * 1; hfv_get_geometry_mode reports whether a slot's simulation currently qualifies. */
+int hfv_set_geometry_mode(hfv_ctx* ctx, int use_static);
+int hfv_get_geometry_mode(hfv_ctx* ctx, int slot, int* is_static);
 
 /* ---------------------------------------------------------------- measurement */
 /* Device time of the step's kernels, from HIP events recorded on the context's stream around each

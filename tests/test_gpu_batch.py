@@ -39,12 +39,12 @@ def _alone(prob, steps):
     return d.get_state(), d.engine_info()
 
 
-@pytest.mark.parametrize("packed", [0, 1])
-def test_batch_equals_standalone(packed):
+@pytest.mark.parametrize("static", [1, 0])
+def test_batch_equals_standalone(static):
     probs = _mix()
-    say(f"[batch] {len(probs)} simulations: {[p.name for p in probs]}, flux variant {packed}")
+    say(f"[batch] {len(probs)} simulations: {[p.name for p in probs]}, static geometry {static}")
     b = fvs.Batch.for_problems(probs, headroom=1.25)
-    b.set_flux_variant(packed)
+    b.set_geometry_mode(static)
     launches = b.run(1 << 40)
     say(f"  batch launches until all done: {launches}")
     for s, p in enumerate(probs):

@@ -153,14 +153,15 @@ def test_full_run_parity(ora, case, steps):
 
 
 @pytest.mark.parametrize("case", list(CASES))
-def test_flux_variants_identical(case):
-    """The packed flux-record variant gives bit-identical states to the SoA gather variant."""
-    say(f"[variants] case={case}")
+def test_static_geometry_identical(case):
+    """Precomputed geometry (fixed meshes) gives bit-identical states to recomputing it every step."""
+    say(f"[geometry] case={case}")
     prob = CASES[case]()
     a = ics.build_device_sim(prob)
     b = ics.build_device_sim(prob)
-    b.set_flux_variant(1)
-    assert b.flux_variant() == 1
+    b.set_geometry_mode(0)
+    assert not b.geometry_static()
+    assert a.geometry_static() == (prob.cfg.motion == fvs.FIXED)
     na, nb = a.run(60), b.run(60)
     assert na == nb
     sa, sb = a.get_state(), b.get_state()
