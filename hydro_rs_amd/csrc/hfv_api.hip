@@ -596,6 +596,9 @@ int refresh_geometry(hfv_ctx* c, int s) {
   c->desc[s].static_geom = st ? 1 : 0;
   c->desc[s].static_grad = st && p.dim == 3 ? 1 : 0;
   HIP_OK(hipMemcpyAsync(c->ddesc + s, &c->desc[s], sizeof(SimDesc), hipMemcpyHostToDevice, c->stream));
+  if (c->nused[s] > 0)  // neighbour primitives/centroids of the gradient sweeps
+    hipLaunchKernelGGL(k_pack_pc, dim3(grid_full(c->nused[s])), dim3(BLOCK), 0, c->stream, c->batch_args(),
+                       c->cbase[s], c->cbase[s] + c->nused[s]);
   if (st) {
     const int64_t cb = c->cbase[s], n = c->nused[s], fb = c->fbase[s], nf = c->fused[s];
     Batch B = c->batch_args();
@@ -754,7 +757,7 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   hipLaunchKernelGGL(k_wakeup, gc, gb, 0, st, B);
   if (t) prof_end(c, KS_WAKEUP);
   if (t) prof_begin(c, KS_LIMIT);
-  hipLaunchKernelGGL(k_limit, gc, gb, 0, st, B);
+  hipLaunchKernelGGL(k_limit, gc, gb, 0, st, B, 0);
   if (t) prof_end(c, KS_LIMIT);
   hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, st, B);
 }
@@ -1073,7 +1076,7 @@ int hfv_conserved_to_primitive(hfv_ctx* c) {
 
 int hfv_gradients(hfv_ctx* c) {
   TRY(require_mesh(c));
-  hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 4);
+  hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 4 | 8);
   launch_grad(c, c->batch_args(), 1, 0);
   HIP_OK(hipGetLastError());
   return sync_counters(c);
@@ -1084,7 +1087,7 @@ int hfv_timesteps(hfv_ctx* c, uint64_t* ti_next) {
   Batch B = c->batch_args();
   launch_grad(c, B, 0, 1);
   hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B);
-  hipLaunchKernelGGL(k_limit, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_limit, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B, 1);
   hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, c->stream, B);
   HIP_OK(hipGetLastError());
   TRY(sync_counters(c));

@@ -224,6 +224,25 @@ __device__ __forceinline__ void stg(double* a, int64_t n, int64_t i, const Gr<do
     a[(3 * k + 2) * n + i] = g.g[k].z;
   }
 }
+// One 64-byte line of the pc pack: prim[5], cen[3].
+__device__ __forceinline__ void store_pc(double* pc, int64_t i, const St<double>& p, const V3<double>& c) {
+  double2* d = (double2*)(pc + 8 * (uint64_t)i);
+  d[0] = make_double2(p.c[0], p.c[1]);
+  d[1] = make_double2(p.c[2], p.c[3]);
+  d[2] = make_double2(p.c[4], c.x);
+  d[3] = make_double2(c.y, c.z);
+}
+__device__ __forceinline__ void load_pc(const double* pc, uint32_t i, St<double>& p, V3<double>& c) {
+  const double2* d = (const double2*)(pc + 8 * (uint64_t)i);
+  double2 a = d[0], b = d[1], e = d[2], g = d[3];
+  p.c[0] = a.x;
+  p.c[1] = a.y;
+  p.c[2] = b.x;
+  p.c[3] = b.y;
+  p.c[4] = e.x;
+  c = mk3(e.y, g.x, g.y);
+}
+
 // periodic shift code: 2 bits per axis, 0 -> 0, 1 -> +1, 2 -> -1
 __device__ __forceinline__ V3<double> decode_shift(uint8_t code, const double* box) {
   double s[3];
@@ -580,7 +599,10 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
       ex = ex + te;
       st5(C.extrap, n, i, ex);
       check_physical(prim, ev);
-      if (ev.check_physical) st5(C.prim, n, i, prim);
+      if (ev.check_physical) {
+        st5(C.prim, n, i, prim);
+        store_pc(C.pc, i, prim, ld3(C.cen, n, i));
+      }
       // volume += dvdt * dt clamped to [V/2, 2V]: dvdt is never set on the path (part.rs:130-131)
     }
     const double cs = prim.c[0] > 0. ? sound_speed(cfg.gd, prim.c[4], 1. / prim.c[0]) : 0.;
@@ -954,27 +976,8 @@ __device__ __forceinline__ St<double> c2p(const St<double>& c, double vol, const
 }
 
 
-// One 64-byte line of the pc pack: prim[5], cen[3].
-__device__ __forceinline__ void store_pc(double* pc, int64_t i, const St<double>& p, const V3<double>& c) {
-  double2* d = (double2*)(pc + 8 * (uint64_t)i);
-  d[0] = make_double2(p.c[0], p.c[1]);
-  d[1] = make_double2(p.c[2], p.c[3]);
-  d[2] = make_double2(p.c[4], c.x);
-  d[3] = make_double2(c.y, c.z);
-}
-__device__ __forceinline__ void load_pc(const double* pc, uint32_t i, St<double>& p, V3<double>& c) {
-  const double2* d = (const double2*)(pc + 8 * (uint64_t)i);
-  double2 a = d[0], b = d[1], e = d[2], g = d[3];
-  p.c[0] = a.x;
-  p.c[1] = a.y;
-  p.c[2] = b.x;
-  p.c[3] = b.y;
-  p.c[4] = e.x;
-  c = mk3(e.y, g.x, g.y);
-}
-
-// mode bit 1: accumulate + apply + kick2; bit 2: conserved -> primitive; bit 4: write the pc pack of
-// every cell (after bits 1 and 2).
+// mode bit 1: accumulate + apply + kick2; bit 2: conserved -> primitive; bit 4: update the pc pack
+// (after bits 1 and 2) of the cells whose primitives changed, bit 8: of every cell.
 __global__ void __launch_bounds__(HFV_BLOCK) k_apply(Batch B, int mode) {
   const BlockJob J = B.cjobs[blockIdx.x];
   if (J.sim < 0) return;
@@ -1056,8 +1059,17 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_apply(Batch B, int mode) {
     st5(C.prim, n, i, prim);
     have_prim = true;
   }
-  if (mode & 4) store_pc(C.pc, i, have_prim ? prim : ld5(C.prim, n, i), ld3(C.cen, n, i));
+  // the pc pack changes where prim changed (active cells), or everywhere on moving meshes / on request
+  if ((mode & 4) && (have_prim || (mode & 8) || cfg.motion != MOT_FIXED))
+    store_pc(C.pc, i, have_prim ? prim : ld5(C.prim, n, i), ld3(C.cen, n, i));
   flush_events(ev, B.cnt, B.eng, s);
+}
+
+// The pc pack of cells [lo, hi) from the SoA fields (after the host changed the state or mesh).
+__global__ void __launch_bounds__(HFV_BLOCK) k_pack_pc(Batch B, int64_t lo, int64_t hi) {
+  const int64_t i = lo + (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= hi) return;
+  store_pc(B.C.pc, i, ld5(B.C.prim, B.C.n, i), ld3(B.C.cen, B.C.n, i));
 }
 
 // glam::DMat3 as three columns
@@ -1437,7 +1449,9 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_wakeup(Batch B) {
 
 // Particle::timestep_limit (part.rs:263-297) + reset_extrapolations (hydrodynamics.rs:193-195) +
 // kick1 (hydro_kick1 part.rs:87-104, grav_kick with a_grav = 0).
-__global__ void __launch_bounds__(HFV_BLOCK) k_limit(Batch B) {
+// canon: apply kick1's "momentum += 0 * mass" (a no-op after kick2 of the same step, which the fused
+// step has already done; the trait-level phases keep it).
+__global__ void __launch_bounds__(HFV_BLOCK) k_limit(Batch B, int canon) {
   const BlockJob J = B.cjobs[blockIdx.x];
   if (J.sim < 0) return;
   const int s = J.sim;
@@ -1477,9 +1491,11 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_limit(Batch B) {
         for (int a = 0; a < cfg.dim; ++a) C.v[a * n + i] -= kick_fac * C.grad[(12 + a) * n + i];
       }
     }
-    C.cons[1 * n + i] += 0.0;
-    C.cons[2 * n + i] += 0.0;
-    C.cons[3 * n + i] += 0.0;
+    if (canon) {
+      C.cons[1 * n + i] += 0.0;
+      C.cons[2 * n + i] += 0.0;
+      C.cons[3 * n + i] += 0.0;
+    }
   }
 }
 
