@@ -145,6 +145,7 @@ struct hfv_ctx {
   int2* fall = nullptr;      // exact-solver fallback queue (AIRS)
   bool use_static = true;    // precomputed geometry for fixed meshes (hfv_set_geometry_mode)
   std::vector<int> v_zero;   // per slot: every particle velocity is +0 (fixed meshes keep it so)
+  std::vector<int> has_stage;  // per slot: some face chunk has a staging list
   bool any_static = false, any_dynamic = false, any_static_grad = false, any_dynamic_grad = false;
   SimDesc* ddesc = nullptr;
   SimEngine* deng = nullptr;
@@ -160,6 +161,7 @@ struct hfv_ctx {
   hipGraphExec_t gexec{};
   bool graph_ok = false;
   bool use_graph = true;
+  bool fr_dirty = true;  // flux records must be rebuilt (k_pack_fr) before the next flux kernel
   hfv_timestep_info last_info{};
   // kernel timing (hfv_profile_enable)
   bool prof = false;
@@ -291,8 +293,6 @@ int alloc_batch(hfv_ctx* c) {
   C.msv = dalloc<double>(c, n);
   C.dt = dalloc<double>(c, n);
   C.srad = dalloc<double>(c, n);
-  C.tex = dalloc<double>(c, 5 * n);
-  C.cs = dalloc<double>(c, n);
   C.pc = dalloc<double>(c, 8 * n);
   C.bin = dalloc<int8_t>(c, n);
   C.wake = dalloc<int8_t>(c, n);
@@ -330,20 +330,28 @@ int alloc_batch(hfv_ctx* c) {
   c->P.face = dalloc<uint32_t>(c, nfc);
   c->P.ti = dalloc<uint64_t>(c, ncc);
   c->fall = dalloc<int2>(c, nf);
+  C.fr = dalloc<double>(c, (int64_t)HFV_FR_W * n);
+  F.scell = dalloc<int32_t>(c, nfc * HFV_STAGE_MAX);
+  F.scnt = dalloc<uint8_t>(c, nfc);
+  F.floc = dalloc<uint8_t>(c, 2 * nf);
   C.minv = dalloc<double>(c, 9 * n);
   F.eg = dalloc<double>(c, 8 * nk);
   F.fs = dalloc<double>(c, 4 * nf);
   c->v_zero.assign(S, 1);
+  c->has_stage.assign(S, 0);
   c->ddesc = dalloc<SimDesc>(c, S);
   c->deng = dalloc<SimEngine>(c, S);
   for (void* p : c->bufs)
     if (!p) return set_err(HFV_ERR_HIP, "hipMalloc failed");
   hipStream_t st = c->stream;
   const size_t sd = sizeof(double) * (size_t)n;
-  double* dbl[] = {C.prim, C.grad, C.gc, C.extrap, C.cons, C.facc, C.vol, C.loc, C.cen, C.v, C.vrel, C.maor, C.msv, C.dt, C.srad, C.tex, C.cs};
-  int mult[] = {5, 15, 3, 5, 5, 5, 1, 3, 3, 3, 3, 1, 1, 1, 1, 5, 1};
-  for (int k = 0; k < 17; ++k) HIP_OK(hipMemsetAsync(dbl[k], 0, sd * mult[k], st));
+  double* dbl[] = {C.prim, C.grad, C.gc, C.extrap, C.cons, C.facc, C.vol, C.loc, C.cen, C.v, C.vrel, C.maor, C.msv, C.dt, C.srad};
+  int mult[] = {5, 15, 3, HFV_EX_W, 5, 5, 1, 3, 3, 3, 3, 1, 1, 1, 1};
+  for (int k = 0; k < 15; ++k) HIP_OK(hipMemsetAsync(dbl[k], 0, sd * mult[k], st));
   HIP_OK(hipMemsetAsync(C.pc, 0, sd * 8, st));
+  HIP_OK(hipMemsetAsync(C.fr, 0, sd * HFV_FR_W, st));
+  HIP_OK(hipMemsetAsync(F.scnt, 0, (size_t)std::max<int64_t>(nfc, 1), st));
+  HIP_OK(hipMemsetAsync(F.floc, 0, 2 * (size_t)std::max<int64_t>(nf, 1), st));
   HIP_OK(hipMemsetAsync(C.bin, 0, (size_t)n, st));
   HIP_OK(hipMemsetAsync(C.wake, 0, (size_t)n, st));
   HIP_OK(hipMemsetAsync(C.act, 0, (size_t)n, st));
@@ -514,6 +522,39 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
   HIP_OK(hipMemcpyAsync(F.off + cb, off.data(), sizeof(int32_t) * noff, hipMemcpyHostToDevice, st));
   if (tot) HIP_OK(hipMemcpyAsync(F.conn + kb, conn.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, st));
   if (tot) HIP_OK(hipMemcpyAsync(F.nbr + kb, nbr.data(), sizeof(int32_t) * tot, hipMemcpyHostToDevice, st));
+  // LDS staging lists of the face chunks (the distinct cells of each chunk's faces, ascending)
+  const int64_t nch = (nf + BLOCK - 1) / BLOCK;
+  std::vector<int32_t> scell(std::max<int64_t>(nch, 1) * HFV_STAGE_MAX, 0);
+  std::vector<uint8_t> scnt(std::max<int64_t>(nch, 1), 0), floc(2 * std::max<int64_t>(nf, 1), 0);
+  {
+    std::vector<int32_t> u;
+    u.reserve(2 * BLOCK);
+    for (int64_t j = 0; j < nch; ++j) {
+      const int64_t f0 = j * BLOCK, f1 = std::min<int64_t>(f0 + BLOCK, nf);
+      u.clear();
+      for (int64_t f = f0; f < f1; ++f) {
+        u.push_back(left[f]);
+        if (right[f] >= 0) u.push_back(right[f]);
+      }
+      std::sort(u.begin(), u.end());
+      u.erase(std::unique(u.begin(), u.end()), u.end());
+      if ((int64_t)u.size() > HFV_STAGE_MAX) continue;
+      scnt[j] = (uint8_t)u.size();
+      std::copy(u.begin(), u.end(), scell.begin() + j * HFV_STAGE_MAX);
+      for (int64_t f = f0; f < f1; ++f) {
+        floc[2 * f] = (uint8_t)(std::lower_bound(u.begin(), u.end(), left[f]) - u.begin());
+        floc[2 * f + 1] = right[f] < 0 ? 255 : (uint8_t)(std::lower_bound(u.begin(), u.end(), right[f]) - u.begin());
+      }
+    }
+  }
+  if (nch) {
+    const int64_t k0 = c->fchunk0[s];
+    HIP_OK(hipMemcpyAsync(F.scell + k0 * HFV_STAGE_MAX, scell.data(), sizeof(int32_t) * nch * HFV_STAGE_MAX,
+                          hipMemcpyHostToDevice, st));
+    HIP_OK(hipMemcpyAsync(F.scnt + k0, scnt.data(), (size_t)nch, hipMemcpyHostToDevice, st));
+  }
+  if (nf) HIP_OK(hipMemcpyAsync(F.floc + 2 * fb, floc.data(), 2 * (size_t)nf, hipMemcpyHostToDevice, st));
+  c->has_stage[s] = std::any_of(scnt.begin(), scnt.end(), [](uint8_t k) { return k > 0; }) ? 1 : 0;
   // Particle::update_geometry (part.rs:151-158) for the selected cells
   const int64_t N = c->n;
   std::vector<double> vol(n), cen(3 * n), srad(n);
@@ -573,15 +614,17 @@ int reset_slot(hfv_ctx* c, int s) {
   struct {
     double* p;
     int w;
-  } fl[] = {{C.prim, 5}, {C.grad, 15}, {C.gc, 3}, {C.extrap, 5}, {C.cons, 5}, {C.facc, 5}, {C.loc, 3}, {C.v, 3},
-            {C.vrel, 3}, {C.maor, 1}, {C.msv, 1}, {C.dt, 1}, {C.tex, 5}, {C.cs, 1}};
+  } fl[] = {{C.prim, 5}, {C.grad, 15}, {C.gc, 3}, {C.cons, 5}, {C.facc, 5}, {C.loc, 3}, {C.v, 3},
+            {C.vrel, 3}, {C.maor, 1}, {C.msv, 1}, {C.dt, 1}};
   for (auto& f : fl)
     for (int k = 0; k < f.w; ++k) HIP_OK(hipMemsetAsync(f.p + k * N + cb, 0, sizeof(double) * cap, st));
+  HIP_OK(hipMemsetAsync(C.extrap + HFV_EX_W * cb, 0, sizeof(double) * HFV_EX_W * cap, st));
   HIP_OK(hipMemsetAsync(C.bin + cb, 0, cap, st));
   HIP_OK(hipMemsetAsync(C.wake + cb, 0, cap, st));
   HIP_OK(hipMemsetAsync(C.act + cb, 0, cap, st));
   (void)n;
   c->v_zero[s] = 1;
+  c->fr_dirty = true;
   c->heng[s] = fresh_engine(c->params[s]);
   HIP_OK(hipMemcpyAsync(c->deng + s, &c->heng[s], sizeof(SimEngine), hipMemcpyHostToDevice, st));
   HIP_OK(hipStreamSynchronize(st));
@@ -595,6 +638,8 @@ int refresh_geometry(hfv_ctx* c, int s) {
   const bool st = c->use_static && p.motion == HFV_FIXED && c->v_zero[s] && c->nused[s] > 0;
   c->desc[s].static_geom = st ? 1 : 0;
   c->desc[s].static_grad = st && p.dim == 3 ? 1 : 0;
+  c->desc[s].staged = st && c->has_stage[s] ? 1 : 0;
+  c->fr_dirty = true;
   HIP_OK(hipMemcpyAsync(c->ddesc + s, &c->desc[s], sizeof(SimDesc), hipMemcpyHostToDevice, c->stream));
   if (c->nused[s] > 0)  // neighbour primitives/centroids of the gradient sweeps
     hipLaunchKernelGGL(k_pack_pc, dim3(grid_full(c->nused[s])), dim3(BLOCK), 0, c->stream, c->batch_args(),
@@ -762,7 +807,16 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, st, B);
 }
 
+// Rebuild the flux records of the static-geometry simulations when the host or a phase call changed
+// their inputs (the fused step keeps them current itself).
+void pack_fr_if_dirty(hfv_ctx* c) {
+  if (c->fr_dirty && c->any_static)
+    hipLaunchKernelGGL(k_pack_fr, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args());
+  c->fr_dirty = false;
+}
+
 int launch_step(hfv_ctx* c) {
+  pack_fr_if_dirty(c);
   if (c->prof || !c->use_graph) {
     enqueue_step(c, true);
     HIP_OK(hipGetLastError());
@@ -841,7 +895,7 @@ void fill_engine_info(const SimEngine& E, hfv_engine_info* out) {
 }
 
 #define STATE_FIELDS(X)                                                                           \
-  X(prim, C.prim, 5) X(grad, C.grad, 15) X(gc, C.gc, 3) X(extrap, C.extrap, 5) X(cons, C.cons, 5) \
+  X(prim, C.prim, 5) X(grad, C.grad, 15) X(gc, C.gc, 3) X(cons, C.cons, 5)                        \
   X(fluxes, C.facc, 5) X(volume, C.vol, 1) X(loc, C.loc, 3) X(centroid, C.cen, 3) X(v, C.v, 3)    \
   X(v_rel, C.vrel, 3) X(maor, C.maor, 1) X(msv, C.msv, 1) X(dt, C.dt, 1)
 
@@ -859,11 +913,28 @@ int copy_state(hfv_ctx* c, int s, hfv_state* st, bool to_device) {
     }
   STATE_FIELDS(CPF)
 #undef CPF
+  // extrapolations: AoS [N][5] on the device, component-major [5][n] at the boundary
+  std::vector<double> exa;
+  if (st->extrap) {
+    exa.resize(HFV_EX_W * (size_t)n);
+    double* d = c->C.extrap + HFV_EX_W * cb;
+    if (to_device) {
+      for (int64_t i = 0; i < n; ++i)
+        for (int q = 0; q < 5; ++q) exa[HFV_EX_W * i + q] = st->extrap[q * n + i];
+      HIP_OK(hipMemcpyAsync(d, exa.data(), sizeof(double) * exa.size(), hipMemcpyHostToDevice, strm));
+    } else {
+      HIP_OK(hipMemcpyAsync(exa.data(), d, sizeof(double) * exa.size(), hipMemcpyDeviceToHost, strm));
+    }
+  }
   if (st->timebin)
     HIP_OK(hipMemcpyAsync(to_device ? (void*)(c->C.bin + cb) : (void*)st->timebin,
                           to_device ? (const void*)st->timebin : (const void*)(c->C.bin + cb), (size_t)n, kind, strm));
   HIP_OK(hipStreamSynchronize(strm));
+  if (st->extrap && !to_device)
+    for (int64_t i = 0; i < n; ++i)
+      for (int q = 0; q < 5; ++q) st->extrap[q * n + i] = exa[HFV_EX_W * i + q];
   if (to_device) {
+    c->fr_dirty = true;
     if (st->v) {  // the static geometry needs every particle velocity to be exactly +0
       int z = 1;
       for (int64_t k = 0; k < 3 * n && z; ++k) {
@@ -1023,6 +1094,7 @@ int hfv_predict(hfv_ctx* c, double dt) {
   HIP_OK(hipMemcpyAsync(c->deng, &E, sizeof(SimEngine), hipMemcpyHostToDevice, c->stream));
   hipLaunchKernelGGL(k_mask, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
   HIP_OK(hipGetLastError());
+  c->fr_dirty = true;
   return sync_counters(c);
 }
 
@@ -1034,6 +1106,8 @@ int hfv_compute_fluxes(hfv_ctx* c, const hfv_timestep_info* info) {
   Batch B = c->batch_args();
   hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, c->stream, B);
   hipLaunchKernelGGL(k_mask, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B, 2);
+  c->fr_dirty = true;
+  pack_fr_if_dirty(c);
   launch_flux(c, B);
   HIP_OK(hipGetLastError());
   return sync_counters(c);
@@ -1064,6 +1138,7 @@ int hfv_apply_fluxes(hfv_ctx* c) {
   TRY(require_mesh(c));
   hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
   HIP_OK(hipGetLastError());
+  c->fr_dirty = true;
   return sync_counters(c);
 }
 
@@ -1071,6 +1146,7 @@ int hfv_conserved_to_primitive(hfv_ctx* c) {
   TRY(require_mesh(c));
   hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 2);
   HIP_OK(hipGetLastError());
+  c->fr_dirty = true;
   return sync_counters(c);
 }
 
@@ -1079,6 +1155,7 @@ int hfv_gradients(hfv_ctx* c) {
   hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 4 | 8);
   launch_grad(c, c->batch_args(), 1, 0);
   HIP_OK(hipGetLastError());
+  c->fr_dirty = true;
   return sync_counters(c);
 }
 
@@ -1090,6 +1167,7 @@ int hfv_timesteps(hfv_ctx* c, uint64_t* ti_next) {
   hipLaunchKernelGGL(k_limit, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B, 1);
   hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, c->stream, B);
   HIP_OK(hipGetLastError());
+  c->fr_dirty = true;
   TRY(sync_counters(c));
   if (c->heng[0].err) return set_err(c->heng[0].err, "engine error in this step");
   if (ti_next) *ti_next = c->heng[0].ti_next;
@@ -1102,6 +1180,7 @@ int hfv_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {
   if (!info) return set_err(HFV_ERR_INVALID, "info is NULL");
   c->last_info = *info;
   TRY(begin_phase_step(c, info));
+  pack_fr_if_dirty(c);
   enqueue_step(c, true);
   HIP_OK(hipGetLastError());
   TRY(sync_counters(c));

@@ -5,17 +5,21 @@
 // from a captured graph) without a host round trip.
 //
 // HBM layout (cells/faces of all simulations concatenated, indices global):
-//   cells, component-major SoA (field[c * N + i]): prim[5], grad[15], gc[3], extrap[5], cons[5],
-//     facc[5] (pending flux accumulator), vol, loc[3], cen[3], v[3], vrel[3], maor, msv, dt, srad,
-//     tex[5] (time-extrapolation rate of this step's primitives), cs (their sound speed, 0 for vacuum),
-//     bin (int8), wake (int8), act (uint8)
+//   cells, component-major SoA (field[c * N + i]): prim[5], grad[15], gc[3], cons[5], facc[5] (pending
+//     flux accumulator), vol, loc[3], cen[3], v[3], vrel[3], maor, msv, dt, srad, bin (int8), wake
+//     (int8), act (uint8); extrap[N][5] (AoS: read and written whole by predict, staged whole by the
+//     flux kernel)
 //     pc[N][8] (AoS: prim[5], cen[3] of every cell after this step's c2p, one 64-byte line per cell
 //     for the gradient sweeps' neighbour gathers)
+//     fr[N][32] (AoS flux record of static-geometry cells, HFV_FR_*: prim, time-extrapolation rate,
+//     sound speed, gradients, dt, gc; two 128-byte lines staged whole by the flux kernel)
 //   faces, SoA: area, nrm[3], fcen[3], left (int32), right (int32, -1 = wall), shc (uint8 periodic
 //     shift code, 2 bits per axis), cur (uint8: record of this step); rec[F][8] (AoS, one 64-byte line
 //     per face: dt*F[5], v_max, a_over_r, unused)
 //   csr: off[N+1] (int32), conn[2F] (int32: face << 1 | side, side 1 = the cell is the face's right),
 //     nbr[2F] (int32: the face's other cell, -1 for a wall)
+//   staging lists: per face chunk its distinct cells (scell, scnt), per face its cells' list slots
+//     (floc)
 //   block tables: one {sim, lo, hi, chunk} per thread block of the cell kernels and of the face
 //     kernels.  A block only holds items of one simulation, so configuration and engine loads are
 //     uniform (scalar) and a finished simulation's blocks exit at once.  Blocks are ordered so that
@@ -34,8 +38,9 @@ namespace hfv {
 struct Cells {
   int64_t n;
   double *prim, *grad, *gc, *extrap, *cons, *facc, *vol, *loc, *cen, *v, *vrel, *maor, *msv, *dt, *srad;
-  double *tex, *cs, *pc;
+  double* pc;
   double* minv;  // [9][N] inverse of the gradient matrix (static geometry only)
+  double* fr;    // [N][HFV_FR_W] flux inputs of static-geometry cells (layout: HFV_FR_*)
   int8_t *bin, *wake;
   uint8_t* act;
 };
@@ -48,6 +53,11 @@ struct Faces {
   uint8_t *shc, *cur;
   double* eg;    // [2F][8] per CSR entry: w, ds[3], dxf[3] (static gradient geometry only)
   double* fs;    // [F][4] per face: dx_centroid[3], a_over_r (static geometry only)
+  // LDS staging of the flux kernel: per face chunk, its distinct cells (scell[chunk][HFV_STAGE_MAX],
+  // ascending) and their count (scnt, 0 = more than HFV_STAGE_MAX: no staging); per face, the local
+  // indices of its left and right cell in that list (floc[2F], 255 for a wall).
+  int32_t* scell;
+  uint8_t *scnt, *floc;
 };
 
 struct Cfg {
@@ -68,6 +78,7 @@ struct SimDesc {
   int periodic;
   int static_geom;  // fixed mesh with zero particle velocities: face geometry precomputed (k_geom_faces)
   int static_grad;  // ... and gradient geometry too (k_geom_cells; chosen for 3D meshes)
+  int staged;       // static_geom and some face chunk has a staging list: the flux records fr are kept
 };
 
 // Thread block -> work: simulation (-1: nothing), items [lo, hi), chunk (index of its partials).
@@ -123,6 +134,23 @@ struct Batch {
   DevCounters* cnt;
   int2* fall;  // (face, simulation) queued for k_flux_fallback
 };
+
+// Flux record fr[i] of a static-geometry cell: the flux phase's per-cell inputs apart from the
+// extrapolations, two 128-byte lines.  Written where the fields change (k_grad for active cells,
+// k_limit for woken cells' dt, k_mask for check_physical resets, k_pack_fr after host changes).
+#define HFV_EX_W 5  // extrapolations: AoS extrap[N][5]
+#define HFV_FR_W 32
+#define HFV_FR_PRIM 0   // prim[5]
+#define HFV_FR_TEX 5    // tex[5]: time-extrapolation rate of prim
+#define HFV_FR_CS 10    // sound speed of prim (0 for vacuum)
+#define HFV_FR_GRAD 11  // grad[15]
+#define HFV_FR_DT 26
+#define HFV_FR_GC 27    // gc[3]
+#define HFV_FR_USED 30
+// LDS staging record: the flux record's used fields, then the extrapolations[5]; odd stride, so that
+// neighbouring records start in different banks
+#define HFV_FR_LDS 35
+#define HFV_STAGE_MAX 128
 
 #define HFV_NUM_TIME_BINS 56
 #define HFV_MAX_NR_TIMESTEPS (1ull << 57)
@@ -241,6 +269,70 @@ __device__ __forceinline__ void load_pc(const double* pc, uint32_t i, St<double>
   p.c[3] = b.y;
   p.c[4] = e.x;
   c = mk3(e.y, g.x, g.y);
+}
+
+// The flux record of a cell as HFV_FR_W values.
+__device__ __forceinline__ void fill_fr(double* v, const St<double>& p, const St<double>& rate, double cs,
+                                        const Gr<double>& g, double dt, const V3<double>& gc) {
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    v[HFV_FR_PRIM + q] = p.c[q];
+    v[HFV_FR_TEX + q] = rate.c[q];
+    v[HFV_FR_GRAD + 3 * q] = g.g[q].x;
+    v[HFV_FR_GRAD + 3 * q + 1] = g.g[q].y;
+    v[HFV_FR_GRAD + 3 * q + 2] = g.g[q].z;
+  }
+  v[HFV_FR_CS] = cs;
+  v[HFV_FR_DT] = dt;
+  v[HFV_FR_GC] = gc.x;
+  v[HFV_FR_GC + 1] = gc.y;
+  v[HFV_FR_GC + 2] = gc.z;
+  v[HFV_FR_USED] = 0.;
+  v[HFV_FR_USED + 1] = 0.;
+}
+// The flux record of cell i (two 128-byte lines).
+__device__ __forceinline__ void store_fr(double* fr, int64_t i, const St<double>& p, const St<double>& rate, double cs,
+                                         const Gr<double>& g, double dt, const V3<double>& gc) {
+  double v[HFV_FR_W];
+  fill_fr(v, p, rate, cs, g, dt, gc);
+  double2* d = (double2*)(fr + HFV_FR_W * (uint64_t)i);
+#pragma unroll
+  for (int q = 0; q < HFV_FR_W / 2; ++q) d[q] = make_double2(v[2 * q], v[2 * q + 1]);
+}
+// A wave's new flux records (lane l: cell i0 + l, when bit l of `has` is set), transposed through the
+// wave's own LDS buffer buf[64 * 17] in two halves, so that every store instruction writes whole
+// 128-byte lines.  Wave-level only (no block barrier): LDS operations of a wave complete in order.
+__device__ __forceinline__ void store_fr_wave(double* fr, int64_t i0, uint64_t has, const double* v, double* buf) {
+  const int lane = (int)__lane_id();
+  const bool mine = (has >> lane) & 1ull;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (mine) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) buf[lane * 17 + j] = v[16 * h + j];
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int it = 0; it < 8; ++it) {
+      const int k = lane + 64 * it, rr = k >> 3, q = k & 7;
+      if ((has >> rr) & 1ull)
+        ((double2*)(fr + HFV_FR_W * (uint64_t)(i0 + rr) + 16 * h))[q] =
+            make_double2(buf[rr * 17 + 2 * q], buf[rr * 17 + 2 * q + 1]);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+// The flux phase's per-cell rate and sound speed of primitives p (k_mask's expressions).
+__device__ __forceinline__ St<double> fr_rate(const St<double>& p, const Gr<double>& g, const V3<double>& vrel,
+                                              const Gas<double>& gd) {
+  return p.c[0] > 0. ? tex_rate(p, g, vrel, gd) : vacuum<double>();
+}
+__device__ __forceinline__ double fr_cs(const St<double>& p, const Gas<double>& gd) {
+  return p.c[0] > 0. ? sound_speed(gd, p.c[4], 1. / p.c[0]) : 0.;
 }
 
 // periodic shift code: 2 bits per axis, 0 -> 0, 1 -> +1, 2 -> -1
@@ -590,24 +682,31 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
     St<double> prim = ld5(C.prim, n, i);
     Gr<double> gr = ldg(C.grad, n, i);
     V3<double> vrel = ld3(C.vrel, n, i);
-    St<double> rate = vacuum<double>();
-    if (prim.c[0] > 0.) rate = tex_rate(prim, gr, vrel, cfg.gd);
-    St<double> ex = ld5(C.extrap, n, i);
+    const St<double> rate = fr_rate(prim, gr, vrel, cfg.gd);
+    double* exp_ = C.extrap + HFV_EX_W * (uint64_t)i;
+    St<double> ex = mkst(exp_[0], mk3(exp_[1], exp_[2], exp_[3]), exp_[4]);
+    bool reset = false;
     if (mode != 2) {
       // extrapolations += time_extrapolations(dt) (its vacuum branch adds +0)
       St<double> te = prim.c[0] > 0. ? (-E.dt_pred) * rate : vacuum<double>();
       ex = ex + te;
-      st5(C.extrap, n, i, ex);
+#pragma unroll
+      for (int q = 0; q < 5; ++q) exp_[q] = ex.c[q];
       check_physical(prim, ev);
       if (ev.check_physical) {
+        reset = true;
         st5(C.prim, n, i, prim);
         store_pc(C.pc, i, prim, ld3(C.cen, n, i));
       }
       // volume += dvdt * dt clamped to [V/2, 2V]: dvdt is never set on the path (part.rs:130-131)
     }
-    const double cs = prim.c[0] > 0. ? sound_speed(cfg.gd, prim.c[4], 1. / prim.c[0]) : 0.;
-    st5(C.tex, n, i, rate);
-    C.cs[i] = cs;
+    if (reset && B.sims[s].staged) {
+      // the flux record holds this rate already; a reset changes prim and the sound speed
+      double* r = C.fr + HFV_FR_W * (uint64_t)i;
+#pragma unroll
+      for (int q = 0; q < 5; ++q) r[HFV_FR_PRIM + q] = prim.c[q];
+      r[HFV_FR_CS] = fr_cs(prim, cfg.gd);
+    }
 
   }
   if (mode != 1) {
@@ -685,22 +784,54 @@ __device__ __forceinline__ const Gas<float>& gas_for<float>(const Cfg& c) {
   return c.gf;
 }
 
-// Per-cell inputs of the flux phase (SoA fields).
+// Per-cell inputs of the flux phase gathered from the cell fields.  The time-extrapolation rate and
+// the sound speed are k_mask's expressions of the same primitives, gradients and v_rel (a check_physical
+// reset in k_mask leaves rho = 0, where neither is used).
 struct SoaCell {
   const Cells* C;
+  const Gas<double>* g;
   uint32_t i;
-  __device__ __forceinline__ SoaCell(const Cells& c, int32_t idx) : C(&c), i((uint32_t)idx) {}
+  __device__ __forceinline__ SoaCell(const Cells& c, const Gas<double>& gd, int32_t idx) : C(&c), g(&gd), i((uint32_t)idx) {}
   __device__ __forceinline__ double dt() const { return ldo(C->dt, i); }
   __device__ __forceinline__ St<double> prim() const { return ld5(C->prim, C->n, i); }
   __device__ __forceinline__ double rho() const { return ldo(C->prim, i); }
   __device__ __forceinline__ V3<double> pvel() const { return mk3(ldo(C->prim + C->n, i), ldo(C->prim + 2 * C->n, i), ldo(C->prim + 3 * C->n, i)); }
-  __device__ __forceinline__ double cs() const { return ldo(C->cs, i); }
+  __device__ __forceinline__ double cs() const { return fr_cs(prim(), *g); }
   __device__ __forceinline__ V3<double> grad(int q) const {
     return mk3(ldo(C->grad + (3 * q) * C->n, i), ldo(C->grad + (3 * q + 1) * C->n, i), ldo(C->grad + (3 * q + 2) * C->n, i));
   }
-  __device__ __forceinline__ double ex(int q) const { return ldo(C->extrap + q * C->n, i); }
-  __device__ __forceinline__ double tex(int q) const { return ldo(C->tex + q * C->n, i); }
+  __device__ __forceinline__ double ex(int q) const { return C->extrap[HFV_EX_W * (uint64_t)i + q]; }
+  __device__ __forceinline__ St<double> rate() const {
+    return tex_rate(prim(), ldg(C->grad, C->n, i), ld3(C->vrel, C->n, i), *g);
+  }
   __device__ __forceinline__ V3<double> gc() const { return ld3(C->gc, C->n, i); }
+  __device__ __forceinline__ V3<double> cen() const { return ld3(C->cen, C->n, i); }
+  __device__ __forceinline__ V3<double> loc() const { return ld3(C->loc, C->n, i); }
+  __device__ __forceinline__ V3<double> v() const { return ld3(C->v, C->n, i); }
+};
+// Per-cell inputs of the flux phase from a cell's flux record and extrapolations staged in LDS
+// (static geometry): record fields at HFV_FR_*, extrapolations at HFV_FR_EX.
+#define HFV_FR_EX HFV_FR_USED
+struct RecCell {
+  const double* r;
+  const Cells* C;
+  uint32_t i;
+  __device__ __forceinline__ RecCell(const double* rec, const Cells& c, int32_t idx) : r(rec), C(&c), i((uint32_t)idx) {}
+  __device__ __forceinline__ double dt() const { return r[HFV_FR_DT]; }
+  __device__ __forceinline__ St<double> prim() const {
+    return mkst(r[HFV_FR_PRIM], mk3(r[HFV_FR_PRIM + 1], r[HFV_FR_PRIM + 2], r[HFV_FR_PRIM + 3]), r[HFV_FR_PRIM + 4]);
+  }
+  __device__ __forceinline__ double rho() const { return r[HFV_FR_PRIM]; }
+  __device__ __forceinline__ V3<double> pvel() const { return mk3(r[HFV_FR_PRIM + 1], r[HFV_FR_PRIM + 2], r[HFV_FR_PRIM + 3]); }
+  __device__ __forceinline__ double cs() const { return r[HFV_FR_CS]; }
+  __device__ __forceinline__ V3<double> grad(int q) const {
+    return mk3(r[HFV_FR_GRAD + 3 * q], r[HFV_FR_GRAD + 3 * q + 1], r[HFV_FR_GRAD + 3 * q + 2]);
+  }
+  __device__ __forceinline__ double ex(int q) const { return r[HFV_FR_EX + q]; }
+  __device__ __forceinline__ St<double> rate() const {
+    return mkst(r[HFV_FR_TEX], mk3(r[HFV_FR_TEX + 1], r[HFV_FR_TEX + 2], r[HFV_FR_TEX + 3]), r[HFV_FR_TEX + 4]);
+  }
+  __device__ __forceinline__ V3<double> gc() const { return mk3(r[HFV_FR_GC], r[HFV_FR_GC + 1], r[HFV_FR_GC + 2]); }
   __device__ __forceinline__ V3<double> cen() const { return ld3(C->cen, C->n, i); }
   __device__ __forceinline__ V3<double> loc() const { return ld3(C->loc, C->n, i); }
   __device__ __forceinline__ V3<double> v() const { return ld3(C->v, C->n, i); }
@@ -713,9 +844,7 @@ __device__ __forceinline__ St<T> face_side(const Acc& X, const St<T>& pT, V3<T> 
 #pragma unroll
   for (int q = 0; q < 5; ++q) out.c[q] = (pT.c[q] + dot(cvt3<T>(X.grad(q)), dxT)) + (T)X.ex(q);
   if (pT.c[0] > T(0)) {
-    St<T> rate;
-#pragma unroll
-    for (int q = 0; q < 5; ++q) rate.c[q] = (T)X.tex(q);
+    const St<T> rate = cvtst<T>(X.rate());
     out = out + (-dte) * rate;
   } else {
     out = out + vacuum<T>();
@@ -730,8 +859,8 @@ __device__ __forceinline__ St<T> face_side(const Acc& X, const St<T>& pT, V3<T> 
 // precomputed line (same expressions, evaluated once by k_geom_faces) and the face velocity is the
 // value the formula yields for v = 0, namely +0 in every component.
 template <class T, int SOLVER, bool DEFER, bool STATIC, class Acc>
-__device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_t f, int32_t l, int32_t r, bool la,
-                                          bool ra, double* outv, Events& ev) {
+__device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_t f, const Acc& L, const Acc& R,
+                                          int32_t r, bool la, bool ra, double* outv, Events& ev) {
   const Faces& F = B.F;
   const int64_t nf = F.nf;
   const Gas<T>& g = gas_for<T>(cfg);
@@ -739,13 +868,11 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
   if (SOLVER >= 0) sc.kind = SOLVER;
 #pragma unroll
   for (int k = 0; k < 7; ++k) outv[k] = 0.;
-  const Acc L(B.C, l);
   const double dtl = L.dt();
   St<T> WL, WR;  // face-frame states
   V3<T> vfT;     // face velocity
   double dt, v_max = 0., len_dx = 1., a_over_r = 0.;
   if (r >= 0) {
-    const Acc R(B.C, r);
     // ---- gate
     const double dtr = R.dt();
     if ((!la && !ra) || (dtr < dtl && !ra) || (dtl < dtr && !la)) return false;
@@ -765,8 +892,8 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
     HFV_FSTAGE();
     // ---- pairwise limiter (gradients.rs:105-152)
     const T dxg = (T)len(dx_left - dx_right);
-    WL = pairwise_limiter(plT, prT, left_dash, len(cvt3<T>(dx_left)) / dxg, ev);
-    WR = pairwise_limiter(prT, plT, right_dash, len(cvt3<T>(dx_right)) / dxg, ev);
+    WL = pairwise_limiter(plT, prT, left_dash, fdiv(len(cvt3<T>(dx_left)), dxg), ev);
+    WR = pairwise_limiter(prT, plT, right_dash, fdiv(len(cvt3<T>(dx_right)), dxg), ev);
     HFV_FSTAGE();
     // ---- signal speed (flux.rs:43-50), face velocity (flux.rs:75-78), a_over_r: FP64
     if (STATIC) {
@@ -826,7 +953,7 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
     const V3<T> dxfT = cvt3<T>(dx_face);
     const V3<T> nT = cvt3<T>(nrm);
     St<T> dash = face_side<T>(L, pT, dxfT, (T)(-0.5 * dt));
-    const T dx_fac = len(dxfT) / (T)dx_gradients;
+    const T dx_fac = fdiv(len(dxfT), (T)dx_gradients);
     if (cfg.boundary == BND_REFLECTIVE) {
       St<T> rp = reflect(pT, nT);
       dash = pairwise_limiter(pT, rp, dash, dx_fac, ev);
@@ -869,6 +996,10 @@ __device__ __forceinline__ void store_rec(double* rec, int64_t f, const double* 
 
 // Faces with an active cell get their record (zero when gated off) and cur = 1; the others cur = 0.
 // With SOLVER == AIRS, faces that need the exact solver are queued for k_flux_fallback.
+// STATIC: when the chunk has a staging list and a face with an active cell, the block stages its
+// distinct cells' flux records and extrapolations in LDS by whole lines (each cell is read once per
+// block, instead of every face gathering its two cells field by field); the list, then every record
+// load of a thread, are issued together (two memory round trips).  Chunks without a list gather.
 template <class T, int SOLVER, bool STATIC>
 __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   const BlockJob J = B.fjobs[blockIdx.x];
@@ -878,22 +1009,79 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   const Cfg& cfg = B.sims[s].cfg;
   const int64_t f = (int64_t)J.lo + threadIdx.x;
   constexpr bool DEFER = SOLVER == SOLVER_AIRS;
-  __shared__ double2 recs[HFV_BLOCK * 4];  // the block's records, written out as whole lines
+  // LDS: the staged records, reused for the block's output records (written out as whole lines)
+  constexpr int STAGE_D = STATIC ? HFV_STAGE_MAX * HFV_FR_LDS : 0;
+  constexpr int REC_D = HFV_BLOCK * 8;
+  __shared__ double lds[STAGE_D > REC_D ? STAGE_D : REC_D];
   __shared__ uint8_t curs[HFV_BLOCK];
+  __shared__ int32_t scl[STATIC ? HFV_STAGE_MAX : 1];
+  double2* recs = (double2*)lds;
+  const int ns = STATIC ? (int)B.F.scnt[J.chunk] : 0;
   bool computed = false;
   bool cur = false;
   Events ev;
   double outv[7];
 #pragma unroll
   for (int k = 0; k < 7; ++k) outv[k] = 0.;
+  int32_t l = 0, r = -1;
+  bool la = false, ra = false;
+  int32_t myc = 0;
+  if (STATIC && (int)threadIdx.x < ns) myc = B.F.scell[(uint64_t)J.chunk * HFV_STAGE_MAX + threadIdx.x];
   if (f < J.hi) {
-    const int32_t l = B.F.left[f], r = B.F.right[f];
-    const bool la = B.C.act[l], ra = r >= 0 && B.C.act[r];
+    l = B.F.left[f];
+    r = B.F.right[f];
+    la = B.C.act[l];
+    ra = r >= 0 && B.C.act[r];
     cur = la || ra;
     B.F.cur[f] = cur ? 1 : 0;
-    if (cur) {
-      computed = face_flux<T, SOLVER, DEFER, STATIC, SoaCell>(B, cfg, f, l, r, la, ra, outv, ev);
+  }
+  if (STATIC) {
+    if ((int)threadIdx.x < ns) scl[threadIdx.x] = myc;
+    const int ncur = __syncthreads_count(cur);  // also publishes scl
+    if (ns > 0 && ncur > 0) {  // block-uniform
+      constexpr int NR = HFV_STAGE_MAX * (HFV_FR_W / 2) / HFV_BLOCK;
+      constexpr int NE = (HFV_STAGE_MAX * HFV_EX_W + HFV_BLOCK - 1) / HFV_BLOCK;
+      static_assert(HFV_STAGE_MAX * (HFV_FR_W / 2) % HFV_BLOCK == 0, "record slots per thread");
+      double2 rv[NR];
+      double xv[NE];
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k >> 4, q = k & 15;
+        if (c < ns && q < HFV_FR_USED / 2) rv[j] = ((const double2*)(B.C.fr + HFV_FR_W * (uint64_t)(uint32_t)scl[c]))[q];
+      }
+#pragma unroll
+      for (int j = 0; j < NE; ++j) {
+        const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / HFV_EX_W, q = k - HFV_EX_W * c;
+        if (c < ns) xv[j] = B.C.extrap[HFV_EX_W * (uint64_t)(uint32_t)scl[c] + q];
+      }
+#pragma unroll
+      for (int j = 0; j < NR; ++j) {
+        const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k >> 4, q = k & 15;
+        if (c < ns && q < HFV_FR_USED / 2) {
+          lds[c * HFV_FR_LDS + 2 * q] = rv[j].x;
+          lds[c * HFV_FR_LDS + 2 * q + 1] = rv[j].y;
+        }
+      }
+#pragma unroll
+      for (int j = 0; j < NE; ++j) {
+        const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / HFV_EX_W, q = k - HFV_EX_W * c;
+        if (c < ns) lds[c * HFV_FR_LDS + HFV_FR_EX + q] = xv[j];
+      }
+      __syncthreads();
+      if (cur) {
+        const uchar2 fl = *(const uchar2*)(B.F.floc + 2 * (uint64_t)f);
+        const RecCell L(lds + fl.x * HFV_FR_LDS, B.C, l);
+        const RecCell R(lds + (r >= 0 ? fl.y : fl.x) * HFV_FR_LDS, B.C, r);
+        computed = face_flux<T, SOLVER, DEFER, true, RecCell>(B, cfg, f, L, R, r, la, ra, outv, ev);
+      }
+      __syncthreads();  // the staged records are dead from here on
+    } else if (cur) {
+      computed = face_flux<T, SOLVER, DEFER, true, SoaCell>(B, cfg, f, SoaCell(B.C, cfg.gd, l),
+                                                            SoaCell(B.C, cfg.gd, r), r, la, ra, outv, ev);
     }
+  } else if (cur) {
+    computed = face_flux<T, SOLVER, DEFER, false, SoaCell>(B, cfg, f, SoaCell(B.C, cfg.gd, l), SoaCell(B.C, cfg.gd, r),
+                                                           r, la, ra, outv, ev);
   }
   // records of faces with an active cell (deferred ones zero here, overwritten by k_flux_fallback)
   curs[threadIdx.x] = cur ? 1 : 0;
@@ -938,7 +1126,9 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_flux_fallback(Batch B) {
     const bool la = B.C.act[l], ra = r >= 0 && B.C.act[r];
     Events ev;
     double outv[7];
-    face_flux<T, SOLVER_AIRS, false, false, SoaCell>(B, B.sims[s].cfg, f, l, r, la, ra, outv, ev);
+    const Cfg& cfg = B.sims[s].cfg;
+    face_flux<T, SOLVER_AIRS, false, false, SoaCell>(B, cfg, f, SoaCell(B.C, cfg.gd, l), SoaCell(B.C, cfg.gd, r), r,
+                                                     la, ra, outv, ev);
     store_rec(B.F.rec, f, outv);
     flush_events(ev, B.cnt, B.eng, s);
   }
@@ -1072,6 +1262,21 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_pack_pc(Batch B, int64_t lo, int6
   store_pc(B.C.pc, i, ld5(B.C.prim, B.C.n, i), ld3(B.C.cen, B.C.n, i));
 }
 
+// The flux records of every loaded staged simulation's cells from the SoA fields (after the
+// host changed the state or the mesh, or a phase call changed fields on its own).
+__global__ void __launch_bounds__(HFV_BLOCK) k_pack_fr(Batch B) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0 || !B.sims[J.sim].staged) return;
+  const int64_t i = (int64_t)J.lo + threadIdx.x;
+  if (i >= J.hi) return;
+  const Cells& C = B.C;
+  const int64_t n = C.n;
+  const Gas<double>& gd = B.sims[J.sim].cfg.gd;
+  const St<double> p = ld5(C.prim, n, i);
+  const Gr<double> g = ldg(C.grad, n, i);
+  store_fr(C.fr, i, p, fr_rate(p, g, ld3(C.vrel, n, i), gd), fr_cs(p, gd), g, C.dt[i], ld3(C.gc, n, i));
+}
+
 // glam::DMat3 as three columns
 struct M3 {
   V3<double> x, y, z;
@@ -1121,7 +1326,7 @@ __device__ __forceinline__ void boundary_neighbour(const Faces& F, const Cfg& cf
 // 1D/2D meshes recomputing them reads fewer bytes than the precomputed lines, so those stay dynamic.)
 template <bool STATIC>
 __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const SimEngine& E, int64_t i, int do_grad,
-                                          int do_ts, uint64_t* dti_out, Events& ev) {
+                                          int do_ts, bool pack_fr, double* frv, uint64_t* dti_out, Events& ev) {
   const Cells& C = B.C;
   const Faces& F = B.F;
   const int64_t n = C.n, nf = F.nf;
@@ -1261,9 +1466,11 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
   // ---- Particle::timestep (part.rs:42-84)
   const St<double> c = ld5(C.cons, n, i);
   double dt_req;
+  V3<double> vrel_now;
   if (c.c[0] == 0.) {
     st3(C.v, n, i, mk3(0., 0., 0.));
     dt_req = INFINITY;
+    if (pack_fr) vrel_now = ld3(C.vrel, n, i);
   } else {
     double m_inv = 1. / c.c[0];
     V3<double> loc = ld3(C.loc, n, i);
@@ -1310,6 +1517,7 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
     V3<double> v_rel = fluid_v - v;
     st3(C.v, n, i, v);
     st3(C.vrel, n, i, v_rel);
+    vrel_now = v_rel;
     C.maor[i] = 0.;
     double v_max = hmax(C.msv[i], len(v_rel) + cs);
     C.msv[i] = 0.;
@@ -1318,9 +1526,12 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
   // ---- timesteps_apply
   int8_t bin = C.bin[i];
   uint64_t dti = make_integer_timestep(dt_req, bin, HFV_NUM_TIME_BINS, E.ti_current, E.time_base_inv);
-  C.dt[i] = (double)dti * E.time_base;
+  const double dt_new = (double)dti * E.time_base;
+  C.dt[i] = dt_new;
   C.bin[i] = get_time_bin(dti);
   *dti_out = dti;
+  // the cell's flux record for the next step (its prim, gradients, v_rel and dt are final here)
+  if (pack_fr) fill_fr(frv, prim, fr_rate(prim, gr, vrel_now, cfg.gd), fr_cs(prim, cfg.gd), gr, dt_new, gcen);
 }
 
 template <bool STATIC>
@@ -1335,7 +1546,16 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_GRAD_WAVES) k_grad(Batch B, int
   const bool a = i < J.hi && B.C.act[i];
   uint64_t my = ~0ull;
   Events ev;
-  if (a) grad_cell<STATIC>(B, cfg, E, i, do_grad, do_ts, &my, ev);
+  const bool pack = do_ts && B.sims[s].staged;  // block-uniform
+  double frv[HFV_FR_W];
+#pragma unroll
+  for (int k = 0; k < HFV_FR_W; ++k) frv[k] = 0.;
+  if (a) grad_cell<STATIC>(B, cfg, E, i, do_grad, do_ts, pack, frv, &my, ev);
+  if (pack) {
+    __shared__ double wbuf[HFV_BLOCK / 64][64 * 17];
+    const uint64_t has = __ballot(a);
+    if (has) store_fr_wave(B.C.fr, i - (int64_t)__lane_id(), has, frv, wbuf[threadIdx.x >> 6]);
+  }
   if (do_ts) {
     uint64_t t = block_min_u64(my);
     if (threadIdx.x == 0) B.P.ti[J.chunk] = t;
@@ -1481,9 +1701,11 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_limit(Batch B, int canon) {
     }
     C.bin[i] = bin;
     C.dt[i] = dt;
+    if (B.sims[s].staged) C.fr[HFV_FR_W * (uint64_t)i + HFV_FR_DT] = dt;
   }
   if (C.act[i]) {
-    st5(C.extrap, n, i, vacuum<double>());
+#pragma unroll
+    for (int q = 0; q < 5; ++q) C.extrap[HFV_EX_W * (uint64_t)i + q] = 0.;
     if (cfg.motion != MOT_FIXED) {
       double rho = C.prim[i];
       if (rho > 0.) {

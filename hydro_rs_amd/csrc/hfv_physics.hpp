@@ -46,6 +46,14 @@ struct Events {
 // ------------------------------------------------------------------ math wrappers
 __device__ __host__ __forceinline__ double hsqrt(double x) { return sqrt(x); }
 __device__ __host__ __forceinline__ float hsqrt(float x) { return sqrtf(x); }
+// Division and square root in the flux path's working type T: IEEE in FP64 (bit parity with the FP64
+// reference); in the FP32 flux path the hardware reciprocal and square root (v_rcp_f32, v_sqrt_f32,
+// about 1 ulp each; two instructions instead of an IEEE-exact expansion of about ten), whose error is
+// far inside the FP32 path's accuracy budget (L1 density error against FP64).
+__device__ __host__ __forceinline__ double fdiv(double a, double b) { return a / b; }
+__device__ __forceinline__ float fdiv(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
+__device__ __host__ __forceinline__ double fsqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 __device__ __host__ __forceinline__ double hpow(double x, double y) { return pow(x, y); }
 __device__ __host__ __forceinline__ float hpow(float x, float y) { return powf(x, y); }
 __device__ __host__ __forceinline__ double habs(double x) { return fabs(x); }
@@ -99,7 +107,7 @@ __device__ __host__ __forceinline__ T len2(V3<T> a) {
 }
 template <class T>
 __device__ __host__ __forceinline__ T len(V3<T> a) {
-  return hsqrt(dot(a, a));
+  return fsqrt(dot(a, a));
 }
 template <class T>
 __device__ __host__ __forceinline__ V3<T> cross(V3<T> a, V3<T> b) {
@@ -211,8 +219,8 @@ __device__ __forceinline__ T gas_p_from_u(const Gas<T>& g, T u, T rho) {
 }
 template <class T>
 __device__ __forceinline__ T sound_speed(const Gas<T>& g, T p, T rho_inv) {
-  if (!g.isothermal) return hsqrt(g.gamma * p * rho_inv);
-  return hsqrt(g.iso_u * g.gamma * (g.gamma - T(1)));
+  if (!g.isothermal) return fsqrt(g.gamma * p * rho_inv);
+  return fsqrt(g.iso_u * g.gamma * (g.gamma - T(1)));
 }
 
 // ------------------------------------------------------------------ physical_quantities.rs:245-267
@@ -266,7 +274,7 @@ struct Star {
 // ------------------------------------------------------------------ riemann_solver.rs:96-256
 template <class T>
 __device__ __forceinline__ T shock_speed(T v, T a, T pdps, const Gas<T>& g) {
-  return v - a * hsqrt(T(0.5) * g.gp1dg * pdps + g.gm1d2g);
+  return v - a * fsqrt(T(0.5) * g.gp1dg * pdps + g.gm1d2g);
 }
 template <class T>
 __device__ __forceinline__ T rarefaction_tail_speed(T u, T a, T pdps, const Gas<T>& g) {
@@ -275,7 +283,7 @@ __device__ __forceinline__ T rarefaction_tail_speed(T u, T a, T pdps, const Gas<
 template <class T>
 __device__ __forceinline__ St<T> sample_rarefaction_fan(const St<T>& s, T a, T v, V3<T> n, const Gas<T>& g) {
   T v_half = g.tdgp1 * (a + g.half_gm1 * v);
-  T base = g.tdgp1 + g.gm1dgp1 / a * v;
+  T base = g.tdgp1 + fdiv(g.gm1dgp1, a) * v;
   return mkst(s.c[0] * hpow(base, g.tdgm1), vel(s) + ((v_half - v) * n), s.c[4] * hpow(base, g.gamma_tdgm1));
 }
 template <class T>
@@ -287,25 +295,25 @@ __device__ __forceinline__ St<T> sample_from_star(const Star<T>& star, const St<
                                                T v_r, T a_l, T a_r, V3<T> n, const Gas<T>& g) {
   if (star.u < T(0)) {
     if (star.p > right.c[4]) {  // right shock
-      if (shock_speed(v_r, -a_r, star.p / right.c[4], g) > T(0))
+      if (shock_speed(v_r, -a_r, fdiv(star.p, right.c[4]), g) > T(0))
         return sample_middle_state(star.rho_r, star.u, star.p, right, v_r, n);
       return right;
     }
     // right rarefaction
     if (v_r - (-a_r) > T(0)) {
-      if (rarefaction_tail_speed(star.u, -a_r, star.p / right.c[4], g) < T(0))
+      if (rarefaction_tail_speed(star.u, -a_r, fdiv(star.p, right.c[4]), g) < T(0))
         return sample_rarefaction_fan(right, -a_r, v_r, n, g);
       return sample_middle_state(star.rho_r, star.u, star.p, right, v_r, n);
     }
     return right;
   } else if (star.p > left.c[4]) {  // left shock
-    if (shock_speed(v_l, a_l, star.p / left.c[4], g) < T(0))
+    if (shock_speed(v_l, a_l, fdiv(star.p, left.c[4]), g) < T(0))
       return sample_middle_state(star.rho_l, star.u, star.p, left, v_l, n);
     return left;
   }
   // left rarefaction
   if (v_l - a_l < T(0)) {
-    if (rarefaction_tail_speed(star.u, a_l, star.p / left.c[4], g) > T(0))
+    if (rarefaction_tail_speed(star.u, a_l, fdiv(star.p, left.c[4]), g) > T(0))
       return sample_rarefaction_fan(left, a_l, v_l, n, g);
     return sample_middle_state(star.rho_l, star.u, star.p, left, v_l, n);
   }
@@ -525,9 +533,9 @@ __device__ __forceinline__ Star<T> pvrs_star(const St<T>& l, const St<T>& r, T v
   T a_bar = T(0.5) * (a_l + a_r);
   Star<T> sv;
   sv.p = pv_p_star(rho_bar, p_bar, a_bar, v_l, v_r);
-  sv.u = T(0.5) * ((v_l + v_r) + (l.c[4] - r.c[4]) / (rho_bar * a_bar));
-  sv.rho_l = l.c[0] + (v_l - sv.u) * rho_bar / a_bar;
-  sv.rho_r = r.c[0] + (sv.u - v_r) * rho_bar / a_bar;
+  sv.u = T(0.5) * ((v_l + v_r) + fdiv(l.c[4] - r.c[4], rho_bar * a_bar));
+  sv.rho_l = l.c[0] + fdiv((v_l - sv.u) * rho_bar, a_bar);
+  sv.rho_r = r.c[0] + fdiv((sv.u - v_r) * rho_bar, a_bar);
   return sv;
 }
 template <class T>
@@ -559,9 +567,9 @@ template <class T>
 __device__ __forceinline__ St<T> hllc_flux_star(const St<T>& s, T v, T e, T S, T s_star, V3<T> n) {
   T s_m_v = S - v;
   T rho = s.c[0];
-  T starfac = rho * s_m_v / (S - s_star);
+  T starfac = fdiv(rho * s_m_v, S - s_star);
   V3<T> v_star = ((s_star - v) * n) + vel(s);
-  T e_star = e + (s_star - v) * (s_star + s.c[4] / (rho * s_m_v));
+  T e_star = e + (s_star - v) * (s_star + fdiv(s.c[4], rho * s_m_v));
   St<T> u_star = starfac * mkst(T(1), v_star, e_star);
   St<T> u_l = mkst(rho, rho * vel(s), rho * e);
   return S * (u_star - u_l);
@@ -585,7 +593,7 @@ __device__ __forceinline__ Star<T> star_for(int kind, double threshold, const St
       Star<T> st = pvrs_star(l, r, v_l, v_r, a_l, a_r);
       T p_max = hmax(l.c[4], r.c[4]);
       T p_min = hmin(l.c[4], r.c[4]);
-      if (p_max / p_min < (T)threshold) return st;
+      if (fdiv(p_max, p_min) < (T)threshold) return st;
       return exact_star(l, r, v_l, v_r, a_l, a_r, g, ev);
     }
     case SOLVER_TSRS:
@@ -611,8 +619,8 @@ __device__ __forceinline__ St<T> riemann_flux(const SolverCfg& sc, const St<T>& 
     return v * (v > T(0) ? left : right);
   }
   if (sc.kind == SOLVER_HLLC) {  // hllc.rs:12-92
-    T rho_l_inv = T(1) / left.c[0];
-    T rho_r_inv = T(1) / right.c[0];
+    T rho_l_inv = fdiv(T(1), left.c[0]);
+    T rho_r_inv = fdiv(T(1), right.c[0]);
     T v_l = dot(vel(left), n);
     T v_r = dot(vel(right), n);
     T a_l = sound_speed(g, left.c[4], rho_l_inv);
@@ -625,13 +633,13 @@ __device__ __forceinline__ St<T> riemann_flux(const SolverCfg& sc, const St<T>& 
     T ppvrs = T(0.5) * (left.c[4] + right.c[4]) - T(0.125) * v_r_m_v_l * (left.c[0] + right.c[0]) * (a_l + a_r);
     T p_star = hmax(ppvrs, T(0));
     T q_l = T(1);
-    if (p_star > left.c[4] && left.c[4] > T(0)) q_l = hsqrt(T(1) + T(0.5) * g.gp1dg * (p_star / left.c[4] - T(1)));
+    if (p_star > left.c[4] && left.c[4] > T(0)) q_l = fsqrt(T(1) + T(0.5) * g.gp1dg * (fdiv(p_star, left.c[4]) - T(1)));
     T q_r = T(1);
-    if (p_star > right.c[4] && right.c[4] > T(0)) q_r = hsqrt(T(1) + T(0.5) * g.gp1dg * (p_star / right.c[4] - T(1)));
+    if (p_star > right.c[4] && right.c[4] > T(0)) q_r = fsqrt(T(1) + T(0.5) * g.gp1dg * (fdiv(p_star, right.c[4]) - T(1)));
     T s_l_m_v_l = -a_l * q_l;
     T s_r_m_v_r = a_r * q_r;
-    T s_star = (right.c[4] - left.c[4] + left.c[0] * v_l * s_l_m_v_l - right.c[0] * v_r * s_r_m_v_r) /
-               (left.c[0] * s_l_m_v_l - right.c[0] * s_r_m_v_r);
+    T s_star = fdiv(right.c[4] - left.c[4] + left.c[0] * v_l * s_l_m_v_l - right.c[0] * v_r * s_r_m_v_r,
+                    left.c[0] * s_l_m_v_l - right.c[0] * s_r_m_v_r);
     St<T> flux;
     if (s_star >= T(0)) {
       T v_l2 = len2(vel(left));
@@ -652,14 +660,14 @@ __device__ __forceinline__ St<T> riemann_flux(const SolverCfg& sc, const St<T>& 
   // blanket impl for star solvers (riemann_solver.rs:258-287)
   T v_l = dot(vel(left), n);
   T v_r = dot(vel(right), n);
-  T a_l = sound_speed(g, left.c[4], T(1) / left.c[0]);
-  T a_r = sound_speed(g, right.c[4], T(1) / right.c[0]);
+  T a_l = sound_speed(g, left.c[4], fdiv(T(1), left.c[0]));
+  T a_r = sound_speed(g, right.c[4], fdiv(T(1), right.c[0]));
   T v_r_m_v_l = v_r - v_l;
   St<T> w;
   if (is_vacuum(left, right, a_l, a_r, v_r_m_v_l, g)) {
     w = vacuum_sample(left, right, v_l, v_r, a_l, a_r, n, g);
   } else if (DEFER) {
-    if (!(hmax(left.c[4], right.c[4]) / hmin(left.c[4], right.c[4]) < (T)sc.threshold)) {
+    if (!(fdiv(hmax(left.c[4], right.c[4]), hmin(left.c[4], right.c[4])) < (T)sc.threshold)) {
       ev.deferred = 1;
       return vacuum<T>();
     }
@@ -668,7 +676,7 @@ __device__ __forceinline__ St<T> riemann_flux(const SolverCfg& sc, const St<T>& 
     Star<T> star;
     bool need_fp64 = (sizeof(T) != sizeof(double)) &&
                      (sc.kind == SOLVER_EXACT ||
-                      (sc.kind == SOLVER_AIRS && !(hmax(left.c[4], right.c[4]) / hmin(left.c[4], right.c[4]) <
+                      (sc.kind == SOLVER_AIRS && !(fdiv(hmax(left.c[4], right.c[4]), hmin(left.c[4], right.c[4])) <
                                                    (T)sc.threshold)));
     if (need_fp64) {
       St<double> ld = cvtst<double>(left), rd = cvtst<double>(right);
@@ -695,11 +703,11 @@ __device__ __forceinline__ T pairwise_single(T q_l, T q_r, T q_dash, T dx_fac) {
   T delta2 = T(0.25) * q_diff;
   if (q_l < q_r) {
     T q_min = hmin(q_l, q_r);
-    T qmin = ((q_min - delta1) * q_min > T(0)) ? q_min - delta1 : q_min * habs(q_min) / (habs(q_min) + delta1);
+    T qmin = ((q_min - delta1) * q_min > T(0)) ? q_min - delta1 : fdiv(q_min * habs(q_min), habs(q_min) + delta1);
     return hmax(qmin, hmin(q_bar + delta2, q_dash));
   }
   T q_max = hmax(q_l, q_r);
-  T qplu = ((q_max + delta1) * q_max > T(0)) ? q_max + delta1 : q_max * habs(q_max) / (habs(q_max) + delta1);
+  T qplu = ((q_max + delta1) * q_max > T(0)) ? q_max + delta1 : fdiv(q_max * habs(q_max), habs(q_max) + delta1);
   return hmin(qplu, hmax(q_bar - delta2, q_dash));
 }
 template <class T>

@@ -39,7 +39,7 @@ def morton_order(idx):
     return np.argsort(code, kind="stable")
 
 
-def jittered_grid(dim, side, box, jitter, rng, order="grid"):
+def jittered_grid(dim, side, box, jitter, rng, order="morton"):
     """Generators at cell centres of a side^dim grid, each coordinate jittered by U(-jitter, jitter)*dx.
     order "grid": row-major (x slowest); "morton": the same generators in Z-order, so that cells close
     in space are close in index (neighbour gathers stay within a small index window)."""
