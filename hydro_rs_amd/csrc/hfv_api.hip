@@ -6,6 +6,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdint>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -146,6 +148,9 @@ struct hfv_ctx {
   bool use_static = true;    // precomputed geometry for fixed meshes (hfv_set_geometry_mode)
   std::vector<int> v_zero;   // per slot: every particle velocity is +0 (fixed meshes keep it so)
   std::vector<int> has_stage;  // per slot: some face chunk has a staging list
+  // per slot, set by hfv_autotune: LDS-staged flux tiling allowed (1) or not (0); precomputed
+  // gradient geometry on (1), off (0) or by dimension (-1: on for 3D)
+  std::vector<int> tune_stage, tune_sgrad;
   bool any_static = false, any_dynamic = false, any_static_grad = false, any_dynamic_grad = false;
   SimDesc* ddesc = nullptr;
   SimEngine* deng = nullptr;
@@ -339,6 +344,8 @@ int alloc_batch(hfv_ctx* c) {
   F.fs = dalloc<double>(c, 4 * nf);
   c->v_zero.assign(S, 1);
   c->has_stage.assign(S, 0);
+  c->tune_stage.assign(S, 1);
+  c->tune_sgrad.assign(S, -1);
   c->ddesc = dalloc<SimDesc>(c, S);
   c->deng = dalloc<SimEngine>(c, S);
   for (void* p : c->bufs)
@@ -637,8 +644,8 @@ int refresh_geometry(hfv_ctx* c, int s) {
   const hfv_params& p = c->params[s];
   const bool st = c->use_static && p.motion == HFV_FIXED && c->v_zero[s] && c->nused[s] > 0;
   c->desc[s].static_geom = st ? 1 : 0;
-  c->desc[s].static_grad = st && p.dim == 3 ? 1 : 0;
-  c->desc[s].staged = st && c->has_stage[s] ? 1 : 0;
+  c->desc[s].static_grad = st && (c->tune_sgrad[s] < 0 ? p.dim == 3 : c->tune_sgrad[s] != 0) ? 1 : 0;
+  c->desc[s].staged = st && c->has_stage[s] && c->tune_stage[s] ? 1 : 0;
   c->fr_dirty = true;
   HIP_OK(hipMemcpyAsync(c->ddesc + s, &c->desc[s], sizeof(SimDesc), hipMemcpyHostToDevice, c->stream));
   if (c->nused[s] > 0)  // neighbour primitives/centroids of the gradient sweeps
@@ -1352,6 +1359,125 @@ int hfv_get_geometry_mode(hfv_ctx* c, int slot, int* is_static) {
   TRY(check_slot(c, slot));
   if (!is_static) return set_err(HFV_ERR_INVALID, "NULL argument");
   *is_static = c->loaded[slot] ? c->desc[slot].static_geom : 0;
+  return HFV_OK;
+}
+
+int hfv_set_tuning(hfv_ctx* c, int slot, int staged, int static_grad) {
+  TRY(check_slot(c, slot));
+  if (staged < -1 || staged > 1 || static_grad < -1 || static_grad > 1)
+    return set_err(HFV_ERR_INVALID, "tuning values must be -1, 0 or 1");
+  HIP_OK(hipStreamSynchronize(c->stream));
+  c->tune_stage[slot] = staged < 0 ? 1 : staged;
+  c->tune_sgrad[slot] = static_grad;
+  if (c->loaded[slot]) TRY(refresh_geometry(c, slot));
+  return HFV_OK;
+}
+
+int hfv_get_tuning(hfv_ctx* c, int slot, int* staged, int* static_grad) {
+  TRY(check_slot(c, slot));
+  if (staged) *staged = c->loaded[slot] ? c->desc[slot].staged : 0;
+  if (static_grad) *static_grad = c->loaded[slot] ? c->desc[slot].static_grad : 0;
+  return HFV_OK;
+}
+
+// Size classes of the autotuner: dimension x cell-count bucket.
+static const int64_t kTuneEdges[] = {0, 1024, 8192, 65536, INT64_MAX};
+
+int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int max_out, int* n_out) {
+  if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
+  if (launches < 1 || rounds < 1) return set_err(HFV_ERR_INVALID, "launches and rounds must be >= 1");
+  HIP_OK(hipSetDevice(c->device));
+  TRY(sync_counters(c));
+  // classes of the live static-geometry simulations
+  struct Cls {
+    int dim, bucket;
+    std::vector<int> slots;
+    bool can_stage = false;
+    double t[4] = {0, 0, 0, 0};
+    double u[4] = {0, 0, 0, 0};
+    int best = -1;
+  };
+  std::vector<Cls> cls;
+  for (int s = 0; s < c->nslots; ++s) {
+    if (!c->loaded[s] || c->heng[s].done || !c->desc[s].static_geom) continue;
+    int b = 0;
+    while (c->nused[s] >= kTuneEdges[b + 1]) ++b;
+    const int dim = c->params[s].dim;
+    size_t k = 0;
+    while (k < cls.size() && !(cls[k].dim == dim && cls[k].bucket == b)) ++k;
+    if (k == cls.size()) {
+      cls.push_back(Cls{});
+      cls[k].dim = dim;
+      cls[k].bucket = b;
+    }
+    cls[k].slots.push_back(s);
+    if (c->has_stage[s]) cls[k].can_stage = true;
+  }
+  auto apply = [&](const Cls& k, int cand) -> int {
+    for (int s : k.slots) {
+      c->tune_stage[s] = cand >> 1;
+      c->tune_sgrad[s] = cand & 1;
+      TRY(refresh_geometry(c, s));
+    }
+    return HFV_OK;
+  };
+  auto updates = [&]() -> long long {
+    long long u = 0;
+    for (int s = 0; s < c->nslots; ++s)
+      if (c->loaded[s]) u += c->heng[s].cell_updates;
+    return u;
+  };
+  const bool prof = c->prof;
+  c->prof = false;  // graph replays, as in a run
+  int rc = HFV_OK;
+  bool all_done = false;
+  for (auto& k : cls) {
+    std::vector<int> cands;
+    for (int cand = 0; cand < 4; ++cand)
+      if (k.can_stage || !(cand >> 1)) cands.push_back(cand);
+    for (int r = 0; r < rounds && !all_done && rc == HFV_OK; ++r)
+      for (int cand : cands) {
+        if ((rc = apply(k, cand)) != HFV_OK) break;
+        pack_fr_if_dirty(c);
+        if ((rc = sync_counters(c)) != HFV_OK) break;
+        const long long u0 = updates();
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int j = 0; j < launches && rc == HFV_OK; ++j) rc = launch_step(c);
+        if (rc != HFV_OK) break;
+        HIP_OK(hipStreamSynchronize(c->stream));
+        const auto t1 = std::chrono::steady_clock::now();
+        if ((rc = sync_counters(c)) != HFV_OK) break;
+        k.t[cand] += std::chrono::duration<double>(t1 - t0).count();
+        k.u[cand] += (double)(updates() - u0);
+        all_done = true;
+        for (int s = 0; s < c->nslots; ++s)
+          if (c->loaded[s] && !c->heng[s].done) all_done = false;
+        if (all_done) break;
+      }
+    if (rc != HFV_OK) break;
+    double best = 0.;
+    for (int cand : cands)
+      if (k.u[cand] > 0. && (k.best < 0 || k.t[cand] / k.u[cand] < best)) {
+        k.best = cand;
+        best = k.t[cand] / k.u[cand];
+      }
+    if (k.best < 0) k.best = (k.can_stage ? 2 : 0) | (k.dim == 3 ? 1 : 0);  // nothing measured: defaults
+    if ((rc = apply(k, k.best)) != HFV_OK) break;
+  }
+  c->prof = prof;
+  if (rc != HFV_OK) return rc;
+  const int n = (int)cls.size();
+  if (n_out) *n_out = n;
+  for (int k = 0; k < n && k < max_out && out; ++k) {
+    hfv_tune_class& o = out[k];
+    o.dim = cls[k].dim;
+    o.cells_lo = kTuneEdges[cls[k].bucket];
+    o.cells_hi = kTuneEdges[cls[k].bucket + 1];
+    o.n_sims = (int)cls[k].slots.size();
+    o.staged = cls[k].best >> 1;
+    o.static_grad = cls[k].best & 1;
+    for (int q = 0; q < 4; ++q) o.ns_per_update[q] = cls[k].u[q] > 0. ? 1e9 * cls[k].t[q] / cls[k].u[q] : -1.;
+  }
   return HFV_OK;
 }
 

@@ -1016,7 +1016,7 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   __shared__ uint8_t curs[HFV_BLOCK];
   __shared__ int32_t scl[STATIC ? HFV_STAGE_MAX : 1];
   double2* recs = (double2*)lds;
-  const int ns = STATIC ? (int)B.F.scnt[J.chunk] : 0;
+  const int ns = STATIC && B.sims[s].staged ? (int)B.F.scnt[J.chunk] : 0;  // records kept only when staged
   bool computed = false;
   bool cur = false;
   Events ev;

@@ -233,6 +233,20 @@ class EulerEqnsFvs:
         _lib.check(self.lib.hfv_get_geometry_mode(self.ptr, int(slot), C.byref(v)))
         return bool(v.value)
 
+    def autotune(self, launches=8, rounds=2):
+        """Choose the flux tiling and gradient geometry per size class by timing real launches
+        (hfv_autotune; bit-identical candidates, so the launches are ordinary steps)."""
+        return _autotune(self.lib, self.ptr, launches, rounds)
+
+    def set_tuning(self, slot=0, staged=-1, static_grad=-1):
+        """Flux tiling (staged) and gradient geometry (static_grad) of one slot: 1/0, -1 = default."""
+        _lib.check(self.lib.hfv_set_tuning(self.ptr, int(slot), int(staged), int(static_grad)))
+
+    def tuning(self, slot=0):
+        a, b = C.c_int(0), C.c_int(0)
+        _lib.check(self.lib.hfv_get_tuning(self.ptr, int(slot), C.byref(a), C.byref(b)))
+        return {"staged": a.value, "static_grad": b.value}
+
 
 def _engine_dict(e):
     d = {k: getattr(e, k) for k, _ in _lib.EngineInfo._fields_ if k != "ts"}
@@ -341,6 +355,29 @@ class Batch:
         v = C.c_int(0)
         _lib.check(self.lib.hfv_get_geometry_mode(self.ptr, int(slot), C.byref(v)))
         return bool(v.value)
+
+    def autotune(self, launches=8, rounds=2):
+        """Choose the flux tiling and gradient geometry per size class by timing real launches
+        (hfv_autotune; bit-identical candidates, so the launches are ordinary steps)."""
+        return _autotune(self.lib, self.ptr, launches, rounds)
+
+    def set_tuning(self, slot=0, staged=-1, static_grad=-1):
+        """Flux tiling (staged) and gradient geometry (static_grad) of one slot: 1/0, -1 = default."""
+        _lib.check(self.lib.hfv_set_tuning(self.ptr, int(slot), int(staged), int(static_grad)))
+
+    def tuning(self, slot=0):
+        a, b = C.c_int(0), C.c_int(0)
+        _lib.check(self.lib.hfv_get_tuning(self.ptr, int(slot), C.byref(a), C.byref(b)))
+        return {"staged": a.value, "static_grad": b.value}
+
+
+def _autotune(lib, ptr, launches, rounds):
+    out = (_lib.TuneClass * 64)()
+    n = C.c_int(0)
+    _lib.check(lib.hfv_autotune(ptr, int(launches), int(rounds), out, 64, C.byref(n)))
+    return [{"dim": o.dim, "n_sims": o.n_sims, "cells": [o.cells_lo, o.cells_hi], "staged": o.staged,
+             "static_grad": o.static_grad, "ns_per_update": [round(x, 4) for x in o.ns_per_update]}
+            for o in out[: min(n.value, 64)]]
 
 
 def riemann_batch(cfg: Config, WL, WR, vface, nrm):

@@ -44,7 +44,7 @@ def test_struct_layouts_match_header(tmp_path):
 
     structs = {"hfv_params": _lib.Params, "hfv_timestep_info": _lib.TimestepInfo,
                "hfv_mesh_arrays": _lib.MeshArrays, "hfv_state": _lib.State, "hfv_engine_info": _lib.EngineInfo,
-               "hfv_stats": _lib.Stats}
+               "hfv_stats": _lib.Stats, "hfv_tune_class": _lib.TuneClass}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "hfv.h"', "int main(void){"]
     for cname, py in structs.items():
         lines.append(f'printf("%zu\\n", sizeof({cname}));')

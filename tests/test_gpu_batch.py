@@ -144,3 +144,58 @@ def test_hard_error_matches_oracle_step(ora):
     bi = b.engine_info(0)
     assert bi["error_code"] == ex.value.code and bi["step_count"] == ri["step_count"]
     assert b.engine_info(1)["done"] and b.engine_info(1)["error_code"] == 0
+
+
+@pytest.mark.parametrize("precision", [fvs.FP64, fvs.FP32_FLUX])
+@pytest.mark.parametrize("staged,static_grad", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_tuning_choices_are_bit_identical(staged, static_grad, precision):
+    """Each flux tiling / gradient geometry choice, forced on every slot, ends a run on the same bits
+    as the defaults (1D reflective, 2D periodic, 3D periodic)."""
+    probs = [ics.sedov_2d(side=40, t_end=0.004, seed=1, precision=precision),
+             ics.riemann_tube_1d(600, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, t_end=0.02, precision=precision),
+             ics.kelvin_helmholtz_3d(side=10, t_end=0.05, precision=precision)]
+    a = fvs.Batch.for_problems(probs)
+    for s in range(len(probs)):
+        a.set_tuning(s, staged, static_grad)
+    a.run(1 << 40)
+    b = fvs.Batch.for_problems(probs)
+    b.run(1 << 40)
+    for s, p in enumerate(probs):
+        ea, eb = a.engine_info(s), b.engine_info(s)
+        say(f"  {p.name}: tuning {a.tuning(s)} steps {ea['step_count']} vs {eb['step_count']} "
+            f"err {ea['error_code']} / {eb['error_code']}")
+        assert ea["error_code"] == 0 and eb["error_code"] == 0, p.name
+        assert ea["step_count"] == eb["step_count"], p.name
+        ga, gb = a.get_state(s), b.get_state(s)
+        for k in ga:
+            assert np.array_equal(ga[k], gb[k]), (p.name, k)
+
+
+def test_autotune_is_bit_identical():
+    """hfv_autotune picks a tiling/geometry per size class by timing real launches; every candidate
+    gives the same bits, so a tuned run ends exactly where an untuned one does."""
+    probs = [ics.sedov_2d(side=40, t_end=0.004, seed=1), ics.sedov_2d(side=12, t_end=0.004, seed=2),
+             ics.riemann_tube_1d(600, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, t_end=0.08),
+             ics.kelvin_helmholtz_3d(side=10, t_end=0.1)]
+    a = fvs.Batch.for_problems(probs)
+    report = a.autotune(launches=3, rounds=1)
+    say(f"[autotune] {report}")
+    assert report, "no size class"
+    dims = sorted({r["dim"] for r in report})
+    assert dims == [1, 2, 3]
+    for r in report:
+        assert r["staged"] in (0, 1) and r["static_grad"] in (0, 1)
+        tried = [x for x in r["ns_per_update"] if x >= 0]
+        assert len(tried) >= 2, r
+    a.run(1 << 40)
+    b = fvs.Batch.for_problems(probs)
+    b.run(1 << 40)
+    for s, p in enumerate(probs):
+        ea, eb = a.engine_info(s), b.engine_info(s)
+        say(f"  {p.name}: steps {ea['step_count']} vs {eb['step_count']} err {ea['error_code']} / {eb['error_code']}")
+        assert ea["done"] and eb["done"], p.name
+        assert ea["error_code"] == 0 and eb["error_code"] == 0, p.name
+        assert ea["step_count"] == eb["step_count"], p.name
+        ga, gb = a.get_state(s), b.get_state(s)
+        for k in ga:
+            assert np.array_equal(ga[k], gb[k]), (p.name, k)
