@@ -5,9 +5,11 @@ python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload all|ensemble|sed
 Headline (value): the batched ensemble -- 512 independent 1D/2D Riemann-problem simulations (cell
 counts log-uniform in [200, 65536], HLLC, FP32 flux / FP64 accumulate) in one device context.  One
 step is one batch launch: every unfinished member advances by one engine step (one
-OptimalOrderRunner::step with its own hierarchical timebins).  W untimed launches, then K timed
-launches bracketed by device synchronisation; value = cell updates (active cells summed over members
-and launches) / wall time of the K launches.
+OptimalOrderRunner::step with its own hierarchical timebins).  The autotuner first times a few real
+launches per size class and candidate (flux tiling, gradient geometry; bit-identical candidates), then
+W untimed launches, then K timed launches bracketed by device synchronisation; value = cell updates
+(active cells summed over members and launches) / wall time of the K launches.  The reported warmup
+counts the tuning launches too.
 
 Printed JSON line: the contract keys plus
   speedup_vs_plain_fp64  value / the plain single-simulation FP64 device path on the same members and
@@ -172,6 +174,29 @@ def flux_kernel_name(fp32, solver, static_geometry=True):
             f"{'true' if static_geometry else 'false'}>(hfv::Batch)")
 
 
+SLOT_KERNELS = {"apply_c2p": ["hfv::k_apply(hfv::Batch, int)"], "mask_predict": ["hfv::k_mask(hfv::Batch, int)"],
+                "wakeup": ["hfv::k_wakeup(hfv::Batch)"], "limit_kick": ["hfv::k_limit(hfv::Batch, int)"]}
+
+
+def slot_kernels(slot, fp32, solver, tuning):
+    """Kernel names that a timing slot's launches consist of (static geometry; the gradient variants
+    the autotuner chose)."""
+    if slot == "flux":
+        return [flux_kernel_name(fp32, solver, True)]
+    if slot == "grad_timestep":
+        chosen = {c["static_grad"] for c in (tuning or [])} or {0}
+        return [f"void hfv::k_grad<{'true' if g else 'false'}>(hfv::Batch, int, int)" for g in sorted(chosen)]
+    return SLOT_KERNELS.get(slot, [])
+
+
+def attach_traffic(rl, label, fp32, solver, tuning):
+    """roofline["traffic"]: HBM bytes per launch of the dominant slot's kernels from profiles/."""
+    names = slot_kernels(rl["kernel"], fp32, solver, tuning)
+    vals = [traffic_for(label, k) for k in names]
+    rl["kernel_name"] = " + ".join(names)
+    rl["traffic"] = sum(vals) if vals and all(v is not None for v in vals) else None
+
+
 # ------------------------------------------------------------------ ensemble (headline)
 def bench_ensemble(args):
     from hydro_rs_amd import ensemble, fvs, ics
@@ -181,6 +206,16 @@ def bench_ensemble(args):
     n_all = sum(p.n for p in probs)
     log(f"[ensemble] {len(probs)} members, {n_all} cells, generated in {time.perf_counter() - t:.1f}s")
     b = fvs.Batch.for_problems(probs)
+    # autotuner (per size class: flux tiling, gradient geometry); its launches are ordinary steps
+    tune_launches = 0
+    tuning = None
+    if args.tune_launches > 0:
+        s0 = [b.engine_info(s)["step_count"] for s in range(len(probs))]
+        t = time.perf_counter()
+        tuning = b.autotune(launches=args.tune_launches, rounds=args.tune_rounds)
+        tune_launches = max(b.engine_info(s)["step_count"] - s0[s] for s in range(len(probs)))
+        log(f"[ensemble] autotune: {tune_launches} launches in {time.perf_counter() - t:.1f}s: "
+            f"{[(c['dim'], c['cells'], c['staged'], c['static_grad']) for c in tuning]}")
     b.run(args.warmup)
     b.synchronize()
     i0 = [b.engine_info(s) for s in range(len(probs))]
@@ -196,17 +231,16 @@ def bench_ensemble(args):
     value = cu / secs
     log(f"[ensemble] fp32: {launches} launches in {secs:.3f}s -> {value:.4e} cell-updates/s")
     res = {"launches": launches, "seconds": secs, "cell_updates": cu, "face_fluxes": ff, "value": value,
+           "autotune": {"launches": tune_launches, "classes": tuning},
            "n_all": n_all, "members_done": sum(1 for i in i1 if i["done"]),
            "members_failed": [s for s, i in enumerate(i1) if i["error_code"]],
            "roofline": roofline(kt, launches, cu, ff, n_all)}
-    if res["roofline"]["kernel"] == "flux":
-        res["roofline"]["kernel_name"] = flux_kernel_name(True, fvs.HLLC, True)
-        res["roofline"]["traffic"] = traffic_for("ensemble512", res["roofline"]["kernel_name"])
+    attach_traffic(res["roofline"], "ensemble512", True, fvs.HLLC, tuning)
     # the same window in FP64 (batched) for the L1 gate
     p64 = [ics.with_precision(p, fvs.FP64) for p in probs]
     b64 = fvs.Batch.for_problems(p64)
     t1 = time.perf_counter()
-    b64.run(args.warmup + launches)
+    b64.run(tune_launches + args.warmup + launches)
     b64.synchronize()
     res["fp64_batched_value"] = sum(b64.engine_info(s)["cell_updates"] for s in range(len(p64))) / (time.perf_counter() - t1)
     l1s, same_t = [], 0
@@ -228,7 +262,7 @@ def bench_ensemble(args):
     b.close()
     # plain single-simulation FP64 path on a sample of members, same windows
     idx = list(range(0, len(probs), args.plain_stride))
-    st = ensemble.run_plain([p64[i] for i in idx], max_steps=args.steps, warmup=args.warmup)
+    st = ensemble.run_plain([p64[i] for i in idx], max_steps=args.steps, warmup=tune_launches + args.warmup)
     res["plain_fp64"] = {"value": st["cell_updates"] / st["seconds"], "members": idx, "steps": st["steps"],
                          "seconds": st["seconds"], "failed": st["failed"]}
     res["speedup_vs_plain_fp64"] = value / res["plain_fp64"]["value"]
@@ -273,10 +307,11 @@ def cpu_baseline_ensemble(probs, stride, warmup, budget_s):
 
 
 # ------------------------------------------------------------------ single simulations
-def run_device(prob, warmup, steps, profile=False):
+def run_device(prob, warmup, steps, profile=False, tune=0):
     from hydro_rs_amd import ics
 
     dev = ics.build_device_sim(prob)
+    tuning = dev.autotune(launches=tune, rounds=1) if tune > 0 else None
     dev.run(warmup)
     dev.synchronize()
     e0 = dev.engine_info()
@@ -291,7 +326,7 @@ def run_device(prob, warmup, steps, profile=False):
     return dev, {
         "steps": done, "seconds": t1 - t0, "cell_updates": e1["cell_updates"] - e0["cell_updates"],
         "face_fluxes": e1["face_fluxes"] - e0["face_fluxes"], "kernel_times": kt, "done": e1["done"],
-        "ti_current": e1["ti_current"],
+        "ti_current": e1["ti_current"], "autotune": tuning,
     }
 
 
@@ -326,11 +361,10 @@ def bench_kh3d(args):
            "warmup": args.kh_warmup, "steps": args.kh_steps}
     devs = {}
     for k, prec in (("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)):
-        d, s = run_device(ics.with_precision(prob, prec), args.kh_warmup, args.kh_steps, profile=True)
+        d, s = run_device(ics.with_precision(prob, prec), args.kh_warmup, args.kh_steps, profile=True,
+                          tune=args.kh_tune)
         rl = roofline(s["kernel_times"], s["steps"], s["cell_updates"], s["face_fluxes"], prob.n)
-        if rl["kernel"] == "flux":
-            rl["kernel_name"] = flux_kernel_name(prec == fvs.FP32_FLUX, fvs.AIRS, True)
-            rl["traffic"] = traffic_for(f"kh3d_{args.kh_side}", rl["kernel_name"])
+        attach_traffic(rl, f"kh3d_{args.kh_side}", prec == fvs.FP32_FLUX, fvs.AIRS, s["autotune"])
         n_act = s["cell_updates"] / max(s["steps"], 1)
         f_act = s["face_fluxes"] / max(s["steps"], 1)
         step_bytes = 192 * prob.n + 1247 * n_act + 256 * f_act
@@ -338,7 +372,7 @@ def bench_kh3d(args):
                   "face_fluxes_per_s": s["face_fluxes"] / s["seconds"],
                   "ms_per_step": 1e3 * s["seconds"] / max(s["steps"], 1),
                   "step_roofline_frac": step_bytes / (s["seconds"] / max(s["steps"], 1)) / (HBM_PEAK_GBS * 1e9),
-                  "roofline": rl, "ti_current": s["ti_current"]}
+                  "roofline": rl, "ti_current": s["ti_current"], "autotune": s["autotune"]}
         devs[k] = d
         log(f"[kh3d] {k}: {out[k]['ms_per_step']:.2f} ms/step, {out[k]['cell_updates_per_s']:.3e} cell-updates/s, "
             f"dominant {rl['kernel']} {rl['achieved']} GB/s")
@@ -386,6 +420,10 @@ def main():
     ap.add_argument("--workload", default="all", choices=["all", "ensemble", "sedov", "kh3d", "serve"])
     ap.add_argument("--members", type=int, default=512)
     ap.add_argument("--plain-stride", type=int, default=16)
+    ap.add_argument("--tune-launches", type=int, default=4,
+                    help="ensemble autotuner: launches per candidate trial (0: no tuning)")
+    ap.add_argument("--tune-rounds", type=int, default=1)
+    ap.add_argument("--kh-tune", type=int, default=2, help="3D workload autotuner: launches per trial (0: none)")
     ap.add_argument("--sedov-side", type=int, default=256)
     ap.add_argument("--kh-side", type=int, default=128)
     ap.add_argument("--kh-steps", type=int, default=20)
@@ -443,7 +481,7 @@ def main():
     if w in ("all", "ensemble"):
         ens, probs = bench_ensemble(args)
         out.update({
-            "value": ens["value"], "steps": ens["launches"], "warmup": args.warmup,
+            "value": ens["value"], "steps": ens["launches"], "warmup": args.warmup + ens["autotune"]["launches"],
             "ms_per_step": 1e3 * ens["seconds"] / max(ens["launches"], 1), "dtype": "f32 flux / f64 accumulate",
             "config": {"workload": f"ensemble{args.members}_riemann_1d2d_hllc_fp32flux", "members": args.members,
                        "cells": ens["n_all"], "cell_range": [200, 65536], "t_end": 0.05,
@@ -453,11 +491,12 @@ def main():
             "speedup_vs_plain_fp64": ens["speedup_vs_plain_fp64"], "plain_fp64": ens["plain_fp64"],
             "fp64_batched_value": ens["fp64_batched_value"], "l1_gate": ens["l1_gate"],
             "members_done": ens["members_done"], "members_failed": ens["members_failed"],
-            "roofline": ens["roofline"],
+            "roofline": ens["roofline"], "autotune": ens["autotune"],
         })
         if not args.no_cpu:
             log(f"[bench] cpu baseline (oracle), budget {args.cpu_seconds}s")
-            out["cpu_baseline"] = cpu_baseline_ensemble(probs, args.plain_stride, args.warmup, args.cpu_seconds)
+            out["cpu_baseline"] = cpu_baseline_ensemble(probs, args.plain_stride,
+                                                        ens["autotune"]["launches"] + args.warmup, args.cpu_seconds)
             log(f"[bench] cpu: {out['cpu_baseline']}")
         del probs
     if w in ("all", "sedov"):
