@@ -135,21 +135,33 @@ struct Batch {
   int2* fall;  // (face, simulation) queued for k_flux_fallback
 };
 
-// Flux record fr[i] of a static-geometry cell: the flux phase's per-cell inputs apart from the
-// extrapolations, two 128-byte lines.  Written where the fields change (k_grad for active cells,
-// k_limit for woken cells' dt, k_mask for check_physical resets, k_pack_fr after host changes).
+// Flux record fr[i] of a staged static-geometry cell: the flux phase's per-cell inputs apart from the
+// extrapolations, in the context's flux precision.  Written where the fields change (k_grad for
+// active cells, k_limit for woken cells' dt, k_mask for check_physical resets, k_pack_fr after host
+// changes).  FP64 flux: 32 doubles (two 128-byte lines).
 #define HFV_EX_W 5  // extrapolations: AoS extrap[N][5]
-#define HFV_FR_W 32
-#define HFV_FR_PRIM 0   // prim[5]
-#define HFV_FR_TEX 5    // tex[5]: time-extrapolation rate of prim
-#define HFV_FR_CS 10    // sound speed of prim (0 for vacuum)
-#define HFV_FR_GRAD 11  // grad[15]
-#define HFV_FR_DT 26
-#define HFV_FR_GC 27    // gc[3]
+#define HFV_FR_W 32     // doubles allocated per cell
+#define HFV_FR_DT 0
+#define HFV_FR_GC 1     // gc[3]
+#define HFV_FR_PRIM 4   // prim[5]
+#define HFV_FR_TEX 9    // tex[5]: time-extrapolation rate of prim
+#define HFV_FR_CS 14    // sound speed of prim (0 for vacuum)
+#define HFV_FR_GRAD 15  // grad[15]
 #define HFV_FR_USED 30
-// LDS staging record: the flux record's used fields, then the extrapolations[5]; odd stride, so that
-// neighbouring records start in different banks
+// FP32 flux: one 128-byte line of 32-bit words: dt in FP64 (words 0-1), the other fields rounded to
+// FP32 (the values the FP32 path computes with; the gathered path rounds the same fields).
+#define HFV_FR32_GC 2
+#define HFV_FR32_PRIM 5
+#define HFV_FR32_TEX 10
+#define HFV_FR32_CS 15
+#define HFV_FR32_GRAD 16
+#define HFV_FR32_USED 31
+// LDS staging records: the used fields, then the extrapolations[5] (doubles / words); odd strides, so
+// that neighbouring records start in different banks
+#define HFV_FR_EX HFV_FR_USED
 #define HFV_FR_LDS 35
+#define HFV_FR32_EX HFV_FR32_USED
+#define HFV_FR32_LDS 37
 #define HFV_STAGE_MAX 128
 
 #define HFV_NUM_TIME_BINS 56
@@ -290,23 +302,59 @@ __device__ __forceinline__ void fill_fr(double* v, const St<double>& p, const St
   v[HFV_FR_USED] = 0.;
   v[HFV_FR_USED + 1] = 0.;
 }
-// The flux record of cell i (two 128-byte lines).
-__device__ __forceinline__ void store_fr(double* fr, int64_t i, const St<double>& p, const St<double>& rate, double cs,
-                                         const Gr<double>& g, double dt, const V3<double>& gc) {
-  double v[HFV_FR_W];
+// The FP32 flux record as its 16 doubles' worth of bits.
+__device__ __forceinline__ void fill_fr32(double* v, const St<double>& p, const St<double>& rate, double cs,
+                                          const Gr<double>& g, double dt, const V3<double>& gc) {
+  float w[32];
+#pragma unroll
+  for (int q = 0; q < 5; ++q) {
+    w[HFV_FR32_PRIM + q] = (float)p.c[q];
+    w[HFV_FR32_TEX + q] = (float)rate.c[q];
+    w[HFV_FR32_GRAD + 3 * q] = (float)g.g[q].x;
+    w[HFV_FR32_GRAD + 3 * q + 1] = (float)g.g[q].y;
+    w[HFV_FR32_GRAD + 3 * q + 2] = (float)g.g[q].z;
+  }
+  w[HFV_FR32_CS] = (float)cs;
+  w[HFV_FR32_GC] = (float)gc.x;
+  w[HFV_FR32_GC + 1] = (float)gc.y;
+  w[HFV_FR32_GC + 2] = (float)gc.z;
+  w[HFV_FR32_USED] = 0.f;
+  v[0] = dt;
+#pragma unroll
+  for (int k = 1; k < 16; ++k) v[k] = __hiloint2double(__float_as_int(w[2 * k + 1]), __float_as_int(w[2 * k]));
+#pragma unroll
+  for (int k = 16; k < HFV_FR_W; ++k) v[k] = 0.;
+}
+// The record in the context's precision: doubles of the record (16 of them carry the FP32 record).
+__device__ __forceinline__ int fill_fr_any(double* v, bool f32, const St<double>& p, const St<double>& rate, double cs,
+                                           const Gr<double>& g, double dt, const V3<double>& gc) {
+  if (f32) {
+    fill_fr32(v, p, rate, cs, g, dt, gc);
+    return 16;
+  }
   fill_fr(v, p, rate, cs, g, dt, gc);
+  return HFV_FR_W;
+}
+// The flux record of cell i (one or two 128-byte lines).
+__device__ __forceinline__ void store_fr(double* fr, int64_t i, bool f32, const St<double>& p, const St<double>& rate,
+                                         double cs, const Gr<double>& g, double dt, const V3<double>& gc) {
+  double v[HFV_FR_W];
+  const int nd = fill_fr_any(v, f32, p, rate, cs, g, dt, gc);
   double2* d = (double2*)(fr + HFV_FR_W * (uint64_t)i);
 #pragma unroll
-  for (int q = 0; q < HFV_FR_W / 2; ++q) d[q] = make_double2(v[2 * q], v[2 * q + 1]);
+  for (int q = 0; q < HFV_FR_W / 2; ++q)
+    if (2 * q < nd) d[q] = make_double2(v[2 * q], v[2 * q + 1]);
 }
 // A wave's new flux records (lane l: cell i0 + l, when bit l of `has` is set), transposed through the
 // wave's own LDS buffer buf[64 * 17] in two halves, so that every store instruction writes whole
 // 128-byte lines.  Wave-level only (no block barrier): LDS operations of a wave complete in order.
-__device__ __forceinline__ void store_fr_wave(double* fr, int64_t i0, uint64_t has, const double* v, double* buf) {
+__device__ __forceinline__ void store_fr_wave(double* fr, int64_t i0, uint64_t has, const double* v, double* buf,
+                                              int halves) {
   const int lane = (int)__lane_id();
   const bool mine = (has >> lane) & 1ull;
 #pragma unroll
   for (int h = 0; h < 2; ++h) {
+    if (h >= halves) break;
     if (mine) {
 #pragma unroll
       for (int j = 0; j < 16; ++j) buf[lane * 17 + j] = v[16 * h + j];
@@ -703,9 +751,16 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
     if (reset && B.sims[s].staged) {
       // the flux record holds this rate already; a reset changes prim and the sound speed
       double* r = C.fr + HFV_FR_W * (uint64_t)i;
+      if (cfg.precision == HFV_FP32_FLUX) {
+        float* w = (float*)r;
 #pragma unroll
-      for (int q = 0; q < 5; ++q) r[HFV_FR_PRIM + q] = prim.c[q];
-      r[HFV_FR_CS] = fr_cs(prim, cfg.gd);
+        for (int q = 0; q < 5; ++q) w[HFV_FR32_PRIM + q] = (float)prim.c[q];
+        w[HFV_FR32_CS] = (float)fr_cs(prim, cfg.gd);
+      } else {
+#pragma unroll
+        for (int q = 0; q < 5; ++q) r[HFV_FR_PRIM + q] = prim.c[q];
+        r[HFV_FR_CS] = fr_cs(prim, cfg.gd);
+      }
     }
 
   }
@@ -786,32 +841,46 @@ __device__ __forceinline__ const Gas<float>& gas_for<float>(const Cfg& c) {
 
 // Per-cell inputs of the flux phase gathered from the cell fields.  The time-extrapolation rate and
 // the sound speed are k_mask's expressions of the same primitives, gradients and v_rel (a check_physical
-// reset in k_mask leaves rho = 0, where neither is used).
+// reset in k_mask leaves rho = 0, where neither is used).  With T = float the fields that the FP32 flux
+// record holds in FP32 (primitives, sound speed, gc) are rounded the same way, so that gathered and
+// staged faces compute the same bits.
+template <class T>
+__device__ __forceinline__ double rnd(double x) {
+  return (double)(T)x;
+}
+template <class T>
 struct SoaCell {
   const Cells* C;
   const Gas<double>* g;
   uint32_t i;
   __device__ __forceinline__ SoaCell(const Cells& c, const Gas<double>& gd, int32_t idx) : C(&c), g(&gd), i((uint32_t)idx) {}
   __device__ __forceinline__ double dt() const { return ldo(C->dt, i); }
-  __device__ __forceinline__ St<double> prim() const { return ld5(C->prim, C->n, i); }
-  __device__ __forceinline__ double rho() const { return ldo(C->prim, i); }
-  __device__ __forceinline__ V3<double> pvel() const { return mk3(ldo(C->prim + C->n, i), ldo(C->prim + 2 * C->n, i), ldo(C->prim + 3 * C->n, i)); }
-  __device__ __forceinline__ double cs() const { return fr_cs(prim(), *g); }
+  __device__ __forceinline__ St<double> prim() const {
+    const St<double> p = ld5(C->prim, C->n, i);
+    return mkst(rnd<T>(p.c[0]), mk3(rnd<T>(p.c[1]), rnd<T>(p.c[2]), rnd<T>(p.c[3])), rnd<T>(p.c[4]));
+  }
+  __device__ __forceinline__ double rho() const { return rnd<T>(ldo(C->prim, i)); }
+  __device__ __forceinline__ V3<double> pvel() const {
+    return mk3(rnd<T>(ldo(C->prim + C->n, i)), rnd<T>(ldo(C->prim + 2 * C->n, i)), rnd<T>(ldo(C->prim + 3 * C->n, i)));
+  }
+  __device__ __forceinline__ double cs() const { return rnd<T>(fr_cs(ld5(C->prim, C->n, i), *g)); }
   __device__ __forceinline__ V3<double> grad(int q) const {
     return mk3(ldo(C->grad + (3 * q) * C->n, i), ldo(C->grad + (3 * q + 1) * C->n, i), ldo(C->grad + (3 * q + 2) * C->n, i));
   }
   __device__ __forceinline__ double ex(int q) const { return C->extrap[HFV_EX_W * (uint64_t)i + q]; }
   __device__ __forceinline__ St<double> rate() const {
-    return tex_rate(prim(), ldg(C->grad, C->n, i), ld3(C->vrel, C->n, i), *g);
+    return tex_rate(ld5(C->prim, C->n, i), ldg(C->grad, C->n, i), ld3(C->vrel, C->n, i), *g);
   }
-  __device__ __forceinline__ V3<double> gc() const { return ld3(C->gc, C->n, i); }
+  __device__ __forceinline__ V3<double> gc() const {
+    const V3<double> v = ld3(C->gc, C->n, i);
+    return mk3(rnd<T>(v.x), rnd<T>(v.y), rnd<T>(v.z));
+  }
   __device__ __forceinline__ V3<double> cen() const { return ld3(C->cen, C->n, i); }
   __device__ __forceinline__ V3<double> loc() const { return ld3(C->loc, C->n, i); }
   __device__ __forceinline__ V3<double> v() const { return ld3(C->v, C->n, i); }
 };
 // Per-cell inputs of the flux phase from a cell's flux record and extrapolations staged in LDS
-// (static geometry): record fields at HFV_FR_*, extrapolations at HFV_FR_EX.
-#define HFV_FR_EX HFV_FR_USED
+// (static geometry): FP64 record fields at HFV_FR_*, extrapolations at HFV_FR_EX.
 struct RecCell {
   const double* r;
   const Cells* C;
@@ -832,6 +901,37 @@ struct RecCell {
     return mkst(r[HFV_FR_TEX], mk3(r[HFV_FR_TEX + 1], r[HFV_FR_TEX + 2], r[HFV_FR_TEX + 3]), r[HFV_FR_TEX + 4]);
   }
   __device__ __forceinline__ V3<double> gc() const { return mk3(r[HFV_FR_GC], r[HFV_FR_GC + 1], r[HFV_FR_GC + 2]); }
+  __device__ __forceinline__ V3<double> cen() const { return ld3(C->cen, C->n, i); }
+  __device__ __forceinline__ V3<double> loc() const { return ld3(C->loc, C->n, i); }
+  __device__ __forceinline__ V3<double> v() const { return ld3(C->v, C->n, i); }
+};
+// The same from an FP32 record staged in LDS (words at HFV_FR32_*, extrapolations at HFV_FR32_EX).
+struct RecCellF {
+  const float* r;
+  const Cells* C;
+  uint32_t i;
+  __device__ __forceinline__ RecCellF(const float* rec, const Cells& c, int32_t idx) : r(rec), C(&c), i((uint32_t)idx) {}
+  __device__ __forceinline__ double dt() const { return __hiloint2double(__float_as_int(r[1]), __float_as_int(r[0])); }
+  __device__ __forceinline__ St<double> prim() const {
+    return mkst((double)r[HFV_FR32_PRIM], mk3((double)r[HFV_FR32_PRIM + 1], (double)r[HFV_FR32_PRIM + 2],
+                (double)r[HFV_FR32_PRIM + 3]), (double)r[HFV_FR32_PRIM + 4]);
+  }
+  __device__ __forceinline__ double rho() const { return r[HFV_FR32_PRIM]; }
+  __device__ __forceinline__ V3<double> pvel() const {
+    return mk3((double)r[HFV_FR32_PRIM + 1], (double)r[HFV_FR32_PRIM + 2], (double)r[HFV_FR32_PRIM + 3]);
+  }
+  __device__ __forceinline__ double cs() const { return r[HFV_FR32_CS]; }
+  __device__ __forceinline__ V3<double> grad(int q) const {
+    return mk3((double)r[HFV_FR32_GRAD + 3 * q], (double)r[HFV_FR32_GRAD + 3 * q + 1], (double)r[HFV_FR32_GRAD + 3 * q + 2]);
+  }
+  __device__ __forceinline__ double ex(int q) const { return r[HFV_FR32_EX + q]; }
+  __device__ __forceinline__ St<double> rate() const {
+    return mkst((double)r[HFV_FR32_TEX], mk3((double)r[HFV_FR32_TEX + 1], (double)r[HFV_FR32_TEX + 2],
+                (double)r[HFV_FR32_TEX + 3]), (double)r[HFV_FR32_TEX + 4]);
+  }
+  __device__ __forceinline__ V3<double> gc() const {
+    return mk3((double)r[HFV_FR32_GC], (double)r[HFV_FR32_GC + 1], (double)r[HFV_FR32_GC + 2]);
+  }
   __device__ __forceinline__ V3<double> cen() const { return ld3(C->cen, C->n, i); }
   __device__ __forceinline__ V3<double> loc() const { return ld3(C->loc, C->n, i); }
   __device__ __forceinline__ V3<double> v() const { return ld3(C->v, C->n, i); }
@@ -1010,7 +1110,8 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   const int64_t f = (int64_t)J.lo + threadIdx.x;
   constexpr bool DEFER = SOLVER == SOLVER_AIRS;
   // LDS: the staged records, reused for the block's output records (written out as whole lines)
-  constexpr int STAGE_D = STATIC ? HFV_STAGE_MAX * HFV_FR_LDS : 0;
+  constexpr bool F32 = sizeof(T) == 4;
+  constexpr int STAGE_D = !STATIC ? 0 : F32 ? (HFV_STAGE_MAX * HFV_FR32_LDS + 1) / 2 : HFV_STAGE_MAX * HFV_FR_LDS;
   constexpr int REC_D = HFV_BLOCK * 8;
   __shared__ double lds[STAGE_D > REC_D ? STAGE_D : REC_D];
   __shared__ uint8_t curs[HFV_BLOCK];
@@ -1039,49 +1140,78 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
     if ((int)threadIdx.x < ns) scl[threadIdx.x] = myc;
     const int ncur = __syncthreads_count(cur);  // also publishes scl
     if (ns > 0 && ncur > 0) {  // block-uniform
-      constexpr int NR = HFV_STAGE_MAX * (HFV_FR_W / 2) / HFV_BLOCK;
+      constexpr int RD2 = F32 ? 8 : HFV_FR_W / 2;  // double2 per record
+      constexpr int NR = HFV_STAGE_MAX * RD2 / HFV_BLOCK;
       constexpr int NE = (HFV_STAGE_MAX * HFV_EX_W + HFV_BLOCK - 1) / HFV_BLOCK;
-      static_assert(HFV_STAGE_MAX * (HFV_FR_W / 2) % HFV_BLOCK == 0, "record slots per thread");
+      static_assert(HFV_STAGE_MAX * RD2 % HFV_BLOCK == 0, "record slots per thread");
       double2 rv[NR];
       double xv[NE];
 #pragma unroll
       for (int j = 0; j < NR; ++j) {
-        const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k >> 4, q = k & 15;
-        if (c < ns && q < HFV_FR_USED / 2) rv[j] = ((const double2*)(B.C.fr + HFV_FR_W * (uint64_t)(uint32_t)scl[c]))[q];
+        const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / RD2, q = k % RD2;
+        if (c < ns && (F32 || q < HFV_FR_USED / 2))
+          rv[j] = ((const double2*)(B.C.fr + HFV_FR_W * (uint64_t)(uint32_t)scl[c]))[q];
       }
 #pragma unroll
       for (int j = 0; j < NE; ++j) {
         const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / HFV_EX_W, q = k - HFV_EX_W * c;
         if (c < ns) xv[j] = B.C.extrap[HFV_EX_W * (uint64_t)(uint32_t)scl[c] + q];
       }
+      if (F32) {
+        float* ldsf = (float*)lds;
 #pragma unroll
-      for (int j = 0; j < NR; ++j) {
-        const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k >> 4, q = k & 15;
-        if (c < ns && q < HFV_FR_USED / 2) {
-          lds[c * HFV_FR_LDS + 2 * q] = rv[j].x;
-          lds[c * HFV_FR_LDS + 2 * q + 1] = rv[j].y;
+        for (int j = 0; j < NR; ++j) {
+          const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / RD2, q = k % RD2;
+          if (c < ns) {
+            const int2 a = *(const int2*)&rv[j].x, b = *(const int2*)&rv[j].y;
+            float* d = ldsf + c * HFV_FR32_LDS + 4 * q;
+            d[0] = __int_as_float(a.x);
+            d[1] = __int_as_float(a.y);
+            d[2] = __int_as_float(b.x);
+            if (4 * q + 3 < HFV_FR32_USED) d[3] = __int_as_float(b.y);
+          }
         }
-      }
 #pragma unroll
-      for (int j = 0; j < NE; ++j) {
-        const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / HFV_EX_W, q = k - HFV_EX_W * c;
-        if (c < ns) lds[c * HFV_FR_LDS + HFV_FR_EX + q] = xv[j];
+        for (int j = 0; j < NE; ++j) {
+          const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / HFV_EX_W, q = k - HFV_EX_W * c;
+          if (c < ns) ldsf[c * HFV_FR32_LDS + HFV_FR32_EX + q] = (float)xv[j];
+        }
+      } else {
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+          const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / RD2, q = k % RD2;
+          if (c < ns && q < HFV_FR_USED / 2) {
+            lds[c * HFV_FR_LDS + 2 * q] = rv[j].x;
+            lds[c * HFV_FR_LDS + 2 * q + 1] = rv[j].y;
+          }
+        }
+#pragma unroll
+        for (int j = 0; j < NE; ++j) {
+          const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / HFV_EX_W, q = k - HFV_EX_W * c;
+          if (c < ns) lds[c * HFV_FR_LDS + HFV_FR_EX + q] = xv[j];
+        }
       }
       __syncthreads();
       if (cur) {
         const uchar2 fl = *(const uchar2*)(B.F.floc + 2 * (uint64_t)f);
-        const RecCell L(lds + fl.x * HFV_FR_LDS, B.C, l);
-        const RecCell R(lds + (r >= 0 ? fl.y : fl.x) * HFV_FR_LDS, B.C, r);
-        computed = face_flux<T, SOLVER, DEFER, true, RecCell>(B, cfg, f, L, R, r, la, ra, outv, ev);
+        const int rs = r >= 0 ? fl.y : fl.x;
+        if constexpr (F32) {
+          const float* ldsf = (const float*)lds;
+          const RecCellF L(ldsf + fl.x * HFV_FR32_LDS, B.C, l), R(ldsf + rs * HFV_FR32_LDS, B.C, r);
+          computed = face_flux<T, SOLVER, DEFER, true, RecCellF>(B, cfg, f, L, R, r, la, ra, outv, ev);
+        } else {
+          const RecCell L(lds + fl.x * HFV_FR_LDS, B.C, l), R(lds + rs * HFV_FR_LDS, B.C, r);
+          computed = face_flux<T, SOLVER, DEFER, true, RecCell>(B, cfg, f, L, R, r, la, ra, outv, ev);
+        }
       }
       __syncthreads();  // the staged records are dead from here on
     } else if (cur) {
-      computed = face_flux<T, SOLVER, DEFER, true, SoaCell>(B, cfg, f, SoaCell(B.C, cfg.gd, l),
-                                                            SoaCell(B.C, cfg.gd, r), r, la, ra, outv, ev);
+      computed = face_flux<T, SOLVER, DEFER, true, SoaCell<T>>(B, cfg, f, SoaCell<T>(B.C, cfg.gd, l),
+                                                               SoaCell<T>(B.C, cfg.gd, r), r, la, ra, outv, ev);
     }
   } else if (cur) {
-    computed = face_flux<T, SOLVER, DEFER, false, SoaCell>(B, cfg, f, SoaCell(B.C, cfg.gd, l), SoaCell(B.C, cfg.gd, r),
-                                                           r, la, ra, outv, ev);
+    computed = face_flux<T, SOLVER, DEFER, false, SoaCell<T>>(B, cfg, f, SoaCell<T>(B.C, cfg.gd, l),
+                                                              SoaCell<T>(B.C, cfg.gd, r), r, la, ra, outv, ev);
   }
   // records of faces with an active cell (deferred ones zero here, overwritten by k_flux_fallback)
   curs[threadIdx.x] = cur ? 1 : 0;
@@ -1127,8 +1257,8 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_flux_fallback(Batch B) {
     Events ev;
     double outv[7];
     const Cfg& cfg = B.sims[s].cfg;
-    face_flux<T, SOLVER_AIRS, false, false, SoaCell>(B, cfg, f, SoaCell(B.C, cfg.gd, l), SoaCell(B.C, cfg.gd, r), r,
-                                                     la, ra, outv, ev);
+    face_flux<T, SOLVER_AIRS, false, false, SoaCell<T>>(B, cfg, f, SoaCell<T>(B.C, cfg.gd, l),
+                                                        SoaCell<T>(B.C, cfg.gd, r), r, la, ra, outv, ev);
     store_rec(B.F.rec, f, outv);
     flush_events(ev, B.cnt, B.eng, s);
   }
@@ -1274,7 +1404,8 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_pack_fr(Batch B) {
   const Gas<double>& gd = B.sims[J.sim].cfg.gd;
   const St<double> p = ld5(C.prim, n, i);
   const Gr<double> g = ldg(C.grad, n, i);
-  store_fr(C.fr, i, p, fr_rate(p, g, ld3(C.vrel, n, i), gd), fr_cs(p, gd), g, C.dt[i], ld3(C.gc, n, i));
+  store_fr(C.fr, i, B.sims[J.sim].cfg.precision == HFV_FP32_FLUX, p, fr_rate(p, g, ld3(C.vrel, n, i), gd), fr_cs(p, gd),
+           g, C.dt[i], ld3(C.gc, n, i));
 }
 
 // glam::DMat3 as three columns
@@ -1531,7 +1662,9 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
   C.bin[i] = get_time_bin(dti);
   *dti_out = dti;
   // the cell's flux record for the next step (its prim, gradients, v_rel and dt are final here)
-  if (pack_fr) fill_fr(frv, prim, fr_rate(prim, gr, vrel_now, cfg.gd), fr_cs(prim, cfg.gd), gr, dt_new, gcen);
+  if (pack_fr)
+    fill_fr_any(frv, cfg.precision == HFV_FP32_FLUX, prim, fr_rate(prim, gr, vrel_now, cfg.gd), fr_cs(prim, cfg.gd),
+                gr, dt_new, gcen);
 }
 
 template <bool STATIC>
@@ -1554,7 +1687,9 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_GRAD_WAVES) k_grad(Batch B, int
   if (pack) {
     __shared__ double wbuf[HFV_BLOCK / 64][64 * 17];
     const uint64_t has = __ballot(a);
-    if (has) store_fr_wave(B.C.fr, i - (int64_t)__lane_id(), has, frv, wbuf[threadIdx.x >> 6]);
+    if (has)
+      store_fr_wave(B.C.fr, i - (int64_t)__lane_id(), has, frv, wbuf[threadIdx.x >> 6],
+                    cfg.precision == HFV_FP32_FLUX ? 1 : 2);
   }
   if (do_ts) {
     uint64_t t = block_min_u64(my);
