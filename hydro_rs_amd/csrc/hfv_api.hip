@@ -1128,7 +1128,14 @@ int hfv_get_fluxes(hfv_ctx* c, double* out7f) {
   std::vector<uint8_t> act(n);
   std::vector<int32_t> left(nf), right(nf);
   (void)NF;
-  HIP_OK(hipMemcpyAsync(aos.data(), c->F.rec + 8 * fb, sizeof(double) * 8 * nf, hipMemcpyDeviceToHost, c->stream));
+  std::vector<float> aosf;
+  if (c->precision == HFV_FP32_FLUX) {  // FP32-flux records: 8 floats per face
+    aosf.resize(8 * nf);
+    HIP_OK(hipMemcpyAsync(aosf.data(), (const float*)c->F.rec + 8 * fb, sizeof(float) * 8 * nf, hipMemcpyDeviceToHost,
+                          c->stream));
+  } else {
+    HIP_OK(hipMemcpyAsync(aos.data(), c->F.rec + 8 * fb, sizeof(double) * 8 * nf, hipMemcpyDeviceToHost, c->stream));
+  }
   HIP_OK(hipMemcpyAsync(act.data(), c->C.act + cb, n, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipMemcpyAsync(left.data(), c->F.left + fb, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipMemcpyAsync(right.data(), c->F.right + fb, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, c->stream));
@@ -1136,7 +1143,8 @@ int hfv_get_fluxes(hfv_ctx* c, double* out7f) {
   for (int64_t f = 0; f < nf; ++f) {
     // records are current for faces with an active cell; the others are gated off (zero)
     bool valid = act[left[f] - cb] || (right[f] >= 0 && act[right[f] - cb]);
-    for (int k = 0; k < 7; ++k) out7f[7 * f + k] = valid ? aos[8 * f + k] : 0.;
+    for (int k = 0; k < 7; ++k)
+      out7f[7 * f + k] = !valid ? 0. : aosf.empty() ? aos[8 * f + k] : (double)aosf[8 * f + k];
   }
   return HFV_OK;
 }

@@ -1090,7 +1090,16 @@ __device__ __forceinline__ bool face_flux(const Batch& B, const Cfg& cfg, int64_
   return true;
 }
 
-__device__ __forceinline__ void store_rec(double* rec, int64_t f, const double* outv) {
+// Face record: FP64 contexts 8 doubles (64 bytes) per face; FP32-flux contexts 8 floats (32 bytes):
+// dt*F[5], v_max, a_over_r, unused.  The FP32 record rounds dt*A*F once; both cells accumulate that
+// same value in FP64 (with opposite signs), so conservation stays exact.
+__device__ __forceinline__ void store_rec(double* rec, int64_t f, const double* outv, bool f32) {
+  if (f32) {
+    float4* dst = (float4*)((float*)rec + 8 * (uint64_t)f);
+    dst[0] = make_float4((float)outv[0], (float)outv[1], (float)outv[2], (float)outv[3]);
+    dst[1] = make_float4((float)outv[4], (float)outv[5], (float)outv[6], 0.f);
+    return;
+  }
   double2* dst = (double2*)(rec + 8 * (uint64_t)f);
 #pragma unroll
   for (int q = 0; q < 3; ++q) dst[q] = make_double2(outv[2 * q], outv[2 * q + 1]);
@@ -1218,12 +1227,24 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   }
   // records of faces with an active cell (deferred ones zero here, overwritten by k_flux_fallback)
   curs[threadIdx.x] = cur ? 1 : 0;
-  recs[4 * threadIdx.x + 0] = make_double2(outv[0], outv[1]);
-  recs[4 * threadIdx.x + 1] = make_double2(outv[2], outv[3]);
-  recs[4 * threadIdx.x + 2] = make_double2(outv[4], outv[5]);
-  recs[4 * threadIdx.x + 3] = make_double2(outv[6], 0.);
-  __syncthreads();
-  {
+  if constexpr (F32) {
+    float4* rf = (float4*)lds;
+    rf[2 * threadIdx.x] = make_float4((float)outv[0], (float)outv[1], (float)outv[2], (float)outv[3]);
+    rf[2 * threadIdx.x + 1] = make_float4((float)outv[4], (float)outv[5], (float)outv[6], 0.f);
+    __syncthreads();
+    float4* dst = (float4*)((float*)B.F.rec + 8 * (uint64_t)J.lo);
+    const int nvalid = 2 * (J.hi - J.lo);
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int c = q * HFV_BLOCK + (int)threadIdx.x;
+      if (c < nvalid && curs[c >> 1]) dst[c] = rf[c];
+    }
+  } else {
+    recs[4 * threadIdx.x + 0] = make_double2(outv[0], outv[1]);
+    recs[4 * threadIdx.x + 1] = make_double2(outv[2], outv[3]);
+    recs[4 * threadIdx.x + 2] = make_double2(outv[4], outv[5]);
+    recs[4 * threadIdx.x + 3] = make_double2(outv[6], 0.);
+    __syncthreads();
     double2* dst = (double2*)(B.F.rec + 8 * (uint64_t)J.lo);
     const int nvalid = 4 * (J.hi - J.lo);
 #pragma unroll
@@ -1262,7 +1283,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_flux_fallback(Batch B) {
     const Cfg& cfg = B.sims[s].cfg;
     face_flux<T, SOLVER_AIRS, false, false, SoaCell<T>>(B, cfg, f, SoaCell<T>(B.C, cfg.gd, l),
                                                         SoaCell<T>(B.C, cfg.gd, r), r, la, ra, outv, ev);
-    store_rec(B.F.rec, f, outv);
+    store_rec(B.F.rec, f, outv, sizeof(T) == 4);
     flush_events(ev, B.cnt, B.eng, s);
   }
 }
@@ -1312,6 +1333,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_apply(Batch B, int mode) {
   const Faces& F = B.F;
   const int64_t n = C.n;
   const Cfg& cfg = B.sims[s].cfg;
+  const bool f32 = cfg.precision == HFV_FP32_FLUX;
   const bool a = C.act[i];
   Events ev;
   bool have_prim = false;
@@ -1335,16 +1357,28 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_apply(Batch B, int mode) {
         acc = ld5(C.facc, n, i);
         have = true;
       }
-      const double2* r2 = (const double2*)(F.rec + 8 * (uint64_t)f);
-      const double2 r01 = r2[0], r23 = r2[1], r45 = r2[2], r67 = r2[3];
-      const St<double> rec = mkst(r01.x, mk3(r01.y, r23.x, r23.y), r45.x);
+      St<double> rec;
+      double vmx, aor;
+      if (f32) {
+        const float4* r4 = (const float4*)((const float*)F.rec + 8 * (uint64_t)f);
+        const float4 p = r4[0], q = r4[1];
+        rec = mkst((double)p.x, mk3((double)p.y, (double)p.z, (double)p.w), (double)q.x);
+        vmx = q.y;
+        aor = q.z;
+      } else {
+        const double2* r2 = (const double2*)(F.rec + 8 * (uint64_t)f);
+        const double2 r01 = r2[0], r23 = r2[1], r45 = r2[2], r67 = r2[3];
+        rec = mkst(r01.x, mk3(r01.y, r23.x, r23.y), r45.x);
+        vmx = r45.y;
+        aor = r67.x;
+      }
       if ((c & 1) == 0)
         acc = acc - rec;
       else
         acc = acc + rec;
       if (a) {
-        msv = hmax(r45.y, msv);
-        maor = hmax(maor, r67.x);
+        msv = hmax(vmx, msv);
+        maor = hmax(maor, aor);
       }
     }
     if (have) {
