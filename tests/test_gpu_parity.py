@@ -250,3 +250,27 @@ def test_trait_phases_equal_fused_step():
             assert np.array_equal(sa[k], sb[k]), k
         ti_old, ti_cur = ti_cur, t1
     assert rel_err(sa["prim"], sb["prim"]) == 0.0
+
+
+def test_snapshot_matches_oracle(ora, tmp_path):
+    """Space::dump of the device state equals the dump of the oracle's state after the same steps (the
+    whole-run tolerance: the step sequence compounds last-bit differences of pow)."""
+    from hydro_rs_amd import snapshot
+
+    prob = ics.sedov_2d(side=24, t_end=0.01)
+    say("[snapshot] sedov2d side=24, 40 steps")
+    osim = oracle_sim(prob)
+    n_ref = osim.advance(40)
+    dev = ics.build_device_sim(prob)
+    n_dev = dev.run(40)
+    assert n_dev == n_ref
+    a = snapshot.write_snapshot(tmp_path / "dev.npz", dev, prob.cfg, faces=prob.mesh)
+    oi = osim.info()
+    b = snapshot.snapshot_arrays(osim.get_fields(), prob.n, prob.cfg, oi["ti_current"] * oi["time_base"],
+                                 faces=prob.mesh)
+    assert set(a) == set(b)
+    with np.load(tmp_path / "dev.npz", allow_pickle=False) as z:
+        assert set(z.files) == set(a)
+    errs = {k: rel_err(a[k], b[k]) for k in a}
+    say(f"  worst: {max(errs.items(), key=lambda kv: kv[1])}")
+    assert max(errs.values()) <= 1e-9, errs
