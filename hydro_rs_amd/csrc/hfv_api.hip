@@ -166,7 +166,7 @@ struct hfv_ctx {
   hipGraphExec_t gexec{};
   bool graph_ok = false;
   bool use_graph = true;
-  bool fr_dirty = true;  // flux records must be rebuilt (k_pack_fr) before the next flux kernel
+  bool fr_dirty = true;  // rates and flux records must be rebuilt (k_pack_fr) before the next predict/flux
   hfv_timestep_info last_info{};
   // kernel timing (hfv_profile_enable)
   bool prof = false;
@@ -299,6 +299,7 @@ int alloc_batch(hfv_ctx* c) {
   C.dt = dalloc<double>(c, n);
   C.srad = dalloc<double>(c, n);
   C.pc = dalloc<double>(c, 8 * n);
+  C.tex = dalloc<double>(c, 5 * n);
   C.bin = dalloc<int8_t>(c, n);
   C.wake = dalloc<int8_t>(c, n);
   C.act = dalloc<uint8_t>(c, n);
@@ -352,9 +353,9 @@ int alloc_batch(hfv_ctx* c) {
     if (!p) return set_err(HFV_ERR_HIP, "hipMalloc failed");
   hipStream_t st = c->stream;
   const size_t sd = sizeof(double) * (size_t)n;
-  double* dbl[] = {C.prim, C.grad, C.gc, C.extrap, C.cons, C.facc, C.vol, C.loc, C.cen, C.v, C.vrel, C.maor, C.msv, C.dt, C.srad};
-  int mult[] = {5, 15, 3, HFV_EX_W, 5, 5, 1, 3, 3, 3, 3, 1, 1, 1, 1};
-  for (int k = 0; k < 15; ++k) HIP_OK(hipMemsetAsync(dbl[k], 0, sd * mult[k], st));
+  double* dbl[] = {C.prim, C.grad, C.gc, C.extrap, C.cons, C.facc, C.vol, C.loc, C.cen, C.v, C.vrel, C.maor, C.msv, C.dt, C.srad, C.tex};
+  int mult[] = {5, 15, 3, HFV_EX_W, 5, 5, 1, 3, 3, 3, 3, 1, 1, 1, 1, 5};
+  for (int k = 0; k < 16; ++k) HIP_OK(hipMemsetAsync(dbl[k], 0, sd * mult[k], st));
   HIP_OK(hipMemsetAsync(C.pc, 0, sd * 8, st));
   HIP_OK(hipMemsetAsync(C.fr, 0, sd * HFV_FR_W, st));
   HIP_OK(hipMemsetAsync(F.scnt, 0, (size_t)std::max<int64_t>(nfc, 1), st));
@@ -622,7 +623,7 @@ int reset_slot(hfv_ctx* c, int s) {
     double* p;
     int w;
   } fl[] = {{C.prim, 5}, {C.grad, 15}, {C.gc, 3}, {C.cons, 5}, {C.facc, 5}, {C.loc, 3}, {C.v, 3},
-            {C.vrel, 3}, {C.maor, 1}, {C.msv, 1}, {C.dt, 1}};
+            {C.vrel, 3}, {C.maor, 1}, {C.msv, 1}, {C.dt, 1}, {C.tex, 5}};
   for (auto& f : fl)
     for (int k = 0; k < f.w; ++k) HIP_OK(hipMemsetAsync(f.p + k * N + cb, 0, sizeof(double) * cap, st));
   HIP_OK(hipMemsetAsync(C.extrap + HFV_EX_W * cb, 0, sizeof(double) * HFV_EX_W * cap, st));
@@ -814,10 +815,10 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, st, B);
 }
 
-// Rebuild the flux records of the static-geometry simulations when the host or a phase call changed
-// their inputs (the fused step keeps them current itself).
+// Rebuild the kept time-extrapolation rates (and the staged simulations' flux records) when the host or a
+// phase call changed their inputs (the fused step keeps them current itself).
 void pack_fr_if_dirty(hfv_ctx* c) {
-  if (c->fr_dirty && c->any_static)
+  if (c->fr_dirty)
     hipLaunchKernelGGL(k_pack_fr, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args());
   c->fr_dirty = false;
 }
@@ -1099,6 +1100,7 @@ int hfv_predict(hfv_ctx* c, double dt) {
   HIP_OK(hipStreamSynchronize(c->stream));
   E.dt_pred = dt;
   HIP_OK(hipMemcpyAsync(c->deng, &E, sizeof(SimEngine), hipMemcpyHostToDevice, c->stream));
+  pack_fr_if_dirty(c);  // predict reads the kept time-extrapolation rates
   hipLaunchKernelGGL(k_mask, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
   HIP_OK(hipGetLastError());
   c->fr_dirty = true;

@@ -7,8 +7,9 @@
 // HBM layout (cells/faces of all simulations concatenated, indices global):
 //   cells, component-major SoA (field[c * N + i]): prim[5], grad[15], gc[3], cons[5], facc[5] (pending
 //     flux accumulator), vol, loc[3], cen[3], v[3], vrel[3], maor, msv, dt, srad, bin (int8), wake
-//     (int8), act (uint8); extrap[N][5] (AoS: read and written whole by predict, staged whole by the
-//     flux kernel)
+//     (int8), act (uint8), tex[5] (time-extrapolation rate of prim, grad and v_rel, written by k_grad
+//     when they change); extrap[N][5] (AoS: read and written whole by predict, staged whole by the flux
+//     kernel)
 //     pc[N][8] (AoS: prim[5], cen[3] of every cell after this step's c2p, one 64-byte line per cell
 //     for the gradient sweeps' neighbour gathers)
 //     fr[N][32] (AoS flux record of static-geometry cells, HFV_FR_*: prim, time-extrapolation rate,
@@ -39,6 +40,7 @@ struct Cells {
   int64_t n;
   double *prim, *grad, *gc, *extrap, *cons, *facc, *vol, *loc, *cen, *v, *vrel, *maor, *msv, *dt, *srad;
   double* pc;
+  double* tex;   // [5][N] time-extrapolation rate of the cell's prim/grad/v_rel (written when they change)
   double* minv;  // [9][N] inverse of the gradient matrix (static geometry only)
   double* fr;    // [N][HFV_FR_W] flux inputs of static-geometry cells (layout: HFV_FR_*)
   int8_t *bin, *wake;
@@ -731,9 +733,8 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
       C.act[i] = act;
     }
     St<double> prim = ld5(C.prim, n, i);
-    Gr<double> gr = ldg(C.grad, n, i);
-    V3<double> vrel = ld3(C.vrel, n, i);
-    const St<double> rate = fr_rate(prim, gr, vrel, cfg.gd);
+    // the rate of this prim/grad/v_rel, kept by k_grad (k_pack_fr after host changes)
+    const St<double> rate = prim.c[0] > 0. ? ld5(C.tex, n, i) : vacuum<double>();
     double* exp_ = C.extrap + HFV_EX_W * (uint64_t)i;
     St<double> ex = mkst(exp_[0], mk3(exp_[1], exp_[2], exp_[3]), exp_[4]);
     bool reset = false;
@@ -842,9 +843,9 @@ __device__ __forceinline__ const Gas<float>& gas_for<float>(const Cfg& c) {
   return c.gf;
 }
 
-// Per-cell inputs of the flux phase gathered from the cell fields.  The time-extrapolation rate and
-// the sound speed are k_mask's expressions of the same primitives, gradients and v_rel (a check_physical
-// reset in k_mask leaves rho = 0, where neither is used).  With T = float the fields that the FP32 flux
+// Per-cell inputs of the flux phase gathered from the cell fields.  The sound speed is k_mask's
+// expression of the same primitives (a check_physical reset in k_mask leaves rho = 0, where neither the
+// sound speed nor the rate is used).  With T = float the fields that the FP32 flux
 // record holds in FP32 (primitives, sound speed, gc) are rounded the same way, so that gathered and
 // staged faces compute the same bits.
 template <class T>
@@ -871,9 +872,7 @@ struct SoaCell {
     return mk3(ldo(C->grad + (3 * q) * C->n, i), ldo(C->grad + (3 * q + 1) * C->n, i), ldo(C->grad + (3 * q + 2) * C->n, i));
   }
   __device__ __forceinline__ double ex(int q) const { return C->extrap[HFV_EX_W * (uint64_t)i + q]; }
-  __device__ __forceinline__ St<double> rate() const {
-    return tex_rate(ld5(C->prim, C->n, i), ldg(C->grad, C->n, i), ld3(C->vrel, C->n, i), *g);
-  }
+  __device__ __forceinline__ St<double> rate() const { return ld5(C->tex, C->n, i); }
   __device__ __forceinline__ V3<double> gc() const {
     const V3<double> v = ld3(C->gc, C->n, i);
     return mk3(rnd<T>(v.x), rnd<T>(v.y), rnd<T>(v.z));
@@ -1429,20 +1428,24 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_pack_pc(Batch B, int64_t lo, int6
   store_pc(B.C.pc, i, ld5(B.C.prim, B.C.n, i), ld3(B.C.cen, B.C.n, i));
 }
 
-// The flux records of every loaded staged simulation's cells from the SoA fields (after the
-// host changed the state or the mesh, or a phase call changed fields on its own).
+// The time-extrapolation rates of every loaded simulation's cells, and the flux records of the staged
+// ones, from the SoA fields (after the host changed the state or the mesh, or a phase call changed
+// fields on its own).
 __global__ void __launch_bounds__(HFV_BLOCK) k_pack_fr(Batch B) {
   const BlockJob J = B.cjobs[blockIdx.x];
-  if (J.sim < 0 || !B.sims[J.sim].staged) return;
+  if (J.sim < 0) return;
   const int64_t i = (int64_t)J.lo + threadIdx.x;
   if (i >= J.hi) return;
   const Cells& C = B.C;
   const int64_t n = C.n;
-  const Gas<double>& gd = B.sims[J.sim].cfg.gd;
+  const Cfg& cfg = B.sims[J.sim].cfg;
+  const Gas<double>& gd = cfg.gd;
   const St<double> p = ld5(C.prim, n, i);
   const Gr<double> g = ldg(C.grad, n, i);
-  store_fr(C.fr, i, B.sims[J.sim].cfg.precision == HFV_FP32_FLUX, p, fr_rate(p, g, ld3(C.vrel, n, i), gd), fr_cs(p, gd),
-           g, C.dt[i], ld3(C.gc, n, i));
+  const St<double> rate = fr_rate(p, g, ld3(C.vrel, n, i), gd);
+  st5(C.tex, n, i, rate);
+  if (B.sims[J.sim].staged)
+    store_fr(C.fr, i, cfg.precision == HFV_FP32_FLUX, p, rate, fr_cs(p, gd), g, C.dt[i], ld3(C.gc, n, i));
 }
 
 // glam::DMat3 as three columns
@@ -1638,7 +1641,7 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
   if (c.c[0] == 0.) {
     st3(C.v, n, i, mk3(0., 0., 0.));
     dt_req = INFINITY;
-    if (pack_fr) vrel_now = ld3(C.vrel, n, i);
+    vrel_now = ld3(C.vrel, n, i);
   } else {
     double m_inv = 1. / c.c[0];
     V3<double> loc = ld3(C.loc, n, i);
@@ -1698,10 +1701,12 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
   C.dt[i] = dt_new;
   C.bin[i] = get_time_bin(dti);
   *dti_out = dti;
-  // the cell's flux record for the next step (its prim, gradients, v_rel and dt are final here)
+  // the time-extrapolation rate and the flux record for the next step (prim, gradients, v_rel and dt
+  // are final here)
+  const St<double> rate_now = fr_rate(prim, gr, vrel_now, cfg.gd);
+  st5(C.tex, n, i, rate_now);
   if (pack_fr)
-    fill_fr_any(frv, cfg.precision == HFV_FP32_FLUX, prim, fr_rate(prim, gr, vrel_now, cfg.gd), fr_cs(prim, cfg.gd),
-                gr, dt_new, gcen);
+    fill_fr_any(frv, cfg.precision == HFV_FP32_FLUX, prim, rate_now, fr_cs(prim, cfg.gd), gr, dt_new, gcen);
 }
 
 template <bool STATIC>
