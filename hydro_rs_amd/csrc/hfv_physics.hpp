@@ -280,6 +280,25 @@ template <class T>
 __device__ __forceinline__ T rarefaction_tail_speed(T u, T a, T pdps, const Gas<T>& g) {
   return u - a * hpow(pdps, g.gm1d2g);
 }
+// rarefaction_tail_speed where only its sign against 0 is used.  FP64: the tail speed is first
+// estimated with an FP32 pow; for pdps in [1e-30, 1e30] the estimate's relative error on a*pdps^e is
+// below 2e-6 (a few ulp of powf, plus the rounding of pdps and e to FP32 amplified by |e ln pdps| < 35),
+// so an estimate farther than 4e-5 |a y| (+ 1e-14 of the operands for the FP64 rounding) from 0 has the
+// sign of the FP64 expression; only estimates inside that band use the FP64 pow.  Same decision as
+// evaluating rarefaction_tail_speed in FP64, at a fraction of the cost on most faces.
+template <class T>
+__device__ __forceinline__ T tail_speed_for_sign(T u, T a, T pdps, const Gas<T>& g) {
+  if constexpr (sizeof(T) == 8) {
+    if (pdps > 1e-30 && pdps < 1e30) {
+      const double y = (double)powf((float)pdps, (float)g.gm1d2g);
+      const double ay = a * y;
+      const double t = u - ay;
+      const double bound = 4e-5 * fabs(ay) + 1e-14 * (fabs(u) + fabs(ay));
+      if (t < -bound || t > bound) return t;
+    }
+  }
+  return rarefaction_tail_speed(u, a, pdps, g);
+}
 template <class T>
 __device__ __forceinline__ St<T> sample_rarefaction_fan(const St<T>& s, T a, T v, V3<T> n, const Gas<T>& g) {
   T v_half = g.tdgp1 * (a + g.half_gm1 * v);
@@ -301,7 +320,7 @@ __device__ __forceinline__ St<T> sample_from_star(const Star<T>& star, const St<
     }
     // right rarefaction
     if (v_r - (-a_r) > T(0)) {
-      if (rarefaction_tail_speed(star.u, -a_r, fdiv(star.p, right.c[4]), g) < T(0))
+      if (tail_speed_for_sign(star.u, -a_r, fdiv(star.p, right.c[4]), g) < T(0))
         return sample_rarefaction_fan(right, -a_r, v_r, n, g);
       return sample_middle_state(star.rho_r, star.u, star.p, right, v_r, n);
     }
@@ -313,7 +332,7 @@ __device__ __forceinline__ St<T> sample_from_star(const Star<T>& star, const St<
   }
   // left rarefaction
   if (v_l - a_l < T(0)) {
-    if (rarefaction_tail_speed(star.u, a_l, fdiv(star.p, left.c[4]), g) > T(0))
+    if (tail_speed_for_sign(star.u, a_l, fdiv(star.p, left.c[4]), g) > T(0))
       return sample_rarefaction_fan(left, a_l, v_l, n, g);
     return sample_middle_state(star.rho_l, star.u, star.p, left, v_l, n);
   }
