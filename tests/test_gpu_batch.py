@@ -153,10 +153,13 @@ def test_tuning_choices_are_bit_identical(staged, static_grad, precision):
     as the defaults (1D reflective, 2D periodic, 3D periodic)."""
     probs = [ics.sedov_2d(side=40, t_end=0.004, seed=1, precision=precision),
              ics.riemann_tube_1d(600, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, t_end=0.02, precision=precision),
-             ics.kelvin_helmholtz_3d(side=10, t_end=0.05, precision=precision)]
+             ics.kelvin_helmholtz_3d(side=16, t_end=0.03, precision=precision)]
     a = fvs.Batch.for_problems(probs)
     for s in range(len(probs)):
         a.set_tuning(s, staged, static_grad)
+    # the 2D and 3D members have staging lists, so staged = 1 runs the LDS-staged kernels
+    for s in (0, 2):
+        assert a.tuning(s) == {"staged": staged, "static_grad": static_grad}, probs[s].name
     a.run(1 << 40)
     b = fvs.Batch.for_problems(probs)
     b.run(1 << 40)
