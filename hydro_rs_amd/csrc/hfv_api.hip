@@ -623,10 +623,13 @@ int reset_slot(hfv_ctx* c, int s) {
     double* p;
     int w;
   } fl[] = {{C.prim, 5}, {C.grad, 15}, {C.gc, 3}, {C.cons, 5}, {C.facc, 5}, {C.loc, 3}, {C.v, 3},
-            {C.vrel, 3}, {C.maor, 1}, {C.msv, 1}, {C.dt, 1}, {C.tex, 5}};
+            {C.vrel, 3}, {C.maor, 1}, {C.msv, 1}, {C.dt, 1}};
   for (auto& f : fl)
     for (int k = 0; k < f.w; ++k) HIP_OK(hipMemsetAsync(f.p + k * N + cb, 0, sizeof(double) * cap, st));
-  HIP_OK(hipMemsetAsync(C.extrap + HFV_EX_W * cb, 0, sizeof(double) * HFV_EX_W * cap, st));
+  // extrapolations and kept rates: doubles, or floats in FP32-flux contexts (one precision per batch)
+  const size_t es = c->params[s].precision == HFV_FP32_FLUX ? sizeof(float) : sizeof(double);
+  HIP_OK(hipMemsetAsync((char*)C.extrap + es * HFV_EX_W * cb, 0, es * HFV_EX_W * cap, st));
+  for (int k = 0; k < 5; ++k) HIP_OK(hipMemsetAsync((char*)C.tex + es * (k * N + cb), 0, es * cap, st));
   HIP_OK(hipMemsetAsync(C.bin + cb, 0, cap, st));
   HIP_OK(hipMemsetAsync(C.wake + cb, 0, cap, st));
   HIP_OK(hipMemsetAsync(C.act + cb, 0, cap, st));
@@ -922,16 +925,25 @@ int copy_state(hfv_ctx* c, int s, hfv_state* st, bool to_device) {
   STATE_FIELDS(CPF)
 #undef CPF
   // extrapolations: AoS [N][5] on the device, component-major [5][n] at the boundary
+  // (floats in FP32-flux contexts)
+  const bool ex32 = c->params[s].precision == HFV_FP32_FLUX;
   std::vector<double> exa;
+  std::vector<float> exf;
   if (st->extrap) {
     exa.resize(HFV_EX_W * (size_t)n);
-    double* d = c->C.extrap + HFV_EX_W * cb;
+    if (ex32) exf.resize(HFV_EX_W * (size_t)n);
+    void* d = ex32 ? (void*)((float*)c->C.extrap + HFV_EX_W * cb) : (void*)(c->C.extrap + HFV_EX_W * cb);
+    const size_t bytes = (ex32 ? sizeof(float) : sizeof(double)) * HFV_EX_W * (size_t)n;
     if (to_device) {
       for (int64_t i = 0; i < n; ++i)
-        for (int q = 0; q < 5; ++q) exa[HFV_EX_W * i + q] = st->extrap[q * n + i];
-      HIP_OK(hipMemcpyAsync(d, exa.data(), sizeof(double) * exa.size(), hipMemcpyHostToDevice, strm));
+        for (int q = 0; q < 5; ++q) {
+          exa[HFV_EX_W * i + q] = st->extrap[q * n + i];
+          if (ex32) exf[HFV_EX_W * i + q] = (float)st->extrap[q * n + i];
+        }
+      HIP_OK(hipMemcpyAsync(d, ex32 ? (const void*)exf.data() : (const void*)exa.data(), bytes, hipMemcpyHostToDevice,
+                            strm));
     } else {
-      HIP_OK(hipMemcpyAsync(exa.data(), d, sizeof(double) * exa.size(), hipMemcpyDeviceToHost, strm));
+      HIP_OK(hipMemcpyAsync(ex32 ? (void*)exf.data() : (void*)exa.data(), d, bytes, hipMemcpyDeviceToHost, strm));
     }
   }
   if (st->timebin)
@@ -940,7 +952,7 @@ int copy_state(hfv_ctx* c, int s, hfv_state* st, bool to_device) {
   HIP_OK(hipStreamSynchronize(strm));
   if (st->extrap && !to_device)
     for (int64_t i = 0; i < n; ++i)
-      for (int q = 0; q < 5; ++q) st->extrap[q * n + i] = exa[HFV_EX_W * i + q];
+      for (int q = 0; q < 5; ++q) st->extrap[q * n + i] = ex32 ? (double)exf[HFV_EX_W * i + q] : exa[HFV_EX_W * i + q];
   if (to_device) {
     c->fr_dirty = true;
     if (st->v) {  // the static geometry needs every particle velocity to be exactly +0

@@ -41,6 +41,7 @@ struct Cells {
   double *prim, *grad, *gc, *extrap, *cons, *facc, *vol, *loc, *cen, *v, *vrel, *maor, *msv, *dt, *srad;
   double* pc;
   double* tex;   // [5][N] time-extrapolation rate of the cell's prim/grad/v_rel (written when they change)
+  // extrap and tex hold floats (same layouts) in FP32-flux contexts: ld_ex/st_ex/ld_tex/st_tex
   double* minv;  // [9][N] inverse of the gradient matrix (static geometry only)
   double* fr;    // [N][HFV_FR_W] flux inputs of static-geometry cells (layout: HFV_FR_*)
   int8_t *bin, *wake;
@@ -376,6 +377,47 @@ __device__ __forceinline__ void store_fr_wave(double* fr, int64_t i0, uint64_t h
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
+// Extrapolations (AoS [N][5]) and kept rates (SoA [5][N]) in the context's flux precision: doubles in
+// FP64 contexts, floats in the same layouts in FP32-flux contexts (reconstruction inputs; the FP32 path
+// rounds them to FP32 where it uses them anyway, so storing them rounded changes nothing downstream
+// except the FP32 rounding of the stored extrapolation sum at each step).
+__device__ __forceinline__ St<double> ld_ex(const Cells& C, int64_t i, bool f32) {
+  if (f32) {
+    const float* e = (const float*)C.extrap + HFV_EX_W * (uint64_t)i;
+    return mkst((double)e[0], mk3((double)e[1], (double)e[2], (double)e[3]), (double)e[4]);
+  }
+  const double* e = C.extrap + HFV_EX_W * (uint64_t)i;
+  return mkst(e[0], mk3(e[1], e[2], e[3]), e[4]);
+}
+__device__ __forceinline__ void st_ex(const Cells& C, int64_t i, bool f32, const St<double>& v) {
+  if (f32) {
+    float* e = (float*)C.extrap + HFV_EX_W * (uint64_t)i;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) e[q] = (float)v.c[q];
+    return;
+  }
+  double* e = C.extrap + HFV_EX_W * (uint64_t)i;
+#pragma unroll
+  for (int q = 0; q < 5; ++q) e[q] = v.c[q];
+}
+__device__ __forceinline__ St<double> ld_tex(const Cells& C, int64_t i, bool f32) {
+  if (f32) {
+    const float* t = (const float*)C.tex;
+    const uint64_t n = (uint64_t)C.n;
+    return mkst((double)t[i], mk3((double)t[n + i], (double)t[2 * n + i], (double)t[3 * n + i]), (double)t[4 * n + i]);
+  }
+  return ld5(C.tex, C.n, i);
+}
+__device__ __forceinline__ void st_tex(const Cells& C, int64_t i, bool f32, const St<double>& v) {
+  if (f32) {
+    float* t = (float*)C.tex;
+#pragma unroll
+    for (int q = 0; q < 5; ++q) t[(uint64_t)q * (uint64_t)C.n + i] = (float)v.c[q];
+    return;
+  }
+  st5(C.tex, C.n, i, v);
+}
+
 // The flux phase's per-cell rate and sound speed of primitives p (k_mask's expressions).
 __device__ __forceinline__ St<double> fr_rate(const St<double>& p, const Gr<double>& g, const V3<double>& vrel,
                                               const Gas<double>& gd) {
@@ -733,17 +775,14 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
       C.act[i] = act;
     }
     St<double> prim = ld5(C.prim, n, i);
-    // the rate of this prim/grad/v_rel, kept by k_grad (k_pack_fr after host changes)
-    const St<double> rate = prim.c[0] > 0. ? ld5(C.tex, n, i) : vacuum<double>();
-    double* exp_ = C.extrap + HFV_EX_W * (uint64_t)i;
-    St<double> ex = mkst(exp_[0], mk3(exp_[1], exp_[2], exp_[3]), exp_[4]);
+    const bool f32 = cfg.precision == HFV_FP32_FLUX;
     bool reset = false;
     if (mode != 2) {
+      // the rate of this prim/grad/v_rel, kept by k_grad (k_pack_fr after host changes);
       // extrapolations += time_extrapolations(dt) (its vacuum branch adds +0)
+      const St<double> rate = prim.c[0] > 0. ? ld_tex(C, i, f32) : vacuum<double>();
       St<double> te = prim.c[0] > 0. ? (-E.dt_pred) * rate : vacuum<double>();
-      ex = ex + te;
-#pragma unroll
-      for (int q = 0; q < 5; ++q) exp_[q] = ex.c[q];
+      st_ex(C, i, f32, ld_ex(C, i, f32) + te);
       check_physical(prim, ev);
       if (ev.check_physical) {
         reset = true;
@@ -871,8 +910,11 @@ struct SoaCell {
   __device__ __forceinline__ V3<double> grad(int q) const {
     return mk3(ldo(C->grad + (3 * q) * C->n, i), ldo(C->grad + (3 * q + 1) * C->n, i), ldo(C->grad + (3 * q + 2) * C->n, i));
   }
-  __device__ __forceinline__ double ex(int q) const { return C->extrap[HFV_EX_W * (uint64_t)i + q]; }
-  __device__ __forceinline__ St<double> rate() const { return ld5(C->tex, C->n, i); }
+  __device__ __forceinline__ double ex(int q) const {
+    if constexpr (sizeof(T) == 4) return ((const float*)C->extrap)[HFV_EX_W * (uint64_t)i + q];
+    return C->extrap[HFV_EX_W * (uint64_t)i + q];
+  }
+  __device__ __forceinline__ St<double> rate() const { return ld_tex(*C, i, sizeof(T) == 4); }
   __device__ __forceinline__ V3<double> gc() const {
     const V3<double> v = ld3(C->gc, C->n, i);
     return mk3(rnd<T>(v.x), rnd<T>(v.y), rnd<T>(v.z));
@@ -1166,7 +1208,9 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
 #pragma unroll
       for (int j = 0; j < NE; ++j) {
         const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / HFV_EX_W, q = k - HFV_EX_W * c;
-        if (c < ns) xv[j] = B.C.extrap[HFV_EX_W * (uint64_t)(uint32_t)scl[c] + q];
+        if (c < ns)
+          xv[j] = F32 ? (double)((const float*)B.C.extrap)[HFV_EX_W * (uint64_t)(uint32_t)scl[c] + q]
+                      : B.C.extrap[HFV_EX_W * (uint64_t)(uint32_t)scl[c] + q];
       }
       if (F32) {
         float* ldsf = (float*)lds;
@@ -1443,7 +1487,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_pack_fr(Batch B) {
   const St<double> p = ld5(C.prim, n, i);
   const Gr<double> g = ldg(C.grad, n, i);
   const St<double> rate = fr_rate(p, g, ld3(C.vrel, n, i), gd);
-  st5(C.tex, n, i, rate);
+  st_tex(C, i, cfg.precision == HFV_FP32_FLUX, rate);
   if (B.sims[J.sim].staged)
     store_fr(C.fr, i, cfg.precision == HFV_FP32_FLUX, p, rate, fr_cs(p, gd), g, C.dt[i], ld3(C.gc, n, i));
 }
@@ -1704,7 +1748,7 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
   // the time-extrapolation rate and the flux record for the next step (prim, gradients, v_rel and dt
   // are final here)
   const St<double> rate_now = fr_rate(prim, gr, vrel_now, cfg.gd);
-  st5(C.tex, n, i, rate_now);
+  st_tex(C, i, cfg.precision == HFV_FP32_FLUX, rate_now);
   if (pack_fr)
     fill_fr_any(frv, cfg.precision == HFV_FP32_FLUX, prim, rate_now, fr_cs(prim, cfg.gd), gr, dt_new, gcen);
 }
@@ -1881,8 +1925,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_limit(Batch B, int canon) {
     if (B.sims[s].staged) C.fr[HFV_FR_W * (uint64_t)i + HFV_FR_DT] = dt;
   }
   if (C.act[i]) {
-#pragma unroll
-    for (int q = 0; q < 5; ++q) C.extrap[HFV_EX_W * (uint64_t)i + q] = 0.;
+    st_ex(C, i, cfg.precision == HFV_FP32_FLUX, vacuum<double>());
     if (cfg.motion != MOT_FIXED) {
       double rho = C.prim[i];
       if (rho > 0.) {
