@@ -622,7 +622,7 @@ int reset_slot(hfv_ctx* c, int s) {
   struct {
     double* p;
     int w;
-  } fl[] = {{C.prim, 5}, {C.grad, 15}, {C.gc, 3}, {C.cons, 5}, {C.facc, 5}, {C.loc, 3}, {C.v, 3},
+  } fl[] = {{C.prim, 5}, {C.gc, 3}, {C.cons, 5}, {C.facc, 5}, {C.loc, 3}, {C.v, 3},
             {C.vrel, 3}, {C.maor, 1}, {C.msv, 1}, {C.dt, 1}};
   for (auto& f : fl)
     for (int k = 0; k < f.w; ++k) HIP_OK(hipMemsetAsync(f.p + k * N + cb, 0, sizeof(double) * cap, st));
@@ -630,6 +630,7 @@ int reset_slot(hfv_ctx* c, int s) {
   const size_t es = c->params[s].precision == HFV_FP32_FLUX ? sizeof(float) : sizeof(double);
   HIP_OK(hipMemsetAsync((char*)C.extrap + es * HFV_EX_W * cb, 0, es * HFV_EX_W * cap, st));
   for (int k = 0; k < 5; ++k) HIP_OK(hipMemsetAsync((char*)C.tex + es * (k * N + cb), 0, es * cap, st));
+  for (int k = 0; k < 15; ++k) HIP_OK(hipMemsetAsync((char*)C.grad + es * (k * N + cb), 0, es * cap, st));
   HIP_OK(hipMemsetAsync(C.bin + cb, 0, cap, st));
   HIP_OK(hipMemsetAsync(C.wake + cb, 0, cap, st));
   HIP_OK(hipMemsetAsync(C.act + cb, 0, cap, st));
@@ -906,7 +907,7 @@ void fill_engine_info(const SimEngine& E, hfv_engine_info* out) {
 }
 
 #define STATE_FIELDS(X)                                                                           \
-  X(prim, C.prim, 5) X(grad, C.grad, 15) X(gc, C.gc, 3) X(cons, C.cons, 5)                        \
+  X(prim, C.prim, 5) X(gc, C.gc, 3) X(cons, C.cons, 5)                                            \
   X(fluxes, C.facc, 5) X(volume, C.vol, 1) X(loc, C.loc, 3) X(centroid, C.cen, 3) X(v, C.v, 3)    \
   X(v_rel, C.vrel, 3) X(maor, C.maor, 1) X(msv, C.msv, 1) X(dt, C.dt, 1)
 
@@ -925,8 +926,30 @@ int copy_state(hfv_ctx* c, int s, hfv_state* st, bool to_device) {
   STATE_FIELDS(CPF)
 #undef CPF
   // extrapolations: AoS [N][5] on the device, component-major [5][n] at the boundary
-  // (floats in FP32-flux contexts)
+  // (floats in FP32-flux contexts, as the gradients)
   const bool ex32 = c->params[s].precision == HFV_FP32_FLUX;
+  std::vector<float> grf;
+  if (st->grad) {
+    if (ex32) {
+      grf.resize(15 * (size_t)n);
+      for (int q = 0; q < 15; ++q) {
+        float* d = (float*)c->C.grad + (size_t)q * N + cb;
+        if (to_device) {
+          for (int64_t i = 0; i < n; ++i) grf[(size_t)q * n + i] = (float)st->grad[q * n + i];
+          HIP_OK(hipMemcpyAsync(d, grf.data() + (size_t)q * n, sizeof(float) * n, hipMemcpyHostToDevice, strm));
+        } else {
+          HIP_OK(hipMemcpyAsync(grf.data() + (size_t)q * n, d, sizeof(float) * n, hipMemcpyDeviceToHost, strm));
+        }
+      }
+    } else {
+      for (int q = 0; q < 15; ++q) {
+        double* d = c->C.grad + (size_t)q * N + cb;
+        double* h = st->grad + (size_t)q * n;
+        HIP_OK(hipMemcpyAsync(to_device ? (void*)d : (void*)h, to_device ? (const void*)h : (const void*)d,
+                              sizeof(double) * n, kind, strm));
+      }
+    }
+  }
   std::vector<double> exa;
   std::vector<float> exf;
   if (st->extrap) {
@@ -950,6 +973,8 @@ int copy_state(hfv_ctx* c, int s, hfv_state* st, bool to_device) {
     HIP_OK(hipMemcpyAsync(to_device ? (void*)(c->C.bin + cb) : (void*)st->timebin,
                           to_device ? (const void*)st->timebin : (const void*)(c->C.bin + cb), (size_t)n, kind, strm));
   HIP_OK(hipStreamSynchronize(strm));
+  if (st->grad && ex32 && !to_device)
+    for (size_t k = 0; k < grf.size(); ++k) st->grad[k] = (double)grf[k];
   if (st->extrap && !to_device)
     for (int64_t i = 0; i < n; ++i)
       for (int q = 0; q < 5; ++q) st->extrap[q * n + i] = ex32 ? (double)exf[HFV_EX_W * i + q] : exa[HFV_EX_W * i + q];

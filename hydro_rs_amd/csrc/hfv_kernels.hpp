@@ -41,7 +41,8 @@ struct Cells {
   double *prim, *grad, *gc, *extrap, *cons, *facc, *vol, *loc, *cen, *v, *vrel, *maor, *msv, *dt, *srad;
   double* pc;
   double* tex;   // [5][N] time-extrapolation rate of the cell's prim/grad/v_rel (written when they change)
-  // extrap and tex hold floats (same layouts) in FP32-flux contexts: ld_ex/st_ex/ld_tex/st_tex
+  // extrap, tex and grad hold floats (same layouts) in FP32-flux contexts: ld_ex/st_ex/ld_tex/st_tex/
+  // ld_grad/st_grad
   double* minv;  // [9][N] inverse of the gradient matrix (static geometry only)
   double* fr;    // [N][HFV_FR_W] flux inputs of static-geometry cells (layout: HFV_FR_*)
   int8_t *bin, *wake;
@@ -418,6 +419,42 @@ __device__ __forceinline__ void st_tex(const Cells& C, int64_t i, bool f32, cons
   st5(C.tex, C.n, i, v);
 }
 
+// Gradients (SoA [15][N]) in the context's flux precision; FP32-flux contexts keep them rounded to FP32
+// (k_grad rounds its result before any use, so every consumer sees the stored values).
+__device__ __forceinline__ Gr<double> ld_grad(const Cells& C, int64_t i, bool f32) {
+  if (f32) {
+    const float* a = (const float*)C.grad;
+    const uint64_t n = (uint64_t)C.n;
+    Gr<double> g;
+#pragma unroll
+    for (int k = 0; k < 5; ++k)
+      g.g[k] = mk3((double)a[(3 * k) * n + i], (double)a[(3 * k + 1) * n + i], (double)a[(3 * k + 2) * n + i]);
+    return g;
+  }
+  return ldg(C.grad, C.n, i);
+}
+__device__ __forceinline__ void st_grad(const Cells& C, int64_t i, bool f32, const Gr<double>& g) {
+  if (f32) {
+    float* a = (float*)C.grad;
+    const uint64_t n = (uint64_t)C.n;
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+      a[(3 * k) * n + i] = (float)g.g[k].x;
+      a[(3 * k + 1) * n + i] = (float)g.g[k].y;
+      a[(3 * k + 2) * n + i] = (float)g.g[k].z;
+    }
+    return;
+  }
+  stg(C.grad, C.n, i, g);
+}
+__device__ __forceinline__ Gr<double> round_grad(const Gr<double>& g, bool f32) {
+  if (!f32) return g;
+  Gr<double> r;
+#pragma unroll
+  for (int k = 0; k < 5; ++k) r.g[k] = mk3((double)(float)g.g[k].x, (double)(float)g.g[k].y, (double)(float)g.g[k].z);
+  return r;
+}
+
 // The flux phase's per-cell rate and sound speed of primitives p (k_mask's expressions).
 __device__ __forceinline__ St<double> fr_rate(const St<double>& p, const Gr<double>& g, const V3<double>& vrel,
                                               const Gas<double>& gd) {
@@ -714,7 +751,8 @@ __device__ void drift_cell(const Cells& C, const Cfg& cfg, int64_t i, double dt,
         st5(C.prim, n, i, reflect(ld5(C.prim, n, i), nn));
         // Particle::reflect_gradients (part.rs:390-429), including its use of the already reflected
         // density gradient for the pressure gradient
-        Gr<double> g = ldg(C.grad, n, i);
+        const bool f32 = cfg.precision == HFV_FP32_FLUX;
+        Gr<double> g = ld_grad(C, i, f32);
         g.g[0] = g.g[0] - ((2. * dot(g.g[0], nn)) * nn);
         g.g[4] = g.g[4] - ((2. * dot(g.g[0], nn)) * nn);
         V3<double> dv_dx = mk3(g.g[1].x, g.g[2].x, g.g[3].x);
@@ -725,7 +763,7 @@ __device__ void drift_cell(const Cells& C, const Cfg& cfg, int64_t i, double dt,
         g.g[2] = g.g[2] - (nn.y * (2. * dvn));
         g.g[3] = g.g[3] - (nn.z * (2. * dvn));
         for (int q = 1; q <= 3; ++q) g.g[q] = g.g[q] - ((2. * dot(g.g[q], nn)) * nn);
-        stg(C.grad, n, i, g);
+        st_grad(C, i, f32, g);
       }
     } else {
       ev.fail(E_UNSUPPORTED);  // particle removal at open/vacuum walls is not on this path
@@ -908,6 +946,11 @@ struct SoaCell {
   }
   __device__ __forceinline__ double cs() const { return rnd<T>(fr_cs(ld5(C->prim, C->n, i), *g)); }
   __device__ __forceinline__ V3<double> grad(int q) const {
+    if constexpr (sizeof(T) == 4) {
+      const float* a = (const float*)C->grad;
+      const uint64_t n = (uint64_t)C->n;
+      return mk3((double)a[(3 * q) * n + i], (double)a[(3 * q + 1) * n + i], (double)a[(3 * q + 2) * n + i]);
+    }
     return mk3(ldo(C->grad + (3 * q) * C->n, i), ldo(C->grad + (3 * q + 1) * C->n, i), ldo(C->grad + (3 * q + 2) * C->n, i));
   }
   __device__ __forceinline__ double ex(int q) const {
@@ -1485,7 +1528,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_pack_fr(Batch B) {
   const Cfg& cfg = B.sims[J.sim].cfg;
   const Gas<double>& gd = cfg.gd;
   const St<double> p = ld5(C.prim, n, i);
-  const Gr<double> g = ldg(C.grad, n, i);
+  const Gr<double> g = ld_grad(C, i, cfg.precision == HFV_FP32_FLUX);
   const St<double> rate = fr_rate(p, g, ld3(C.vrel, n, i), gd);
   st_tex(C, i, cfg.precision == HFV_FP32_FLUX, rate);
   if (B.sims[J.sim].staged)
@@ -1668,12 +1711,12 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
       double alpha = (emax.c[q] == 0. || emin.c[q] == 0.) ? 1. : hmin(1.0, hmin(d_max / emax.c[q], d_min / emin.c[q]));
       g.g[q] = alpha * g.g[q];
     }
-    gr = g;
+    gr = round_grad(g, cfg.precision == HFV_FP32_FLUX);
     gcen = cen;
-    stg(C.grad, n, i, gr);
+    st_grad(C, i, cfg.precision == HFV_FP32_FLUX, gr);
     st3(C.gc, n, i, gcen);
   } else {
-    gr = ldg(C.grad, n, i);
+    gr = ld_grad(C, i, cfg.precision == HFV_FP32_FLUX);
     gcen = ld3(C.gc, n, i);
   }
   if (!do_ts) return;
@@ -1930,7 +1973,9 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_limit(Batch B, int canon) {
       double rho = C.prim[i];
       if (rho > 0.) {
         double kick_fac = 0.5 * C.dt[i] / rho;
-        for (int a = 0; a < cfg.dim; ++a) C.v[a * n + i] -= kick_fac * C.grad[(12 + a) * n + i];
+        const Gr<double> g = ld_grad(C, i, cfg.precision == HFV_FP32_FLUX);
+        const double dp[3] = {g.g[4].x, g.g[4].y, g.g[4].z};
+        for (int a = 0; a < cfg.dim; ++a) C.v[a * n + i] -= kick_fac * dp[a];
       }
     }
     if (canon) {
