@@ -77,7 +77,7 @@ class KernelTimes(C.Structure):
 
 class TuneClass(C.Structure):
     _fields_ = [("dim", C.c_int), ("n_sims", C.c_int), ("cells_lo", I64), ("cells_hi", I64), ("staged", C.c_int),
-                ("static_grad", C.c_int), ("ns_per_update", C.c_double * 4)]
+                ("static_grad", C.c_int), ("gstaged", C.c_int), ("pad_", C.c_int), ("ns_per_update", C.c_double * 8)]
 
 
 KERNEL_SLOTS = ["mask_predict", "mesh1d", "flux", "apply_c2p", "grad_timestep", "wakeup", "limit_kick", "drift"]
@@ -117,8 +117,8 @@ HFV_SYMBOLS = {
     "hfv_set_geometry_mode": (C.c_int, [C.c_void_p, C.c_int]),
     "hfv_get_geometry_mode": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int)]),
     "hfv_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(TuneClass), C.c_int, C.POINTER(C.c_int)]),
-    "hfv_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
-    "hfv_get_tuning": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "hfv_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "hfv_get_tuning": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "hfv_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "hfv_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTimes)]),
 }

@@ -148,9 +148,11 @@ struct hfv_ctx {
   bool use_static = true;    // precomputed geometry for fixed meshes (hfv_set_geometry_mode)
   std::vector<int> v_zero;   // per slot: every particle velocity is +0 (fixed meshes keep it so)
   std::vector<int> has_stage;  // per slot: some face chunk has a staging list
-  // per slot, set by hfv_autotune: LDS-staged flux tiling allowed (1) or not (0); precomputed
-  // gradient geometry on (1), off (0) or by dimension (-1: on for 3D)
-  std::vector<int> tune_stage, tune_sgrad;
+  std::vector<int> has_gstage;  // per slot: some cell chunk has a neighbour staging list
+  // per slot, set by hfv_autotune / hfv_set_tuning: LDS-staged flux tiling allowed (1) or not (0);
+  // precomputed gradient geometry and LDS-staged gradient neighbours on (1), off (0) or by dimension
+  // (-1: on for 3D)
+  std::vector<int> tune_stage, tune_sgrad, tune_gstage;
   bool any_static = false, any_dynamic = false, any_static_grad = false, any_dynamic_grad = false;
   SimDesc* ddesc = nullptr;
   SimEngine* deng = nullptr;
@@ -340,13 +342,18 @@ int alloc_batch(hfv_ctx* c) {
   F.scell = dalloc<int32_t>(c, nfc * HFV_STAGE_MAX);
   F.scnt = dalloc<uint8_t>(c, nfc);
   F.floc = dalloc<uint8_t>(c, 2 * nf);
+  F.gcell = dalloc<int32_t>(c, ncc * HFV_GSTAGE_MAX);
+  F.gcnt = dalloc<uint16_t>(c, ncc);
+  F.nloc = dalloc<uint16_t>(c, nk);
   C.minv = dalloc<double>(c, 9 * n);
   F.eg = dalloc<double>(c, 8 * nk);
   F.fs = dalloc<double>(c, 4 * nf);
   c->v_zero.assign(S, 1);
   c->has_stage.assign(S, 0);
+  c->has_gstage.assign(S, 0);
   c->tune_stage.assign(S, 1);
   c->tune_sgrad.assign(S, -1);
+  c->tune_gstage.assign(S, -1);
   c->ddesc = dalloc<SimDesc>(c, S);
   c->deng = dalloc<SimEngine>(c, S);
   for (void* p : c->bufs)
@@ -360,6 +367,7 @@ int alloc_batch(hfv_ctx* c) {
   HIP_OK(hipMemsetAsync(C.fr, 0, sd * HFV_FR_W, st));
   HIP_OK(hipMemsetAsync(F.scnt, 0, (size_t)std::max<int64_t>(nfc, 1), st));
   HIP_OK(hipMemsetAsync(F.floc, 0, 2 * (size_t)std::max<int64_t>(nf, 1), st));
+  HIP_OK(hipMemsetAsync(F.gcnt, 0, sizeof(uint16_t) * (size_t)std::max<int64_t>(ncc, 1), st));
   HIP_OK(hipMemsetAsync(C.bin, 0, (size_t)n, st));
   HIP_OK(hipMemsetAsync(C.wake, 0, (size_t)n, st));
   HIP_OK(hipMemsetAsync(C.act, 0, (size_t)n, st));
@@ -563,6 +571,37 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
   }
   if (nf) HIP_OK(hipMemcpyAsync(F.floc + 2 * fb, floc.data(), 2 * (size_t)nf, hipMemcpyHostToDevice, st));
   c->has_stage[s] = std::any_of(scnt.begin(), scnt.end(), [](uint8_t k) { return k > 0; }) ? 1 : 0;
+  // LDS staging lists of the cell chunks (the distinct neighbour cells of each chunk's CSR entries)
+  {
+    const int64_t ncch = (n + BLOCK - 1) / BLOCK;
+    std::vector<int32_t> gcell(std::max<int64_t>(ncch, 1) * HFV_GSTAGE_MAX, 0);
+    std::vector<uint16_t> gcnt(std::max<int64_t>(ncch, 1), 0), nloc(std::max<int64_t>(tot, 1), 0xFFFF);
+    std::vector<int32_t> u;
+    u.reserve(16 * BLOCK);
+    for (int64_t j = 0; j < ncch; ++j) {
+      const int64_t i0 = j * BLOCK, i1 = std::min<int64_t>(i0 + BLOCK, n);
+      const int64_t e0 = off[i0] - kb, e1 = (i1 < n ? off[i1] : kb + acc) - kb;
+      u.clear();
+      for (int64_t e = e0; e < e1; ++e)
+        if (nbr[e] >= 0) u.push_back(nbr[e]);
+      std::sort(u.begin(), u.end());
+      u.erase(std::unique(u.begin(), u.end()), u.end());
+      if ((int64_t)u.size() > HFV_GSTAGE_MAX) continue;
+      gcnt[j] = (uint16_t)u.size();
+      std::copy(u.begin(), u.end(), gcell.begin() + j * HFV_GSTAGE_MAX);
+      for (int64_t e = e0; e < e1; ++e)
+        if (nbr[e] >= 0) nloc[e] = (uint16_t)(std::lower_bound(u.begin(), u.end(), nbr[e]) - u.begin());
+    }
+    if (ncch) {
+      const int64_t k0 = c->cchunk0[s];
+      HIP_OK(hipMemcpyAsync(F.gcell + k0 * HFV_GSTAGE_MAX, gcell.data(), sizeof(int32_t) * ncch * HFV_GSTAGE_MAX,
+                            hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(F.gcnt + k0, gcnt.data(), sizeof(uint16_t) * ncch, hipMemcpyHostToDevice, st));
+    }
+    if (tot) HIP_OK(hipMemcpyAsync(F.nloc + kb, nloc.data(), sizeof(uint16_t) * tot, hipMemcpyHostToDevice, st));
+    c->has_gstage[s] = std::any_of(gcnt.begin(), gcnt.end(), [](uint16_t k) { return k > 0; }) ? 1 : 0;
+    HIP_OK(hipStreamSynchronize(st));
+  }
   // Particle::update_geometry (part.rs:151-158) for the selected cells
   const int64_t N = c->n;
   std::vector<double> vol(n), cen(3 * n), srad(n);
@@ -651,6 +690,8 @@ int refresh_geometry(hfv_ctx* c, int s) {
   c->desc[s].static_geom = st ? 1 : 0;
   c->desc[s].static_grad = st && (c->tune_sgrad[s] < 0 ? p.dim == 3 : c->tune_sgrad[s] != 0) ? 1 : 0;
   c->desc[s].staged = st && c->has_stage[s] && c->tune_stage[s] ? 1 : 0;
+  c->desc[s].gstaged =
+      st && c->has_gstage[s] && (c->tune_gstage[s] < 0 ? p.dim == 3 : c->tune_gstage[s] != 0) ? 1 : 0;
   c->fr_dirty = true;
   HIP_OK(hipMemcpyAsync(c->ddesc + s, &c->desc[s], sizeof(SimDesc), hipMemcpyHostToDevice, c->stream));
   if (c->nused[s] > 0)  // neighbour primitives/centroids of the gradient sweeps
@@ -1409,21 +1450,23 @@ int hfv_get_geometry_mode(hfv_ctx* c, int slot, int* is_static) {
   return HFV_OK;
 }
 
-int hfv_set_tuning(hfv_ctx* c, int slot, int staged, int static_grad) {
+int hfv_set_tuning(hfv_ctx* c, int slot, int staged, int static_grad, int gstaged) {
   TRY(check_slot(c, slot));
-  if (staged < -1 || staged > 1 || static_grad < -1 || static_grad > 1)
+  if (staged < -1 || staged > 1 || static_grad < -1 || static_grad > 1 || gstaged < -1 || gstaged > 1)
     return set_err(HFV_ERR_INVALID, "tuning values must be -1, 0 or 1");
   HIP_OK(hipStreamSynchronize(c->stream));
   c->tune_stage[slot] = staged < 0 ? 1 : staged;
   c->tune_sgrad[slot] = static_grad;
+  c->tune_gstage[slot] = gstaged;
   if (c->loaded[slot]) TRY(refresh_geometry(c, slot));
   return HFV_OK;
 }
 
-int hfv_get_tuning(hfv_ctx* c, int slot, int* staged, int* static_grad) {
+int hfv_get_tuning(hfv_ctx* c, int slot, int* staged, int* static_grad, int* gstaged) {
   TRY(check_slot(c, slot));
   if (staged) *staged = c->loaded[slot] ? c->desc[slot].staged : 0;
   if (static_grad) *static_grad = c->loaded[slot] ? c->desc[slot].static_grad : 0;
+  if (gstaged) *gstaged = c->loaded[slot] ? c->desc[slot].gstaged : 0;
   return HFV_OK;
 }
 
@@ -1439,9 +1482,9 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
   struct Cls {
     int dim, bucket;
     std::vector<int> slots;
-    bool can_stage = false;
-    double t[4] = {0, 0, 0, 0};
-    double u[4] = {0, 0, 0, 0};
+    bool can_stage = false, can_gstage = false;
+    double t[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double u[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     int best = -1;
   };
   std::vector<Cls> cls;
@@ -1459,11 +1502,13 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
     }
     cls[k].slots.push_back(s);
     if (c->has_stage[s]) cls[k].can_stage = true;
+    if (c->has_gstage[s]) cls[k].can_gstage = true;
   }
   auto apply = [&](const Cls& k, int cand) -> int {
     for (int s : k.slots) {
-      c->tune_stage[s] = cand >> 1;
       c->tune_sgrad[s] = cand & 1;
+      c->tune_stage[s] = (cand >> 1) & 1;
+      c->tune_gstage[s] = (cand >> 2) & 1;
       TRY(refresh_geometry(c, s));
     }
     return HFV_OK;
@@ -1480,8 +1525,8 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
   bool all_done = false;
   for (auto& k : cls) {
     std::vector<int> cands;
-    for (int cand = 0; cand < 4; ++cand)
-      if (k.can_stage || !(cand >> 1)) cands.push_back(cand);
+    for (int cand = 0; cand < 8; ++cand)
+      if ((k.can_stage || !(cand & 2)) && (k.can_gstage || !(cand & 4))) cands.push_back(cand);
     for (int r = 0; r < rounds && !all_done && rc == HFV_OK; ++r)
       for (int cand : cands) {
         if ((rc = apply(k, cand)) != HFV_OK) break;
@@ -1508,7 +1553,7 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
         k.best = cand;
         best = k.t[cand] / k.u[cand];
       }
-    if (k.best < 0) k.best = (k.can_stage ? 2 : 0) | (k.dim == 3 ? 1 : 0);  // nothing measured: defaults
+    if (k.best < 0) k.best = (k.can_stage ? 2 : 0) | (k.dim == 3 ? 1 : 0) | (k.dim == 3 && k.can_gstage ? 4 : 0);
     if ((rc = apply(k, k.best)) != HFV_OK) break;
   }
   c->prof = prof;
@@ -1521,9 +1566,10 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
     o.cells_lo = kTuneEdges[cls[k].bucket];
     o.cells_hi = kTuneEdges[cls[k].bucket + 1];
     o.n_sims = (int)cls[k].slots.size();
-    o.staged = cls[k].best >> 1;
+    o.staged = (cls[k].best >> 1) & 1;
     o.static_grad = cls[k].best & 1;
-    for (int q = 0; q < 4; ++q) o.ns_per_update[q] = cls[k].u[q] > 0. ? 1e9 * cls[k].t[q] / cls[k].u[q] : -1.;
+    o.gstaged = (cls[k].best >> 2) & 1;
+    for (int q = 0; q < 8; ++q) o.ns_per_update[q] = cls[k].u[q] > 0. ? 1e9 * cls[k].t[q] / cls[k].u[q] : -1.;
   }
   return HFV_OK;
 }

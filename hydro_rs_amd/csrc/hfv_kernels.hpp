@@ -62,6 +62,11 @@ struct Faces {
   // indices of its left and right cell in that list (floc[2F], 255 for a wall).
   int32_t* scell;
   uint8_t *scnt, *floc;
+  // LDS staging of the gradient sweeps: per cell chunk the distinct neighbour cells of its cells
+  // (gcell[chunk][HFV_GSTAGE_MAX], ascending; gcnt = 0: more than HFV_GSTAGE_MAX), and per CSR entry the
+  // neighbour's list slot (nloc[2F], 0xFFFF for a wall)
+  int32_t* gcell;
+  uint16_t *gcnt, *nloc;
 };
 
 struct Cfg {
@@ -83,6 +88,7 @@ struct SimDesc {
   int static_geom;  // fixed mesh with zero particle velocities: face geometry precomputed (k_geom_faces)
   int static_grad;  // ... and gradient geometry too (k_geom_cells; chosen for 3D meshes)
   int staged;       // static_geom and some face chunk has a staging list: the flux records fr are kept
+  int gstaged;      // static_geom and the gradient sweeps stage their chunks' neighbour lines
 };
 
 // Thread block -> work: simulation (-1: nothing), items [lo, hi), chunk (index of its partials).
@@ -167,6 +173,7 @@ struct Batch {
 #define HFV_FR32_EX HFV_FR32_USED
 #define HFV_FR32_LDS 37
 #define HFV_STAGE_MAX 128
+#define HFV_GSTAGE_MAX 600  // neighbour pc lines a gradient block stages (64 bytes each)
 
 #define HFV_NUM_TIME_BINS 56
 #define HFV_MAX_NR_TIMESTEPS (1ull << 57)
@@ -1584,7 +1591,8 @@ __device__ __forceinline__ void boundary_neighbour(const Faces& F, const Cfg& cf
 // 1D/2D meshes recomputing them reads fewer bytes than the precomputed lines, so those stay dynamic.)
 template <bool STATIC>
 __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const SimEngine& E, int64_t i, int do_grad,
-                                          int do_ts, bool pack_fr, double* frv, uint64_t* dti_out, Events& ev) {
+                                          int do_ts, bool pack_fr, double* frv, bool v_zero, const double4* gst,
+                                          uint64_t* dti_out, Events& ev) {
   const Cells& C = B.C;
   const Faces& F = B.F;
   const int64_t n = C.n, nf = F.nf;
@@ -1610,7 +1618,15 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
     }
     St<double> lmin = prim, lmax = prim;
     for (int32_t k = k0; k < k1; ++k) {
-      const int32_t other = F.nbr[k];
+      // the neighbour's pc line: from the block's LDS stage (gst) or gathered
+      uint32_t li = 0xFFFFu;
+      int32_t other;
+      if (gst) {
+        li = F.nloc[k];
+        other = li == 0xFFFFu ? -1 : 0;
+      } else {
+        other = F.nbr[k];
+      }
       V3<double> ds;
       double w;
       St<double> op;
@@ -1620,7 +1636,8 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
         ds = mk3(a.y, a.z, a.w);
         if (other >= 0) {
           V3<double> oc;
-          load_pc(C.pc, (uint32_t)other, op, oc);
+          if (gst) load_pc((const double*)(gst + 2 * li), 0, op, oc);
+          else load_pc(C.pc, (uint32_t)other, op, oc);
         } else {
           op = prim;  // Space::get_boundary_part (space.rs:213-229)
           if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, ld3(F.nrm, nf, (uint32_t)(F.conn[k] >> 1)));
@@ -1633,7 +1650,8 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
         const V3<double> shift = (e & 1) == 0 ? sh : -sh;
         V3<double> oc;
         if (other >= 0) {
-          load_pc(C.pc, (uint32_t)other, op, oc);
+          if (gst) load_pc((const double*)(gst + 2 * li), 0, op, oc);
+          else load_pc(C.pc, (uint32_t)other, op, oc);
         } else {
           boundary_neighbour(F, cfg, f, cen, prim, oc, op);
         }
@@ -1726,7 +1744,7 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
   double dt_req;
   V3<double> vrel_now;
   if (c.c[0] == 0.) {
-    st3(C.v, n, i, mk3(0., 0., 0.));
+    if (!v_zero) st3(C.v, n, i, mk3(0., 0., 0.));
     dt_req = INFINITY;
     vrel_now = ld3(C.vrel, n, i);
   } else {
@@ -1773,7 +1791,8 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
     }
     if (!finite3(v)) ev.fail(E_NONFINITE);
     V3<double> v_rel = fluid_v - v;
-    st3(C.v, n, i, v);
+    // static geometry: fixed motion with every velocity +0 already, the value written here
+    if (!v_zero) st3(C.v, n, i, v);
     st3(C.vrel, n, i, v_rel);
     vrel_now = v_rel;
     C.maor[i] = 0.;
@@ -1809,16 +1828,56 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_GRAD_WAVES) k_grad(Batch B, int
   uint64_t my = ~0ull;
   Events ev;
   const bool pack = do_ts && B.sims[s].staged;  // block-uniform
+  // LDS: the staged neighbour lines of the gradient sweeps, then (once every wave is done with them)
+  // the per-wave transposes of the new flux records
+  __shared__ double4 lds4[HFV_GSTAGE_MAX * 2];
+  static_assert(HFV_GSTAGE_MAX * 8 >= HFV_BLOCK * 17, "record transposes fit the stage buffer");
+  const int gn = (do_grad && B.sims[s].gstaged) ? (int)B.F.gcnt[J.chunk] : 0;
+  bool staged = false;
+  if (gn > 0) {
+    staged = __syncthreads_count(a) > 0;  // block-uniform
+    if (staged) {
+      // the chunk's list, then every line load of a round, issued together (two rounds)
+      constexpr int NL = (HFV_GSTAGE_MAX * 4 + HFV_BLOCK - 1) / HFV_BLOCK;
+      constexpr int NR = (NL + 1) / 2;
+      const int32_t* gl = B.F.gcell + (uint64_t)J.chunk * HFV_GSTAGE_MAX;
+      double2* d2 = (double2*)lds4;
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        int32_t cell[NR];
+        double2 v[NR];
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+          const int k = (int)threadIdx.x + (r * NR + j) * HFV_BLOCK;
+          cell[j] = (k >> 2) < gn ? gl[k >> 2] : 0;
+        }
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+          const int k = (int)threadIdx.x + (r * NR + j) * HFV_BLOCK;
+          v[j] = (k >> 2) < gn ? ((const double2*)(B.C.pc + 8 * (uint64_t)(uint32_t)cell[j]))[k & 3]
+                               : make_double2(0., 0.);
+        }
+#pragma unroll
+        for (int j = 0; j < NR; ++j) {
+          const int k = (int)threadIdx.x + (r * NR + j) * HFV_BLOCK;
+          if ((k >> 2) < gn) d2[k] = v[j];
+        }
+      }
+      __syncthreads();
+    }
+  }
   double frv[HFV_FR_W];
 #pragma unroll
   for (int k = 0; k < HFV_FR_W; ++k) frv[k] = 0.;
-  if (a) grad_cell<STATIC>(B, cfg, E, i, do_grad, do_ts, pack, frv, &my, ev);
+  if (a)
+    grad_cell<STATIC>(B, cfg, E, i, do_grad, do_ts, pack, frv, B.sims[s].static_geom != 0, staged ? lds4 : nullptr,
+                      &my, ev);
   if (pack) {
-    __shared__ double wbuf[HFV_BLOCK / 64][64 * 17];
+    if (staged) __syncthreads();  // the staged lines are dead from here on
+    double* wbuf = (double*)lds4 + (threadIdx.x >> 6) * (64 * 17);
     const uint64_t has = __ballot(a);
     if (has)
-      store_fr_wave(B.C.fr, i - (int64_t)__lane_id(), has, frv, wbuf[threadIdx.x >> 6],
-                    cfg.precision == HFV_FP32_FLUX ? 1 : 2);
+      store_fr_wave(B.C.fr, i - (int64_t)__lane_id(), has, frv, wbuf, cfg.precision == HFV_FP32_FLUX ? 1 : 2);
   }
   if (do_ts) {
     uint64_t t = block_min_u64(my);

@@ -238,14 +238,15 @@ class EulerEqnsFvs:
         (hfv_autotune; bit-identical candidates, so the launches are ordinary steps)."""
         return _autotune(self.lib, self.ptr, launches, rounds)
 
-    def set_tuning(self, slot=0, staged=-1, static_grad=-1):
-        """Flux tiling (staged) and gradient geometry (static_grad) of one slot: 1/0, -1 = default."""
-        _lib.check(self.lib.hfv_set_tuning(self.ptr, int(slot), int(staged), int(static_grad)))
+    def set_tuning(self, slot=0, staged=-1, static_grad=-1, gstaged=-1):
+        """Flux tiling (staged), gradient geometry (static_grad) and gradient neighbour staging
+        (gstaged) of one slot: 1/0, -1 = default."""
+        _lib.check(self.lib.hfv_set_tuning(self.ptr, int(slot), int(staged), int(static_grad), int(gstaged)))
 
     def tuning(self, slot=0):
-        a, b = C.c_int(0), C.c_int(0)
-        _lib.check(self.lib.hfv_get_tuning(self.ptr, int(slot), C.byref(a), C.byref(b)))
-        return {"staged": a.value, "static_grad": b.value}
+        a, b, g = C.c_int(0), C.c_int(0), C.c_int(0)
+        _lib.check(self.lib.hfv_get_tuning(self.ptr, int(slot), C.byref(a), C.byref(b), C.byref(g)))
+        return {"staged": a.value, "static_grad": b.value, "gstaged": g.value}
 
 
 def _engine_dict(e):
@@ -361,14 +362,15 @@ class Batch:
         (hfv_autotune; bit-identical candidates, so the launches are ordinary steps)."""
         return _autotune(self.lib, self.ptr, launches, rounds)
 
-    def set_tuning(self, slot=0, staged=-1, static_grad=-1):
-        """Flux tiling (staged) and gradient geometry (static_grad) of one slot: 1/0, -1 = default."""
-        _lib.check(self.lib.hfv_set_tuning(self.ptr, int(slot), int(staged), int(static_grad)))
+    def set_tuning(self, slot=0, staged=-1, static_grad=-1, gstaged=-1):
+        """Flux tiling (staged), gradient geometry (static_grad) and gradient neighbour staging
+        (gstaged) of one slot: 1/0, -1 = default."""
+        _lib.check(self.lib.hfv_set_tuning(self.ptr, int(slot), int(staged), int(static_grad), int(gstaged)))
 
     def tuning(self, slot=0):
-        a, b = C.c_int(0), C.c_int(0)
-        _lib.check(self.lib.hfv_get_tuning(self.ptr, int(slot), C.byref(a), C.byref(b)))
-        return {"staged": a.value, "static_grad": b.value}
+        a, b, g = C.c_int(0), C.c_int(0), C.c_int(0)
+        _lib.check(self.lib.hfv_get_tuning(self.ptr, int(slot), C.byref(a), C.byref(b), C.byref(g)))
+        return {"staged": a.value, "static_grad": b.value, "gstaged": g.value}
 
 
 def _autotune(lib, ptr, launches, rounds):
@@ -376,7 +378,8 @@ def _autotune(lib, ptr, launches, rounds):
     n = C.c_int(0)
     _lib.check(lib.hfv_autotune(ptr, int(launches), int(rounds), out, 64, C.byref(n)))
     return [{"dim": o.dim, "n_sims": o.n_sims, "cells": [o.cells_lo, o.cells_hi], "staged": o.staged,
-             "static_grad": o.static_grad, "ns_per_update": [round(x, 4) for x in o.ns_per_update]}
+             "static_grad": o.static_grad, "gstaged": o.gstaged,
+             "ns_per_update": [round(x, 4) for x in o.ns_per_update]}
             for o in out[: min(n.value, 64)]]
 
 

@@ -253,27 +253,30 @@ int hfv_get_geometry_mode(hfv_ctx* ctx, int slot, int* is_static);
 /* ---------------------------------------------------------------- autotuner */
 /* Per size class of the live static-geometry simulations (one dimension, cell count in
  * [cells_lo, cells_hi); buckets 0 / 1024 / 8192 / 65536), hfv_autotune chooses the flux kernel's
- * tiling (its chunks' cells staged through LDS, or each face gathering its cells) and the gradient
- * geometry (precomputed per CSR entry, or evaluated per step) by timing `rounds` x `launches` real
- * launches of each candidate in turn, and keeps the fastest per cell update.  Every candidate gives
- * bit-identical results, so the tuning launches are ordinary steps of the run.  No counterpart in the
- * reference (it has no device kernels to tune).  ns_per_update: measured cost per candidate
- * (staged, static_grad) = (0,0), (0,1), (1,0), (1,1); -1 where not tried.  out may hold max_out
- * classes; n_out receives the number of classes. */
+ * tiling (its chunks' cells staged through LDS, or each face gathering its cells), the gradient
+ * geometry (precomputed per CSR entry, or evaluated per step) and the gradient sweeps' neighbour
+ * access (each cell chunk's neighbour lines staged through LDS, or gathered) by timing `rounds` x
+ * `launches` real launches of each candidate in turn, and keeps the fastest per cell update.  Every
+ * candidate gives bit-identical results, so the tuning launches are ordinary steps of the run.  No
+ * counterpart in the reference (it has no device kernels to tune).  ns_per_update[c]: measured cost of
+ * candidate c = static_grad + 2 staged + 4 gstaged; -1 where not tried.  out may hold max_out classes;
+ * n_out receives the number of classes. */
 typedef struct {
   int dim;
   int n_sims;
   int64_t cells_lo, cells_hi;
   int staged;
   int static_grad;
-  double ns_per_update[4];
+  int gstaged;
+  int pad_;
+  double ns_per_update[8];
 } hfv_tune_class;
 int hfv_autotune(hfv_ctx* ctx, int launches, int rounds, hfv_tune_class* out, int max_out, int* n_out);
-/* The same choices for one slot by hand: staged 1/0 (-1: allowed, the default), static_grad 1/0 (-1:
- * by dimension, on for 3D, the default).  They take effect for static-geometry simulations only;
- * hfv_get_tuning reports what a slot currently runs. */
-int hfv_set_tuning(hfv_ctx* ctx, int slot, int staged, int static_grad);
-int hfv_get_tuning(hfv_ctx* ctx, int slot, int* staged, int* static_grad);
+/* The same choices for one slot by hand: staged 1/0 (-1: allowed, the default), static_grad and
+ * gstaged 1/0 (-1: by dimension, on for 3D, the default).  They take effect for static-geometry
+ * simulations only; hfv_get_tuning reports what a slot currently runs. */
+int hfv_set_tuning(hfv_ctx* ctx, int slot, int staged, int static_grad, int gstaged);
+int hfv_get_tuning(hfv_ctx* ctx, int slot, int* staged, int* static_grad, int* gstaged);
 
 /* ---------------------------------------------------------------- measurement */
 /* Device time of the step's kernels, from HIP events recorded on the context's stream around each

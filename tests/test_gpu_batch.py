@@ -147,8 +147,8 @@ def test_hard_error_matches_oracle_step(ora):
 
 
 @pytest.mark.parametrize("precision", [fvs.FP64, fvs.FP32_FLUX])
-@pytest.mark.parametrize("staged,static_grad", [(0, 0), (0, 1), (1, 0), (1, 1)])
-def test_tuning_choices_are_bit_identical(staged, static_grad, precision):
+@pytest.mark.parametrize("staged,static_grad,gstaged", [(s, g, n) for s in (0, 1) for g in (0, 1) for n in (0, 1)])
+def test_tuning_choices_are_bit_identical(staged, static_grad, gstaged, precision):
     """Each flux tiling / gradient geometry choice, forced on every slot, ends a run on the same bits
     as the defaults (1D reflective, 2D periodic, 3D periodic)."""
     probs = [ics.sedov_2d(side=40, t_end=0.004, seed=1, precision=precision),
@@ -156,10 +156,10 @@ def test_tuning_choices_are_bit_identical(staged, static_grad, precision):
              ics.kelvin_helmholtz_3d(side=16, t_end=0.03, precision=precision)]
     a = fvs.Batch.for_problems(probs)
     for s in range(len(probs)):
-        a.set_tuning(s, staged, static_grad)
-    # the 2D and 3D members have staging lists, so staged = 1 runs the LDS-staged kernels
+        a.set_tuning(s, staged, static_grad, gstaged)
+    # the 2D and 3D members have staging lists, so staged / gstaged = 1 run the LDS-staged kernels
     for s in (0, 2):
-        assert a.tuning(s) == {"staged": staged, "static_grad": static_grad}, probs[s].name
+        assert a.tuning(s) == {"staged": staged, "static_grad": static_grad, "gstaged": gstaged}, probs[s].name
     a.run(1 << 40)
     b = fvs.Batch.for_problems(probs)
     b.run(1 << 40)
@@ -187,7 +187,7 @@ def test_autotune_is_bit_identical():
     dims = sorted({r["dim"] for r in report})
     assert dims == [1, 2, 3]
     for r in report:
-        assert r["staged"] in (0, 1) and r["static_grad"] in (0, 1)
+        assert r["staged"] in (0, 1) and r["static_grad"] in (0, 1) and r["gstaged"] in (0, 1)
         tried = [x for x in r["ns_per_update"] if x >= 0]
         assert len(tried) >= 2, r
     a.run(1 << 40)
