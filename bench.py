@@ -420,10 +420,10 @@ def main():
     ap.add_argument("--workload", default="all", choices=["all", "ensemble", "sedov", "kh3d", "serve"])
     ap.add_argument("--members", type=int, default=512)
     ap.add_argument("--plain-stride", type=int, default=16)
-    ap.add_argument("--tune-launches", type=int, default=4,
-                    help="ensemble autotuner: launches per candidate trial (0: no tuning)")
+    ap.add_argument("--tune-launches", type=int, default=2,
+                    help="ensemble autotuner: launches per timed group of a candidate trial (0: no tuning)")
     ap.add_argument("--tune-rounds", type=int, default=1)
-    ap.add_argument("--kh-tune", type=int, default=2, help="3D workload autotuner: launches per trial (0: none)")
+    ap.add_argument("--kh-tune", type=int, default=4, help="3D workload autotuner: launches per timed group of a trial (0: none)")
     ap.add_argument("--sedov-side", type=int, default=256)
     ap.add_argument("--kh-side", type=int, default=128)
     ap.add_argument("--kh-steps", type=int, default=20)

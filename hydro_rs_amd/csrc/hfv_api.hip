@@ -1519,6 +1519,18 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
       if (c->loaded[s]) u += c->heng[s].cell_updates;
     return u;
   };
+  auto live_cells = [&]() -> long long {
+    long long n = 0;
+    for (int s = 0; s < c->nslots; ++s)
+      if (c->loaded[s] && !c->heng[s].done) n += c->nused[s];
+    return n;
+  };
+  auto finished = [&]() -> bool { return live_cells() == 0; };
+  auto class_done = [&](const Cls& k) -> bool {
+    for (int s : k.slots)
+      if (!c->heng[s].done) return false;
+    return true;
+  };
   const bool prof = c->prof;
   c->prof = false;  // graph replays, as in a run
   int rc = HFV_OK;
@@ -1529,22 +1541,32 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
       if ((k.can_stage || !(cand & 2)) && (k.can_gstage || !(cand & 4))) cands.push_back(cand);
     for (int r = 0; r < rounds && !all_done && rc == HFV_OK; ++r)
       for (int cand : cands) {
+        // a class whose simulations have all finished keeps what it has measured (or the default)
+        if (class_done(k)) break;
         if ((rc = apply(k, cand)) != HFV_OK) break;
+        // one untimed launch: a changed kernel set is captured into a new graph here
         pack_fr_if_dirty(c);
+        if ((rc = launch_step(c)) != HFV_OK) break;
         if ((rc = sync_counters(c)) != HFV_OK) break;
-        const long long u0 = updates();
-        const auto t0 = std::chrono::steady_clock::now();
-        for (int j = 0; j < launches && rc == HFV_OK; ++j) rc = launch_step(c);
+        if ((all_done = finished())) break;
+        // groups of `launches` launches until the sample holds at least `launches` sweeps of the live
+        // cells (at most 4 groups): a run whose steps alternate between many and few active cells
+        // would otherwise be judged on whichever steps its launches happened to hit
+        const long long u0 = updates(), want = (long long)launches * live_cells();
+        double dt = 0.;
+        for (int g = 0; g < 4; ++g) {
+          const auto t0 = std::chrono::steady_clock::now();
+          for (int j = 0; j < launches && rc == HFV_OK; ++j) rc = launch_step(c);
+          if (rc != HFV_OK) break;
+          HIP_OK(hipStreamSynchronize(c->stream));
+          dt += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+          if ((rc = sync_counters(c)) != HFV_OK) break;
+          if (updates() - u0 >= want || finished()) break;
+        }
         if (rc != HFV_OK) break;
-        HIP_OK(hipStreamSynchronize(c->stream));
-        const auto t1 = std::chrono::steady_clock::now();
-        if ((rc = sync_counters(c)) != HFV_OK) break;
-        k.t[cand] += std::chrono::duration<double>(t1 - t0).count();
+        k.t[cand] += dt;
         k.u[cand] += (double)(updates() - u0);
-        all_done = true;
-        for (int s = 0; s < c->nslots; ++s)
-          if (c->loaded[s] && !c->heng[s].done) all_done = false;
-        if (all_done) break;
+        if ((all_done = finished())) break;
       }
     if (rc != HFV_OK) break;
     double best = 0.;

@@ -255,8 +255,10 @@ int hfv_get_geometry_mode(hfv_ctx* ctx, int slot, int* is_static);
  * [cells_lo, cells_hi); buckets 0 / 1024 / 8192 / 65536), hfv_autotune chooses the flux kernel's
  * tiling (its chunks' cells staged through LDS, or each face gathering its cells), the gradient
  * geometry (precomputed per CSR entry, or evaluated per step) and the gradient sweeps' neighbour
- * access (each cell chunk's neighbour lines staged through LDS, or gathered) by timing `rounds` x
- * `launches` real launches of each candidate in turn, and keeps the fastest per cell update.  Every
+ * access (each cell chunk's neighbour lines staged through LDS, or gathered) by timing real launches
+ * of each candidate in turn and keeping the fastest per cell update: per round and candidate, one
+ * untimed launch, then groups of `launches` timed launches until they hold `launches` sweeps of the
+ * live cells or 4 groups have run.  Every
  * candidate gives bit-identical results, so the tuning launches are ordinary steps of the run.  No
  * counterpart in the reference (it has no device kernels to tune).  ns_per_update[c]: measured cost of
  * candidate c = static_grad + 2 staged + 4 gstaged; -1 where not tried.  out may hold max_out classes;
