@@ -1606,34 +1606,40 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
     Gr<double> g;
 #pragma unroll
     for (int q = 0; q < 5; ++q) g.g[q] = mk3(0., 0., 0.);
-    M3 m;
-    m.x = mk3(0., 0., 0.);
-    m.y = mk3(0., 0., 0.);
-    m.z = mk3(0., 0., 0.);
-    if (cfg.dim == 1) {
-      m.y.y = 1.;
-      m.z.z = 1.;
-    } else if (cfg.dim == 2) {
-      m.z.z = 1.;
-    }
+    // the gradient matrix is symmetric bit for bit (w * (ds.a * ds.b) == w * (ds.b * ds.a), summed in the
+    // same order), so its six distinct entries are accumulated
+    double mxx = 0., mxy = 0., mxz = 0., myy = cfg.dim == 1 ? 1. : 0., myz = 0., mzz = cfg.dim <= 2 ? 1. : 0.;
     St<double> lmin = prim, lmax = prim;
-    for (int32_t k = k0; k < k1; ++k) {
-      // the neighbour's pc line: from the block's LDS stage (gst) or gathered
-      uint32_t li = 0xFFFFu;
-      int32_t other;
-      if (gst) {
-        li = F.nloc[k];
-        other = li == 0xFFFFu ? -1 : 0;
-      } else {
-        other = F.nbr[k];
+    // one entry's terms: gradient sums (one component at a time: the scheduler would otherwise form every
+    // product first) and the neighbour extrema
+    auto accumulate = [&](const St<double>& op, const V3<double>& ds, double w) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        g.g[q] = g.g[q] + ((w * (op.c[q] - prim.c[q])) * ds);
+        __builtin_amdgcn_sched_barrier(0);
       }
-      V3<double> ds;
-      double w;
-      St<double> op;
-      if (STATIC) {
+#pragma unroll
+      for (int q = 0; q < 5; ++q) {
+        lmin.c[q] = hmin(lmin.c[q], op.c[q]);
+        lmax.c[q] = hmax(lmax.c[q], op.c[q]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    };
+    if (STATIC) {
+      for (int32_t k = k0; k < k1; ++k) {
+        // the neighbour's pc line: from the block's LDS stage (gst) or gathered
+        uint32_t li = 0xFFFFu;
+        int32_t other;
+        if (gst) {
+          li = F.nloc[k];
+          other = li == 0xFFFFu ? -1 : 0;
+        } else {
+          other = F.nbr[k];
+        }
         const double4 a = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k))[0];
-        w = a.x;
-        ds = mk3(a.y, a.z, a.w);
+        const double w = a.x;
+        const V3<double> ds = mk3(a.y, a.z, a.w);
+        St<double> op;
         if (other >= 0) {
           V3<double> oc;
           if (gst) load_pc((const double*)(gst + 2 * li), 0, op, oc);
@@ -1643,30 +1649,74 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
           if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, ld3(F.nrm, nf, (uint32_t)(F.conn[k] >> 1)));
           else if (cfg.boundary == BND_VACUUM) op = vacuum<double>();
         }
-      } else {
-        const int32_t e = F.conn[k];
-        const uint32_t f = (uint32_t)(e >> 1);
-        const V3<double> sh = decode_shift(F.shc[f], cfg.box);
-        const V3<double> shift = (e & 1) == 0 ? sh : -sh;
-        V3<double> oc;
-        if (other >= 0) {
-          if (gst) load_pc((const double*)(gst + 2 * li), 0, op, oc);
-          else load_pc(C.pc, (uint32_t)other, op, oc);
-        } else {
-          boundary_neighbour(F, cfg, f, cen, prim, oc, op);
-        }
-        ds = (oc + shift) - cen;
-        w = ldo(F.area, f) / len2(ds);
-        m.x = m.x + (w * (ds.x * ds));
-        m.y = m.y + (w * (ds.y * ds));
-        m.z = m.z + (w * (ds.z * ds));
+        accumulate(op, ds, w);
       }
-#pragma unroll
-      for (int q = 0; q < 5; ++q) g.g[q] = g.g[q] + ((w * (op.c[q] - prim.c[q])) * ds);
-#pragma unroll
-      for (int q = 0; q < 5; ++q) {
-        lmin.c[q] = hmin(lmin.c[q], op.c[q]);
-        lmax.c[q] = hmax(lmax.c[q], op.c[q]);
+    } else {
+      // Software-pipelined: the indices of entry k+2 and the loads of entry k+1 are issued before entry k
+      // is accumulated, so that each entry's dependent loads (CSR entry -> face and neighbour -> their
+      // data) overlap with the arithmetic instead of running one after the other.
+      // An entry's neighbour reference: its slot in the block's LDS stage (gst), else its cell; -1 at a
+      // wall.
+      auto ref_of = [&](int32_t k) -> int32_t {
+        if (gst) {
+          const uint32_t li = F.nloc[k];
+          return li == 0xFFFFu ? -1 : (int32_t)li;
+        }
+        return F.nbr[k];
+      };
+      int32_t e_c = 0, r_c = -1, e_n = 0, r_n = -1;  // indices: the entry in flight, the one after it
+      uint8_t s_c = 0;                                // the entry in flight: face shift code, area,
+      double a_c = 0.;                                // neighbour pc line
+      St<double> op_c = prim;
+      V3<double> oc_c = cen;
+      auto fetch = [&]() {
+        const uint32_t f = (uint32_t)(e_c >> 1);
+        s_c = F.shc[f];
+        a_c = ldo(F.area, f);
+        if (r_c >= 0) {
+          if (gst) load_pc((const double*)(gst + 2 * r_c), 0, op_c, oc_c);
+          else load_pc(C.pc, (uint32_t)r_c, op_c, oc_c);
+        }
+      };
+      if (k0 < k1) {
+        e_c = F.conn[k0];
+        r_c = ref_of(k0);
+      }
+      if (k0 + 1 < k1) {
+        e_n = F.conn[k0 + 1];
+        r_n = ref_of(k0 + 1);
+      }
+      if (k0 < k1) fetch();
+      for (int32_t k = k0; k < k1; ++k) {
+        const int32_t e = e_c, other = r_c;
+        const uint8_t code = s_c;
+        const double area = a_c;
+        St<double> op = op_c;
+        V3<double> oc = oc_c;
+        if (k + 1 < k1) {
+          e_c = e_n;
+          r_c = r_n;
+          fetch();
+        }
+        if (k + 2 < k1) {
+          e_n = F.conn[k + 2];
+          r_n = ref_of(k + 2);
+        }
+        const uint32_t f = (uint32_t)(e >> 1);
+        const V3<double> sh = decode_shift(code, cfg.box);
+        const V3<double> shift = (e & 1) == 0 ? sh : -sh;
+        if (other < 0) boundary_neighbour(F, cfg, f, cen, prim, oc, op);
+        const V3<double> ds = (oc + shift) - cen;
+        const double w = area / len2(ds);
+        __builtin_amdgcn_sched_barrier(0);
+        mxx = mxx + w * (ds.x * ds.x);
+        mxy = mxy + w * (ds.x * ds.y);
+        mxz = mxz + w * (ds.x * ds.z);
+        myy = myy + w * (ds.y * ds.y);
+        myz = myz + w * (ds.y * ds.z);
+        mzz = mzz + w * (ds.z * ds.z);
+        __builtin_amdgcn_sched_barrier(0);
+        accumulate(op, ds, w);
       }
     }
     HFV_STAGE();
@@ -1678,6 +1728,10 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
         inv.y = mk3(ldo(C.minv + 3 * n, (uint32_t)i), ldo(C.minv + 4 * n, (uint32_t)i), ldo(C.minv + 5 * n, (uint32_t)i));
         inv.z = mk3(ldo(C.minv + 6 * n, (uint32_t)i), ldo(C.minv + 7 * n, (uint32_t)i), ldo(C.minv + 8 * n, (uint32_t)i));
       } else {
+        M3 m;
+        m.x = mk3(mxx, mxy, mxz);
+        m.y = mk3(mxy, myy, myz);
+        m.z = mk3(mxz, myz, mzz);
         inv = inverse_cols(m);
       }
       bool fin = true;
@@ -1704,23 +1758,44 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
       emin.c[q] = INFINITY;
       emax.c[q] = -INFINITY;
     }
-    for (int32_t k = k0; k < k1; ++k) {
-      V3<double> dxf;
-      if (STATIC) {
-        const double4 b = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k))[1];
-        dxf = mk3(b.x, b.y, b.z);
-      } else {
-        const int32_t e = F.conn[k];
-        const uint32_t f = (uint32_t)(e >> 1);
-        const V3<double> sh = decode_shift(F.shc[f], cfg.box);
-        const V3<double> shift = (e & 1) == 0 ? sh : -sh;
-        dxf = (ld3(F.fcen, nf, f) - cen) - shift;
-      }
+    auto extrema = [&](const V3<double>& dxf) {
       const St<double> ext = gdot(g, dxf);
 #pragma unroll
       for (int q = 0; q < 5; ++q) {
         emin.c[q] = hmin(emin.c[q], ext.c[q]);
         emax.c[q] = hmax(emax.c[q], ext.c[q]);
+      }
+    };
+    if (STATIC) {
+      for (int32_t k = k0; k < k1; ++k) {
+        const double4 b = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k))[1];
+        extrema(mk3(b.x, b.y, b.z));
+      }
+    } else {
+      // pipelined as the estimate sweep: entry k+2's index and entry k+1's face loads ahead of entry k
+      int32_t e_c = 0, e_n = 0;
+      uint8_t s_c = 0;
+      V3<double> fc_c = cen;
+      auto fetch = [&]() {
+        const uint32_t f = (uint32_t)(e_c >> 1);
+        s_c = F.shc[f];
+        fc_c = ld3(F.fcen, nf, f);
+      };
+      if (k0 < k1) e_c = F.conn[k0];
+      if (k0 + 1 < k1) e_n = F.conn[k0 + 1];
+      if (k0 < k1) fetch();
+      for (int32_t k = k0; k < k1; ++k) {
+        const int32_t e = e_c;
+        const uint8_t code = s_c;
+        const V3<double> fc = fc_c;
+        if (k + 1 < k1) {
+          e_c = e_n;
+          fetch();
+        }
+        if (k + 2 < k1) e_n = F.conn[k + 2];
+        const V3<double> sh = decode_shift(code, cfg.box);
+        const V3<double> shift = (e & 1) == 0 ? sh : -sh;
+        extrema((fc - cen) - shift);
       }
     }
 #pragma unroll
@@ -1867,8 +1942,6 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_GRAD_WAVES) k_grad(Batch B, int
     }
   }
   double frv[HFV_FR_W];
-#pragma unroll
-  for (int k = 0; k < HFV_FR_W; ++k) frv[k] = 0.;
   if (a)
     grad_cell<STATIC>(B, cfg, E, i, do_grad, do_ts, pack, frv, B.sims[s].static_geom != 0, staged ? lds4 : nullptr,
                       &my, ev);
