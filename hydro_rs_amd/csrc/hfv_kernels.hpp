@@ -1442,28 +1442,50 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_apply(Batch B, int mode) {
       maor = C.maor[i];
     }
     const int32_t k0 = F.off[i], k1 = F.off[i + 1];
-    for (int32_t e = k0; e < k1; ++e) {
-      const int32_t c = F.conn[e];
-      const uint32_t f = (uint32_t)(c >> 1);
-      if (!F.cur[f]) continue;
-      if (!have) {
-        acc = ld5(C.facc, n, i);
-        have = true;
-      }
-      St<double> rec;
-      double vmx, aor;
+    // a face's record: its five fluxes, v_max and a_over_r
+    auto load_rec = [&](uint32_t f, St<double>& r, double& vmx, double& aor) {
       if (f32) {
         const float4* r4 = (const float4*)((const float*)F.rec + 8 * (uint64_t)f);
         const float4 p = r4[0], q = r4[1];
-        rec = mkst((double)p.x, mk3((double)p.y, (double)p.z, (double)p.w), (double)q.x);
+        r = mkst((double)p.x, mk3((double)p.y, (double)p.z, (double)p.w), (double)q.x);
         vmx = q.y;
         aor = q.z;
       } else {
         const double2* r2 = (const double2*)(F.rec + 8 * (uint64_t)f);
         const double2 r01 = r2[0], r23 = r2[1], r45 = r2[2], r67 = r2[3];
-        rec = mkst(r01.x, mk3(r01.y, r23.x, r23.y), r45.x);
+        r = mkst(r01.x, mk3(r01.y, r23.x, r23.y), r45.x);
         vmx = r45.y;
         aor = r67.x;
+      }
+    };
+    // Three-stage pipeline over the cell's CSR entries: entry e+3's CSR value, entry e+2's face flag
+    // and entry e+1's record (when its face is current) are loaded while entry e is applied, so the
+    // per-entry chain of dependent loads overlaps with the work.  Entries are applied in CSR order.
+    int32_t c0 = 0, c1 = 0, c2 = 0;  // CSR values of entries e, e+1, e+2
+    uint8_t u0 = 0, u1 = 0;          // face flags of entries e, e+1
+    St<double> r0 = vacuum<double>();  // record of entry e
+    double v0 = 0., o0 = 0.;
+    if (k0 < k1) c0 = F.conn[k0];
+    if (k0 + 1 < k1) c1 = F.conn[k0 + 1];
+    if (k0 + 2 < k1) c2 = F.conn[k0 + 2];
+    if (k0 < k1) u0 = F.cur[(uint32_t)(c0 >> 1)];
+    if (k0 + 1 < k1) u1 = F.cur[(uint32_t)(c1 >> 1)];
+    if (k0 < k1 && u0) load_rec((uint32_t)(c0 >> 1), r0, v0, o0);
+    for (int32_t e = k0; e < k1; ++e) {
+      const int32_t c = c0;
+      const uint8_t cu = u0;
+      const St<double> rec = r0;
+      const double vmx = v0, aor = o0;
+      c0 = c1;
+      u0 = u1;
+      if (e + 1 < k1 && u0) load_rec((uint32_t)(c0 >> 1), r0, v0, o0);
+      c1 = c2;
+      if (e + 2 < k1) u1 = F.cur[(uint32_t)(c1 >> 1)];
+      if (e + 3 < k1) c2 = F.conn[e + 3];
+      if (!cu) continue;
+      if (!have) {
+        acc = ld5(C.facc, n, i);
+        have = true;
       }
       if ((c & 1) == 0)
         acc = acc - rec;
