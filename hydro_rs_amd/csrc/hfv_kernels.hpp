@@ -1648,17 +1648,25 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
       __builtin_amdgcn_sched_barrier(0);
     };
     if (STATIC) {
-      for (int32_t k = k0; k < k1; ++k) {
-        // the neighbour's pc line: from the block's LDS stage (gst) or gathered
-        uint32_t li = 0xFFFFu;
-        int32_t other;
+      // the entry's neighbour reference (LDS stage slot when staged, else its cell; -1 at a wall) and
+      // geometry line, loaded one entry ahead
+      int32_t r_n = -1;
+      double4 a_n = make_double4(0., 0., 0., 0.);
+      auto fetch = [&](int32_t k) {
         if (gst) {
-          li = F.nloc[k];
-          other = li == 0xFFFFu ? -1 : 0;
+          const uint32_t li = F.nloc[k];
+          r_n = li == 0xFFFFu ? -1 : (int32_t)li;
         } else {
-          other = F.nbr[k];
+          r_n = F.nbr[k];
         }
-        const double4 a = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k))[0];
+        a_n = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k))[0];
+      };
+      if (k0 < k1) fetch(k0);
+      for (int32_t k = k0; k < k1; ++k) {
+        const int32_t other = r_n;
+        const uint32_t li = (uint32_t)other;
+        const double4 a = a_n;
+        if (k + 1 < k1) fetch(k + 1);
         const double w = a.x;
         const V3<double> ds = mk3(a.y, a.z, a.w);
         St<double> op;
@@ -1789,8 +1797,11 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
       }
     };
     if (STATIC) {
+      double4 b_n = make_double4(0., 0., 0., 0.);
+      if (k0 < k1) b_n = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k0))[1];
       for (int32_t k = k0; k < k1; ++k) {
-        const double4 b = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k))[1];
+        const double4 b = b_n;
+        if (k + 1 < k1) b_n = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)(k + 1)))[1];
         extrema(mk3(b.x, b.y, b.z));
       }
     } else {
@@ -2075,11 +2086,29 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_wakeup(Batch B) {
   if (!valid) return;
   int8_t w = C.bin[i];
   const int32_t k0 = F.off[i], k1 = F.off[i + 1];
+  // pipelined: entry e+2's neighbour and entry e+1's neighbour flag and bin are loaded while entry e is
+  // taken (the bin is read whether or not the neighbour is active: same result, no dependent load)
+  int32_t o0 = -1, o1 = -1;  // neighbours of entries e, e+1
+  uint8_t a0 = 0;            // active flag and bin of entry e's neighbour
+  int8_t b0 = 0;
+  if (k0 < k1) o0 = F.nbr[k0];
+  if (k0 + 1 < k1) o1 = F.nbr[k0 + 1];
+  if (o0 >= 0) {
+    a0 = C.act[o0];
+    b0 = C.bin[o0];
+  }
   for (int32_t e = k0; e < k1; ++e) {
-    const int32_t other = F.nbr[e];
-    if (other < 0) continue;
-    if (!C.act[other]) continue;
-    int8_t cand = (int8_t)(C.bin[other] + 2);
+    const int32_t other = o0;
+    const uint8_t act = a0;
+    const int8_t bin = b0;
+    o0 = o1;
+    if (e + 1 < k1 && o0 >= 0) {
+      a0 = C.act[o0];
+      b0 = C.bin[o0];
+    }
+    if (e + 2 < k1) o1 = F.nbr[e + 2];
+    if (other < 0 || !act) continue;
+    int8_t cand = (int8_t)(bin + 2);
     if (cand < w) w = cand;
   }
   C.wake[i] = w;
