@@ -54,15 +54,18 @@ CASES = {
     "sedov2d": lambda: ics.sedov_2d(side=32, t_end=0.05),
     "kh3d": lambda: ics.kelvin_helmholtz_3d(side=10, t_end=0.5),
     "tube2d_reflective_exact": lambda: _tube2d(),
+    # the other wall kinds (flux.rs:148-169, space.rs:213-229), and walls on a 3D fixed mesh (precomputed
+    # gradient geometry with LDS-staged neighbours)
+    "tube2d_open_hllc": lambda: _tube2d(fvs.OPEN, fvs.HLLC),
+    "tube2d_vacuum_hllc": lambda: _tube2d(fvs.VACUUM, fvs.HLLC),
+    "blast3d_reflective": lambda: _blast3d(),
     # pressure ratio 100 at the interface: AIRS hands those faces to the exact-solver fallback pass
     "tube1d_airs": lambda: ics.riemann_tube_1d(300, 1.0, 10.0, 0.0, 0.125, 0.1, 0.0, 1.4, solver=fvs.AIRS,
                                                t_end=0.05),
 }
 
 
-def _tube2d():
-    from hydro_rs_amd.fvs import EXACT, REFLECTIVE
-
+def _tube2d(boundary=fvs.REFLECTIVE, solver=fvs.EXACT):
     rng = np.random.default_rng(11)
     loc = ics.jittered_grid(2, 24, (1.0, 1.0, 1.0), 0.05, rng)
     left = loc[:, 0] < 0.5
@@ -70,9 +73,23 @@ def _tube2d():
     P = np.where(left, 1.0, 0.2)
     vel = np.zeros_like(loc)
     vel[:, 1] = 0.1
-    cfg = fvs.Config(dim=2, box=(1.0, 1.0, 1.0), gamma=1.4, cfl=0.3, solver=EXACT, boundary=REFLECTIVE,
+    cfg = fvs.Config(dim=2, box=(1.0, 1.0, 1.0), gamma=1.4, cfl=0.3, solver=solver, boundary=boundary,
                      motion=fvs.FIXED, t_end=0.1)
     return ics._finish("tube2d", cfg, loc, rho, vel, P)
+
+
+def _blast3d(boundary=fvs.REFLECTIVE, solver=fvs.HLLC):
+    """8^3 jittered cells in the unit box with walls: an overpressured sphere drifting along x."""
+    rng = np.random.default_rng(12)
+    loc = ics.jittered_grid(3, 8, (1.0, 1.0, 1.0), 0.1, rng)
+    r = np.linalg.norm(loc - 0.5, axis=1)
+    rho = np.ones(len(loc))
+    P = np.where(r < 0.25, 10.0, 0.1)
+    vel = np.zeros_like(loc)
+    vel[:, 0] = 0.2
+    cfg = fvs.Config(dim=3, box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, cfl=0.3, solver=solver, boundary=boundary,
+                     motion=fvs.FIXED, t_end=0.05)
+    return ics._finish("blast3d", cfg, loc, rho, vel, P)
 
 
 def _synced(prob, k):
@@ -132,7 +149,8 @@ def test_step_parity(ora, case, k):
     assert st["error_code"] == 0
 
 
-@pytest.mark.parametrize("case,steps", [("sod1d", 1 << 40), ("sedov2d", 400), ("tube2d_reflective_exact", 1 << 40)])
+@pytest.mark.parametrize("case,steps", [("sod1d", 1 << 40), ("sedov2d", 400), ("tube2d_reflective_exact", 1 << 40),
+                                        ("tube2d_vacuum_hllc", 1 << 40), ("blast3d_reflective", 1 << 40)])
 def test_full_run_parity(ora, case, steps):
     say(f"[run] case={case}")
     prob = CASES[case]()
