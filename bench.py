@@ -38,14 +38,17 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-updates/s vs plain FP64 GPU path; p50/p99 job latency at L1 err<1e-5"
 
-# Algorithmic HBM bytes per unit of work (device state is FP64 in both precisions; the FP32 path
-# converts in registers), one read of every operand a kernel needs and one write of every output:
+# Algorithmic HBM bytes per unit of work, one read of every operand a kernel needs and one write of
+# every output, at the widths the context stores (FP64 contexts: all FP64):
 #   flux: per active cell 329 B (prim 40, grad 120, gc 24, extrap 40, centroid 24, loc 24, v 24, v_rel 24,
 #         dt 8, flag 1) + per computed face 121 B (area 8, normal 24, centroid 24, left/right 8, shift 1,
 #         flux record 56)
 #   apply_c2p: per cell 89 B (flag 1, accumulator r/w 80, CSR offset 8) + per active cell 208 B + per
 #         face 120 B (record 56 x 2 sides, CSR entries 4 x 2)
-#   grad_timestep: per active cell 600 B + per face of an active cell 120 B
+#   grad_timestep: per active cell 600 B (of which the gradient write 120) + per face of an active cell 120 B
+#   mask_predict: per cell 193 B (of which 9 gradient components 72 and the extrapolation r/w 80)
+# FP32-flux contexts store gradients, extrapolations and face records as floats: the same terms at
+# 4-byte widths (grad 60, extrap 20, record 28).
 ALGO = {
     "flux": lambda n_act, f_act, n_all: 329 * n_act + 121 * f_act,
     "apply_c2p": lambda n_act, f_act, n_all: 89 * n_all + 208 * n_act + 120 * f_act,
@@ -54,22 +57,30 @@ ALGO = {
     "wakeup": lambda n_act, f_act, n_all: 18 * n_all + 8 * f_act,
     "limit_kick": lambda n_act, f_act, n_all: 19 * n_all + 80 * n_act,
 }
+ALGO_FP32 = {
+    "flux": lambda n_act, f_act, n_all: 249 * n_act + 93 * f_act,
+    "apply_c2p": lambda n_act, f_act, n_all: 89 * n_all + 208 * n_act + 64 * f_act,
+    "grad_timestep": lambda n_act, f_act, n_all: 540 * n_act + 120 * f_act,
+    "mask_predict": lambda n_act, f_act, n_all: 117 * n_all,
+    "wakeup": ALGO["wakeup"],
+    "limit_kick": ALGO["limit_kick"],
+}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def roofline(kt, launches_timed, cell_updates, face_fluxes, n_all):
-    """Dominant kernel (most device time) of a timed region; bytes from ALGO at the region's mean
-    active counts per launch."""
+def roofline(kt, launches_timed, cell_updates, face_fluxes, n_all, fp32=False):
+    """Dominant kernel (most device time) of a timed region; bytes from ALGO (ALGO_FP32 for FP32-flux
+    contexts) at the region's mean active counts per launch."""
     name, rec = max(((k, v) for k, v in kt.items() if v["launches"] > 0), key=lambda kv: kv[1]["ms"])
     launches = rec["launches"]
     avg_s = rec["ms"] / 1e3 / launches
     steps = max(launches_timed, 1)
     n_act = cell_updates / steps
     f_act = face_fluxes / steps
-    algo = ALGO.get(name, lambda *_: 0)(n_act, f_act, n_all)
+    algo = (ALGO_FP32 if fp32 else ALGO).get(name, lambda *_: 0)(n_act, f_act, n_all)
     achieved = algo / avg_s / 1e9
     return {
         "kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -97,15 +108,18 @@ def _counter_rows(d):
     return rows
 
 
-def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, label, command, copy_mib):
+def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, label, command, copy_mib, window=0):
     """profiles/<label>_pmc.md: HBM bytes per dispatch of every kernel from a FETCH_SIZE run and a
     WRITE_SIZE run (KiB per dispatch), each scaled by what the counter reads for a plain device copy of
-    copy_mib MiB (true bytes / counted bytes), plus a TRAFFIC_JSON line that bench.py reads back."""
-    def per_kernel(d, counter):
+    copy_mib MiB (true bytes / counted bytes), plus a TRAFFIC_JSON line that bench.py reads back.  With
+    a window, each kernel's figures are the mean over its last `window` dispatches (the timed window of
+    a run that ends with it)."""
+    def per_kernel(d, counter, win=0):
+        rows = [r for r in _counter_rows(d) if r["Counter_Name"] == counter]
+        if win > 0:
+            return _last_dispatches(rows, win, lambda r: float(r["Counter_Value"]) * 1024.0)
         acc = {}
-        for r in _counter_rows(d):
-            if r["Counter_Name"] != counter:
-                continue
+        for r in rows:
             acc.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]) * 1024.0)
         return acc
 
@@ -115,8 +129,8 @@ def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, label, c
     big = lambda acc: max(v for vals in acc.values() for v in vals)  # the copy dispatch of copy_mib MiB
     f_scale = true_bytes / big(cf)
     w_scale = true_bytes / big(cw)
-    fetch = per_kernel(fetch_dir, "FETCH_SIZE")
-    write = per_kernel(write_dir, "WRITE_SIZE")
+    fetch = per_kernel(fetch_dir, "FETCH_SIZE", window)
+    write = per_kernel(write_dir, "WRITE_SIZE", window)
     table = {}
     for k in fetch:
         if k not in write or not k.startswith(("void hfv::", "hfv::")):
@@ -127,8 +141,10 @@ def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, label, c
     lines = [f"# HBM traffic per dispatch: {label}", "", "Commands (from /tmp on the GPU box, R = repository root):", ""]
     lines += [f"    {c}" for c in command]
     lines += ["", f"Calibration: a {copy_mib} MiB device copy reads FETCH_SIZE x {f_scale:.3f} and WRITE_SIZE x "
-                  f"{w_scale:.3f} of its true bytes; every kernel's counters below are scaled by these factors.", "",
-              "| kernel | dispatches | HBM read MB | HBM write MB | total MB |", "|---|---|---|---|---|"]
+                  f"{w_scale:.3f} of its true bytes; every kernel's counters below are scaled by these factors."]
+    if window > 0:
+        lines += ["", f"Per kernel: the mean over its last {window} dispatches (the run's final timed window)."]
+    lines += ["", "| kernel | dispatches | HBM read MB | HBM write MB | total MB |", "|---|---|---|---|---|"]
     for k, v in sorted(table.items(), key=lambda kv: -kv[1]["traffic_bytes"]):
         lines.append(f"| `{k}` | {v['dispatches']} | {v['fetch_bytes'] / 1e6:.1f} | {v['write_bytes'] / 1e6:.1f} | "
                      f"{v['traffic_bytes'] / 1e6:.1f} |")
@@ -138,18 +154,39 @@ def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, label, c
     return out
 
 
-def summarize_trace(stats_csv, label, command, note=""):
-    """profiles/<label>_kernel_stats.md from rocprofv3 --kernel-trace --stats (kernel_stats.csv)."""
+def _last_dispatches(rows, window, key):
+    """Per kernel, the `key` values of its last `window` dispatches (by dispatch id): in a run that ends
+    with its timed window, the dispatches of that window."""
+    per = {}
+    for r in sorted(rows, key=lambda r: int(r["Dispatch_Id"])):
+        per.setdefault(r["Kernel_Name"], []).append(key(r))
+    return {k: v[-window:] for k, v in per.items()}
+
+
+def summarize_trace(stats_csv, label, command, note="", trace_csv=None, window=0):
+    """profiles/<label>_kernel_stats.md from rocprofv3 --kernel-trace --stats (kernel_stats.csv); with
+    the per-dispatch trace and a window, also each kernel's mean duration over its last `window`
+    dispatches (the timed window of a run that ends with it)."""
     import csv
 
     rows = list(csv.DictReader(open(stats_csv)))
+    win = {}
+    if trace_csv and window > 0:
+        win = _last_dispatches(list(csv.DictReader(open(trace_csv))), window,
+                               lambda r: int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
+    wcol = f" last {window} avg us |" if win else ""
     lines = [f"# Kernel stats: {label}", "", "Command (from /tmp on the GPU box, R = repository root):", "",
-             f"    {command}", "", "| kernel | calls | total us | avg us | min us | max us | % |",
-             "|---|---|---|---|---|---|---|"]
+             f"    {command}", "", "| kernel | calls | total us | avg us | min us | max us | % |" + wcol,
+             "|---|---|---|---|---|---|---|" + ("---|" if win else "")]
     for r in rows:
+        w = win.get(r["Name"])
+        wv = (f" {np.mean(w) / 1e3:.2f} |" if w else " |") if win else ""
         lines.append(f"| `{r['Name']}` | {r['Calls']} | {float(r['TotalDurationNs']) / 1e3:.1f} | "
                      f"{float(r['AverageNs']) / 1e3:.2f} | {float(r['MinNs']) / 1e3:.2f} | {float(r['MaxNs']) / 1e3:.2f} | "
-                     f"{float(r['Percentage']):.2f} |")
+                     f"{float(r['Percentage']):.2f} |" + wv)
+    if win:
+        lines += ["", f"`last {window} avg us`: mean duration of each kernel's last {window} dispatches, i.e. over "
+                      "the run's final timed window (the launches bench.py times with HIP events)."]
     if note:
         lines += ["", note]
     out = os.path.join(PROFILES, f"{label}_kernel_stats.md")
@@ -197,6 +234,23 @@ def attach_traffic(rl, label, fp32, solver, tuning):
     rl["traffic"] = sum(vals) if vals and all(v is not None for v in vals) else None
 
 
+# ------------------------------------------------------------------ tuning taken from an earlier run
+def load_report(path):
+    """The JSON line of an earlier bench.py run (the file's last non-empty line)."""
+    lines = [ln for ln in open(path).read().splitlines() if ln.strip()]
+    return json.loads(lines[-1])
+
+
+def apply_tuning(sim, members, classes):
+    """Set each slot's tuning to the choice the autotuner made for its size class (dimension and cell
+    count in [lo, hi)); members: (slot, dim, cells)."""
+    for slot, dim, n in members:
+        for c in classes or []:
+            if c["dim"] == dim and c["cells"][0] <= n < c["cells"][1]:
+                sim.set_tuning(slot, c["staged"], c["static_grad"], c["gstaged"])
+                break
+
+
 # ------------------------------------------------------------------ ensemble (headline)
 def bench_ensemble(args):
     from hydro_rs_amd import ensemble, fvs, ics
@@ -209,14 +263,24 @@ def bench_ensemble(args):
     # autotuner (per size class: flux tiling, gradient geometry); its launches are ordinary steps
     tune_launches = 0
     tuning = None
-    if args.tune_launches > 0:
+    warmup = args.warmup
+    if args.tuning_from:
+        # the earlier run's choices, and as many extra warmup launches as its tuner ran: every candidate
+        # gives the same bits, so the timed window holds the same steps as that run's
+        rep = load_report(args.tuning_from)["autotune"]
+        apply_tuning(b, [(s, p.cfg.dim, p.n) for s, p in enumerate(probs)], rep["classes"])
+        warmup += rep["launches"]
+        tune_launches = rep["launches"]
+        tuning = rep["classes"]
+        log(f"[ensemble] tuning from {args.tuning_from}; {warmup} warmup launches")
+    elif args.tune_launches > 0:
         s0 = [b.engine_info(s)["step_count"] for s in range(len(probs))]
         t = time.perf_counter()
         tuning = b.autotune(launches=args.tune_launches, rounds=args.tune_rounds)
         tune_launches = max(b.engine_info(s)["step_count"] - s0[s] for s in range(len(probs)))
         log(f"[ensemble] autotune: {tune_launches} launches in {time.perf_counter() - t:.1f}s: "
             f"{[(c['dim'], c['cells'], c['staged'], c['static_grad']) for c in tuning]}")
-    b.run(args.warmup)
+    b.run(warmup)
     b.synchronize()
     i0 = [b.engine_info(s) for s in range(len(probs))]
     b.profile(True)
@@ -234,8 +298,12 @@ def bench_ensemble(args):
            "autotune": {"launches": tune_launches, "classes": tuning},
            "n_all": n_all, "members_done": sum(1 for i in i1 if i["done"]),
            "members_failed": [s for s, i in enumerate(i1) if i["error_code"]],
-           "roofline": roofline(kt, launches, cu, ff, n_all)}
+           "roofline": roofline(kt, launches, cu, ff, n_all, fp32=True)}
     attach_traffic(res["roofline"], "ensemble512", True, fvs.HLLC, tuning)
+    if args.no_gate:  # a profiling run: the timed window only
+        res.update({"fp64_batched_value": None, "l1_gate": None, "plain_fp64": None, "speedup_vs_plain_fp64": None})
+        b.close()
+        return res, probs
     # the same window in FP64 (batched) for the L1 gate
     p64 = [ics.with_precision(p, fvs.FP64) for p in probs]
     b64 = fvs.Batch.for_problems(p64)
@@ -307,11 +375,20 @@ def cpu_baseline_ensemble(probs, stride, warmup, budget_s):
 
 
 # ------------------------------------------------------------------ single simulations
-def run_device(prob, warmup, steps, profile=False, tune=0):
+def run_device(prob, warmup, steps, profile=False, tune=0, tuning_from=None):
+    """tuning_from: (classes, launches) of an earlier run's autotuner, applied instead of tuning, with
+    its launches added to the warmup."""
     from hydro_rs_amd import ics
 
     dev = ics.build_device_sim(prob)
-    tuning = dev.autotune(launches=tune, rounds=1) if tune > 0 else None
+    s0 = dev.engine_info()["step_count"]
+    tuning = None
+    if tuning_from:
+        apply_tuning(dev, [(0, prob.cfg.dim, prob.n)], tuning_from[0])
+        tuning, warmup = tuning_from[0], warmup + tuning_from[1]
+    elif tune > 0:
+        tuning = dev.autotune(launches=tune, rounds=1)
+    tune_launches = tuning_from[1] if tuning_from else dev.engine_info()["step_count"] - s0
     dev.run(warmup)
     dev.synchronize()
     e0 = dev.engine_info()
@@ -326,7 +403,7 @@ def run_device(prob, warmup, steps, profile=False, tune=0):
     return dev, {
         "steps": done, "seconds": t1 - t0, "cell_updates": e1["cell_updates"] - e0["cell_updates"],
         "face_fluxes": e1["face_fluxes"] - e0["face_fluxes"], "kernel_times": kt, "done": e1["done"],
-        "ti_current": e1["ti_current"], "autotune": tuning,
+        "ti_current": e1["ti_current"], "autotune": tuning, "tune_launches": tune_launches,
     }
 
 
@@ -360,19 +437,27 @@ def bench_kh3d(args):
     out = {"workload": f"kh3d_{args.kh_side}^3_periodic_airs", "cells": prob.n, "faces": prob.mesh.n_faces,
            "warmup": args.kh_warmup, "steps": args.kh_steps}
     devs = {}
+    rep = load_report(args.tuning_from).get("kh3d") if args.tuning_from else None
     for k, prec in (("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)):
+        tf = (rep[k]["autotune"], rep[k]["tune_launches"]) if rep else None
         d, s = run_device(ics.with_precision(prob, prec), args.kh_warmup, args.kh_steps, profile=True,
-                          tune=args.kh_tune)
-        rl = roofline(s["kernel_times"], s["steps"], s["cell_updates"], s["face_fluxes"], prob.n)
+                          tune=args.kh_tune, tuning_from=tf)
+        rl = roofline(s["kernel_times"], s["steps"], s["cell_updates"], s["face_fluxes"], prob.n,
+                      fp32=prec == fvs.FP32_FLUX)
         attach_traffic(rl, f"kh3d_{args.kh_side}", prec == fvs.FP32_FLUX, fvs.AIRS, s["autotune"])
         n_act = s["cell_updates"] / max(s["steps"], 1)
         f_act = s["face_fluxes"] / max(s["steps"], 1)
-        step_bytes = 192 * prob.n + 1247 * n_act + 256 * f_act
+        # B_step at the context's storage widths (FP32-flux: gradients, extrapolations, records as floats)
+        if prec == fvs.FP32_FLUX:
+            step_bytes = 116 * prob.n + 1107 * n_act + 200 * f_act
+        else:
+            step_bytes = 192 * prob.n + 1247 * n_act + 256 * f_act
         out[k] = {"cell_updates_per_s": s["cell_updates"] / s["seconds"],
                   "face_fluxes_per_s": s["face_fluxes"] / s["seconds"],
                   "ms_per_step": 1e3 * s["seconds"] / max(s["steps"], 1),
                   "step_roofline_frac": step_bytes / (s["seconds"] / max(s["steps"], 1)) / (HBM_PEAK_GBS * 1e9),
-                  "roofline": rl, "ti_current": s["ti_current"], "autotune": s["autotune"]}
+                  "roofline": rl, "ti_current": s["ti_current"], "autotune": s["autotune"],
+                  "tune_launches": s["tune_launches"]}
         devs[k] = d
         log(f"[kh3d] {k}: {out[k]['ms_per_step']:.2f} ms/step, {out[k]['cell_updates_per_s']:.3e} cell-updates/s, "
             f"dominant {rl['kernel']} {rl['achieved']} GB/s")
@@ -447,13 +532,22 @@ def main():
                     help="write profiles/LABEL_kernel_stats.md from rocprofv3 --stats output (no GPU needed)")
     ap.add_argument("--command", action="append", default=[], help="command line(s) recorded in a summary")
     ap.add_argument("--note", default="")
+    ap.add_argument("--trace-csv", default=None, help="per-dispatch kernel_trace.csv for --summarize-trace")
+    ap.add_argument("--window", type=int, default=0,
+                    help="summaries: also/only each kernel's last N dispatches (the run's timed window)")
+    ap.add_argument("--tuning-from", default=None,
+                    help="apply the autotuner choices of an earlier run's JSON line (and run its tuning launches "
+                         "as warmup) instead of tuning: same timed window, no tuning phase (profiling runs)")
+    ap.add_argument("--no-gate", action="store_true",
+                    help="ensemble: time the window only (no FP64 gate batch, no plain path); for profiling")
     args = ap.parse_args()
     if args.summarize_pmc:
         fd, wd, cfd, cwd, label = args.summarize_pmc
-        print(summarize_pmc(fd, wd, cfd, cwd, label, args.command, args.copy_probe or 1024.0))
+        print(summarize_pmc(fd, wd, cfd, cwd, label, args.command, args.copy_probe or 1024.0, args.window))
         return
     if args.summarize_trace:
-        print(summarize_trace(args.summarize_trace[0], args.summarize_trace[1], " ".join(args.command), args.note))
+        print(summarize_trace(args.summarize_trace[0], args.summarize_trace[1], " ".join(args.command), args.note,
+                              args.trace_csv, args.window))
         return
     if args.copy_probe > 0:
         import torch
