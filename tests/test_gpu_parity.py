@@ -51,6 +51,9 @@ def mesh_from_oracle(sim, fields, dim, box, periodic):
 
 CASES = {
     "sod1d": lambda: ics.sod_1d(n=200, t_end=0.2),
+    # the other particle-motion modes of set_particle_velocity (part.rs:328-372), on a moving 1D mesh
+    "sod1d_steer": lambda: ics.sod_1d(n=200, t_end=0.2, motion=fvs.STEER),
+    "sod1d_steer_pakmor": lambda: ics.sod_1d(n=200, t_end=0.2, motion=fvs.STEER_PAKMOR),
     "sedov2d": lambda: ics.sedov_2d(side=32, t_end=0.05),
     "kh3d": lambda: ics.kelvin_helmholtz_3d(side=10, t_end=0.5),
     "tube2d_reflective_exact": lambda: _tube2d(),
@@ -149,7 +152,8 @@ def test_step_parity(ora, case, k):
     assert st["error_code"] == 0
 
 
-@pytest.mark.parametrize("case,steps", [("sod1d", 1 << 40), ("sedov2d", 400), ("tube2d_reflective_exact", 1 << 40),
+@pytest.mark.parametrize("case,steps", [("sod1d", 1 << 40), ("sod1d_steer", 1 << 40), ("sod1d_steer_pakmor", 1 << 40),
+                                        ("sedov2d", 400), ("tube2d_reflective_exact", 1 << 40),
                                         ("tube2d_vacuum_hllc", 1 << 40), ("blast3d_reflective", 1 << 40)])
 def test_full_run_parity(ora, case, steps):
     say(f"[run] case={case}")
