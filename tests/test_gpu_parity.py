@@ -54,6 +54,15 @@ CASES = {
     # the other particle-motion modes of set_particle_velocity (part.rs:328-372), on a moving 1D mesh
     "sod1d_steer": lambda: ics.sod_1d(n=200, t_end=0.2, motion=fvs.STEER),
     "sod1d_steer_pakmor": lambda: ics.sod_1d(n=200, t_end=0.2, motion=fvs.STEER_PAKMOR),
+    # engine and runner options: GodunovHydroRunner (godunov.rs:14-58, no gradient phases), timestep_sync
+    # instead of the limiter (hydrodynamics.rs:186-191), snapshot sync points splitting the drifts
+    # (engine.rs), and the isothermal equation of state (gas_law.rs)
+    "sod1d_godunov": lambda: _with(ics.sod_1d(n=200, t_end=0.2), runner=fvs.GODUNOV),
+    "sod1d_sync_all": lambda: _with(ics.sod_1d(n=200, t_end=0.2), sync_all=True),
+    "sod1d_dt_snap": lambda: _with(ics.sod_1d(n=200, t_end=0.2), dt_snap=0.013),
+    "sedov2d_sync_all": lambda: _with(ics.sedov_2d(side=32, t_end=0.05), sync_all=True),
+    "tube1d_isothermal": lambda: _with(ics.riemann_tube_1d(300, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, t_end=0.05),
+                                       eos=1, isothermal_u=1.5),
     "sedov2d": lambda: ics.sedov_2d(side=32, t_end=0.05),
     "kh3d": lambda: ics.kelvin_helmholtz_3d(side=10, t_end=0.5),
     "tube2d_reflective_exact": lambda: _tube2d(),
@@ -66,6 +75,13 @@ CASES = {
     "tube1d_airs": lambda: ics.riemann_tube_1d(300, 1.0, 10.0, 0.0, 0.125, 0.1, 0.0, 1.4, solver=fvs.AIRS,
                                                t_end=0.05),
 }
+
+
+def _with(prob, **cfg):
+    """The same problem with some configuration fields replaced."""
+    from dataclasses import replace
+
+    return replace(prob, cfg=replace(prob.cfg, **cfg))
 
 
 def _tube2d(boundary=fvs.REFLECTIVE, solver=fvs.EXACT):
@@ -153,6 +169,9 @@ def test_step_parity(ora, case, k):
 
 
 @pytest.mark.parametrize("case,steps", [("sod1d", 1 << 40), ("sod1d_steer", 1 << 40), ("sod1d_steer_pakmor", 1 << 40),
+                                        ("sod1d_godunov", 1 << 40), ("sod1d_sync_all", 1 << 40),
+                                        ("sod1d_dt_snap", 1 << 40), ("sedov2d_sync_all", 400),
+                                        ("tube1d_isothermal", 1 << 40),
                                         ("sedov2d", 400), ("tube2d_reflective_exact", 1 << 40),
                                         ("tube2d_vacuum_hllc", 1 << 40), ("blast3d_reflective", 1 << 40)])
 def test_full_run_parity(ora, case, steps):
