@@ -267,6 +267,35 @@ def test_fp32_flux_l1_density():
     assert l1 < 1e-5
 
 
+FP32_CASES = {
+    **{f"tube1d_{k}": (lambda k=k: ics.riemann_tube_1d(300, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, t_end=0.05,
+                                                      solver=fvs.SOLVERS[k]))
+       for k in ("Exact", "HLLC", "PVRS", "AIRS", "TSRS", "TRRS")},
+    "tube1d_LinearAdvection": lambda: _with(ics.riemann_tube_1d(300, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, t_end=0.05),
+                                            solver=fvs.LINEAR_ADVECTION, linadv_velocity=(0.7, 0.0, 0.0)),
+    "kh3d_airs": lambda: ics.kelvin_helmholtz_3d(side=10, t_end=0.3),
+    "tube2d_reflective_hllc": lambda: _tube2d(fvs.REFLECTIVE, fvs.HLLC),
+}
+
+
+@pytest.mark.parametrize("case", list(FP32_CASES))
+def test_fp32_flux_l1_per_solver(case):
+    """Every solver's FP32 instantiation (reconstruction and solve in FP32, FP64 accumulation), run to the
+    end time next to the FP64 path: L1(rho) < 1e-5 at the same time."""
+    say(f"[fp32] case={case}")
+    p64 = FP32_CASES[case]()
+    p32 = _with(p64, precision=fvs.FP32_FLUX)
+    d64, d32 = ics.build_device_sim(p64), ics.build_device_sim(p32)
+    s64, s32 = d64.run(1 << 40), d32.run(1 << 40)
+    e64, e32 = d64.engine_info(), d32.engine_info()
+    assert e64["done"] and e32["done"] and e64["error_code"] == 0 and e32["error_code"] == 0
+    r64 = d64.get_state(["prim"])["prim"][: p64.n]
+    r32 = d32.get_state(["prim"])["prim"][: p64.n]
+    l1 = float(np.mean(np.abs(r32 - r64)))
+    say(f"  steps {s64}/{s32}, L1(rho) = {l1:.3e}")
+    assert l1 < 1e-5
+
+
 def test_trait_phases_equal_fused_step():
     """predict/compute_fluxes/apply/c2p/gradients/timesteps through the trait entry points produce the
     fused hfv_step result bit for bit."""
