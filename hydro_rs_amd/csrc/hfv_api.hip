@@ -141,6 +141,9 @@ struct hfv_ctx {
   // table (position b of the grid holds chunk perm(b)), its host copy and the per-chunk partials
   std::vector<int64_t> cchunk0, fchunk0;
   int64_t ncchunk = 0, nfchunk = 0, ncpos = 0, nfpos = 0;
+  // launched grid sizes: positions holding the chunks of live slots (loaded, and in a batch not finished),
+  // rounded up to 8; the block tables are rebuilt without finished slots as they finish
+  int64_t ncg = 8, nfg = 8, live_chunks = 0;
   std::vector<BlockJob> hcjobs, hfjobs;
   BlockJob *dcjobs = nullptr, *dfjobs = nullptr;
   Partials P{};
@@ -225,14 +228,28 @@ void drop_graph(hfv_ctx* c) {
   c->graph_ok = false;
 }
 
-// Rebuild the block tables from the loaded slots (their used cells and faces) and upload them with
-// the simulation descriptors.  Chunk k sits at grid position (k % Q) * 8 + k / Q, Q = positions / 8:
-// the XCD that runs position b (b % 8) gets the contiguous chunks [xcd * Q, (xcd + 1) * Q).
+// Rebuild the block tables from the live slots (loaded, and in a batch not finished: a finished slot's
+// blocks would only exit) and upload them with the simulation descriptors.  The j-th live chunk sits at
+// grid position (j % Q) * 8 + j / Q, Q = launched positions / 8: the XCD that runs position b (b % 8)
+// gets the contiguous live chunks [xcd * Q, (xcd + 1) * Q).  Chunk ids (partials) stay global.
 int write_jobs(hfv_ctx* c) {
   const int S = c->nslots;
+  auto live = [&](int s) { return c->loaded[s] && !(c->batch && c->heng[s].done); };
+  int64_t lc = 0, lf = 0;
+  for (int s = 0; s < S; ++s) {
+    if (!live(s)) continue;
+    lc += (c->nused[s] + BLOCK - 1) / BLOCK;
+    lf += (c->fused[s] + BLOCK - 1) / BLOCK;
+  }
+  const int64_t ncg = std::max<int64_t>(8, (lc + 7) / 8 * 8), nfg = std::max<int64_t>(8, (lf + 7) / 8 * 8);
+  if (ncg != c->ncg || nfg != c->nfg) drop_graph(c);  // the step graph holds the grid sizes
+  c->ncg = ncg;
+  c->nfg = nfg;
+  c->live_chunks = lc;
   c->hcjobs.assign(c->ncpos, BlockJob{-1, 0, 0, 0});
   c->hfjobs.assign(c->nfpos, BlockJob{-1, 0, 0, 0});
-  const int64_t qc = c->ncpos / 8, qf = c->nfpos / 8;
+  const int64_t qc = ncg / 8, qf = nfg / 8;
+  int64_t jc = 0, jf = 0;  // running index over the live chunks
   for (int s = 0; s < S; ++s) {
     SimDesc& D = c->desc[s];
     const int64_t nc = c->loaded[s] ? (c->nused[s] + BLOCK - 1) / BLOCK : 0;
@@ -241,17 +258,18 @@ int write_jobs(hfv_ctx* c) {
     D.cchunk1 = (int32_t)(c->cchunk0[s] + nc);
     D.fchunk0 = (int32_t)c->fchunk0[s];
     D.fchunk1 = (int32_t)(c->fchunk0[s] + nfc);
-    for (int64_t j = 0; j < nc; ++j) {
+    if (!live(s)) continue;
+    for (int64_t j = 0; j < nc; ++j, ++jc) {
       const int64_t k = c->cchunk0[s] + j;
       const int64_t lo = c->cbase[s] + j * BLOCK;
       const int64_t hi = std::min<int64_t>(lo + BLOCK, c->cbase[s] + c->nused[s]);
-      c->hcjobs[(k % qc) * 8 + k / qc] = BlockJob{s, (int32_t)lo, (int32_t)hi, (int32_t)k};
+      c->hcjobs[(jc % qc) * 8 + jc / qc] = BlockJob{s, (int32_t)lo, (int32_t)hi, (int32_t)k};
     }
-    for (int64_t j = 0; j < nfc; ++j) {
+    for (int64_t j = 0; j < nfc; ++j, ++jf) {
       const int64_t k = c->fchunk0[s] + j;
       const int64_t lo = c->fbase[s] + j * BLOCK;
       const int64_t hi = std::min<int64_t>(lo + BLOCK, c->fbase[s] + c->fused[s]);
-      c->hfjobs[(k % qf) * 8 + k / qf] = BlockJob{s, (int32_t)lo, (int32_t)hi, (int32_t)k};
+      c->hfjobs[(jf % qf) * 8 + jf / qf] = BlockJob{s, (int32_t)lo, (int32_t)hi, (int32_t)k};
     }
   }
   hipStream_t st = c->stream;
@@ -330,8 +348,9 @@ int alloc_batch(hfv_ctx* c) {
   }
   c->ncchunk = ncc;
   c->nfchunk = nfc;
-  c->ncpos = (ncc + 7) / 8 * 8;
-  c->nfpos = (nfc + 7) / 8 * 8;
+  // at least the 8 positions of the smallest launched grid
+  c->ncpos = std::max<int64_t>(8, (ncc + 7) / 8 * 8);
+  c->nfpos = std::max<int64_t>(8, (nfc + 7) / 8 * 8);
   c->dcjobs = dalloc<BlockJob>(c, c->ncpos);
   c->dfjobs = dalloc<BlockJob>(c, c->nfpos);
   c->P.act = dalloc<uint32_t>(c, ncc);
@@ -797,7 +816,7 @@ void launch_flux_ts(hfv_ctx* c, const Batch& B, unsigned g) {
 // Flux kernels of the step: the static-geometry and the general variant each take the blocks of their
 // simulations; AIRS queues exact-solver faces for the fallback pass.
 void launch_flux(hfv_ctx* c, const Batch& B) {
-  const unsigned g = (unsigned)c->nfpos;
+  const unsigned g = (unsigned)c->nfg;
   const bool f32 = c->precision == HFV_FP32_FLUX;
   if (c->any_static) {
     if (f32) launch_flux_ts<float, true>(c, B, g);
@@ -814,7 +833,7 @@ void launch_flux(hfv_ctx* c, const Batch& B) {
 }
 
 void launch_grad(hfv_ctx* c, const Batch& B, int do_grad, int do_ts) {
-  const dim3 g((unsigned)c->ncpos), b(BLOCK);
+  const dim3 g((unsigned)c->ncg), b(BLOCK);
   if (c->any_static_grad) hipLaunchKernelGGL(k_grad<true>, g, b, 0, c->stream, B, do_grad, do_ts);
   if (c->any_dynamic_grad) hipLaunchKernelGGL(k_grad<false>, g, b, 0, c->stream, B, do_grad, do_ts);
 }
@@ -824,7 +843,7 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   Batch B = c->batch_args();
   hipStream_t st = c->stream;
   const unsigned gs = (unsigned)std::max(1, (c->nslots + 63) / 64);
-  const dim3 gc((unsigned)c->ncpos), gf((unsigned)c->nfpos), gb(BLOCK);
+  const dim3 gc((unsigned)c->ncg), gf((unsigned)c->nfg), gb(BLOCK);
   const bool t = timed && c->prof;
   if (t) prof_next(c);
   hipLaunchKernelGGL(k_begin, dim3(gs), dim3(64), 0, st, B);
@@ -864,7 +883,7 @@ void enqueue_step(hfv_ctx* c, bool timed) {
 // phase call changed their inputs (the fused step keeps them current itself).
 void pack_fr_if_dirty(hfv_ctx* c) {
   if (c->fr_dirty)
-    hipLaunchKernelGGL(k_pack_fr, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args());
+    hipLaunchKernelGGL(k_pack_fr, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, c->batch_args());
   c->fr_dirty = false;
 }
 
@@ -1179,7 +1198,7 @@ int hfv_predict(hfv_ctx* c, double dt) {
   E.dt_pred = dt;
   HIP_OK(hipMemcpyAsync(c->deng, &E, sizeof(SimEngine), hipMemcpyHostToDevice, c->stream));
   pack_fr_if_dirty(c);  // predict reads the kept time-extrapolation rates
-  hipLaunchKernelGGL(k_mask, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
+  hipLaunchKernelGGL(k_mask, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
   HIP_OK(hipGetLastError());
   c->fr_dirty = true;
   return sync_counters(c);
@@ -1192,7 +1211,7 @@ int hfv_compute_fluxes(hfv_ctx* c, const hfv_timestep_info* info) {
   TRY(begin_phase_step(c, info));
   Batch B = c->batch_args();
   hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, c->stream, B);
-  hipLaunchKernelGGL(k_mask, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B, 2);
+  hipLaunchKernelGGL(k_mask, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B, 2);
   c->fr_dirty = true;
   pack_fr_if_dirty(c);
   launch_flux(c, B);
@@ -1231,7 +1250,7 @@ int hfv_get_fluxes(hfv_ctx* c, double* out7f) {
 
 int hfv_apply_fluxes(hfv_ctx* c) {
   TRY(require_mesh(c));
-  hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
+  hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
   HIP_OK(hipGetLastError());
   c->fr_dirty = true;
   return sync_counters(c);
@@ -1239,7 +1258,7 @@ int hfv_apply_fluxes(hfv_ctx* c) {
 
 int hfv_conserved_to_primitive(hfv_ctx* c) {
   TRY(require_mesh(c));
-  hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 2);
+  hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, c->batch_args(), 2);
   HIP_OK(hipGetLastError());
   c->fr_dirty = true;
   return sync_counters(c);
@@ -1247,7 +1266,7 @@ int hfv_conserved_to_primitive(hfv_ctx* c) {
 
 int hfv_gradients(hfv_ctx* c) {
   TRY(require_mesh(c));
-  hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, c->batch_args(), 4 | 8);
+  hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, c->batch_args(), 4 | 8);
   launch_grad(c, c->batch_args(), 1, 0);
   HIP_OK(hipGetLastError());
   c->fr_dirty = true;
@@ -1258,8 +1277,8 @@ int hfv_timesteps(hfv_ctx* c, uint64_t* ti_next) {
   TRY(require_mesh(c));
   Batch B = c->batch_args();
   launch_grad(c, B, 0, 1);
-  hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B);
-  hipLaunchKernelGGL(k_limit, dim3((unsigned)c->ncpos), dim3(BLOCK), 0, c->stream, B, 1);
+  hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_limit, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B, 1);
   hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, c->stream, B);
   HIP_OK(hipGetLastError());
   c->fr_dirty = true;
@@ -1401,9 +1420,16 @@ int hfv_batch_run(hfv_ctx* c, int64_t max_launches, int64_t* launches) {
     TRY(sync_counters(c));
     prof_collect(c);
     bool all_done = true;
+    int64_t lc = 0;
     for (int s = 0; s < c->nslots; ++s)
-      if (c->loaded[s] && !c->heng[s].done) all_done = false;
+      if (c->loaded[s] && !c->heng[s].done) {
+        all_done = false;
+        lc += (c->nused[s] + BLOCK - 1) / BLOCK;
+      }
     if (all_done) break;
+    // drop finished slots from the grids once they hold a tenth of the launched chunks (each new grid
+    // size re-captures the step graph)
+    if (10 * lc < 9 * c->live_chunks) TRY(write_jobs(c));
     chunk = std::min<int64_t>(chunk * 2, 64);
   }
   if (launches) *launches = done;
