@@ -228,10 +228,29 @@ void drop_graph(hfv_ctx* c) {
   c->graph_ok = false;
 }
 
+// Block-cyclic placement of live chunks on the 8 XCDs (the XCD that runs grid position b is b % 8):
+// groups of G consecutive chunks go to the XCDs in turn, so that each XCD's L2 sees runs of neighbouring
+// chunks (their gathers overlap) while every XCD gets a share of every part of the work.  G about L/64
+// as a power of two, at most 8 in a batch (many simulations of unequal work: balance) and 32 for a
+// single simulation (one large mesh: locality).  Measured: ensemble 1.47 -> 1.37 ms per launch against
+// whole-simulation ranges per XCD; KH 128^3 2.86 -> 2.81 ms per step.
+constexpr int64_t kMaxGroup = 32;
+int64_t group_size(const hfv_ctx* c, int64_t live) {
+  const int64_t cap = c->batch ? 8 : kMaxGroup;
+  int64_t g = 1;
+  while (2 * g <= cap && 64 * (2 * g) <= live) g *= 2;
+  return g;
+}
+// Launched positions for `live` chunks in groups of g: whole rounds of 8 groups.
+int64_t grid_size(int64_t live, int64_t g) {
+  return std::max<int64_t>(8 * g, (live + 8 * g - 1) / (8 * g) * 8 * g);
+}
+// Position of live chunk j: group j / g goes to XCD (j / g) % 8, in round j / (8 g).
+int64_t grid_position(int64_t j, int64_t g) { return 8 * ((j / g / 8) * g + j % g) + (j / g) % 8; }
+
 // Rebuild the block tables from the live slots (loaded, and in a batch not finished: a finished slot's
-// blocks would only exit) and upload them with the simulation descriptors.  The j-th live chunk sits at
-// grid position (j % Q) * 8 + j / Q, Q = launched positions / 8: the XCD that runs position b (b % 8)
-// gets the contiguous live chunks [xcd * Q, (xcd + 1) * Q).  Chunk ids (partials) stay global.
+// blocks would only exit) and upload them with the simulation descriptors.  Chunk ids (partials) stay
+// global.
 int write_jobs(hfv_ctx* c) {
   const int S = c->nslots;
   auto live = [&](int s) { return c->loaded[s] && !(c->batch && c->heng[s].done); };
@@ -241,14 +260,14 @@ int write_jobs(hfv_ctx* c) {
     lc += (c->nused[s] + BLOCK - 1) / BLOCK;
     lf += (c->fused[s] + BLOCK - 1) / BLOCK;
   }
-  const int64_t ncg = std::max<int64_t>(8, (lc + 7) / 8 * 8), nfg = std::max<int64_t>(8, (lf + 7) / 8 * 8);
+  const int64_t gc = group_size(c, lc), gf = group_size(c, lf);
+  const int64_t ncg = grid_size(lc, gc), nfg = grid_size(lf, gf);
   if (ncg != c->ncg || nfg != c->nfg) drop_graph(c);  // the step graph holds the grid sizes
   c->ncg = ncg;
   c->nfg = nfg;
   c->live_chunks = lc;
   c->hcjobs.assign(c->ncpos, BlockJob{-1, 0, 0, 0});
   c->hfjobs.assign(c->nfpos, BlockJob{-1, 0, 0, 0});
-  const int64_t qc = ncg / 8, qf = nfg / 8;
   int64_t jc = 0, jf = 0;  // running index over the live chunks
   for (int s = 0; s < S; ++s) {
     SimDesc& D = c->desc[s];
@@ -263,13 +282,13 @@ int write_jobs(hfv_ctx* c) {
       const int64_t k = c->cchunk0[s] + j;
       const int64_t lo = c->cbase[s] + j * BLOCK;
       const int64_t hi = std::min<int64_t>(lo + BLOCK, c->cbase[s] + c->nused[s]);
-      c->hcjobs[(jc % qc) * 8 + jc / qc] = BlockJob{s, (int32_t)lo, (int32_t)hi, (int32_t)k};
+      c->hcjobs[grid_position(jc, gc)] = BlockJob{s, (int32_t)lo, (int32_t)hi, (int32_t)k};
     }
     for (int64_t j = 0; j < nfc; ++j, ++jf) {
       const int64_t k = c->fchunk0[s] + j;
       const int64_t lo = c->fbase[s] + j * BLOCK;
       const int64_t hi = std::min<int64_t>(lo + BLOCK, c->fbase[s] + c->fused[s]);
-      c->hfjobs[(jf % qf) * 8 + jf / qf] = BlockJob{s, (int32_t)lo, (int32_t)hi, (int32_t)k};
+      c->hfjobs[grid_position(jf, gf)] = BlockJob{s, (int32_t)lo, (int32_t)hi, (int32_t)k};
     }
   }
   hipStream_t st = c->stream;
@@ -348,9 +367,9 @@ int alloc_batch(hfv_ctx* c) {
   }
   c->ncchunk = ncc;
   c->nfchunk = nfc;
-  // at least the 8 positions of the smallest launched grid
-  c->ncpos = std::max<int64_t>(8, (ncc + 7) / 8 * 8);
-  c->nfpos = std::max<int64_t>(8, (nfc + 7) / 8 * 8);
+  // room for the largest launched grid (all chunks live, groups of kMaxGroup)
+  c->ncpos = grid_size(ncc, kMaxGroup);
+  c->nfpos = grid_size(nfc, kMaxGroup);
   c->dcjobs = dalloc<BlockJob>(c, c->ncpos);
   c->dfjobs = dalloc<BlockJob>(c, c->nfpos);
   c->P.act = dalloc<uint32_t>(c, ncc);
