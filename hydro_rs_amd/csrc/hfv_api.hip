@@ -234,9 +234,15 @@ void drop_graph(hfv_ctx* c) {
 // as a power of two, at most 8 in a batch (many simulations of unequal work: balance) and 32 for a
 // single simulation (one large mesh: locality).  Measured: ensemble 1.47 -> 1.37 ms per launch against
 // whole-simulation ranges per XCD; KH 128^3 2.86 -> 2.81 ms per step.
-constexpr int64_t kMaxGroup = 32;
+#ifndef HFV_GROUP_BATCH
+#define HFV_GROUP_BATCH 8
+#endif
+#ifndef HFV_GROUP_SINGLE
+#define HFV_GROUP_SINGLE 32
+#endif
+constexpr int64_t kMaxGroup = HFV_GROUP_BATCH > HFV_GROUP_SINGLE ? HFV_GROUP_BATCH : HFV_GROUP_SINGLE;
 int64_t group_size(const hfv_ctx* c, int64_t live) {
-  const int64_t cap = c->batch ? 8 : kMaxGroup;
+  const int64_t cap = c->batch ? HFV_GROUP_BATCH : HFV_GROUP_SINGLE;
   int64_t g = 1;
   while (2 * g <= cap && 64 * (2 * g) <= live) g *= 2;
   return g;
