@@ -138,7 +138,7 @@ def summarize_pmc(fetch_dir, write_dir, copy_fetch_dir, copy_write_dir, label, c
         fb = float(np.mean(fetch[k])) * f_scale
         wb = float(np.mean(write[k])) * w_scale
         table[k] = {"dispatches": len(fetch[k]), "fetch_bytes": fb, "write_bytes": wb, "traffic_bytes": fb + wb}
-    lines = [f"# HBM traffic per dispatch: {label}", "", "Commands (from /tmp on the GPU box, R = repository root):", ""]
+    lines = [f"# HBM traffic per dispatch: {label}", "", "Commands (run from /tmp with TMPDIR=/tmp on the MI355X host, R = repository root):", ""]
     lines += [f"    {c}" for c in command]
     lines += ["", f"Calibration: a {copy_mib} MiB device copy reads FETCH_SIZE x {f_scale:.3f} and WRITE_SIZE x "
                   f"{w_scale:.3f} of its true bytes; every kernel's counters below are scaled by these factors."]
@@ -175,7 +175,7 @@ def summarize_trace(stats_csv, label, command, note="", trace_csv=None, window=0
         win = _last_dispatches(list(csv.DictReader(open(trace_csv))), window,
                                lambda r: int(r["End_Timestamp"]) - int(r["Start_Timestamp"]))
     wcol = f" last {window} avg us |" if win else ""
-    lines = [f"# Kernel stats: {label}", "", "Command (from /tmp on the GPU box, R = repository root):", "",
+    lines = [f"# Kernel stats: {label}", "", "Command (run from /tmp with TMPDIR=/tmp on the MI355X host, R = repository root):", "",
              f"    {command}", "", "| kernel | calls | total us | avg us | min us | max us | % |" + wcol,
              "|---|---|---|---|---|---|---|" + ("---|" if win else "")]
     for r in rows:
@@ -342,7 +342,7 @@ def bench_ensemble(args):
 def cpu_baseline_ensemble(probs, stride, warmup, budget_s):
     """The oracle over members 0, stride, 2*stride, ... (FP64, as the reference), each for up to
     budget/len steps' worth of time after `warmup` untimed steps."""
-    from oracle import ora  # checker / CPU baseline only
+    from oracle import ora  # timed CPU baseline only
     from tests.helpers import oracle_sim
     from hydro_rs_amd import fvs, ics
 

@@ -1,8 +1,8 @@
 """ctypes front-end of the CPU FP64 oracle (oracle/*.c).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, always as the checker or the timed CPU baseline,
-never by the product package hydro_rs_amd.
+cpu_baseline leg of bench.py, always as the reference that results are compared against or as the
+timed CPU baseline, never by the product package hydro_rs_amd.
 """
 import ctypes as C
 import os

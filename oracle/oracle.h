@@ -3,8 +3,8 @@
  * (yuyttenhove/hydro_rs) along the per-face finite-volume update.
  *
  * TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and the
- * cpu_baseline leg of bench.py may load this library, and only as the checker
- * or as the timed CPU baseline.  The product (hydro_rs_amd) never links it.
+ * cpu_baseline leg of bench.py may load this library, and only as the reference
+ * that results are compared against or as the timed CPU baseline.  The product (hydro_rs_amd) never links it.
  *
  * Every function cites the reference file:line it restates.  Arithmetic is
  * written in the reference's evaluation order (Rust evaluates left to right,
