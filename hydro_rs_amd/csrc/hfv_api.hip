@@ -100,6 +100,7 @@ int check_params(const hfv_params* p) {
   if (p->motion < 0 || p->motion > 3) return set_err(HFV_ERR_INVALID, "unknown particle motion");
   if (p->runner < 0 || p->runner > 1) return set_err(HFV_ERR_INVALID, "unknown runner");
   if (p->precision < 0 || p->precision > 1) return set_err(HFV_ERR_INVALID, "unknown precision");
+  if (p->eos < 0 || p->eos > 1) return set_err(HFV_ERR_INVALID, "unknown equation of state");
   if (!(p->gamma > 1.)) return set_err(HFV_ERR_INVALID, "gamma must be > 1");
   if (!(p->t_end > 0.)) return set_err(HFV_ERR_INVALID, "t_end must be > 0");
   return HFV_OK;
@@ -1378,6 +1379,9 @@ int hfv_conservation(hfv_ctx* c, double* out5) {
 int hfv_batch_create(int nslots, const int64_t* cell_cap, const int64_t* face_cap, const int64_t* conn_cap,
                      hfv_ctx** out) {
   if (nslots <= 0 || !cell_cap || !face_cap || !conn_cap || !out) return set_err(HFV_ERR_INVALID, "bad batch arguments");
+  for (int s = 0; s < nslots; ++s)
+    if (cell_cap[s] < 1 || face_cap[s] < 0 || conn_cap[s] < 0)
+      return set_err(HFV_ERR_INVALID, "slot capacities: cells >= 1, faces and CSR entries >= 0");
   hfv_ctx* c = nullptr;
   TRY(new_ctx(&c, nslots));
   c->batch = true;
