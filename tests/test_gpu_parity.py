@@ -63,6 +63,10 @@ CASES = {
     "sedov2d_sync_all": lambda: _with(ics.sedov_2d(side=32, t_end=0.05), sync_all=True),
     "tube1d_isothermal": lambda: _with(ics.riemann_tube_1d(300, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, t_end=0.05),
                                        eos=1, isothermal_u=1.5),
+    # tiny meshes: one partly filled chunk, launch grids of padding blocks
+    "tube1d_n3": lambda: ics.riemann_tube_1d(3, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, t_end=0.5),
+    "sedov2d_side4": lambda: ics.sedov_2d(side=4, t_end=0.5),
+    "kh3d_side3": lambda: ics.kelvin_helmholtz_3d(side=3, t_end=1.0),
     "sedov2d": lambda: ics.sedov_2d(side=32, t_end=0.05),
     "kh3d": lambda: ics.kelvin_helmholtz_3d(side=10, t_end=0.5),
     "tube2d_reflective_exact": lambda: _tube2d(),
@@ -171,7 +175,8 @@ def test_step_parity(ora, case, k):
 @pytest.mark.parametrize("case,steps", [("sod1d", 1 << 40), ("sod1d_steer", 1 << 40), ("sod1d_steer_pakmor", 1 << 40),
                                         ("sod1d_godunov", 1 << 40), ("sod1d_sync_all", 1 << 40),
                                         ("sod1d_dt_snap", 1 << 40), ("sedov2d_sync_all", 400),
-                                        ("tube1d_isothermal", 1 << 40),
+                                        ("tube1d_isothermal", 1 << 40), ("tube1d_n3", 1 << 40),
+                                        ("sedov2d_side4", 1 << 40), ("kh3d_side3", 1 << 40),
                                         ("sedov2d", 400), ("tube2d_reflective_exact", 1 << 40),
                                         ("tube2d_vacuum_hllc", 1 << 40), ("blast3d_reflective", 1 << 40)])
 def test_full_run_parity(ora, case, steps):
