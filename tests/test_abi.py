@@ -31,6 +31,13 @@ def test_bindings_cover_header():
     assert set(declared("hfv_host.h")) == set(_lib.HOST_SYMBOLS)
 
 
+def test_integration_guide_declares_every_entry_point():
+    """INTEGRATION.md's Rust extern blocks declare exactly the entry points of include/hfv.h."""
+    guide = open(os.path.join(ROOT, "INTEGRATION.md")).read()
+    rust = set(re.findall(r"pub fn (hfv_[a-z0-9_]+)\(", guide))
+    assert rust == set(declared("hfv.h"))
+
+
 def test_bindings_load():
     assert _lib.hfv() is not None
     assert _lib.host() is not None
