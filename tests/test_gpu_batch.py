@@ -59,6 +59,27 @@ def test_batch_equals_standalone(static):
         say(f"  {p.name}: {ei['step_count']} steps, identical to standalone")
 
 
+@pytest.mark.parametrize("precision", [fvs.FP64, fvs.FP32_FLUX])
+def test_run_chunking_does_not_change_results(precision):
+    """Finished simulations leave the launch grids when hfv_batch_run polls, so the grids (and the
+    captured step graph) depend on how a caller splits its runs; the results must not."""
+    probs = [ics.with_precision(p, precision) for p in _mix()]
+    a = fvs.Batch.for_problems(probs)
+    na = a.run(1 << 40)
+    b = fvs.Batch.for_problems(probs)
+    nb, calls = 0, 0
+    while not all(b.engine_info(s)["done"] for s in range(len(probs))):
+        nb += b.run(3)
+        calls += 1
+    say(f"[chunking] one call: {na} launches; {calls} calls of 3: {nb} launches")
+    for s, p in enumerate(probs):
+        ea, eb = a.engine_info(s), b.engine_info(s)
+        assert ea["done"] and eb["done"] and ea["step_count"] == eb["step_count"], p.name
+        ga, gb = a.get_state(s), b.get_state(s)
+        for k in ga:
+            assert np.array_equal(ga[k], gb[k]), (p.name, k)
+
+
 def test_batch_slot_reload_and_oracle(ora):
     probs = _mix()
     b = fvs.Batch.for_problems(probs[:3], headroom=2.0)
