@@ -1331,8 +1331,19 @@ int hfv_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {
 }
 
 // ---- engine (src/engine.rs:152-268), device-resident
+// The device engine loops drift generators (k_drift) but rebuild meshes only in 1D: a 2D/3D simulation
+// with moving generators needs the caller's re-tessellation between drift and fluxes (hfv_step with
+// hfv_set_mesh), so the loops refuse it rather than step it on a stale mesh.
+int check_engine_loop(const hfv_params& p) {
+  if (p.dim >= 2 && p.motion != HFV_FIXED)
+    return set_err(HFV_ERR_UNSUPPORTED,
+                   "2D/3D moving meshes are rebuilt by the caller each step: use hfv_step with hfv_set_mesh");
+  return HFV_OK;
+}
+
 int hfv_run(hfv_ctx* c, int64_t max_steps, int64_t* steps_done) {
   TRY(require_mesh(c));
+  TRY(check_engine_loop(c->params[0]));
   TRY(sync_counters(c));
   const long long s0 = c->heng[0].step_count;
   int64_t done = 0;
@@ -1401,6 +1412,7 @@ int hfv_batch_load(hfv_ctx* c, int s, const hfv_params* params, const hfv_mesh_a
                    const double* mass, const double* vel, const double* u_int) {
   TRY(check_slot(c, s));
   TRY(check_params(params));
+  TRY(check_engine_loop(*params));
   if (!c->batch) return set_err(HFV_ERR_INVALID, "not a batch context");
   if (!mesh) return set_err(HFV_ERR_INVALID, "mesh is NULL");
   for (int k = 0; k < c->nslots; ++k)

@@ -209,7 +209,9 @@ int hfv_timesteps(hfv_ctx* ctx, uint64_t* ti_next);
  * (optimal_order.rs:15-64, godunov.rs:14-58).  For moving meshes in 1D the Voronoi rebuild runs on
  * the device; 2D/3D moving meshes are rebuilt by the caller through hfv_set_mesh. */
 int hfv_step(hfv_ctx* ctx, const hfv_timestep_info* info, uint64_t* ti_next);
-/* Engine::run (src/engine.rs:152-192): advance until max_steps Step sync points ran or t_end. */
+/* Engine::run (src/engine.rs:152-192): advance until max_steps Step sync points ran or t_end.  The
+ * device loop drifts generators and rebuilds 1D meshes; a 2D/3D simulation with moving generators
+ * returns HFV_ERR_UNSUPPORTED (drive it with hfv_step and hfv_set_mesh instead). */
 int hfv_run(hfv_ctx* ctx, int64_t max_steps, int64_t* steps_done);
 int hfv_get_engine_info(hfv_ctx* ctx, hfv_engine_info* out);
 /* Space::status sums (src/space.rs:1248-1280): sum of conserved + pending fluxes per component,
@@ -229,7 +231,9 @@ int hfv_batch_create(int nslots, const int64_t* cell_cap, const int64_t* face_ca
                      hfv_ctx** out);
 /* Load a simulation into a slot: Space::initialize + first_init (src/space.rs:88-211) from the mesh and
  * the IC arrays (loc, vel [3n] xyz interleaved; mass, u_int [n]); resets the slot's engine (Engine::new,
- * src/engine.rs:104-149).  Fails with HFV_ERR_INVALID if the mesh exceeds the slot's capacity. */
+ * src/engine.rs:104-149).  Fails with HFV_ERR_INVALID if the mesh exceeds the slot's capacity, and with
+ * HFV_ERR_UNSUPPORTED for a 2D/3D simulation with moving generators (batches run the device engine
+ * loop, which rebuilds only 1D meshes). */
 int hfv_batch_load(hfv_ctx* ctx, int slot, const hfv_params* params, const hfv_mesh_arrays* mesh,
                    const double* loc, const double* mass, const double* vel, const double* u_int);
 /* Mark a slot empty (its cells stop participating). */

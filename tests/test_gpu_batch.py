@@ -80,6 +80,26 @@ def test_run_chunking_does_not_change_results(precision):
             assert np.array_equal(ga[k], gb[k]), (p.name, k)
 
 
+def test_engine_loops_refuse_2d_moving_meshes():
+    """The device engine loops rebuild only 1D meshes: a 2D simulation with moving generators is
+    refused (HFV_ERR_UNSUPPORTED) by hfv_run and by hfv_batch_load rather than stepped on a stale mesh."""
+    p = ics.isentropic_vortex_2d(side=12, t_end=0.1)
+    moving = ics.Problem(p.name, _with_motion(p.cfg, fvs.FLUID), p.loc, p.mass, p.vel, p.u_int, p.mesh)
+    d = ics.build_device_sim(moving)
+    with pytest.raises(RuntimeError, match="moving meshes"):
+        d.run(1)
+    with pytest.raises(RuntimeError, match="moving meshes"):
+        fvs.Batch.for_problems([moving])
+    # fixed generators run
+    assert fvs.Batch.for_problems([p]).run(2) == 2
+
+
+def _with_motion(cfg, motion):
+    from dataclasses import replace
+
+    return replace(cfg, motion=motion)
+
+
 def test_batch_slot_reload_and_oracle(ora):
     probs = _mix()
     b = fvs.Batch.for_problems(probs[:3], headroom=2.0)
