@@ -19,7 +19,8 @@ Printed JSON line: the contract keys plus
   roofline               dominant kernel of the timed launches (HIP events on the context's stream):
                          algorithmic bytes per launch / average launch time vs 8 TB/s
   cpu_baseline           the oracle (C restatement of the reference path, OpenMP at the reference's
-                         rayon sites) on the host cores, over a bounded sample of the same members
+                         rayon sites) on all host cores, over a bounded sample of the same members,
+                         with the host CPU model and a shorter one-thread sample (single_thread)
   sedov2d / kh3d / serve the 2D Sedov blast (plain FP64 and FP32), the 3D Kelvin-Helmholtz layer
                          (plain FP64 with its roofline, and FP32) and the served request stream
                          (p50/p99 job latency, cell updates/s, L1 gate)
@@ -339,39 +340,63 @@ def bench_ensemble(args):
     return res, probs
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline_ensemble(probs, stride, warmup, budget_s):
     """The oracle over members 0, stride, 2*stride, ... (FP64, as the reference), each for up to
-    budget/len steps' worth of time after `warmup` untimed steps."""
+    budget/len steps' worth of time after `warmup` untimed steps, on all host threads (OpenMP at the
+    reference's rayon sites, as rayon uses every logical CPU); then a shorter sample on one thread."""
     from oracle import ora  # timed CPU baseline only
     from tests.helpers import oracle_sim
     from hydro_rs_amd import fvs, ics
 
+    def sample(idx, per, threads):
+        cu, secs, steps = 0, 0.0, 0
+        for i in idx:
+            sim = oracle_sim(ics.with_precision(probs[i], fvs.FP64))
+            ora.set_threads(0)  # warmup on every thread, whatever the timed count
+            try:
+                sim.advance(warmup)
+            except RuntimeError:
+                continue
+            ora.set_threads(threads)
+            t0 = time.perf_counter()
+            while time.perf_counter() - t0 < per:
+                try:
+                    if sim.advance(1) == 0:
+                        break
+                except RuntimeError:
+                    break
+                cu += sim.info()["last_n_active"]
+                steps += 1
+            secs += time.perf_counter() - t0
+        ora.set_threads(0)
+        return cu, secs, steps
+
     threads = ora.set_threads(0)
     idx = list(range(0, len(probs), stride))
     per = budget_s / len(idx)
-    cu = 0
-    secs = 0.0
-    steps = 0
-    for i in idx:
-        sim = oracle_sim(ics.with_precision(probs[i], fvs.FP64))
-        try:
-            sim.advance(warmup)
-        except RuntimeError:
-            continue
-        t0 = time.perf_counter()
-        while time.perf_counter() - t0 < per:
-            try:
-                if sim.advance(1) == 0:
-                    break
-            except RuntimeError:
-                break
-            cu += sim.info()["last_n_active"]
-            steps += 1
-        secs += time.perf_counter() - t0
+    cu, secs, steps = sample(idx, per, 0)
+    idx1 = idx[::4]
+    per1 = budget_s / 3 / len(idx1)
+    cu1, secs1, steps1 = sample(idx1, per1, 1)
     return {"value": cu / secs if secs > 0 else 0.0, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"members {idx[0]}..{idx[-1]} step {len(idx)} of the ensemble, each for {per:.2f}s after "
-                      f"{warmup} untimed steps: {steps} steps, {cu} cell updates in {secs:.1f}s, "
-                      f"OpenMP threads={threads}"}
+            "sample": f"members {idx[0]}..{idx[-1]} every {stride} ({len(idx)}) of the ensemble, each for "
+                      f"{per:.2f}s after {warmup} untimed steps: {steps} steps, {cu} cell updates in {secs:.1f}s, "
+                      f"OpenMP threads={threads}",
+            "cpu_model": _cpu_model(),
+            "single_thread": {"value": cu1 / secs1 if secs1 > 0 else 0.0, "cores": 1,
+                              "sample": f"members {idx1[0]}..{idx1[-1]} every {4 * stride} ({len(idx1)}), each for "
+                                        f"{per1:.2f}s after the same warmup: {steps1} steps, {cu1} cell updates "
+                                        f"in {secs1:.1f}s"}}
 
 
 # ------------------------------------------------------------------ single simulations
