@@ -49,7 +49,10 @@ METRIC = "cell-updates/s vs plain FP64 GPU path; p50/p99 job latency at L1 err<1
 #   grad_timestep: per active cell 600 B (of which the gradient write 120) + per face of an active cell 120 B
 #   mask_predict: per cell 193 B (of which 9 gradient components 72 and the extrapolation r/w 80)
 # FP32-flux contexts store gradients, extrapolations and face records as floats: the same terms at
-# 4-byte widths (grad 60, extrap 20, record 28).
+# 4-byte widths (grad 60, extrap 20, record 28).  They also defer the predict: mask_predict reads the
+# time bin and writes the active flag only (2 B per cell), the flux kernel reads each cell's 8-byte
+# extrapolation tag, and the reset in limit_kick writes the float extrapolations (20 B instead of 40)
+# and the tag (8 B).
 ALGO = {
     "flux": lambda n_act, f_act, n_all: 329 * n_act + 121 * f_act,
     "apply_c2p": lambda n_act, f_act, n_all: 89 * n_all + 208 * n_act + 120 * f_act,
@@ -59,12 +62,12 @@ ALGO = {
     "limit_kick": lambda n_act, f_act, n_all: 19 * n_all + 80 * n_act,
 }
 ALGO_FP32 = {
-    "flux": lambda n_act, f_act, n_all: 249 * n_act + 93 * f_act,
+    "flux": lambda n_act, f_act, n_all: 257 * n_act + 93 * f_act,
     "apply_c2p": lambda n_act, f_act, n_all: 89 * n_all + 208 * n_act + 64 * f_act,
     "grad_timestep": lambda n_act, f_act, n_all: 540 * n_act + 120 * f_act,
-    "mask_predict": lambda n_act, f_act, n_all: 117 * n_all,
+    "mask_predict": lambda n_act, f_act, n_all: 2 * n_all,
     "wakeup": ALGO["wakeup"],
-    "limit_kick": ALGO["limit_kick"],
+    "limit_kick": lambda n_act, f_act, n_all: 19 * n_all + 68 * n_act,
 }
 
 
@@ -474,7 +477,7 @@ def bench_kh3d(args):
         f_act = s["face_fluxes"] / max(s["steps"], 1)
         # B_step at the context's storage widths (FP32-flux: gradients, extrapolations, records as floats)
         if prec == fvs.FP32_FLUX:
-            step_bytes = 116 * prob.n + 1107 * n_act + 200 * f_act
+            step_bytes = 2 * prob.n + 1103 * n_act + 200 * f_act
         else:
             step_bytes = 192 * prob.n + 1247 * n_act + 256 * f_act
         out[k] = {"cell_updates_per_s": s["cell_updates"] / s["seconds"],
@@ -533,7 +536,7 @@ def main():
     ap.add_argument("--tune-launches", type=int, default=2,
                     help="ensemble autotuner: launches per timed group of a candidate trial (0: no tuning)")
     ap.add_argument("--tune-rounds", type=int, default=1)
-    ap.add_argument("--kh-tune", type=int, default=4, help="3D workload autotuner: launches per timed group of a trial (0: none)")
+    ap.add_argument("--kh-tune", type=int, default=8, help="3D workload autotuner: launches per timed group of a trial (0: none)")
     ap.add_argument("--sedov-side", type=int, default=256)
     ap.add_argument("--kh-side", type=int, default=128)
     ap.add_argument("--kh-steps", type=int, default=20)

@@ -118,6 +118,7 @@ SimEngine fresh_engine(const hfv_params& p) {
   E.between = between ? between : HFV_MAX_NR_TIMESTEPS;
   E.ti_min = ~0ull;
   E.min_bin = 127;
+  E.chk = 1;  // the first predict checks the primitives the host writes
   return E;
 }
 
@@ -346,6 +347,7 @@ int alloc_batch(hfv_ctx* c) {
   C.srad = dalloc<double>(c, n);
   C.pc = dalloc<double>(c, 8 * n);
   C.tex = dalloc<double>(c, 5 * n);
+  C.tiex = dalloc<uint64_t>(c, n);
   C.bin = dalloc<int8_t>(c, n);
   C.wake = dalloc<int8_t>(c, n);
   C.act = dalloc<uint8_t>(c, n);
@@ -409,6 +411,7 @@ int alloc_batch(hfv_ctx* c) {
   int mult[] = {5, 15, 3, HFV_EX_W, 5, 5, 1, 3, 3, 3, 3, 1, 1, 1, 1, 5};
   for (int k = 0; k < 16; ++k) HIP_OK(hipMemsetAsync(dbl[k], 0, sd * mult[k], st));
   HIP_OK(hipMemsetAsync(C.pc, 0, sd * 8, st));
+  HIP_OK(hipMemsetAsync(C.tiex, 0, sizeof(uint64_t) * (size_t)n, st));
   HIP_OK(hipMemsetAsync(C.fr, 0, sd * HFV_FR_W, st));
   HIP_OK(hipMemsetAsync(F.scnt, 0, (size_t)std::max<int64_t>(nfc, 1), st));
   HIP_OK(hipMemsetAsync(F.floc, 0, 2 * (size_t)std::max<int64_t>(nf, 1), st));
@@ -715,6 +718,7 @@ int reset_slot(hfv_ctx* c, int s) {
   HIP_OK(hipMemsetAsync((char*)C.extrap + es * HFV_EX_W * cb, 0, es * HFV_EX_W * cap, st));
   for (int k = 0; k < 5; ++k) HIP_OK(hipMemsetAsync((char*)C.tex + es * (k * N + cb), 0, es * cap, st));
   for (int k = 0; k < 15; ++k) HIP_OK(hipMemsetAsync((char*)C.grad + es * (k * N + cb), 0, es * cap, st));
+  HIP_OK(hipMemsetAsync(C.tiex + cb, 0, sizeof(uint64_t) * cap, st));  // = the fresh engine's ti_pred
   HIP_OK(hipMemsetAsync(C.bin + cb, 0, cap, st));
   HIP_OK(hipMemsetAsync(C.wake + cb, 0, cap, st));
   HIP_OK(hipMemsetAsync(C.act + cb, 0, cap, st));
@@ -759,6 +763,14 @@ int refresh_geometry(hfv_ctx* c, int s) {
   return HFV_OK;
 }
 
+// FP32-flux contexts defer the engine steps' predicts (SimEngine::ti_pred): before a host write or a
+// trait-level phase changes what a cell's rate is made of, its unsummed predict time is summed into its
+// extrapolations with the old rate (k_settle; slot s, or every slot for s < 0).
+void settle(hfv_ctx* c, int s) {
+  if (c->precision != HFV_FP32_FLUX || c->ncg <= 0) return;
+  hipLaunchKernelGGL(k_settle, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, c->batch_args(), s);
+}
+
 int init_ic(hfv_ctx* c, int s, int64_t n, const double* loc, const double* mass, const double* vel, const double* u_int) {
   if (n != c->nused[s] || !loc || !mass || !vel || !u_int) return set_err(HFV_ERR_INVALID, "bad IC arrays");
   const int64_t N = c->n, cb = c->cbase[s];
@@ -778,6 +790,7 @@ int init_ic(hfv_ctx* c, int s, int64_t n, const double* loc, const double* mass,
   HIP_OK(hipMemcpyAsync(dm, mass, sizeof(double) * n, hipMemcpyHostToDevice, st));
   HIP_OK(hipMemcpyAsync(dv, vels.data(), sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
   HIP_OK(hipMemcpyAsync(du, u_int, sizeof(double) * n, hipMemcpyHostToDevice, st));
+  settle(c, s);
   hipLaunchKernelGGL(k_init_ic, dim3(grid_full(n)), dim3(BLOCK), 0, st, c->batch_args(), s, cb, cb + n, dm, dv, du);
   HIP_OK(hipGetLastError());
   HIP_OK(hipStreamSynchronize(st));
@@ -1001,6 +1014,7 @@ int copy_state(hfv_ctx* c, int s, hfv_state* st, bool to_device) {
   const int64_t N = c->n, cb = c->cbase[s], n = c->nused[s];
   hipStream_t strm = c->stream;
   hipMemcpyKind kind = to_device ? hipMemcpyHostToDevice : hipMemcpyDeviceToHost;
+  if (to_device) settle(c, s);
 #define CPF(name, dev, w)                                                                        \
   if (st->name)                                                                                  \
     for (int q = 0; q < (w); ++q) {                                                              \
@@ -1055,6 +1069,24 @@ int copy_state(hfv_ctx* c, int s, hfv_state* st, bool to_device) {
       HIP_OK(hipMemcpyAsync(ex32 ? (void*)exf.data() : (void*)exa.data(), d, bytes, hipMemcpyDeviceToHost, strm));
     }
   }
+  // FP32-flux reads: the extrapolations due now, summed on the host (k_settle's expression) so that a
+  // read leaves the device state as it is
+  const bool due = st->extrap && ex32 && !to_device;
+  std::vector<uint64_t> tix;
+  std::vector<float> txf;
+  std::vector<double> rho;
+  SimEngine eh;
+  if (due) {
+    tix.resize(n);
+    txf.resize(5 * (size_t)n);
+    rho.resize(n);
+    HIP_OK(hipMemcpyAsync(tix.data(), c->C.tiex + cb, sizeof(uint64_t) * n, hipMemcpyDeviceToHost, strm));
+    for (int q = 0; q < 5; ++q)
+      HIP_OK(hipMemcpyAsync(txf.data() + (size_t)q * n, (const float*)c->C.tex + (size_t)q * N + cb, sizeof(float) * n,
+                            hipMemcpyDeviceToHost, strm));
+    HIP_OK(hipMemcpyAsync(rho.data(), c->C.prim + cb, sizeof(double) * n, hipMemcpyDeviceToHost, strm));
+    HIP_OK(hipMemcpyAsync(&eh, c->deng + s, sizeof(SimEngine), hipMemcpyDeviceToHost, strm));
+  }
   if (st->timebin)
     HIP_OK(hipMemcpyAsync(to_device ? (void*)(c->C.bin + cb) : (void*)st->timebin,
                           to_device ? (const void*)st->timebin : (const void*)(c->C.bin + cb), (size_t)n, kind, strm));
@@ -1062,8 +1094,14 @@ int copy_state(hfv_ctx* c, int s, hfv_state* st, bool to_device) {
   if (st->grad && ex32 && !to_device)
     for (size_t k = 0; k < grf.size(); ++k) st->grad[k] = (double)grf[k];
   if (st->extrap && !to_device)
-    for (int64_t i = 0; i < n; ++i)
-      for (int q = 0; q < 5; ++q) st->extrap[q * n + i] = ex32 ? (double)exf[HFV_EX_W * i + q] : exa[HFV_EX_W * i + q];
+    for (int64_t i = 0; i < n; ++i) {
+      const double dt = due ? (double)(eh.ti_pred - tix[i]) * eh.time_base : 0.;
+      for (int q = 0; q < 5; ++q) {
+        double v = ex32 ? (double)exf[HFV_EX_W * i + q] : exa[HFV_EX_W * i + q];
+        if (due && rho[i] > 0.) v = (double)(float)(v + (-dt) * (double)txf[(size_t)q * n + i]);
+        st->extrap[q * n + i] = v;
+      }
+    }
   if (to_device) {
     c->fr_dirty = true;
     if (st->v) {  // the static geometry needs every particle velocity to be exactly +0
@@ -1081,7 +1119,8 @@ int copy_state(hfv_ctx* c, int s, hfv_state* st, bool to_device) {
 }
 
 // Trait-level phases operate on slot 0 with the caller's TimestepInfo.
-int begin_phase_step(hfv_ctx* c, const hfv_timestep_info* info) {
+// pred: the step's predict is the engine's own (hfv_step), not a separate hfv_predict call.
+int begin_phase_step(hfv_ctx* c, const hfv_timestep_info* info, bool pred) {
   SimEngine& E = c->heng[0];
   HIP_OK(hipMemcpyAsync(&E, c->deng, sizeof(SimEngine), hipMemcpyDeviceToHost, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
@@ -1092,6 +1131,7 @@ int begin_phase_step(hfv_ctx* c, const hfv_timestep_info* info) {
   E.time_base_inv = info->time_base_inv;
   E.force = 1;
   E.done = 0;
+  E.no_pred = pred ? 0 : 1;
   HIP_OK(hipMemcpyAsync(c->deng, &E, sizeof(SimEngine), hipMemcpyHostToDevice, c->stream));
   return HFV_OK;
 }
@@ -1234,7 +1274,7 @@ int hfv_compute_fluxes(hfv_ctx* c, const hfv_timestep_info* info) {
   TRY(require_mesh(c));
   if (!info) return set_err(HFV_ERR_INVALID, "info is NULL");
   c->last_info = *info;
-  TRY(begin_phase_step(c, info));
+  TRY(begin_phase_step(c, info, false));
   Batch B = c->batch_args();
   hipLaunchKernelGGL(k_begin, dim3(1), dim3(64), 0, c->stream, B);
   hipLaunchKernelGGL(k_mask, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B, 2);
@@ -1276,6 +1316,7 @@ int hfv_get_fluxes(hfv_ctx* c, double* out7f) {
 
 int hfv_apply_fluxes(hfv_ctx* c) {
   TRY(require_mesh(c));
+  settle(c, -1);
   hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, c->batch_args(), 1);
   HIP_OK(hipGetLastError());
   c->fr_dirty = true;
@@ -1284,6 +1325,7 @@ int hfv_apply_fluxes(hfv_ctx* c) {
 
 int hfv_conserved_to_primitive(hfv_ctx* c) {
   TRY(require_mesh(c));
+  settle(c, -1);
   hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, c->batch_args(), 2);
   HIP_OK(hipGetLastError());
   c->fr_dirty = true;
@@ -1292,6 +1334,7 @@ int hfv_conserved_to_primitive(hfv_ctx* c) {
 
 int hfv_gradients(hfv_ctx* c) {
   TRY(require_mesh(c));
+  settle(c, -1);
   hipLaunchKernelGGL(k_apply, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, c->batch_args(), 4 | 8);
   launch_grad(c, c->batch_args(), 1, 0);
   HIP_OK(hipGetLastError());
@@ -1301,6 +1344,7 @@ int hfv_gradients(hfv_ctx* c) {
 
 int hfv_timesteps(hfv_ctx* c, uint64_t* ti_next) {
   TRY(require_mesh(c));
+  settle(c, -1);
   Batch B = c->batch_args();
   launch_grad(c, B, 0, 1);
   hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B);
@@ -1319,7 +1363,7 @@ int hfv_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {
   TRY(require_mesh(c));
   if (!info) return set_err(HFV_ERR_INVALID, "info is NULL");
   c->last_info = *info;
-  TRY(begin_phase_step(c, info));
+  TRY(begin_phase_step(c, info, true));
   pack_fr_if_dirty(c);
   enqueue_step(c, true);
   HIP_OK(hipGetLastError());

@@ -41,6 +41,7 @@ struct Cells {
   double *prim, *grad, *gc, *extrap, *cons, *facc, *vol, *loc, *cen, *v, *vrel, *maor, *msv, *dt, *srad;
   double* pc;
   double* tex;   // [5][N] time-extrapolation rate of the cell's prim/grad/v_rel (written when they change)
+  uint64_t* tiex;  // [N] FP32-flux contexts: the predict clock (SimEngine::ti_pred) at which extrap is due
   // extrap, tex and grad hold floats (same layouts) in FP32-flux contexts: ld_ex/st_ex/ld_tex/st_tex/
   // ld_grad/st_grad
   double* minv;  // [9][N] inverse of the gradient matrix (static geometry only)
@@ -115,7 +116,14 @@ struct SimEngine {
   long long step_count, cell_updates, face_fluxes;
   long long last_active, last_face_active;
   int err;
-  int pad;
+  // FP32-flux contexts defer the predict: a cell's extrapolations are extrap + (ti_pred - tiex) *
+  // time_base * (-rate), with the rate constant between the tag and now.  ti_pred advances by every
+  // engine step's predict interval (k_begin), not by the trait-level phases (no_pred), whose predict
+  // settles the sums itself.  chk: host-written primitives wait for predict's check_physical.
+  int chk;
+  int no_pred;
+  int pred_step;
+  uint64_t ti_pred;
 };
 
 // Batch-global counters (soft events; the first hard error of any simulation).
@@ -171,7 +179,9 @@ struct Batch {
 #define HFV_FR_EX HFV_FR_USED
 #define HFV_FR_LDS 35
 #define HFV_FR32_EX HFV_FR32_USED
+#define HFV_FR32_EL (HFV_FR32_EX + 5)  // the cell's unsummed predict time (float)
 #define HFV_FR32_LDS 37
+static_assert(HFV_FR32_EL < HFV_FR32_LDS, "staged FP32 record words");
 #define HFV_STAGE_MAX 128
 #define HFV_GSTAGE_MAX 600  // neighbour pc lines a gradient block stages (64 bytes each)
 
@@ -618,6 +628,9 @@ __global__ void k_begin(Batch B) {
     E.stepping = 1;
   }
   E.dt_pred = (double)(E.ti_current - E.ti_old) * E.time_base;
+  E.pred_step = !E.no_pred;
+  if (E.pred_step) E.ti_pred += E.ti_current - E.ti_old;
+  E.no_pred = 0;
   E.mab = get_max_active_bin(E.ti_current);
 }
 
@@ -674,6 +687,7 @@ __global__ void __launch_bounds__(HFV_END_BLOCK) k_end(Batch B) {
   }
   __syncthreads();
   if (threadIdx.x != 0) return;
+  if (E.pred_step) E.chk = 0;  // this step's k_mask has checked the host-written primitives
   na = 0;
   nfc = 0;
   tmin = ~0ull;
@@ -799,8 +813,11 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_drift(Batch B) {
 
 // ------------------------------------------------------------------ k_mask
 // Active mask (optimal_order.rs:17) and predict (euler_eqns_fvs.rs:20-24, part.rs:121-149) of every
-// cell; also the flux phase's per-cell inputs tex (time-extrapolation rate) and cs (sound speed).
-// mode 0: step; 1: predict only (dt from the engine's dt_pred); 2: mask only.
+// cell.  mode 0: step; 1: predict only (dt from the engine's dt_pred); 2: mask only.
+// FP32-flux contexts defer the step's predict (k_begin advanced ti_pred; the flux kernel's readers add
+// rate x elapsed time, face_side): check_physical runs only after host writes of the primitives (a no-op otherwise,
+// since c2p and the apply_flux reset leave physical states), settling the sums first.  Mode 1 settles
+// the sums and adds dt_pred at once.
 __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
   const BlockJob J = B.cjobs[blockIdx.x];
   if (J.sim < 0) return;
@@ -819,38 +836,41 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
       act = C.bin[i] <= E.mab;
       C.act[i] = act;
     }
-    St<double> prim = ld5(C.prim, n, i);
     const bool f32 = cfg.precision == HFV_FP32_FLUX;
-    bool reset = false;
-    if (mode != 2) {
+    if (mode != 2 && (!f32 || mode == 1 || E.chk)) {
+      St<double> prim = ld5(C.prim, n, i);
       // the rate of this prim/grad/v_rel, kept by k_grad (k_pack_fr after host changes);
       // extrapolations += time_extrapolations(dt) (its vacuum branch adds +0)
       const St<double> rate = prim.c[0] > 0. ? ld_tex(C, i, f32) : vacuum<double>();
-      St<double> te = prim.c[0] > 0. ? (-E.dt_pred) * rate : vacuum<double>();
-      st_ex(C, i, f32, ld_ex(C, i, f32) + te);
+      if (f32) {
+        const double dt = (double)(E.ti_pred - C.tiex[i]) * E.time_base + (mode == 1 ? E.dt_pred : 0.);
+        st_ex(C, i, true, ld_ex(C, i, true) + (prim.c[0] > 0. ? (-dt) * rate : vacuum<double>()));
+        C.tiex[i] = E.ti_pred;
+      } else {
+        St<double> te = prim.c[0] > 0. ? (-E.dt_pred) * rate : vacuum<double>();
+        st_ex(C, i, false, ld_ex(C, i, false) + te);
+      }
       check_physical(prim, ev);
       if (ev.check_physical) {
-        reset = true;
         st5(C.prim, n, i, prim);
         store_pc(C.pc, i, prim, ld3(C.cen, n, i));
+        if (B.sims[s].staged) {
+          // the flux record holds this rate already; a reset changes prim and the sound speed
+          double* r = C.fr + HFV_FR_W * (uint64_t)i;
+          if (f32) {
+            float* w = (float*)r;
+#pragma unroll
+            for (int q = 0; q < 5; ++q) w[HFV_FR32_PRIM + q] = (float)prim.c[q];
+            w[HFV_FR32_CS] = (float)fr_cs(prim, cfg.gd);
+          } else {
+#pragma unroll
+            for (int q = 0; q < 5; ++q) r[HFV_FR_PRIM + q] = prim.c[q];
+            r[HFV_FR_CS] = fr_cs(prim, cfg.gd);
+          }
+        }
       }
       // volume += dvdt * dt clamped to [V/2, 2V]: dvdt is never set on the path (part.rs:130-131)
     }
-    if (reset && B.sims[s].staged) {
-      // the flux record holds this rate already; a reset changes prim and the sound speed
-      double* r = C.fr + HFV_FR_W * (uint64_t)i;
-      if (cfg.precision == HFV_FP32_FLUX) {
-        float* w = (float*)r;
-#pragma unroll
-        for (int q = 0; q < 5; ++q) w[HFV_FR32_PRIM + q] = (float)prim.c[q];
-        w[HFV_FR32_CS] = (float)fr_cs(prim, cfg.gd);
-      } else {
-#pragma unroll
-        for (int q = 0; q < 5; ++q) r[HFV_FR_PRIM + q] = prim.c[q];
-        r[HFV_FR_CS] = fr_cs(prim, cfg.gd);
-      }
-    }
-
   }
   if (mode != 1) {
     uint32_t cnt = block_sum_u32(act ? 1u : 0u);
@@ -940,9 +960,13 @@ template <class T>
 struct SoaCell {
   const Cells* C;
   const Gas<double>* g;
+  const SimEngine* E;
   uint32_t i;
-  __device__ __forceinline__ SoaCell(const Cells& c, const Gas<double>& gd, int32_t idx) : C(&c), g(&gd), i((uint32_t)idx) {}
+  __device__ __forceinline__ SoaCell(const Cells& c, const Gas<double>& gd, const SimEngine& e, int32_t idx)
+      : C(&c), g(&gd), E(&e), i((uint32_t)idx) {}
   __device__ __forceinline__ double dt() const { return ldo(C->dt, i); }
+  // predict time not yet summed into the stored extrapolations (FP32-flux contexts)
+  __device__ __forceinline__ double el() const { return (double)(E->ti_pred - C->tiex[i]) * E->time_base; }
   __device__ __forceinline__ St<double> prim() const {
     const St<double> p = ld5(C->prim, C->n, i);
     return mkst(rnd<T>(p.c[0]), mk3(rnd<T>(p.c[1]), rnd<T>(p.c[2]), rnd<T>(p.c[3])), rnd<T>(p.c[4]));
@@ -991,6 +1015,7 @@ struct RecCell {
     return mk3(r[HFV_FR_GRAD + 3 * q], r[HFV_FR_GRAD + 3 * q + 1], r[HFV_FR_GRAD + 3 * q + 2]);
   }
   __device__ __forceinline__ double ex(int q) const { return r[HFV_FR_EX + q]; }
+  __device__ __forceinline__ double el() const { return 0.; }  // FP64 contexts sum every step
   __device__ __forceinline__ St<double> rate() const {
     return mkst(r[HFV_FR_TEX], mk3(r[HFV_FR_TEX + 1], r[HFV_FR_TEX + 2], r[HFV_FR_TEX + 3]), r[HFV_FR_TEX + 4]);
   }
@@ -1019,6 +1044,7 @@ struct RecCellF {
     return mk3((double)r[HFV_FR32_GRAD + 3 * q], (double)r[HFV_FR32_GRAD + 3 * q + 1], (double)r[HFV_FR32_GRAD + 3 * q + 2]);
   }
   __device__ __forceinline__ double ex(int q) const { return r[HFV_FR32_EX + q]; }
+  __device__ __forceinline__ double el() const { return r[HFV_FR32_EL]; }
   __device__ __forceinline__ St<double> rate() const {
     return mkst((double)r[HFV_FR32_TEX], mk3((double)r[HFV_FR32_TEX + 1], (double)r[HFV_FR32_TEX + 2],
                 (double)r[HFV_FR32_TEX + 3]), (double)r[HFV_FR32_TEX + 4]);
@@ -1031,7 +1057,9 @@ struct RecCellF {
   __device__ __forceinline__ V3<double> v() const { return ld3(C->v, C->n, i); }
 };
 // One side's reconstructed state at the face before limiting (flux.rs:52-73):
-// ((W + grad . dx) + extrapolations) + time_extrapolations(-dt/2), in T.
+// ((W + grad . dx) + extrapolations) + time_extrapolations(-dt/2), in T.  FP32-flux contexts keep the
+// steps' predicts since the cell's tag unsummed (el): the rate is constant over them, so the
+// extrapolations' pending part and the half step back (dte = -dt/2) share one product, rate x -(dte + el).
 template <class T, class Acc>
 __device__ __forceinline__ St<T> face_side(const Acc& X, const St<T>& pT, V3<T> dxT, T dte) {
   St<T> out;
@@ -1039,7 +1067,8 @@ __device__ __forceinline__ St<T> face_side(const Acc& X, const St<T>& pT, V3<T> 
   for (int q = 0; q < 5; ++q) out.c[q] = (pT.c[q] + dot(cvt3<T>(X.grad(q)), dxT)) + (T)X.ex(q);
   if (pT.c[0] > T(0)) {
     const St<T> rate = cvtst<T>(X.rate());
-    out = out + (-dte) * rate;
+    if constexpr (sizeof(T) == 4) out = out + (-(dte + (T)X.el())) * rate;
+    else out = out + (-dte) * rate;
   } else {
     out = out + vacuum<T>();
   }
@@ -1249,6 +1278,8 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
       static_assert(HFV_STAGE_MAX * RD2 % HFV_BLOCK == 0, "record slots per thread");
       double2 rv[NR];
       double xv[NE];
+      uint64_t tv = 0;  // FP32: the staged cell's extrapolation tag
+      if (F32 && (int)threadIdx.x < ns) tv = B.C.tiex[(uint32_t)myc];
 #pragma unroll
       for (int j = 0; j < NR; ++j) {
         const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / RD2, q = k % RD2;
@@ -1281,6 +1312,10 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
           const int k = (int)threadIdx.x + j * HFV_BLOCK, c = k / HFV_EX_W, q = k - HFV_EX_W * c;
           if (c < ns) ldsf[c * HFV_FR32_LDS + HFV_FR32_EX + q] = (float)xv[j];
         }
+        if ((int)threadIdx.x < ns) {
+          const SimEngine& E = B.eng[s];
+          ldsf[threadIdx.x * HFV_FR32_LDS + HFV_FR32_EL] = (float)((double)(E.ti_pred - tv) * E.time_base);
+        }
       } else {
 #pragma unroll
         for (int j = 0; j < NR; ++j) {
@@ -1311,12 +1346,12 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
       }
       __syncthreads();  // the staged records are dead from here on
     } else if (cur) {
-      computed = face_flux<T, SOLVER, DEFER, true, SoaCell<T>>(B, cfg, f, SoaCell<T>(B.C, cfg.gd, l),
-                                                               SoaCell<T>(B.C, cfg.gd, r), r, la, ra, outv, ev);
+      computed = face_flux<T, SOLVER, DEFER, true, SoaCell<T>>(B, cfg, f, SoaCell<T>(B.C, cfg.gd, B.eng[s], l),
+                                                               SoaCell<T>(B.C, cfg.gd, B.eng[s], r), r, la, ra, outv, ev);
     }
   } else if (cur) {
-    computed = face_flux<T, SOLVER, DEFER, false, SoaCell<T>>(B, cfg, f, SoaCell<T>(B.C, cfg.gd, l),
-                                                              SoaCell<T>(B.C, cfg.gd, r), r, la, ra, outv, ev);
+    computed = face_flux<T, SOLVER, DEFER, false, SoaCell<T>>(B, cfg, f, SoaCell<T>(B.C, cfg.gd, B.eng[s], l),
+                                                              SoaCell<T>(B.C, cfg.gd, B.eng[s], r), r, la, ra, outv, ev);
   }
   // records of faces with an active cell (deferred ones zero here, overwritten by k_flux_fallback)
   curs[threadIdx.x] = cur ? 1 : 0;
@@ -1374,8 +1409,8 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_flux_fallback(Batch B) {
     Events ev;
     double outv[7];
     const Cfg& cfg = B.sims[s].cfg;
-    face_flux<T, SOLVER_AIRS, false, false, SoaCell<T>>(B, cfg, f, SoaCell<T>(B.C, cfg.gd, l),
-                                                        SoaCell<T>(B.C, cfg.gd, r), r, la, ra, outv, ev);
+    face_flux<T, SOLVER_AIRS, false, false, SoaCell<T>>(B, cfg, f, SoaCell<T>(B.C, cfg.gd, B.eng[s], l),
+                                                        SoaCell<T>(B.C, cfg.gd, B.eng[s], r), r, la, ra, outv, ev);
     store_rec(B.F.rec, f, outv, sizeof(T) == 4);
     flush_events(ev, B.cnt, B.eng, s);
   }
@@ -1562,6 +1597,25 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_pack_fr(Batch B) {
   st_tex(C, i, cfg.precision == HFV_FP32_FLUX, rate);
   if (B.sims[J.sim].staged)
     store_fr(C.fr, i, cfg.precision == HFV_FP32_FLUX, p, rate, fr_cs(p, gd), g, C.dt[i], ld3(C.gc, n, i));
+}
+
+// FP32-flux contexts: sum the unsummed predict time of the cells of simulation s (every loaded
+// simulation for s < 0) into their extrapolations, with the rate of the current primitives, gradients
+// and v_rel, before anything changes those (host writes, trait-level phases); flags the primitives for
+// predict's check_physical.
+__global__ void __launch_bounds__(HFV_BLOCK) k_settle(Batch B, int s) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0 || (s >= 0 && J.sim != s)) return;
+  if (B.sims[J.sim].cfg.precision != HFV_FP32_FLUX) return;
+  SimEngine& E = B.eng[J.sim];
+  if (threadIdx.x == 0) E.chk = 1;
+  const int64_t i = (int64_t)J.lo + threadIdx.x;
+  if (i >= J.hi) return;
+  const Cells& C = B.C;
+  const double rho = C.prim[i];
+  const double dt = (double)(E.ti_pred - C.tiex[i]) * E.time_base;
+  st_ex(C, i, true, ld_ex(C, i, true) + (rho > 0. ? (-dt) * ld_tex(C, i, true) : vacuum<double>()));
+  C.tiex[i] = E.ti_pred;
 }
 
 // glam::DMat3 as three columns
@@ -2152,6 +2206,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_limit(Batch B, int canon) {
   }
   if (C.act[i]) {
     st_ex(C, i, cfg.precision == HFV_FP32_FLUX, vacuum<double>());
+    if (cfg.precision == HFV_FP32_FLUX) C.tiex[i] = E.ti_pred;  // the sum restarts at zero now
     if (cfg.motion != MOT_FIXED) {
       double rho = C.prim[i];
       if (rho > 0.) {

@@ -65,7 +65,11 @@ extern "C" {
 #define HFV_IDEAL 0
 #define HFV_ISOTHERMAL 1
 /* Arithmetic of the face reconstruction and Riemann solve. Accumulation, conserved->primitive and
- * the exact solver's pressure iteration stay FP64 in both modes. */
+ * the exact solver's pressure iteration stay FP64 in both modes.  FP32_FLUX contexts store gradients,
+ * extrapolations and face records as floats and sum the engine steps' predicts where they are read
+ * (rate x elapsed time): hfv_get_state reports the sums due at the current time, hfv_set_state and
+ * the trait-level phases sum the pending part first, so results differ from per-step sums by FP32
+ * rounding only. */
 #define HFV_FP64 0
 #define HFV_FP32_FLUX 1
 

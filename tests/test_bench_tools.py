@@ -77,15 +77,18 @@ def test_trace_summary_window_column(tmp_path, profiles):
 
 def test_fp32_byte_model_uses_stored_widths():
     n_act, f_act, n_all = 1000.0, 3000.0, 5000.0
-    # gradients 120 -> 60, extrapolations 40 -> 20, records 56 -> 28 bytes
+    # gradients 120 -> 60, extrapolations 40 -> 20, records 56 -> 28 bytes; the deferred predict adds an
+    # 8-byte extrapolation tag per cell read by the flux kernel and written by the reset
     assert bench.ALGO["flux"](n_act, f_act, n_all) - bench.ALGO_FP32["flux"](n_act, f_act, n_all) == \
-        (60 + 20) * n_act + 28 * f_act
+        (60 + 20 - 8) * n_act + 28 * f_act
     assert bench.ALGO["apply_c2p"](n_act, f_act, n_all) - bench.ALGO_FP32["apply_c2p"](n_act, f_act, n_all) == \
         2 * 28 * f_act
     assert bench.ALGO["grad_timestep"](n_act, f_act, n_all) - bench.ALGO_FP32["grad_timestep"](n_act, f_act, n_all) \
         == 60 * n_act
-    assert bench.ALGO["mask_predict"](n_act, f_act, n_all) - bench.ALGO_FP32["mask_predict"](n_act, f_act, n_all) \
-        == (36 + 40) * n_all
+    # the deferred predict: time bin and active flag only
+    assert bench.ALGO_FP32["mask_predict"](n_act, f_act, n_all) == 2 * n_all
+    assert bench.ALGO["limit_kick"](n_act, f_act, n_all) - bench.ALGO_FP32["limit_kick"](n_act, f_act, n_all) \
+        == (20 - 8) * n_act
 
 
 def test_tuning_from_report(tmp_path):
