@@ -404,19 +404,22 @@ def cpu_baseline_ensemble(probs, stride, warmup, budget_s):
 
 # ------------------------------------------------------------------ single simulations
 def run_device(prob, warmup, steps, profile=False, tune=0, tuning_from=None):
-    """tuning_from: (classes, launches) of an earlier run's autotuner, applied instead of tuning, with
-    its launches added to the warmup."""
+    """W untimed steps, then K timed ones, of a fresh context.  tune > 0: the autotuner runs on a
+    separate context first and its choice is applied, so the timed window is the same steps whatever
+    the tuner's length; tuning_from: (classes, launches) of an earlier run's tuner, applied the same way."""
     from hydro_rs_amd import ics
 
-    dev = ics.build_device_sim(prob)
-    s0 = dev.engine_info()["step_count"]
-    tuning = None
+    tuning, tune_launches = None, 0
     if tuning_from:
-        apply_tuning(dev, [(0, prob.cfg.dim, prob.n)], tuning_from[0])
-        tuning, warmup = tuning_from[0], warmup + tuning_from[1]
+        tuning, tune_launches = tuning_from
     elif tune > 0:
-        tuning = dev.autotune(launches=tune, rounds=1)
-    tune_launches = tuning_from[1] if tuning_from else dev.engine_info()["step_count"] - s0
+        probe = ics.build_device_sim(prob)
+        tuning = probe.autotune(launches=tune, rounds=1)
+        tune_launches = probe.engine_info()["step_count"]
+        probe.close()
+    dev = ics.build_device_sim(prob)
+    if tuning:
+        apply_tuning(dev, [(0, prob.cfg.dim, prob.n)], tuning)
     dev.run(warmup)
     dev.synchronize()
     e0 = dev.engine_info()
@@ -540,7 +543,7 @@ def main():
     ap.add_argument("--sedov-side", type=int, default=256)
     ap.add_argument("--kh-side", type=int, default=128)
     ap.add_argument("--kh-steps", type=int, default=20)
-    ap.add_argument("--kh-warmup", type=int, default=3)
+    ap.add_argument("--kh-warmup", type=int, default=80)
     ap.add_argument("--serve-jobs", type=int, default=48)
     ap.add_argument("--serve-rate", type=float, default=6.0)
     ap.add_argument("--serve-side-max", type=int, default=512)
@@ -564,8 +567,9 @@ def main():
     ap.add_argument("--window", type=int, default=0,
                     help="summaries: also/only each kernel's last N dispatches (the run's timed window)")
     ap.add_argument("--tuning-from", default=None,
-                    help="apply the autotuner choices of an earlier run's JSON line (and run its tuning launches "
-                         "as warmup) instead of tuning: same timed window, no tuning phase (profiling runs)")
+                    help="apply the autotuner choices of an earlier run's JSON line instead of tuning (the "
+                         "ensemble also runs its tuning launches as warmup): same timed window, no tuning phase "
+                         "(profiling runs)")
     ap.add_argument("--no-gate", action="store_true",
                     help="ensemble: time the window only (no FP64 gate batch, no plain path); for profiling")
     args = ap.parse_args()
