@@ -403,10 +403,11 @@ def cpu_baseline_ensemble(probs, stride, warmup, budget_s):
 
 
 # ------------------------------------------------------------------ single simulations
-def run_device(prob, warmup, steps, profile=False, tune=0, tuning_from=None):
-    """W untimed steps, then K timed ones, of a fresh context.  tune > 0: the autotuner runs on a
-    separate context first and its choice is applied, so the timed window is the same steps whatever
-    the tuner's length; tuning_from: (classes, launches) of an earlier run's tuner, applied the same way."""
+def run_device(prob, warmup, steps, profile=False, tune=0, tuning_from=None, tune_rounds=1):
+    """W untimed steps, then K timed ones, of a fresh context.  tune > 0: the autotuner runs first on a
+    separate context, after the same W steps (so it samples the phase of the run that is timed), and
+    its choice is applied, so the timed window is the same steps whatever the tuner's length;
+    tuning_from: (classes, launches) of an earlier run's tuner, applied the same way."""
     from hydro_rs_amd import ics
 
     tuning, tune_launches = None, 0
@@ -414,8 +415,9 @@ def run_device(prob, warmup, steps, profile=False, tune=0, tuning_from=None):
         tuning, tune_launches = tuning_from
     elif tune > 0:
         probe = ics.build_device_sim(prob)
-        tuning = probe.autotune(launches=tune, rounds=1)
-        tune_launches = probe.engine_info()["step_count"]
+        probe.run(warmup)
+        tuning = probe.autotune(launches=tune, rounds=tune_rounds)
+        tune_launches = probe.engine_info()["step_count"] - warmup
         probe.close()
     dev = ics.build_device_sim(prob)
     if tuning:
@@ -472,7 +474,7 @@ def bench_kh3d(args):
     for k, prec in (("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)):
         tf = (rep[k]["autotune"], rep[k]["tune_launches"]) if rep else None
         d, s = run_device(ics.with_precision(prob, prec), args.kh_warmup, args.kh_steps, profile=True,
-                          tune=args.kh_tune, tuning_from=tf)
+                          tune=args.kh_tune, tuning_from=tf, tune_rounds=args.kh_tune_rounds)
         rl = roofline(s["kernel_times"], s["steps"], s["cell_updates"], s["face_fluxes"], prob.n,
                       fp32=prec == fvs.FP32_FLUX)
         attach_traffic(rl, f"kh3d_{args.kh_side}", prec == fvs.FP32_FLUX, fvs.AIRS, s["autotune"])
@@ -540,6 +542,8 @@ def main():
                     help="ensemble autotuner: launches per timed group of a candidate trial (0: no tuning)")
     ap.add_argument("--tune-rounds", type=int, default=1)
     ap.add_argument("--kh-tune", type=int, default=8, help="3D workload autotuner: launches per timed group of a trial (0: none)")
+    ap.add_argument("--kh-tune-rounds", type=int, default=2,
+                    help="3D workload autotuner: rounds over the candidates (each samples another part of the time-bin cycle)")
     ap.add_argument("--sedov-side", type=int, default=256)
     ap.add_argument("--kh-side", type=int, default=128)
     ap.add_argument("--kh-steps", type=int, default=20)

@@ -1583,6 +1583,7 @@ int hfv_get_tuning(hfv_ctx* c, int slot, int* staged, int* static_grad, int* gst
 
 // Size classes of the autotuner: dimension x cell-count bucket.
 static const int64_t kTuneEdges[] = {0, 1024, 8192, 65536, INT64_MAX};
+static const double kTuneTie = 0.01;
 
 int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int max_out, int* n_out) {
   if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
@@ -1686,7 +1687,9 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
         k.best = cand;
         best = k.t[cand] / k.u[cand];
       }
-    if (k.best < 0) k.best = (k.can_stage ? 2 : 0) | (k.dim == 3 ? 1 : 0) | (k.dim == 3 && k.can_gstage ? 4 : 0);
+    // the default stays unless beaten by more than the measurement's own spread
+    const int def = (k.can_stage ? 2 : 0) | (k.dim == 3 ? 1 : 0) | (k.dim == 3 && k.can_gstage ? 4 : 0);
+    if (k.best < 0 || (k.u[def] > 0. && k.t[def] / k.u[def] <= best * (1. + kTuneTie))) k.best = def;
     if ((rc = apply(k, k.best)) != HFV_OK) break;
   }
   c->prof = prof;

@@ -266,7 +266,9 @@ int hfv_get_geometry_mode(hfv_ctx* ctx, int slot, int* is_static);
  * access (each cell chunk's neighbour lines staged through LDS, or gathered) by timing real launches
  * of each candidate in turn and keeping the fastest per cell update: per round and candidate, one
  * untimed launch, then groups of `launches` timed launches until they hold `launches` sweeps of the
- * live cells or 4 groups have run.  Every
+ * live cells or 4 groups have run.  A class keeps its default (staged tiling; precomputed geometry and
+ * staged neighbours in 3D only) unless the fastest candidate beats it by more than 1%, the size of the
+ * differences that the launches' changing mix of active cells produces on its own.  Every
  * candidate gives bit-identical results, so the tuning launches are ordinary steps of the run.  No
  * counterpart in the reference (it has no device kernels to tune).  ns_per_update[c]: measured cost of
  * candidate c = static_grad + 2 staged + 4 gstaged; -1 where not tried.  out may hold max_out classes;
