@@ -1663,7 +1663,7 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
         if ((all_done = finished())) break;
         // groups of `launches` launches until the sample holds at least `launches` sweeps of the live
         // cells (at most 4 groups): a run whose steps alternate between many and few active cells
-        // would otherwise be judged on whichever steps its launches happened to hit
+        // would otherwise be measured on whichever steps its launches happened to hit
         const long long u0 = updates(), want = (long long)launches * live_cells();
         double dt = 0.;
         for (int g = 0; g < 4; ++g) {

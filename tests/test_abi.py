@@ -32,7 +32,7 @@ def test_bindings_cover_header():
 
 
 def test_integration_guide_declares_every_entry_point():
-    """INTEGRATION.md's Rust extern blocks declare exactly the entry points of include/hfv.h."""
+    """The integration guide's Rust extern blocks declare exactly the entry points of include/hfv.h."""
     guide = open(os.path.join(ROOT, "INTEGRATION.md")).read()
     rust = set(re.findall(r"pub fn (hfv_[a-z0-9_]+)\(", guide))
     assert rust == set(declared("hfv.h"))
