@@ -266,8 +266,8 @@ int hfv_get_geometry_mode(hfv_ctx* ctx, int slot, int* is_static);
  * access (each cell chunk's neighbour lines staged through LDS, or gathered) by timing real launches
  * of each candidate in turn and keeping the fastest per cell update: per round and candidate, one
  * untimed launch, then groups of `launches` timed launches until they hold `launches` sweeps of the
- * live cells or 4 groups have run.  A class keeps its default (staged tiling; precomputed geometry and
- * staged neighbours in 3D only) unless the fastest candidate beats it by more than 1%, the size of the
+ * live cells or 4 groups have run.  A class keeps its default (staged tiling in 2D and 3D; precomputed
+ * geometry and staged neighbours in 3D) unless the fastest candidate beats it by more than 1%, the size of the
  * differences that the launches' changing mix of active cells produces on its own.  Every
  * candidate gives bit-identical results, so the tuning launches are ordinary steps of the run.  No
  * counterpart in the reference (it has no device kernels to tune).  ns_per_update[c]: measured cost of
@@ -284,8 +284,8 @@ typedef struct {
   double ns_per_update[8];
 } hfv_tune_class;
 int hfv_autotune(hfv_ctx* ctx, int launches, int rounds, hfv_tune_class* out, int max_out, int* n_out);
-/* The same choices for one slot by hand: staged 1/0 (-1: allowed, the default), static_grad and
- * gstaged 1/0 (-1: by dimension, on for 3D, the default).  They take effect for static-geometry
+/* The same choices for one slot by hand: staged 1/0 (-1: by dimension, on for 2D and 3D, the
+ * default), static_grad and gstaged 1/0 (-1: by dimension, on for 3D, the default).  They take effect for static-geometry
  * simulations only; hfv_get_tuning reports what a slot currently runs. */
 int hfv_set_tuning(hfv_ctx* ctx, int slot, int staged, int static_grad, int gstaged);
 int hfv_get_tuning(hfv_ctx* ctx, int slot, int* staged, int* static_grad, int* gstaged);

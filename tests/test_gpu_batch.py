@@ -234,11 +234,12 @@ def test_autotune_is_bit_identical():
         tried = [x for x in r["ns_per_update"] if x >= 0]
         assert len(tried) >= 2, r
         # the choice follows from the measured costs: the fastest candidate, unless the class's default
-        # (staged tiling where the class can stage; precomputed geometry and staged neighbours in 3D) is
-        # within 1% of it
+        # (staged tiling in 2D/3D where the class can stage; precomputed geometry and staged neighbours
+        # in 3D) is within 1% of it
         ns = r["ns_per_update"]
         can_stage, can_gstage = ns[2] >= 0, ns[4] >= 0
-        default = (2 if can_stage else 0) | (1 if r["dim"] == 3 else 0) | (4 if r["dim"] == 3 and can_gstage else 0)
+        default = (2 if can_stage and r["dim"] != 1 else 0) | (1 if r["dim"] == 3 else 0) | \
+            (4 if r["dim"] == 3 and can_gstage else 0)
         best = min((c for c in range(8) if ns[c] >= 0), key=lambda c: ns[c])
         chosen = r["static_grad"] + 2 * r["staged"] + 4 * r["gstaged"]
         expect = default if 0 <= ns[default] <= ns[best] * 1.01 else best
