@@ -7,9 +7,10 @@ counts log-uniform in [200, 65536], HLLC, FP32 flux / FP64 accumulate) in one de
 step is one batch launch: every unfinished member advances by one engine step (one
 OptimalOrderRunner::step with its own hierarchical timebins).  The autotuner first times a few real
 launches per size class and candidate (flux tiling, gradient geometry; bit-identical candidates), then
-W untimed launches, then K timed launches bracketed by device synchronisation; value = cell updates
-(active cells summed over members and launches) / wall time of the K launches.  The reported warmup
-counts the tuning launches too.
+W untimed launches (more if needed for the window to start at launch 260, whatever the tuner's
+length), then K timed launches bracketed by device synchronisation; value = cell updates (active
+cells summed over members and launches) / wall time of the K launches.  The reported warmup counts
+the tuning launches too.
 
 Printed JSON line: the contract keys plus
   speedup_vs_plain_fp64  value / the plain single-simulation FP64 device path on the same members and
@@ -268,14 +269,15 @@ def bench_ensemble(args):
     tune_launches = 0
     tuning = None
     warmup = args.warmup
+    # the timed window starts at launch max(window_start, tuning + W), whatever the tuner's length
     if args.tuning_from:
-        # the earlier run's choices, and as many extra warmup launches as its tuner ran: every candidate
-        # gives the same bits, so the timed window holds the same steps as that run's
+        # the earlier run's choices, and its tuner's launches as warmup: every candidate gives the same
+        # bits, so the timed window holds the same steps as that run's
         rep = load_report(args.tuning_from)["autotune"]
         apply_tuning(b, [(s, p.cfg.dim, p.n) for s, p in enumerate(probs)], rep["classes"])
-        warmup += rep["launches"]
         tune_launches = rep["launches"]
         tuning = rep["classes"]
+        warmup = max(args.window_start, tune_launches + args.warmup)
         log(f"[ensemble] tuning from {args.tuning_from}; {warmup} warmup launches")
     elif args.tune_launches > 0:
         s0 = [b.engine_info(s)["step_count"] for s in range(len(probs))]
@@ -283,7 +285,9 @@ def bench_ensemble(args):
         tuning = b.autotune(launches=args.tune_launches, rounds=args.tune_rounds)
         tune_launches = max(b.engine_info(s)["step_count"] - s0[s] for s in range(len(probs)))
         log(f"[ensemble] autotune: {tune_launches} launches in {time.perf_counter() - t:.1f}s: "
-            f"{[(c['dim'], c['cells'], c['staged'], c['static_grad']) for c in tuning]}")
+            f"{[(c['dim'], c['cells'], c['staged'], c['static_grad'], c['gstaged']) for c in tuning]}")
+        warmup = max(args.window_start - tune_launches, args.warmup)
+    start = warmup if args.tuning_from else tune_launches + warmup  # launches before the window
     b.run(warmup)
     b.synchronize()
     i0 = [b.engine_info(s) for s in range(len(probs))]
@@ -299,7 +303,7 @@ def bench_ensemble(args):
     value = cu / secs
     log(f"[ensemble] fp32: {launches} launches in {secs:.3f}s -> {value:.4e} cell-updates/s")
     res = {"launches": launches, "seconds": secs, "cell_updates": cu, "face_fluxes": ff, "value": value,
-           "autotune": {"launches": tune_launches, "classes": tuning},
+           "window_start": start, "autotune": {"launches": tune_launches, "classes": tuning},
            "n_all": n_all, "members_done": sum(1 for i in i1 if i["done"]),
            "members_failed": [s for s, i in enumerate(i1) if i["error_code"]],
            "roofline": roofline(kt, launches, cu, ff, n_all, fp32=True)}
@@ -312,7 +316,7 @@ def bench_ensemble(args):
     p64 = [ics.with_precision(p, fvs.FP64) for p in probs]
     b64 = fvs.Batch.for_problems(p64)
     t1 = time.perf_counter()
-    b64.run(tune_launches + args.warmup + launches)
+    b64.run(start + launches)
     b64.synchronize()
     res["fp64_batched_value"] = sum(b64.engine_info(s)["cell_updates"] for s in range(len(p64))) / (time.perf_counter() - t1)
     l1s, same_t = [], 0
@@ -334,7 +338,7 @@ def bench_ensemble(args):
     b.close()
     # plain single-simulation FP64 path on a sample of members, same windows
     idx = list(range(0, len(probs), args.plain_stride))
-    st = ensemble.run_plain([p64[i] for i in idx], max_steps=args.steps, warmup=tune_launches + args.warmup)
+    st = ensemble.run_plain([p64[i] for i in idx], max_steps=args.steps, warmup=start)
     res["plain_fp64"] = {"value": st["cell_updates"] / st["seconds"], "members": idx, "steps": st["steps"],
                          "seconds": st["seconds"], "failed": st["failed"]}
     res["speedup_vs_plain_fp64"] = value / res["plain_fp64"]["value"]
@@ -535,6 +539,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000, help="timed batch launches of the ensemble")
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--window-start", type=int, default=260,
+                    help="ensemble: untimed launches (tuning included) before the timed window, at least")
     ap.add_argument("--workload", default="all", choices=["all", "ensemble", "sedov", "kh3d", "serve"])
     ap.add_argument("--members", type=int, default=512)
     ap.add_argument("--plain-stride", type=int, default=16)
@@ -611,7 +617,7 @@ def main():
     if w in ("all", "ensemble"):
         ens, probs = bench_ensemble(args)
         out.update({
-            "value": ens["value"], "steps": ens["launches"], "warmup": args.warmup + ens["autotune"]["launches"],
+            "value": ens["value"], "steps": ens["launches"], "warmup": ens["window_start"],
             "ms_per_step": 1e3 * ens["seconds"] / max(ens["launches"], 1), "dtype": "f32 flux / f64 accumulate",
             "config": {"workload": f"ensemble{args.members}_riemann_1d2d_hllc_fp32flux", "members": args.members,
                        "cells": ens["n_all"], "cell_range": [200, 65536], "t_end": 0.05,
@@ -625,8 +631,8 @@ def main():
         })
         if not args.no_cpu:
             log(f"[bench] cpu baseline (oracle), budget {args.cpu_seconds}s")
-            out["cpu_baseline"] = cpu_baseline_ensemble(probs, args.plain_stride,
-                                                        ens["autotune"]["launches"] + args.warmup, args.cpu_seconds)
+            out["cpu_baseline"] = cpu_baseline_ensemble(probs, args.plain_stride, ens["window_start"],
+                                                        args.cpu_seconds)
             log(f"[bench] cpu: {out['cpu_baseline']}")
         del probs
     if w in ("all", "sedov"):

@@ -1616,7 +1616,9 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
     }
     cls[k].slots.push_back(s);
     if (c->has_stage[s]) cls[k].can_stage = true;
-    if (c->has_gstage[s]) cls[k].can_gstage = true;
+    // staged gradient neighbours are a 3D candidate only: in 1D and 2D each neighbour line is read
+    // 2-6 times and L2 already holds it (timed windows of the ensemble: 0.5-1.9% slower per class)
+    if (c->has_gstage[s] && dim == 3) cls[k].can_gstage = true;
   }
   auto apply = [&](const Cls& k, int cand) -> int {
     for (int s : k.slots) {

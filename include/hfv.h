@@ -262,8 +262,8 @@ int hfv_get_geometry_mode(hfv_ctx* ctx, int slot, int* is_static);
 /* Per size class of the live static-geometry simulations (one dimension, cell count in
  * [cells_lo, cells_hi); buckets 0 / 1024 / 8192 / 65536), hfv_autotune chooses the flux kernel's
  * tiling (its chunks' cells staged through LDS, or each face gathering its cells), the gradient
- * geometry (precomputed per CSR entry, or evaluated per step) and the gradient sweeps' neighbour
- * access (each cell chunk's neighbour lines staged through LDS, or gathered) by timing real launches
+ * geometry (precomputed per CSR entry, or evaluated per step) and, in 3D, the gradient sweeps'
+ * neighbour access (each cell chunk's neighbour lines staged through LDS, or gathered) by timing real launches
  * of each candidate in turn and keeping the fastest per cell update: per round and candidate, one
  * untimed launch, then groups of `launches` timed launches until they hold `launches` sweeps of the
  * live cells or 4 groups have run.  A class keeps its default (staged tiling in 2D and 3D; precomputed
