@@ -257,6 +257,14 @@ def apply_tuning(sim, members, classes):
 
 
 # ------------------------------------------------------------------ ensemble (headline)
+def window_warmup(window_start, warmup, tune_launches, replay):
+    """Untimed launches to run after tuning (or, replaying an earlier run's tuning, from launch 0) so
+    that the timed window starts at launch max(window_start, tune_launches + warmup); returns
+    (launches to run, window start)."""
+    start = max(window_start, tune_launches + warmup)
+    return (start if replay else start - tune_launches), start
+
+
 def bench_ensemble(args):
     from hydro_rs_amd import ensemble, fvs, ics
 
@@ -277,7 +285,7 @@ def bench_ensemble(args):
         apply_tuning(b, [(s, p.cfg.dim, p.n) for s, p in enumerate(probs)], rep["classes"])
         tune_launches = rep["launches"]
         tuning = rep["classes"]
-        warmup = max(args.window_start, tune_launches + args.warmup)
+        warmup, start = window_warmup(args.window_start, args.warmup, tune_launches, True)
         log(f"[ensemble] tuning from {args.tuning_from}; {warmup} warmup launches")
     elif args.tune_launches > 0:
         s0 = [b.engine_info(s)["step_count"] for s in range(len(probs))]
@@ -286,8 +294,9 @@ def bench_ensemble(args):
         tune_launches = max(b.engine_info(s)["step_count"] - s0[s] for s in range(len(probs)))
         log(f"[ensemble] autotune: {tune_launches} launches in {time.perf_counter() - t:.1f}s: "
             f"{[(c['dim'], c['cells'], c['staged'], c['static_grad'], c['gstaged']) for c in tuning]}")
-        warmup = max(args.window_start - tune_launches, args.warmup)
-    start = warmup if args.tuning_from else tune_launches + warmup  # launches before the window
+        warmup, start = window_warmup(args.window_start, args.warmup, tune_launches, False)
+    else:
+        warmup, start = window_warmup(args.window_start, args.warmup, 0, False)
     b.run(warmup)
     b.synchronize()
     i0 = [b.engine_info(s) for s in range(len(probs))]

@@ -111,3 +111,14 @@ def test_tuning_from_report(tmp_path):
     # slot 2 (2D, 900 cells) falls in no class and keeps its defaults
     bench.apply_tuning(sim, [(0, 1, 1023), (1, 2, 1024), (2, 2, 900), (3, 3, 2097152)], rep["autotune"]["classes"])
     assert sim.calls == [(0, 0, 0, 1), (1, 1, 0, 0), (3, 1, 1, 1)]
+
+
+def test_ensemble_window_start_does_not_depend_on_the_tuner():
+    # the tuner ran 120 or 240 launches: the window still starts at launch 260
+    assert bench.window_warmup(260, 20, 120, False) == (140, 260)
+    assert bench.window_warmup(260, 20, 240, False) == (20, 260)
+    # a replay of that run's tuning runs no tuner: every launch before the window is warmup
+    assert bench.window_warmup(260, 20, 120, True) == (260, 260)
+    # a long tuner pushes the window back, with W launches after it
+    assert bench.window_warmup(260, 20, 300, False) == (20, 320)
+    assert bench.window_warmup(260, 20, 300, True) == (320, 320)
