@@ -130,6 +130,18 @@ def lib():
         L.ora_phase_compute_fluxes.restype = None
         L.ora_sim_conservation.argtypes = [C.c_void_p, DP]
         L.ora_sim_conservation.restype = None
+        L.ora_phase_gradients.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
+        L.ora_phase_gradients.restype = None
+        L.ora_t_c2p.argtypes = [C.c_double, DP, C.c_double, DP]
+        L.ora_t_time_extrapolations.argtypes = [C.c_double, DP, DP, DP, C.c_double, DP]
+        L.ora_t_time_extrapolations.restype = None
+        L.ora_t_radius.argtypes = [C.c_double, C.c_int]
+        L.ora_t_radius.restype = C.c_double
+        L.ora_t_particle_timestep.argtypes = [C.c_double, C.c_double, C.c_int, C.c_int, DP, DP]
+        L.ora_t_timestep_limit.argtypes = [C.c_double, C.c_int8, C.c_int8, C.c_uint64, C.c_double, DP]
+        L.ora_t_timestep_limit.restype = C.c_int8
+        L.ora_t_face_velocity.argtypes = [DP, DP, DP, DP, DP, DP, DP]
+        L.ora_t_face_velocity.restype = None
         L.ora_voronoi_1d.argtypes = [I64, DP, C.c_double, C.c_int, DP, DP, DP, I64P, I64P, DP, I64P, I64P, I64P,
                                      DP, DP, DP, I64P]
         _lib = L
@@ -340,6 +352,51 @@ class Sim:
         out = np.zeros(5)
         self.lib.ora_sim_conservation(self.ptr, _dp(out))
         return out
+
+    def gradients(self, active):
+        act = np.ascontiguousarray(active, dtype=np.uint8)
+        self.lib.ora_phase_gradients(self.ptr, act.ctypes.data_as(C.POINTER(C.c_uint8)))
+
+
+# single functions of the path (ideal gas), for the tests that pin them
+def c2p(gamma, cons, volume):
+    out = np.zeros(5)
+    e = lib().ora_t_c2p(gamma, _dp(_f64(cons, 5)), volume, _dp(out))
+    return e, out
+
+
+def time_extrapolations(gamma, prim, grad, v_rel, dt):
+    out = np.zeros(5)
+    lib().ora_t_time_extrapolations(gamma, _dp(_f64(prim, 5)), _dp(_f64(grad, 15)), _dp(_f64(v_rel, 3)), dt,
+                                    _dp(out))
+    return out
+
+
+def radius(volume, dim):
+    return lib().ora_t_radius(volume, dim)
+
+
+def particle_timestep(gamma, cfl, motion, dim, cons, prim, grad, loc, gc, centroid, volume, msv=0.0, maor=0.0):
+    """Particle::timestep: returns (dt, v, v_rel, msv after the call, error flag)."""
+    inp = np.concatenate([_f64(cons, 5), _f64(prim, 5), _f64(grad, 15), _f64(loc, 3), _f64(gc, 3),
+                          _f64(centroid, 3), [volume, msv, maor]])
+    out = np.zeros(8)
+    e = lib().ora_t_particle_timestep(gamma, cfl, motion, dim, _dp(inp), _dp(out))
+    return out[0], out[1:4].copy(), out[4:7].copy(), out[7], e
+
+
+def timestep_limit(dt, timebin, wakeup, ti_current, time_base):
+    """Particle::timestep_limit: returns (timebin, dt)."""
+    d = C.c_double(0.0)
+    b = lib().ora_t_timestep_limit(dt, timebin, wakeup, ti_current, time_base, C.byref(d))
+    return b, d.value
+
+
+def face_velocity(loc_l, loc_r, shift, v_l, v_r, fc):
+    out = np.zeros(3)
+    lib().ora_t_face_velocity(_dp(_f64(loc_l, 3)), _dp(_f64(loc_r, 3)), _dp(_f64(shift, 3)), _dp(_f64(v_l, 3)),
+                              _dp(_f64(v_r, 3)), _dp(_f64(fc, 3)), _dp(out))
+    return out
 
 
 def set_threads(n):

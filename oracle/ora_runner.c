@@ -352,6 +352,15 @@ static flux_info flux_zero(void) {
   return f;
 }
 
+/* Face velocity (flux.rs:75-78): the mean generator velocity minus the part of their relative velocity
+ * that moves the face point along dx. */
+static vec3 face_velocity(vec3 loc_l, vec3 loc_r, vec3 shift, vec3 v_l, vec3 v_r, vec3 fc) {
+  vec3 dx = v3_sub(v3_add(loc_r, shift), loc_l);
+  vec3 midpoint = v3_scale(0.5, v3_add(v3_add(loc_l, loc_r), shift));
+  double fac = v3_dot(v3_sub(v_r, v_l), v3_sub(fc, midpoint)) / v3_len2(dx);
+  return v3_sub(v3_scale(0.5, v3_add(v_l, v_r)), v3_scale(fac, dx));
+}
+
 /* flux.rs:23-100 */
 static int flux_exchange(const particle* left, const particle* right, double dt, const face_t* face,
                          double time_extrapolate_fac, const gaslaw* eos, const ora_riemann* rs, flux_info* out,
@@ -379,9 +388,7 @@ static int flux_exchange(const particle* left, const particle* right, double dt,
   st5 prim_right =
       ora_pairwise(right->primitives, left->primitives, right_dash, v3_len(dx_right) / dx_gradients, cnt);
 
-  vec3 midpoint = v3_scale(0.5, v3_add(v3_add(left->loc, right->loc), shift));
-  double fac = v3_dot(v3_sub(right->v, left->v), v3_sub(face->centroid, midpoint)) / v3_len2(dx);
-  vec3 v_face = v3_sub(v3_scale(0.5, v3_add(left->v, right->v)), v3_scale(fac, dx));
+  vec3 v_face = face_velocity(left->loc, right->loc, shift, left->v, right->v, face->centroid);
 
   /* State::boost (physical_quantities.rs:228-236) */
   vec3 nvf = v3_neg(v_face);
@@ -1316,3 +1323,103 @@ void ora_sim_conservation(const ora_sim* s, double* out5) {
   for (int64_t i = 0; i < s->n; ++i)
     for (int k = 0; k < 5; ++k) out5[k] += s->parts[i].conserved.c[k] + s->parts[i].fluxes.c[k];
 }
+
+/* ======================= single functions, for the tests that pin them ======================= */
+static gaslaw ideal_gas(double gamma) {
+  gaslaw g;
+  g.gamma = adiabatic_new(gamma);
+  g.eos = ORA_IDEAL;
+  g.iso_u = 0.;
+  return g;
+}
+
+int ora_t_c2p(double gamma, const double* cons5, double volume, double* prim5) {
+  gaslaw g = ideal_gas(gamma);
+  particle p;
+  memset(&p, 0, sizeof(p));
+  for (int k = 0; k < 5; ++k) p.conserved.c[k] = cons5[k];
+  p.volume = volume;
+  int e = convert_conserved_to_primitive(&p, &g);
+  for (int k = 0; k < 5; ++k) prim5[k] = p.primitives.c[k];
+  return e;
+}
+
+void ora_t_time_extrapolations(double gamma, const double* prim5, const double* grad15, const double* v_rel3,
+                               double dt, double* out5) {
+  gaslaw g = ideal_gas(gamma);
+  particle p;
+  memset(&p, 0, sizeof(p));
+  for (int k = 0; k < 5; ++k) {
+    p.primitives.c[k] = prim5[k];
+    p.gradients.g[k] = v3(grad15[3 * k], grad15[3 * k + 1], grad15[3 * k + 2]);
+  }
+  p.v_rel = v3(v_rel3[0], v_rel3[1], v_rel3[2]);
+  st5 r = time_extrapolations(&p, dt, &g);
+  for (int k = 0; k < 5; ++k) out5[k] = r.c[k];
+}
+
+double ora_t_radius(double volume, int dim) {
+  particle p;
+  memset(&p, 0, sizeof(p));
+  p.volume = volume;
+  return radius(&p, dim);
+}
+
+/* in: cons[5], prim[5], grad[15] (component-major xyz per component), loc[3], gc[3], centroid[3], volume,
+ * msv, maor; out: dt, v[3], v_rel[3], msv after the call. Returns the error flag of set_particle_velocity. */
+int ora_t_particle_timestep(double gamma, double cfl, int motion, int dim, const double* in, double* out) {
+  gaslaw g = ideal_gas(gamma);
+  particle p;
+  memset(&p, 0, sizeof(p));
+  for (int k = 0; k < 5; ++k) {
+    p.conserved.c[k] = in[k];
+    p.primitives.c[k] = in[5 + k];
+    p.gradients.g[k] = v3(in[10 + 3 * k], in[11 + 3 * k], in[12 + 3 * k]);
+  }
+  p.loc = v3(in[25], in[26], in[27]);
+  p.gradients_centroid = v3(in[28], in[29], in[30]);
+  p.centroid = v3(in[31], in[32], in[33]);
+  p.volume = in[34];
+  p.max_signal_velocity = in[35];
+  p.max_a_over_r = in[36];
+  int err = 0;
+  out[0] = particle_timestep(&p, cfl, motion, &g, dim, &err);
+  out[1] = p.v.x;
+  out[2] = p.v.y;
+  out[3] = p.v.z;
+  out[4] = p.v_rel.x;
+  out[5] = p.v_rel.y;
+  out[6] = p.v_rel.z;
+  out[7] = p.max_signal_velocity;
+  return err;
+}
+
+int8_t ora_t_timestep_limit(double dt, int8_t timebin, int8_t wakeup, uint64_t ti_current, double time_base,
+                            double* dt_out) {
+  particle p;
+  memset(&p, 0, sizeof(p));
+  p.dt = dt;
+  p.timebin = timebin;
+  tsinfo t;
+  memset(&t, 0, sizeof(t));
+  t.ti_current = ti_current;
+  t.time_base = time_base;
+  t.time_base_inv = 1. / time_base;
+  timestep_limit(&p, wakeup, &t);
+  *dt_out = p.dt;
+  return p.timebin;
+}
+
+void ora_t_face_velocity(const double* loc_l, const double* loc_r, const double* shift, const double* v_l,
+                         const double* v_r, const double* fc, double* out3) {
+  vec3 w = face_velocity(v3(loc_l[0], loc_l[1], loc_l[2]), v3(loc_r[0], loc_r[1], loc_r[2]),
+                         v3(shift[0], shift[1], shift[2]), v3(v_l[0], v_l[1], v_l[2]), v3(v_r[0], v_r[1], v_r[2]),
+                         v3(fc[0], fc[1], fc[2]));
+  out3[0] = w.x;
+  out3[1] = w.y;
+  out3[2] = w.z;
+}
+
+/* gradient_estimate + gradient_limit + gradient_apply (hydrodynamics.rs:67-143) on the sim's current
+ * primitives, for the cells flagged in `active`. */
+void ora_phase_gradients(ora_sim* s, const uint8_t* active) { gradients_phase(s, active); }

@@ -17,8 +17,16 @@
  *   - vacuum solver: golden half-state (src/riemann_solver/vacuum.rs:211-219).
  *   - HLLC / PVRS / AIRS / TSRS / TRRS: symmetry/invariance tests of the
  *     reference, consistency F(W,W) = Euler flux, HLLC within 5e-2 of exact.
- *   - pairwise limiter: brute-force band properties.
- *   - WLS gradients: exact for linear fields.
+ *   - AIRS: bitwise PVRS below the pressure-ratio threshold, exact at or above it.
+ *   - pairwise limiter: hand-computed clamps; monotone, idempotent, scale/shift invariant.
+ *   - WLS gradients: exact for linear fields; LimiterData alpha in closed form on a
+ *     uniform 1D mesh (linear, extremum, one-sided, flat).
+ *   - c2p: the reference's round trip (physical_quantities.rs:318-339, 4 ulp) and
+ *     random round trips through the textbook p2c.
+ *   - time_extrapolations: the conservative Euler flux derivative (1D), rotation and
+ *     moving-frame invariance.
+ *   - radius / Particle::timestep: closed forms; timestep_limit: brute force against
+ *     the particle's time line; face velocity: the moving bisector plane.
  *   - timeline: brute force over all bins, round trip test (timeline.rs:149-155).
  *   - step: conservation of mass/momentum/energy to roundoff.
  *   - 1D tessellation: test_init_1d known answers (src/space.rs:1453-1491).
@@ -192,6 +200,22 @@ void ora_phase_predict(ora_sim* s, double dt);
 void ora_phase_compute_fluxes(ora_sim* s, const uint8_t* active, double* fluxrec);
 /* Status line sums (src/space.rs:1248-1280): mass+pending, energy+pending, plus momentum+pending. */
 void ora_sim_conservation(const ora_sim* s, double* out5);
+/* gradient_estimate/limit/apply (src/runner/hydrodynamics.rs:67-143) on the current primitives. */
+void ora_phase_gradients(ora_sim* s, const uint8_t* active);
+
+/* Single functions of the path (ideal gas), so that tests can pin each against closed forms:
+ * conserved -> primitive (physical_quantities.rs:213-226, part.rs:219-261), time_extrapolations
+ * (part.rs:134-149), radius (part.rs:439-447), Particle::timestep (part.rs:42-84; in/out layout in
+ * ora_runner.c), timestep_limit (part.rs:263-297) and the face velocity (flux.rs:75-78). */
+int ora_t_c2p(double gamma, const double* cons5, double volume, double* prim5);
+void ora_t_time_extrapolations(double gamma, const double* prim5, const double* grad15, const double* v_rel3,
+                               double dt, double* out5);
+double ora_t_radius(double volume, int dim);
+int ora_t_particle_timestep(double gamma, double cfl, int motion, int dim, const double* in, double* out);
+int8_t ora_t_timestep_limit(double dt, int8_t timebin, int8_t wakeup, uint64_t ti_current, double time_base,
+                            double* dt_out);
+void ora_t_face_velocity(const double* loc_l, const double* loc_r, const double* shift, const double* v_l,
+                         const double* v_r, const double* fc, double* out3);
 
 /* 1D tessellation (meshless_voronoi 1D semantics pinned by src/space.rs:1453-1491). Output arrays
  * sized: faces F = n+1 (non periodic) or n (periodic); conn 2n. */

@@ -219,33 +219,3 @@ def test_consistency_equal_states(ora, kind):
         exp = np.array([rho * vn, *(rho * vn * v + p * n), (E + p) * vn])
         np.testing.assert_allclose(f, exp, rtol=1e-10, atol=1e-12)
 
-
-def test_pairwise_limiter_properties(ora):
-    rng = np.random.default_rng(3)
-    for _ in range(2000):
-        ql, qr = rng.uniform(0.01, 2.0, 2)
-        if rng.random() < 0.1:
-            qr = ql
-        qd = rng.uniform(-1.0, 3.0)
-        fac = rng.uniform(0.0, 1.0)
-        pl = np.array([ql, 0.0, 0.0, 0.0, ql])
-        pr = np.array([qr, 0.0, 0.0, 0.0, qr])
-        pd = np.array([qd, 0.0, 0.0, 0.0, qd])
-        out = ora.pairwise_limiter(pl, pr, pd, fac)
-        q = out[0]
-        if ql == qr:
-            assert q == ql
-            continue
-        lo, hi = min(ql, qr), max(ql, qr)
-        d1 = 0.5 * (hi - lo)
-        # result lies between the positivity-preserving floor and the extended band
-        if ql < qr:
-            qmin = lo - d1 if (lo - d1) * lo > 0 else lo * abs(lo) / (abs(lo) + d1)
-            assert q >= qmin - 1e-15
-            assert q <= max(qmin, ql + fac * (qr - ql) + 0.25 * (hi - lo)) + 1e-15
-        else:
-            qplu = hi + d1
-            assert q <= qplu + 1e-15
-            assert q >= min(qplu, ql + fac * (qr - ql) - 0.25 * (hi - lo)) - 1e-15 or q == 0.0
-        # a negative limited density/pressure resets the whole state to vacuum (check_physical)
-        assert q > 0.0 or np.all(out == 0.0)
