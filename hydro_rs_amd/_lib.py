@@ -102,6 +102,8 @@ HFV_SYMBOLS = {
     "hfv_conserved_to_primitive": (C.c_int, [C.c_void_p]),
     "hfv_gradients": (C.c_int, [C.c_void_p]),
     "hfv_timesteps": (C.c_int, [C.c_void_p, C.POINTER(U64)]),
+    "hfv_compute_timesteps": (C.c_int, [C.c_void_p, DP]),
+    "hfv_apply_timesteps": (C.c_int, [C.c_void_p, DP, C.POINTER(U64)]),
     "hfv_step": (C.c_int, [C.c_void_p, C.POINTER(TimestepInfo), C.POINTER(U64)]),
     "hfv_run": (C.c_int, [C.c_void_p, I64, C.POINTER(I64)]),
     "hfv_get_engine_info": (C.c_int, [C.c_void_p, C.POINTER(EngineInfo)]),

@@ -386,6 +386,7 @@ int alloc_batch(hfv_ctx* c) {
   c->P.ti = dalloc<uint64_t>(c, ncc);
   c->fall = dalloc<int2>(c, nf);
   C.fr = dalloc<double>(c, (int64_t)HFV_FR_W * n);
+  C.dtq = dalloc<double>(c, n);
   F.scell = dalloc<int32_t>(c, nfc * HFV_STAGE_MAX);
   F.scnt = dalloc<uint8_t>(c, nfc);
   F.floc = dalloc<uint8_t>(c, 2 * nf);
@@ -502,6 +503,21 @@ void recompute_uniforms(hfv_ctx* c) {
     if (c->desc[s].static_grad) c->any_static_grad = true;
     else c->any_dynamic_grad = true;
   }
+}
+
+// The flags that decide which kernels a captured step launches.
+struct LaunchSet {
+  int solver;
+  int precision;
+  bool moving1d, moving, st, dyn, sgrad, dgrad;
+  bool operator==(const LaunchSet& o) const {
+    return solver == o.solver && precision == o.precision && moving1d == o.moving1d && moving == o.moving &&
+           st == o.st && dyn == o.dyn && sgrad == o.sgrad && dgrad == o.dgrad;
+  }
+};
+LaunchSet launch_set(const hfv_ctx* c) {
+  return LaunchSet{c->solver, c->precision, c->any_moving1d, c->any_moving, c->any_static,
+                   c->any_dynamic, c->any_static_grad, c->any_dynamic_grad};
 }
 
 // Upload a mesh into a slot.  update_mask: see hfv_set_mesh.
@@ -757,11 +773,9 @@ int refresh_geometry(hfv_ctx* c, int s) {
     HIP_OK(hipGetLastError());
   }
   HIP_OK(hipStreamSynchronize(c->stream));
-  const bool prev[4] = {c->any_static, c->any_dynamic, c->any_static_grad, c->any_dynamic_grad};
+  const LaunchSet prev = launch_set(c);
   recompute_uniforms(c);
-  if (prev[0] != c->any_static || prev[1] != c->any_dynamic || prev[2] != c->any_static_grad ||
-      prev[3] != c->any_dynamic_grad)
-    drop_graph(c);
+  if (!(launch_set(c) == prev)) drop_graph(c);
   return HFV_OK;
 }
 
@@ -1360,6 +1374,43 @@ int hfv_timesteps(hfv_ctx* c, uint64_t* ti_next) {
   return HFV_OK;
 }
 
+int hfv_compute_timesteps(hfv_ctx* c, double* dt_out) {
+  TRY(require_mesh(c));
+  if (!dt_out) return set_err(HFV_ERR_INVALID, "out is NULL");
+  settle(c, -1);
+  launch_grad(c, c->batch_args(), 0, 2);
+  HIP_OK(hipGetLastError());
+  c->fr_dirty = true;
+  TRY(sync_counters(c));
+  const int64_t n = c->nused[0], cb = c->cbase[0];
+  std::vector<uint8_t> act(n);
+  HIP_OK(hipMemcpyAsync(dt_out, c->C.dtq + cb, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipMemcpyAsync(act.data(), c->C.act + cb, n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < n; ++i)
+    if (!act[i]) dt_out[i] = INFINITY;  // euler_eqns_fvs.rs:52
+  return HFV_OK;
+}
+
+int hfv_apply_timesteps(hfv_ctx* c, const double* dt, uint64_t* ti_next) {
+  TRY(require_mesh(c));
+  if (!dt) return set_err(HFV_ERR_INVALID, "dt is NULL");
+  settle(c, -1);
+  const int64_t n = c->nused[0], cb = c->cbase[0];
+  HIP_OK(hipMemcpyAsync(c->C.dtq + cb, dt, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  Batch B = c->batch_args();
+  hipLaunchKernelGGL(k_ts_apply, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_limit, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B, 1);
+  hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, c->stream, B);
+  HIP_OK(hipGetLastError());
+  c->fr_dirty = true;
+  TRY(sync_counters(c));
+  if (c->heng[0].err) return set_err(c->heng[0].err, "engine error in this step");
+  if (ti_next) *ti_next = c->heng[0].ti_next;
+  return HFV_OK;
+}
+
 // ---- fused step with the caller's TimestepInfo
 int hfv_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {
   TRY(require_mesh(c));
@@ -1470,11 +1521,10 @@ int hfv_batch_load(hfv_ctx* c, int s, const hfv_params* params, const hfv_mesh_a
   TRY(reset_slot(c, s));
   c->loaded[s] = 1;
   TRY(write_jobs(c));
+  const LaunchSet prev = launch_set(c);
   TRY(init_ic(c, s, mesh->n_cells, loc, mass, vel, u_int));
-  int prev_solver = c->solver;
-  bool prev_m = c->any_moving1d;
   recompute_uniforms(c);
-  if (prev_solver != c->solver || prev_m != c->any_moving1d) drop_graph(c);
+  if (!(launch_set(c) == prev)) drop_graph(c);
   return HFV_OK;
 }
 
@@ -1491,7 +1541,9 @@ int hfv_batch_unload(hfv_ctx* c, int s) {
   c->heng[s].force = 0;
   HIP_OK(hipMemcpyAsync(c->deng + s, c->heng + s, sizeof(SimEngine), hipMemcpyHostToDevice, c->stream));
   HIP_OK(hipStreamSynchronize(c->stream));
+  const LaunchSet prev = launch_set(c);
   recompute_uniforms(c);
+  if (!(launch_set(c) == prev)) drop_graph(c);  // kernels of simulations that left are not launched
   return HFV_OK;
 }
 
@@ -1674,7 +1726,12 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
           const auto t0 = std::chrono::steady_clock::now();
           for (int j = 0; j < launches && rc == HFV_OK; ++j) rc = launch_step(c);
           if (rc != HFV_OK) break;
-          HIP_OK(hipStreamSynchronize(c->stream));
+          // a failed synchronisation ends the tuning like any other error: the restore below still runs
+          const hipError_t se = hipStreamSynchronize(c->stream);
+          if (se != hipSuccess) {
+            rc = set_err(HFV_ERR_HIP, hipGetErrorString(se));
+            break;
+          }
           dt += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
           if ((rc = sync_counters(c)) != HFV_OK) break;
           if (updates() - u0 >= want || finished()) break;
@@ -1697,7 +1754,15 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
     if ((rc = apply(k, k.best)) != HFV_OK) break;
   }
   c->prof = prof;
-  if (rc != HFV_OK) return rc;
+  if (rc != HFV_OK) {
+    // leave every class on its default choices rather than on the candidate being measured
+    for (auto& k : cls)
+      for (int s : k.slots) {
+        c->tune_sgrad[s] = c->tune_stage[s] = c->tune_gstage[s] = -1;
+        (void)refresh_geometry(c, s);
+      }
+    return rc;
+  }
   const int n = (int)cls.size();
   if (n_out) *n_out = n;
   for (int k = 0; k < n && k < max_out && out; ++k) {

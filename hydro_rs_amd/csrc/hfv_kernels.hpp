@@ -46,6 +46,7 @@ struct Cells {
   // ld_grad/st_grad
   double* minv;  // [9][N] inverse of the gradient matrix (static geometry only)
   double* fr;    // [N][HFV_FR_W] flux inputs of static-geometry cells (layout: HFV_FR_*)
+  double* dtq;   // [N] requested timesteps of the trait-level compute_timesteps (before timesteps_apply)
   int8_t *bin, *wake;
   uint8_t* act;
 };
@@ -726,7 +727,11 @@ __global__ void __launch_bounds__(HFV_END_BLOCK) k_end(Batch B) {
 
 // ------------------------------------------------------------------ drift (space.rs:833-854)
 // Particle::drift + box_wrap / box_reflect (part.rs:107-119, 449-469) for one sub-interval.
-__device__ void drift_cell(const Cells& C, const Cfg& cfg, int64_t i, double dt, Events& ev) {
+// A reflected cell's kept time-extrapolation rate is rewritten from its reflected primitives and
+// gradients (its v_rel is not reflected), so that this step's predict (optimal_order.rs:20, after the
+// drift) extrapolates the reflected state; FP32-flux contexts first sum their unsummed predict time up
+// to the start of this step with the old rate.
+__device__ void drift_cell(const Cells& C, const Cfg& cfg, const SimEngine& E, int64_t i, double dt, Events& ev) {
   const int64_t n = C.n;
   const int dim = cfg.dim;
   double loc[3], cen[3], gc[3];
@@ -769,10 +774,18 @@ __device__ void drift_cell(const Cells& C, const Cfg& cfg, int64_t i, double dt,
       V3<double> sh = mk3(loc[0] - old[0], loc[1] - old[1], loc[2] - old[2]);
       if (len2(sh) > 0.) {
         V3<double> nn = (1. / len(sh)) * sh;
-        st5(C.prim, n, i, reflect(ld5(C.prim, n, i), nn));
+        const St<double> p_old = ld5(C.prim, n, i);
+        const St<double> p_new = reflect(p_old, nn);
+        st5(C.prim, n, i, p_new);
         // Particle::reflect_gradients (part.rs:390-429), including its use of the already reflected
         // density gradient for the pressure gradient
         const bool f32 = cfg.precision == HFV_FP32_FLUX;
+        if (f32) {
+          const uint64_t t0 = E.pred_step ? E.ti_pred - (E.ti_current - E.ti_old) : E.ti_pred;
+          const double el = (double)(t0 - C.tiex[i]) * E.time_base;
+          st_ex(C, i, true, ld_ex(C, i, true) + (p_old.c[0] > 0. ? (-el) * ld_tex(C, i, true) : vacuum<double>()));
+          C.tiex[i] = t0;
+        }
         Gr<double> g = ld_grad(C, i, f32);
         g.g[0] = g.g[0] - ((2. * dot(g.g[0], nn)) * nn);
         g.g[4] = g.g[4] - ((2. * dot(g.g[0], nn)) * nn);
@@ -785,6 +798,7 @@ __device__ void drift_cell(const Cells& C, const Cfg& cfg, int64_t i, double dt,
         g.g[3] = g.g[3] - (nn.z * (2. * dvn));
         for (int q = 1; q <= 3; ++q) g.g[q] = g.g[q] - ((2. * dot(g.g[q], nn)) * nn);
         st_grad(C, i, f32, g);
+        st_tex(C, i, f32, fr_rate(p_new, round_grad(g, f32), ld3(C.vrel, n, i), cfg.gd));
       }
     } else {
       ev.fail(E_UNSUPPORTED);  // particle removal at open/vacuum walls is not on this path
@@ -807,7 +821,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_drift(Batch B) {
   const int64_t i = (int64_t)J.lo + threadIdx.x;
   if (i >= J.hi) return;
   Events ev;
-  for (int k = 0; k < E.n_sub; ++k) drift_cell(B.C, cfg, i, E.sub_dt[k], ev);
+  for (int k = 0; k < E.n_sub; ++k) drift_cell(B.C, cfg, E, i, E.sub_dt[k], ev);
   flush_events(ev, B.cnt, B.eng, J.sim);
 }
 
@@ -1962,6 +1976,11 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
     C.msv[i] = 0.;
     dt_req = v_max > 0. ? cfg.cfl * cell_radius(V, cfg) / v_max : INFINITY;
   }
+  if (do_ts == 2) {  // FiniteVolumeSolver::compute_timesteps alone: the requested dt, not yet applied
+    C.dtq[i] = dt_req;
+    st_tex(C, i, cfg.precision == HFV_FP32_FLUX, fr_rate(prim, gr, vrel_now, cfg.gd));
+    return;
+  }
   // ---- timesteps_apply
   int8_t bin = C.bin[i];
   uint64_t dti = make_integer_timestep(dt_req, bin, HFV_NUM_TIME_BINS, E.ti_current, E.time_base_inv);
@@ -1989,7 +2008,7 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_GRAD_WAVES) k_grad(Batch B, int
   const bool a = i < J.hi && B.C.act[i];
   uint64_t my = ~0ull;
   Events ev;
-  const bool pack = do_ts && B.sims[s].staged;  // block-uniform
+  const bool pack = do_ts == 1 && B.sims[s].staged;  // block-uniform
   // LDS: the staged neighbour lines of the gradient sweeps, then (once every wave is done with them)
   // the per-wave transposes of the new flux records
   __shared__ double4 lds4[HFV_GSTAGE_MAX * 2];
@@ -2039,11 +2058,30 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_GRAD_WAVES) k_grad(Batch B, int
     if (has)
       store_fr_wave(B.C.fr, i - (int64_t)__lane_id(), has, frv, wbuf, cfg.precision == HFV_FP32_FLUX ? 1 : 2);
   }
-  if (do_ts) {
+  if (do_ts == 1) {
     uint64_t t = block_min_u64(my);
     if (threadIdx.x == 0) B.P.ti[J.chunk] = t;
   }
   flush_events(ev, B.cnt, B.eng, s);
+}
+
+// timesteps_apply (hydrodynamics.rs:145-158) of the requested timesteps in dtq (the caller's, e.g. the
+// minimum of the hydro and gravity timesteps, optimal_order.rs:43-47), for the active cells.
+__global__ void __launch_bounds__(HFV_BLOCK) k_ts_apply(Batch B) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const SimEngine& E = B.eng[J.sim];
+  if (!E.stepping) return;
+  const int64_t i = (int64_t)J.lo + threadIdx.x;
+  uint64_t my = ~0ull;
+  if (i < J.hi && B.C.act[i]) {
+    const uint64_t dti = make_integer_timestep(B.C.dtq[i], B.C.bin[i], HFV_NUM_TIME_BINS, E.ti_current, E.time_base_inv);
+    B.C.dt[i] = (double)dti * E.time_base;
+    B.C.bin[i] = get_time_bin(dti);
+    my = dti;
+  }
+  const uint64_t t = block_min_u64(my);
+  if (threadIdx.x == 0) B.P.ti[J.chunk] = t;
 }
 
 // ------------------------------------------------------------------ static geometry

@@ -185,6 +185,20 @@ class EulerEqnsFvs:
         _lib.check(self.lib.hfv_timesteps(self.ptr, C.byref(t)))
         return t.value
 
+    def compute_timesteps(self):
+        """FiniteVolumeSolver::compute_timesteps: each particle's requested dt (+inf if inactive)."""
+        out = np.zeros(self.n)
+        _lib.check(self.lib.hfv_compute_timesteps(self.ptr, _dp(out)))
+        return out
+
+    def apply_timesteps(self, dt):
+        """timesteps_apply + limiter/sync + reset + kick1 of requested timesteps; returns ti_next."""
+        d = np.ascontiguousarray(dt, dtype=np.float64)
+        assert d.size == self.n
+        t = _lib.U64(0)
+        _lib.check(self.lib.hfv_apply_timesteps(self.ptr, _dp(d), C.byref(t)))
+        return t.value
+
     # ------------------------------------------------------------ step / engine
     def step(self, info):
         t = _lib.U64(0)

@@ -30,6 +30,7 @@ class Job:
     slot: int = -1
     steps: int = 0
     cell_updates: int = 0
+    error_code: int = 0  # the simulation's hard error (HFV_ERR_*): it stopped at that step, no result
     result: dict = field(default_factory=dict)
 
     @property
@@ -138,7 +139,8 @@ class Server:
                     job.finished = clock() - t0
                     job.steps = info["step_count"]
                     job.cell_updates = info["cell_updates"]
-                    if fetch:
+                    job.error_code = info["error_code"]
+                    if fetch and not job.error_code:
                         job.result = self.batch.get_state(s, list(fetch))
                     self.batch.unload(s)
                     self.free.append(s)
@@ -149,8 +151,13 @@ class Server:
 
 
 def latency_summary(jobs, wall):
-    lat = np.array([j.latency for j in jobs])
-    cu = sum(j.cell_updates for j in jobs)
-    return {"jobs": len(jobs), "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
+    """Latency percentiles and throughput over the jobs that completed; jobs that stopped on a hard
+    error are counted apart (failed_jobs) and contribute neither latency nor cell updates."""
+    ok = [j for j in jobs if not j.error_code]
+    lat = np.array([j.latency for j in ok]) if ok else np.array([np.nan])
+    cu = sum(j.cell_updates for j in ok)
+    return {"jobs": len(ok), "failed_jobs": len(jobs) - len(ok),
+            "failed": [(j.jid, j.error_code) for j in jobs if j.error_code],
+            "p50_latency_s": float(np.percentile(lat, 50)), "p99_latency_s": float(np.percentile(lat, 99)),
             "mean_latency_s": float(lat.mean()), "wall_s": wall, "cell_updates": int(cu),
             "cell_updates_per_s": cu / wall if wall > 0 else 0.0}

@@ -194,7 +194,10 @@ int hfv_synchronize(hfv_ctx* ctx);
 int hfv_predict(hfv_ctx* ctx, double dt);
 /* FiniteVolumeSolver::compute_fluxes (src/finite_volume_solver.rs:12; euler_eqns_fvs.rs:26-48).
  * The active mask is bin_is_ending(timebin) at info->ti_current (optimal_order.rs:17).
- * Flux records stay in HBM; hfv_get_fluxes copies them out as [7F]: dt*F (5), v_max, a_over_r. */
+ * Flux records stay in HBM; hfv_get_fluxes copies them out as [7F]: dt*F (5), v_max, a_over_r.
+ * FluxInfo.mflux (src/flux.rs:10-15, 96) is not computed: its only consumer is gravity_mflux in
+ * grav_kick (part.rs:160-176, 303-312), where it is multiplied by a_grav, which the reference never
+ * sets on this path (gravity is not wired into OptimalOrderRunner::step, optimal_order.rs:33). */
 int hfv_compute_fluxes(hfv_ctx* ctx, const hfv_timestep_info* info);
 int hfv_get_fluxes(hfv_ctx* ctx, double* out7f);
 /* runner apply_fluxes + kick2 (src/runner/hydrodynamics.rs:13-57). */
@@ -207,6 +210,16 @@ int hfv_gradients(hfv_ctx* ctx);
  * reset_extrapolations + kick1 (src/finite_volume_solver.rs:16; src/runner/hydrodynamics.rs:41-49,
  * 145-195); writes ti_current + min(dti) to *ti_next. */
 int hfv_timesteps(hfv_ctx* ctx, uint64_t* ti_next);
+/* FiniteVolumeSolver::compute_timesteps (src/finite_volume_solver.rs:16; euler_eqns_fvs.rs:50-61) on
+ * its own: for the active cells Particle::timestep (part.rs:42-84) sets v and v_rel and resets
+ * max_signal_velocity and max_a_over_r; dt_out [N] receives each particle's requested timestep, +inf
+ * for inactive particles (the reference's Vec<f64>).  Nothing is applied: pass these (or the minimum
+ * with other timesteps, optimal_order.rs:43-47) to hfv_apply_timesteps. */
+int hfv_compute_timesteps(hfv_ctx* ctx, double* dt_out);
+/* timesteps_apply + timestep_limiter/timestep_sync + reset_extrapolations + kick1
+ * (src/runner/hydrodynamics.rs:145-195, optimal_order.rs:48-59) of the requested timesteps dt [N];
+ * writes ti_current + min(dti) to *ti_next.  hfv_timesteps is hfv_compute_timesteps followed by this. */
+int hfv_apply_timesteps(hfv_ctx* ctx, const double* dt, uint64_t* ti_next);
 
 /* ---------------------------------------------------------------- fused step and engine */
 /* OptimalOrderRunner::step / GodunovHydroRunner::step without the tessellation

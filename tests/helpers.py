@@ -1,6 +1,8 @@
 """Shared test helpers: build the oracle simulation for a product Problem, and compare states."""
 import numpy as np
 
+from hydro_rs_amd import fvs
+from hydro_rs_amd.mesh import VoronoiMesh
 from oracle import ora
 
 FLOAT_FIELDS = ["prim", "grad", "gc", "extrap", "cons", "fluxes", "volume", "loc", "centroid", "v", "v_rel",
@@ -61,3 +63,37 @@ def face_flux_rel(dev_rec, ref_rec):
     r6 = np.where(ref_rec[:, 5] == dev_rec[:, 5], 0.0, r6)
     r7 = np.where(ref_rec[:, 6] == dev_rec[:, 6], 0.0, r7)
     return float(max(np.max(r5, initial=0.0), np.max(r6, initial=0.0), np.max(r7, initial=0.0)))
+
+
+def mesh_from_oracle(sim, fields, dim, box, periodic):
+    m = sim.get_mesh()
+    mesh = object.__new__(VoronoiMesh)
+    mesh.dim, mesh.box, mesh.periodic = dim, np.asarray(box, float), periodic
+    mesh.n_cells = sim.n
+    mesh.n_faces = m["n_faces"]
+    mesh.area = m["area"]
+    mesh.normal = m["normal"].reshape(-1, 3)
+    mesh.centroid = m["centroid"].reshape(-1, 3)
+    mesh.shift = m["shift"].reshape(-1, 3)
+    mesh.left, mesh.right = m["left"], m["right"]
+    mesh.offset, mesh.count = m["offset"], m["count"]
+    mesh.conn = m["conn"][: int(m["count"].sum())] if m["conn"].size > m["count"].sum() else m["conn"]
+    n = sim.n
+    mesh.volume = fields["volume"].copy()
+    mesh.cell_centroid = fields["centroid"].reshape(3, n).T.copy()
+    mesh.search_radius = np.zeros(n)
+    return mesh
+
+
+def synced(prob, k):
+    """Oracle advanced k engine steps, and a device sim holding the same state and mesh."""
+    osim = oracle_sim(prob)
+    if k:
+        osim.advance(k)
+    f = osim.get_fields()
+    inf = osim.info()
+    dev = fvs.EulerEqnsFvs(prob.cfg)
+    c = prob.cfg
+    dev.set_mesh(mesh_from_oracle(osim, f, c.dim, c.box, c.boundary == fvs.PERIODIC))
+    dev.set_state(f)
+    return osim, dev, inf

@@ -160,14 +160,22 @@ def test_failed_member_stops_alone():
             assert np.array_equal(got[k], alone[k]), (s, k)
 
 
-def test_hard_error_matches_oracle_step(ora):
-    """A member of the 512-simulation ensemble whose rarefaction drives a cell to negative pressure: the
-    reference's c2p assert (physical_quantities.rs:220-221) fires; the device path raises the same
-    error in the same step as the oracle, alone and as a batch slot, and the batch's other slot finishes."""
+@pytest.fixture(scope="module")
+def ensemble64():
     from hydro_rs_amd import ensemble
 
-    probs = ensemble.ensemble_problems(count=512, seed=4, precision=fvs.FP64)
-    p = probs[470]  # 1D, 1283 cells
+    return ensemble.ensemble_problems(count=512, seed=4, precision=fvs.FP64)
+
+
+@pytest.mark.parametrize("member", [20, 112, 470])
+def test_hard_error_matches_oracle_step(ora, ensemble64, member):
+    """The members of the 512-simulation ensemble whose flows drive a cell to negative pressure (1D
+    members 20, 112 and 470: 7810, 25193 and 1283 cells): the reference's c2p assert
+    (physical_quantities.rs:220-221) fires; the device path raises the same error in the same step as
+    the oracle, alone and as a batch slot, and the batch's other slot finishes."""
+    probs = ensemble64
+    p = probs[member]
+    say(f"[hard error] member {member}: {p.cfg.dim}D, {p.n} cells")
     ref = oracle_sim(p)
     with pytest.raises(RuntimeError) as oe:
         ref.advance(1 << 40)

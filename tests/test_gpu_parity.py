@@ -10,8 +10,7 @@ import numpy as np
 import pytest
 
 from hydro_rs_amd import fvs, ics
-from hydro_rs_amd.mesh import VoronoiMesh
-from tests.helpers import compare_states, face_flux_rel, oracle_sim, rel_err
+from tests.helpers import compare_states, face_flux_rel, oracle_sim, rel_err, synced as _synced
 
 pytestmark = pytest.mark.gpu
 
@@ -27,26 +26,6 @@ def say(*a):
 def _device():
     if fvs.device_count() < 1:
         pytest.skip("no HIP device")
-
-
-def mesh_from_oracle(sim, fields, dim, box, periodic):
-    m = sim.get_mesh()
-    mesh = object.__new__(VoronoiMesh)
-    mesh.dim, mesh.box, mesh.periodic = dim, np.asarray(box, float), periodic
-    mesh.n_cells = sim.n
-    mesh.n_faces = m["n_faces"]
-    mesh.area = m["area"]
-    mesh.normal = m["normal"].reshape(-1, 3)
-    mesh.centroid = m["centroid"].reshape(-1, 3)
-    mesh.shift = m["shift"].reshape(-1, 3)
-    mesh.left, mesh.right = m["left"], m["right"]
-    mesh.offset, mesh.count = m["offset"], m["count"]
-    mesh.conn = m["conn"][: int(m["count"].sum())] if m["conn"].size > m["count"].sum() else m["conn"]
-    n = sim.n
-    mesh.volume = fields["volume"].copy()
-    mesh.cell_centroid = fields["centroid"].reshape(3, n).T.copy()
-    mesh.search_radius = np.zeros(n)
-    return mesh
 
 
 CASES = {
@@ -81,6 +60,24 @@ CASES = {
 }
 
 
+def wall_crossing_tube(precision=fvs.FP64):
+    """1D moving mesh (fluid motion) with reflective walls whose outermost generators sit 0.05 dx from the
+    walls and move into them at |v| = 0.5: they cross the walls at steps 6, 8, 10 and 12 and are
+    reflected (Particle::box_reflect, part.rs:457-469); the reference raises a negative pressure at step
+    13."""
+    n = 40
+    dx = 1.0 / n
+    x = (np.arange(n) + 0.5) * dx
+    x[0], x[-1] = 0.05 * dx, 1.0 - 0.05 * dx
+    loc = np.zeros((n, 3))
+    loc[:, 0] = x
+    vel = np.zeros((n, 3))
+    vel[:, 0] = np.where(x < 0.5, -0.5, 0.5)
+    cfg = fvs.Config(dim=1, box=(1.0, 1.0, 1.0), gamma=1.4, cfl=0.3, solver=fvs.EXACT, boundary=fvs.REFLECTIVE,
+                     motion=fvs.FLUID, t_end=0.05, precision=precision)
+    return ics._finish("wall_crossing", cfg, loc, np.ones(n), vel, np.ones(n))
+
+
 def _with(prob, **cfg):
     """The same problem with some configuration fields replaced."""
     from dataclasses import replace
@@ -113,20 +110,6 @@ def _blast3d(boundary=fvs.REFLECTIVE, solver=fvs.HLLC):
     cfg = fvs.Config(dim=3, box=(1.0, 1.0, 1.0), gamma=5.0 / 3.0, cfl=0.3, solver=solver, boundary=boundary,
                      motion=fvs.FIXED, t_end=0.05)
     return ics._finish("blast3d", cfg, loc, rho, vel, P)
-
-
-def _synced(prob, k):
-    """Oracle advanced k engine steps, and a device sim holding the same state and mesh."""
-    osim = oracle_sim(prob)
-    if k:
-        osim.advance(k)
-    f = osim.get_fields()
-    inf = osim.info()
-    dev = fvs.EulerEqnsFvs(prob.cfg)
-    c = prob.cfg
-    dev.set_mesh(mesh_from_oracle(osim, f, c.dim, c.box, c.boundary == fvs.PERIODIC))
-    dev.set_state(f)
-    return osim, dev, inf
 
 
 @pytest.mark.parametrize("case", list(CASES))
@@ -349,3 +332,100 @@ def test_snapshot_matches_oracle(ora, tmp_path):
     errs = {k: rel_err(a[k], b[k]) for k in a}
     say(f"  worst: {max(errs.items(), key=lambda kv: kv[1])}")
     assert max(errs.values()) <= 1e-9, errs
+
+
+def _crossings(osim, steps):
+    """Generators reflected at a wall during `steps` oracle steps (they end up moving against their
+    drift velocity)."""
+    n = osim.n
+    count = 0
+    prev = osim.get_fields()
+    for _ in range(steps):
+        osim.advance(1)
+        f = osim.get_fields()
+        x0, x1, v = prev["loc"][:n], f["loc"][:n], prev["v"][:n]
+        count += int(np.sum((v > 0) & (x1 < x0)) + np.sum((v < 0) & (x1 > x0)))
+        prev = f
+    return count
+
+
+def test_wall_crossing_reflection_parity(ora):
+    """Generators drifting through a reflective wall are reflected with their primitives and gradients
+    on the device as in the oracle (box_reflect, reflect_quantities, reflect_gradients), and the next
+    predict extrapolates the reflected state: 11 steps (3 reflections of each outer generator) within the
+    whole-run tolerance, then the reference's negative-pressure panic at the same step with the same code."""
+    prob = wall_crossing_tube()
+    osim = oracle_sim(prob)
+    assert _crossings(osim, 11) == 6
+    dev = ics.build_device_sim(prob)
+    assert dev.run(11) == 11
+    ref, got = osim.get_fields(), dev.get_state()
+    errs = compare_states(got, ref, prob.n)
+    say(f"  11 steps: errs={ {k: f'{v:.1e}' for k, v in errs.items()} }")
+    assert errs.pop("timebin_mismatch") == 0
+    assert max(errs.values()) <= 1e-9, errs
+    with pytest.raises(RuntimeError) as oe:
+        osim.advance(1 << 40)
+    with pytest.raises(fvs.HfvError) as de:
+        dev.run(1 << 40)
+    assert de.value.code == int(str(oe.value).split()[-1]) == -4
+    assert dev.engine_info()["step_count"] == osim.info()["step_count"]
+
+
+def test_wall_crossing_reflection_fp32():
+    """The FP32-flux context sums its deferred predicts with the rate each reflected generator had before
+    the reflection and extrapolates the reflected state afterwards: L1(rho) against the FP64 path after the
+    same 11 steps (3 reflections of each outer generator) stays < 1e-5."""
+    d64 = ics.build_device_sim(wall_crossing_tube(fvs.FP64))
+    d32 = ics.build_device_sim(wall_crossing_tube(fvs.FP32_FLUX))
+    assert d64.run(11) == 11 and d32.run(11) == 11
+    assert d64.engine_info()["ti_current"] == d32.engine_info()["ti_current"]
+    a = d64.get_state(["prim", "extrap"])
+    b = d32.get_state(["prim", "extrap"])
+    n = 40
+    l1 = float(np.mean(np.abs(a["prim"][:n] - b["prim"][:n])))
+    say(f"  L1(rho) fp32 vs fp64 after 11 steps: {l1:.2e}")
+    assert l1 < 1e-5
+    np.testing.assert_allclose(b["extrap"], a["extrap"], rtol=0, atol=1e-5 * np.max(np.abs(a["extrap"])))
+
+
+def test_compute_timesteps_entry(ora):
+    """hfv_compute_timesteps returns each active particle's Particle::timestep (the oracle's, from the same
+    state, within 1e-12) and +inf for inactive ones; followed by hfv_apply_timesteps it equals the fused
+    hfv_timesteps bit for bit."""
+    prob = ics.sedov_2d(side=24, t_end=0.05)
+    a = ics.build_device_sim(prob)
+    b = ics.build_device_sim(prob)
+    tb = prob.cfg.t_end / float(1 << 57)
+    ti_old, ti_cur = 0, 0
+    checked = 0
+    for step in range(6):
+        info = a.timestep_info(ti_old, ti_cur, tb)
+        for d in (a, b):
+            d.predict((ti_cur - ti_old) * tb)
+            d.compute_fluxes(info)
+            d.apply_fluxes()
+            d.convert_conserved_to_primitive()
+            d.gradients()
+        t1 = a.timesteps()
+        st = b.get_state()
+        dts = b.compute_timesteps()
+        act = st["timebin"] <= ora.get_max_active_bin(ti_cur)
+        assert np.all(np.isinf(dts[~act]))
+        n = prob.n
+        for i in np.nonzero(act)[0][::7]:
+            col = lambda k, w: st[k].reshape(w, n)[:, i]
+            dt, _, vrel, _, e = ora.particle_timestep(prob.cfg.gamma, prob.cfg.cfl, prob.cfg.motion, 2,
+                                                      col("cons", 5), col("prim", 5), col("grad", 15).reshape(5, 3),
+                                                      col("loc", 3), col("gc", 3), col("centroid", 3),
+                                                      st["volume"][i], msv=st["msv"][i], maor=st["maor"][i])
+            assert e == 0
+            assert abs(dts[i] - dt) <= 1e-12 * dt, (step, i, dts[i], dt)
+            checked += 1
+        t2 = b.apply_timesteps(dts)
+        assert t1 == t2
+        sa, sb = a.get_state(), b.get_state()
+        for k in sa:
+            assert np.array_equal(sa[k], sb[k]), (step, k)
+        ti_old, ti_cur = ti_cur, t1
+    assert checked > 100

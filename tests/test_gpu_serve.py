@@ -39,6 +39,7 @@ def test_served_results_equal_alone_and_l1():
     summ = serve.latency_summary(done, srv.wall)
     say(f"[serve] {summ}")
     assert len(done) == len(jobs)
+    assert summ["failed_jobs"] == 0 and all(j.error_code == 0 for j in done)
     overlapped = sum(1 for a in done for b in done if a is not b and a.admitted < b.finished and b.admitted < a.finished)
     assert overlapped > 0  # jobs were resident together
     for j in done:
