@@ -378,8 +378,8 @@ def radius(volume, dim):
 
 def particle_timestep(gamma, cfl, motion, dim, cons, prim, grad, loc, gc, centroid, volume, msv=0.0, maor=0.0):
     """Particle::timestep: returns (dt, v, v_rel, msv after the call, error flag)."""
-    inp = np.concatenate([_f64(cons, 5), _f64(prim, 5), _f64(grad, 15), _f64(loc, 3), _f64(gc, 3),
-                          _f64(centroid, 3), [volume, msv, maor]])
+    inp = np.concatenate([_f64(cons, 5).ravel(), _f64(prim, 5).ravel(), _f64(grad, 15).ravel(), _f64(loc, 3).ravel(),
+                          _f64(gc, 3).ravel(), _f64(centroid, 3).ravel(), [volume, msv, maor]])
     out = np.zeros(8)
     e = lib().ora_t_particle_timestep(gamma, cfl, motion, dim, _dp(inp), _dp(out))
     return out[0], out[1:4].copy(), out[4:7].copy(), out[7], e

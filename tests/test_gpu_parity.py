@@ -413,7 +413,7 @@ def test_compute_timesteps_entry(ora):
         act = st["timebin"] <= ora.get_max_active_bin(ti_cur)
         assert np.all(np.isinf(dts[~act]))
         n = prob.n
-        for i in np.nonzero(act)[0][::7]:
+        for i in np.nonzero(act)[0][::5]:
             col = lambda k, w: st[k].reshape(w, n)[:, i]
             dt, _, vrel, _, e = ora.particle_timestep(prob.cfg.gamma, prob.cfg.cfl, prob.cfg.motion, 2,
                                                       col("cons", 5), col("prim", 5), col("grad", 15).reshape(5, 3),
@@ -428,4 +428,4 @@ def test_compute_timesteps_entry(ora):
         for k in sa:
             assert np.array_equal(sa[k], sb[k]), (step, k)
         ti_old, ti_cur = ti_cur, t1
-    assert checked > 100
+    assert checked > 60
