@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -123,6 +124,38 @@ SimEngine fresh_engine(const hfv_params& p) {
 }
 
 enum { KS_PREDICT = 0, KS_MESH1D, KS_FLUX, KS_APPLY, KS_GRAD, KS_WAKEUP, KS_LIMIT, KS_DRIFT };
+
+// Device memory of hfv_riemann_batch_noalloc, reserved with the code object on every GPU the library
+// runs on (held until the process ends): the Riemann kernel's counters in the first 256 bytes, then the
+// staged problems and fluxes of one call, WL[5n] WR[5n] vface[3n] nrm[3n] F[5n], n <= HFV_NOALLOC_MAX_FACES.
+// Host code never uses the name `scratch` as an address (on the host it is only a stand-in for the
+// symbol): each device's address comes from hipGetSymbolAddress, taken once per device.
+constexpr int kScratchHead = 32;  // doubles: the counters' 256 bytes
+static_assert(sizeof(DevCounters) <= kScratchHead * sizeof(double), "counters fit the scratch head");
+static __device__ __align__(256) double scratch[kScratchHead + 21 * (size_t)HFV_NOALLOC_MAX_FACES];
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxScratchDevices] = {};
+
+// The scratch array's address on the current device (looked up on the first call on that device).
+int scratch_of_current_device(double** out) {
+  int dev = 0;
+  HIP_OK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxScratchDevices) return set_err(HFV_ERR_INVALID, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      char b[256];
+      snprintf(b, sizeof(b), "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+      return set_err(HFV_ERR_HIP, b);
+    }
+    g_scratch_ptr[dev] = (double*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return HFV_OK;
+}
 
 }  // namespace
 
@@ -1843,6 +1876,43 @@ int hfv_riemann_batch(const hfv_params* params, int64_t nb, const double* WL, co
   (void)hipFree(dn);
   (void)hipFree(df);
   (void)hipFree(cnt);
+  if (h.err) return set_err(h.err, "device raised a hard error in the Riemann solver");
+  return HFV_OK;
+}
+
+int hfv_riemann_batch_noalloc(const hfv_params* params, int64_t nb, const double* WL, const double* WR,
+                              const double* vface, const double* nrm, double* Fo) {
+  TRY(check_params(params));
+  if (nb <= 0 || !WL || !WR || !vface || !nrm || !Fo) return set_err(HFV_ERR_INVALID, "bad batch arrays");
+  if (nb > HFV_NOALLOC_MAX_FACES)
+    return set_err(HFV_ERR_INVALID, "hfv_riemann_batch_noalloc: n exceeds HFV_NOALLOC_MAX_FACES (65536 problems per call)");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return set_err(HFV_ERR_NO_DEVICE, "no HIP device");
+  double* base = nullptr;
+  TRY(scratch_of_current_device(&base));
+  DevCounters* cnt = (DevCounters*)base;
+  double* dl = base + kScratchHead;
+  double* dr = dl + 5 * nb;
+  double* dv = dr + 5 * nb;
+  double* dn = dv + 3 * nb;
+  double* df = dn + 3 * nb;
+  const Cfg cfg = make_cfg(*params);
+  DevCounters h;
+  std::memset(&h, 0, sizeof(h));
+  // the scratch keeps the previous call's contents: the counters start at zero, every staged array is
+  // overwritten for [0, n)
+  HIP_OK(hipMemcpy(cnt, &h, sizeof(DevCounters), hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dl, WL, sizeof(double) * 5 * nb, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dr, WR, sizeof(double) * 5 * nb, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dv, vface, sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
+  HIP_OK(hipMemcpy(dn, nrm, sizeof(double) * 3 * nb, hipMemcpyHostToDevice));
+  if (params->precision == HFV_FP32_FLUX)
+    hipLaunchKernelGGL(k_riemann_batch<float>, dim3(grid_full(nb)), dim3(BLOCK), 0, 0, cfg, nb, dl, dr, dv, dn, df, cnt);
+  else
+    hipLaunchKernelGGL(k_riemann_batch<double>, dim3(grid_full(nb)), dim3(BLOCK), 0, 0, cfg, nb, dl, dr, dv, dn, df, cnt);
+  HIP_OK(hipGetLastError());
+  HIP_OK(hipMemcpy(Fo, df, sizeof(double) * 5 * nb, hipMemcpyDeviceToHost));
+  HIP_OK(hipMemcpy(&h, cnt, sizeof(DevCounters), hipMemcpyDeviceToHost));
   if (h.err) return set_err(h.err, "device raised a hard error in the Riemann solver");
   return HFV_OK;
 }

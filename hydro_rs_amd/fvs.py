@@ -397,8 +397,12 @@ def _autotune(lib, ptr, launches, rounds):
             for o in out[: min(n.value, 64)]]
 
 
-def riemann_batch(cfg: Config, WL, WR, vface, nrm):
-    """RiemannFluxSolver::solve_for_flux for n face problems; arrays are [n,5] / [n,3]."""
+NOALLOC_MAX_FACES = 65536  # HFV_NOALLOC_MAX_FACES
+
+
+def riemann_batch(cfg: Config, WL, WR, vface, nrm, noalloc=False):
+    """RiemannFluxSolver::solve_for_flux for n face problems; arrays are [n,5] / [n,3].  noalloc: through
+    hfv_riemann_batch_noalloc (no device allocation, at most NOALLOC_MAX_FACES problems)."""
     WL = np.ascontiguousarray(np.asarray(WL, dtype=np.float64).T)
     WR = np.ascontiguousarray(np.asarray(WR, dtype=np.float64).T)
     vf = np.ascontiguousarray(np.asarray(vface, dtype=np.float64).T)
@@ -406,7 +410,8 @@ def riemann_batch(cfg: Config, WL, WR, vface, nrm):
     n = WL.shape[1]
     out = np.zeros((5, n))
     p = cfg.to_c()
-    _lib.check(_lib.hfv().hfv_riemann_batch(C.byref(p), n, _dp(WL), _dp(WR), _dp(vf), _dp(nr), _dp(out)))
+    fn = _lib.hfv().hfv_riemann_batch_noalloc if noalloc else _lib.hfv().hfv_riemann_batch
+    _lib.check(fn(C.byref(p), n, _dp(WL), _dp(WR), _dp(vf), _dp(nr), _dp(out)))
     return out.T.copy()
 
 

@@ -322,6 +322,15 @@ int hfv_profile_read(hfv_ctx* ctx, hfv_kernel_times* out);
  * WL, WR [5n] primitive states (rho, vx, vy, vz, P), vface, nrm [3n], F [5n] out.  Component-major. */
 int hfv_riemann_batch(const hfv_params* params, int64_t n, const double* WL, const double* WR,
                       const double* vface, const double* nrm, double* F);
+/* hfv_riemann_batch without any device allocation: the same arguments and results (the same kernel),
+ * staged through a fixed device array that the library reserves on each GPU it runs on (about 11 MB, held
+ * until the process ends).  No counterpart in the reference: it is for callers that cannot allocate
+ * device memory (inside HIP graph capture, tight serving loops, allocation-free harnesses).  n is at most
+ * HFV_NOALLOC_MAX_FACES per call (HFV_ERR_INVALID beyond it).  Every call on a GPU uses that GPU's one
+ * array, so calls that overlap on one GPU (from several threads) race; calls on distinct GPUs do not. */
+#define HFV_NOALLOC_MAX_FACES 65536
+int hfv_riemann_batch_noalloc(const hfv_params* params, int64_t n, const double* WL, const double* WR,
+                              const double* vface, const double* nrm, double* F);
 
 #ifdef __cplusplus
 }
