@@ -8,23 +8,27 @@ step is one batch launch: every unfinished member advances by one engine step (o
 OptimalOrderRunner::step with its own hierarchical timebins).  The autotuner first times a few real
 launches per size class and candidate (flux tiling, gradient geometry; bit-identical candidates), then
 W untimed launches (more if needed for the window to start at launch 260, whatever the tuner's
-length), then K timed launches bracketed by device synchronisation; value = cell updates (active
-cells summed over members and launches) / wall time of the K launches.  The reported warmup counts
-the tuning launches too.
+length; `warmup` reports the window's start, `warmup_requested` W), then K timed launches, replayed
+from the captured step graph with no timing events, bracketed by device synchronisation; value = cell
+updates (active cells summed over members and launches) / wall time of the K launches.
 
 Printed JSON line: the contract keys plus
   speedup_vs_plain_fp64  value / the plain single-simulation FP64 device path on the same members and
-                         step windows (members 0, 16, 32, ... each run alone, timed the same way)
-  l1_gate                L1(rho) of the FP32-flux batch vs the FP64 batch, over members that reached
-                         t_end in both runs (all members of the window otherwise at equal time)
-  roofline               dominant kernel of the timed launches (HIP events on the context's stream):
-                         algorithmic bytes per launch / average launch time vs 8 TB/s
+                         step windows (members 0, 16, 32, ... each run alone, graph replays)
+  l1_vs_oracle           L1(rho) of sampled FP32-flux members against the FP64 oracle advanced to the
+                         member's step count (at equal time)
+  l1_gate                L1(rho) of every FP32-flux member against the FP64 batch on the device
+  roofline               dominant kernel of the timed window: the survey's algorithmic bytes of that
+                         kernel per launch (ALGO below) / its mean duration from HIP events on the
+                         context's stream, measured on a replica context replaying the same window
   cpu_baseline           the oracle (C restatement of the reference path, OpenMP at the reference's
                          rayon sites) on all host cores, over a bounded sample of the same members,
                          with the host CPU model and a shorter one-thread sample (single_thread)
-  sedov2d / kh3d / serve the 2D Sedov blast (plain FP64 and FP32), the 3D Kelvin-Helmholtz layer
-                         (plain FP64 with its roofline, and FP32) and the served request stream
-                         (p50/p99 job latency, cell updates/s, L1 gate)
+  sedov2d / kh3d / serve the 2D Sedov blast (plain FP64 and FP32, L1 against the oracle at t_end), the
+                         3D Kelvin-Helmholtz layer (plain FP64 with its step roofline and FP32; L1
+                         against the oracle at a fixed time) and the served request stream over a sweep
+                         of arrival rates up to saturation (p50/p99 job latency, cell updates/s, L1
+                         against the oracle)
 """
 import argparse
 import json
@@ -40,45 +44,58 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-updates/s vs plain FP64 GPU path; p50/p99 job latency at L1 err<1e-5"
 
-# Algorithmic HBM bytes per unit of work, one read of every operand a kernel needs and one write of
-# every output, at the widths the context stores (FP64 contexts: all FP64):
-#   flux: per active cell 329 B (prim 40, grad 120, gc 24, extrap 40, centroid 24, loc 24, v 24, v_rel 24,
-#         dt 8, flag 1) + per computed face 121 B (area 8, normal 24, centroid 24, left/right 8, shift 1,
-#         flux record 56)
-#   apply_c2p: per cell 89 B (flag 1, accumulator r/w 80, CSR offset 8) + per active cell 208 B + per
-#         face 120 B (record 56 x 2 sides, CSR entries 4 x 2)
-#   grad_timestep: per active cell 600 B (of which the gradient write 120) + per face of an active cell 120 B
-#   mask_predict: per cell 193 B (of which 9 gradient components 72 and the extrapolation r/w 80)
-# FP32-flux contexts store gradients, extrapolations and face records as floats: the same terms at
-# 4-byte widths (grad 60, extrap 20, record 28).  They also defer the predict: mask_predict reads the
-# time bin and writes the active flag only (2 B per cell), the flux kernel reads each cell's 8-byte
-# extrapolation tag, and the reset in limit_kick writes the float extrapolations (20 B instead of 40)
-# and the tag (8 B).
+# Algorithmic HBM bytes of one OptimalOrderRunner::step on the FP64 SoA (the survey's contract,
+#   B_step = 192 N_all + 1247 N_act + 248 F_act (+8 F_act periodic)),
+# attributed to the kernels that do each part of the step:
+#   mask_predict   192 / cell      predict of every cell: rho, P, 9 gradient components, v_rel; extrap r/w
+#   flux           329 / act cell  phase-1 gather (prim 40, grad 120, gc 24, extrap 40, centroid 24, loc 24,
+#                                  v 24, v_rel 24, dt 8, bin 1)
+#                  120 / act face  geometry 64 (area, normal, centroid, left/right) + flux record write 56
+#                                  (+8 periodic shift)
+#   apply_c2p      281 / act cell  accumulate 193 (accumulator r/w 80, conserved r/w 80, msv/maor r/w 32,
+#                                  flag 1) + conserved -> primitive 88
+#                   64 / act face  flux record read 56 + CSR 8
+#   grad_timestep  557 / act cell  gradients 208 (neighbour prim + centroid 64, gradient write 120, gc write
+#                                  24) + the timestep share of the 429 B timestep/limiter/reset/kick1 part
+#                                  that k_limit does not carry (349)
+#                   48 / act face  CSR 8 + gradient-phase geometry re-read 40
+#   wakeup          16 / act face  CSR 8 + limiter neighbour ids 8
+#   limit_kick      80 / act cell  reset of the extrapolations and kick1
+# Sum: 192 N + 1247 N_act + 248 F_act.  FP32-flux contexts store gradients, extrapolations and face
+# records as floats: the same terms at 4-byte widths for those fields (gradients 120 -> 60,
+# extrapolations 40 -> 20, records 56 -> 28), i.e. 116 N + 1087 N_act + 192 F_act (+8 F_act periodic).
 ALGO = {
-    "flux": lambda n_act, f_act, n_all: 329 * n_act + 121 * f_act,
-    "apply_c2p": lambda n_act, f_act, n_all: 89 * n_all + 208 * n_act + 120 * f_act,
-    "grad_timestep": lambda n_act, f_act, n_all: 600 * n_act + 120 * f_act,
-    "mask_predict": lambda n_act, f_act, n_all: 193 * n_all,
-    "wakeup": lambda n_act, f_act, n_all: 18 * n_all + 8 * f_act,
-    "limit_kick": lambda n_act, f_act, n_all: 19 * n_all + 80 * n_act,
+    "mask_predict": lambda n_act, f_act, n_all: 192 * n_all,
+    "flux": lambda n_act, f_act, n_all: 329 * n_act + 120 * f_act,
+    "apply_c2p": lambda n_act, f_act, n_all: 281 * n_act + 64 * f_act,
+    "grad_timestep": lambda n_act, f_act, n_all: 557 * n_act + 48 * f_act,
+    "wakeup": lambda n_act, f_act, n_all: 16 * f_act,
+    "limit_kick": lambda n_act, f_act, n_all: 80 * n_act,
 }
 ALGO_FP32 = {
-    "flux": lambda n_act, f_act, n_all: 257 * n_act + 93 * f_act,
-    "apply_c2p": lambda n_act, f_act, n_all: 89 * n_all + 208 * n_act + 64 * f_act,
-    "grad_timestep": lambda n_act, f_act, n_all: 540 * n_act + 120 * f_act,
-    "mask_predict": lambda n_act, f_act, n_all: 2 * n_all,
+    "mask_predict": lambda n_act, f_act, n_all: 116 * n_all,
+    "flux": lambda n_act, f_act, n_all: 249 * n_act + 92 * f_act,
+    "apply_c2p": lambda n_act, f_act, n_all: 281 * n_act + 36 * f_act,
+    "grad_timestep": lambda n_act, f_act, n_all: 497 * n_act + 48 * f_act,
     "wakeup": ALGO["wakeup"],
-    "limit_kick": lambda n_act, f_act, n_all: 19 * n_all + 68 * n_act,
+    "limit_kick": lambda n_act, f_act, n_all: 60 * n_act,
 }
+PERIODIC_SHIFT = 8  # bytes per active face of a periodic mesh (the flux phase's shift read)
+
+
+def step_bytes(n_all, n_act, f_act, fp32=False, periodic=False):
+    """B_step of the survey's contract at the context's storage widths."""
+    t = ALGO_FP32 if fp32 else ALGO
+    return sum(f(n_act, f_act, n_all) for f in t.values()) + (PERIODIC_SHIFT * f_act if periodic else 0)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def roofline(kt, launches_timed, cell_updates, face_fluxes, n_all, fp32=False):
-    """Dominant kernel (most device time) of a timed region; bytes from ALGO (ALGO_FP32 for FP32-flux
-    contexts) at the region's mean active counts per launch."""
+def roofline(kt, launches_timed, cell_updates, face_fluxes, n_all, fp32=False, periodic=False):
+    """Dominant kernel (most device time) of a timed window; bytes from ALGO (ALGO_FP32 for FP32-flux
+    contexts) at the window's mean active counts per launch."""
     name, rec = max(((k, v) for k, v in kt.items() if v["launches"] > 0), key=lambda kv: kv[1]["ms"])
     launches = rec["launches"]
     avg_s = rec["ms"] / 1e3 / launches
@@ -86,11 +103,15 @@ def roofline(kt, launches_timed, cell_updates, face_fluxes, n_all, fp32=False):
     n_act = cell_updates / steps
     f_act = face_fluxes / steps
     algo = (ALGO_FP32 if fp32 else ALGO).get(name, lambda *_: 0)(n_act, f_act, n_all)
+    if name == "flux" and periodic:
+        algo += PERIODIC_SHIFT * f_act
     achieved = algo / avg_s / 1e9
     return {
         "kernel": name, "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "avg_launch_us": round(avg_s * 1e6, 3),
         "algorithmic_bytes_per_launch": int(algo), "launches": launches,
+        "mean_active_cells": round(n_act, 1), "mean_active_faces": round(f_act, 1), "cells": int(n_all),
+        "byte_model": "survey B_step per kernel (bench.py ALGO%s)" % ("_FP32" if fp32 else ""),
         "per_kernel_ms": {k: round(v["ms"], 3) for k, v in kt.items() if v["launches"]},
     }
 
@@ -271,18 +292,18 @@ def bench_ensemble(args):
     t = time.perf_counter()
     probs = ensemble.ensemble_problems(count=args.members, seed=4, precision=fvs.FP32_FLUX)
     n_all = sum(p.n for p in probs)
+    members = [(s, p.cfg.dim, p.n) for s, p in enumerate(probs)]
     log(f"[ensemble] {len(probs)} members, {n_all} cells, generated in {time.perf_counter() - t:.1f}s")
     b = fvs.Batch.for_problems(probs)
     # autotuner (per size class: flux tiling, gradient geometry); its launches are ordinary steps
     tune_launches = 0
     tuning = None
-    warmup = args.warmup
     # the timed window starts at launch max(window_start, tuning + W), whatever the tuner's length
     if args.tuning_from:
         # the earlier run's choices, and its tuner's launches as warmup: every candidate gives the same
         # bits, so the timed window holds the same steps as that run's
         rep = load_report(args.tuning_from)["autotune"]
-        apply_tuning(b, [(s, p.cfg.dim, p.n) for s, p in enumerate(probs)], rep["classes"])
+        apply_tuning(b, members, rep["classes"])
         tune_launches = rep["launches"]
         tuning = rep["classes"]
         warmup, start = window_warmup(args.window_start, args.warmup, tune_launches, True)
@@ -300,17 +321,21 @@ def bench_ensemble(args):
     b.run(warmup)
     b.synchronize()
     i0 = [b.engine_info(s) for s in range(len(probs))]
-    b.profile(True)
+    # the timed window: graph replays, no timing events (as a production run executes)
     t0 = time.perf_counter()
     launches = b.run(args.steps)
     b.synchronize()
     secs = time.perf_counter() - t0
     i1 = [b.engine_info(s) for s in range(len(probs))]
-    kt = b.kernel_times()
     cu = sum(a["cell_updates"] - z["cell_updates"] for a, z in zip(i1, i0))
     ff = sum(a["face_fluxes"] - z["face_fluxes"] for a, z in zip(i1, i0))
     value = cu / secs
     log(f"[ensemble] fp32: {launches} launches in {secs:.3f}s -> {value:.4e} cell-updates/s")
+    # per-kernel device times over the same window: a replica context with the same tuning replays the
+    # first `start` launches (bit-identical states), then the window's launches with HIP events on its
+    # stream around every kernel
+    kt, rep_cu = kernel_times_replay(lambda: fvs.Batch.for_problems(probs), members, tuning, start, launches)
+    assert rep_cu == cu, "the replica window must hold the timed window's steps"
     res = {"launches": launches, "seconds": secs, "cell_updates": cu, "face_fluxes": ff, "value": value,
            "window_start": start, "autotune": {"launches": tune_launches, "classes": tuning},
            "n_all": n_all, "members_done": sum(1 for i in i1 if i["done"]),
@@ -318,10 +343,16 @@ def bench_ensemble(args):
            "roofline": roofline(kt, launches, cu, ff, n_all, fp32=True)}
     attach_traffic(res["roofline"], "ensemble512", True, fvs.HLLC, tuning)
     if args.no_gate:  # a profiling run: the timed window only
-        res.update({"fp64_batched_value": None, "l1_gate": None, "plain_fp64": None, "speedup_vs_plain_fp64": None})
+        res.update({"fp64_batched_value": None, "l1_gate": None, "plain_fp64": None, "speedup_vs_plain_fp64": None,
+                    "l1_vs_oracle": None})
         b.close()
         return res, probs
-    # the same window in FP64 (batched) for the L1 gate
+    # L1(rho) against the FP64 oracle: sampled members advanced by the oracle to the batch member's own
+    # step count, compared where both stand at the same time
+    res["l1_vs_oracle"] = None if args.no_oracle else \
+        l1_members_vs_oracle(b, probs, i1, range(0, len(probs), args.oracle_stride))
+    log(f"[ensemble] L1 vs oracle: {res['l1_vs_oracle']}")
+    # the same window in FP64 (batched) for the L1 gate over every member
     p64 = [ics.with_precision(p, fvs.FP64) for p in probs]
     b64 = fvs.Batch.for_problems(p64)
     t1 = time.perf_counter()
@@ -341,11 +372,11 @@ def bench_ensemble(args):
         l1s.append(_l1(r32, r64))
     res["l1_gate"] = {"members_compared": same_t, "max_l1_rho": max(l1s) if l1s else None,
                       "mean_l1_rho": float(np.mean(l1s)) if l1s else None,
-                      "pass": bool(l1s) and max(l1s) < 1e-5}
+                      "pass": bool(l1s) and max(l1s) < 1e-5, "against": "FP64 batch on the device, every member"}
     log(f"[ensemble] L1 gate: {res['l1_gate']}")
     b64.close()
     b.close()
-    # plain single-simulation FP64 path on a sample of members, same windows
+    # plain single-simulation FP64 path on a sample of members, same windows (graph replays)
     idx = list(range(0, len(probs), args.plain_stride))
     st = ensemble.run_plain([p64[i] for i in idx], max_steps=args.steps, warmup=start)
     res["plain_fp64"] = {"value": st["cell_updates"] / st["seconds"], "members": idx, "steps": st["steps"],
@@ -354,6 +385,54 @@ def bench_ensemble(args):
     log(f"[ensemble] plain fp64 sample: {res['plain_fp64']['value']:.4e} cell-updates/s -> "
         f"speedup {res['speedup_vs_plain_fp64']:.1f}x")
     return res, probs
+
+
+def kernel_times_replay(make, members, tuning, start, launches):
+    """Per-kernel device times (HIP events on the context's stream) of launches [start, start + launches)
+    of a fresh replica with the given tuning; returns (kernel_times, cell updates of that window)."""
+    r = make()
+    if tuning:
+        apply_tuning(r, members, tuning)
+    r.run(start)
+    r.synchronize()
+    u0 = sum(r.engine_info(s)["cell_updates"] for s, _, _ in members)
+    r.profile(True)
+    r.run(launches)
+    r.synchronize()
+    kt = r.kernel_times()
+    r.profile(False)
+    u1 = sum(r.engine_info(s)["cell_updates"] for s, _, _ in members)
+    r.close()
+    return kt, u1 - u0
+
+
+def l1_members_vs_oracle(batch, probs, infos, idx):
+    """L1(rho) of FP32-flux batch members against the FP64 oracle advanced to the member's step count;
+    members that stopped on a hard error or stand at another time than the oracle are skipped."""
+    from hydro_rs_amd import fvs, ics
+    from tests.helpers import oracle_sim  # the oracle is the reference here, not the measured path
+
+    l1s, skipped = {}, []
+    for s in idx:
+        e = infos[s]
+        if e["error_code"]:
+            skipped.append(s)
+            continue
+        osim = oracle_sim(ics.with_precision(probs[s], fvs.FP64))
+        try:
+            osim.advance(e["step_count"])
+        except RuntimeError:
+            skipped.append(s)
+            continue
+        if osim.info()["ti_current"] != e["ti_current"]:
+            skipped.append(s)
+            continue
+        ref = osim.get_fields()["prim"][: probs[s].n]
+        l1s[s] = _l1(batch.get_state(s, ["prim"])["prim"][: probs[s].n], ref)
+    vals = list(l1s.values())
+    return {"members_compared": len(vals), "members": sorted(l1s), "skipped": skipped,
+            "max_l1_rho": max(vals) if vals else None, "mean_l1_rho": float(np.mean(vals)) if vals else None,
+            "pass": bool(vals) and max(vals) < 1e-5}
 
 
 def _cpu_model():
@@ -417,10 +496,12 @@ def cpu_baseline_ensemble(probs, stride, warmup, budget_s):
 
 # ------------------------------------------------------------------ single simulations
 def run_device(prob, warmup, steps, profile=False, tune=0, tuning_from=None, tune_rounds=1):
-    """W untimed steps, then K timed ones, of a fresh context.  tune > 0: the autotuner runs first on a
-    separate context, after the same W steps (so it samples the phase of the run that is timed), and
-    its choice is applied, so the timed window is the same steps whatever the tuner's length;
-    tuning_from: (classes, launches) of an earlier run's tuner, applied the same way."""
+    """W untimed steps, then K timed ones (graph replays, no timing events), of a fresh context.  tune > 0:
+    the autotuner runs first on a separate context, after the same W steps (so it samples the phase of the
+    run that is timed), and its choice is applied, so the timed window is the same steps whatever the
+    tuner's length; tuning_from: (classes, launches) of an earlier run's tuner, applied the same way.
+    profile: per-kernel device times of the same window from a replica context (same tuning, the same W
+    steps first, then the window with HIP events around every kernel)."""
     from hydro_rs_amd import ics
 
     tuning, tune_launches = None, 0
@@ -432,20 +513,31 @@ def run_device(prob, warmup, steps, profile=False, tune=0, tuning_from=None, tun
         tuning = probe.autotune(launches=tune, rounds=tune_rounds)
         tune_launches = probe.engine_info()["step_count"] - warmup
         probe.close()
-    dev = ics.build_device_sim(prob)
-    if tuning:
-        apply_tuning(dev, [(0, prob.cfg.dim, prob.n)], tuning)
-    dev.run(warmup)
-    dev.synchronize()
+
+    def fresh():
+        d = ics.build_device_sim(prob)
+        if tuning:
+            apply_tuning(d, [(0, prob.cfg.dim, prob.n)], tuning)
+        d.run(warmup)
+        d.synchronize()
+        return d
+
+    dev = fresh()
     e0 = dev.engine_info()
-    if profile:
-        dev.profile(True)
     t0 = time.perf_counter()
     done = dev.run(steps)
     dev.synchronize()
     t1 = time.perf_counter()
     e1 = dev.engine_info()
-    kt = dev.kernel_times() if profile else None
+    kt = None
+    if profile:
+        r = fresh()
+        r.profile(True)
+        r.run(done)
+        r.synchronize()
+        kt = r.kernel_times()
+        assert r.engine_info()["cell_updates"] == e1["cell_updates"], "replica window differs"
+        r.close()
     return dev, {
         "steps": done, "seconds": t1 - t0, "cell_updates": e1["cell_updates"] - e0["cell_updates"],
         "face_fluxes": e1["face_fluxes"] - e0["face_fluxes"], "kernel_times": kt, "done": e1["done"],
@@ -453,21 +545,39 @@ def run_device(prob, warmup, steps, profile=False, tune=0, tuning_from=None, tun
     }
 
 
+def oracle_run(prob, steps=1 << 40):
+    """The FP64 oracle (OpenMP on every host thread) advanced by `steps` engine steps: (density, info)."""
+    from hydro_rs_amd import fvs, ics
+    from tests.helpers import oracle_sim  # the oracle is the reference here, not the measured path
+
+    osim = oracle_sim(ics.with_precision(prob, fvs.FP64))
+    osim.advance(steps)
+    return osim.get_fields()["prim"][: prob.n], osim.info()
+
+
 def bench_sedov(args):
     from hydro_rs_amd import fvs, ics
 
     prob = ics.sedov_2d(side=args.sedov_side, t_end=0.05)
-    log(f"[sedov] {prob.n} cells, {prob.mesh.n_faces} faces; FP64 to t_end")
+    log(f"[sedov] {prob.n} cells, {prob.mesh.n_faces} faces; FP64 and FP32 flux to t_end")
     d64, s64 = run_device(prob, 0, 1 << 40)
     d32, s32 = run_device(ics.with_precision(prob, fvs.FP32_FLUX), 0, 1 << 40)
-    l1 = _l1(d32.get_state(["prim"])["prim"][: prob.n], d64.get_state(["prim"])["prim"][: prob.n])
+    r64 = d64.get_state(["prim"])["prim"][: prob.n]
+    r32 = d32.get_state(["prim"])["prim"][: prob.n]
     out = {"workload": f"sedov2d_{args.sedov_side}x{args.sedov_side}_periodic_hllc_t0.05", "cells": prob.n,
            "faces": prob.mesh.n_faces}
     for k, s in (("fp64", s64), ("fp32_flux", s32)):
         out[k] = {"cell_updates_per_s": s["cell_updates"] / s["seconds"],
                   "face_fluxes_per_s": s["face_fluxes"] / s["seconds"], "steps": s["steps"],
                   "us_per_step": 1e6 * s["seconds"] / max(s["steps"], 1), "reached_t_end": bool(s["done"])}
-    out["l1_rho_fp32_vs_fp64_at_t_end"] = l1
+    out["l1_rho_fp32_vs_fp64_at_t_end"] = _l1(r32, r64)
+    if not args.no_oracle:
+        log("[sedov] the FP64 oracle to t_end (L1 check)")
+        t = time.perf_counter()
+        ref, oi = oracle_run(prob)
+        out["l1_vs_oracle"] = {"fp32_flux": _l1(r32, ref), "fp64": _l1(r64, ref), "at": "t_end 0.05",
+                               "oracle_steps": oi["step_count"], "oracle_seconds": time.perf_counter() - t,
+                               "pass": _l1(r32, ref) < 1e-5}
     log(f"[sedov] {out}")
     d64.close()
     d32.close()
@@ -486,61 +596,97 @@ def bench_kh3d(args):
     rep = load_report(args.tuning_from).get("kh3d") if args.tuning_from else None
     for k, prec in (("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)):
         tf = (rep[k]["autotune"], rep[k]["tune_launches"]) if rep else None
+        fp32 = prec == fvs.FP32_FLUX
         d, s = run_device(ics.with_precision(prob, prec), args.kh_warmup, args.kh_steps, profile=True,
                           tune=args.kh_tune, tuning_from=tf, tune_rounds=args.kh_tune_rounds)
-        rl = roofline(s["kernel_times"], s["steps"], s["cell_updates"], s["face_fluxes"], prob.n,
-                      fp32=prec == fvs.FP32_FLUX)
-        attach_traffic(rl, f"kh3d_{args.kh_side}", prec == fvs.FP32_FLUX, fvs.AIRS, s["autotune"])
+        rl = roofline(s["kernel_times"], s["steps"], s["cell_updates"], s["face_fluxes"], prob.n, fp32=fp32,
+                      periodic=True)
+        attach_traffic(rl, f"kh3d_{args.kh_side}", fp32, fvs.AIRS, s["autotune"])
         n_act = s["cell_updates"] / max(s["steps"], 1)
         f_act = s["face_fluxes"] / max(s["steps"], 1)
-        # B_step at the context's storage widths (FP32-flux: gradients, extrapolations, records as floats)
-        if prec == fvs.FP32_FLUX:
-            step_bytes = 2 * prob.n + 1103 * n_act + 200 * f_act
-        else:
-            step_bytes = 192 * prob.n + 1247 * n_act + 256 * f_act
+        sec_step = s["seconds"] / max(s["steps"], 1)
         out[k] = {"cell_updates_per_s": s["cell_updates"] / s["seconds"],
                   "face_fluxes_per_s": s["face_fluxes"] / s["seconds"],
-                  "ms_per_step": 1e3 * s["seconds"] / max(s["steps"], 1),
-                  "step_roofline_frac": step_bytes / (s["seconds"] / max(s["steps"], 1)) / (HBM_PEAK_GBS * 1e9),
+                  "ms_per_step": 1e3 * sec_step,
+                  "step_bytes": step_bytes(prob.n, n_act, f_act, fp32=fp32, periodic=True),
+                  "step_roofline_frac": step_bytes(prob.n, n_act, f_act, fp32=fp32, periodic=True) / sec_step
+                  / (HBM_PEAK_GBS * 1e9),
+                  "mean_active_cells": n_act, "mean_active_faces": f_act,
                   "roofline": rl, "ti_current": s["ti_current"], "autotune": s["autotune"],
                   "tune_launches": s["tune_launches"]}
         devs[k] = d
         log(f"[kh3d] {k}: {out[k]['ms_per_step']:.2f} ms/step, {out[k]['cell_updates_per_s']:.3e} cell-updates/s, "
-            f"dominant {rl['kernel']} {rl['achieved']} GB/s")
-    if out["fp64"]["ti_current"] == out["fp32_flux"]["ti_current"]:
-        out["l1_rho_fp32_vs_fp64"] = _l1(devs["fp32_flux"].get_state(["prim"])["prim"][: prob.n],
-                                         devs["fp64"].get_state(["prim"])["prim"][: prob.n])
+            f"step {out[k]['step_roofline_frac']:.3f} of HBM, dominant {rl['kernel']} {rl['achieved']} GB/s")
     for d in devs.values():
         d.close()
+    if not args.no_oracle:
+        # FP32-flux result against the FP64 oracle at a common time: both run to t_end = kh_l1_t
+        t = time.perf_counter()
+        short = ics.with_precision(prob, fvs.FP32_FLUX)
+        from dataclasses import replace
+
+        short = replace(short, cfg=replace(short.cfg, t_end=args.kh_l1_t))
+        log(f"[kh3d] FP32 flux and the FP64 oracle to t = {args.kh_l1_t} (L1 check)")
+        d32 = ics.build_device_sim(short)
+        n32 = d32.run()
+        r32 = d32.get_state(["prim"])["prim"][: prob.n]
+        d32.close()
+        ref, oi = oracle_run(short)
+        out["l1_vs_oracle"] = {"fp32_flux": _l1(r32, ref), "at": f"t_end {args.kh_l1_t}",
+                               "device_steps": n32, "oracle_steps": oi["step_count"],
+                               "oracle_seconds": time.perf_counter() - t, "pass": _l1(r32, ref) < 1e-5}
+        log(f"[kh3d] L1 vs oracle: {out['l1_vs_oracle']}")
     return out
 
 
-def bench_serve(args):
-    from hydro_rs_amd import fvs, ics, serve
+def serve_once(args, rate, count, seed=5):
+    """One served stream: Poisson arrivals at `rate` jobs/s (rate <= 0: every job arrives at t = 0)."""
+    from hydro_rs_amd import serve
 
-    jobs = serve.served_jobs(count=args.serve_jobs, rate=args.serve_rate, seed=5, side_min=32,
+    jobs = serve.served_jobs(count=count, rate=rate if rate > 0 else 1e12, seed=seed, side_min=32,
                              side_max=args.serve_side_max, sedov_t_end=args.serve_sedov_t_end,
                              vortex_t_end=args.serve_vortex_t_end, threads=args.threads)
-    log(f"[serve] {len(jobs)} jobs, rate {args.serve_rate}/s, {sum(j.prob.n for j in jobs)} cells in total")
     srv = serve.Server(hbm_budget_bytes=int(args.serve_hbm_gb * 2 ** 30), launches_per_poll=args.serve_poll)
-    done = srv.serve(jobs, fetch=("prim",), progress=lambda t: log(f"[serve] {t}"))
+    done = srv.serve(jobs, fetch=("prim",), progress=lambda t: log(f"[serve] rate {rate}: {t}"))
     summ = serve.latency_summary(done, srv.wall)
-    # L1 gate: jobs 0, k, 2k, ... rerun alone in FP64
-    l1s = {}
-    for j in done[:: max(1, len(done) // args.serve_l1_jobs)]:
-        d = ics.build_device_sim(ics.with_precision(j.prob, fvs.FP64))
-        d.run()
-        ref = d.get_state(["prim"])["prim"][: j.prob.n]
-        d.close()
-        l1s[j.jid] = _l1(j.result["prim"][: j.prob.n], ref)
-    summ.update({"rate_jobs_per_s": args.serve_rate, "hbm_budget_gb": args.serve_hbm_gb,
-                 "slots": len(srv.caps[0]), "l1_rho_vs_fp64": {"jobs": len(l1s), "max": max(l1s.values()),
-                                                               "pass": max(l1s.values()) < 1e-5},
-                 "jobs_desc": f"Poisson arrivals, 50/50 Sedov (t_end {args.serve_sedov_t_end}) / isentropic vortex "
-                              f"(t_end {args.serve_vortex_t_end}), side log-uniform in [32, {args.serve_side_max}], "
-                              "gamma 1.4, FP32 flux"})
-    log(f"[serve] {summ}")
-    return summ
+    summ.update({"rate_jobs_per_s": rate if rate > 0 else "all at t=0", "offered_cells": int(sum(j.prob.n for j in jobs)),
+                 "slots": len(srv.caps[0])})
+    return summ, done
+
+
+def bench_serve(args):
+    """The served stream over a sweep of arrival rates up to saturation (every job queued at t = 0): p50/p99
+    latency and cell-updates/s per rate; L1(rho) of sampled jobs against the FP64 oracle."""
+    from hydro_rs_amd import fvs, ics
+
+    rates = [float(r) for r in args.serve_rates.split(",")]
+    table, last = [], None
+    for r in rates:
+        summ, done = serve_once(args, r, args.serve_jobs)
+        log(f"[serve] rate {r}: {summ}")
+        table.append(summ)
+        last = done
+    sat = max(table, key=lambda x: x["cell_updates_per_s"])
+    out = {"rates": table, "saturated_cell_updates_per_s": sat["cell_updates_per_s"],
+           "saturated_rate": sat["rate_jobs_per_s"], "jobs_per_rate": args.serve_jobs,
+           "hbm_budget_gb": args.serve_hbm_gb,
+           "jobs_desc": f"Poisson arrivals, 50/50 Sedov (t_end {args.serve_sedov_t_end}) / isentropic vortex "
+                        f"(t_end {args.serve_vortex_t_end}), side log-uniform in [32, {args.serve_side_max}], "
+                        "gamma 1.4, FP32 flux / FP64 accumulate; the same job draw at every rate"}
+    # L1 gate: the smallest jobs of each kind of the last stream, against the FP64 oracle at t_end
+    if not args.no_oracle:
+        l1s = {}
+        for kind in ("sedov", "vortex"):
+            for j in sorted((j for j in last if j.kind == kind and not j.error_code), key=lambda j: j.prob.n)[:args.serve_l1_jobs]:
+                log(f"[serve] the FP64 oracle on job {j.jid} ({kind}, {j.prob.n} cells)")
+                ref, _ = oracle_run(j.prob)
+                l1s[j.jid] = {"kind": kind, "side": int(round(np.sqrt(j.prob.n))),
+                              "l1_rho": _l1(j.result["prim"][: j.prob.n], ref)}
+        vals = [v["l1_rho"] for v in l1s.values()]
+        out["l1_vs_oracle"] = {"jobs": l1s, "max": max(vals) if vals else None,
+                               "pass": bool(vals) and max(vals) < 1e-5}
+    log(f"[serve] {out}")
+    return out
 
 
 def main():
@@ -553,6 +699,7 @@ def main():
     ap.add_argument("--workload", default="all", choices=["all", "ensemble", "sedov", "kh3d", "serve"])
     ap.add_argument("--members", type=int, default=512)
     ap.add_argument("--plain-stride", type=int, default=16)
+    ap.add_argument("--oracle-stride", type=int, default=64, help="ensemble members compared with the oracle")
     ap.add_argument("--tune-launches", type=int, default=2,
                     help="ensemble autotuner: launches per timed group of a candidate trial (0: no tuning)")
     ap.add_argument("--tune-rounds", type=int, default=1)
@@ -563,14 +710,17 @@ def main():
     ap.add_argument("--kh-side", type=int, default=128)
     ap.add_argument("--kh-steps", type=int, default=20)
     ap.add_argument("--kh-warmup", type=int, default=80)
-    ap.add_argument("--serve-jobs", type=int, default=48)
-    ap.add_argument("--serve-rate", type=float, default=6.0)
+    ap.add_argument("--kh-l1-t", type=float, default=0.002,
+                    help="3D workload: end time of the L1 check against the oracle (12 engine steps at 0.002)")
+    ap.add_argument("--no-oracle", action="store_true", help="skip the L1 checks against the oracle")
+    ap.add_argument("--serve-jobs", type=int, default=32)
+    ap.add_argument("--serve-rates", default="4,16,0", help="arrival rates in jobs/s (0: all jobs at t = 0)")
     ap.add_argument("--serve-side-max", type=int, default=512)
-    ap.add_argument("--serve-sedov-t-end", type=float, default=0.02)
+    ap.add_argument("--serve-sedov-t-end", type=float, default=0.05)
     ap.add_argument("--serve-vortex-t-end", type=float, default=0.5)
     ap.add_argument("--serve-hbm-gb", type=float, default=24.0)
     ap.add_argument("--serve-poll", type=int, default=8)
-    ap.add_argument("--serve-l1-jobs", type=int, default=8)
+    ap.add_argument("--serve-l1-jobs", type=int, default=2, help="smallest jobs of each kind checked against the oracle")
     ap.add_argument("--threads", type=int, default=0, help="host tessellation threads (0: all)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -635,6 +785,7 @@ def main():
             "face_fluxes_per_s": ens["face_fluxes"] / ens["seconds"],
             "speedup_vs_plain_fp64": ens["speedup_vs_plain_fp64"], "plain_fp64": ens["plain_fp64"],
             "fp64_batched_value": ens["fp64_batched_value"], "l1_gate": ens["l1_gate"],
+            "l1_vs_oracle": ens["l1_vs_oracle"], "warmup_requested": args.warmup,
             "members_done": ens["members_done"], "members_failed": ens["members_failed"],
             "roofline": ens["roofline"], "autotune": ens["autotune"],
         })
@@ -662,7 +813,7 @@ def main():
                         "ms_per_step": s["ms_per_step"], "dtype": "f64", "roofline": s["roofline"],
                         "config": {"workload": out["kh3d"]["workload"]}})
         else:
-            s = out["serve"]
+            s = max(out["serve"]["rates"], key=lambda x: x["cell_updates_per_s"])
             out.update({"value": s["cell_updates_per_s"], "steps": s["jobs"], "warmup": 0,
                         "ms_per_step": 1e3 * s["wall_s"] / max(s["jobs"], 1), "dtype": "f32 flux / f64 accumulate",
                         "config": {"workload": "served_stream", "jobs": s["jobs"]}})

@@ -125,6 +125,8 @@ struct SimEngine {
   int no_pred;
   int pred_step;
   uint64_t ti_pred;
+  int n_down;  // 1D periodic moving meshes: faces of this step's rebuild where the generator order steps down
+  int pad_;
 };
 
 // Batch-global counters (soft events; the first hard error of any simulation).
@@ -585,6 +587,7 @@ __global__ void k_begin(Batch B) {
   SimEngine& E = B.eng[s];
   E.stepping = 0;
   E.n_sub = 0;
+  E.n_down = 0;
   E.n_active = 0;
   E.n_face_active = 0;
   E.ti_min = ~0ull;
@@ -895,8 +898,11 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
 
 // ------------------------------------------------------------------ 1D moving mesh on the device
 // Space::volume_calculation for 1D (space.rs:236-263; semantics pinned by test_init_1d,
-// space.rs:1453-1491): generators stay sorted, so the face topology is fixed; face k of a simulation
-// sits between its cells k-1 and k; walls are faces 0 and n (non periodic); periodic meshes wrap at 0.
+// space.rs:1453-1491): generators keep their order along x, so the face topology is fixed; face k of a
+// simulation sits between its cells k-1 and k; walls are faces 0 and n (non periodic).  On a periodic
+// line the order is cyclic: a generator that drifted through the box edge keeps its neighbours, and the one
+// face where the order steps down (face 0 until something wraps) joins its left generator to the image of
+// its right one a box to the right, carrying the periodic shift.
 __global__ void __launch_bounds__(HFV_BLOCK) k_mesh1d_faces(Batch B) {
   const BlockJob J = B.fjobs[blockIdx.x];
   if (J.sim < 0) return;
@@ -907,17 +913,23 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mesh1d_faces(Batch B) {
   if (f >= J.hi) return;
   int64_t k = f - D.face_base, n = D.n, cb = D.cell_base;
   double x;
-  if (k >= 1 && k < n) {
-    double a = B.C.loc[cb + k - 1], b = B.C.loc[cb + k];
-    if (!(a < b)) {
-      atomicCAS(&B.cnt->err, 0, E_UNSUPPORTED);
-      atomicCAS(&B.eng[s].err, 0, E_UNSUPPORTED);
+  if (!D.periodic) {
+    if (k >= 1 && k < n) {
+      double a = B.C.loc[cb + k - 1], b = B.C.loc[cb + k];
+      if (!(a < b)) {
+        atomicCAS(&B.cnt->err, 0, E_UNSUPPORTED);
+        atomicCAS(&B.eng[s].err, 0, E_UNSUPPORTED);
+      }
+      x = 0.5 * (a + b);
+    } else {
+      x = (k == 0) ? 0. : D.cfg.box[0];
     }
-    x = 0.5 * (a + b);
-  } else if (!D.periodic) {
-    x = (k == 0) ? 0. : D.cfg.box[0];
   } else {
-    x = 0.5 * (B.C.loc[cb + n - 1] + (B.C.loc[cb] + D.cfg.box[0]));
+    const double a = B.C.loc[cb + (k == 0 ? n - 1 : k - 1)], b = B.C.loc[cb + k];
+    const bool down = !(a < b);
+    x = down ? 0.5 * (a + (b + D.cfg.box[0])) : 0.5 * (a + b);
+    B.F.shc[f] = down ? 1 : 0;  // +box along x
+    if (down) atomicAdd(&B.eng[s].n_down, 1);
   }
   B.F.fcen[f] = x;
 }
@@ -935,7 +947,12 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mesh1d_cells(Batch B) {
     xl = B.F.fcen[fb + k];
     xr = B.F.fcen[fb + k + 1];
   } else {
-    xl = (k == 0) ? B.F.fcen[fb] - D.cfg.box[0] : B.F.fcen[fb + k];
+    if (B.eng[J.sim].n_down != 1) {  // not a rotation of the line's order: the topology changed
+      atomicCAS(&B.cnt->err, 0, E_UNSUPPORTED);
+      atomicCAS(&B.eng[J.sim].err, 0, E_UNSUPPORTED);
+    }
+    // the left face's centroid is in cell k-1's frame: a box to the right when it carries the shift
+    xl = B.F.fcen[fb + k] - (B.F.shc[fb + k] ? D.cfg.box[0] : 0.);
     xr = (k == n - 1) ? B.F.fcen[fb] : B.F.fcen[fb + k + 1];
   }
   double V = xr - xl;

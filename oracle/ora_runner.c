@@ -890,9 +890,15 @@ int ora_voronoi_1d(int64_t n, const double* x, double box_x, int periodic, doubl
                    double* centroid, int64_t* left, int64_t* right, double* shift, int64_t* conn, int64_t* offset,
                    int64_t* count, double* volume, double* cell_centroid, double* search_radius,
                    int64_t* n_faces_out) {
-  for (int64_t i = 0; i + 1 < n; ++i)
-    if (!(x[i] < x[i + 1])) return ORA_ERR_UNSUPPORTED;
   if (n < 1) return ORA_ERR_UNSUPPORTED;
+  /* Generators in index order along x; on a periodic line they may be a rotation of that order (a
+   * generator that drifted through the box edge keeps its neighbours): exactly one step of the cycle then
+   * goes down, and that face carries the periodic shift. */
+  int64_t descents = 0;
+  for (int64_t i = 0; i + 1 < n; ++i)
+    if (!(x[i] < x[i + 1])) ++descents;
+  if (!periodic && descents) return ORA_ERR_UNSUPPORTED;
+  if (periodic && descents + (x[n - 1] < x[0] ? 0 : 1) != 1) return ORA_ERR_UNSUPPORTED;
   int64_t nf = periodic ? n : n + 1;
   for (int64_t f = 0; f < nf; ++f) {
     area[f] = 1.;
@@ -904,9 +910,15 @@ int ora_voronoi_1d(int64_t n, const double* x, double box_x, int periodic, doubl
       shift[3 * f + k] = 0.;
     }
   }
-  /* face k (1 <= k <= n-1) sits between generators k-1 and k */
+  /* face k (1 <= k <= n-1) sits between generators k-1 and k (on a rotated periodic line, the face of
+   * the downward step joins generator k-1 to the image of generator k one box to the right) */
   for (int64_t k = 1; k < n; ++k) {
-    centroid[3 * k] = 0.5 * (x[k - 1] + x[k]);
+    if (x[k - 1] < x[k]) {
+      centroid[3 * k] = 0.5 * (x[k - 1] + x[k]);
+    } else {
+      centroid[3 * k] = 0.5 * (x[k - 1] + (x[k] + box_x));
+      shift[3 * k] = box_x;
+    }
     left[k] = k - 1;
     right[k] = k;
   }
@@ -919,11 +931,16 @@ int ora_voronoi_1d(int64_t n, const double* x, double box_x, int periodic, doubl
     left[n] = n - 1;
     right[n] = -1;
   } else {
-    /* face 0 is the wrap face between generator n-1 (left) and the image of generator 0 */
-    centroid[0] = 0.5 * (x[n - 1] + (x[0] + box_x));
+    /* face 0 is the face between generator n-1 (left) and generator 0: the wrap face with the periodic
+     * shift, unless a rotation moved the downward step elsewhere */
     left[0] = n - 1;
     right[0] = 0;
-    shift[0] = box_x;
+    if (x[n - 1] < x[0]) {
+      centroid[0] = 0.5 * (x[n - 1] + x[0]);
+    } else {
+      centroid[0] = 0.5 * (x[n - 1] + (x[0] + box_x));
+      shift[0] = box_x;
+    }
   }
   for (int64_t i = 0; i < n; ++i) {
     double xl, xr;
@@ -931,7 +948,8 @@ int ora_voronoi_1d(int64_t n, const double* x, double box_x, int periodic, doubl
       xl = centroid[3 * i];
       xr = centroid[3 * (i + 1)];
     } else {
-      xl = (i == 0) ? centroid[0] - box_x : centroid[3 * i];
+      /* the cell's left face (face i) is given in cell i-1's frame: one box to the right when shifted */
+      xl = centroid[3 * i] - shift[3 * i];
       xr = (i == n - 1) ? centroid[0] : centroid[3 * (i + 1)];
     }
     volume[i] = xr - xl;

@@ -75,20 +75,22 @@ def test_trace_summary_window_column(tmp_path, profiles):
     assert line.rstrip().endswith("| 3.50 |")  # (3000 + 4000) / 2 ns = 3.5 us
 
 
-def test_fp32_byte_model_uses_stored_widths():
+def test_byte_model_is_the_survey_step():
+    """The per-kernel attribution sums to B_step = 192 N + 1247 N_act + 248 F_act (+8 F_act periodic) in
+    FP64 contexts, and the FP32-flux widths remove exactly the stored-FP32 fields: gradients 120 -> 60 (read
+    by predict 72 -> 36 and the flux gather, written by the gradient phase), extrapolations 40 -> 20 (predict
+    r/w, flux gather, reset) and face records 56 -> 28 (written and read back)."""
     n_act, f_act, n_all = 1000.0, 3000.0, 5000.0
-    # gradients 120 -> 60, extrapolations 40 -> 20, records 56 -> 28 bytes; the deferred predict adds an
-    # 8-byte extrapolation tag per cell read by the flux kernel and written by the reset
-    assert bench.ALGO["flux"](n_act, f_act, n_all) - bench.ALGO_FP32["flux"](n_act, f_act, n_all) == \
-        (60 + 20 - 8) * n_act + 28 * f_act
-    assert bench.ALGO["apply_c2p"](n_act, f_act, n_all) - bench.ALGO_FP32["apply_c2p"](n_act, f_act, n_all) == \
-        2 * 28 * f_act
-    assert bench.ALGO["grad_timestep"](n_act, f_act, n_all) - bench.ALGO_FP32["grad_timestep"](n_act, f_act, n_all) \
-        == 60 * n_act
-    # the deferred predict: time bin and active flag only
-    assert bench.ALGO_FP32["mask_predict"](n_act, f_act, n_all) == 2 * n_all
-    assert bench.ALGO["limit_kick"](n_act, f_act, n_all) - bench.ALGO_FP32["limit_kick"](n_act, f_act, n_all) \
-        == (20 - 8) * n_act
+    assert bench.step_bytes(n_all, n_act, f_act) == 192 * n_all + 1247 * n_act + 248 * f_act
+    assert bench.step_bytes(n_all, n_act, f_act, periodic=True) == 192 * n_all + 1247 * n_act + 256 * f_act
+    d = {k: bench.ALGO[k](n_act, f_act, n_all) - bench.ALGO_FP32[k](n_act, f_act, n_all) for k in bench.ALGO}
+    assert d["mask_predict"] == (36 + 40) * n_all
+    assert d["flux"] == (60 + 20) * n_act + 28 * f_act
+    assert d["apply_c2p"] == 28 * f_act
+    assert d["grad_timestep"] == 60 * n_act
+    assert d["limit_kick"] == 20 * n_act
+    assert d["wakeup"] == 0
+    assert bench.step_bytes(n_all, n_act, f_act, fp32=True) == 116 * n_all + 1087 * n_act + 192 * f_act
 
 
 def test_tuning_from_report(tmp_path):

@@ -433,3 +433,27 @@ def test_pairwise_limiter_properties(ora):
             np.testing.assert_allclose(_lim(ora, ql + c, qr + c, q + c, fac), _lim(ora, ql, qr, q, fac) + c
                                        if (min(ql, qr) - 0.5 * abs(ql - qr)) > 0 else _lim(ora, ql + c, qr + c, q + c, fac),
                                        rtol=0, atol=1e-13)
+
+
+# ----------------------------------------------------------------------------------- 1D periodic rotation
+def test_voronoi_1d_periodic_rotation(ora):
+    """A periodic line whose generators are a rotation of the sorted order (generators that drifted through
+    the box edge) has the same cells as the sorted line: every generator keeps its volume and its two
+    neighbours, and the one face where the order steps down carries the periodic shift.  Known answer:
+    x = (0.8, 0.1, 0.5) on [0, 1) gives the cells [0.65, 0.95], [-0.05, 0.3], [0.3, 0.65]."""
+    m = ora.voronoi_1d(np.array([0.8, 0.1, 0.5]), 1.0, periodic=True)
+    np.testing.assert_allclose(m["volume"], [0.3, 0.35, 0.35], rtol=1e-15)
+    np.testing.assert_allclose(m["centroid"].reshape(-1, 3)[:, 0], [0.65, 0.95, 0.3], rtol=1e-15)
+    assert list(m["shift"].reshape(-1, 3)[:, 0]) == [0.0, 1.0, 0.0]
+    rng = np.random.default_rng(60)
+    for _ in range(50):
+        n = int(rng.integers(3, 40))
+        x = np.sort(rng.uniform(0.0, 1.0, n))
+        base = ora.voronoi_1d(x, 1.0, periodic=True)
+        r = int(rng.integers(1, n))
+        rot = ora.voronoi_1d(np.roll(x, -r), 1.0, periodic=True)
+        np.testing.assert_allclose(np.roll(base["volume"], -r), rot["volume"], rtol=0, atol=1e-15)
+        assert abs(rot["volume"].sum() - 1.0) < 1e-12
+        assert int(np.count_nonzero(rot["shift"])) == 1
+    with pytest.raises(RuntimeError):  # not a rotation: two downward steps
+        ora.voronoi_1d(np.array([0.5, 0.1, 0.8, 0.2]), 1.0, periodic=True)
