@@ -594,14 +594,17 @@ def bench_kh3d(args):
            "warmup": args.kh_warmup, "steps": args.kh_steps}
     devs = {}
     rep = load_report(args.tuning_from).get("kh3d") if args.tuning_from else None
-    for k, prec in (("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)):
+    precs = [("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)]
+    if args.kh_precision != "both":  # profiling runs: one precision, so that a kernel's last dispatches are its window
+        precs = [kp for kp in precs if kp[0].startswith(args.kh_precision)]
+    for k, prec in precs:
         tf = (rep[k]["autotune"], rep[k]["tune_launches"]) if rep else None
         fp32 = prec == fvs.FP32_FLUX
         d, s = run_device(ics.with_precision(prob, prec), args.kh_warmup, args.kh_steps, profile=True,
                           tune=args.kh_tune, tuning_from=tf, tune_rounds=args.kh_tune_rounds)
         rl = roofline(s["kernel_times"], s["steps"], s["cell_updates"], s["face_fluxes"], prob.n, fp32=fp32,
                       periodic=True)
-        attach_traffic(rl, f"kh3d_{args.kh_side}", fp32, fvs.AIRS, s["autotune"])
+        attach_traffic(rl, f"kh3d_{args.kh_side}_{'fp32' if fp32 else 'fp64'}", fp32, fvs.AIRS, s["autotune"])
         n_act = s["cell_updates"] / max(s["steps"], 1)
         f_act = s["face_fluxes"] / max(s["steps"], 1)
         sec_step = s["seconds"] / max(s["steps"], 1)
@@ -639,18 +642,19 @@ def bench_kh3d(args):
     return out
 
 
-def serve_once(args, rate, count, seed=5):
+def serve_once(args, rate, count, hbm_gb, seed=5):
     """One served stream: Poisson arrivals at `rate` jobs/s (rate <= 0: every job arrives at t = 0)."""
     from hydro_rs_amd import serve
 
     jobs = serve.served_jobs(count=count, rate=rate if rate > 0 else 1e12, seed=seed, side_min=32,
                              side_max=args.serve_side_max, sedov_t_end=args.serve_sedov_t_end,
                              vortex_t_end=args.serve_vortex_t_end, threads=args.threads)
-    srv = serve.Server(hbm_budget_bytes=int(args.serve_hbm_gb * 2 ** 30), launches_per_poll=args.serve_poll)
+    srv = serve.Server(hbm_budget_bytes=int(hbm_gb * 2 ** 30), launches_per_poll=args.serve_poll)
     done = srv.serve(jobs, fetch=("prim",), progress=lambda t: log(f"[serve] rate {rate}: {t}"))
     summ = serve.latency_summary(done, srv.wall)
     summ.update({"rate_jobs_per_s": rate if rate > 0 else "all at t=0", "offered_cells": int(sum(j.prob.n for j in jobs)),
-                 "slots": len(srv.caps[0])})
+                 "slots": len(srv.caps[0]), "hbm_budget_gb": hbm_gb,
+                 "cell_updates_per_s_while_queued": srv.saturated_rate()})
     return summ, done
 
 
@@ -662,14 +666,17 @@ def bench_serve(args):
     rates = [float(r) for r in args.serve_rates.split(",")]
     table, last = [], None
     for r in rates:
-        summ, done = serve_once(args, r, args.serve_jobs)
+        summ, done = serve_once(args, r, args.serve_jobs, args.serve_hbm_gb)
         log(f"[serve] rate {r}: {summ}")
         table.append(summ)
         last = done
-    sat = max(table, key=lambda x: x["cell_updates_per_s"])
-    out = {"rates": table, "saturated_cell_updates_per_s": sat["cell_updates_per_s"],
-           "saturated_rate": sat["rate_jobs_per_s"], "jobs_per_rate": args.serve_jobs,
-           "hbm_budget_gb": args.serve_hbm_gb,
+    # saturation: more jobs than the slots of a smaller budget, all queued at t = 0, so that every slot a job
+    # fits is busy until the queue drains; throughput while jobs wait in the queue
+    sat, _ = serve_once(args, 0.0, args.serve_sat_jobs, args.serve_sat_hbm_gb, seed=6)
+    log(f"[serve] saturation: {sat}")
+    out = {"rates": table, "saturation": sat,
+           "saturated_cell_updates_per_s": sat["cell_updates_per_s_while_queued"],
+           "jobs_per_rate": args.serve_jobs, "hbm_budget_gb": args.serve_hbm_gb,
            "jobs_desc": f"Poisson arrivals, 50/50 Sedov (t_end {args.serve_sedov_t_end}) / isentropic vortex "
                         f"(t_end {args.serve_vortex_t_end}), side log-uniform in [32, {args.serve_side_max}], "
                         "gamma 1.4, FP32 flux / FP64 accumulate; the same job draw at every rate"}
@@ -710,6 +717,8 @@ def main():
     ap.add_argument("--kh-side", type=int, default=128)
     ap.add_argument("--kh-steps", type=int, default=20)
     ap.add_argument("--kh-warmup", type=int, default=80)
+    ap.add_argument("--kh-precision", default="both", choices=["both", "fp64", "fp32"],
+                    help="3D workload: the precisions run (profiling runs take one)")
     ap.add_argument("--kh-l1-t", type=float, default=0.002,
                     help="3D workload: end time of the L1 check against the oracle (12 engine steps at 0.002)")
     ap.add_argument("--no-oracle", action="store_true", help="skip the L1 checks against the oracle")
@@ -719,6 +728,8 @@ def main():
     ap.add_argument("--serve-sedov-t-end", type=float, default=0.05)
     ap.add_argument("--serve-vortex-t-end", type=float, default=0.5)
     ap.add_argument("--serve-hbm-gb", type=float, default=24.0)
+    ap.add_argument("--serve-sat-jobs", type=int, default=96, help="jobs of the saturation stream")
+    ap.add_argument("--serve-sat-hbm-gb", type=float, default=3.0, help="HBM budget of the saturation stream")
     ap.add_argument("--serve-poll", type=int, default=8)
     ap.add_argument("--serve-l1-jobs", type=int, default=2, help="smallest jobs of each kind checked against the oracle")
     ap.add_argument("--threads", type=int, default=0, help="host tessellation threads (0: all)")
@@ -808,13 +819,15 @@ def main():
                         "ms_per_step": s["us_per_step"] / 1e3, "dtype": "f64",
                         "config": {"workload": out["sedov2d"]["workload"]}})
         elif w == "kh3d":
-            s = out["kh3d"]["fp64"]
+            key = "fp64" if "fp64" in out["kh3d"] else "fp32_flux"
+            s = out["kh3d"][key]
             out.update({"value": s["cell_updates_per_s"], "steps": args.kh_steps, "warmup": args.kh_warmup,
-                        "ms_per_step": s["ms_per_step"], "dtype": "f64", "roofline": s["roofline"],
-                        "config": {"workload": out["kh3d"]["workload"]}})
+                        "ms_per_step": s["ms_per_step"], "dtype": "f64" if key == "fp64" else "f32 flux / f64 accumulate",
+                        "roofline": s["roofline"], "config": {"workload": out["kh3d"]["workload"]}})
         else:
-            s = max(out["serve"]["rates"], key=lambda x: x["cell_updates_per_s"])
-            out.update({"value": s["cell_updates_per_s"], "steps": s["jobs"], "warmup": 0,
+            s = out["serve"]["saturation"]
+            out.update({"value": s["cell_updates_per_s_while_queued"] or s["cell_updates_per_s"], "steps": s["jobs"],
+                        "warmup": 0,
                         "ms_per_step": 1e3 * s["wall_s"] / max(s["jobs"], 1), "dtype": "f32 flux / f64 accumulate",
                         "config": {"workload": "served_stream", "jobs": s["jobs"]}})
     print(json.dumps(out), flush=True)

@@ -85,6 +85,7 @@ class Server:
         self.launches_per_poll = launches_per_poll
         self.resident = {}
         self.launches = 0
+        self.samples = []  # (time, cell updates of every admitted job so far, jobs queued) at each poll
 
     def _fits(self, slot, p):
         cc, fc, kc = self.caps
@@ -133,8 +134,10 @@ class Server:
                 continue
             self.launches += self.batch.run(self.launches_per_poll)
             now = clock() - t0
+            updates = sum(j.cell_updates for j in done)
             for s, job in list(self.resident.items()):
                 info = self.batch.engine_info(s)
+                updates += info["cell_updates"]
                 if info["done"]:
                     job.finished = clock() - t0
                     job.steps = info["step_count"]
@@ -146,8 +149,18 @@ class Server:
                     self.free.append(s)
                     del self.resident[s]
                     done.append(job)
+            self.samples.append((now, updates, len(queue)))
         self.wall = clock() - t0
         return done
+
+    def saturated_rate(self):
+        """Cell updates per second while jobs waited in the queue (every slot they fit was busy): the
+        server's throughput at saturation, without the drain at the end of a finite stream; None when the
+        queue never held a job at two polls."""
+        busy = [(t, u) for t, u, q in self.samples if q > 0]
+        if len(busy) < 2 or busy[-1][0] <= busy[0][0]:
+            return None
+        return (busy[-1][1] - busy[0][1]) / (busy[-1][0] - busy[0][0])
 
 
 def latency_summary(jobs, wall):

@@ -18,6 +18,7 @@ VACUUM_SAMPLER = 100
 PERIODIC, REFLECTIVE, OPEN, VACUUM = range(4)
 FIXED, STEER, STEER_PAKMOR, FLUID = range(4)
 OPTIMAL_ORDER, GODUNOV = range(2)
+GRAV_NONE, GRAV_SELF, GRAV_CONSTANT, GRAV_KEPLERIAN = range(4)
 
 I64 = C.c_int64
 DP = C.POINTER(C.c_double)
@@ -40,7 +41,8 @@ class Params(C.Structure):
         ("dim", C.c_int), ("boundary", C.c_int), ("box", C.c_double * 3), ("gamma", C.c_double),
         ("eos", C.c_int), ("isothermal_u", C.c_double), ("cfl", C.c_double), ("riemann", Riemann),
         ("runner", C.c_int), ("motion", C.c_int), ("sync_all", C.c_int), ("t_end", C.c_double),
-        ("dt_snap", C.c_double),
+        ("dt_snap", C.c_double), ("gravity", C.c_int), ("grav_softening", C.c_double),
+        ("grav_position", C.c_double * 3), ("grav_acceleration", C.c_double * 3),
     ]
 
 
@@ -130,6 +132,10 @@ def lib():
         L.ora_phase_compute_fluxes.restype = None
         L.ora_sim_conservation.argtypes = [C.c_void_p, DP]
         L.ora_sim_conservation.restype = None
+        L.ora_gravity_accelerations.argtypes = [C.c_void_p, DP]
+        L.ora_gravity_accelerations.restype = None
+        L.ora_gravity_timesteps.argtypes = [C.c_void_p, DP]
+        L.ora_gravity_timesteps.restype = None
         L.ora_phase_gradients.argtypes = [C.c_void_p, C.POINTER(C.c_uint8)]
         L.ora_phase_gradients.restype = None
         L.ora_t_c2p.argtypes = [C.c_double, DP, C.c_double, DP]
@@ -229,8 +235,15 @@ def voronoi_1d(x, box_x, periodic=False):
 
 
 def make_params(dim, boundary, box, gamma, cfl, solver, threshold=2.0, runner=OPTIMAL_ORDER, motion=FIXED,
-                sync_all=False, t_end=1.0, dt_snap=0.0, eos=0, isothermal_u=0.0, velocity=(0.0, 0.0, 0.0)):
+                sync_all=False, t_end=1.0, dt_snap=0.0, eos=0, isothermal_u=0.0, velocity=(0.0, 0.0, 0.0),
+                gravity=GRAV_NONE, grav_softening=0.0, grav_position=(0.0, 0.0, 0.0),
+                grav_acceleration=(0.0, 0.0, 0.0)):
     p = Params()
+    p.gravity = gravity
+    p.grav_softening = grav_softening
+    for i in range(3):
+        p.grav_position[i] = grav_position[i]
+        p.grav_acceleration[i] = grav_acceleration[i]
     p.dim = dim
     p.boundary = boundary
     for i in range(3):
@@ -351,6 +364,16 @@ class Sim:
     def conservation(self):
         out = np.zeros(5)
         self.lib.ora_sim_conservation(self.ptr, _dp(out))
+        return out
+
+    def gravity_accelerations(self):
+        out = np.zeros(3 * self.n)
+        self.lib.ora_gravity_accelerations(self.ptr, _dp(out))
+        return out
+
+    def gravity_timesteps(self):
+        out = np.zeros(self.n)
+        self.lib.ora_gravity_timesteps(self.ptr, _dp(out))
         return out
 
     def gradients(self, active):

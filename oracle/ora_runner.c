@@ -789,6 +789,61 @@ static void timestep_limit(particle* p, int8_t wakeup, const tsinfo* t) {
   }
 }
 
+/* ======================= gravity (src/gravity.rs) ======================= */
+/* KeplerianPotential::acceleration (gravity.rs:127-138) */
+static vec3 kepler_acceleration(const ora_params* p, vec3 pos) {
+  vec3 dx = v3_sub(pos, v3(p->grav_position[0], p->grav_position[1], p->grav_position[2]));
+  double r2 = v3_len2(dx);
+  double r = sqrt(r2);
+  if (r < p->grav_softening) r2 = r2 + p->grav_softening * p->grav_softening;
+  double d = r * r2;
+  return v3(-dx.x / d, -dx.y / d, -dx.z / d);
+}
+/* KeplerianPotential::get_timestep + circular_velocity (gravity.rs:121-125, 140-148), eta = 0.5 */
+static double kepler_timestep(const ora_params* p, const particle* part) {
+  vec3 dx = v3_sub(part->loc, v3(p->grav_position[0], p->grav_position[1], p->grav_position[2]));
+  double r = v3_len(dx);
+  double r_softened = r < p->grav_softening ? r + p->grav_softening : r;
+  return 0.5 * radius(part, 2) / sqrt(1. / r_softened);
+}
+/* a particle's gravity timestep with a_grav = 0 (SelfGravity and Constant give infinity) */
+static double gravity_timestep(const ora_params* p, const particle* part) {
+  if (p->gravity == ORA_GRAV_KEPLERIAN) return kepler_timestep(p, part);
+  return INFINITY;
+}
+
+void ora_gravity_accelerations(const ora_sim* s, double* out3n) {
+  int64_t n = s->n;
+  const ora_params* p = &s->p;
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    vec3 a = V3_ZERO;
+    const particle* pi = &s->parts[i];
+    if (p->gravity == ORA_GRAV_SELF) {
+      /* SelfGravity::compute_accelerations (gravity.rs:24-44): only j > i contribute */
+      for (int64_t j = i + 1; j < n; ++j) {
+        const particle* pj = &s->parts[j];
+        double r = v3_len(v3_sub(pj->loc, pi->loc)) + p->grav_softening;
+        vec3 dir = v3_sub(pj->loc, pi->loc);
+        vec3 md = v3_scale(pj->conserved.c[0], dir);
+        double r3 = r * r * r;
+        a = v3_add(a, v3(md.x / r3, md.y / r3, md.z / r3));
+      }
+    } else if (p->gravity == ORA_GRAV_CONSTANT) {
+      a = v3(p->grav_acceleration[0], p->grav_acceleration[1], p->grav_acceleration[2]);
+    } else if (p->gravity == ORA_GRAV_KEPLERIAN) {
+      a = kepler_acceleration(p, pi->loc);
+    }
+    out3n[i] = a.x;
+    out3n[n + i] = a.y;
+    out3n[2 * n + i] = a.z;
+  }
+}
+
+void ora_gravity_timesteps(const ora_sim* s, double* out) {
+  for (int64_t i = 0; i < s->n; ++i) out[i] = gravity_timestep(&s->p, &s->parts[i]);
+}
+
 /* OptimalOrderRunner::step (optimal_order.rs:15-64) / GodunovHydroRunner::step (godunov.rs:14-58),
  * without the tessellation (the caller rebuilds the mesh between predict and fluxes). */
 static int mesh_update_1d(ora_sim* s, const uint8_t* mask);
@@ -829,6 +884,8 @@ int ora_sim_runner_step(ora_sim* s, uint64_t* ti_next_out) {
       if (!active[i]) continue;
       particle* p = &s->parts[i];
       double dts = particle_timestep(p, s->p.cfl, s->p.motion, &s->eos, s->p.dim, &lerr);
+      /* the minimum with the gravity solver's timestep (optimal_order.rs:43-47; f64::min) */
+      if (s->p.gravity != ORA_GRAV_NONE) dts = rmin(dts, gravity_timestep(&s->p, p));
       uint64_t dti = ora_make_integer_timestep(dts, p->timebin, NUM_TIME_BINS, t->ti_current, t->time_base_inv);
       p->dt = make_timestep(dti, t->time_base);
       p->timebin = ora_get_time_bin(dti);

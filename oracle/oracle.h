@@ -121,7 +121,13 @@ typedef struct {
   int sync_all;
   double t_end;
   double dt_snap;   /* dump sync points are drifted to (no file I/O); <=0 => none besides t=0 */
+  int gravity;      /* ORA_GRAV_* (src/gravity.rs) */
+  double grav_softening;
+  double grav_position[3];      /* Keplerian potential centre */
+  double grav_acceleration[3];  /* constant potential */
 } ora_params;
+/* GravitySolver kinds (src/gravity.rs): none, SelfGravity, ExternalPotentialGravity(Constant|Keplerian) */
+enum { ORA_GRAV_NONE = 0, ORA_GRAV_SELF = 1, ORA_GRAV_CONSTANT = 2, ORA_GRAV_KEPLERIAN = 3 };
 
 /* Mesh as produced by the tessellator (replaces meshless_voronoi output, src/space.rs:236-263). */
 typedef struct {
@@ -202,6 +208,12 @@ void ora_phase_compute_fluxes(ora_sim* s, const uint8_t* active, double* fluxrec
 void ora_sim_conservation(const ora_sim* s, double* out5);
 /* gradient_estimate/limit/apply (src/runner/hydrodynamics.rs:67-143) on the current primitives. */
 void ora_phase_gradients(ora_sim* s, const uint8_t* active);
+/* GravitySolver::compute_accelerations (src/gravity.rs:24-44, 64-69, 82-87, 116-148) on the current state:
+ * out [3N] component-major.  The reference's step never calls it (optimal_order.rs:33), so a_grav stays
+ * zero there; this returns the accelerations without storing them. */
+void ora_gravity_accelerations(const ora_sim* s, double* out3n);
+/* GravitySolver::compute_timesteps (src/gravity.rs:46-55, 71-74, 90-95, 107-110) with a_grav = 0: [N]. */
+void ora_gravity_timesteps(const ora_sim* s, double* out);
 
 /* Single functions of the path (ideal gas), so that tests can pin each against closed forms:
  * conserved -> primitive (physical_quantities.rs:213-226, part.rs:219-261), time_extrapolations

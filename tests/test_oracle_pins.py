@@ -457,3 +457,54 @@ def test_voronoi_1d_periodic_rotation(ora):
         assert int(np.count_nonzero(rot["shift"])) == 1
     with pytest.raises(RuntimeError):  # not a rotation: two downward steps
         ora.voronoi_1d(np.array([0.5, 0.1, 0.8, 0.2]), 1.0, periodic=True)
+
+
+# ----------------------------------------------------------------------------------- gravity
+def _gravity_sim(ora, n, gravity, soft, pos=(0.0, 0.0, 0.0), acc=(0.0, 0.0, 0.0), masses=None):
+    x = (np.arange(n) + 0.5) / n
+    mesh = ora.voronoi_1d(x, 1.0, periodic=True)
+    p = ora.make_params(1, ora.PERIODIC, (1.0, 1.0, 1.0), 1.4, 0.3, ora.HLLC, gravity=gravity, grav_softening=soft,
+                        grav_position=pos, grav_acceleration=acc)
+    loc = np.zeros((n, 3))
+    loc[:, 0] = x
+    m = mesh["volume"].copy() if masses is None else np.asarray(masses, float)
+    return ora.Sim(p, loc.ravel(), m, np.zeros(3 * n), np.ones(n), mesh), x, m
+
+
+def test_self_gravity_direct_sum(ora):
+    """SelfGravity::compute_accelerations (gravity.rs:24-44): particle i feels m_j (x_j - x_i) / (|x_j - x_i| +
+    eps)^3 from the particles after it (j > i) only; with eps = 0 that is Newton's m/r^2 toward each of them."""
+    sim, x, m = _gravity_sim(ora, 3, ora.GRAV_SELF, 0.0, masses=[1.0, 2.0, 4.0])
+    a = sim.gravity_accelerations().reshape(3, 3)
+    exp0 = 2.0 / (x[1] - x[0]) ** 2 + 4.0 / (x[2] - x[0]) ** 2
+    np.testing.assert_allclose(a[0], [exp0, 4.0 / (x[2] - x[1]) ** 2, 0.0], rtol=1e-14)
+    assert np.all(a[1:] == 0.0)
+    sim, x, m = _gravity_sim(ora, 3, ora.GRAV_SELF, 0.1, masses=[1.0, 2.0, 4.0])
+    a = sim.gravity_accelerations().reshape(3, 3)
+    np.testing.assert_allclose(a[0, 1], 4.0 * (x[2] - x[1]) / (x[2] - x[1] + 0.1) ** 3, rtol=1e-14)
+    assert np.all(np.isinf(sim.gravity_timesteps()))  # a_grav = 0 on the path: no gravity timestep
+
+
+def test_keplerian_potential(ora):
+    """KeplerianPotential (gravity.rs:101-148): a = -dx / |dx|^3 outside the softening length, -dx / (r (r^2 +
+    eps^2)) inside; timestep 0.5 R_2D / v_circ with v_circ = sqrt(1 / r) (r + eps inside), R_2D = sqrt(V/pi)."""
+    n, eps, c = 8, 0.2, (0.3, 0.0, 0.0)
+    sim, x, m = _gravity_sim(ora, n, ora.GRAV_KEPLERIAN, eps, pos=c)
+    a = sim.gravity_accelerations().reshape(3, n)
+    dt = sim.gravity_timesteps()
+    V = 1.0 / n
+    for i in range(n):
+        dx = x[i] - c[0]
+        r = abs(dx)
+        exp_a = -dx / r ** 3 if r >= eps else -dx / (r * (r * r + eps * eps))
+        np.testing.assert_allclose(a[0, i], exp_a, rtol=1e-14)
+        rs = r if r >= eps else r + eps
+        np.testing.assert_allclose(dt[i], 0.5 * np.sqrt(V / np.pi) * np.sqrt(rs), rtol=1e-14)
+    assert np.all(a[1:] == 0.0)
+
+
+def test_constant_potential(ora):
+    sim, x, m = _gravity_sim(ora, 4, ora.GRAV_CONSTANT, 0.0, acc=(0.0, -1.5, 0.25))
+    a = sim.gravity_accelerations().reshape(3, 4)
+    assert np.all(a[0] == 0.0) and np.all(a[1] == -1.5) and np.all(a[2] == 0.25)
+    assert np.all(np.isinf(sim.gravity_timesteps()))
