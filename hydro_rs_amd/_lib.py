@@ -38,7 +38,8 @@ class Params(C.Structure):
         ("eos", C.c_int), ("isothermal_u", C.c_double), ("cfl", C.c_double), ("solver", C.c_int),
         ("airs_threshold", C.c_double), ("linadv_velocity", C.c_double * 3), ("runner", C.c_int),
         ("motion", C.c_int), ("sync_all", C.c_int), ("t_end", C.c_double), ("dt_snap", C.c_double),
-        ("precision", C.c_int),
+        ("precision", C.c_int), ("gravity", C.c_int), ("grav_softening", C.c_double),
+        ("grav_position", C.c_double * 3), ("grav_acceleration", C.c_double * 3),
     ]
 
 
@@ -104,6 +105,8 @@ HFV_SYMBOLS = {
     "hfv_timesteps": (C.c_int, [C.c_void_p, C.POINTER(U64)]),
     "hfv_compute_timesteps": (C.c_int, [C.c_void_p, DP]),
     "hfv_apply_timesteps": (C.c_int, [C.c_void_p, DP, C.POINTER(U64)]),
+    "hfv_gravity_timesteps": (C.c_int, [C.c_void_p, DP]),
+    "hfv_gravity_accelerations": (C.c_int, [C.c_void_p, DP]),
     "hfv_step": (C.c_int, [C.c_void_p, C.POINTER(TimestepInfo), C.POINTER(U64)]),
     "hfv_run": (C.c_int, [C.c_void_p, I64, C.POINTER(I64)]),
     "hfv_get_engine_info": (C.c_int, [C.c_void_p, C.POINTER(EngineInfo)]),

@@ -90,6 +90,12 @@ Cfg make_cfg(const hfv_params& p) {
   for (int a = 0; a < 3; ++a) c.solver.linv[a] = p.linadv_velocity[a];
   c.gd = make_gas<double>(p.gamma, p.eos, p.isothermal_u);
   c.gf = make_gas<float>(p.gamma, p.eos, p.isothermal_u);
+  c.gravity = p.gravity;
+  c.grav_soft = p.grav_softening;
+  for (int a = 0; a < 3; ++a) {
+    c.grav_pos[a] = p.grav_position[a];
+    c.grav_acc[a] = p.grav_acceleration[a];
+  }
   return c;
 }
 
@@ -104,6 +110,7 @@ int check_params(const hfv_params* p) {
   if (p->eos < 0 || p->eos > 1) return set_err(HFV_ERR_INVALID, "unknown equation of state");
   if (!(p->gamma > 1.)) return set_err(HFV_ERR_INVALID, "gamma must be > 1");
   if (!(p->t_end > 0.)) return set_err(HFV_ERR_INVALID, "t_end must be > 0");
+  if (p->gravity < HFV_GRAV_NONE || p->gravity > HFV_GRAV_KEPLERIAN) return set_err(HFV_ERR_INVALID, "unknown gravity");
   return HFV_OK;
 }
 
@@ -183,6 +190,7 @@ struct hfv_ctx {
   BlockJob *dcjobs = nullptr, *dfjobs = nullptr;
   Partials P{};
   int2* fall = nullptr;      // exact-solver fallback queue (AIRS)
+  double* gbuf = nullptr;    // gravity solver output [3n], allocated on first use
   bool use_static = true;    // precomputed geometry for fixed meshes (hfv_set_geometry_mode)
   std::vector<int> v_zero;   // per slot: every particle velocity is +0 (fixed meshes keep it so)
   std::vector<int> has_stage;  // per slot: some face chunk has a staging list
@@ -1443,6 +1451,33 @@ int hfv_apply_timesteps(hfv_ctx* c, const double* dt, uint64_t* ti_next) {
   if (ti_next) *ti_next = c->heng[0].ti_next;
   return HFV_OK;
 }
+
+// ---- GravitySolver trait (src/gravity.rs:6-10)
+static int gravity_phase(hfv_ctx* c, double* out, bool accelerations) {
+  TRY(require_mesh(c));
+  if (!out) return set_err(HFV_ERR_INVALID, "out is NULL");
+  if (c->params[0].gravity == HFV_GRAV_NONE) return set_err(HFV_ERR_INVALID, "the context has no gravity solver");
+  if (!c->gbuf && !(c->gbuf = dalloc<double>(c, 3 * c->n))) return set_err(HFV_ERR_HIP, "out of device memory");
+  settle(c, -1);
+  const int64_t n = c->nused[0], cb = c->cbase[0];
+  const Cfg cfg = make_cfg(c->params[0]);
+  const unsigned grid = (unsigned)((n + BLOCK - 1) / BLOCK);
+  if (n > 0) {
+    if (accelerations)
+      hipLaunchKernelGGL(k_grav_acc, dim3(grid), dim3(BLOCK), 0, c->stream, c->batch_args(), cfg, cb, n, c->gbuf);
+    else
+      hipLaunchKernelGGL(k_grav_dt, dim3(grid), dim3(BLOCK), 0, c->stream, c->batch_args(), cfg, cb, n, c->gbuf);
+    HIP_OK(hipGetLastError());
+  }
+  const int64_t width = accelerations ? 3 : 1;
+  HIP_OK(hipMemcpyAsync(out, c->gbuf, sizeof(double) * width * n, hipMemcpyDeviceToHost, c->stream));
+  HIP_OK(hipStreamSynchronize(c->stream));
+  return HFV_OK;
+}
+
+int hfv_gravity_timesteps(hfv_ctx* c, double* dt_out) { return gravity_phase(c, dt_out, false); }
+
+int hfv_gravity_accelerations(hfv_ctx* c, double* a_out) { return gravity_phase(c, a_out, true); }
 
 // ---- fused step with the caller's TimestepInfo
 int hfv_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {

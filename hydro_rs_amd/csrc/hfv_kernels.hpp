@@ -79,6 +79,10 @@ struct Cfg {
   SolverCfg solver;
   Gas<double> gd;
   Gas<float> gf;
+  int gravity;  // HFV_GRAV_*
+  double grav_soft;
+  double grav_pos[3];
+  double grav_acc[3];
 };
 
 struct SimDesc {
@@ -1661,6 +1665,15 @@ __device__ __forceinline__ double cell_radius(double V, const Cfg& cfg) {
   return pow(cfg.radius3_fac * V, 1. / 3.);
 }
 
+// KeplerianPotential::get_timestep (gravity.rs:120-123) with circular_velocity (gravity.rs:138-148):
+// eta = 0.5 times the particle's 2D radius over sqrt(1 / r), r softened below the softening length.
+__device__ __forceinline__ double kepler_timestep(const Cfg& cfg, V3<double> loc, double V) {
+  const V3<double> dx = loc - mk3(cfg.grav_pos[0], cfg.grav_pos[1], cfg.grav_pos[2]);
+  const double r = len(dx);
+  const double r_soft = r < cfg.grav_soft ? r + cfg.grav_soft : r;
+  return 0.5 * sqrt(0.318309886183790671537767526745028724 * V) / sqrt(1. / r_soft);
+}
+
 // DMat3::inverse (glam 0.27): cofactors by cross products, det = z . (x cross y), x 1/det, transposed;
 // returned as the three columns of the inverse.
 __device__ __forceinline__ M3 inverse_cols(const M3& m) {
@@ -1993,6 +2006,9 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
     C.msv[i] = 0.;
     dt_req = v_max > 0. ? cfg.cfl * cell_radius(V, cfg) / v_max : INFINITY;
   }
+  // the minimum with the gravity solver's timestep (optimal_order.rs:47-50, godunov.rs:41-44); only the
+  // Keplerian potential's is finite while a_grav = 0 (gravity.rs:45-54, 90-95)
+  if (do_ts == 1 && cfg.gravity == HFV_GRAV_KEPLERIAN) dt_req = fmin(dt_req, kepler_timestep(cfg, ld3(C.loc, n, i), C.vol[i]));
   if (do_ts == 2) {  // FiniteVolumeSolver::compute_timesteps alone: the requested dt, not yet applied
     C.dtq[i] = dt_req;
     st_tex(C, i, cfg.precision == HFV_FP32_FLUX, fr_rate(prim, gr, vrel_now, cfg.gd));
@@ -2099,6 +2115,67 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_ts_apply(Batch B) {
   }
   const uint64_t t = block_min_u64(my);
   if (threadIdx.x == 0) B.P.ti[J.chunk] = t;
+}
+
+// GravitySolver::compute_timesteps (gravity.rs:6-10) of cells [lo, lo + n) of a simulation, a_grav = 0.
+__global__ void __launch_bounds__(HFV_BLOCK) k_grav_dt(Batch B, Cfg cfg, int64_t lo, int64_t n, double* out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  const int64_t i = lo + k;
+  out[k] = cfg.gravity == HFV_GRAV_KEPLERIAN ? kepler_timestep(cfg, ld3(B.C.loc, B.C.n, i), B.C.vol[i]) : INFINITY;
+}
+
+// GravitySolver::compute_accelerations (gravity.rs:6-10) of cells [lo, lo + n); out [3n] component-major.
+// Self gravity (gravity.rs:23-44): particle k sums m_j (x_j - x_k) / r^3, r = |x_j - x_k| + softening, over
+// j = k+1 .. n-1 in increasing order (the reference's order, so sums agree bit for bit).  The block's tiles
+// of positions and masses go through LDS; blocks of lower k have more partners and start first.
+__global__ void __launch_bounds__(HFV_BLOCK) k_grav_acc(Batch B, Cfg cfg, int64_t lo, int64_t n, double* out) {
+  __shared__ double tx[HFV_BLOCK], ty[HFV_BLOCK], tz[HFV_BLOCK], tm[HFV_BLOCK];
+  const int64_t k0 = (int64_t)blockIdx.x * HFV_BLOCK;
+  const int64_t k = k0 + threadIdx.x;
+  const int64_t nc = B.C.n;
+  const double* L = B.C.loc;
+  V3<double> a = mk3(0., 0., 0.);
+  if (cfg.gravity == HFV_GRAV_SELF) {
+    const V3<double> xi = k < n ? ld3(L, nc, lo + k) : mk3(0., 0., 0.);
+    for (int64_t t0 = k0; t0 < n; t0 += HFV_BLOCK) {  // block-uniform trip count
+      const int64_t j = t0 + threadIdx.x;
+      __syncthreads();
+      if (j < n) {
+        tx[threadIdx.x] = L[lo + j];
+        ty[threadIdx.x] = L[nc + lo + j];
+        tz[threadIdx.x] = L[2 * nc + lo + j];
+        tm[threadIdx.x] = B.C.cons[lo + j];
+      }
+      __syncthreads();
+      const int cnt = (int)(n - t0 < HFV_BLOCK ? n - t0 : HFV_BLOCK);
+      if (k < n) {
+        for (int q = 0; q < cnt; ++q) {
+          if (t0 + q <= k) continue;
+          const V3<double> dir = mk3(tx[q] - xi.x, ty[q] - xi.y, tz[q] - xi.z);
+          const double r = len(dir) + cfg.grav_soft;
+          const double r3 = r * r * r;
+          a = a + mk3(tm[q] * dir.x / r3, tm[q] * dir.y / r3, tm[q] * dir.z / r3);
+        }
+      }
+    }
+  } else if (k < n) {
+    if (cfg.gravity == HFV_GRAV_CONSTANT) {
+      a = mk3(cfg.grav_acc[0], cfg.grav_acc[1], cfg.grav_acc[2]);
+    } else {  // KeplerianPotential::acceleration (gravity.rs:125-136)
+      const V3<double> dx = ld3(L, nc, lo + k) - mk3(cfg.grav_pos[0], cfg.grav_pos[1], cfg.grav_pos[2]);
+      double r2 = dot(dx, dx);
+      const double r = sqrt(r2);
+      if (r < cfg.grav_soft) r2 = r2 + cfg.grav_soft * cfg.grav_soft;
+      const double d = r * r2;
+      a = mk3(-dx.x / d, -dx.y / d, -dx.z / d);
+    }
+  }
+  if (k < n) {
+    out[k] = a.x;
+    out[n + k] = a.y;
+    out[2 * n + k] = a.z;
+  }
 }
 
 // ------------------------------------------------------------------ static geometry

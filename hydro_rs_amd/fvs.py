@@ -25,6 +25,8 @@ MOTIONS = {"fixed": FIXED, "steer": STEER, "steer_pakmor": STEER_PAKMOR, "fluid"
 OPTIMAL_ORDER, GODUNOV = range(2)
 RUNNERS = {"OptimalOrder": OPTIMAL_ORDER, "Godunov": GODUNOV}
 FP64, FP32_FLUX = range(2)
+# gravity solvers (src/bin/main.rs:643-666)
+GRAV_NONE, GRAV_SELF, GRAV_CONSTANT, GRAV_KEPLERIAN = range(4)
 
 FIELD_WIDTH = {"prim": 5, "grad": 15, "gc": 3, "extrap": 5, "cons": 5, "fluxes": 5, "volume": 1, "loc": 3,
                "centroid": 3, "v": 3, "v_rel": 3, "maor": 1, "msv": 1, "dt": 1}
@@ -50,6 +52,10 @@ class Config:
     eos: int = 0
     isothermal_u: float = 0.0
     precision: int = FP64
+    gravity: int = GRAV_NONE
+    grav_softening: float = 0.0
+    grav_position: tuple = (0.0, 0.0, 0.0)
+    grav_acceleration: tuple = (0.0, 0.0, 0.0)
     extra: dict = field(default_factory=dict)
 
     def to_c(self):
@@ -71,6 +77,11 @@ class Config:
         p.t_end = self.t_end
         p.dt_snap = self.dt_snap
         p.precision = self.precision
+        p.gravity = self.gravity
+        p.grav_softening = self.grav_softening
+        for a in range(3):
+            p.grav_position[a] = float(self.grav_position[a])
+            p.grav_acceleration[a] = float(self.grav_acceleration[a])
         return p
 
 
@@ -198,6 +209,19 @@ class EulerEqnsFvs:
         t = _lib.U64(0)
         _lib.check(self.lib.hfv_apply_timesteps(self.ptr, _dp(d), C.byref(t)))
         return t.value
+
+    # ------------------------------------------------------------ GravitySolver
+    def gravity_timesteps(self):
+        """GravitySolver::compute_timesteps of every particle (a_grav = 0, as on the step path)."""
+        out = np.zeros(self.n)
+        _lib.check(self.lib.hfv_gravity_timesteps(self.ptr, _dp(out)))
+        return out
+
+    def gravity_accelerations(self):
+        """GravitySolver::compute_accelerations at the current positions and masses, [3, N]."""
+        out = np.zeros(3 * self.n)
+        _lib.check(self.lib.hfv_gravity_accelerations(self.ptr, _dp(out)))
+        return out.reshape(3, self.n)
 
     # ------------------------------------------------------------ step / engine
     def step(self, info):

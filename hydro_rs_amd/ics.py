@@ -9,7 +9,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .fvs import AIRS, EXACT, FIXED, FLUID, HLLC, PERIODIC, REFLECTIVE, Config
+from .fvs import AIRS, EXACT, FIXED, FLUID, GRAV_KEPLERIAN, HLLC, PERIODIC, REFLECTIVE, Config
 from .mesh import VoronoiMesh
 
 
@@ -133,6 +133,26 @@ def isentropic_vortex_2d(side=64, seed=5, t_end=1.0, solver=HLLC, gamma=1.4, bet
     cfg = Config(dim=2, box=box, gamma=gamma, cfl=0.3, solver=solver, boundary=PERIODIC, motion=FIXED,
                  t_end=t_end, **kw)
     return _finish(f"vortex2d_{side}", cfg, loc, rho, vel, P, threads)
+
+
+def keplerian_disc_2d(side=24, seed=7, t_end=0.05, solver=HLLC, gamma=5.0 / 3.0, softening=0.05, threads=0, **kw):
+    """2D gas around a Keplerian point mass at the centre of the unit box (reflective walls): rho 1, a
+    pressure bump P = 0.01 + 0.5 exp(-r^2 / 0.01) and a slow solid-body rotation v = 0.5 r.  The
+    Keplerian timestep (eta R / v_circ) binds in the cold outer gas, the hydro one in the bump."""
+    rng = np.random.default_rng(seed)
+    box = (1.0, 1.0, 1.0)
+    loc = jittered_grid(2, side, box, 0.05, rng)
+    xy = loc[:, :2] - 0.5
+    r2 = np.sum(xy * xy, axis=1)
+    rho = np.ones(loc.shape[0])
+    P = 0.01 + 0.5 * np.exp(-r2 / 0.01)
+    vel = np.zeros((loc.shape[0], 3))
+    vel[:, 0] = -0.5 * xy[:, 1]
+    vel[:, 1] = 0.5 * xy[:, 0]
+    kw.setdefault("gravity", GRAV_KEPLERIAN)
+    cfg = Config(dim=2, box=box, gamma=gamma, cfl=0.3, solver=solver, boundary=REFLECTIVE, motion=FIXED,
+                 t_end=t_end, grav_softening=softening, grav_position=(0.5, 0.5, 0.0), **kw)
+    return _finish(f"kepler2d_{side}", cfg, loc, rho, vel, P, threads)
 
 
 def riemann_tube_1d(n, rho_l, p_l, v_l, rho_r, p_r, v_r, gamma, seed=0, solver=HLLC, t_end=0.1, **kw):

@@ -72,6 +72,12 @@ extern "C" {
  * rounding only. */
 #define HFV_FP64 0
 #define HFV_FP32_FLUX 1
+/* Gravity solver (src/bin/main.rs:643-666): none, SelfGravity (src/gravity.rs:12-55), the external
+ * constant potential and the external Keplerian potential (gravity.rs:57-149). */
+#define HFV_GRAV_NONE 0
+#define HFV_GRAV_SELF 1
+#define HFV_GRAV_CONSTANT 2
+#define HFV_GRAV_KEPLERIAN 3
 
 /* ---------------------------------------------------------------- types */
 /* EulerEqnsFvs::new + GasLaw + engine configuration (src/finite_volume_solver/euler_eqns_fvs.rs:7-17,
@@ -93,6 +99,10 @@ typedef struct {
   double t_end;
   double dt_snap;          /* dump sync points (drift only, no file output); <= 0: only t = 0 */
   int precision;           /* HFV_FP64 / HFV_FP32_FLUX */
+  int gravity;             /* HFV_GRAV_NONE .. HFV_GRAV_KEPLERIAN (gravity: kind / potential: kind) */
+  double grav_softening;   /* gravity:softening_length (self gravity, Keplerian potential) */
+  double grav_position[3]; /* gravity:potential:position (Keplerian) */
+  double grav_acceleration[3]; /* gravity:potential:acceleration (constant) */
 } hfv_params;
 
 /* Face-indexed mesh (host arrays) as produced by the tessellator; replaces the VoronoiFace slice and
@@ -220,6 +230,22 @@ int hfv_compute_timesteps(hfv_ctx* ctx, double* dt_out);
  * (src/runner/hydrodynamics.rs:145-195, optimal_order.rs:48-59) of the requested timesteps dt [N];
  * writes ti_current + min(dti) to *ti_next.  hfv_timesteps is hfv_compute_timesteps followed by this. */
 int hfv_apply_timesteps(hfv_ctx* ctx, const double* dt, uint64_t* ti_next);
+
+/* ---------------------------------------------------------------- GravitySolver trait */
+/* GravitySolver::compute_timesteps (src/gravity.rs:6-10) for every particle, as the runner step sees it
+ * (a_grav = 0): the Keplerian potential's eta * radius_2D / v_circ (gravity.rs:121-125, 140-148), +inf for
+ * self gravity and the constant potential (gravity.rs:45-54, 98-103); dt_out [N].  The fused step
+ * (hfv_step, hfv_run, batches) takes the minimum with the hydro timesteps itself (optimal_order.rs:47-50,
+ * godunov.rs:41-44); with hfv_compute_timesteps / hfv_apply_timesteps the caller takes it.
+ * HFV_ERR_INVALID when the context has no gravity solver (HFV_GRAV_NONE). */
+int hfv_gravity_timesteps(hfv_ctx* ctx, double* dt_out);
+/* GravitySolver::compute_accelerations (src/gravity.rs:6-10) at the current positions and masses: self
+ * gravity's direct sum over the particles after each one (gravity.rs:24-44), the constant vector
+ * (gravity.rs:82-89) or the softened Keplerian point mass (gravity.rs:127-138); a_out [3N] component-major.
+ * The reference stores them in Particle::a_grav; this entry point only returns them.  No runner step calls
+ * compute_accelerations (optimal_order.rs:33, godunov.rs:32), so on the step path a_grav stays zero and the
+ * gravity kicks (hydrodynamics.rs:41-57) add nothing.  HFV_ERR_INVALID without a gravity solver. */
+int hfv_gravity_accelerations(hfv_ctx* ctx, double* a_out);
 
 /* ---------------------------------------------------------------- fused step and engine */
 /* OptimalOrderRunner::step / GodunovHydroRunner::step without the tessellation

@@ -13,7 +13,9 @@ def oracle_sim(prob):
     c = prob.cfg
     p = ora.make_params(c.dim, c.boundary, c.box, c.gamma, c.cfl, c.solver, threshold=c.airs_threshold,
                         runner=c.runner, motion=c.motion, sync_all=c.sync_all, t_end=c.t_end, dt_snap=c.dt_snap,
-                        eos=c.eos, isothermal_u=c.isothermal_u, velocity=c.linadv_velocity)
+                        eos=c.eos, isothermal_u=c.isothermal_u, velocity=c.linadv_velocity,
+                        gravity=c.gravity, grav_softening=c.grav_softening, grav_position=c.grav_position,
+                        grav_acceleration=c.grav_acceleration)
     return ora.Sim(p, prob.loc.ravel(), prob.mass, prob.vel.ravel(), prob.u_int, prob.mesh.as_dict())
 
 
