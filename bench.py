@@ -629,14 +629,18 @@ def bench_kh3d(args):
         from dataclasses import replace
 
         short = replace(short, cfg=replace(short.cfg, t_end=args.kh_l1_t))
-        log(f"[kh3d] FP32 flux and the FP64 oracle to t = {args.kh_l1_t} (L1 check)")
+        log(f"[kh3d] FP64, FP32 flux and the FP64 oracle to t = {args.kh_l1_t} (L1 check)")
         d32 = ics.build_device_sim(short)
         n32 = d32.run()
         r32 = d32.get_state(["prim"])["prim"][: prob.n]
         d32.close()
+        d64 = ics.build_device_sim(ics.with_precision(short, fvs.FP64))
+        n64 = d64.run()
+        r64 = d64.get_state(["prim"])["prim"][: prob.n]
+        d64.close()
         ref, oi = oracle_run(short)
-        out["l1_vs_oracle"] = {"fp32_flux": _l1(r32, ref), "at": f"t_end {args.kh_l1_t}",
-                               "device_steps": n32, "oracle_steps": oi["step_count"],
+        out["l1_vs_oracle"] = {"fp32_flux": _l1(r32, ref), "fp64": _l1(r64, ref), "at": f"t_end {args.kh_l1_t}",
+                               "device_steps": {"fp32_flux": n32, "fp64": n64}, "oracle_steps": oi["step_count"],
                                "oracle_seconds": time.perf_counter() - t, "pass": _l1(r32, ref) < 1e-5}
         log(f"[kh3d] L1 vs oracle: {out['l1_vs_oracle']}")
     return out
@@ -672,10 +676,23 @@ def bench_serve(args):
         last = done
     # saturation: more jobs than the slots of a smaller budget, all queued at t = 0, so that every slot a job
     # fits is busy until the queue drains; throughput while jobs wait in the queue
-    sat, _ = serve_once(args, 0.0, args.serve_sat_jobs, args.serve_sat_hbm_gb, seed=6)
+    sat, sat_done = serve_once(args, 0.0, args.serve_sat_jobs, args.serve_sat_hbm_gb, seed=6)
     log(f"[serve] saturation: {sat}")
+    # the plain FP64 path on the same jobs: a size-stratified sample of the saturation stream, each job alone
+    # in its own single-simulation context (hfv_run, graph replays) from t = 0 to t_end
+    from hydro_rs_amd import ensemble
+
+    sample = sorted(sat_done, key=lambda j: j.prob.n)[::args.serve_plain_stride]
+    log(f"[serve] plain FP64 path on {len(sample)} of the saturation stream's jobs")
+    plain = ensemble.run_plain([ics.with_precision(j.prob, fvs.FP64) for j in sample])
+    plain_rate = plain["cell_updates"] / plain["seconds"] if plain["seconds"] > 0 else None
+    sat_rate = sat["cell_updates_per_s_while_queued"]
     out = {"rates": table, "saturation": sat,
-           "saturated_cell_updates_per_s": sat["cell_updates_per_s_while_queued"],
+           "saturated_cell_updates_per_s": sat_rate,
+           "plain_fp64_same_jobs": {"cell_updates_per_s": plain_rate, "jobs": len(sample),
+                                    "cells": [int(j.prob.n) for j in sample], "steps": plain["steps"],
+                                    "seconds": plain["seconds"], "failed": plain["failed"]},
+           "saturated_vs_plain_fp64_same_jobs": (sat_rate / plain_rate) if sat_rate and plain_rate else None,
            "jobs_per_rate": args.serve_jobs, "hbm_budget_gb": args.serve_hbm_gb,
            "jobs_desc": f"Poisson arrivals, 50/50 Sedov (t_end {args.serve_sedov_t_end}) / isentropic vortex "
                         f"(t_end {args.serve_vortex_t_end}), side log-uniform in [32, {args.serve_side_max}], "
@@ -713,6 +730,8 @@ def main():
     ap.add_argument("--kh-tune", type=int, default=8, help="3D workload autotuner: launches per timed group of a trial (0: none)")
     ap.add_argument("--kh-tune-rounds", type=int, default=2,
                     help="3D workload autotuner: rounds over the candidates (each samples another part of the time-bin cycle)")
+    ap.add_argument("--serve-plain-stride", type=int, default=8,
+                    help="serving: every k-th job of the saturation stream (by size) run alone on the plain FP64 path")
     ap.add_argument("--sedov-side", type=int, default=256)
     ap.add_argument("--kh-side", type=int, default=128)
     ap.add_argument("--kh-steps", type=int, default=20)

@@ -111,6 +111,7 @@ HFV_SYMBOLS = {
     "hfv_run": (C.c_int, [C.c_void_p, I64, C.POINTER(I64)]),
     "hfv_get_engine_info": (C.c_int, [C.c_void_p, C.POINTER(EngineInfo)]),
     "hfv_conservation": (C.c_int, [C.c_void_p, DP]),
+    "hfv_get_mesh": (C.c_int, [C.c_void_p, C.POINTER(MeshArrays)]),
     "hfv_riemann_batch": (C.c_int, [C.POINTER(Params), I64, DP, DP, DP, DP, DP]),
     "hfv_riemann_batch_noalloc": (C.c_int, [C.POINTER(Params), I64, DP, DP, DP, DP, DP]),
     "hfv_batch_create": (C.c_int, [C.c_int, I64P, I64P, I64P, C.POINTER(C.c_void_p)]),

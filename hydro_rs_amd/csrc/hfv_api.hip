@@ -17,6 +17,7 @@
 
 #include "../../include/hfv.h"
 #include "hfv_kernels.hpp"
+#include "hfv_voronoi2d.hpp"
 
 using namespace hfv;
 
@@ -191,6 +192,8 @@ struct hfv_ctx {
   Partials P{};
   int2* fall = nullptr;      // exact-solver fallback queue (AIRS)
   double* gbuf = nullptr;    // gravity solver output [3n], allocated on first use
+  V2Bufs v2{};               // 2D device rebuild (allocated when a 2D moving simulation is loaded)
+  bool v2_alloc = false;
   bool use_static = true;    // precomputed geometry for fixed meshes (hfv_set_geometry_mode)
   std::vector<int> v_zero;   // per slot: every particle velocity is +0 (fixed meshes keep it so)
   std::vector<int> has_stage;  // per slot: some face chunk has a staging list
@@ -209,6 +212,7 @@ struct hfv_ctx {
   int precision = HFV_FP64;
   int solver = -1;  // uniform solver of the loaded simulations, or -1
   bool any_moving1d = false;
+  bool any_moving2d = false;
   bool any_moving = false;
   hipGraph_t graph{};
   hipGraphExec_t gexec{};
@@ -261,6 +265,9 @@ T* dalloc(hfv_ctx* c, int64_t count) {
 void free_all(hfv_ctx* c) {
   for (void* p : c->bufs) (void)hipFree(p);
   c->bufs.clear();
+  c->v2 = V2Bufs{};
+  c->v2_alloc = false;
+  c->gbuf = nullptr;
 }
 
 void drop_graph(hfv_ctx* c) {
@@ -269,6 +276,41 @@ void drop_graph(hfv_ctx* c) {
   c->gexec = nullptr;
   c->graph = nullptr;
   c->graph_ok = false;
+}
+
+// Buffers of the 2D device rebuild for the whole capacity (hfv_voronoi2d.hpp), on first need.
+int ensure_v2(hfv_ctx* c) {
+  if (c->v2_alloc) return HFV_OK;
+  const int64_t ns = c->n + c->nslots + 1;
+  V2Bufs v;
+  v.gcnt = dalloc<int32_t>(c, ns);
+  v.gfill = dalloc<int32_t>(c, ns);
+  v.femit = dalloc<int32_t>(c, ns);
+  v.ccnt = dalloc<int32_t>(c, ns);
+  v.gord = dalloc<int32_t>(c, c->n);
+  v.gkey = dalloc<int32_t>(c, c->n);
+  v.vst = dalloc<int32_t>(c, c->n);
+  v.cfill = dalloc<int32_t>(c, c->n);
+  v.vf = dalloc<V2Face>(c, c->nf);
+  if (!v.gcnt || !v.gfill || !v.femit || !v.ccnt || !v.gord || !v.gkey || !v.vst || !v.cfill || !v.vf)
+    return set_err(HFV_ERR_HIP, "out of device memory (2D rebuild buffers)");
+  c->v2 = v;
+  c->v2_alloc = true;
+  drop_graph(c);  // a captured step holds the old buffer set
+  return HFV_OK;
+}
+
+// The host tessellator's bin grid for n generators in a 2D box (hydro_rs_amd/csrc/host/voronoi.cpp,
+// Grid::build), evaluated with the same operations.
+int v2_grid(const hfv_params& p, int64_t n, SimDesc& D) {
+  const double vol = p.box[0] * p.box[1];
+  const double spacing = std::sqrt(vol / (double)std::max<int64_t>(n, 1));
+  for (int a = 0; a < 2; ++a) {
+    D.v2nb[a] = std::max(1, (int)std::floor(p.box[a] / spacing));
+    D.v2w[a] = p.box[a] / D.v2nb[a];
+  }
+  if ((int64_t)D.v2nb[0] * D.v2nb[1] > n) return set_err(HFV_ERR_INVALID, "internal: more bins than generators");
+  return HFV_OK;
 }
 
 // Block-cyclic placement of live chunks on the 8 XCDs (the XCD that runs grid position b is b % 8):
@@ -297,6 +339,10 @@ int64_t grid_size(int64_t live, int64_t g) {
 // Position of live chunk j: group j / g goes to XCD (j / g) % 8, in round j / (8 g).
 int64_t grid_position(int64_t j, int64_t g) { return 8 * ((j / g / 8) * g + j % g) + (j / g) % 8; }
 
+// Faces the face kernels cover: the mesh's, or the whole capacity where the device rebuild changes the
+// face count (the faces past the current mesh have no cells and are skipped).
+int64_t face_range(const hfv_ctx* c, int s) { return c->desc[s].moving2d ? c->fcap[s] : c->fused[s]; }
+
 // Rebuild the block tables from the live slots (loaded, and in a batch not finished: a finished slot's
 // blocks would only exit) and upload them with the simulation descriptors.  Chunk ids (partials) stay
 // global.
@@ -307,7 +353,7 @@ int write_jobs(hfv_ctx* c) {
   for (int s = 0; s < S; ++s) {
     if (!live(s)) continue;
     lc += (c->nused[s] + BLOCK - 1) / BLOCK;
-    lf += (c->fused[s] + BLOCK - 1) / BLOCK;
+    lf += (face_range(c, s) + BLOCK - 1) / BLOCK;
   }
   const int64_t gc = group_size(c, lc), gf = group_size(c, lf);
   const int64_t ncg = grid_size(lc, gc), nfg = grid_size(lf, gf);
@@ -321,7 +367,7 @@ int write_jobs(hfv_ctx* c) {
   for (int s = 0; s < S; ++s) {
     SimDesc& D = c->desc[s];
     const int64_t nc = c->loaded[s] ? (c->nused[s] + BLOCK - 1) / BLOCK : 0;
-    const int64_t nfc = c->loaded[s] ? (c->fused[s] + BLOCK - 1) / BLOCK : 0;
+    const int64_t nfc = c->loaded[s] ? (face_range(c, s) + BLOCK - 1) / BLOCK : 0;
     D.cchunk0 = (int32_t)c->cchunk0[s];
     D.cchunk1 = (int32_t)(c->cchunk0[s] + nc);
     D.fchunk0 = (int32_t)c->fchunk0[s];
@@ -336,7 +382,7 @@ int write_jobs(hfv_ctx* c) {
     for (int64_t j = 0; j < nfc; ++j, ++jf) {
       const int64_t k = c->fchunk0[s] + j;
       const int64_t lo = c->fbase[s] + j * BLOCK;
-      const int64_t hi = std::min<int64_t>(lo + BLOCK, c->fbase[s] + c->fused[s]);
+      const int64_t hi = std::min<int64_t>(lo + BLOCK, c->fbase[s] + face_range(c, s));
       c->hfjobs[grid_position(jf, gf)] = BlockJob{s, (int32_t)lo, (int32_t)hi, (int32_t)k};
     }
   }
@@ -523,6 +569,7 @@ int push_engines(hfv_ctx* c) {
 void recompute_uniforms(hfv_ctx* c) {
   int sol = -2;
   c->any_moving1d = false;
+  c->any_moving2d = false;
   c->any_moving = false;
   int prec = -1;
   for (int s = 0; s < c->nslots; ++s) {
@@ -531,6 +578,7 @@ void recompute_uniforms(hfv_ctx* c) {
     if (sol == -2) sol = p.solver;
     else if (sol != p.solver) sol = -1;
     if (p.dim == 1 && p.motion != HFV_FIXED) c->any_moving1d = true;
+    if (p.dim == 2 && p.motion != HFV_FIXED) c->any_moving2d = true;
     if (p.motion != HFV_FIXED) c->any_moving = true;
     if (prec < 0) prec = p.precision;
   }
@@ -550,14 +598,15 @@ void recompute_uniforms(hfv_ctx* c) {
 struct LaunchSet {
   int solver;
   int precision;
-  bool moving1d, moving, st, dyn, sgrad, dgrad;
+  bool moving1d, moving2d, moving, st, dyn, sgrad, dgrad;
   bool operator==(const LaunchSet& o) const {
-    return solver == o.solver && precision == o.precision && moving1d == o.moving1d && moving == o.moving &&
+    return solver == o.solver && precision == o.precision && moving1d == o.moving1d && moving2d == o.moving2d &&
+           moving == o.moving &&
            st == o.st && dyn == o.dyn && sgrad == o.sgrad && dgrad == o.dgrad;
   }
 };
 LaunchSet launch_set(const hfv_ctx* c) {
-  return LaunchSet{c->solver, c->precision, c->any_moving1d, c->any_moving, c->any_static,
+  return LaunchSet{c->solver, c->precision, c->any_moving1d, c->any_moving2d, c->any_moving, c->any_static,
                    c->any_dynamic, c->any_static_grad, c->any_dynamic_grad};
 }
 
@@ -746,6 +795,18 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
   D.nf = nf;
   D.moving1d = p.dim == 1 && p.motion != HFV_FIXED;
   D.periodic = p.boundary == HFV_PERIODIC;
+  D.moving2d = p.dim == 2 && p.motion != HFV_FIXED;
+  D.conn_base = c->kbase[s];
+  D.conn_cap = c->kcap[s];
+  D.face_cap = c->fcap[s];
+  D.noff = (int)(icap(c, s) + (s == c->nslots - 1 ? 1 : 0));
+  if (D.moving2d) {
+    TRY(v2_grid(p, n, D));
+    TRY(ensure_v2(c));
+    c->heng[s].nf_mesh = (int)nf;
+    HIP_OK(hipMemcpyAsync(&c->deng[s].nf_mesh, &c->heng[s].nf_mesh, sizeof(int), hipMemcpyHostToDevice, st));
+    HIP_OK(hipStreamSynchronize(st));
+  }
   if (c->loaded[s]) TRY(write_jobs(c));  // a live slot's mesh changed (moving mesh)
   // invariant: every slot's first CSR offset is its CSR base (this slot and the next one)
   for (int t = s; t <= std::min(s + 1, c->nslots - 1); ++t) {
@@ -783,6 +844,7 @@ int reset_slot(hfv_ctx* c, int s) {
   c->v_zero[s] = 1;
   c->fr_dirty = true;
   c->heng[s] = fresh_engine(c->params[s]);
+  c->heng[s].nf_mesh = (int)c->fused[s];
   HIP_OK(hipMemcpyAsync(c->deng + s, &c->heng[s], sizeof(SimEngine), hipMemcpyHostToDevice, st));
   HIP_OK(hipStreamSynchronize(st));
   return HFV_OK;
@@ -935,6 +997,30 @@ void launch_grad(hfv_ctx* c, const Batch& B, int do_grad, int do_ts) {
 }
 
 // The kernels of one batch step (k_begin .. k_end).  Timing events only outside graph capture.
+// Space::volume_calculation on the device for the moving meshes being stepped (1D and 2D).
+void enqueue_rebuild(hfv_ctx* c, const Batch& B) {
+  hipStream_t st = c->stream;
+  const dim3 gc((unsigned)c->ncg), gf((unsigned)c->nfg), gb(BLOCK);
+  if (c->any_moving1d) {
+    hipLaunchKernelGGL(k_mesh1d_faces, gf, gb, 0, st, B);
+    hipLaunchKernelGGL(k_mesh1d_cells, gc, gb, 0, st, B);
+  }
+  if (c->any_moving2d) {
+    const V2Bufs& V = c->v2;
+    const dim3 gs((unsigned)c->nslots), gsb(HFV_SCAN_BLOCK);
+    hipLaunchKernelGGL(k_v2_clear, gc, gb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_bin, gc, gb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_bins, gs, gsb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_place, gc, gb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_cells, gc, gb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_faces, gs, gsb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_faces, gf, gb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_csr, gs, gsb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_csr_fill, gf, gb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_csr_sort, gc, gb, 0, st, B, V);
+  }
+}
+
 void enqueue_step(hfv_ctx* c, bool timed) {
   Batch B = c->batch_args();
   hipStream_t st = c->stream;
@@ -951,10 +1037,9 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   if (t) prof_begin(c, KS_PREDICT);
   hipLaunchKernelGGL(k_mask, gc, gb, 0, st, B, 0);
   if (t) prof_end(c, KS_PREDICT);
-  if (c->any_moving1d) {
+  if (c->any_moving1d || c->any_moving2d) {
     if (t) prof_begin(c, KS_MESH1D);
-    hipLaunchKernelGGL(k_mesh1d_faces, gf, gb, 0, st, B);
-    hipLaunchKernelGGL(k_mesh1d_cells, gc, gb, 0, st, B);
+    enqueue_rebuild(c, B);
     if (t) prof_end(c, KS_MESH1D);
   }
   if (t) prof_begin(c, KS_FLUX);
@@ -1001,6 +1086,9 @@ int launch_step(hfv_ctx* c) {
   HIP_OK(hipGraphLaunch(c->gexec, c->stream));
   return HFV_OK;
 }
+
+// Faces of a slot's current mesh (after sync_counters for device-rebuilt meshes).
+int64_t mesh_faces(const hfv_ctx* c, int s) { return c->desc[s].moving2d ? c->heng[s].nf_mesh : c->fused[s]; }
 
 int check_slot(hfv_ctx* c, int s) {
   if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
@@ -1345,7 +1433,8 @@ int hfv_compute_fluxes(hfv_ctx* c, const hfv_timestep_info* info) {
 int hfv_get_fluxes(hfv_ctx* c, double* out7f) {
   TRY(require_mesh(c));
   if (!out7f) return set_err(HFV_ERR_INVALID, "out is NULL");
-  const int64_t NF = c->nf, nf = c->fused[0], fb = c->fbase[0], n = c->nused[0], cb = c->cbase[0];
+  TRY(sync_counters(c));
+  const int64_t NF = c->nf, nf = mesh_faces(c, 0), fb = c->fbase[0], n = c->nused[0], cb = c->cbase[0];
   std::vector<double> aos(8 * nf);
   std::vector<uint8_t> act(n);
   std::vector<int32_t> left(nf), right(nf);
@@ -1452,6 +1541,69 @@ int hfv_apply_timesteps(hfv_ctx* c, const double* dt, uint64_t* ti_next) {
   return HFV_OK;
 }
 
+// ---- the current mesh (Space's faces and connections, space.rs:76-77, and the cells' geometry)
+int hfv_get_mesh(hfv_ctx* c, hfv_mesh_arrays* out) {
+  TRY(require_mesh(c));
+  if (!out) return set_err(HFV_ERR_INVALID, "out is NULL");
+  TRY(sync_counters(c));
+  const int64_t n = c->nused[0], nf = mesh_faces(c, 0), cb = c->cbase[0], fb = c->fbase[0], NF = c->nf, N = c->n;
+  hipStream_t st = c->stream;
+  std::vector<int32_t> off(n + 1);
+  HIP_OK(hipMemcpyAsync(off.data(), c->F.off + cb, sizeof(int32_t) * (n + 1), hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  const int64_t nk = off[n] - off[0];
+  out->n_cells = n;
+  out->n_faces = nf;
+  out->n_conn = nk;
+  if (!out->area) return HFV_OK;  // sizes only
+  if (!out->normal || !out->centroid || !out->shift || !out->left || !out->right || !out->volume ||
+      !out->cell_centroid || !out->search_radius || !out->offset || !out->count || !out->conn)
+    return set_err(HFV_ERR_INVALID, "an output array is NULL");
+  std::vector<double> nrm(3 * nf), fcen(3 * nf), cen(3 * n);
+  std::vector<int32_t> left(nf), right(nf), conn(nk);
+  std::vector<uint8_t> shc(nf);
+  HIP_OK(hipMemcpyAsync((double*)out->area, c->F.area + fb, sizeof(double) * nf, hipMemcpyDeviceToHost, st));
+  for (int a = 0; a < 3; ++a) {
+    HIP_OK(hipMemcpyAsync(nrm.data() + a * nf, c->F.nrm + a * NF + fb, sizeof(double) * nf, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(fcen.data() + a * nf, c->F.fcen + a * NF + fb, sizeof(double) * nf, hipMemcpyDeviceToHost, st));
+    HIP_OK(hipMemcpyAsync(cen.data() + a * n, c->C.cen + a * N + cb, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  }
+  HIP_OK(hipMemcpyAsync(left.data(), c->F.left + fb, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(right.data(), c->F.right + fb, sizeof(int32_t) * nf, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync(shc.data(), c->F.shc + fb, nf, hipMemcpyDeviceToHost, st));
+  if (nk) HIP_OK(hipMemcpyAsync(conn.data(), c->F.conn + off[0], sizeof(int32_t) * nk, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync((double*)out->volume, c->C.vol + cb, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipMemcpyAsync((double*)out->search_radius, c->C.srad + cb, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+  HIP_OK(hipStreamSynchronize(st));
+  const hfv_params& p = c->params[0];
+  double* onrm = (double*)out->normal;
+  double* ocen = (double*)out->centroid;
+  double* osh = (double*)out->shift;
+  int64_t* ol = (int64_t*)out->left;
+  int64_t* orr = (int64_t*)out->right;
+  for (int64_t f = 0; f < nf; ++f) {
+    for (int a = 0; a < 3; ++a) {
+      onrm[3 * f + a] = nrm[a * nf + f];
+      ocen[3 * f + a] = fcen[a * nf + f];
+      const int code = (shc[f] >> (2 * a)) & 3;
+      osh[3 * f + a] = code == 1 ? p.box[a] : code == 2 ? -p.box[a] : 0.;
+    }
+    ol[f] = left[f] - cb;
+    orr[f] = right[f] < 0 ? -1 : right[f] - cb;
+  }
+  double* occ = (double*)out->cell_centroid;
+  int64_t* ooff = (int64_t*)out->offset;
+  int64_t* ocnt = (int64_t*)out->count;
+  int64_t* oconn = (int64_t*)out->conn;
+  for (int64_t i = 0; i < n; ++i) {
+    for (int a = 0; a < 3; ++a) occ[3 * i + a] = cen[a * n + i];
+    ooff[i] = off[i] - off[0];
+    ocnt[i] = off[i + 1] - off[i];
+  }
+  for (int64_t k = 0; k < nk; ++k) oconn[k] = (conn[k] >> 1) - fb;
+  return HFV_OK;
+}
+
 // ---- GravitySolver trait (src/gravity.rs:6-10)
 static int gravity_phase(hfv_ctx* c, double* out, bool accelerations) {
   TRY(require_mesh(c));
@@ -1496,13 +1648,13 @@ int hfv_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {
 }
 
 // ---- engine (src/engine.rs:152-268), device-resident
-// The device engine loops drift generators (k_drift) but rebuild meshes only in 1D: a 2D/3D simulation
-// with moving generators needs the caller's re-tessellation between drift and fluxes (hfv_step with
+// The device engine loops drift generators (k_drift) and rebuild 1D and 2D meshes: a 3D simulation with
+// moving generators needs the caller's re-tessellation between drift and fluxes (hfv_step with
 // hfv_set_mesh), so the loops refuse it rather than step it on a stale mesh.
 int check_engine_loop(const hfv_params& p) {
-  if (p.dim >= 2 && p.motion != HFV_FIXED)
+  if (p.dim == 3 && p.motion != HFV_FIXED)
     return set_err(HFV_ERR_UNSUPPORTED,
-                   "2D/3D moving meshes are rebuilt by the caller each step: use hfv_step with hfv_set_mesh");
+                   "3D moving meshes are rebuilt by the caller each step: use hfv_step with hfv_set_mesh");
   return HFV_OK;
 }
 

@@ -95,6 +95,13 @@ struct SimDesc {
   int static_grad;  // ... and gradient geometry too (k_geom_cells; chosen for 3D meshes)
   int staged;       // static_geom and some face chunk has a staging list: the flux records fr are kept
   int gstaged;      // static_geom and the gradient sweeps stage their chunks' neighbour lines
+  // 2D moving mesh rebuilt on the device (hfv_voronoi2d.hpp): the bin grid of the tessellator (nb bins of
+  // width w per axis), the slot's CSR base and capacities, and its count of CSR offsets
+  int moving2d;
+  int v2nb[2];
+  int noff;
+  double v2w[2];
+  int64_t conn_base, conn_cap, face_cap;
 };
 
 // Thread block -> work: simulation (-1: nothing), items [lo, hi), chunk (index of its partials).
@@ -130,6 +137,8 @@ struct SimEngine {
   int pred_step;
   uint64_t ti_pred;
   int n_down;  // 1D periodic moving meshes: faces of this step's rebuild where the generator order steps down
+  int nf_mesh;  // 2D moving meshes: faces of the current mesh (the device rebuild's count)
+  int nf_tmp;   // 2D moving meshes: the rebuild's provisional face cursor
   int pad_;
 };
 
@@ -592,6 +601,7 @@ __global__ void k_begin(Batch B) {
   E.stepping = 0;
   E.n_sub = 0;
   E.n_down = 0;
+  E.nf_tmp = 0;
   E.n_active = 0;
   E.n_face_active = 0;
   E.ti_min = ~0ull;
@@ -1298,9 +1308,14 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   if (f < J.hi) {
     l = B.F.left[f];
     r = B.F.right[f];
-    la = B.C.act[l];
-    ra = r >= 0 && B.C.act[r];
-    cur = la || ra;
+    if (l < 0) {  // past a device-rebuilt mesh's faces: no cells
+      l = 0;
+      r = -1;
+    } else {
+      la = B.C.act[l];
+      ra = r >= 0 && B.C.act[r];
+      cur = la || ra;
+    }
     B.F.cur[f] = cur ? 1 : 0;
   }
   if (STATIC) {

@@ -24,6 +24,7 @@ enum : int {
   E_BRENT = -6,
   E_UNSUPPORTED = -7,
   E_DRIFT = -8,
+  E_MESH = -22,  // the device tessellation failed (HFV_ERR_MESH)
 };
 enum { SOLVER_EXACT = 0, SOLVER_HLLC, SOLVER_PVRS, SOLVER_AIRS, SOLVER_TSRS, SOLVER_TRRS, SOLVER_LINADV };
 enum { BND_PERIODIC = 0, BND_REFLECTIVE, BND_OPEN, BND_VACUUM };

@@ -346,7 +346,10 @@ struct Grid {
   void build(int64_t n, const double* loc) {
     double vol = 1;
     for (int a = 0; a < dim; ++a) vol *= box[a];
-    double spacing = std::pow(vol / (double)std::max<int64_t>(n, 1), 1.0 / dim);
+    // mean generator spacing (sqrt / cbrt, not pow: the device rebuild evaluates the same correctly
+    // rounded or libm call, so both grids agree bin for bin)
+    const double per_gen = vol / (double)std::max<int64_t>(n, 1);
+    double spacing = dim == 1 ? per_gen : dim == 2 ? std::sqrt(per_gen) : std::cbrt(per_gen);
     for (int a = 0; a < 3; ++a) {
       if (a < dim) {
         nb[a] = std::max(1, (int)std::floor(box[a] / spacing));

@@ -251,6 +251,28 @@ class EulerEqnsFvs:
         _lib.check(self.lib.hfv_conservation(self.ptr, _dp(out)))
         return out
 
+    def get_mesh(self):
+        """The current mesh (for moving meshes the device's latest rebuild) as a dict of host arrays in
+        the tessellator's layout."""
+        m = _lib.MeshArrays()
+        _lib.check(self.lib.hfv_get_mesh(self.ptr, C.byref(m)))
+        n, nf, nk = m.n_cells, m.n_faces, m.n_conn
+        out = {"n_cells": n, "n_faces": nf, "n_conn": nk}
+        keep = {}
+        for k, size, dt in (("area", nf, np.float64), ("normal", 3 * nf, np.float64), ("centroid", 3 * nf, np.float64),
+                            ("shift", 3 * nf, np.float64), ("left", nf, np.int64), ("right", nf, np.int64),
+                            ("volume", n, np.float64), ("cell_centroid", 3 * n, np.float64),
+                            ("search_radius", n, np.float64), ("offset", n, np.int64), ("count", n, np.int64),
+                            ("conn", nk, np.int64)):
+            keep[k] = np.zeros(max(size, 1), dtype=dt)
+            setattr(m, k, keep[k].ctypes.data_as(_lib.DP if dt == np.float64 else _lib.I64P))
+        _lib.check(self.lib.hfv_get_mesh(self.ptr, C.byref(m)))
+        for k, a in keep.items():
+            size = {"normal": 3 * nf, "centroid": 3 * nf, "shift": 3 * nf, "cell_centroid": 3 * n}.get(
+                k, nf if k in ("area", "left", "right") else nk if k == "conn" else n)
+            out[k] = a[:size]
+        return out
+
     def profile(self, enable=True):
         _lib.check(self.lib.hfv_profile_enable(self.ptr, int(bool(enable))))
 

@@ -248,18 +248,26 @@ int hfv_gravity_timesteps(hfv_ctx* ctx, double* dt_out);
 int hfv_gravity_accelerations(hfv_ctx* ctx, double* a_out);
 
 /* ---------------------------------------------------------------- fused step and engine */
-/* OptimalOrderRunner::step / GodunovHydroRunner::step without the tessellation
- * (optimal_order.rs:15-64, godunov.rs:14-58).  For moving meshes in 1D the Voronoi rebuild runs on
- * the device; 2D/3D moving meshes are rebuilt by the caller through hfv_set_mesh. */
+/* OptimalOrderRunner::step / GodunovHydroRunner::step (optimal_order.rs:15-64, godunov.rs:14-58).  For
+ * moving meshes in 1D and 2D the step's Voronoi rebuild (Space::volume_calculation, space.rs:236-263)
+ * runs on the device from the current generator positions; 3D moving meshes are rebuilt by the caller
+ * through hfv_set_mesh. */
 int hfv_step(hfv_ctx* ctx, const hfv_timestep_info* info, uint64_t* ti_next);
 /* Engine::run (src/engine.rs:152-192): advance until max_steps Step sync points ran or t_end.  The
- * device loop drifts generators and rebuilds 1D meshes; a 2D/3D simulation with moving generators
- * returns HFV_ERR_UNSUPPORTED (drive it with hfv_step and hfv_set_mesh instead). */
+ * device loop drifts generators and rebuilds 1D and 2D meshes; a 3D simulation with moving generators
+ * returns HFV_ERR_UNSUPPORTED (drive it with hfv_step and hfv_set_mesh instead).  A device rebuild that
+ * fails (more faces or CSR entries than the context holds, a cell of more than 40 edges) stops the
+ * simulation with HFV_ERR_MESH. */
 int hfv_run(hfv_ctx* ctx, int64_t max_steps, int64_t* steps_done);
 int hfv_get_engine_info(hfv_ctx* ctx, hfv_engine_info* out);
 /* Space::status sums (src/space.rs:1248-1280): sum of conserved + pending fluxes per component,
  * reduced on the device in a fixed order. */
 int hfv_conservation(hfv_ctx* ctx, double* out5);
+/* The current mesh in hfv_set_mesh's layout: Space's faces and cell -> face connections
+ * (src/space.rs:76-77; Space::faces, space.rs:1232) and the cells' volume, centroid and search radius;
+ * for moving meshes the device's latest rebuild.  With out->area == NULL only n_cells, n_faces and n_conn are written; then
+ * call again with every array allocated at those sizes (they are written through the const pointers). */
+int hfv_get_mesh(hfv_ctx* ctx, hfv_mesh_arrays* out);
 
 /* ---------------------------------------------------------------- batched ensembles and serving */
 /* Many independent simulations in one context.  No counterpart in the reference, which runs one
@@ -275,8 +283,8 @@ int hfv_batch_create(int nslots, const int64_t* cell_cap, const int64_t* face_ca
 /* Load a simulation into a slot: Space::initialize + first_init (src/space.rs:88-211) from the mesh and
  * the IC arrays (loc, vel [3n] xyz interleaved; mass, u_int [n]); resets the slot's engine (Engine::new,
  * src/engine.rs:104-149).  Fails with HFV_ERR_INVALID if the mesh exceeds the slot's capacity, and with
- * HFV_ERR_UNSUPPORTED for a 2D/3D simulation with moving generators (batches run the device engine
- * loop, which rebuilds only 1D meshes). */
+ * HFV_ERR_UNSUPPORTED for a 3D simulation with moving generators (batches run the device engine
+ * loop, which rebuilds 1D and 2D meshes). */
 int hfv_batch_load(hfv_ctx* ctx, int slot, const hfv_params* params, const hfv_mesh_arrays* mesh,
                    const double* loc, const double* mass, const double* vel, const double* u_int);
 /* Mark a slot empty (its cells stop participating). */

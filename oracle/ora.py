@@ -132,6 +132,10 @@ def lib():
         L.ora_phase_compute_fluxes.restype = None
         L.ora_sim_conservation.argtypes = [C.c_void_p, DP]
         L.ora_sim_conservation.restype = None
+        L.ora_sim_set_mesher.argtypes = [C.c_void_p, MESHER, C.c_void_p]
+        L.ora_sim_set_mesher.restype = None
+        L.ora_sim_apply_mesh.argtypes = [C.c_void_p, C.POINTER(Mesh)]
+        L.ora_sim_apply_mesh.restype = None
         L.ora_gravity_accelerations.argtypes = [C.c_void_p, DP]
         L.ora_gravity_accelerations.restype = None
         L.ora_gravity_timesteps.argtypes = [C.c_void_p, DP]
@@ -261,6 +265,24 @@ def make_params(dim, boundary, box, gamma, cfl, solver, threshold=2.0, runner=OP
     return p
 
 
+def _mesh_struct(mesh):
+    """An ora_mesh over a mesh dict (the tessellator's layout) and the arrays it points into."""
+    mk = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in mesh.items()}
+    m = Mesh()
+    m.n_faces = int(mk["n_faces"])
+    for k in ("area", "normal", "centroid", "shift", "volume", "cell_centroid", "search_radius"):
+        mk[k] = np.ascontiguousarray(mk[k], dtype=np.float64)
+        setattr(m, k, _dp(mk[k]))
+    for k in ("left", "right", "conn", "offset", "count"):
+        mk[k] = np.ascontiguousarray(mk[k], dtype=np.int64)
+        setattr(m, k, _i64p(mk[k]))
+    m.n_conn = int(mk["conn"].size)
+    return m, mk
+
+
+MESHER = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double))
+
+
 class Sim:
     """The oracle's Space + Engine for one simulation."""
 
@@ -268,17 +290,7 @@ class Sim:
         self.lib = lib()
         n = len(mass)
         self.n = n
-        self._keep = {k: (np.ascontiguousarray(v) if isinstance(v, np.ndarray) else v) for k, v in mesh.items()}
-        mk = self._keep
-        m = Mesh()
-        m.n_faces = int(mk["n_faces"])
-        for k in ("area", "normal", "centroid", "shift", "volume", "cell_centroid", "search_radius"):
-            mk[k] = np.ascontiguousarray(mk[k], dtype=np.float64)
-            setattr(m, k, _dp(mk[k]))
-        for k in ("left", "right", "conn", "offset", "count"):
-            mk[k] = np.ascontiguousarray(mk[k], dtype=np.int64)
-            setattr(m, k, _i64p(mk[k]))
-        m.n_conn = int(mk["conn"].size)
+        m, self._keep = _mesh_struct(mesh)
         self.params = params
         self.ptr = self.lib.ora_sim_new(C.byref(params), n, _dp(_f64(loc, 3 * n)), _dp(_f64(mass, n)),
                                         _dp(_f64(vel, 3 * n)), _dp(_f64(u_int, n)), C.byref(m))
@@ -287,6 +299,24 @@ class Sim:
         if getattr(self, "ptr", None):
             self.lib.ora_sim_free(self.ptr)
             self.ptr = None
+
+    def set_mesher(self, fn):
+        """2D/3D moving meshes: fn(loc (n, 3) array) -> the mesh dict of those generators (the
+        tessellator standing in for meshless_voronoi), called by every step after the predict."""
+        import traceback
+
+        def cb(_user, sim, n, loc):
+            try:
+                pos = np.ctypeslib.as_array(loc, shape=(3 * n,)).reshape(n, 3).copy()
+                m, _keep = _mesh_struct(fn(pos))
+                self.lib.ora_sim_apply_mesh(sim, C.byref(m))
+                return 0
+            except Exception:  # noqa: BLE001 - reported, and the step stops with ORA_ERR_MESH
+                traceback.print_exc()
+                return 1
+
+        self._mesher = MESHER(cb)
+        self.lib.ora_sim_set_mesher(self.ptr, self._mesher, None)
 
     def advance(self, max_steps):
         r = self.lib.ora_sim_advance(self.ptr, max_steps)

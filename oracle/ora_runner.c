@@ -119,6 +119,9 @@ struct ora_sim {
   int error;
   ora_counters cnt;
   int64_t last_n_active, last_n_face_active;
+  ora_mesher mesher; /* 2D/3D moving meshes: the caller's tessellator */
+  void* mesher_user;
+  const uint8_t* mesh_mask;
 };
 
 enum { SP_DUMP = 0, SP_STEP = 3 };
@@ -847,6 +850,7 @@ void ora_gravity_timesteps(const ora_sim* s, double* out) {
 /* OptimalOrderRunner::step (optimal_order.rs:15-64) / GodunovHydroRunner::step (godunov.rs:14-58),
  * without the tessellation (the caller rebuilds the mesh between predict and fluxes). */
 static int mesh_update_1d(ora_sim* s, const uint8_t* mask);
+static int mesh_update_caller(ora_sim* s, const uint8_t* mask);
 
 int ora_sim_runner_step(ora_sim* s, uint64_t* ti_next_out) {
   int64_t n = s->n;
@@ -861,7 +865,7 @@ int ora_sim_runner_step(ora_sim* s, uint64_t* ti_next_out) {
   double dt = dt_from_dti(t, t->ti_current - t->ti_old);
   ora_phase_predict(s, dt);
   if (s->p.motion != ORA_FIXED) {
-    int e = mesh_update_1d(s, active);
+    int e = s->p.dim == 1 ? mesh_update_1d(s, active) : mesh_update_caller(s, active);
     if (e) set_error(s, e);
   }
   flux_info* fl = (flux_info*)malloc(sizeof(flux_info) * (size_t)(s->nf ? s->nf : 1));
@@ -1092,6 +1096,30 @@ static int mesh_update_1d(ora_sim* s, const uint8_t* mask) {
   free(conn); free(offset); free(count); free(vol); free(cc); free(sr);
   return e;
 }
+
+/* Space::volume_calculation (space.rs:236-263) for 2D/3D through the caller's tessellator. */
+static int mesh_update_caller(ora_sim* s, const uint8_t* mask) {
+  if (!s->mesher) return ORA_ERR_UNSUPPORTED;
+  int64_t n = s->n;
+  double* loc = (double*)malloc(sizeof(double) * 3 * (size_t)(n ? n : 1));
+  for (int64_t i = 0; i < n; ++i) {
+    loc[3 * i] = s->parts[i].loc.x;
+    loc[3 * i + 1] = s->parts[i].loc.y;
+    loc[3 * i + 2] = s->parts[i].loc.z;
+  }
+  s->mesh_mask = mask;
+  int e = s->mesher(s->mesher_user, s, n, loc);
+  s->mesh_mask = NULL;
+  free(loc);
+  return e ? ORA_ERR_MESH : ORA_OK;
+}
+
+void ora_sim_set_mesher(ora_sim* s, ora_mesher mesher, void* user) {
+  s->mesher = mesher;
+  s->mesher_user = user;
+}
+
+void ora_sim_apply_mesh(ora_sim* s, const ora_mesh* mesh) { set_mesh(s, mesh, s->mesh_mask); }
 
 /* ======================= sim / engine ======================= */
 ora_sim* ora_sim_new(const ora_params* p, int64_t n, const double* loc, const double* mass, const double* vel,

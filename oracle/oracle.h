@@ -64,6 +64,7 @@ enum {
   ORA_ERR_BRENT = -6,           /* exact.rs:132-134 */
   ORA_ERR_UNSUPPORTED = -7,
   ORA_ERR_DRIFT = -8,           /* part.rs:118, space.rs box_reflect assert */
+  ORA_ERR_MESH = -22,           /* the tessellation of a 2D/3D moving mesh failed */
 };
 
 /* Soft-event counters (the reference eprintln!s these). */
@@ -152,6 +153,14 @@ typedef struct {
 ora_sim* ora_sim_new(const ora_params* p, int64_t n, const double* loc, const double* mass, const double* vel,
                      const double* u_int, const ora_mesh* mesh);
 void ora_sim_free(ora_sim* s);
+/* 2D/3D moving meshes: Space::volume_calculation (space.rs:236-263) calls meshless_voronoi's
+ * Voronoi::build_partial, which the oracle does not restate.  The step calls `mesher` with the generator
+ * positions (loc [3N], xyz interleaved) after the predict; the mesher tessellates them and hands the mesh
+ * to ora_sim_apply_mesh, which replaces the faces and connections and updates the geometry of the step's
+ * active cells only.  A nonzero return stops the simulation with ORA_ERR_MESH. */
+typedef int (*ora_mesher)(void* user, ora_sim* s, int64_t n, const double* loc);
+void ora_sim_set_mesher(ora_sim* s, ora_mesher mesher, void* user);
+void ora_sim_apply_mesh(ora_sim* s, const ora_mesh* mesh);
 int ora_sim_set_threads(int n);
 
 /* Engine::run (src/engine.rs:152-192): process sync points until `max_steps` Step sync

@@ -16,7 +16,12 @@ def oracle_sim(prob):
                         eos=c.eos, isothermal_u=c.isothermal_u, velocity=c.linadv_velocity,
                         gravity=c.gravity, grav_softening=c.grav_softening, grav_position=c.grav_position,
                         grav_acceleration=c.grav_acceleration)
-    return ora.Sim(p, prob.loc.ravel(), prob.mass, prob.vel.ravel(), prob.u_int, prob.mesh.as_dict())
+    sim = ora.Sim(p, prob.loc.ravel(), prob.mass, prob.vel.ravel(), prob.u_int, prob.mesh.as_dict())
+    if c.dim >= 2 and c.motion != fvs.FIXED:
+        # moving 2D/3D meshes: every step re-tessellates the drifted generators with the host tessellator
+        # (the stand-in for meshless_voronoi that the device rebuild reproduces)
+        sim.set_mesher(lambda pos: VoronoiMesh(c.dim, c.box, c.boundary == fvs.PERIODIC, pos).as_dict())
+    return sim
 
 
 def rel_err(a, b, floor=1e-300):

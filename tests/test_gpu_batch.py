@@ -80,10 +80,10 @@ def test_run_chunking_does_not_change_results(precision):
             assert np.array_equal(ga[k], gb[k]), (p.name, k)
 
 
-def test_engine_loops_refuse_2d_moving_meshes():
-    """The device engine loops rebuild only 1D meshes: a 2D simulation with moving generators is
+def test_engine_loops_refuse_3d_moving_meshes():
+    """The device engine loops rebuild 1D and 2D meshes only: a 3D simulation with moving generators is
     refused (HFV_ERR_UNSUPPORTED) by hfv_run and by hfv_batch_load rather than stepped on a stale mesh."""
-    p = ics.isentropic_vortex_2d(side=12, t_end=0.1)
+    p = ics.kelvin_helmholtz_3d(side=6, t_end=0.1)
     moving = ics.Problem(p.name, _with_motion(p.cfg, fvs.FLUID), p.loc, p.mass, p.vel, p.u_int, p.mesh)
     d = ics.build_device_sim(moving)
     with pytest.raises(RuntimeError, match="moving meshes"):
@@ -92,6 +92,22 @@ def test_engine_loops_refuse_2d_moving_meshes():
         fvs.Batch.for_problems([moving])
     # fixed generators run
     assert fvs.Batch.for_problems([p]).run(2) == 2
+
+
+def test_batch_2d_moving_member_equals_alone():
+    """A 2D moving-mesh simulation in a batch (beside fixed 1D/2D ones and a moving 1D one) is rebuilt on
+    the device in its slot and ends bit for bit where it ends alone."""
+    v = ics.isentropic_vortex_2d(side=16, t_end=0.2)
+    moving = ics.Problem(v.name, _with_motion(v.cfg, fvs.FLUID), v.loc, v.mass, v.vel, v.u_int, v.mesh)
+    probs = [ics.sod_1d(n=150, t_end=0.05), moving, ics.sedov_2d(side=20, t_end=0.01), moving]
+    b = fvs.Batch.for_problems(probs, headroom=1.5)
+    b.run(1 << 40)
+    for s in (1, 3):
+        got = b.get_state(s)
+        want, info = _alone(moving, 1 << 40)
+        assert b.engine_info(s)["step_count"] == info["step_count"] and info["done"]
+        for k in want:
+            assert np.array_equal(got[k], want[k]), (s, k)
 
 
 def _with_motion(cfg, motion):
