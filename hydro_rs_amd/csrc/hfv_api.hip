@@ -292,7 +292,8 @@ int ensure_v2(hfv_ctx* c) {
   v.vst = dalloc<int32_t>(c, c->n);
   v.cfill = dalloc<int32_t>(c, c->n);
   v.vf = dalloc<V2Face>(c, c->nf);
-  if (!v.gcnt || !v.gfill || !v.femit || !v.ccnt || !v.gord || !v.gkey || !v.vst || !v.cfill || !v.vf)
+  v.part = dalloc<int32_t>(c, c->ncchunk);
+  if (!v.gcnt || !v.gfill || !v.femit || !v.ccnt || !v.gord || !v.gkey || !v.vst || !v.cfill || !v.vf || !v.part)
     return set_err(HFV_ERR_HIP, "out of device memory (2D rebuild buffers)");
   c->v2 = v;
   c->v2_alloc = true;
@@ -1010,12 +1011,19 @@ void enqueue_rebuild(hfv_ctx* c, const Batch& B) {
     const dim3 gs((unsigned)c->nslots), gsb(HFV_SCAN_BLOCK);
     hipLaunchKernelGGL(k_v2_clear, gc, gb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_bin, gc, gb, 0, st, B, V);
-    hipLaunchKernelGGL(k_v2_scan_bins, gs, gsb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_up<0>, gc, gb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_mid<0>, gs, gsb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_down<0>, gc, gb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_place, gc, gb, 0, st, B, V);
-    hipLaunchKernelGGL(k_v2_cells, gc, gb, 0, st, B, V);
-    hipLaunchKernelGGL(k_v2_scan_faces, gs, gsb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_cells<false>, dim3((unsigned)(c->ncg * HFV_V2_SUB)), gb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_cells<true>, dim3((unsigned)(c->ncg * HFV_V2_SUB)), gb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_up<1>, gc, gb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_mid<1>, gs, gsb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_down<1>, gc, gb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_faces, gf, gb, 0, st, B, V);
-    hipLaunchKernelGGL(k_v2_scan_csr, gs, gsb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_up<2>, gc, gb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_mid<2>, gs, gsb, 0, st, B, V);
+    hipLaunchKernelGGL(k_v2_scan_down<2>, gc, gb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_csr_fill, gf, gb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_csr_sort, gc, gb, 0, st, B, V);
   }

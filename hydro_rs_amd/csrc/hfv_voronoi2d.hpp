@@ -6,26 +6,25 @@
 // bit for bit: every generator's cell is the initial region (the box, or a square of K bin widths around
 // the generator on a periodic box) clipped by the bisectors of the generators of the (2K+1)^2 bins around
 // its own, visited in increasing (squared distance, index, image code) order until no unvisited one can
-// cut the cell (a quarter of its squared distance beyond every vertex), with K = 2 .. 6 until the cell
+// cut the cell (a quarter of its squared distance beyond every vertex), with K = 2, 3, .. (up to the whole grid) until the cell
 // lies within K bin widths (and, periodic, has no wall edge).  Faces are emitted by the lower-index cell
 // of each pair in the order of its polygon's edges; face indices follow the cells; each cell's CSR lists
 // its faces in ascending face index.
 //
 // Per step (simulations with moving2d, while stepping), after the drift and the predict:
-//   k_v2_clear, k_v2_bin, k_v2_scan_bins, k_v2_place   the bin grid (counting sort of the generators)
-//   k_v2_cells                                         every cell's polygon; the active cells' volume,
+//   k_v2_clear, k_v2_bin, scan<0>, k_v2_place          the bin grid (counting sort of the generators)
+//   k_v2_cells<0>, k_v2_cells<1>                       every cell's polygon; the active cells' volume,
 //                                                      centroid and search radius (Particle::
 //                                                      update_geometry, part.rs:151-158); the cell's
 //                                                      emitted faces in a provisional order
-//   k_v2_scan_faces, k_v2_faces                        face indices in cell order, the face arrays
-//   k_v2_scan_csr, k_v2_csr_fill, k_v2_csr_sort        offsets and CSR entries in ascending face order
+//   scan<1>, k_v2_faces                                face indices in cell order, the face arrays
+//   scan<2>, k_v2_csr_fill, k_v2_csr_sort              offsets and CSR entries in ascending face order
+// (scan<A>: k_v2_scan_up, k_v2_scan_mid, k_v2_scan_down)
 // Inactive cells keep their geometry and get new CSR ranges (the reference's note at space.rs:233-235).
 #pragma once
 
 namespace hfv {
 
-#define HFV_V2_MAXL 40      // edges of a cell polygon while it is clipped
-#define HFV_V2_MAXC 64      // CSR entries of one cell
 #define HFV_SCAN_BLOCK 1024
 
 // A face in the provisional order: length, centroid and unit normal relative to the emitting generator,
@@ -45,6 +44,7 @@ struct V2Bufs {
   int32_t* ccnt;   // [N + S + 1]: CSR entries per cell, then exclusive offsets
   int32_t* cfill;  // [N] CSR fill cursors (global entry index)
   V2Face* vf;      // [F] at face_base: faces in provisional order
+  int32_t* part;   // [cell chunks] chunk sums, then chunk offsets, of the prefix sums
 };
 
 __device__ __forceinline__ int v2_bin(double x, double w, int nb) {
@@ -64,77 +64,6 @@ __device__ __forceinline__ bool v2_less(double ad2, int32_t aj, int32_t ak, doub
 __device__ __forceinline__ int32_t v2_shift_code(int s0, int s1) {
   return (s0 == 1 ? 1 : s0 == -1 ? 2 : 0) | ((s1 == 1 ? 1 : s1 == -1 ? 2 : 0) << 2);
 }
-
-struct V2Poly {
-  double nx[HFV_V2_MAXL], ny[HFV_V2_MAXL], d[HFV_V2_MAXL];
-  int32_t nb[HFV_V2_MAXL], code[HFV_V2_MAXL];
-  double vx[HFV_V2_MAXL], vy[HFV_V2_MAXL];
-  int m;
-
-  // vertex k: the intersection of edges k and k+1 (0 where they are parallel)
-  __device__ void recompute() {
-    for (int k = 0; k < m; ++k) {
-      const int k1 = k + 1 == m ? 0 : k + 1;
-      const double det = nx[k] * ny[k1] - ny[k] * nx[k1];
-      if (det == 0.0) {
-        vx[k] = 0.;
-        vy[k] = 0.;
-      } else {
-        vx[k] = (d[k] * ny[k1] - d[k1] * ny[k]) / det;
-        vy[k] = (nx[k] * d[k1] - nx[k1] * d[k]) / det;
-      }
-    }
-  }
-  __device__ double rmax2() const {
-    double r = 0.;
-    for (int k = 0; k < m; ++k) {
-      const double q = vx[k] * vx[k] + vy[k] * vy[k];
-      r = r < q ? q : r;
-    }
-    return r;
-  }
-  __device__ void set(int k, double a, double b, double dd, int32_t n_, int32_t c_) {
-    nx[k] = a;
-    ny[k] = b;
-    d[k] = dd;
-    nb[k] = n_;
-    code[k] = c_;
-  }
-  // Clip by the half-plane n.y <= dd: 0 no cut, 1 cut, -1 too many edges.  The run of outside vertices
-  // start .. start+nout-1 is contiguous; edges start+1 .. start+nout-1 go, the new edge follows edge
-  // start, and the list restarts at the first kept edge after the removed run.
-  __device__ int clip(double a, double b, double dd, int32_t n_, int32_t c_, V2Poly& t) {
-    uint64_t out = 0;
-    int nout = 0;
-    for (int k = 0; k < m; ++k)
-      if ((a * vx[k] + b * vy[k]) > dd) {
-        out |= 1ull << k;
-        ++nout;
-      }
-    if (nout == 0 || nout == m) return 0;
-    if (m - nout + 2 > HFV_V2_MAXL) return -1;
-    int start = 0;
-    for (int k = 0; k < m; ++k) {
-      const int km = k == 0 ? m - 1 : k - 1;
-      if (((out >> k) & 1) && !((out >> km) & 1)) {
-        start = k;
-        break;
-      }
-    }
-    int k = (start + nout) % m;
-    int q = 0;
-    while (true) {
-      t.set(q++, nx[k], ny[k], d[k], nb[k], code[k]);
-      if (k == start) break;
-      k = k + 1 == m ? 0 : k + 1;
-    }
-    t.set(q++, a, b, dd, n_, c_);
-    for (int j = 0; j < q; ++j) set(j, t.nx[j], t.ny[j], t.d[j], t.nb[j], t.code[j]);
-    m = q;
-    recompute();
-    return 1;
-  }
-};
 
 // ---------------------------------------------------------------- bin grid
 __global__ void __launch_bounds__(HFV_BLOCK) k_v2_clear(Batch B, V2Bufs V) {
@@ -170,48 +99,107 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_bin(Batch B, V2Bufs V) {
   atomicAdd(&V.gcnt[D.cell_base + s + key], 1);
 }
 
-// Exclusive prefix sum of a[0, len) in place by one block (wave scans, then the waves' totals); returns
-// the total to every thread.
-__device__ int32_t block_scan_excl(int32_t* a, int64_t len) {
-  __shared__ int32_t wsum[HFV_SCAN_BLOCK / 64];
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
-  int32_t carry = 0;
-  for (int64_t base = 0; base < len; base += blockDim.x) {
-    const int64_t k = base + threadIdx.x;
-    const int32_t v = k < len ? a[k] : 0;
-    int32_t x = v;
-    for (int o = 1; o < 64; o <<= 1) {
-      const int32_t y = __shfl_up(x, o, 64);
-      if (lane >= o) x += y;
-    }
-    if (lane == 63) wsum[wid] = x;
-    __syncthreads();
-    if (wid == 0) {
-      int32_t w = lane < nw ? wsum[lane] : 0;
-      for (int o = 1; o < 64; o <<= 1) {
-        const int32_t y = __shfl_up(w, o, 64);
-        if (lane >= o) w += y;
-      }
-      if (lane < nw) wsum[lane] = w;
-    }
-    __syncthreads();
-    const int32_t before = wid > 0 ? wsum[wid - 1] : 0;
-    if (k < len) a[k] = carry + before + x - v;
-    carry += wsum[nw - 1];
-    __syncthreads();
-  }
-  return carry;
+// ---------------------------------------------------------------- prefix sums
+// Exclusive prefix sums over a simulation's region of one of three arrays, in three launches over the
+// cell block table (chunks of HFV_BLOCK elements): k_v2_scan_up sums each chunk, k_v2_scan_mid (one block
+// per simulation) scans the chunk sums and stores the total after the region, k_v2_scan_down scans each
+// chunk from its offset.  A: 0 bin counts (length nbins; the bin starts, copied to the placement
+// cursors), 1 emitted faces per cell (length n; the face offsets, the total is the mesh's face count),
+// 2 CSR entries per cell (length n; the CSR offsets of the cells).
+template <int A>
+__device__ __forceinline__ int32_t* v2_scan_region(const V2Bufs& V, const SimDesc& D, int s, int64_t& len) {
+  len = A == 0 ? (int64_t)D.v2nb[0] * D.v2nb[1] : D.n;
+  return (A == 0 ? V.gcnt : A == 1 ? V.femit : V.ccnt) + D.cell_base + s;
 }
 
-// one block per simulation
-__global__ void __launch_bounds__(HFV_SCAN_BLOCK) k_v2_scan_bins(Batch B, V2Bufs V) {
-  const int s = blockIdx.x;
+// exclusive scan of one value per thread over a block of HFV_BLOCK threads; *total receives the sum
+__device__ __forceinline__ int32_t block_excl_scan(int32_t v, int32_t* total) {
+  __shared__ int32_t ws[HFV_BLOCK / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  int32_t x = v;
+  for (int o = 1; o < 64; o <<= 1) {
+    const int32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) ws[wid] = x;
+  __syncthreads();
+  int32_t before = 0, tot = 0;
+  for (int w = 0; w < HFV_BLOCK / 64; ++w) {
+    if (w < wid) before += ws[w];
+    tot += ws[w];
+  }
+  __syncthreads();
+  *total = tot;
+  return before + x - v;
+}
+
+template <int A>
+__global__ void __launch_bounds__(HFV_BLOCK) k_v2_scan_up(Batch B, V2Bufs V) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const int s = J.sim;
   const SimDesc& D = B.sims[s];
   if (!D.moving2d || !B.eng[s].stepping) return;
-  const int64_t nbins = (int64_t)D.v2nb[0] * D.v2nb[1];
-  int32_t* g = V.gcnt + D.cell_base + s;
-  block_scan_excl(g, nbins + 1);
-  for (int64_t k = threadIdx.x; k < nbins; k += blockDim.x) V.gfill[D.cell_base + s + k] = g[k];
+  int64_t len;
+  const int32_t* a = v2_scan_region<A>(V, D, s, len);
+  const int64_t k = (int64_t)J.lo - D.cell_base + threadIdx.x;
+  int32_t tot;
+  block_excl_scan(k < len ? a[k] : 0, &tot);
+  if (threadIdx.x == 0) V.part[J.chunk] = tot;
+}
+
+template <int A>
+__global__ void __launch_bounds__(HFV_SCAN_BLOCK) k_v2_scan_mid(Batch B, V2Bufs V) {
+  const int s = blockIdx.x;
+  const SimDesc& D = B.sims[s];
+  SimEngine& E = B.eng[s];
+  if (!D.moving2d || !E.stepping) return;
+  int64_t len;
+  int32_t* a = v2_scan_region<A>(V, D, s, len);
+  __shared__ int32_t tot_s;
+  if (threadIdx.x == 0) {  // a few hundred chunk sums at most per simulation: one lane
+    int32_t run = 0;
+    for (int32_t c = D.cchunk0; c < D.cchunk1; ++c) {
+      const int32_t t = V.part[c];
+      V.part[c] = run;
+      run += t;
+    }
+    a[len] = run;
+    tot_s = run;
+    if (A == 1) E.nf_mesh = E.err ? 0 : run;  // a failed cell emitted nothing: the simulation stops
+    if (A == 2 && run > D.conn_cap) {
+      atomicCAS(&B.cnt->err, 0, E_MESH);
+      atomicCAS(&E.err, 0, E_MESH);
+    }
+  }
+  __syncthreads();
+  if (A == 2) {  // offsets past the last cell: the slot's end of the CSR
+    const int32_t end = (int32_t)D.conn_base + (E.err ? 0 : tot_s);
+    for (int64_t k = D.n + threadIdx.x; k < D.noff; k += blockDim.x) B.F.off[D.cell_base + k] = end;
+  }
+}
+
+template <int A>
+__global__ void __launch_bounds__(HFV_BLOCK) k_v2_scan_down(Batch B, V2Bufs V) {
+  const BlockJob J = B.cjobs[blockIdx.x];
+  if (J.sim < 0) return;
+  const int s = J.sim;
+  const SimDesc& D = B.sims[s];
+  const SimEngine& E = B.eng[s];
+  if (!D.moving2d || !E.stepping) return;
+  int64_t len;
+  int32_t* a = v2_scan_region<A>(V, D, s, len);
+  const int64_t k = (int64_t)J.lo - D.cell_base + threadIdx.x;
+  int32_t tot;
+  const int32_t x = V.part[J.chunk] + block_excl_scan(k < len ? a[k] : 0, &tot);
+  if (k >= len) return;
+  a[k] = x;
+  if (A == 0) V.gfill[D.cell_base + s + k] = x;
+  if (A == 2) {
+    const int32_t o = (int32_t)D.conn_base + (E.err ? 0 : x);
+    B.F.off[D.cell_base + k] = o;
+    V.cfill[D.cell_base + k] = o;
+  }
 }
 
 __global__ void __launch_bounds__(HFV_BLOCK) k_v2_place(Batch B, V2Bufs V) {
@@ -226,136 +214,35 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_place(Batch B, V2Bufs V) {
   V.gord[D.cell_base + p] = (int32_t)(gi - D.cell_base);
 }
 
-// ---------------------------------------------------------------- cells
-// The cell of generator i with candidate bins within K: 0 built, 1 retry with a larger K, -1 overflow.
-__device__ int v2_build_cell(const Batch& B, const SimDesc& D, const V2Bufs& V, int s, int32_t i, double xi, double yi,
-                             double zi, int K, V2Poly& P, V2Poly& T, double& r2) {
-  const int64_t N = B.C.n, cb = D.cell_base;
-  const double* L = B.C.loc;
-  const double bx = D.cfg.box[0], by = D.cfg.box[1], bz = D.cfg.box[2];
-  const int nb0 = D.v2nb[0], nb1 = D.v2nb[1];
-  const double w0 = D.v2w[0], w1 = D.v2w[1];
-  const bool per = D.periodic != 0;
-  const int bi0 = v2_bin(xi, w0, nb0), bi1 = v2_bin(yi, w1, nb1);
-  const int32_t* gst = V.gcnt + cb + s;
-  const int32_t* gord = V.gord + cb;
-  // initial region: the box, or K bin widths around the generator on a periodic box
-  double reach = 0.;
-  reach = reach < K * w0 ? K * w0 : reach;
-  reach = reach < K * w1 ? K * w1 : reach;
-  const double lox = per ? -reach : -xi, loy = per ? -reach : -yi;
-  const double hix = per ? reach : bx - xi, hiy = per ? reach : by - yi;
-  P.m = 4;  // counter-clockwise: bottom, right, top, left (walls -3, -2, -4, -1)
-  P.set(0, 0., -1., -loy, -3, 0);
-  P.set(1, 1., 0., hix, -2, 0);
-  P.set(2, 0., 1., hiy, -4, 0);
-  P.set(3, -1., 0., -lox, -1, 0);
-  P.recompute();
-  r2 = P.rmax2();
-  bool have_last = false;
-  double ld2 = 0.;
-  int32_t lj = 0, lk = 0;
-  while (true) {
-    bool have = false;
-    double bd2 = 0., brx = 0., bry = 0.;
-    int32_t bj = 0, bk = 0, bcode = 0;
-    for (int dx = -K; dx <= K; ++dx) {
-      int b0 = bi0 + dx, s0 = 0;
-      if (b0 < 0 || b0 >= nb0) {
-        if (!per) continue;
-        if (b0 < 0) {
-          b0 += nb0;
-          s0 = -1;
-        } else {
-          b0 -= nb0;
-          s0 = 1;
-        }
-        if (b0 < 0 || b0 >= nb0) continue;
-      }
-      for (int dy = -K; dy <= K; ++dy) {
-        int b1 = bi1 + dy, s1 = 0;
-        if (b1 < 0 || b1 >= nb1) {
-          if (!per) continue;
-          if (b1 < 0) {
-            b1 += nb1;
-            s1 = -1;
-          } else {
-            b1 -= nb1;
-            s1 = 1;
-          }
-          if (b1 < 0 || b1 >= nb1) continue;
-        }
-        const int key = b0 * nb1 + b1;
-        const int32_t kk = ((s0 & 0xFF) << 16) | ((s1 & 0xFF) << 8);
-        for (int32_t q = gst[key]; q < gst[key + 1]; ++q) {
-          const int32_t j = gord[q];
-          if (j == i && s0 == 0 && s1 == 0) continue;
-          const int64_t gj = cb + j;
-          const double xj = L[gj] + (double)s0 * bx, yj = L[N + gj] + (double)s1 * by, zj = L[2 * N + gj] + 0.0 * bz;
-          const double rx = xj - xi, ry = yj - yi, rz = zj - zi;
-          const double d2 = rx * rx + ry * ry + rz * rz;
-          if (have_last && !v2_less(ld2, lj, lk, d2, j, kk)) continue;
-          if (!have || v2_less(d2, j, kk, bd2, bj, bk)) {
-            have = true;
-            bd2 = d2;
-            bj = j;
-            bk = kk;
-            brx = rx;
-            bry = ry;
-            bcode = v2_shift_code(s0, s1);
-          }
-        }
-      }
-    }
-    if (!have || 0.25 * bd2 > r2) break;  // security radius: the bisector lies beyond every vertex
-    const int c = P.clip(brx, bry, 0.5 * bd2, bj, bcode, T);
-    if (c < 0) return -1;
-    if (c > 0) r2 = P.rmax2();
-    have_last = true;
-    ld2 = bd2;
-    lj = bj;
-    lk = bk;
-  }
-  // completeness: every unvisited generator lies farther than K bin widths
-  double minw = 1e300;
-  minw = K * w0 < minw ? K * w0 : minw;
-  minw = K * w1 < minw ? K * w1 : minw;
-  if (4.0 * r2 > minw * minw) return 1;
-  if (per) {  // a periodic cell must not touch the initial region
-    for (int k = 0; k < P.m; ++k) {
-      const int km = k == 0 ? P.m - 1 : k - 1;
-      const double ex = P.vx[k] - P.vx[km], ey = P.vy[k] - P.vy[km];
-      if (P.nb[k] < 0 && sqrt(ex * ex + ey * ey) > 0.) return 1;
-    }
-  }
-  return 0;
-}
+// ---------------------------------------------------------------- cells, a team of lanes per cell
+// HFV_V2_TEAM lanes of one wavefront build one cell together.  The polygon lives in LDS, each lane
+// testing, moving and intersecting every TEAM-th edge; the next candidate in (squared distance, index,
+// image code) order is a team minimum over the lanes' smallest keys above the last one, each lane taking
+// every TEAM-th bin of the (2K+1)^2 block.  k_v2_cells keeps each lane's candidates in registers (at most
+// HFV_V2_TC) and polygons of up to HFV_V2_TL edges; cells that outgrow either go to k_v2_cells_wide,
+// which rescans the bins for every candidate and holds polygons of up to HFV_V2_TLW edges.  Both make the
+// host tessellator's operations in its order, so the same bits.
+#define HFV_V2_TEAM 8
+#define HFV_V2_TL 24
+#define HFV_V2_TLW 48
+#define HFV_V2_TC 8
+#define HFV_V2_SUB HFV_V2_TEAM  // blocks per cell chunk
+#define HFV_V2_WIDE (-1)         // femit mark: the cell goes to k_v2_cells_wide
 
-__global__ void __launch_bounds__(HFV_BLOCK) k_v2_cells(Batch B, V2Bufs V) {
-  const BlockJob J = B.cjobs[blockIdx.x];
-  if (J.sim < 0) return;
-  const int s = J.sim;
-  const SimDesc& D = B.sims[s];
-  SimEngine& E = B.eng[s];
-  if (!D.moving2d || !E.stepping) return;
-  const int64_t gi = (int64_t)J.lo + threadIdx.x;
-  if (gi >= J.hi) return;
+template <int TL>
+struct V2TeamPoly {
+  double nx[TL], ny[TL], d[TL], vx[TL], vy[TL];
+  int32_t nb[TL], code[TL];
+};
+
+// The finished cell: area and centroid summed over the edges in order (the host tessellator's
+// Cell2::finish), the emitted faces (the lower-index cell of each pair, and walls) appended to vf in edge
+// order, and the active cells' geometry (Particle::update_geometry, part.rs:151-158).
+template <int TL>
+__device__ void v2_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, SimEngine& E, int s, int64_t gi,
+                          int32_t i, double xi, double yi, double zi, const V2TeamPoly<TL>& P, int m) {
   const int64_t N = B.C.n, cb = D.cell_base;
-  const int32_t i = (int32_t)(gi - cb);
-  const double xi = B.C.loc[gi], yi = B.C.loc[N + gi], zi = B.C.loc[2 * N + gi];
-  V2Poly P, T;
-  double r2 = 0.;
-  int r = 1;
-  for (int K = 2; K <= 6 && r == 1; ++K) r = v2_build_cell(B, D, V, s, i, xi, yi, zi, K, P, T, r2);
-  if (r != 0) {
-    V.femit[cb + s + i] = 0;
-    atomicCAS(&B.cnt->err, 0, E_MESH);
-    atomicCAS(&E.err, 0, E_MESH);
-    return;
-  }
-  // area, centroid and edges (the host tessellator's Cell2::finish)
-  const int m = P.m;
-  double area = 0., cx = 0., cy = 0.;
+  double area = 0., cx = 0., cy = 0., r2 = 0.;
   int cnt = 0;
   for (int k = 0; k < m; ++k) {
     const int km = k == 0 ? m - 1 : k - 1;
@@ -366,6 +253,8 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_cells(Batch B, V2Bufs V) {
     cy = cy + (cr / 6.0) * (ay + byy);
     const double len = sqrt((bxx - ax) * (bxx - ax) + (byy - ay) * (byy - ay));
     if (len > 0. && (P.nb[k] < 0 || P.nb[k] > i)) ++cnt;
+    const double q = bxx * bxx + byy * byy;
+    r2 = r2 < q ? q : r2;
   }
   const int32_t start = atomicAdd(&E.nf_tmp, cnt);
   if ((int64_t)start + cnt > D.face_cap) {
@@ -394,30 +283,336 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_cells(Batch B, V2Bufs V) {
     f.code = P.code[k];
     out[q++] = f;
   }
-  if (B.C.act[gi]) {  // Particle::update_geometry (part.rs:151-158)
-    const double rmax = sqrt(P.rmax2());
+  if (B.C.act[gi]) {
     B.C.vol[gi] = area;
     const double ia = area > 0. ? 1.0 / area : 0.;
     B.C.cen[gi] = xi + (area > 0. ? ia * cx : 0.);
     B.C.cen[N + gi] = yi + (area > 0. ? ia * cy : 0.);
     B.C.cen[2 * N + gi] = zi;
-    B.C.srad[gi] = 1.5 * 2.0 * rmax;
+    B.C.srad[gi] = 1.5 * 2.0 * sqrt(r2);
+  }
+}
+
+// LDS writes of a team visible to the team's next reads (a team is part of one wavefront)
+__device__ __forceinline__ void team_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+__device__ __forceinline__ uint32_t team_ballot(bool p, int tiw) {
+  return (uint32_t)((__ballot(p) >> (HFV_V2_TEAM * tiw)) & ((1ull << HFV_V2_TEAM) - 1));
+}
+
+// vertex k: the intersection of edges k and k+1 (0 where they are parallel)
+template <int TL>
+__device__ __forceinline__ void team_recompute(V2TeamPoly<TL>& P, int m, int lane) {
+#pragma unroll
+  for (int t = 0; t < TL; t += HFV_V2_TEAM) {
+    const int k = t + lane;
+    if (k < m) {
+      const int k1 = k + 1 == m ? 0 : k + 1;
+      const double nx0 = P.nx[k], ny0 = P.ny[k], d0 = P.d[k], nx1 = P.nx[k1], ny1 = P.ny[k1], d1 = P.d[k1];
+      const double det = nx0 * ny1 - ny0 * nx1;
+      if (det == 0.0) {
+        P.vx[k] = 0.;
+        P.vy[k] = 0.;
+      } else {
+        P.vx[k] = (d0 * ny1 - d1 * ny0) / det;
+        P.vy[k] = (nx0 * d1 - nx1 * d0) / det;
+      }
+    }
+  }
+  team_sync();
+}
+
+template <int TL>
+__device__ __forceinline__ double team_rmax2(const V2TeamPoly<TL>& P, int m, int lane) {
+  double r = 0.;
+#pragma unroll
+  for (int t = 0; t < TL; t += HFV_V2_TEAM) {
+    const int k = t + lane;
+    if (k < m) {
+      const double q = P.vx[k] * P.vx[k] + P.vy[k] * P.vy[k];
+      r = r < q ? q : r;
+    }
+  }
+#pragma unroll
+  for (int o = HFV_V2_TEAM / 2; o > 0; o >>= 1) {
+    const double q = __shfl_xor(r, o, HFV_V2_TEAM);
+    r = r < q ? q : r;
+  }
+  return r;
+}
+
+// Clip by the half-plane n.y <= dd: 0 no cut, 1 cut, -1 more than TL edges.  The run of outside vertices
+// start .. start+nout-1 is contiguous; edges start+1 .. start+nout-1 go, the new edge follows edge start,
+// and the list restarts at the first kept edge after the removed run.
+template <int TL>
+__device__ __forceinline__ int team_clip(V2TeamPoly<TL>& P, int& m, double a, double b, double dd, int32_t n_,
+                                         int32_t c_, int lane, int tiw) {
+  static_assert(TL <= 64 && TL % HFV_V2_TEAM == 0, "edge masks");
+  uint64_t om = 0;
+#pragma unroll
+  for (int t = 0; t < TL; t += HFV_V2_TEAM) {
+    const int k = t + lane;
+    const bool o = k < m && (a * P.vx[k] + b * P.vy[k]) > dd;
+    om |= (uint64_t)team_ballot(o, tiw) << t;
+  }
+  const int nout = __popcll(om);
+  if (nout == 0 || nout == m) return 0;
+  const int q = m - nout + 2;
+  if (q > TL) return -1;
+  const uint64_t full = m == 64 ? ~0ull : (1ull << m) - 1ull;
+  const uint64_t prev = ((om << 1) | (om >> (m - 1))) & full;  // bit k: vertex k-1 is outside
+  const uint64_t sm = om & ~prev;
+  const int start = sm ? __builtin_ctzll(sm) : 0;
+  const int keep = (start + nout) % m;
+  constexpr int R = TL / HFV_V2_TEAM;
+  double rx[R], ry[R], rd[R];
+  int32_t rn[R], rc[R];
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int p = t * HFV_V2_TEAM + lane;
+    if (p < q - 1) {
+      int o = keep + p;
+      o = o >= m ? o - m : o;
+      rx[t] = P.nx[o];
+      ry[t] = P.ny[o];
+      rd[t] = P.d[o];
+      rn[t] = P.nb[o];
+      rc[t] = P.code[o];
+    }
+  }
+  team_sync();
+#pragma unroll
+  for (int t = 0; t < R; ++t) {
+    const int p = t * HFV_V2_TEAM + lane;
+    if (p < q - 1) {
+      P.nx[p] = rx[t];
+      P.ny[p] = ry[t];
+      P.d[p] = rd[t];
+      P.nb[p] = rn[t];
+      P.code[p] = rc[t];
+    }
+  }
+  if (lane == 0) {
+    P.nx[q - 1] = a;
+    P.ny[q - 1] = b;
+    P.d[q - 1] = dd;
+    P.nb[q - 1] = n_;
+    P.code[q - 1] = c_;
+  }
+  team_sync();
+  m = q;
+  team_recompute(P, m, lane);
+  return 1;
+}
+
+// The candidate key and plane of one generator image (bins wrapped once on a periodic box).
+struct V2Cand {
+  double d2, rx, ry;
+  int32_t j, k, code;
+};
+
+// Visit this lane's candidates of the (2K+1)^2 block: f(cand) for each.
+template <class Fn>
+__device__ __forceinline__ void v2_lane_candidates(const Batch& B, const SimDesc& D, const V2Bufs& V, int s, int32_t i,
+                                                   double xi, double yi, double zi, int K, int lane, Fn&& f) {
+  const int64_t N = B.C.n, cb = D.cell_base;
+  const double* L = B.C.loc;
+  const double bx = D.cfg.box[0], by = D.cfg.box[1], bz = D.cfg.box[2];
+  const int nb0 = D.v2nb[0], nb1 = D.v2nb[1];
+  const bool per = D.periodic != 0;
+  const int bi0 = v2_bin(xi, D.v2w[0], nb0), bi1 = v2_bin(yi, D.v2w[1], nb1);
+  const int32_t* gst = V.gcnt + cb + s;
+  const int32_t* gord = V.gord + cb;
+  const int side = 2 * K + 1;
+  for (int bb = lane; bb < side * side; bb += HFV_V2_TEAM) {
+    int b0 = bi0 + bb / side - K, b1 = bi1 + bb % side - K, s0 = 0, s1 = 0;
+    if (b0 < 0 || b0 >= nb0) {
+      if (!per) continue;
+      if (b0 < 0) {
+        b0 += nb0;
+        s0 = -1;
+      } else {
+        b0 -= nb0;
+        s0 = 1;
+      }
+      if (b0 < 0 || b0 >= nb0) continue;
+    }
+    if (b1 < 0 || b1 >= nb1) {
+      if (!per) continue;
+      if (b1 < 0) {
+        b1 += nb1;
+        s1 = -1;
+      } else {
+        b1 -= nb1;
+        s1 = 1;
+      }
+      if (b1 < 0 || b1 >= nb1) continue;
+    }
+    const int key = b0 * nb1 + b1;
+    const int32_t kk = ((s0 & 0xFF) << 16) | ((s1 & 0xFF) << 8);  // the host's memcmp order of the int8 codes
+    const int32_t code = v2_shift_code(s0, s1);
+    for (int32_t q = gst[key]; q < gst[key + 1]; ++q) {
+      const int32_t j = gord[q];
+      if (j == i && s0 == 0 && s1 == 0) continue;
+      const int64_t gj = cb + j;
+      const double xj = L[gj] + (double)s0 * bx, yj = L[N + gj] + (double)s1 * by, zj = L[2 * N + gj] + 0.0 * bz;
+      V2Cand c;
+      c.rx = xj - xi;
+      c.ry = yj - yi;
+      const double rz = zj - zi;
+      c.d2 = c.rx * c.rx + c.ry * c.ry + rz * rz;
+      c.j = j;
+      c.k = kk;
+      c.code = code;
+      f(c);
+    }
+  }
+}
+
+// Team minimum of the lanes' best candidates (unique keys: every lane ends with the same one).
+__device__ __forceinline__ void team_min(bool& have, V2Cand& b) {
+#pragma unroll
+  for (int o = HFV_V2_TEAM / 2; o > 0; o >>= 1) {
+    const int oh = __shfl_xor((int)have, o, HFV_V2_TEAM);
+    V2Cand c;
+    c.d2 = __shfl_xor(b.d2, o, HFV_V2_TEAM);
+    c.rx = __shfl_xor(b.rx, o, HFV_V2_TEAM);
+    c.ry = __shfl_xor(b.ry, o, HFV_V2_TEAM);
+    c.j = __shfl_xor(b.j, o, HFV_V2_TEAM);
+    c.k = __shfl_xor(b.k, o, HFV_V2_TEAM);
+    c.code = __shfl_xor(b.code, o, HFV_V2_TEAM);
+    if (oh && (!have || v2_less(c.d2, c.j, c.k, b.d2, b.j, b.k))) {
+      have = true;
+      b = c;
+    }
+  }
+}
+
+// The cell of generator i with candidate bins within K: 0 built, 1 retry with a larger K, 2 too many
+// candidates or edges for this variant (LIST: candidates kept in registers; otherwise rescanned).
+template <bool LIST, int TL>
+__device__ int v2_team_build(const Batch& B, const SimDesc& D, const V2Bufs& V, int s, int32_t i, double xi,
+                             double yi, double zi, int K, V2TeamPoly<TL>& P, int& m, double& r2, int lane, int tiw) {
+  const double w0 = D.v2w[0], w1 = D.v2w[1];
+  const bool per = D.periodic != 0;
+  // initial region: the box, or K bin widths around the generator on a periodic box
+  double reach = 0.;
+  reach = reach < K * w0 ? K * w0 : reach;
+  reach = reach < K * w1 ? K * w1 : reach;
+  const double lox = per ? -reach : -xi, loy = per ? -reach : -yi;
+  const double hix = per ? reach : D.cfg.box[0] - xi, hiy = per ? reach : D.cfg.box[1] - yi;
+  if (lane < 4) {  // counter-clockwise: bottom, right, top, left (walls -3, -2, -4, -1)
+    P.nx[lane] = lane == 1 ? 1. : lane == 3 ? -1. : 0.;
+    P.ny[lane] = lane == 0 ? -1. : lane == 2 ? 1. : 0.;
+    P.d[lane] = lane == 0 ? -loy : lane == 1 ? hix : lane == 2 ? hiy : -lox;
+    P.nb[lane] = lane == 0 ? -3 : lane == 1 ? -2 : lane == 2 ? -4 : -1;
+    P.code[lane] = 0;
+  }
+  team_sync();
+  m = 4;
+  team_recompute(P, m, lane);
+  r2 = team_rmax2(P, m, lane);
+  V2Cand cl[LIST ? HFV_V2_TC : 1];
+  int cnt = 0;
+  if (LIST) {
+    v2_lane_candidates(B, D, V, s, i, xi, yi, zi, K, lane, [&](const V2Cand& c) {
+#pragma unroll
+      for (int q = 0; q < (LIST ? HFV_V2_TC : 1); ++q)
+        if (q == cnt) cl[q] = c;
+      ++cnt;
+    });
+    if (team_ballot(cnt > HFV_V2_TC, tiw)) return 2;
+  }
+  bool have_last = false;
+  V2Cand last{};
+  while (true) {
+    bool have = false;
+    V2Cand best{};
+    auto consider = [&](const V2Cand& c) {
+      if ((!have_last || v2_less(last.d2, last.j, last.k, c.d2, c.j, c.k)) &&
+          (!have || v2_less(c.d2, c.j, c.k, best.d2, best.j, best.k))) {
+        have = true;
+        best = c;
+      }
+    };
+    if (LIST) {
+#pragma unroll
+      for (int q = 0; q < (LIST ? HFV_V2_TC : 1); ++q)
+        if (q < cnt) consider(cl[q]);
+    } else {
+      v2_lane_candidates(B, D, V, s, i, xi, yi, zi, K, lane, consider);
+    }
+    team_min(have, best);
+    if (!have || 0.25 * best.d2 > r2) break;  // security radius: the bisector lies beyond every vertex
+    const int c = team_clip(P, m, best.rx, best.ry, 0.5 * best.d2, best.j, best.code, lane, tiw);
+    if (c < 0) return 2;
+    if (c > 0) r2 = team_rmax2(P, m, lane);
+    have_last = true;
+    last = best;
+  }
+  // completeness: every unvisited generator lies farther than K bin widths
+  double minw = 1e300;
+  minw = K * w0 < minw ? K * w0 : minw;
+  minw = K * w1 < minw ? K * w1 : minw;
+  if (4.0 * r2 > minw * minw) return 1;
+  if (per) {  // a periodic cell must not touch the initial region
+    bool wall = false;
+#pragma unroll
+    for (int t = 0; t < TL; t += HFV_V2_TEAM) {
+      const int k = t + lane;
+      if (k < m) {
+        const int km = k == 0 ? m - 1 : k - 1;
+        const double ex = P.vx[k] - P.vx[km], ey = P.vy[k] - P.vy[km];
+        wall = wall || (P.nb[k] < 0 && sqrt(ex * ex + ey * ey) > 0.);
+      }
+    }
+    if (team_ballot(wall, tiw)) return 1;
+  }
+  return 0;
+}
+
+// grid: HFV_V2_SUB blocks per cell chunk, HFV_BLOCK / HFV_V2_TEAM cells per block.  WIDE: only the cells
+// that the first pass marked.
+template <bool WIDE>
+__global__ void __launch_bounds__(HFV_BLOCK) k_v2_cells(Batch B, V2Bufs V) {
+  constexpr int TL = WIDE ? HFV_V2_TLW : HFV_V2_TL;
+  __shared__ V2TeamPoly<TL> polys[HFV_BLOCK / HFV_V2_TEAM];
+  const BlockJob J = B.cjobs[blockIdx.x / HFV_V2_SUB];
+  if (J.sim < 0) return;
+  const int s = J.sim;
+  const SimDesc& D = B.sims[s];
+  SimEngine& E = B.eng[s];
+  if (!D.moving2d || !E.stepping) return;
+  const int team = threadIdx.x / HFV_V2_TEAM, lane = threadIdx.x % HFV_V2_TEAM;
+  const int tiw = (threadIdx.x & 63) / HFV_V2_TEAM;
+  const int64_t gi = (int64_t)J.lo + (blockIdx.x % HFV_V2_SUB) * (HFV_BLOCK / HFV_V2_TEAM) + team;
+  if (gi >= J.hi) return;  // whole teams; no block barrier follows
+  const int64_t N = B.C.n, cb = D.cell_base;
+  const int32_t i = (int32_t)(gi - cb);
+  if (WIDE && V.femit[cb + s + i] != HFV_V2_WIDE) return;
+  const double xi = B.C.loc[gi], yi = B.C.loc[N + gi], zi = B.C.loc[2 * N + gi];
+  V2TeamPoly<TL>& P = polys[team];
+  int m = 0, r = 1;
+  double r2 = 0.;
+  const int kmax = max(6, max(D.v2nb[0], D.v2nb[1]));
+  for (int K = 2; K <= kmax && r == 1; ++K)
+    r = v2_team_build<!WIDE, TL>(B, D, V, s, i, xi, yi, zi, K, P, m, r2, lane, tiw);
+  if (lane != 0) return;
+  if (r == 2 && !WIDE) {
+    V.femit[cb + s + i] = HFV_V2_WIDE;
+  } else if (r != 0) {
+    V.femit[cb + s + i] = 0;
+    atomicCAS(&B.cnt->err, 0, E_MESH);
+    atomicCAS(&E.err, 0, E_MESH);
+  } else {
+    v2_finish(B, V, D, E, s, gi, i, xi, yi, zi, P, m);
   }
 }
 
 // ---------------------------------------------------------------- faces and CSR
-__global__ void __launch_bounds__(HFV_SCAN_BLOCK) k_v2_scan_faces(Batch B, V2Bufs V) {
-  const int s = blockIdx.x;
-  const SimDesc& D = B.sims[s];
-  SimEngine& E = B.eng[s];
-  if (!D.moving2d || !E.stepping) return;
-  const int32_t tot = block_scan_excl(V.femit + D.cell_base + s, D.n + 1);
-  if (threadIdx.x == 0) {
-    // a failed cell emitted nothing: the mesh is incomplete, the simulation stops with E_MESH
-    E.nf_mesh = E.err ? 0 : tot;
-  }
-}
-
 __global__ void __launch_bounds__(HFV_BLOCK) k_v2_faces(Batch B, V2Bufs V) {
   const BlockJob J = B.fjobs[blockIdx.x];
   if (J.sim < 0) return;
@@ -457,30 +652,6 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_faces(Batch B, V2Bufs V) {
   if (r.nb >= 0) atomicAdd(&V.ccnt[cb + s + r.nb], 1);
 }
 
-__global__ void __launch_bounds__(HFV_SCAN_BLOCK) k_v2_scan_csr(Batch B, V2Bufs V) {
-  const int s = blockIdx.x;
-  const SimDesc& D = B.sims[s];
-  SimEngine& E = B.eng[s];
-  if (!D.moving2d || !E.stepping) return;
-  const int64_t cb = D.cell_base, n = D.n;
-  int32_t* c = V.ccnt + cb + s;
-  const int32_t tot = block_scan_excl(c, n + 1);
-  if (tot > D.conn_cap) {
-    if (threadIdx.x == 0) {
-      atomicCAS(&B.cnt->err, 0, E_MESH);
-      atomicCAS(&E.err, 0, E_MESH);
-    }
-    for (int64_t k = threadIdx.x; k < D.noff; k += blockDim.x) B.F.off[cb + k] = (int32_t)D.conn_base;
-    return;
-  }
-  for (int64_t k = threadIdx.x; k < D.noff; k += blockDim.x) {
-    const int32_t o = (int32_t)D.conn_base + (k <= n ? c[k] : tot);
-    B.F.off[cb + k] = o;
-    if (k < n) V.cfill[cb + k] = o;
-  }
-  if (tot > D.conn_cap) E.nf_mesh = 0;
-}
-
 __global__ void __launch_bounds__(HFV_BLOCK) k_v2_csr_fill(Batch B, V2Bufs V) {
   const BlockJob J = B.fjobs[blockIdx.x];
   if (J.sim < 0) return;
@@ -511,31 +682,18 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_csr_sort(Batch B, V2Bufs V) {
   const int64_t gi = (int64_t)J.lo + threadIdx.x;
   if (gi >= J.hi) return;
   const int32_t e0 = B.F.off[gi], e1 = B.F.off[gi + 1];
-  if (e1 - e0 > HFV_V2_MAXC) {
-    atomicCAS(&B.cnt->err, 0, E_MESH);
-    atomicCAS(&E.err, 0, E_MESH);
-    return;
-  }
-  int32_t c[HFV_V2_MAXC], nb[HFV_V2_MAXC];
-  const int m = e1 - e0;
-  for (int k = 0; k < m; ++k) {
-    c[k] = B.F.conn[e0 + k];
-    nb[k] = B.F.nbr[e0 + k];
-  }
-  for (int k = 1; k < m; ++k) {  // insertion sort by face index (then side)
+  int32_t* c = B.F.conn;
+  int32_t* nb = B.F.nbr;
+  for (int32_t k = e0 + 1; k < e1; ++k) {  // insertion sort of the cell's entries by face index (then side)
     const int32_t x = c[k], y = nb[k];
-    int j = k - 1;
-    while (j >= 0 && c[j] > x) {
+    int32_t j = k - 1;
+    while (j >= e0 && c[j] > x) {
       c[j + 1] = c[j];
       nb[j + 1] = nb[j];
       --j;
     }
     c[j + 1] = x;
     nb[j + 1] = y;
-  }
-  for (int k = 0; k < m; ++k) {
-    B.F.conn[e0 + k] = c[k];
-    B.F.nbr[e0 + k] = nb[k];
   }
 }
 

@@ -583,6 +583,9 @@ extern "C" int hfv_voronoi_build(int dim, const double* box, int periodic, int64
   for (int a = 0; a < 3; ++a) g.box[a] = box[a];
   g.build(n, loc);
   std::vector<CellOut> cells(n);
+  // candidate blocks of K = 2, 3, .. bins around a generator, up to the whole grid (cells next to large
+  // voids need wide blocks)
+  const int kmax = std::max(6, std::max(g.nb[0], std::max(g.nb[1], g.nb[2])));
   std::atomic<int> failed(0);
 #ifdef _OPENMP
   int nt = n_threads > 0 ? n_threads : omp_get_max_threads();
@@ -597,7 +600,7 @@ extern "C" int hfv_voronoi_build(int dim, const double* box, int periodic, int64
 #endif
     for (int64_t i = 0; i < n; ++i) {
       int ok = 1;
-      for (int K = 2; K <= 6; ++K) {
+      for (int K = 2; K <= kmax; ++K) {
         int r = (dim == 3) ? build_cell(g, n, loc, i, K, c3, cands, &cells[i])
                            : build_cell(g, n, loc, i, K, c2, cands, &cells[i]);
         if (r == 0) {
