@@ -584,6 +584,64 @@ def bench_sedov(args):
     return out, prob
 
 
+def bench_moving2d(args):
+    """The 2D Sedov blast on a moving mesh (steered generators, the reference's default particle motion):
+    drift, device Voronoi rebuild and step per engine step.  The rebuild's device time against the host
+    tessellator's (OpenMP, every host thread) on the same generators; L1(rho) against the FP64 oracle
+    (which re-tessellates with the host tessellator) at t = --moving-l1-t."""
+    from dataclasses import replace
+
+    from hydro_rs_amd import fvs, ics
+    from hydro_rs_amd.mesh import VoronoiMesh
+
+    base = ics.sedov_2d(side=args.moving_side, t_end=0.05)
+    prob = replace(base, cfg=replace(base.cfg, motion=fvs.STEER))
+    log(f"[moving2d] {prob.n} cells, steered generators, {args.moving_warmup} + {args.moving_steps} steps")
+    out = {"workload": f"sedov2d_{args.moving_side}x{args.moving_side}_periodic_hllc_steer", "cells": prob.n,
+           "warmup": args.moving_warmup, "steps": args.moving_steps}
+    for k, prec in (("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)):
+        d, st = run_device(ics.with_precision(prob, prec), args.moving_warmup, args.moving_steps, profile=True)
+        kt = st["kernel_times"]
+        per = {name: round(v["ms"] / max(v["launches"], 1) * 1e3, 2) for name, v in kt.items() if v["launches"]}
+        out[k] = {"cell_updates_per_s": st["cell_updates"] / st["seconds"],
+                  "us_per_step": 1e6 * st["seconds"] / max(st["steps"], 1),
+                  "mean_active_cells": st["cell_updates"] / max(st["steps"], 1),
+                  "per_kernel_us": per}
+        if k == "fp64":
+            loc = d.get_state(["loc"])["loc"].reshape(3, prob.n).T.copy()
+            faces = d.get_mesh()["n_faces"]
+        d.close()
+        log(f"[moving2d] {k}: {out[k]}")
+    # the host tessellator on the same generator positions (the device rebuild's CPU counterpart)
+    t = time.perf_counter()
+    reps = 3
+    for _ in range(reps):
+        VoronoiMesh(2, prob.cfg.box, True, loc, n_threads=args.threads)
+    host_ms = 1e3 * (time.perf_counter() - t) / reps
+    rebuild_us = out["fp64"]["per_kernel_us"].get("mesh_rebuild")
+    out["rebuild"] = {"device_us": rebuild_us, "host_tessellator_ms": host_ms, "faces": int(faces),
+                      "host_threads": args.threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
+                      "speedup_vs_host": (1e3 * host_ms / rebuild_us) if rebuild_us else None,
+                      "device_time_from": "HIP events around the rebuild's kernels (eager launches)"}
+    if not args.no_oracle:
+        short = replace(prob, cfg=replace(prob.cfg, t_end=args.moving_l1_t))
+        log(f"[moving2d] FP64, FP32 flux and the FP64 oracle to t = {args.moving_l1_t} (L1 check)")
+        res = {}
+        for k, prec in (("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)):
+            dd = ics.build_device_sim(ics.with_precision(short, prec))
+            dd.run()
+            res[k] = dd.get_state(["prim"])["prim"][: prob.n]
+            dd.close()
+        t = time.perf_counter()
+        ref, oi = oracle_run(short)
+        out["l1_vs_oracle"] = {k: _l1(v, ref) for k, v in res.items()}
+        out["l1_vs_oracle"].update({"at": f"t_end {args.moving_l1_t}", "oracle_steps": oi["step_count"],
+                                    "oracle_seconds": time.perf_counter() - t,
+                                    "pass": _l1(res["fp32_flux"], ref) < 1e-5})
+    log(f"[moving2d] {out}")
+    return out
+
+
 def bench_kh3d(args):
     from hydro_rs_amd import fvs, ics
 
@@ -720,7 +778,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--window-start", type=int, default=260,
                     help="ensemble: untimed launches (tuning included) before the timed window, at least")
-    ap.add_argument("--workload", default="all", choices=["all", "ensemble", "sedov", "kh3d", "serve"])
+    ap.add_argument("--workload", default="all", choices=["all", "ensemble", "sedov", "moving2d", "kh3d", "serve"])
+    ap.add_argument("--moving-side", type=int, default=256)
+    ap.add_argument("--moving-steps", type=int, default=100)
+    ap.add_argument("--moving-warmup", type=int, default=20)
+    ap.add_argument("--moving-l1-t", type=float, default=0.002,
+                    help="moving 2D workload: end time of the L1 check against the oracle")
     ap.add_argument("--members", type=int, default=512)
     ap.add_argument("--plain-stride", type=int, default=16)
     ap.add_argument("--oracle-stride", type=int, default=64, help="ensemble members compared with the oracle")
@@ -827,6 +890,8 @@ def main():
         del probs
     if w in ("all", "sedov"):
         out["sedov2d"], _ = bench_sedov(args)
+    if w in ("all", "moving2d"):
+        out["moving2d"] = bench_moving2d(args)
     if w in ("all", "kh3d"):
         out["kh3d"] = bench_kh3d(args)
     if w in ("all", "serve"):
@@ -837,6 +902,11 @@ def main():
             out.update({"value": s["cell_updates_per_s"], "steps": s["steps"], "warmup": 0,
                         "ms_per_step": s["us_per_step"] / 1e3, "dtype": "f64",
                         "config": {"workload": out["sedov2d"]["workload"]}})
+        elif w == "moving2d":
+            s = out["moving2d"]["fp64"]
+            out.update({"value": s["cell_updates_per_s"], "steps": args.moving_steps, "warmup": args.moving_warmup,
+                        "ms_per_step": s["us_per_step"] / 1e3, "dtype": "f64",
+                        "config": {"workload": out["moving2d"]["workload"]}})
         elif w == "kh3d":
             key = "fp64" if "fp64" in out["kh3d"] else "fp32_flux"
             s = out["kh3d"][key]

@@ -81,7 +81,7 @@ class TuneClass(C.Structure):
                 ("static_grad", C.c_int), ("gstaged", C.c_int), ("pad_", C.c_int), ("ns_per_update", C.c_double * 8)]
 
 
-KERNEL_SLOTS = ["mask_predict", "mesh1d", "flux", "apply_c2p", "grad_timestep", "wakeup", "limit_kick", "drift"]
+KERNEL_SLOTS = ["mask_predict", "mesh_rebuild", "flux", "apply_c2p", "grad_timestep", "wakeup", "limit_kick", "drift"]
 
 
 # Every symbol include/hfv.h and include/hfv_host.h declare, with its ctypes signature.
@@ -131,6 +131,7 @@ HFV_SYMBOLS = {
 }
 HOST_SYMBOLS = {
     "hfv_voronoi_build": (C.c_int, [C.c_int, DP, C.c_int, I64, DP, C.c_int, C.POINTER(C.c_void_p)]),
+    "hfv_voronoi_build_partial": (C.c_int, [C.c_int, DP, C.c_int, I64, DP, U8P, C.c_int, C.POINTER(C.c_void_p)]),
     "hfv_mesh_view": (C.c_int, [C.c_void_p, C.POINTER(MeshArrays)]),
     "hfv_mesh_free": (None, [C.c_void_p]),
 }

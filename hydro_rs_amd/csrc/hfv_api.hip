@@ -131,7 +131,7 @@ SimEngine fresh_engine(const hfv_params& p) {
   return E;
 }
 
-enum { KS_PREDICT = 0, KS_MESH1D, KS_FLUX, KS_APPLY, KS_GRAD, KS_WAKEUP, KS_LIMIT, KS_DRIFT };
+enum { KS_PREDICT = 0, KS_MESH, KS_FLUX, KS_APPLY, KS_GRAD, KS_WAKEUP, KS_LIMIT, KS_DRIFT };
 
 // Device memory of hfv_riemann_batch_noalloc, reserved with the code object on every GPU the library
 // runs on (held until the process ends): the Riemann kernel's counters in the first 256 bytes, then the
@@ -1046,9 +1046,9 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   hipLaunchKernelGGL(k_mask, gc, gb, 0, st, B, 0);
   if (t) prof_end(c, KS_PREDICT);
   if (c->any_moving1d || c->any_moving2d) {
-    if (t) prof_begin(c, KS_MESH1D);
+    if (t) prof_begin(c, KS_MESH);
     enqueue_rebuild(c, B);
-    if (t) prof_end(c, KS_MESH1D);
+    if (t) prof_end(c, KS_MESH);
   }
   if (t) prof_begin(c, KS_FLUX);
   launch_flux(c, B);

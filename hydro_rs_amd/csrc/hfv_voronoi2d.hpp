@@ -1,19 +1,21 @@
 // 2D moving meshes rebuilt on the device: Space::volume_calculation (src/space.rs:236-263) for 2D, the
-// reference's call of meshless_voronoi's Voronoi::build_partial.  Included by hfv_api.hip.
+// reference's call of meshless_voronoi's Voronoi::build_partial with the step's active mask: only the
+// active cells are built (their faces, including those with inactive neighbours), and every cell gets
+// its CSR range over the faces that exist.  Included by hfv_api.hip.
 //
 // The tessellation is the host tessellator's (hydro_rs_amd/csrc/host/voronoi.cpp, which stands in for
 // the crate) evaluated with the same operations in the same order, so the device mesh equals the host's
 // bit for bit: every generator's cell is the initial region (the box, or a square of K bin widths around
 // the generator on a periodic box) clipped by the bisectors of the generators of the (2K+1)^2 bins around
 // its own, visited in increasing (squared distance, index, image code) order until no unvisited one can
-// cut the cell (a quarter of its squared distance beyond every vertex), with K = 2, 3, .. (up to the whole grid) until the cell
-// lies within K bin widths (and, periodic, has no wall edge).  Faces are emitted by the lower-index cell
-// of each pair in the order of its polygon's edges; face indices follow the cells; each cell's CSR lists
-// its faces in ascending face index.
+// cut the cell (a quarter of its squared distance beyond every vertex), with K = 2, 3, .. (up to the whole
+// grid) until the cell lies within K bin widths (and, periodic, has no wall edge).  Faces are emitted in
+// the order of the emitting cell's polygon edges (v2_emits); face indices follow the cells; each cell's
+// CSR lists its faces in ascending face index.
 //
 // Per step (simulations with moving2d, while stepping), after the drift and the predict:
 //   k_v2_clear, k_v2_bin, scan<0>, k_v2_place          the bin grid (counting sort of the generators)
-//   k_v2_cells<0>, k_v2_cells<1>                       every cell's polygon; the active cells' volume,
+//   k_v2_cells<0>, k_v2_cells<1>                       the active cells' polygons, volume,
 //                                                      centroid and search radius (Particle::
 //                                                      update_geometry, part.rs:151-158); the cell's
 //                                                      emitted faces in a provisional order
@@ -156,18 +158,35 @@ __global__ void __launch_bounds__(HFV_SCAN_BLOCK) k_v2_scan_mid(Batch B, V2Bufs 
   if (!D.moving2d || !E.stepping) return;
   int64_t len;
   int32_t* a = v2_scan_region<A>(V, D, s, len);
+  __shared__ int32_t wsum[HFV_SCAN_BLOCK / 64];
   __shared__ int32_t tot_s;
-  if (threadIdx.x == 0) {  // a few hundred chunk sums at most per simulation: one lane
-    int32_t run = 0;
-    for (int32_t c = D.cchunk0; c < D.cchunk1; ++c) {
-      const int32_t t = V.part[c];
-      V.part[c] = run;
-      run += t;
+  // the chunk sums, HFV_SCAN_BLOCK at a time: wave scans, then the waves' totals
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  int32_t carry = 0;
+  for (int32_t c0 = D.cchunk0; c0 < D.cchunk1; c0 += blockDim.x) {
+    const int32_t c = c0 + (int32_t)threadIdx.x;
+    const int32_t v = c < D.cchunk1 ? V.part[c] : 0;
+    int32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      const int32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
     }
-    a[len] = run;
-    tot_s = run;
-    if (A == 1) E.nf_mesh = E.err ? 0 : run;  // a failed cell emitted nothing: the simulation stops
-    if (A == 2 && run > D.conn_cap) {
+    if (lane == 63) wsum[wid] = x;
+    __syncthreads();
+    int32_t before = 0, tot = 0;
+    for (int w = 0; w < nw; ++w) {
+      if (w < wid) before += wsum[w];
+      tot += wsum[w];
+    }
+    if (c < D.cchunk1) V.part[c] = carry + before + x - v;
+    carry += tot;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    a[len] = carry;
+    tot_s = carry;
+    if (A == 1) E.nf_mesh = E.err ? 0 : carry;  // a failed cell emitted nothing: the simulation stops
+    if (A == 2 && carry > D.conn_cap) {
       atomicCAS(&B.cnt->err, 0, E_MESH);
       atomicCAS(&E.err, 0, E_MESH);
     }
@@ -222,10 +241,20 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_place(Batch B, V2Bufs V) {
 // HFV_V2_TC) and polygons of up to HFV_V2_TL edges; cells that outgrow either go to k_v2_cells_wide,
 // which rescans the bins for every candidate and holds polygons of up to HFV_V2_TLW edges.  Both make the
 // host tessellator's operations in its order, so the same bits.
-#define HFV_V2_TEAM 8
-#define HFV_V2_TL 24
+// 16 lanes per cell: 205 us per step on Sedov 256^2 with steered generators, against 325 with 8 and 217
+// with 32 (the rebuild is latency-bound: few active cells per step, each team's chain of clips)
+#ifndef HFV_V2_TEAM
+#define HFV_V2_TEAM 16
+#endif
+#ifndef HFV_V2_TL
+#define HFV_V2_TL 32
+#endif
+#ifndef HFV_V2_TLW
 #define HFV_V2_TLW 48
+#endif
+#ifndef HFV_V2_TC
 #define HFV_V2_TC 8
+#endif
 #define HFV_V2_SUB HFV_V2_TEAM  // blocks per cell chunk
 #define HFV_V2_WIDE (-1)         // femit mark: the cell goes to k_v2_cells_wide
 
@@ -235,9 +264,15 @@ struct V2TeamPoly {
   int32_t nb[TL], code[TL];
 };
 
+// Voronoi::build_partial builds the active cells only: a face is emitted by the lower-index cell of a pair
+// of active cells, by the active cell of a pair with an inactive one, and a wall by its cell.
+__device__ __forceinline__ bool v2_emits(const Batch& B, int64_t cb, int32_t nb, int32_t i) {
+  return nb < 0 || nb > i || !B.C.act[cb + nb];
+}
+
 // The finished cell: area and centroid summed over the edges in order (the host tessellator's
-// Cell2::finish), the emitted faces (the lower-index cell of each pair, and walls) appended to vf in edge
-// order, and the active cells' geometry (Particle::update_geometry, part.rs:151-158).
+// Cell2::finish), the emitted faces appended to vf in edge order, and the cell's geometry
+// (Particle::update_geometry, part.rs:151-158).
 template <int TL>
 __device__ void v2_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, SimEngine& E, int s, int64_t gi,
                           int32_t i, double xi, double yi, double zi, const V2TeamPoly<TL>& P, int m) {
@@ -252,7 +287,7 @@ __device__ void v2_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, Sim
     cx = cx + (cr / 6.0) * (ax + bxx);
     cy = cy + (cr / 6.0) * (ay + byy);
     const double len = sqrt((bxx - ax) * (bxx - ax) + (byy - ay) * (byy - ay));
-    if (len > 0. && (P.nb[k] < 0 || P.nb[k] > i)) ++cnt;
+    if (len > 0. && v2_emits(B, cb, P.nb[k], i)) ++cnt;
     const double q = bxx * bxx + byy * byy;
     r2 = r2 < q ? q : r2;
   }
@@ -271,7 +306,7 @@ __device__ void v2_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, Sim
     const int km = k == 0 ? m - 1 : k - 1;
     const double ax = P.vx[km], ay = P.vy[km], bxx = P.vx[k], byy = P.vy[k];
     const double len = sqrt((bxx - ax) * (bxx - ax) + (byy - ay) * (byy - ay));
-    if (!(len > 0.) || !(P.nb[k] < 0 || P.nb[k] > i)) continue;
+    if (!(len > 0.) || !v2_emits(B, cb, P.nb[k], i)) continue;
     const double inv = 1.0 / sqrt(P.nx[k] * P.nx[k] + P.ny[k] * P.ny[k]);
     V2Face f;
     f.area = len;
@@ -592,6 +627,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_cells(Batch B, V2Bufs V) {
   if (gi >= J.hi) return;  // whole teams; no block barrier follows
   const int64_t N = B.C.n, cb = D.cell_base;
   const int32_t i = (int32_t)(gi - cb);
+  if (!B.C.act[gi]) return;  // unbuilt in a partial build (femit stays 0)
   if (WIDE && V.femit[cb + s + i] != HFV_V2_WIDE) return;
   const double xi = B.C.loc[gi], yi = B.C.loc[N + gi], zi = B.C.loc[2 * N + gi];
   V2TeamPoly<TL>& P = polys[team];

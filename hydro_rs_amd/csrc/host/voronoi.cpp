@@ -564,6 +564,11 @@ static int build_1d(hfv_mesh* m, const double* loc, double L, int periodic) {
 
 extern "C" int hfv_voronoi_build(int dim, const double* box, int periodic, int64_t n, const double* loc, int n_threads,
                                  hfv_mesh** out) {
+  return hfv_voronoi_build_partial(dim, box, periodic, n, loc, nullptr, n_threads, out);
+}
+
+extern "C" int hfv_voronoi_build_partial(int dim, const double* box, int periodic, int64_t n, const double* loc,
+                                         const uint8_t* mask, int n_threads, hfv_mesh** out) {
   if (!out || !loc || !box || n <= 0 || dim < 1 || dim > 3) return HFV_ERR_INVALID;
   hfv_mesh* m = new hfv_mesh();
   m->dim = dim;
@@ -599,6 +604,7 @@ extern "C" int hfv_voronoi_build(int dim, const double* box, int periodic, int64
 #pragma omp for schedule(dynamic, 256)
 #endif
     for (int64_t i = 0; i < n; ++i) {
+      if (mask && !mask[i]) continue;  // a partial build leaves unmasked cells unbuilt
       int ok = 1;
       for (int K = 2; K <= kmax; ++K) {
         int r = (dim == 3) ? build_cell(g, n, loc, i, K, c3, cands, &cells[i])
@@ -615,12 +621,14 @@ extern "C" int hfv_voronoi_build(int dim, const double* box, int periodic, int64
     delete m;
     return HFV_ERR_MESH;
   }
-  // faces: emitted by the lower-index cell of each pair (and walls by their cell)
+  // faces: emitted by the lower-index cell of each pair of built cells, by the built cell of a pair with
+  // an unbuilt one (partial builds), and walls by their cell
+  auto emits = [&](const FaceOut& f, int64_t i) { return f.nb < 0 || f.nb > i || (mask && !mask[f.nb]); };
   std::vector<int64_t> nfc(n + 1, 0);
   for (int64_t i = 0; i < n; ++i) {
     int64_t c = 0;
     for (const FaceOut& f : cells[i].faces)
-      if (f.nb < 0 || f.nb > i) c++;
+      if (emits(f, i)) c++;
     nfc[i + 1] = c;
   }
   for (int64_t i = 0; i < n; ++i) nfc[i + 1] += nfc[i];
@@ -648,7 +656,7 @@ extern "C" int hfv_voronoi_build(int dim, const double* box, int periodic, int64
     m->search_radius[i] = 1.5 * 2.0 * c.rmax;
     int64_t f = nfc[i];
     for (const FaceOut& fo : c.faces) {
-      if (!(fo.nb < 0 || fo.nb > i)) continue;
+      if (!emits(fo, i)) continue;
       m->area[f] = fo.area;
       m->normal[3 * f + 0] = fo.normal.x;
       m->normal[3 * f + 1] = fo.normal.y;

@@ -13,13 +13,21 @@ class VoronoiMesh:
     (-1 = wall), volume[N], cell_centroid[N,3], search_radius[N], offset[N], count[N], conn[*].
     """
 
-    def __init__(self, dim, box, periodic, loc, n_threads=0):
+    def __init__(self, dim, box, periodic, loc, n_threads=0, mask=None):
+        """mask (optional, [N] bool): Voronoi::build_partial, only the masked generators' cells are built."""
         loc = np.ascontiguousarray(loc, dtype=np.float64).reshape(-1, 3)
         box = np.ascontiguousarray(box, dtype=np.float64)
         h = _lib.host()
         ptr = C.c_void_p()
-        code = h.hfv_voronoi_build(int(dim), box.ctypes.data_as(_lib.DP), int(bool(periodic)), loc.shape[0],
-                                   loc.ctypes.data_as(_lib.DP), int(n_threads), C.byref(ptr))
+        if mask is None:
+            code = h.hfv_voronoi_build(int(dim), box.ctypes.data_as(_lib.DP), int(bool(periodic)), loc.shape[0],
+                                       loc.ctypes.data_as(_lib.DP), int(n_threads), C.byref(ptr))
+        else:
+            m8 = np.ascontiguousarray(mask, dtype=np.uint8)
+            assert m8.size == loc.shape[0]
+            code = h.hfv_voronoi_build_partial(int(dim), box.ctypes.data_as(_lib.DP), int(bool(periodic)),
+                                               loc.shape[0], loc.ctypes.data_as(_lib.DP),
+                                               m8.ctypes.data_as(_lib.U8P), int(n_threads), C.byref(ptr))
         if code != 0:
             raise _lib.HfvError(code, "tessellation failed")
         try:

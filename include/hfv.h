@@ -339,9 +339,9 @@ int hfv_get_tuning(hfv_ctx* ctx, int slot, int* staged, int* static_grad, int* g
 
 /* ---------------------------------------------------------------- measurement */
 /* Device time of the step's kernels, from HIP events recorded on the context's stream around each
- * launch (no counterpart in the reference, which has no device).  Slots: 0 mask+predict, 1 1D mesh
- * rebuild, 2 face flux, 3 accumulate+c2p, 4 gradients+timestep, 5 limiter wakeup, 6 limiter+kick1,
- * 7 drift. */
+ * launch (no counterpart in the reference, which has no device).  Slots: 0 mask+predict, 1 mesh
+ * rebuild (1D and 2D moving meshes), 2 face flux, 3 accumulate+c2p, 4 gradients+timestep, 5 limiter
+ * wakeup, 6 limiter+kick1, 7 drift. */
 #define HFV_NUM_KERNEL_SLOTS 8
 typedef struct {
   int64_t launches[HFV_NUM_KERNEL_SLOTS];

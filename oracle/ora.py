@@ -280,7 +280,7 @@ def _mesh_struct(mesh):
     return m, mk
 
 
-MESHER = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double))
+MESHER = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.c_int64, C.POINTER(C.c_double), C.POINTER(C.c_uint8))
 
 
 class Sim:
@@ -301,14 +301,16 @@ class Sim:
             self.ptr = None
 
     def set_mesher(self, fn):
-        """2D/3D moving meshes: fn(loc (n, 3) array) -> the mesh dict of those generators (the
-        tessellator standing in for meshless_voronoi), called by every step after the predict."""
+        """2D/3D moving meshes: fn(loc (n, 3) array, active mask (n,) bool) -> the mesh dict of those
+        generators with the active ones' cells built (the tessellator standing in for meshless_voronoi's
+        build_partial), called by every step after the predict."""
         import traceback
 
-        def cb(_user, sim, n, loc):
+        def cb(_user, sim, n, loc, mask):
             try:
                 pos = np.ctypeslib.as_array(loc, shape=(3 * n,)).reshape(n, 3).copy()
-                m, _keep = _mesh_struct(fn(pos))
+                act = np.ctypeslib.as_array(mask, shape=(n,)).astype(bool)
+                m, _keep = _mesh_struct(fn(pos, act))
                 self.lib.ora_sim_apply_mesh(sim, C.byref(m))
                 return 0
             except Exception:  # noqa: BLE001 - reported, and the step stops with ORA_ERR_MESH

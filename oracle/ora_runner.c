@@ -1108,7 +1108,7 @@ static int mesh_update_caller(ora_sim* s, const uint8_t* mask) {
     loc[3 * i + 2] = s->parts[i].loc.z;
   }
   s->mesh_mask = mask;
-  int e = s->mesher(s->mesher_user, s, n, loc);
+  int e = s->mesher(s->mesher_user, s, n, loc, mask);
   s->mesh_mask = NULL;
   free(loc);
   return e ? ORA_ERR_MESH : ORA_OK;

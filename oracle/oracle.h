@@ -155,10 +155,11 @@ ora_sim* ora_sim_new(const ora_params* p, int64_t n, const double* loc, const do
 void ora_sim_free(ora_sim* s);
 /* 2D/3D moving meshes: Space::volume_calculation (space.rs:236-263) calls meshless_voronoi's
  * Voronoi::build_partial, which the oracle does not restate.  The step calls `mesher` with the generator
- * positions (loc [3N], xyz interleaved) after the predict; the mesher tessellates them and hands the mesh
- * to ora_sim_apply_mesh, which replaces the faces and connections and updates the geometry of the step's
- * active cells only.  A nonzero return stops the simulation with ORA_ERR_MESH. */
-typedef int (*ora_mesher)(void* user, ora_sim* s, int64_t n, const double* loc);
+ * positions (loc [3N], xyz interleaved) and the active mask [N] after the predict; the mesher tessellates
+ * them (the cells of the active generators) and hands the mesh to ora_sim_apply_mesh, which replaces the
+ * faces and connections and updates the geometry of the step's active cells only.  A nonzero return stops
+ * the simulation with ORA_ERR_MESH. */
+typedef int (*ora_mesher)(void* user, ora_sim* s, int64_t n, const double* loc, const uint8_t* mask);
 void ora_sim_set_mesher(ora_sim* s, ora_mesher mesher, void* user);
 void ora_sim_apply_mesh(ora_sim* s, const ora_mesh* mesh);
 int ora_sim_set_threads(int n);

@@ -20,7 +20,8 @@ def oracle_sim(prob):
     if c.dim >= 2 and c.motion != fvs.FIXED:
         # moving 2D/3D meshes: every step re-tessellates the drifted generators with the host tessellator
         # (the stand-in for meshless_voronoi that the device rebuild reproduces)
-        sim.set_mesher(lambda pos: VoronoiMesh(c.dim, c.box, c.boundary == fvs.PERIODIC, pos).as_dict())
+        sim.set_mesher(lambda pos, act: VoronoiMesh(c.dim, c.box, c.boundary == fvs.PERIODIC, pos,
+                                                    mask=act).as_dict())
     return sim
 
 

@@ -78,3 +78,41 @@ def test_linear_field_divergence_theorem():
     np.add.at(s, m.left, m.area[:, None] * m.normal)
     np.add.at(s, m.right, -m.area[:, None] * m.normal)
     assert np.max(np.abs(s)) < 1e-12
+
+
+@pytest.mark.parametrize("periodic", [True, False])
+def test_partial_build_is_the_full_build_around_the_masked_cells(periodic):
+    """Voronoi::build_partial (space.rs:236-245): the masked cells' geometry equals the full build's, the
+    faces are exactly the full build's faces with a masked cell (each from a masked cell's polygon: within
+    1e-12 of the full build's, and bit for bit when both cells are masked), every cell's CSR lists exactly
+    its faces, and unmasked cells are unbuilt."""
+    rng = np.random.default_rng(11)
+    box = np.array([1.0, 1.0, 1.0])
+    loc = jittered_grid(2, 20, box, 0.3, rng)
+    n = loc.shape[0]
+    mask = rng.uniform(size=n) < 0.3
+    full = VoronoiMesh(2, box, periodic, loc)
+    part = VoronoiMesh(2, box, periodic, loc, mask=mask)
+    for k in ("volume", "search_radius"):
+        assert np.array_equal(getattr(part, k)[mask], getattr(full, k)[mask]), k
+        assert np.all(getattr(part, k)[~mask] == 0.0), k
+    assert np.array_equal(part.cell_centroid[mask], full.cell_centroid[mask])
+
+    def key(m, f):
+        a, b = int(m.left[f]), int(m.right[f])
+        return (min(a, b), max(a, b), tuple(np.round(m.shift[f] * (1 if a <= b else -1), 12))) if b >= 0 else (a, -1, f"w{m.normal[f].round(6).tolist()}")
+
+    fk = {key(full, f): f for f in range(full.n_faces)}
+    touched = {k for k in fk if mask[k[0]] or (k[1] >= 0 and mask[k[1]])}
+    pk = {key(part, f): f for f in range(part.n_faces)}
+    assert set(pk) == touched
+    for k, f in pk.items():
+        g = fk[k]
+        both = mask[k[0]] and k[1] >= 0 and mask[k[1]]
+        assert abs(part.area[f] - full.area[g]) <= 1e-12 * full.area[g]
+        if both:
+            assert part.area[f] == full.area[g] and np.array_equal(part.centroid[f], full.centroid[g])
+    for i in range(n):
+        mine = sorted(part.conn[part.offset[i]: part.offset[i] + part.count[i]])
+        want = sorted(f for f in range(part.n_faces) if part.left[f] == i or part.right[f] == i)
+        assert mine == want
