@@ -472,6 +472,9 @@ int alloc_batch(hfv_ctx* c) {
   c->P.act = dalloc<uint32_t>(c, ncc);
   c->P.face = dalloc<uint32_t>(c, nfc);
   c->P.ti = dalloc<uint64_t>(c, ncc);
+  c->P.cflag = dalloc<uint32_t>(c, ncc);
+  F.fcc = dalloc<int32_t>(c, nfc * HFV_FCC_MAX);
+  F.fccn = dalloc<uint8_t>(c, nfc);
   c->fall = dalloc<int2>(c, nf);
   C.fr = dalloc<double>(c, (int64_t)HFV_FR_W * n);
   C.dtq = dalloc<double>(c, n);
@@ -512,6 +515,8 @@ int alloc_batch(hfv_ctx* c) {
   HIP_OK(hipMemsetAsync(c->P.act, 0, sizeof(uint32_t) * (size_t)ncc, st));
   HIP_OK(hipMemsetAsync(c->P.face, 0, sizeof(uint32_t) * (size_t)nfc, st));
   HIP_OK(hipMemsetAsync(c->P.ti, 0xFF, sizeof(uint64_t) * (size_t)ncc, st));
+  HIP_OK(hipMemsetAsync(c->P.cflag, 0, sizeof(uint32_t) * (size_t)std::max<int64_t>(ncc, 1), st));
+  HIP_OK(hipMemsetAsync(F.fccn, 0, (size_t)std::max<int64_t>(nfc, 1), st));
   HIP_OK(hipMemsetAsync(F.rec, 0, sizeof(double) * 8 * (size_t)nf, st));
   HIP_OK(hipMemsetAsync(F.left, 0xFF, sizeof(int32_t) * (size_t)nf, st));
   HIP_OK(hipMemsetAsync(F.right, 0xFF, sizeof(int32_t) * (size_t)nf, st));
@@ -726,6 +731,34 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
   }
   if (nf) HIP_OK(hipMemcpyAsync(F.floc + 2 * fb, floc.data(), 2 * (size_t)nf, hipMemcpyHostToDevice, st));
   c->has_stage[s] = std::any_of(scnt.begin(), scnt.end(), [](uint8_t k) { return k > 0; }) ? 1 : 0;
+  // the cell chunks each face chunk touches (k_flux skips a chunk whose touched chunks hold no active
+  // cell); 2D moving meshes change their faces on the device: their chunks always run
+  {
+    const int64_t fcap_ch = chunks_of(c->fcap[s]);
+    std::vector<int32_t> fcc(std::max<int64_t>(fcap_ch, 1) * HFV_FCC_MAX, 0);
+    std::vector<uint8_t> fccn(std::max<int64_t>(fcap_ch, 1), 0);
+    const bool moving2d = p.dim == 2 && p.motion != HFV_FIXED;
+    std::vector<int32_t> u;
+    for (int64_t j = 0; j < nch && !moving2d; ++j) {
+      const int64_t f0 = j * BLOCK, f1 = std::min<int64_t>(f0 + BLOCK, nf);
+      u.clear();
+      for (int64_t f = f0; f < f1; ++f) {
+        u.push_back((int32_t)(c->cchunk0[s] + (left[f] - cb) / BLOCK));
+        if (right[f] >= 0) u.push_back((int32_t)(c->cchunk0[s] + (right[f] - cb) / BLOCK));
+      }
+      std::sort(u.begin(), u.end());
+      u.erase(std::unique(u.begin(), u.end()), u.end());
+      if (u.empty() || (int64_t)u.size() > HFV_FCC_MAX) continue;
+      fccn[j] = (uint8_t)u.size();
+      std::copy(u.begin(), u.end(), fcc.begin() + j * HFV_FCC_MAX);
+    }
+    if (fcap_ch) {
+      const int64_t k0 = c->fchunk0[s];
+      HIP_OK(hipMemcpyAsync(F.fcc + k0 * HFV_FCC_MAX, fcc.data(), sizeof(int32_t) * fcap_ch * HFV_FCC_MAX,
+                            hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(F.fccn + k0, fccn.data(), (size_t)fcap_ch, hipMemcpyHostToDevice, st));
+    }
+  }
   // LDS staging lists of the cell chunks (the distinct neighbour cells of each chunk's CSR entries)
   {
     const int64_t ncch = (n + BLOCK - 1) / BLOCK;
@@ -1060,7 +1093,7 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   launch_grad(c, B, 1, 1);
   if (t) prof_end(c, KS_GRAD);
   if (t) prof_begin(c, KS_WAKEUP);
-  hipLaunchKernelGGL(k_wakeup, gc, gb, 0, st, B);
+  hipLaunchKernelGGL(k_wakeup, gc, gb, 0, st, B, 1);
   if (t) prof_end(c, KS_WAKEUP);
   if (t) prof_begin(c, KS_LIMIT);
   hipLaunchKernelGGL(k_limit, gc, gb, 0, st, B, 0);
@@ -1501,7 +1534,7 @@ int hfv_timesteps(hfv_ctx* c, uint64_t* ti_next) {
   settle(c, -1);
   Batch B = c->batch_args();
   launch_grad(c, B, 0, 1);
-  hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B, 0);
   hipLaunchKernelGGL(k_limit, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B, 1);
   hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, c->stream, B);
   HIP_OK(hipGetLastError());
@@ -1538,7 +1571,7 @@ int hfv_apply_timesteps(hfv_ctx* c, const double* dt, uint64_t* ti_next) {
   HIP_OK(hipMemcpyAsync(c->C.dtq + cb, dt, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   Batch B = c->batch_args();
   hipLaunchKernelGGL(k_ts_apply, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B);
-  hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B);
+  hipLaunchKernelGGL(k_wakeup, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B, 0);
   hipLaunchKernelGGL(k_limit, dim3((unsigned)c->ncg), dim3(BLOCK), 0, c->stream, B, 1);
   hipLaunchKernelGGL(k_end, dim3((unsigned)c->nslots), dim3(HFV_END_BLOCK), 0, c->stream, B);
   HIP_OK(hipGetLastError());

@@ -69,6 +69,10 @@ struct Faces {
   // neighbour's list slot (nloc[2F], 0xFFFF for a wall)
   int32_t* gcell;
   uint16_t *gcnt, *nloc;
+  // per face chunk the cell chunks its faces touch (fcc[chunk][HFV_FCC_MAX]) and their count (fccn, 0:
+  // unknown, the chunk always runs): a step whose touched chunks hold no active cell has no face to compute
+  int32_t* fcc;
+  uint8_t* fccn;
 };
 
 struct Cfg {
@@ -110,6 +114,7 @@ struct BlockJob {
 };
 
 #define HFV_MAX_SUBDRIFTS 8
+#define HFV_FCC_MAX 8
 // Engine of one simulation (src/engine.rs:33-101, 152-268).
 struct SimEngine {
   uint64_t ti_old, ti_current, ti_next;
@@ -154,6 +159,7 @@ struct Partials {
   uint32_t* act;    // active cells of a cell chunk (k_mask)
   uint32_t* face;   // computed faces of a face chunk (k_flux)
   uint64_t* ti;     // minimum new dti of a cell chunk's active cells (k_grad)
+  uint32_t* cflag;  // a cell chunk holds a cell of a computed face this step (k_flux; cleared by k_mask)
 };
 
 struct Batch {
@@ -905,7 +911,10 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
   }
   if (mode != 1) {
     uint32_t cnt = block_sum_u32(act ? 1u : 0u);
-    if (threadIdx.x == 0) B.P.act[J.chunk] = cnt;
+    if (threadIdx.x == 0) {
+      B.P.act[J.chunk] = cnt;
+      B.P.cflag[J.chunk] = 0;
+    }
   }
   if (valid) flush_events(ev, B.cnt, B.eng, s);
 }
@@ -1285,6 +1294,17 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   if (!B.eng[s].stepping || B.sims[s].static_geom != (STATIC ? 1 : 0)) return;
   const Cfg& cfg = B.sims[s].cfg;
   const int64_t f = (int64_t)J.lo + threadIdx.x;
+  {  // no active cell in the chunks this chunk's faces touch: no face is computed (block-uniform)
+    const int nt = B.F.fccn[J.chunk];
+    if (nt > 0) {
+      const bool any = (int)threadIdx.x < nt && B.P.act[B.F.fcc[(uint64_t)J.chunk * HFV_FCC_MAX + threadIdx.x]] > 0;
+      if (!__syncthreads_or(any)) {
+        if (f < J.hi) B.F.cur[f] = 0;
+        if (threadIdx.x == 0) B.P.face[J.chunk] = 0;
+        return;
+      }
+    }
+  }
   constexpr bool DEFER = SOLVER == SOLVER_AIRS;
   // LDS: the staged records, reused for the block's output records (written out as whole lines)
   constexpr bool F32 = sizeof(T) == 4;
@@ -1317,6 +1337,11 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
       cur = la || ra;
     }
     B.F.cur[f] = cur ? 1 : 0;
+    if (cur) {  // the chunks of both cells have work in the accumulation and limiter sweeps
+      const SimDesc& D = B.sims[s];
+      B.P.cflag[D.cchunk0 + (l - D.cell_base) / HFV_BLOCK] = 1;
+      if (r >= 0) B.P.cflag[D.cchunk0 + (r - D.cell_base) / HFV_BLOCK] = 1;
+    }
   }
   if (STATIC) {
     if ((int)threadIdx.x < ns) scl[threadIdx.x] = myc;
@@ -1505,6 +1530,9 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_apply(Batch B, int mode) {
   if (J.sim < 0) return;
   const int s = J.sim;
   if (!B.eng[s].stepping) return;
+  // fused step on a fixed mesh: a chunk without a computed face and without an active cell has nothing to
+  // accumulate, apply or pack
+  if (mode == 7 && B.sims[s].cfg.motion == MOT_FIXED && !B.P.cflag[J.chunk] && !B.P.act[J.chunk]) return;
   const int64_t i = (int64_t)J.lo + threadIdx.x;
   if (i >= J.hi) return;
   const Cells& C = B.C;
@@ -2051,6 +2079,10 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_GRAD_WAVES) k_grad(Batch B, int
   const int s = J.sim;
   const SimEngine& E = B.eng[s];
   if (!E.stepping || B.sims[s].static_grad != (STATIC ? 1 : 0)) return;
+  if (do_grad == 1 && do_ts == 1 && !B.P.act[J.chunk]) {  // fused step, no active cell in the chunk
+    if (threadIdx.x == 0) B.P.ti[J.chunk] = ~0ull;
+    return;
+  }
   const Cfg& cfg = B.sims[s].cfg;
   const int64_t i = (int64_t)J.lo + threadIdx.x;
   const bool a = i < J.hi && B.C.act[i];
@@ -2270,11 +2302,18 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_geom_faces(Batch B, int s, int64_
 // ------------------------------------------------------------------ limiter
 // timestep_limiter first sweep (hydrodynamics.rs:165-179): wakeup from active neighbours' new bins;
 // timestep_sync (hydrodynamics.rs:186-191) reduces the minimum bin per simulation instead.
-__global__ void __launch_bounds__(HFV_BLOCK) k_wakeup(Batch B) {
+// A chunk with no computed face and no active cell has no active neighbour anywhere: its wakeups are the
+// cells' own bins and k_limit changes nothing there, so the fused step (fused = 1) skips it in both.
+__device__ __forceinline__ bool limiter_idle(const Batch& B, const BlockJob& J) {
+  return !B.sims[J.sim].cfg.sync_all && !B.P.cflag[J.chunk] && !B.P.act[J.chunk];
+}
+
+__global__ void __launch_bounds__(HFV_BLOCK) k_wakeup(Batch B, int fused) {
   const BlockJob J = B.cjobs[blockIdx.x];
   if (J.sim < 0) return;
   const int s = J.sim;
   if (!B.eng[s].stepping) return;
+  if (fused && limiter_idle(B, J)) return;
   const Cells& C = B.C;
   const Faces& F = B.F;
   const int64_t i = (int64_t)J.lo + threadIdx.x;
@@ -2325,6 +2364,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_limit(Batch B, int canon) {
   const int s = J.sim;
   const SimEngine& E = B.eng[s];
   if (!E.stepping) return;
+  if (!canon && limiter_idle(B, J)) return;
   const int64_t i = (int64_t)J.lo + threadIdx.x;
   if (i >= J.hi) return;
   const Cells& C = B.C;
