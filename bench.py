@@ -267,9 +267,12 @@ def load_report(path):
     return json.loads(lines[-1])
 
 
-def apply_tuning(sim, members, classes):
+def apply_tuning(sim, members, classes, shape=None):
     """Set each slot's tuning to the choice the autotuner made for its size class (dimension and cell
-    count in [lo, hi)); members: (slot, dim, cells)."""
+    count in [lo, hi)); members: (slot, dim, cells).  shape: the tuner's flux launch shape (faces per
+    thread), if it chose one."""
+    if shape:
+        sim.set_launch_shape(shape)
     for slot, dim, n in members:
         for c in classes or []:
             if c["dim"] == dim and c["cells"][0] <= n < c["cells"][1]:
@@ -298,12 +301,14 @@ def bench_ensemble(args):
     # autotuner (per size class: flux tiling, gradient geometry); its launches are ordinary steps
     tune_launches = 0
     tuning = None
+    shape = None
     # the timed window starts at launch max(window_start, tuning + W), whatever the tuner's length
     if args.tuning_from:
         # the earlier run's choices, and its tuner's launches as warmup: every candidate gives the same
         # bits, so the timed window holds the same steps as that run's
         rep = load_report(args.tuning_from)["autotune"]
-        apply_tuning(b, members, rep["classes"])
+        shape = (rep.get("launch_shape") or {}).get("flux_faces_per_thread")
+        apply_tuning(b, members, rep["classes"], shape)
         tune_launches = rep["launches"]
         tuning = rep["classes"]
         warmup, start = window_warmup(args.window_start, args.warmup, tune_launches, True)
@@ -312,6 +317,7 @@ def bench_ensemble(args):
         s0 = [b.engine_info(s)["step_count"] for s in range(len(probs))]
         t = time.perf_counter()
         tuning = b.autotune(launches=args.tune_launches, rounds=args.tune_rounds)
+        shape = b.launch_shape()["flux_faces_per_thread"]
         tune_launches = max(b.engine_info(s)["step_count"] - s0[s] for s in range(len(probs)))
         log(f"[ensemble] autotune: {tune_launches} launches in {time.perf_counter() - t:.1f}s: "
             f"{[(c['dim'], c['cells'], c['staged'], c['static_grad'], c['gstaged']) for c in tuning]}")
@@ -334,10 +340,12 @@ def bench_ensemble(args):
     # per-kernel device times over the same window: a replica context with the same tuning replays the
     # first `start` launches (bit-identical states), then the window's launches with HIP events on its
     # stream around every kernel
-    kt, rep_cu = kernel_times_replay(lambda: fvs.Batch.for_problems(probs), members, tuning, start, launches)
+    kt, rep_cu = kernel_times_replay(lambda: fvs.Batch.for_problems(probs), members, tuning, start, launches,
+                                     shape)
     assert rep_cu == cu, "the replica window must hold the timed window's steps"
     res = {"launches": launches, "seconds": secs, "cell_updates": cu, "face_fluxes": ff, "value": value,
-           "window_start": start, "autotune": {"launches": tune_launches, "classes": tuning},
+           "window_start": start,
+           "autotune": {"launches": tune_launches, "classes": tuning, "launch_shape": b.launch_shape()},
            "n_all": n_all, "members_done": sum(1 for i in i1 if i["done"]),
            "members_failed": [s for s, i in enumerate(i1) if i["error_code"]],
            "roofline": roofline(kt, launches, cu, ff, n_all, fp32=True)}
@@ -387,12 +395,12 @@ def bench_ensemble(args):
     return res, probs
 
 
-def kernel_times_replay(make, members, tuning, start, launches):
+def kernel_times_replay(make, members, tuning, start, launches, shape=None):
     """Per-kernel device times (HIP events on the context's stream) of launches [start, start + launches)
     of a fresh replica with the given tuning; returns (kernel_times, cell updates of that window)."""
     r = make()
-    if tuning:
-        apply_tuning(r, members, tuning)
+    if tuning or shape:
+        apply_tuning(r, members, tuning, shape)
     r.run(start)
     r.synchronize()
     u0 = sum(r.engine_info(s)["cell_updates"] for s, _, _ in members)
@@ -504,20 +512,21 @@ def run_device(prob, warmup, steps, profile=False, tune=0, tuning_from=None, tun
     steps first, then the window with HIP events around every kernel)."""
     from hydro_rs_amd import ics
 
-    tuning, tune_launches = None, 0
+    tuning, tune_launches, shape = None, 0, None
     if tuning_from:
-        tuning, tune_launches = tuning_from
+        tuning, tune_launches, shape = tuning_from
     elif tune > 0:
         probe = ics.build_device_sim(prob)
         probe.run(warmup)
         tuning = probe.autotune(launches=tune, rounds=tune_rounds)
+        shape = probe.launch_shape()["flux_faces_per_thread"]
         tune_launches = probe.engine_info()["step_count"] - warmup
         probe.close()
 
     def fresh():
         d = ics.build_device_sim(prob)
-        if tuning:
-            apply_tuning(d, [(0, prob.cfg.dim, prob.n)], tuning)
+        if tuning or shape:
+            apply_tuning(d, [(0, prob.cfg.dim, prob.n)], tuning, shape)
         d.run(warmup)
         d.synchronize()
         return d
@@ -542,6 +551,7 @@ def run_device(prob, warmup, steps, profile=False, tune=0, tuning_from=None, tun
         "steps": done, "seconds": t1 - t0, "cell_updates": e1["cell_updates"] - e0["cell_updates"],
         "face_fluxes": e1["face_fluxes"] - e0["face_fluxes"], "kernel_times": kt, "done": e1["done"],
         "ti_current": e1["ti_current"], "autotune": tuning, "tune_launches": tune_launches,
+        "launch_shape": shape,
     }
 
 
@@ -612,16 +622,21 @@ def bench_moving2d(args):
             faces = d.get_mesh()["n_faces"]
         d.close()
         log(f"[moving2d] {k}: {out[k]}")
-    # the host tessellator on the same generator positions (the device rebuild's CPU counterpart)
+    # a full rebuild (every cell active: timestep_sync) on the device, against the host tessellator's full
+    # build of the same generator positions (the device rebuild's CPU counterpart)
+    dfull, sfull = run_device(replace(prob, cfg=replace(prob.cfg, sync_all=True)), 2, 10, profile=True)
+    loc_full = dfull.get_state(["loc"])["loc"].reshape(3, prob.n).T.copy()
+    dfull.close()
+    full_us = sfull["kernel_times"]["mesh_rebuild"]["ms"] / max(sfull["kernel_times"]["mesh_rebuild"]["launches"], 1) * 1e3
     t = time.perf_counter()
     reps = 3
     for _ in range(reps):
-        VoronoiMesh(2, prob.cfg.box, True, loc, n_threads=args.threads)
+        VoronoiMesh(2, prob.cfg.box, True, loc_full, n_threads=args.threads)
     host_ms = 1e3 * (time.perf_counter() - t) / reps
-    rebuild_us = out["fp64"]["per_kernel_us"].get("mesh_rebuild")
-    out["rebuild"] = {"device_us": rebuild_us, "host_tessellator_ms": host_ms, "faces": int(faces),
+    out["rebuild"] = {"partial_device_us": out["fp64"]["per_kernel_us"].get("mesh_rebuild"),
+                      "partial_faces": int(faces), "full_device_us": full_us, "full_host_tessellator_ms": host_ms,
                       "host_threads": args.threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
-                      "speedup_vs_host": (1e3 * host_ms / rebuild_us) if rebuild_us else None,
+                      "full_speedup_vs_host": 1e3 * host_ms / full_us if full_us else None,
                       "device_time_from": "HIP events around the rebuild's kernels (eager launches)"}
     if not args.no_oracle:
         short = replace(prob, cfg=replace(prob.cfg, t_end=args.moving_l1_t))
@@ -656,7 +671,7 @@ def bench_kh3d(args):
     if args.kh_precision != "both":  # profiling runs: one precision, so that a kernel's last dispatches are its window
         precs = [kp for kp in precs if kp[0].startswith(args.kh_precision)]
     for k, prec in precs:
-        tf = (rep[k]["autotune"], rep[k]["tune_launches"]) if rep else None
+        tf = (rep[k]["autotune"], rep[k]["tune_launches"], rep[k].get("flux_faces_per_thread")) if rep else None
         fp32 = prec == fvs.FP32_FLUX
         d, s = run_device(ics.with_precision(prob, prec), args.kh_warmup, args.kh_steps, profile=True,
                           tune=args.kh_tune, tuning_from=tf, tune_rounds=args.kh_tune_rounds)
@@ -674,7 +689,7 @@ def bench_kh3d(args):
                   / (HBM_PEAK_GBS * 1e9),
                   "mean_active_cells": n_act, "mean_active_faces": f_act,
                   "roofline": rl, "ti_current": s["ti_current"], "autotune": s["autotune"],
-                  "tune_launches": s["tune_launches"]}
+                  "tune_launches": s["tune_launches"], "flux_faces_per_thread": s["launch_shape"]}
         devs[k] = d
         log(f"[kh3d] {k}: {out[k]['ms_per_step']:.2f} ms/step, {out[k]['cell_updates_per_s']:.3e} cell-updates/s, "
             f"step {out[k]['step_roofline_frac']:.3f} of HBM, dominant {rl['kernel']} {rl['achieved']} GB/s")

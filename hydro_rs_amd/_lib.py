@@ -126,6 +126,8 @@ HFV_SYMBOLS = {
     "hfv_autotune": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(TuneClass), C.c_int, C.POINTER(C.c_int)]),
     "hfv_set_tuning": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int]),
     "hfv_get_tuning": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "hfv_set_launch_shape": (C.c_int, [C.c_void_p, C.c_int]),
+    "hfv_get_launch_shape": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), DP]),
     "hfv_profile_enable": (C.c_int, [C.c_void_p, C.c_int]),
     "hfv_profile_read": (C.c_int, [C.c_void_p, C.POINTER(KernelTimes)]),
 }

@@ -218,6 +218,8 @@ struct hfv_ctx {
   hipGraphExec_t gexec{};
   bool graph_ok = false;
   bool use_graph = true;
+  int flux_fpt = 1;  // face chunks per flux block (hfv_set_launch_shape, hfv_autotune)
+  double tune_fpt_ns[3] = {-1., -1., -1.};  // the tuner's cost per cell update with 1, 2, 4
   bool fr_dirty = true;  // rates and flux records must be rebuilt (k_pack_fr) before the next predict/flux
   hfv_timestep_info last_info{};
   // kernel timing (hfv_profile_enable)
@@ -988,27 +990,33 @@ void prof_next(hfv_ctx* c) {
   prof_take(c, c->ring, true);
 }
 
-template <class T, bool STATIC>
-void launch_flux_ts(hfv_ctx* c, const Batch& B, unsigned g) {
+template <class T, bool STATIC, int FPT>
+void launch_flux_fpt(hfv_ctx* c, const Batch& B, unsigned g) {
   switch (c->solver) {
     case HFV_HLLC:
-      hipLaunchKernelGGL((k_flux<T, SOLVER_HLLC, STATIC>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, SOLVER_HLLC, STATIC, FPT>), dim3(g), dim3(BLOCK), 0, c->stream, B);
       break;
     case HFV_AIRS:
-      hipLaunchKernelGGL((k_flux<T, SOLVER_AIRS, STATIC>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, SOLVER_AIRS, STATIC, FPT>), dim3(g), dim3(BLOCK), 0, c->stream, B);
       break;
     case HFV_EXACT:
-      hipLaunchKernelGGL((k_flux<T, SOLVER_EXACT, STATIC>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, SOLVER_EXACT, STATIC, FPT>), dim3(g), dim3(BLOCK), 0, c->stream, B);
       break;
     default:
-      hipLaunchKernelGGL((k_flux<T, -1, STATIC>), dim3(g), dim3(BLOCK), 0, c->stream, B);
+      hipLaunchKernelGGL((k_flux<T, -1, STATIC, FPT>), dim3(g), dim3(BLOCK), 0, c->stream, B);
   }
+}
+template <class T, bool STATIC>
+void launch_flux_ts(hfv_ctx* c, const Batch& B, unsigned g) {
+  if (c->flux_fpt == 4) launch_flux_fpt<T, STATIC, 4>(c, B, g);
+  else if (c->flux_fpt == 2) launch_flux_fpt<T, STATIC, 2>(c, B, g);
+  else launch_flux_fpt<T, STATIC, 1>(c, B, g);
 }
 
 // Flux kernels of the step: the static-geometry and the general variant each take the blocks of their
 // simulations; AIRS queues exact-solver faces for the fallback pass.
 void launch_flux(hfv_ctx* c, const Batch& B) {
-  const unsigned g = (unsigned)c->nfg;
+  const unsigned g = (unsigned)(c->nfg / c->flux_fpt);  // nfg is a multiple of 8
   const bool f32 = c->precision == HFV_FP32_FLUX;
   if (c->any_static) {
     if (f32) launch_flux_ts<float, true>(c, B, g);
@@ -1888,6 +1896,27 @@ int hfv_set_tuning(hfv_ctx* c, int slot, int staged, int static_grad, int gstage
   return HFV_OK;
 }
 
+int hfv_set_launch_shape(hfv_ctx* c, int flux_faces_per_thread) {
+  if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
+  const int f = flux_faces_per_thread;
+  if (f != 1 && f != 2 && f != 4) return set_err(HFV_ERR_INVALID, "faces per thread must be 1, 2 or 4");
+  HIP_OK(hipSetDevice(c->device));
+  if (f != c->flux_fpt) {
+    HIP_OK(hipStreamSynchronize(c->stream));
+    c->flux_fpt = f;
+    drop_graph(c);  // the captured step holds the flux grid
+  }
+  return HFV_OK;
+}
+
+int hfv_get_launch_shape(hfv_ctx* c, int* flux_faces_per_thread, double* ns_per_update3) {
+  if (!c || !flux_faces_per_thread) return set_err(HFV_ERR_INVALID, "NULL argument");
+  *flux_faces_per_thread = c->flux_fpt;
+  if (ns_per_update3)
+    for (int q = 0; q < 3; ++q) ns_per_update3[q] = c->tune_fpt_ns[q];
+  return HFV_OK;
+}
+
 int hfv_get_tuning(hfv_ctx* c, int slot, int* staged, int* static_grad, int* gstaged) {
   TRY(check_slot(c, slot));
   if (staged) *staged = c->loaded[slot] ? c->desc[slot].staged : 0;
@@ -2013,6 +2042,52 @@ int hfv_autotune(hfv_ctx* c, int launches, int rounds, hfv_tune_class* out, int 
     const int def = (k.can_stage && k.dim != 1 ? 2 : 0) | (k.dim == 3 ? 1 : 0) | (k.dim == 3 && k.can_gstage ? 4 : 0);
     if (k.best < 0 || (k.u[def] > 0. && k.t[def] / k.u[def] <= best * (1. + kTuneTie))) k.best = def;
     if ((rc = apply(k, k.best)) != HFV_OK) break;
+  }
+  // the flux kernel's launch shape (face chunks per block: faces per thread) for the whole context, with
+  // the classes' choices in place; 1 stays unless beaten by more than the tie margin
+  const int fpt0 = c->flux_fpt;
+  double ft[3] = {0., 0., 0.}, fu[3] = {0., 0., 0.};
+  for (int r = 0; r < rounds && !all_done && rc == HFV_OK && !cls.empty(); ++r)
+    for (int q = 0; q < 3; ++q) {
+      const int fpt = 1 << q;
+      if (fpt != c->flux_fpt) {
+        c->flux_fpt = fpt;
+        drop_graph(c);
+      }
+      if ((rc = launch_step(c)) != HFV_OK) break;  // untimed: captures the new grid
+      if ((rc = sync_counters(c)) != HFV_OK) break;
+      if ((all_done = finished())) break;
+      const long long u0 = updates(), want = (long long)launches * live_cells();
+      double dt = 0.;
+      for (int g = 0; g < 4; ++g) {
+        const auto t0 = std::chrono::steady_clock::now();
+        for (int j = 0; j < launches && rc == HFV_OK; ++j) rc = launch_step(c);
+        if (rc != HFV_OK) break;
+        const hipError_t se = hipStreamSynchronize(c->stream);
+        if (se != hipSuccess) {
+          rc = set_err(HFV_ERR_HIP, hipGetErrorString(se));
+          break;
+        }
+        dt += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if ((rc = sync_counters(c)) != HFV_OK) break;
+        if (updates() - u0 >= want || finished()) break;
+      }
+      if (rc != HFV_OK) break;
+      ft[q] += dt;
+      fu[q] += (double)(updates() - u0);
+      if ((all_done = finished())) break;
+    }
+  for (int q = 0; q < 3; ++q) c->tune_fpt_ns[q] = fu[q] > 0. ? 1e9 * ft[q] / fu[q] : -1.;
+  {
+    int bq = 0;
+    if (rc == HFV_OK && fu[0] > 0.)
+      for (int q = 1; q < 3; ++q)
+        if (fu[q] > 0. && ft[q] / fu[q] * (1. + kTuneTie) < ft[bq] / fu[bq]) bq = q;
+    const int keep = rc == HFV_OK && fu[0] > 0. ? 1 << bq : fpt0;
+    if (keep != c->flux_fpt) {
+      c->flux_fpt = keep;
+      drop_graph(c);
+    }
   }
   c->prof = prof;
   if (rc != HFV_OK) {

@@ -1287,23 +1287,25 @@ __device__ __forceinline__ void store_rec(double* rec, int64_t f, const double* 
 // block, instead of every face gathering its two cells field by field); the list, then every record
 // load of a thread, are issued together (two memory round trips).  Chunks without a list gather.
 template <class T, int SOLVER, bool STATIC>
-__global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
-  const BlockJob J = B.fjobs[blockIdx.x];
+__device__ __forceinline__ void flux_chunk(const Batch& B, const BlockJob J) {
   if (J.sim < 0) return;
   const int s = J.sim;
   if (!B.eng[s].stepping || B.sims[s].static_geom != (STATIC ? 1 : 0)) return;
   const Cfg& cfg = B.sims[s].cfg;
   const int64_t f = (int64_t)J.lo + threadIdx.x;
-  {  // no active cell in the chunks this chunk's faces touch: no face is computed (block-uniform)
-    const int nt = B.F.fccn[J.chunk];
-    if (nt > 0) {
-      const bool any = (int)threadIdx.x < nt && B.P.act[B.F.fcc[(uint64_t)J.chunk * HFV_FCC_MAX + threadIdx.x]] > 0;
-      if (!__syncthreads_or(any)) {
-        if (f < J.hi) B.F.cur[f] = 0;
-        if (threadIdx.x == 0) B.P.face[J.chunk] = 0;
-        return;
-      }
+  // no active cell in the chunks this chunk's faces touch: no face is computed (block-uniform); else the
+  // touched chunks are all marked as having work (a superset of the chunks of computed faces: one store
+  // per chunk instead of two per face)
+  const int nt = B.F.fccn[J.chunk];
+  if (nt > 0) {
+    const int32_t tc = (int)threadIdx.x < nt ? B.F.fcc[(uint64_t)J.chunk * HFV_FCC_MAX + threadIdx.x] : 0;
+    const bool any = (int)threadIdx.x < nt && B.P.act[tc] > 0;
+    if (!__syncthreads_or(any)) {
+      if (f < J.hi) B.F.cur[f] = 0;
+      if (threadIdx.x == 0) B.P.face[J.chunk] = 0;
+      return;
     }
+    if ((int)threadIdx.x < nt) B.P.cflag[tc] = 1;
   }
   constexpr bool DEFER = SOLVER == SOLVER_AIRS;
   // LDS: the staged records, reused for the block's output records (written out as whole lines)
@@ -1337,7 +1339,7 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
       cur = la || ra;
     }
     B.F.cur[f] = cur ? 1 : 0;
-    if (cur) {  // the chunks of both cells have work in the accumulation and limiter sweeps
+    if (cur && nt == 0) {  // the chunks of both cells have work in the accumulation and limiter sweeps
       const SimDesc& D = B.sims[s];
       B.P.cflag[D.cchunk0 + (l - D.cell_base) / HFV_BLOCK] = 1;
       if (r >= 0) B.P.cflag[D.cchunk0 + (r - D.cell_base) / HFV_BLOCK] = 1;
@@ -1469,6 +1471,18 @@ __global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
   if (threadIdx.x == 0) B.P.face[J.chunk] = cnt;
   ev.deferred = 0;
   flush_events(ev, B.cnt, B.eng, s);
+}
+
+// One block per fpt face chunks (fpt faces per thread: 1, 2 or 4, the autotuner's launch-shape choice):
+// block b takes the table's positions b, b + grid, .. (the chunks of a block keep the XCD their positions
+// were placed on whenever the grid is a multiple of 8).
+template <class T, int SOLVER, bool STATIC, int FPT>
+__global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
+#pragma unroll
+  for (int q = 0; q < FPT; ++q) {
+    flux_chunk<T, SOLVER, STATIC>(B, B.fjobs[blockIdx.x + q * gridDim.x]);
+    if (q + 1 < FPT) __syncthreads();  // the chunk's LDS is dead before the next chunk stages
+  }
 }
 
 // The queued faces of this step with the full AIRS solver (exact fallback), grid-stride.

@@ -308,6 +308,14 @@ class EulerEqnsFvs:
         _lib.check(self.lib.hfv_get_tuning(self.ptr, int(slot), C.byref(a), C.byref(b), C.byref(g)))
         return {"staged": a.value, "static_grad": b.value, "gstaged": g.value}
 
+    def set_launch_shape(self, flux_faces_per_thread=1):
+        """The flux kernel's face chunks per block (faces per thread: 1, 2 or 4) for the whole context."""
+        _lib.check(self.lib.hfv_set_launch_shape(self.ptr, int(flux_faces_per_thread)))
+
+    def launch_shape(self):
+        """{"flux_faces_per_thread": current choice, "ns_per_update": the tuner's cost of 1, 2, 4}."""
+        return _launch_shape(self.lib, self.ptr)
+
 
 def _engine_dict(e):
     d = {k: getattr(e, k) for k, _ in _lib.EngineInfo._fields_ if k != "ts"}
@@ -431,6 +439,21 @@ class Batch:
         a, b, g = C.c_int(0), C.c_int(0), C.c_int(0)
         _lib.check(self.lib.hfv_get_tuning(self.ptr, int(slot), C.byref(a), C.byref(b), C.byref(g)))
         return {"staged": a.value, "static_grad": b.value, "gstaged": g.value}
+
+    def set_launch_shape(self, flux_faces_per_thread=1):
+        """The flux kernel's face chunks per block (faces per thread: 1, 2 or 4) for the whole context."""
+        _lib.check(self.lib.hfv_set_launch_shape(self.ptr, int(flux_faces_per_thread)))
+
+    def launch_shape(self):
+        """{"flux_faces_per_thread": current choice, "ns_per_update": the tuner's cost of 1, 2, 4}."""
+        return _launch_shape(self.lib, self.ptr)
+
+
+def _launch_shape(lib, ptr):
+    f = C.c_int(0)
+    ns = (C.c_double * 3)()
+    _lib.check(lib.hfv_get_launch_shape(ptr, C.byref(f), ns))
+    return {"flux_faces_per_thread": f.value, "ns_per_update": [round(x, 4) for x in ns]}
 
 
 def _autotune(lib, ptr, launches, rounds):

@@ -336,6 +336,13 @@ int hfv_autotune(hfv_ctx* ctx, int launches, int rounds, hfv_tune_class* out, in
  * simulations only; hfv_get_tuning reports what a slot currently runs. */
 int hfv_set_tuning(hfv_ctx* ctx, int slot, int staged, int static_grad, int gstaged);
 int hfv_get_tuning(hfv_ctx* ctx, int slot, int* staged, int* static_grad, int* gstaged);
+/* The flux kernel's launch shape for the whole context: face chunks per thread block, i.e. faces per
+ * thread (1, 2 or 4; default 1): a block of 256 threads computes that many chunks of 256 faces in turn.
+ * hfv_autotune, after the classes, times the three shapes the same way and keeps the fastest unless 1 is
+ * within 1% of it; ns_per_update3 (nullable) receives its measured cost per cell update of each shape
+ * (-1 where not tried).  Bit-identical results; no counterpart in the reference. */
+int hfv_set_launch_shape(hfv_ctx* ctx, int flux_faces_per_thread);
+int hfv_get_launch_shape(hfv_ctx* ctx, int* flux_faces_per_thread, double* ns_per_update3);
 
 /* ---------------------------------------------------------------- measurement */
 /* Device time of the step's kernels, from HIP events recorded on the context's stream around each

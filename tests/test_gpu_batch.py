@@ -239,6 +239,39 @@ def test_tuning_choices_are_bit_identical(staged, static_grad, gstaged, precisio
             assert np.array_equal(ga[k], gb[k]), (p.name, k)
 
 
+@pytest.mark.parametrize("precision", [fvs.FP64, fvs.FP32_FLUX])
+@pytest.mark.parametrize("fpt", [2, 4])
+def test_flux_launch_shapes_are_bit_identical(fpt, precision):
+    """2 or 4 face chunks per flux block (AIRS with its exact fallback, staged and gathered tilings, a
+    single-simulation context and a batch) end a run on the same bits as one chunk per block."""
+    probs = [ics.sedov_2d(side=40, t_end=0.004, seed=1, precision=precision),
+             ics.riemann_tube_1d(300, 1.0, 10.0, 0.0, 0.125, 0.1, 0.0, 1.4, solver=fvs.AIRS, t_end=0.02,
+                                 precision=precision),
+             ics.kelvin_helmholtz_3d(side=12, t_end=0.03, precision=precision)]
+    a = fvs.Batch.for_problems(probs)
+    a.set_launch_shape(fpt)
+    assert a.launch_shape()["flux_faces_per_thread"] == fpt
+    a.run(1 << 40)
+    b = fvs.Batch.for_problems(probs)
+    b.run(1 << 40)
+    for s, p in enumerate(probs):
+        ea, eb = a.engine_info(s), b.engine_info(s)
+        assert ea["error_code"] == 0 and eb["error_code"] == 0 and ea["step_count"] == eb["step_count"], p.name
+        ga, gb = a.get_state(s), b.get_state(s)
+        for k in ga:
+            assert np.array_equal(ga[k], gb[k]), (p.name, k)
+    d1 = ics.build_device_sim(probs[2])
+    d2 = ics.build_device_sim(probs[2])
+    d2.set_launch_shape(fpt)
+    d1.run(1 << 40)
+    d2.run(1 << 40)
+    s1, s2 = d1.get_state(), d2.get_state()
+    for k in s1:
+        assert np.array_equal(s1[k], s2[k]), k
+    with pytest.raises(fvs.HfvError):
+        d2.set_launch_shape(3)
+
+
 def test_autotune_is_bit_identical():
     """hfv_autotune picks a tiling/geometry per size class by timing real launches; every candidate
     gives the same bits, so a tuned run ends exactly where an untuned one does."""
@@ -268,6 +301,14 @@ def test_autotune_is_bit_identical():
         chosen = r["static_grad"] + 2 * r["staged"] + 4 * r["gstaged"]
         expect = default if 0 <= ns[default] <= ns[best] * 1.01 else best
         assert chosen == expect, (r, default, best)
+    # the flux launch shape, timed after the classes: 1 face per thread unless another beats it by 1%
+    shape = a.launch_shape()
+    say(f"[autotune] launch shape {shape}")
+    ns = shape["ns_per_update"]
+    assert all(x > 0 for x in ns), shape
+    q = min(range(3), key=lambda k: ns[k])
+    want = 1 if ns[0] <= ns[q] * 1.01 else 1 << q
+    assert shape["flux_faces_per_thread"] == want, shape
     a.run(1 << 40)
     b = fvs.Batch.for_problems(probs)
     b.run(1 << 40)
