@@ -475,8 +475,8 @@ int alloc_batch(hfv_ctx* c) {
   c->P.face = dalloc<uint32_t>(c, nfc);
   c->P.ti = dalloc<uint64_t>(c, ncc);
   c->P.cflag = dalloc<uint32_t>(c, ncc);
-  F.fcc = dalloc<int32_t>(c, nfc * HFV_FCC_MAX);
-  F.fccn = dalloc<uint8_t>(c, nfc);
+  F.nch = dalloc<int32_t>(c, ncc * HFV_NCH_MAX);
+  F.nchn = dalloc<uint8_t>(c, ncc);
   c->fall = dalloc<int2>(c, nf);
   C.fr = dalloc<double>(c, (int64_t)HFV_FR_W * n);
   C.dtq = dalloc<double>(c, n);
@@ -518,7 +518,7 @@ int alloc_batch(hfv_ctx* c) {
   HIP_OK(hipMemsetAsync(c->P.face, 0, sizeof(uint32_t) * (size_t)nfc, st));
   HIP_OK(hipMemsetAsync(c->P.ti, 0xFF, sizeof(uint64_t) * (size_t)ncc, st));
   HIP_OK(hipMemsetAsync(c->P.cflag, 0, sizeof(uint32_t) * (size_t)std::max<int64_t>(ncc, 1), st));
-  HIP_OK(hipMemsetAsync(F.fccn, 0, (size_t)std::max<int64_t>(nfc, 1), st));
+  HIP_OK(hipMemsetAsync(F.nchn, 0xFF, (size_t)std::max<int64_t>(ncc, 1), st));
   HIP_OK(hipMemsetAsync(F.rec, 0, sizeof(double) * 8 * (size_t)nf, st));
   HIP_OK(hipMemsetAsync(F.left, 0xFF, sizeof(int32_t) * (size_t)nf, st));
   HIP_OK(hipMemsetAsync(F.right, 0xFF, sizeof(int32_t) * (size_t)nf, st));
@@ -733,34 +733,6 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
   }
   if (nf) HIP_OK(hipMemcpyAsync(F.floc + 2 * fb, floc.data(), 2 * (size_t)nf, hipMemcpyHostToDevice, st));
   c->has_stage[s] = std::any_of(scnt.begin(), scnt.end(), [](uint8_t k) { return k > 0; }) ? 1 : 0;
-  // the cell chunks each face chunk touches (k_flux skips a chunk whose touched chunks hold no active
-  // cell); 2D moving meshes change their faces on the device: their chunks always run
-  {
-    const int64_t fcap_ch = chunks_of(c->fcap[s]);
-    std::vector<int32_t> fcc(std::max<int64_t>(fcap_ch, 1) * HFV_FCC_MAX, 0);
-    std::vector<uint8_t> fccn(std::max<int64_t>(fcap_ch, 1), 0);
-    const bool moving2d = p.dim == 2 && p.motion != HFV_FIXED;
-    std::vector<int32_t> u;
-    for (int64_t j = 0; j < nch && !moving2d; ++j) {
-      const int64_t f0 = j * BLOCK, f1 = std::min<int64_t>(f0 + BLOCK, nf);
-      u.clear();
-      for (int64_t f = f0; f < f1; ++f) {
-        u.push_back((int32_t)(c->cchunk0[s] + (left[f] - cb) / BLOCK));
-        if (right[f] >= 0) u.push_back((int32_t)(c->cchunk0[s] + (right[f] - cb) / BLOCK));
-      }
-      std::sort(u.begin(), u.end());
-      u.erase(std::unique(u.begin(), u.end()), u.end());
-      if (u.empty() || (int64_t)u.size() > HFV_FCC_MAX) continue;
-      fccn[j] = (uint8_t)u.size();
-      std::copy(u.begin(), u.end(), fcc.begin() + j * HFV_FCC_MAX);
-    }
-    if (fcap_ch) {
-      const int64_t k0 = c->fchunk0[s];
-      HIP_OK(hipMemcpyAsync(F.fcc + k0 * HFV_FCC_MAX, fcc.data(), sizeof(int32_t) * fcap_ch * HFV_FCC_MAX,
-                            hipMemcpyHostToDevice, st));
-      HIP_OK(hipMemcpyAsync(F.fccn + k0, fccn.data(), (size_t)fcap_ch, hipMemcpyHostToDevice, st));
-    }
-  }
   // LDS staging lists of the cell chunks (the distinct neighbour cells of each chunk's CSR entries)
   {
     const int64_t ncch = (n + BLOCK - 1) / BLOCK;
@@ -789,6 +761,34 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
       HIP_OK(hipMemcpyAsync(F.gcnt + k0, gcnt.data(), sizeof(uint16_t) * ncch, hipMemcpyHostToDevice, st));
     }
     if (tot) HIP_OK(hipMemcpyAsync(F.nloc + kb, nloc.data(), sizeof(uint16_t) * tot, hipMemcpyHostToDevice, st));
+    // the other cell chunks each cell chunk's neighbours lie in (k_apply's idle test); meshes that change
+    // on the device (2D moving) keep 0xFF: their chunks always run
+    {
+      const int64_t cap_ch = chunks_of(c->ccap[s]);
+      std::vector<int32_t> nch(cap_ch * HFV_NCH_MAX, 0);
+      std::vector<uint8_t> nchn(cap_ch, 0xFF);
+      const bool moving2d = p.dim == 2 && p.motion != HFV_FIXED;
+      for (int64_t j = 0; j < ncch && !moving2d; ++j) {
+        const int64_t i0 = j * BLOCK, i1 = std::min<int64_t>(i0 + BLOCK, n);
+        const int64_t e0 = off[i0] - kb, e1 = (i1 < n ? off[i1] : kb + acc) - kb;
+        const int32_t own = (int32_t)(c->cchunk0[s] + j);
+        u.clear();
+        for (int64_t e = e0; e < e1; ++e)
+          if (nbr[e] >= 0) {
+            const int32_t ch = (int32_t)(c->cchunk0[s] + (nbr[e] - cb) / BLOCK);
+            if (ch != own) u.push_back(ch);
+          }
+        std::sort(u.begin(), u.end());
+        u.erase(std::unique(u.begin(), u.end()), u.end());
+        if ((int64_t)u.size() > HFV_NCH_MAX) continue;
+        nchn[j] = (uint8_t)u.size();
+        std::copy(u.begin(), u.end(), nch.begin() + j * HFV_NCH_MAX);
+      }
+      const int64_t k0 = c->cchunk0[s];
+      HIP_OK(hipMemcpyAsync(F.nch + k0 * HFV_NCH_MAX, nch.data(), sizeof(int32_t) * cap_ch * HFV_NCH_MAX,
+                            hipMemcpyHostToDevice, st));
+      HIP_OK(hipMemcpyAsync(F.nchn + k0, nchn.data(), (size_t)cap_ch, hipMemcpyHostToDevice, st));
+    }
     c->has_gstage[s] = std::any_of(gcnt.begin(), gcnt.end(), [](uint16_t k) { return k > 0; }) ? 1 : 0;
     HIP_OK(hipStreamSynchronize(st));
   }

@@ -69,10 +69,10 @@ struct Faces {
   // neighbour's list slot (nloc[2F], 0xFFFF for a wall)
   int32_t* gcell;
   uint16_t *gcnt, *nloc;
-  // per face chunk the cell chunks its faces touch (fcc[chunk][HFV_FCC_MAX]) and their count (fccn, 0:
-  // unknown, the chunk always runs): a step whose touched chunks hold no active cell has no face to compute
-  int32_t* fcc;
-  uint8_t* fccn;
+  // per cell chunk the other cell chunks its cells' neighbours lie in (nch[chunk][HFV_NCH_MAX]) and their
+  // count (nchn; 0xFF: more than HFV_NCH_MAX, or a mesh that changes on the device)
+  int32_t* nch;
+  uint8_t* nchn;
 };
 
 struct Cfg {
@@ -114,7 +114,7 @@ struct BlockJob {
 };
 
 #define HFV_MAX_SUBDRIFTS 8
-#define HFV_FCC_MAX 8
+#define HFV_NCH_MAX 8
 // Engine of one simulation (src/engine.rs:33-101, 152-268).
 struct SimEngine {
   uint64_t ti_old, ti_current, ti_next;
@@ -159,7 +159,7 @@ struct Partials {
   uint32_t* act;    // active cells of a cell chunk (k_mask)
   uint32_t* face;   // computed faces of a face chunk (k_flux)
   uint64_t* ti;     // minimum new dti of a cell chunk's active cells (k_grad)
-  uint32_t* cflag;  // a cell chunk holds a cell of a computed face this step (k_flux; cleared by k_mask)
+  uint32_t* cflag;  // a cell chunk has work in the accumulation and limiter sweeps this step (k_apply)
 };
 
 struct Batch {
@@ -913,7 +913,6 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_mask(Batch B, int mode) {
     uint32_t cnt = block_sum_u32(act ? 1u : 0u);
     if (threadIdx.x == 0) {
       B.P.act[J.chunk] = cnt;
-      B.P.cflag[J.chunk] = 0;
     }
   }
   if (valid) flush_events(ev, B.cnt, B.eng, s);
@@ -1293,20 +1292,6 @@ __device__ __forceinline__ void flux_chunk(const Batch& B, const BlockJob J) {
   if (!B.eng[s].stepping || B.sims[s].static_geom != (STATIC ? 1 : 0)) return;
   const Cfg& cfg = B.sims[s].cfg;
   const int64_t f = (int64_t)J.lo + threadIdx.x;
-  // no active cell in the chunks this chunk's faces touch: no face is computed (block-uniform); else the
-  // touched chunks are all marked as having work (a superset of the chunks of computed faces: one store
-  // per chunk instead of two per face)
-  const int nt = B.F.fccn[J.chunk];
-  if (nt > 0) {
-    const int32_t tc = (int)threadIdx.x < nt ? B.F.fcc[(uint64_t)J.chunk * HFV_FCC_MAX + threadIdx.x] : 0;
-    const bool any = (int)threadIdx.x < nt && B.P.act[tc] > 0;
-    if (!__syncthreads_or(any)) {
-      if (f < J.hi) B.F.cur[f] = 0;
-      if (threadIdx.x == 0) B.P.face[J.chunk] = 0;
-      return;
-    }
-    if ((int)threadIdx.x < nt) B.P.cflag[tc] = 1;
-  }
   constexpr bool DEFER = SOLVER == SOLVER_AIRS;
   // LDS: the staged records, reused for the block's output records (written out as whole lines)
   constexpr bool F32 = sizeof(T) == 4;
@@ -1330,7 +1315,7 @@ __device__ __forceinline__ void flux_chunk(const Batch& B, const BlockJob J) {
   if (f < J.hi) {
     l = B.F.left[f];
     r = B.F.right[f];
-    if (l < 0) {  // past a device-rebuilt mesh's faces: no cells
+    if (!STATIC && l < 0) {  // past a device-rebuilt mesh's faces (dynamic geometry only): no cells
       l = 0;
       r = -1;
     } else {
@@ -1339,11 +1324,6 @@ __device__ __forceinline__ void flux_chunk(const Batch& B, const BlockJob J) {
       cur = la || ra;
     }
     B.F.cur[f] = cur ? 1 : 0;
-    if (cur && nt == 0) {  // the chunks of both cells have work in the accumulation and limiter sweeps
-      const SimDesc& D = B.sims[s];
-      B.P.cflag[D.cchunk0 + (l - D.cell_base) / HFV_BLOCK] = 1;
-      if (r >= 0) B.P.cflag[D.cchunk0 + (r - D.cell_base) / HFV_BLOCK] = 1;
-    }
   }
   if (STATIC) {
     if ((int)threadIdx.x < ns) scl[threadIdx.x] = myc;
@@ -1544,9 +1524,19 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_apply(Batch B, int mode) {
   if (J.sim < 0) return;
   const int s = J.sim;
   if (!B.eng[s].stepping) return;
-  // fused step on a fixed mesh: a chunk without a computed face and without an active cell has nothing to
-  // accumulate, apply or pack
-  if (mode == 7 && B.sims[s].cfg.motion == MOT_FIXED && !B.P.cflag[J.chunk] && !B.P.act[J.chunk]) return;
+  // fused step: a chunk with no active cell whose cells have no active neighbour (no chunk on its list of
+  // neighbour chunks holds an active cell) has nothing to accumulate, apply or pack on a fixed mesh, and
+  // nothing for the limiter sweeps (which read the decision from cflag)
+  if (mode == 7) {
+    bool work = B.P.act[J.chunk] > 0 || B.sims[s].cfg.motion != MOT_FIXED;
+    if (!work) {  // block-uniform
+      const int nn = B.F.nchn[J.chunk];
+      const bool w = nn == 0xFF || ((int)threadIdx.x < nn && B.P.act[B.F.nch[(uint64_t)J.chunk * HFV_NCH_MAX + threadIdx.x]] > 0);
+      work = __syncthreads_or(w);
+    }
+    if (threadIdx.x == 0) B.P.cflag[J.chunk] = work ? 1 : 0;
+    if (!work) return;
+  }
   const int64_t i = (int64_t)J.lo + threadIdx.x;
   if (i >= J.hi) return;
   const Cells& C = B.C;
@@ -2316,10 +2306,10 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_geom_faces(Batch B, int s, int64_
 // ------------------------------------------------------------------ limiter
 // timestep_limiter first sweep (hydrodynamics.rs:165-179): wakeup from active neighbours' new bins;
 // timestep_sync (hydrodynamics.rs:186-191) reduces the minimum bin per simulation instead.
-// A chunk with no computed face and no active cell has no active neighbour anywhere: its wakeups are the
+// A chunk without active cells or active neighbours (k_apply's decision in cflag): its wakeups are the
 // cells' own bins and k_limit changes nothing there, so the fused step (fused = 1) skips it in both.
 __device__ __forceinline__ bool limiter_idle(const Batch& B, const BlockJob& J) {
-  return !B.sims[J.sim].cfg.sync_all && !B.P.cflag[J.chunk] && !B.P.act[J.chunk];
+  return !B.sims[J.sim].cfg.sync_all && !B.P.cflag[J.chunk];
 }
 
 __global__ void __launch_bounds__(HFV_BLOCK) k_wakeup(Batch B, int fused) {
