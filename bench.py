@@ -594,23 +594,31 @@ def bench_sedov(args):
     return out, prob
 
 
-def bench_moving2d(args):
-    """The 2D Sedov blast on a moving mesh (steered generators, the reference's default particle motion):
-    drift, device Voronoi rebuild and step per engine step.  The rebuild's device time against the host
-    tessellator's (OpenMP, every host thread) on the same generators; L1(rho) against the FP64 oracle
-    (which re-tessellates with the host tessellator) at t = --moving-l1-t."""
+def bench_moving(args, dim):
+    """A moving mesh (steered generators, the reference's default particle motion): drift, device Voronoi
+    rebuild and step per engine step.  2D: the Sedov blast; 3D: the Kelvin-Helmholtz shear layer.  The
+    rebuild's device time against the host tessellator's (OpenMP, every host thread) on the same
+    generators; L1(rho) against the FP64 oracle (which re-tessellates with the host tessellator) at
+    t = --moving-l1-t."""
     from dataclasses import replace
 
     from hydro_rs_amd import fvs, ics
     from hydro_rs_amd.mesh import VoronoiMesh
 
-    base = ics.sedov_2d(side=args.moving_side, t_end=0.05)
+    tag = f"moving{dim}d"
+    if dim == 2:
+        side, steps, warmup = args.moving_side, args.moving_steps, args.moving_warmup
+        base = ics.sedov_2d(side=side, t_end=0.05)
+        workload = f"sedov2d_{side}x{side}_periodic_hllc_steer"
+    else:
+        side, steps, warmup = args.moving3d_side, args.moving3d_steps, args.moving3d_warmup
+        base = ics.kelvin_helmholtz_3d(side=side, t_end=1.0, threads=args.threads)
+        workload = f"kh3d_{side}^3_periodic_airs_steer"
     prob = replace(base, cfg=replace(base.cfg, motion=fvs.STEER))
-    log(f"[moving2d] {prob.n} cells, steered generators, {args.moving_warmup} + {args.moving_steps} steps")
-    out = {"workload": f"sedov2d_{args.moving_side}x{args.moving_side}_periodic_hllc_steer", "cells": prob.n,
-           "warmup": args.moving_warmup, "steps": args.moving_steps}
+    log(f"[{tag}] {prob.n} cells, steered generators, {warmup} + {steps} steps")
+    out = {"workload": workload, "cells": prob.n, "warmup": warmup, "steps": steps}
     for k, prec in (("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)):
-        d, st = run_device(ics.with_precision(prob, prec), args.moving_warmup, args.moving_steps, profile=True)
+        d, st = run_device(ics.with_precision(prob, prec), warmup, steps, profile=True)
         kt = st["kernel_times"]
         per = {name: round(v["ms"] / max(v["launches"], 1) * 1e3, 2) for name, v in kt.items() if v["launches"]}
         out[k] = {"cell_updates_per_s": st["cell_updates"] / st["seconds"],
@@ -618,20 +626,20 @@ def bench_moving2d(args):
                   "mean_active_cells": st["cell_updates"] / max(st["steps"], 1),
                   "per_kernel_us": per}
         if k == "fp64":
-            loc = d.get_state(["loc"])["loc"].reshape(3, prob.n).T.copy()
             faces = d.get_mesh()["n_faces"]
         d.close()
-        log(f"[moving2d] {k}: {out[k]}")
+        log(f"[{tag}] {k}: {out[k]}")
     # a full rebuild (every cell active: timestep_sync) on the device, against the host tessellator's full
     # build of the same generator positions (the device rebuild's CPU counterpart)
-    dfull, sfull = run_device(replace(prob, cfg=replace(prob.cfg, sync_all=True)), 2, 10, profile=True)
+    dfull, sfull = run_device(replace(prob, cfg=replace(prob.cfg, sync_all=True)), 2, 10 if dim == 2 else 4,
+                              profile=True)
     loc_full = dfull.get_state(["loc"])["loc"].reshape(3, prob.n).T.copy()
     dfull.close()
     full_us = sfull["kernel_times"]["mesh_rebuild"]["ms"] / max(sfull["kernel_times"]["mesh_rebuild"]["launches"], 1) * 1e3
     t = time.perf_counter()
-    reps = 3
+    reps = 3 if dim == 2 else 1
     for _ in range(reps):
-        VoronoiMesh(2, prob.cfg.box, True, loc_full, n_threads=args.threads)
+        VoronoiMesh(dim, prob.cfg.box, True, loc_full, n_threads=args.threads)
     host_ms = 1e3 * (time.perf_counter() - t) / reps
     out["rebuild"] = {"partial_device_us": out["fp64"]["per_kernel_us"].get("mesh_rebuild"),
                       "partial_faces": int(faces), "full_device_us": full_us, "full_host_tessellator_ms": host_ms,
@@ -639,8 +647,9 @@ def bench_moving2d(args):
                       "full_speedup_vs_host": 1e3 * host_ms / full_us if full_us else None,
                       "device_time_from": "HIP events around the rebuild's kernels (eager launches)"}
     if not args.no_oracle:
-        short = replace(prob, cfg=replace(prob.cfg, t_end=args.moving_l1_t))
-        log(f"[moving2d] FP64, FP32 flux and the FP64 oracle to t = {args.moving_l1_t} (L1 check)")
+        l1_t = args.moving_l1_t if dim == 2 else args.moving3d_l1_t
+        short = replace(prob, cfg=replace(prob.cfg, t_end=l1_t))
+        log(f"[{tag}] FP64, FP32 flux and the FP64 oracle to t = {l1_t} (L1 check)")
         res = {}
         for k, prec in (("fp64", fvs.FP64), ("fp32_flux", fvs.FP32_FLUX)):
             dd = ics.build_device_sim(ics.with_precision(short, prec))
@@ -650,10 +659,10 @@ def bench_moving2d(args):
         t = time.perf_counter()
         ref, oi = oracle_run(short)
         out["l1_vs_oracle"] = {k: _l1(v, ref) for k, v in res.items()}
-        out["l1_vs_oracle"].update({"at": f"t_end {args.moving_l1_t}", "oracle_steps": oi["step_count"],
+        out["l1_vs_oracle"].update({"at": f"t_end {l1_t}", "oracle_steps": oi["step_count"],
                                     "oracle_seconds": time.perf_counter() - t,
                                     "pass": _l1(res["fp32_flux"], ref) < 1e-5})
-    log(f"[moving2d] {out}")
+    log(f"[{tag}] {out}")
     return out
 
 
@@ -793,12 +802,18 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--window-start", type=int, default=260,
                     help="ensemble: untimed launches (tuning included) before the timed window, at least")
-    ap.add_argument("--workload", default="all", choices=["all", "ensemble", "sedov", "moving2d", "kh3d", "serve"])
+    ap.add_argument("--workload", default="all",
+                    choices=["all", "ensemble", "sedov", "moving2d", "moving3d", "kh3d", "serve"])
     ap.add_argument("--moving-side", type=int, default=256)
     ap.add_argument("--moving-steps", type=int, default=100)
     ap.add_argument("--moving-warmup", type=int, default=20)
     ap.add_argument("--moving-l1-t", type=float, default=0.002,
                     help="moving 2D workload: end time of the L1 check against the oracle")
+    ap.add_argument("--moving3d-side", type=int, default=48)
+    ap.add_argument("--moving3d-steps", type=int, default=20)
+    ap.add_argument("--moving3d-warmup", type=int, default=5)
+    ap.add_argument("--moving3d-l1-t", type=float, default=0.002,
+                    help="moving 3D workload: end time of the L1 check against the oracle")
     ap.add_argument("--members", type=int, default=512)
     ap.add_argument("--plain-stride", type=int, default=16)
     ap.add_argument("--oracle-stride", type=int, default=64, help="ensemble members compared with the oracle")
@@ -906,7 +921,9 @@ def main():
     if w in ("all", "sedov"):
         out["sedov2d"], _ = bench_sedov(args)
     if w in ("all", "moving2d"):
-        out["moving2d"] = bench_moving2d(args)
+        out["moving2d"] = bench_moving(args, 2)
+    if w in ("all", "moving3d"):
+        out["moving3d"] = bench_moving(args, 3)
     if w in ("all", "kh3d"):
         out["kh3d"] = bench_kh3d(args)
     if w in ("all", "serve"):
@@ -917,11 +934,11 @@ def main():
             out.update({"value": s["cell_updates_per_s"], "steps": s["steps"], "warmup": 0,
                         "ms_per_step": s["us_per_step"] / 1e3, "dtype": "f64",
                         "config": {"workload": out["sedov2d"]["workload"]}})
-        elif w == "moving2d":
-            s = out["moving2d"]["fp64"]
-            out.update({"value": s["cell_updates_per_s"], "steps": args.moving_steps, "warmup": args.moving_warmup,
+        elif w in ("moving2d", "moving3d"):
+            s = out[w]["fp64"]
+            out.update({"value": s["cell_updates_per_s"], "steps": out[w]["steps"], "warmup": out[w]["warmup"],
                         "ms_per_step": s["us_per_step"] / 1e3, "dtype": "f64",
-                        "config": {"workload": out["moving2d"]["workload"]}})
+                        "config": {"workload": out[w]["workload"]}})
         elif w == "kh3d":
             key = "fp64" if "fp64" in out["kh3d"] else "fp32_flux"
             s = out["kh3d"][key]

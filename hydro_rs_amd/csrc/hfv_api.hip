@@ -18,6 +18,7 @@
 #include "../../include/hfv.h"
 #include "hfv_kernels.hpp"
 #include "hfv_voronoi2d.hpp"
+#include "hfv_voronoi3d.hpp"
 
 using namespace hfv;
 
@@ -212,7 +213,8 @@ struct hfv_ctx {
   int precision = HFV_FP64;
   int solver = -1;  // uniform solver of the loaded simulations, or -1
   bool any_moving1d = false;
-  bool any_moving2d = false;
+  bool any_vmesh = false;   // a 2D or 3D moving mesh rebuilt on the device
+  bool any_vmesh3 = false;  // ... a 3D one
   bool any_moving = false;
   hipGraph_t graph{};
   hipGraphExec_t gexec{};
@@ -303,16 +305,23 @@ int ensure_v2(hfv_ctx* c) {
   return HFV_OK;
 }
 
-// The host tessellator's bin grid for n generators in a 2D box (hydro_rs_amd/csrc/host/voronoi.cpp,
+// The host tessellator's bin grid for n generators in a 2D or 3D box (hydro_rs_amd/csrc/host/voronoi.cpp,
 // Grid::build), evaluated with the same operations.
-int v2_grid(const hfv_params& p, int64_t n, SimDesc& D) {
-  const double vol = p.box[0] * p.box[1];
-  const double spacing = std::sqrt(vol / (double)std::max<int64_t>(n, 1));
-  for (int a = 0; a < 2; ++a) {
-    D.v2nb[a] = std::max(1, (int)std::floor(p.box[a] / spacing));
-    D.v2w[a] = p.box[a] / D.v2nb[a];
+int v_grid(const hfv_params& p, int64_t n, SimDesc& D) {
+  double vol = 1;
+  for (int a = 0; a < p.dim; ++a) vol *= p.box[a];
+  const double per_gen = vol / (double)std::max<int64_t>(n, 1);
+  const double spacing = p.dim == 2 ? std::sqrt(per_gen) : std::cbrt(per_gen);
+  for (int a = 0; a < 3; ++a) {
+    if (a < p.dim) {
+      D.vnb[a] = std::max(1, (int)std::floor(p.box[a] / spacing));
+      D.vw[a] = p.box[a] / D.vnb[a];
+    } else {
+      D.vnb[a] = 1;
+      D.vw[a] = 1;
+    }
   }
-  if ((int64_t)D.v2nb[0] * D.v2nb[1] > n) return set_err(HFV_ERR_INVALID, "internal: more bins than generators");
+  if ((int64_t)D.vnb[0] * D.vnb[1] * D.vnb[2] > n) return set_err(HFV_ERR_INVALID, "internal: more bins than generators");
   return HFV_OK;
 }
 
@@ -344,7 +353,7 @@ int64_t grid_position(int64_t j, int64_t g) { return 8 * ((j / g / 8) * g + j % 
 
 // Faces the face kernels cover: the mesh's, or the whole capacity where the device rebuild changes the
 // face count (the faces past the current mesh have no cells and are skipped).
-int64_t face_range(const hfv_ctx* c, int s) { return c->desc[s].moving2d ? c->fcap[s] : c->fused[s]; }
+int64_t face_range(const hfv_ctx* c, int s) { return c->desc[s].vmesh ? c->fcap[s] : c->fused[s]; }
 
 // Rebuild the block tables from the live slots (loaded, and in a batch not finished: a finished slot's
 // blocks would only exit) and upload them with the simulation descriptors.  Chunk ids (partials) stay
@@ -577,7 +586,8 @@ int push_engines(hfv_ctx* c) {
 void recompute_uniforms(hfv_ctx* c) {
   int sol = -2;
   c->any_moving1d = false;
-  c->any_moving2d = false;
+  c->any_vmesh = false;
+  c->any_vmesh3 = false;
   c->any_moving = false;
   int prec = -1;
   for (int s = 0; s < c->nslots; ++s) {
@@ -586,7 +596,8 @@ void recompute_uniforms(hfv_ctx* c) {
     if (sol == -2) sol = p.solver;
     else if (sol != p.solver) sol = -1;
     if (p.dim == 1 && p.motion != HFV_FIXED) c->any_moving1d = true;
-    if (p.dim == 2 && p.motion != HFV_FIXED) c->any_moving2d = true;
+    if (p.dim >= 2 && p.motion != HFV_FIXED) c->any_vmesh = true;
+    if (p.dim == 3 && p.motion != HFV_FIXED) c->any_vmesh3 = true;
     if (p.motion != HFV_FIXED) c->any_moving = true;
     if (prec < 0) prec = p.precision;
   }
@@ -606,15 +617,16 @@ void recompute_uniforms(hfv_ctx* c) {
 struct LaunchSet {
   int solver;
   int precision;
-  bool moving1d, moving2d, moving, st, dyn, sgrad, dgrad;
+  bool moving1d, vmesh, vmesh3, moving, st, dyn, sgrad, dgrad;
   bool operator==(const LaunchSet& o) const {
-    return solver == o.solver && precision == o.precision && moving1d == o.moving1d && moving2d == o.moving2d &&
+    return solver == o.solver && precision == o.precision && moving1d == o.moving1d && vmesh == o.vmesh &&
+           vmesh3 == o.vmesh3 &&
            moving == o.moving &&
            st == o.st && dyn == o.dyn && sgrad == o.sgrad && dgrad == o.dgrad;
   }
 };
 LaunchSet launch_set(const hfv_ctx* c) {
-  return LaunchSet{c->solver, c->precision, c->any_moving1d, c->any_moving2d, c->any_moving, c->any_static,
+  return LaunchSet{c->solver, c->precision, c->any_moving1d, c->any_vmesh, c->any_vmesh3, c->any_moving, c->any_static,
                    c->any_dynamic, c->any_static_grad, c->any_dynamic_grad};
 }
 
@@ -767,8 +779,8 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
       const int64_t cap_ch = chunks_of(c->ccap[s]);
       std::vector<int32_t> nch(cap_ch * HFV_NCH_MAX, 0);
       std::vector<uint8_t> nchn(cap_ch, 0xFF);
-      const bool moving2d = p.dim == 2 && p.motion != HFV_FIXED;
-      for (int64_t j = 0; j < ncch && !moving2d; ++j) {
+      const bool vmesh = p.dim >= 2 && p.motion != HFV_FIXED;
+      for (int64_t j = 0; j < ncch && !vmesh; ++j) {
         const int64_t i0 = j * BLOCK, i1 = std::min<int64_t>(i0 + BLOCK, n);
         const int64_t e0 = off[i0] - kb, e1 = (i1 < n ? off[i1] : kb + acc) - kb;
         const int32_t own = (int32_t)(c->cchunk0[s] + j);
@@ -831,13 +843,13 @@ int upload_mesh(hfv_ctx* c, int s, const hfv_mesh_arrays* m, const uint8_t* upda
   D.nf = nf;
   D.moving1d = p.dim == 1 && p.motion != HFV_FIXED;
   D.periodic = p.boundary == HFV_PERIODIC;
-  D.moving2d = p.dim == 2 && p.motion != HFV_FIXED;
+  D.vmesh = p.dim >= 2 && p.motion != HFV_FIXED ? p.dim : 0;
   D.conn_base = c->kbase[s];
   D.conn_cap = c->kcap[s];
   D.face_cap = c->fcap[s];
   D.noff = (int)(icap(c, s) + (s == c->nslots - 1 ? 1 : 0));
-  if (D.moving2d) {
-    TRY(v2_grid(p, n, D));
+  if (D.vmesh) {
+    TRY(v_grid(p, n, D));
     TRY(ensure_v2(c));
     c->heng[s].nf_mesh = (int)nf;
     HIP_OK(hipMemcpyAsync(&c->deng[s].nf_mesh, &c->heng[s].nf_mesh, sizeof(int), hipMemcpyHostToDevice, st));
@@ -1039,7 +1051,7 @@ void launch_grad(hfv_ctx* c, const Batch& B, int do_grad, int do_ts) {
 }
 
 // The kernels of one batch step (k_begin .. k_end).  Timing events only outside graph capture.
-// Space::volume_calculation on the device for the moving meshes being stepped (1D and 2D).
+// Space::volume_calculation on the device for the moving meshes being stepped (1D, 2D and 3D).
 void enqueue_rebuild(hfv_ctx* c, const Batch& B) {
   hipStream_t st = c->stream;
   const dim3 gc((unsigned)c->ncg), gf((unsigned)c->nfg), gb(BLOCK);
@@ -1047,7 +1059,7 @@ void enqueue_rebuild(hfv_ctx* c, const Batch& B) {
     hipLaunchKernelGGL(k_mesh1d_faces, gf, gb, 0, st, B);
     hipLaunchKernelGGL(k_mesh1d_cells, gc, gb, 0, st, B);
   }
-  if (c->any_moving2d) {
+  if (c->any_vmesh) {
     const V2Bufs& V = c->v2;
     const dim3 gs((unsigned)c->nslots), gsb(HFV_SCAN_BLOCK);
     hipLaunchKernelGGL(k_v2_clear, gc, gb, 0, st, B, V);
@@ -1058,6 +1070,11 @@ void enqueue_rebuild(hfv_ctx* c, const Batch& B) {
     hipLaunchKernelGGL(k_v2_place, gc, gb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_cells<false>, dim3((unsigned)(c->ncg * HFV_V2_SUB)), gb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_cells<true>, dim3((unsigned)(c->ncg * HFV_V2_SUB)), gb, 0, st, B, V);
+    if (c->any_vmesh3) {
+      const dim3 g3((unsigned)(c->ncg * BLOCK)), w3(64);
+      hipLaunchKernelGGL(k_v3_cells<false>, g3, w3, 0, st, B, V);
+      hipLaunchKernelGGL(k_v3_cells<true>, g3, w3, 0, st, B, V);
+    }
     hipLaunchKernelGGL(k_v2_scan_up<1>, gc, gb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_scan_mid<1>, gs, gsb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_scan_down<1>, gc, gb, 0, st, B, V);
@@ -1086,7 +1103,7 @@ void enqueue_step(hfv_ctx* c, bool timed) {
   if (t) prof_begin(c, KS_PREDICT);
   hipLaunchKernelGGL(k_mask, gc, gb, 0, st, B, 0);
   if (t) prof_end(c, KS_PREDICT);
-  if (c->any_moving1d || c->any_moving2d) {
+  if (c->any_moving1d || c->any_vmesh) {
     if (t) prof_begin(c, KS_MESH);
     enqueue_rebuild(c, B);
     if (t) prof_end(c, KS_MESH);
@@ -1137,7 +1154,7 @@ int launch_step(hfv_ctx* c) {
 }
 
 // Faces of a slot's current mesh (after sync_counters for device-rebuilt meshes).
-int64_t mesh_faces(const hfv_ctx* c, int s) { return c->desc[s].moving2d ? c->heng[s].nf_mesh : c->fused[s]; }
+int64_t mesh_faces(const hfv_ctx* c, int s) { return c->desc[s].vmesh ? c->heng[s].nf_mesh : c->fused[s]; }
 
 int check_slot(hfv_ctx* c, int s) {
   if (!c) return set_err(HFV_ERR_INVALID, "ctx is NULL");
@@ -1697,19 +1714,10 @@ int hfv_step(hfv_ctx* c, const hfv_timestep_info* info, uint64_t* ti_next) {
 }
 
 // ---- engine (src/engine.rs:152-268), device-resident
-// The device engine loops drift generators (k_drift) and rebuild 1D and 2D meshes: a 3D simulation with
-// moving generators needs the caller's re-tessellation between drift and fluxes (hfv_step with
-// hfv_set_mesh), so the loops refuse it rather than step it on a stale mesh.
-int check_engine_loop(const hfv_params& p) {
-  if (p.dim == 3 && p.motion != HFV_FIXED)
-    return set_err(HFV_ERR_UNSUPPORTED,
-                   "3D moving meshes are rebuilt by the caller each step: use hfv_step with hfv_set_mesh");
-  return HFV_OK;
-}
+// The device engine loops drift generators (k_drift) and rebuild moving meshes in every dimension.
 
 int hfv_run(hfv_ctx* c, int64_t max_steps, int64_t* steps_done) {
   TRY(require_mesh(c));
-  TRY(check_engine_loop(c->params[0]));
   TRY(sync_counters(c));
   const long long s0 = c->heng[0].step_count;
   int64_t done = 0;
@@ -1778,7 +1786,6 @@ int hfv_batch_load(hfv_ctx* c, int s, const hfv_params* params, const hfv_mesh_a
                    const double* mass, const double* vel, const double* u_int) {
   TRY(check_slot(c, s));
   TRY(check_params(params));
-  TRY(check_engine_loop(*params));
   if (!c->batch) return set_err(HFV_ERR_INVALID, "not a batch context");
   if (!mesh) return set_err(HFV_ERR_INVALID, "mesh is NULL");
   for (int k = 0; k < c->nslots; ++k)

@@ -99,12 +99,14 @@ struct SimDesc {
   int static_grad;  // ... and gradient geometry too (k_geom_cells; chosen for 3D meshes)
   int staged;       // static_geom and some face chunk has a staging list: the flux records fr are kept
   int gstaged;      // static_geom and the gradient sweeps stage their chunks' neighbour lines
-  // 2D moving mesh rebuilt on the device (hfv_voronoi2d.hpp): the bin grid of the tessellator (nb bins of
-  // width w per axis), the slot's CSR base and capacities, and its count of CSR offsets
-  int moving2d;
-  int v2nb[2];
+  // 2D/3D moving mesh rebuilt on the device (hfv_voronoi2d.hpp, hfv_voronoi3d.hpp; vmesh = the dimension,
+  // 0: none): the bin grid of the tessellator (nb bins of width w per axis; 1 bin of width 1 along an
+  // axis past the dimension), the slot's CSR base and capacities, and its count of CSR offsets
+  int vmesh;
+  int vnb[3];
   int noff;
-  double v2w[2];
+  int pad_v;
+  double vw[3];
   int64_t conn_base, conn_cap, face_cap;
 };
 

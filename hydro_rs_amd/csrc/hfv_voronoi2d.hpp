@@ -13,7 +13,7 @@
 // the order of the emitting cell's polygon edges (v2_emits); face indices follow the cells; each cell's
 // CSR lists its faces in ascending face index.
 //
-// Per step (simulations with moving2d, while stepping), after the drift and the predict:
+// Per step (simulations with vmesh = 2, while stepping; 3D: hfv_voronoi3d.hpp), after the drift and the predict:
 //   k_v2_clear, k_v2_bin, scan<0>, k_v2_place          the bin grid (counting sort of the generators)
 //   k_v2_cells<0>, k_v2_cells<1>                       the active cells' polygons, volume,
 //                                                      centroid and search radius (Particle::
@@ -29,10 +29,10 @@ namespace hfv {
 
 #define HFV_SCAN_BLOCK 1024
 
-// A face in the provisional order: length, centroid and unit normal relative to the emitting generator,
-// the neighbour's local index (< 0: a wall) and periodic image code.
+// A face in the provisional order: area (length in 2D), centroid and unit normal relative to the emitting
+// generator, the neighbour's local index (< 0: a wall) and periodic image code.
 struct V2Face {
-  double area, cx, cy, nx, ny;
+  double area, cx, cy, cz, nx, ny, nz;
   int32_t nb, code;
 };
 
@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_clear(Batch B, V2Bufs V) {
   if (J.sim < 0) return;
   const int s = J.sim;
   const SimDesc& D = B.sims[s];
-  if (!D.moving2d || !B.eng[s].stepping) return;
+  if (!D.vmesh || !B.eng[s].stepping) return;
   const int64_t gi = (int64_t)J.lo + threadIdx.x;
   if (gi >= J.hi) return;
   const int64_t k = D.cell_base + s + (gi - D.cell_base);
@@ -92,11 +92,13 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_bin(Batch B, V2Bufs V) {
   if (J.sim < 0) return;
   const int s = J.sim;
   const SimDesc& D = B.sims[s];
-  if (!D.moving2d || !B.eng[s].stepping) return;
+  if (!D.vmesh || !B.eng[s].stepping) return;
   const int64_t gi = (int64_t)J.lo + threadIdx.x;
   if (gi >= J.hi) return;
   const int64_t N = B.C.n;
-  const int key = v2_bin(B.C.loc[gi], D.v2w[0], D.v2nb[0]) * D.v2nb[1] + v2_bin(B.C.loc[N + gi], D.v2w[1], D.v2nb[1]);
+  const int key = (v2_bin(B.C.loc[gi], D.vw[0], D.vnb[0]) * D.vnb[1] + v2_bin(B.C.loc[N + gi], D.vw[1], D.vnb[1])) *
+                      D.vnb[2] +
+                  (D.vmesh == 3 ? v2_bin(B.C.loc[2 * N + gi], D.vw[2], D.vnb[2]) : 0);
   V.gkey[gi] = key;
   atomicAdd(&V.gcnt[D.cell_base + s + key], 1);
 }
@@ -110,7 +112,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_bin(Batch B, V2Bufs V) {
 // 2 CSR entries per cell (length n; the CSR offsets of the cells).
 template <int A>
 __device__ __forceinline__ int32_t* v2_scan_region(const V2Bufs& V, const SimDesc& D, int s, int64_t& len) {
-  len = A == 0 ? (int64_t)D.v2nb[0] * D.v2nb[1] : D.n;
+  len = A == 0 ? (int64_t)D.vnb[0] * D.vnb[1] * D.vnb[2] : D.n;
   return (A == 0 ? V.gcnt : A == 1 ? V.femit : V.ccnt) + D.cell_base + s;
 }
 
@@ -141,7 +143,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_scan_up(Batch B, V2Bufs V) {
   if (J.sim < 0) return;
   const int s = J.sim;
   const SimDesc& D = B.sims[s];
-  if (!D.moving2d || !B.eng[s].stepping) return;
+  if (!D.vmesh || !B.eng[s].stepping) return;
   int64_t len;
   const int32_t* a = v2_scan_region<A>(V, D, s, len);
   const int64_t k = (int64_t)J.lo - D.cell_base + threadIdx.x;
@@ -155,7 +157,7 @@ __global__ void __launch_bounds__(HFV_SCAN_BLOCK) k_v2_scan_mid(Batch B, V2Bufs 
   const int s = blockIdx.x;
   const SimDesc& D = B.sims[s];
   SimEngine& E = B.eng[s];
-  if (!D.moving2d || !E.stepping) return;
+  if (!D.vmesh || !E.stepping) return;
   int64_t len;
   int32_t* a = v2_scan_region<A>(V, D, s, len);
   __shared__ int32_t wsum[HFV_SCAN_BLOCK / 64];
@@ -205,7 +207,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_scan_down(Batch B, V2Bufs V) {
   const int s = J.sim;
   const SimDesc& D = B.sims[s];
   const SimEngine& E = B.eng[s];
-  if (!D.moving2d || !E.stepping) return;
+  if (!D.vmesh || !E.stepping) return;
   int64_t len;
   int32_t* a = v2_scan_region<A>(V, D, s, len);
   const int64_t k = (int64_t)J.lo - D.cell_base + threadIdx.x;
@@ -226,7 +228,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_place(Batch B, V2Bufs V) {
   if (J.sim < 0) return;
   const int s = J.sim;
   const SimDesc& D = B.sims[s];
-  if (!D.moving2d || !B.eng[s].stepping) return;
+  if (!D.vmesh || !B.eng[s].stepping) return;
   const int64_t gi = (int64_t)J.lo + threadIdx.x;
   if (gi >= J.hi) return;
   const int32_t p = atomicAdd(&V.gfill[D.cell_base + s + V.gkey[gi]], 1);
@@ -312,8 +314,10 @@ __device__ void v2_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, Sim
     f.area = len;
     f.cx = 0.5 * (ax + bxx);
     f.cy = 0.5 * (ay + byy);
+    f.cz = 0.;
     f.nx = inv * P.nx[k];
     f.ny = inv * P.ny[k];
+    f.nz = 0.;
     f.nb = P.nb[k] < 0 ? -1 : P.nb[k];
     f.code = P.code[k];
     out[q++] = f;
@@ -456,9 +460,9 @@ __device__ __forceinline__ void v2_lane_candidates(const Batch& B, const SimDesc
   const int64_t N = B.C.n, cb = D.cell_base;
   const double* L = B.C.loc;
   const double bx = D.cfg.box[0], by = D.cfg.box[1], bz = D.cfg.box[2];
-  const int nb0 = D.v2nb[0], nb1 = D.v2nb[1];
+  const int nb0 = D.vnb[0], nb1 = D.vnb[1];
   const bool per = D.periodic != 0;
-  const int bi0 = v2_bin(xi, D.v2w[0], nb0), bi1 = v2_bin(yi, D.v2w[1], nb1);
+  const int bi0 = v2_bin(xi, D.vw[0], nb0), bi1 = v2_bin(yi, D.vw[1], nb1);
   const int32_t* gst = V.gcnt + cb + s;
   const int32_t* gord = V.gord + cb;
   const int side = 2 * K + 1;
@@ -531,7 +535,7 @@ __device__ __forceinline__ void team_min(bool& have, V2Cand& b) {
 template <bool LIST, int TL>
 __device__ int v2_team_build(const Batch& B, const SimDesc& D, const V2Bufs& V, int s, int32_t i, double xi,
                              double yi, double zi, int K, V2TeamPoly<TL>& P, int& m, double& r2, int lane, int tiw) {
-  const double w0 = D.v2w[0], w1 = D.v2w[1];
+  const double w0 = D.vw[0], w1 = D.vw[1];
   const bool per = D.periodic != 0;
   // initial region: the box, or K bin widths around the generator on a periodic box
   double reach = 0.;
@@ -620,7 +624,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_cells(Batch B, V2Bufs V) {
   const int s = J.sim;
   const SimDesc& D = B.sims[s];
   SimEngine& E = B.eng[s];
-  if (!D.moving2d || !E.stepping) return;
+  if (D.vmesh != 2 || !E.stepping) return;
   const int team = threadIdx.x / HFV_V2_TEAM, lane = threadIdx.x % HFV_V2_TEAM;
   const int tiw = (threadIdx.x & 63) / HFV_V2_TEAM;
   const int64_t gi = (int64_t)J.lo + (blockIdx.x % HFV_V2_SUB) * (HFV_BLOCK / HFV_V2_TEAM) + team;
@@ -633,7 +637,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_cells(Batch B, V2Bufs V) {
   V2TeamPoly<TL>& P = polys[team];
   int m = 0, r = 1;
   double r2 = 0.;
-  const int kmax = max(6, max(D.v2nb[0], D.v2nb[1]));
+  const int kmax = max(6, max(D.vnb[0], D.vnb[1]));
   for (int K = 2; K <= kmax && r == 1; ++K)
     r = v2_team_build<!WIDE, TL>(B, D, V, s, i, xi, yi, zi, K, P, m, r2, lane, tiw);
   if (lane != 0) return;
@@ -655,7 +659,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_faces(Batch B, V2Bufs V) {
   const int s = J.sim;
   const SimDesc& D = B.sims[s];
   const SimEngine& E = B.eng[s];
-  if (!D.moving2d || !E.stepping) return;
+  if (!D.vmesh || !E.stepping) return;
   const int64_t f = (int64_t)J.lo + threadIdx.x;
   if (f >= J.hi) return;
   Faces& F = B.F;
@@ -677,10 +681,10 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_faces(Batch B, V2Bufs V) {
   F.area[f] = r.area;
   F.nrm[f] = r.nx;
   F.nrm[NF + f] = r.ny;
-  F.nrm[2 * NF + f] = 0.;
+  F.nrm[2 * NF + f] = D.vmesh == 3 ? r.nz : 0.;
   F.fcen[f] = B.C.loc[gi] + r.cx;
   F.fcen[NF + f] = B.C.loc[N + gi] + r.cy;
-  F.fcen[2 * NF + f] = B.C.loc[2 * N + gi];
+  F.fcen[2 * NF + f] = D.vmesh == 3 ? B.C.loc[2 * N + gi] + r.cz : B.C.loc[2 * N + gi];
   F.left[f] = (int32_t)gi;
   F.right[f] = r.nb < 0 ? -1 : (int32_t)(cb + r.nb);
   F.shc[f] = (uint8_t)r.code;
@@ -694,7 +698,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_csr_fill(Batch B, V2Bufs V) {
   const int s = J.sim;
   const SimDesc& D = B.sims[s];
   const SimEngine& E = B.eng[s];
-  if (!D.moving2d || !E.stepping || E.err) return;
+  if (!D.vmesh || !E.stepping || E.err) return;
   const int64_t f = (int64_t)J.lo + threadIdx.x;
   if (f >= J.hi || f - D.face_base >= E.nf_mesh) return;
   const int32_t l = B.F.left[f], r = B.F.right[f];
@@ -714,7 +718,7 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_csr_sort(Batch B, V2Bufs V) {
   const int s = J.sim;
   const SimDesc& D = B.sims[s];
   SimEngine& E = B.eng[s];
-  if (!D.moving2d || !E.stepping || E.err) return;
+  if (!D.vmesh || !E.stepping || E.err) return;
   const int64_t gi = (int64_t)J.lo + threadIdx.x;
   if (gi >= J.hi) return;
   const int32_t e0 = B.F.off[gi], e1 = B.F.off[gi + 1];

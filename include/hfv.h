@@ -249,15 +249,13 @@ int hfv_gravity_accelerations(hfv_ctx* ctx, double* a_out);
 
 /* ---------------------------------------------------------------- fused step and engine */
 /* OptimalOrderRunner::step / GodunovHydroRunner::step (optimal_order.rs:15-64, godunov.rs:14-58).  For
- * moving meshes in 1D and 2D the step's Voronoi rebuild (Space::volume_calculation, space.rs:236-263)
- * runs on the device from the current generator positions; 3D moving meshes are rebuilt by the caller
- * through hfv_set_mesh. */
+ * moving meshes the step's Voronoi rebuild (Space::volume_calculation, space.rs:236-263; in 2D and 3D the
+ * partial build of the step's active cells) runs on the device from the current generator positions. */
 int hfv_step(hfv_ctx* ctx, const hfv_timestep_info* info, uint64_t* ti_next);
 /* Engine::run (src/engine.rs:152-192): advance until max_steps Step sync points ran or t_end.  The
- * device loop drifts generators and rebuilds 1D and 2D meshes; a 3D simulation with moving generators
- * returns HFV_ERR_UNSUPPORTED (drive it with hfv_step and hfv_set_mesh instead).  A device rebuild that
- * fails (more faces or CSR entries than the context holds, a cell of more than 40 edges) stops the
- * simulation with HFV_ERR_MESH. */
+ * device loop drifts generators and rebuilds moving meshes.  A device rebuild that fails (more faces or
+ * CSR entries than the context holds; a 2D cell of more than 48 edges; a 3D cell of more than 64 faces or
+ * 128 vertices, or a face of more than 24 vertices) stops the simulation with HFV_ERR_MESH. */
 int hfv_run(hfv_ctx* ctx, int64_t max_steps, int64_t* steps_done);
 int hfv_get_engine_info(hfv_ctx* ctx, hfv_engine_info* out);
 /* Space::status sums (src/space.rs:1248-1280): sum of conserved + pending fluxes per component,
@@ -282,9 +280,8 @@ int hfv_batch_create(int nslots, const int64_t* cell_cap, const int64_t* face_ca
                      hfv_ctx** out);
 /* Load a simulation into a slot: Space::initialize + first_init (src/space.rs:88-211) from the mesh and
  * the IC arrays (loc, vel [3n] xyz interleaved; mass, u_int [n]); resets the slot's engine (Engine::new,
- * src/engine.rs:104-149).  Fails with HFV_ERR_INVALID if the mesh exceeds the slot's capacity, and with
- * HFV_ERR_UNSUPPORTED for a 3D simulation with moving generators (batches run the device engine
- * loop, which rebuilds 1D and 2D meshes). */
+ * src/engine.rs:104-149).  Fails with HFV_ERR_INVALID if the mesh exceeds the slot's capacity.  Slots of
+ * moving simulations need face and CSR capacity above the initial mesh's (rebuilt meshes' counts vary). */
 int hfv_batch_load(hfv_ctx* ctx, int slot, const hfv_params* params, const hfv_mesh_arrays* mesh,
                    const double* loc, const double* mass, const double* vel, const double* u_int);
 /* Mark a slot empty (its cells stop participating). */
@@ -347,7 +344,7 @@ int hfv_get_launch_shape(hfv_ctx* ctx, int* flux_faces_per_thread, double* ns_pe
 /* ---------------------------------------------------------------- measurement */
 /* Device time of the step's kernels, from HIP events recorded on the context's stream around each
  * launch (no counterpart in the reference, which has no device).  Slots: 0 mask+predict, 1 mesh
- * rebuild (1D and 2D moving meshes), 2 face flux, 3 accumulate+c2p, 4 gradients+timestep, 5 limiter
+ * rebuild (moving meshes), 2 face flux, 3 accumulate+c2p, 4 gradients+timestep, 5 limiter
  * wakeup, 6 limiter+kick1, 7 drift. */
 #define HFV_NUM_KERNEL_SLOTS 8
 typedef struct {

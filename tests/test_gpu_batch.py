@@ -80,20 +80,6 @@ def test_run_chunking_does_not_change_results(precision):
             assert np.array_equal(ga[k], gb[k]), (p.name, k)
 
 
-def test_engine_loops_refuse_3d_moving_meshes():
-    """The device engine loops rebuild 1D and 2D meshes only: a 3D simulation with moving generators is
-    refused (HFV_ERR_UNSUPPORTED) by hfv_run and by hfv_batch_load rather than stepped on a stale mesh."""
-    p = ics.kelvin_helmholtz_3d(side=6, t_end=0.1)
-    moving = ics.Problem(p.name, _with_motion(p.cfg, fvs.FLUID), p.loc, p.mass, p.vel, p.u_int, p.mesh)
-    d = ics.build_device_sim(moving)
-    with pytest.raises(RuntimeError, match="moving meshes"):
-        d.run(1)
-    with pytest.raises(RuntimeError, match="moving meshes"):
-        fvs.Batch.for_problems([moving])
-    # fixed generators run
-    assert fvs.Batch.for_problems([p]).run(2) == 2
-
-
 def test_batch_2d_moving_member_equals_alone():
     """A 2D moving-mesh simulation in a batch (beside fixed 1D/2D ones and a moving 1D one) is rebuilt on
     the device in its slot and ends bit for bit where it ends alone."""
