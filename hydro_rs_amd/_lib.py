@@ -9,7 +9,8 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_DIR = os.path.join(_HERE, "lib")
 HFV_SO = os.path.join(LIB_DIR, "libhfv.so")
-HOST_SO = os.path.join(LIB_DIR, "libhfv_host.so")
+# HFV_HOST_LIB: another build of the host tessellator (e.g. one instrumented with sanitizers)
+HOST_SO = os.environ.get("HFV_HOST_LIB") or os.path.join(LIB_DIR, "libhfv_host.so")
 
 I64 = C.c_int64
 U64 = C.c_uint64

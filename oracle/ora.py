@@ -11,7 +11,8 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+# ORA_LIB: a build of the same sources elsewhere (the sanitizer test's instrumented copy)
+_LIB_PATH = os.environ.get("ORA_LIB") or os.path.join(_HERE, "_build", "liboracle.so")
 
 EXACT, HLLC, PVRS, AIRS, TSRS, TRRS, LINADV = range(7)
 VACUUM_SAMPLER = 100
@@ -27,6 +28,8 @@ I8P = C.POINTER(C.c_int8)
 
 
 def build(force=False):
+    if os.environ.get("ORA_LIB"):
+        return _LIB_PATH
     if force or not os.path.exists(_LIB_PATH):
         subprocess.check_call(["make", "-s", "-C", _HERE])
     return _LIB_PATH
