@@ -1072,8 +1072,7 @@ void enqueue_rebuild(hfv_ctx* c, const Batch& B) {
     hipLaunchKernelGGL(k_v2_cells<true>, dim3((unsigned)(c->ncg * HFV_V2_SUB)), gb, 0, st, B, V);
     if (c->any_vmesh3) {
       const dim3 g3((unsigned)(c->ncg * BLOCK)), w3(64);
-      hipLaunchKernelGGL(k_v3_cells<false>, g3, w3, 0, st, B, V);
-      hipLaunchKernelGGL(k_v3_cells<true>, g3, w3, 0, st, B, V);
+      hipLaunchKernelGGL(k_v3_cells, g3, w3, 0, st, B, V);
     }
     hipLaunchKernelGGL(k_v2_scan_up<1>, gc, gb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_scan_mid<1>, gs, gsb, 0, st, B, V);

@@ -11,11 +11,13 @@
 // order and appends the new ones.  The faces are the planes' vertex fans chained into polygons; area
 // vector, centroid and the cell's volume and centroid are the host's triangle-fan sums in its order.
 //
-// One wavefront (a block of its own) builds one cell: the polyhedron,
-// the removed list, the new vertices and the planes' fans and polygons live in LDS, each lane handling
-// every 64th vertex, edge or plane; the next candidate is a wave minimum.  k_v3_cells<false> keeps each
-// lane's candidates in registers (at most HFV_V3_TC); cells that need more go to k_v3_cells<true>, which
-// rescans the bins for every candidate.
+// One wavefront (a block of its own) builds one cell, everything in LDS: the block's candidates are
+// gathered once per K and ranked by their key (each lane ranks every 64th candidate against all), then
+// taken in rank order; a clip tests two vertices per lane, lists the removed ones with two ballots, makes
+// the new vertices one boundary edge per lane and compacts the kept ones through registers; the largest
+// vertex radius is a DPP reduction.  The finish builds each plane's fan and polygon on its own lane,
+// with the face sums and the cell's tetrahedron terms, which one lane then adds in the host's order.
+// Blocks of more than HFV_V3_CAP candidates take the next one as a wave minimum over a rescan of the bins.
 #pragma once
 
 namespace hfv {
@@ -23,22 +25,46 @@ namespace hfv {
 #define HFV_V3_MAXP 64   // planes of a cell: 6 walls and the bisectors that cut it
 #define HFV_V3_MAXT 128  // vertices (dual triangles)
 #define HFV_V3_FAN 24    // vertices of one face
-#define HFV_V3_TC 4
+#define HFV_V3_CAP 192   // candidates held in LDS (a 5^3-bin block at about one generator per bin)
+#define HFV_V3_OWN ((HFV_V3_CAP + 63) / 64)
+#define HFV_V3_NTRI 192  // fan triangles of a cell (2 V - 4: up to 98 vertices)
+static_assert(HFV_V3_FAN % 8 == 0, "fans are searched 8 entries a load");
+
+struct __align__(16) V3Key {
+  double d2;
+  int32_t j, k;
+};
+
+// while clipping: the clip scratch (the removed vertices' indices and plane ids, the new vertices) and the
+// block's candidates
+struct V3Build {
+  uint8_t rem[HFV_V3_MAXT];
+  uint32_t remp[HFV_V3_MAXT];
+  double sx[HFV_V3_MAXT], sy[HFV_V3_MAXT], sz[HFV_V3_MAXT];
+  uint32_t sp[HFV_V3_MAXT];
+  V3Key key[HFV_V3_CAP];
+  double rx[HFV_V3_CAP], ry[HFV_V3_CAP], rz[HFV_V3_CAP];
+  uint16_t ord[HFV_V3_CAP];  // candidate of each rank
+};
+
+// at the finish
+struct V3Fin {
+  // per plane its fan (u, v, vertex) in vertex order, then its polygon
+  uint8_t fu[HFV_V3_MAXP][HFV_V3_FAN], fv[HFV_V3_MAXP][HFV_V3_FAN], ft[HFV_V3_MAXP][HFV_V3_FAN];
+  uint8_t poly[HFV_V3_MAXP][HFV_V3_FAN];
+  // the cell's tetrahedron terms (volume, centroid numerators) in plane and fan order
+  double qv[HFV_V3_NTRI], qx[HFV_V3_NTRI], qy[HFV_V3_NTRI], qz[HFV_V3_NTRI];
+};
 
 struct V3Cell {
   double px[HFV_V3_MAXP], py[HFV_V3_MAXP], pz[HFV_V3_MAXP], pd[HFV_V3_MAXP];
   int32_t pnb[HFV_V3_MAXP], pcode[HFV_V3_MAXP];
   double tx[HFV_V3_MAXT], ty[HFV_V3_MAXT], tz[HFV_V3_MAXT];
-  uint8_t ta[HFV_V3_MAXT], tb[HFV_V3_MAXT], tc[HFV_V3_MAXT];
-  // clip scratch: the removed vertices, the new vertices
-  uint8_t rem[HFV_V3_MAXT];
-  double sx[HFV_V3_MAXT], sy[HFV_V3_MAXT], sz[HFV_V3_MAXT];
-  uint8_t sa[HFV_V3_MAXT], sb[HFV_V3_MAXT];
-  // finish: per plane its fan (u, v, vertex) in vertex order, then its polygon (vertex ids)
-  uint8_t fu[HFV_V3_MAXP][HFV_V3_FAN], fv[HFV_V3_MAXP][HFV_V3_FAN], ft[HFV_V3_MAXP][HFV_V3_FAN];
-  uint8_t fn[HFV_V3_MAXP], pn[HFV_V3_MAXP];
-  uint8_t poly[HFV_V3_MAXP][HFV_V3_FAN];
-  double farea[HFV_V3_MAXP], fcx[HFV_V3_MAXP], fcy[HFV_V3_MAXP], fcz[HFV_V3_MAXP];
+  uint32_t tp[HFV_V3_MAXT];  // plane ids a | b << 8 | c << 16
+  union {
+    V3Build b;
+    V3Fin f;
+  } u;
 };
 
 __device__ __forceinline__ void wave_sync() {
@@ -52,7 +78,7 @@ __device__ __forceinline__ double v3dot(double ax, double ay, double az, double 
   return ax * bx + ay * by + az * bz;
 }
 
-// intersect3 of planes a, b, c: false where they meet in no point
+// intersect3 of planes a, b and (cn, cd): false where they meet in no point
 __device__ __forceinline__ bool v3_intersect(const V3Cell& C, int a, int b, double cnx, double cny, double cnz,
                                              double cd, double& ox, double& oy, double& oz) {
   const double anx = C.px[a], any = C.py[a], anz = C.pz[a], ad = C.pd[a];
@@ -71,28 +97,47 @@ __device__ __forceinline__ bool v3_intersect(const V3Cell& C, int a, int b, doub
   return true;
 }
 
-__device__ __forceinline__ double wave_max(double r) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) {
-    const double q = __shfl_xor(r, o, 64);
-    r = r < q ? q : r;
-  }
-  return r;
+template <int CTRL, int ROW, int BANK>
+__device__ __forceinline__ double dpp_max_step(double v) {
+  const int lo = __double2loint(v), hi = __double2hiint(v);
+  const double t = __hiloint2double(__builtin_amdgcn_update_dpp(hi, hi, CTRL, ROW, BANK, false),
+                                    __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROW, BANK, false));
+  return v < t ? t : v;
 }
 
+// the maximum over the wavefront (no NaN): within rows by row_shr 1, 2, 4, 8, across rows by
+// row_bcast 15 and 31, which leave it in lane 63
+__device__ __forceinline__ double wave_max(double v) {
+  v = dpp_max_step<0x111, 0xf, 0xf>(v);
+  v = dpp_max_step<0x112, 0xf, 0xf>(v);
+  v = dpp_max_step<0x114, 0xf, 0xf>(v);
+  v = dpp_max_step<0x118, 0xf, 0xf>(v);
+  v = dpp_max_step<0x142, 0xa, 0xf>(v);
+  v = dpp_max_step<0x143, 0xc, 0xf>(v);
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), 63),
+                          __builtin_amdgcn_readlane(__double2loint(v), 63));
+}
+
+// Cell3::rmax2
 __device__ __forceinline__ double v3_rmax2(const V3Cell& C, int nt) {
   const int lane = __lane_id();
   double r = 0.;
-  for (int t = lane; t < nt; t += 64) {
-    const double q = v3dot(C.tx[t], C.ty[t], C.tz[t], C.tx[t], C.ty[t], C.tz[t]);
-    r = r < q ? q : r;
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int t = lane + 64 * h;
+    if (t < nt) {
+      const double q = v3dot(C.tx[t], C.ty[t], C.tz[t], C.tx[t], C.ty[t], C.tz[t]);
+      r = r < q ? q : r;
+    }
   }
   return wave_max(r);
 }
 
 // Cell3::clip by plane (n, d, nb, code): 0 no cut, 1 cut, -1 more than the LDS bounds hold.
-__device__ int v3_clip(V3Cell& C, int& np, int& nt, double nx, double ny, double nz, double d, int32_t nb, int32_t code) {
+__device__ int v3_clip(V3Cell& C, int& np, int& nt, double nx, double ny, double nz, double d, int32_t nb,
+                       int32_t code) {
   const int lane = __lane_id();
+  V3Build& S = C.u.b;
   // the removed vertices (ascending), as a 128-bit mask
   const bool o0 = lane < nt && v3dot(nx, ny, nz, C.tx[lane], C.ty[lane], C.tz[lane]) > d;
   const bool o1 = lane + 64 < nt && v3dot(nx, ny, nz, C.tx[lane + 64], C.ty[lane + 64], C.tz[lane + 64]) > d;
@@ -101,8 +146,15 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double nx, double ny, double
   if (nrem == 0 || nrem == nt) return 0;
   if (np + 1 > HFV_V3_MAXP) return -1;
   const uint64_t below = (1ull << lane) - 1ull;
-  if (o0) C.rem[__popcll(m0 & below)] = (uint8_t)lane;
-  if (o1) C.rem[__popcll(m0) + __popcll(m1 & below)] = (uint8_t)(lane + 64);
+  const int r0 = __popcll(m0 & below), r1 = __popcll(m0) + __popcll(m1 & below);
+  if (o0) {
+    S.rem[r0] = (uint8_t)lane;
+    S.remp[r0] = C.tp[lane];
+  }
+  if (o1) {
+    S.rem[r1] = (uint8_t)(lane + 64);
+    S.remp[r1] = C.tp[lane + 64];
+  }
   wave_sync();
   const int pid = np;
   // boundary edges of the removed region, in (removed vertex, edge) order: one new vertex each
@@ -113,19 +165,27 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double nx, double ny, double
     int x = 0, y = 0;
     double ox = 0., oy = 0., oz = 0.;
     if (q < 3 * nrem) {
-      const int ri = C.rem[q / 3], e = q % 3;
-      const int p0 = C.ta[ri], p1 = C.tb[ri], p2 = C.tc[ri];
+      const int rq = q / 3, e = q - 3 * rq;
+      const uint32_t w = S.remp[rq];
+      const int p0 = w & 0xFF, p1 = (w >> 8) & 0xFF, p2 = w >> 16;
       x = e == 0 ? p0 : e == 1 ? p1 : p2;
       y = e == 0 ? p1 : e == 1 ? p2 : p0;
+      // internal: another removed vertex has the edge (y, x) (loads batched, no early exit)
       bool internal = false;
-      for (int k = 0; k < nrem && !internal; ++k) {
-        const int u = C.rem[k];
-        const int u0 = C.ta[u], u1 = C.tb[u], u2 = C.tc[u];
-        internal = (u0 == y && u1 == x) || (u1 == y && u2 == x) || (u2 == y && u0 == x);
+      for (int k = 0; k < nrem; k += 4) {
+        uint32_t u[4];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) u[v] = k + v < nrem ? S.remp[k + v] : 0xFFFFFFFFu;  // matches no plane id
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          const int u0 = u[v] & 0xFF, u1 = (u[v] >> 8) & 0xFF, u2 = u[v] >> 16;
+          internal = internal | ((u0 == y) & (u1 == x)) | ((u1 == y) & (u2 == x)) | ((u2 == y) & (u0 == x));
+        }
       }
       if (!internal) {
         bnd = true;
         if (!v3_intersect(C, x, y, nx, ny, nz, d, ox, oy, oz)) {
+          const int ri = S.rem[rq];
           ox = C.tx[ri];
           oy = C.ty[ri];
           oz = C.tz[ri];
@@ -136,11 +196,10 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double nx, double ny, double
     if (bnd) {
       const int slot = nnew + __popcll(mb & below);
       if (slot < HFV_V3_MAXT) {
-        C.sx[slot] = ox;
-        C.sy[slot] = oy;
-        C.sz[slot] = oz;
-        C.sa[slot] = (uint8_t)x;
-        C.sb[slot] = (uint8_t)y;
+        S.sx[slot] = ox;
+        S.sy[slot] = oy;
+        S.sz[slot] = oz;
+        S.sp[slot] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)pid << 16);
       }
     }
     nnew += __popcll(mb);
@@ -148,43 +207,42 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double nx, double ny, double
   const int nkeep = nt - nrem;
   if (nkeep + nnew > HFV_V3_MAXT) return -1;
   // compact the kept vertices in order (staged through registers), then append the new ones
-  double kx[2] = {0., 0.}, ky[2] = {0., 0.}, kz[2] = {0., 0.};
-  int ka[2] = {0, 0}, kb[2] = {0, 0}, kc[2] = {0, 0}, kd[2] = {-1, -1};
-#pragma unroll
-  for (int h = 0; h < 2; ++h) {
-    const int t = lane + 64 * h;
-    const bool out = h == 0 ? o0 : o1;
-    if (t < nt && !out) {
-      const int before = h == 0 ? __popcll(m0 & below) : __popcll(m0) + __popcll(m1 & below);
-      kd[h] = t - before;
-      kx[h] = C.tx[t];
-      ky[h] = C.ty[t];
-      kz[h] = C.tz[t];
-      ka[h] = C.ta[t];
-      kb[h] = C.tb[t];
-      kc[h] = C.tc[t];
-    }
+  double kx0 = 0., ky0 = 0., kz0 = 0., kx1 = 0., ky1 = 0., kz1 = 0.;
+  uint32_t kp0 = 0, kp1 = 0;
+  const bool keep0 = lane < nt && !o0, keep1 = lane + 64 < nt && !o1;
+  if (keep0) {
+    kx0 = C.tx[lane];
+    ky0 = C.ty[lane];
+    kz0 = C.tz[lane];
+    kp0 = C.tp[lane];
+  }
+  if (keep1) {
+    kx1 = C.tx[lane + 64];
+    ky1 = C.ty[lane + 64];
+    kz1 = C.tz[lane + 64];
+    kp1 = C.tp[lane + 64];
   }
   wave_sync();
-#pragma unroll
-  for (int h = 0; h < 2; ++h)
-    if (kd[h] >= 0) {
-      const int t = kd[h];
-      C.tx[t] = kx[h];
-      C.ty[t] = ky[h];
-      C.tz[t] = kz[h];
-      C.ta[t] = (uint8_t)ka[h];
-      C.tb[t] = (uint8_t)kb[h];
-      C.tc[t] = (uint8_t)kc[h];
-    }
+  if (keep0) {
+    const int t = lane - r0;
+    C.tx[t] = kx0;
+    C.ty[t] = ky0;
+    C.tz[t] = kz0;
+    C.tp[t] = kp0;
+  }
+  if (keep1) {
+    const int t = lane + 64 - r1;
+    C.tx[t] = kx1;
+    C.ty[t] = ky1;
+    C.tz[t] = kz1;
+    C.tp[t] = kp1;
+  }
   for (int k = lane; k < nnew; k += 64) {
     const int t = nkeep + k;
-    C.tx[t] = C.sx[k];
-    C.ty[t] = C.sy[k];
-    C.tz[t] = C.sz[k];
-    C.ta[t] = C.sa[k];
-    C.tb[t] = C.sb[k];
-    C.tc[t] = (uint8_t)pid;
+    C.tx[t] = S.sx[k];
+    C.ty[t] = S.sy[k];
+    C.tz[t] = S.sz[k];
+    C.tp[t] = S.sp[k];
   }
   if (lane == 0) {
     C.px[pid] = nx;
@@ -207,17 +265,19 @@ struct V3Cand {
 
 __device__ __forceinline__ bool v3_less(const V3Cand& a, const V3Cand& b) { return v2_less(a.d2, a.j, a.k, b.d2, b.j, b.k); }
 
-// this lane's candidates of the (2K+1)^3 block: f(cand) for each
+__device__ __forceinline__ int32_t v3_code_of(int32_t kk) {
+  const int s0 = (int8_t)((kk >> 16) & 0xFF), s1 = (int8_t)((kk >> 8) & 0xFF), s2 = (int8_t)(kk & 0xFF);
+  return (s0 == 1 ? 1 : s0 == -1 ? 2 : 0) | ((s1 == 1 ? 1 : s1 == -1 ? 2 : 0) << 2) |
+         ((s2 == 1 ? 1 : s2 == -1 ? 2 : 0) << 4);
+}
+
+// this lane's bins of the (2K+1)^3 block (every 64th): f(bin key, shift codes, whether it is the
+// generator's own bin unshifted)
 template <class Fn>
-__device__ __forceinline__ void v3_lane_candidates(const Batch& B, const SimDesc& D, const V2Bufs& V, int s, int32_t i,
-                                                   double xi, double yi, double zi, int K, Fn&& f) {
+__device__ __forceinline__ void v3_lane_bins(const SimDesc& D, double xi, double yi, double zi, int K, Fn&& f) {
   const int lane = __lane_id();
-  const int64_t N = B.C.n, cb = D.cell_base;
-  const double* L = B.C.loc;
   const bool per = D.periodic != 0;
   const int bi[3] = {v2_bin(xi, D.vw[0], D.vnb[0]), v2_bin(yi, D.vw[1], D.vnb[1]), v2_bin(zi, D.vw[2], D.vnb[2])};
-  const int32_t* gst = V.gcnt + cb + s;
-  const int32_t* gord = V.gord + cb;
   const int side = 2 * K + 1;
   for (int bb = lane; bb < side * side * side; bb += 64) {
     const int dd[3] = {bb / (side * side) - K, (bb / side) % side - K, bb % side - K};
@@ -243,25 +303,137 @@ __device__ __forceinline__ void v3_lane_candidates(const Batch& B, const SimDesc
       }
     }
     if (skip) continue;
-    const int key = (b[0] * D.vnb[1] + b[1]) * D.vnb[2] + b[2];
-    const int32_t kk = ((sc[0] & 0xFF) << 16) | ((sc[1] & 0xFF) << 8) | (sc[2] & 0xFF);
-    const int32_t code = (sc[0] == 1 ? 1 : sc[0] == -1 ? 2 : 0) | ((sc[1] == 1 ? 1 : sc[1] == -1 ? 2 : 0) << 2) |
-                         ((sc[2] == 1 ? 1 : sc[2] == -1 ? 2 : 0) << 4);
+    f((b[0] * D.vnb[1] + b[1]) * D.vnb[2] + b[2], sc[0], sc[1], sc[2], dd[0] == 0 && dd[1] == 0 && dd[2] == 0);
+  }
+}
+
+// this lane's candidates of the (2K+1)^3 block: f(cand) for each
+template <class Fn>
+__device__ __forceinline__ void v3_lane_candidates(const Batch& B, const SimDesc& D, const V2Bufs& V, int s, int32_t i,
+                                                   double xi, double yi, double zi, int K, Fn&& f) {
+  const int64_t N = B.C.n, cb = D.cell_base;
+  const double* L = B.C.loc;
+  const int32_t* gst = V.gcnt + cb + s;
+  const int32_t* gord = V.gord + cb;
+  v3_lane_bins(D, xi, yi, zi, K, [&](int key, int s0, int s1, int s2, bool) {
+    const int32_t kk = ((s0 & 0xFF) << 16) | ((s1 & 0xFF) << 8) | (s2 & 0xFF);  // the host's memcmp order
+    const int32_t code = v3_code_of(kk);
     for (int32_t q = gst[key]; q < gst[key + 1]; ++q) {
       const int32_t j = gord[q];
-      if (j == i && sc[0] == 0 && sc[1] == 0 && sc[2] == 0) continue;
+      if (j == i && s0 == 0 && s1 == 0 && s2 == 0) continue;
       const int64_t gj = cb + j;
       V3Cand c;
-      c.rx = (L[gj] + (double)sc[0] * D.cfg.box[0]) - xi;
-      c.ry = (L[N + gj] + (double)sc[1] * D.cfg.box[1]) - yi;
-      c.rz = (L[2 * N + gj] + (double)sc[2] * D.cfg.box[2]) - zi;
+      c.rx = (L[gj] + (double)s0 * D.cfg.box[0]) - xi;
+      c.ry = (L[N + gj] + (double)s1 * D.cfg.box[1]) - yi;
+      c.rz = (L[2 * N + gj] + (double)s2 * D.cfg.box[2]) - zi;
       c.d2 = v3dot(c.rx, c.ry, c.rz, c.rx, c.ry, c.rz);
       c.j = j;
       c.k = kk;
       c.code = code;
       f(c);
     }
+  });
+}
+
+// The block's candidates into LDS, ranked by (squared distance, index, image code): their count, or -1
+// beyond HFV_V3_CAP.
+__device__ int v3_gather(const Batch& B, const SimDesc& D, const V2Bufs& V, int s, int32_t i, double xi, double yi,
+                         double zi, int K, V3Cell& C) {
+  const int lane = __lane_id();
+  const int32_t* gst = V.gcnt + D.cell_base + s;
+  int cnt = 0;
+  v3_lane_bins(D, xi, yi, zi, K, [&](int key, int, int, int, bool own) { cnt += gst[key + 1] - gst[key] - (own ? 1 : 0); });
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
   }
+  const int n = __builtin_amdgcn_readlane(incl, 63);
+  if (n > HFV_V3_CAP) return -1;
+  V3Build& Q = C.u.b;
+  int at = incl - cnt;
+  v3_lane_candidates(B, D, V, s, i, xi, yi, zi, K, [&](const V3Cand& c) {
+    if (at >= n) return;  // (the counts agree: the generator's own entry is the one skipped)
+    V3Key key;
+    key.d2 = c.d2;
+    key.j = c.j;
+    key.k = c.k;
+    Q.key[at] = key;
+    Q.rx[at] = c.rx;
+    Q.ry[at] = c.ry;
+    Q.rz[at] = c.rz;
+    ++at;
+  });
+  wave_sync();
+  // rank of each of this lane's candidates: the number of keys below it (keys are distinct).  A
+  // non-negative double orders as its bits, so keys compare as integers.
+  const int nown = __builtin_amdgcn_readfirstlane((n + 63) >> 6);
+  uint64_t od[HFV_V3_OWN];
+  int32_t oj[HFV_V3_OWN], ok[HFV_V3_OWN], rk[HFV_V3_OWN];
+#pragma unroll
+  for (int h = 0; h < HFV_V3_OWN; ++h) {
+    const int a = lane + 64 * h;
+    const V3Key key = Q.key[a < n ? a : 0];
+    od[h] = (uint64_t)__double_as_longlong(key.d2);
+    oj[h] = key.j;
+    ok[h] = key.k;
+    rk[h] = 0;
+  }
+  // By the squared distance alone first (keys below and keys at most equal; broadcast reads, 8 keys in
+  // flight); the whole key only where two squared distances are equal.
+  int le[HFV_V3_OWN];
+#pragma unroll
+  for (int h = 0; h < HFV_V3_OWN; ++h) le[h] = 0;
+  int b = 0;
+  for (; b + 8 <= n; b += 8) {
+    uint64_t db[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) db[u] = (uint64_t)__double_as_longlong(Q.key[b + u].d2);
+#pragma unroll
+    for (int h = 0; h < HFV_V3_OWN; ++h) {
+      if (h >= nown) break;
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        rk[h] += db[u] < od[h] ? 1 : 0;
+        le[h] += db[u] <= od[h] ? 1 : 0;
+      }
+    }
+  }
+  for (; b < n; ++b) {
+    const uint64_t db = (uint64_t)__double_as_longlong(Q.key[b].d2);
+#pragma unroll
+    for (int h = 0; h < HFV_V3_OWN; ++h) {
+      if (h >= nown) break;
+      rk[h] += db < od[h] ? 1 : 0;
+      le[h] += db <= od[h] ? 1 : 0;
+    }
+  }
+  bool tie = false;
+#pragma unroll
+  for (int h = 0; h < HFV_V3_OWN; ++h) tie = tie | ((h < nown) & (lane + 64 * h < n) & (le[h] - rk[h] > 1));
+  if (__ballot(tie)) {
+#pragma unroll
+    for (int h = 0; h < HFV_V3_OWN; ++h) rk[h] = 0;
+    for (int c = 0; c < n; ++c) {
+      const V3Key kb = Q.key[c];
+      const uint64_t db = (uint64_t)__double_as_longlong(kb.d2);
+#pragma unroll
+      for (int h = 0; h < HFV_V3_OWN; ++h) {
+        if (h >= nown) break;
+        // (bitwise, not short-circuit: no branches)
+        const bool lt = (db < od[h]) | ((db == od[h]) & ((kb.j < oj[h]) | ((kb.j == oj[h]) & (kb.k < ok[h]))));
+        rk[h] += lt ? 1 : 0;
+      }
+    }
+  }
+#pragma unroll
+  for (int h = 0; h < HFV_V3_OWN; ++h) {
+    const int a = lane + 64 * h;
+    if (h < nown && a < n) Q.ord[rk[h]] = (uint16_t)a;
+  }
+  wave_sync();
+  return n;
 }
 
 __device__ __forceinline__ void wave_min_cand(bool& have, V3Cand& b) {
@@ -283,9 +455,8 @@ __device__ __forceinline__ void wave_min_cand(bool& have, V3Cand& b) {
   }
 }
 
-// The cell for candidate bins within K: 0 clipped (finish next), 1 retry with a larger K, 2 too many
-// candidates for the registers (LIST), 3 beyond the LDS bounds.
-template <bool LIST>
+// The cell for candidate bins within K: 0 clipped (finish next), 1 retry with a larger K, 3 beyond the
+// LDS bounds.
 __device__ int v3_build(const Batch& B, const SimDesc& D, const V2Bufs& V, int s, int32_t i, double xi, double yi,
                         double zi, int K, V3Cell& C, int& np, int& nt, double& r2) {
   const int lane = __lane_id();
@@ -315,9 +486,7 @@ __device__ int v3_build(const Batch& B, const SimDesc& D, const V2Bufs& V, int s
                  cz = C.px[b] * C.py[c] - C.py[b] * C.px[c];
     const double det = v3dot(C.px[a], C.py[a], C.pz[a], cx, cy, cz);
     const int t0 = det > 0 ? a : b, t1 = det > 0 ? b : a;
-    C.ta[lane] = (uint8_t)t0;
-    C.tb[lane] = (uint8_t)t1;
-    C.tc[lane] = (uint8_t)c;
+    C.tp[lane] = (uint32_t)t0 | ((uint32_t)t1 << 8) | ((uint32_t)c << 16);
     double ox = 0., oy = 0., oz = 0.;
     v3_intersect(C, t0, t1, C.px[c], C.py[c], C.pz[c], C.pd[c], ox, oy, oz);
     C.tx[lane] = ox;
@@ -328,43 +497,39 @@ __device__ int v3_build(const Batch& B, const SimDesc& D, const V2Bufs& V, int s
   np = 6;
   nt = 8;
   r2 = v3_rmax2(C, nt);
-  V3Cand cl[LIST ? HFV_V3_TC : 1];
-  int cnt = 0;
-  if (LIST) {
-    v3_lane_candidates(B, D, V, s, i, xi, yi, zi, K, [&](const V3Cand& c) {
-#pragma unroll
-      for (int q = 0; q < (LIST ? HFV_V3_TC : 1); ++q)
-        if (q == cnt) cl[q] = c;
-      ++cnt;
-    });
-    if (__ballot(cnt > HFV_V3_TC)) return 2;
-  }
-  bool have_last = false;
-  V3Cand last{};
-  while (true) {
-    bool have = false;
-    V3Cand best{};
-    auto consider = [&](const V3Cand& c) {
-      if ((!have_last || v3_less(last, c)) && (!have || v3_less(c, best))) {
-        have = true;
-        best = c;
-      }
-    };
-    if (LIST) {
-#pragma unroll
-      for (int q = 0; q < (LIST ? HFV_V3_TC : 1); ++q)
-        if (q < cnt) consider(cl[q]);
-    } else {
-      v3_lane_candidates(B, D, V, s, i, xi, yi, zi, K, consider);
+  const int n = v3_gather(B, D, V, s, i, xi, yi, zi, K, C);
+  if (n >= 0) {
+    const V3Build& Q = C.u.b;
+    for (int q = 0; q < n; ++q) {
+      const int a = __builtin_amdgcn_readfirstlane(Q.ord[q]);
+      const V3Key key = Q.key[a];
+      if (0.25 * key.d2 > r2) break;  // security radius: the bisector lies beyond every vertex
+      const int c = v3_clip(C, np, nt, Q.rx[a], Q.ry[a], Q.rz[a], 0.5 * key.d2, key.j, v3_code_of(key.k));
+      if (c < 0) return 3;
+      if (c > 0) r2 = v3_rmax2(C, nt);
     }
-    wave_min_cand(have, best);
-    if (!have || 0.25 * best.d2 > r2) break;  // security radius
-    const int c = v3_clip(C, np, nt, best.rx, best.ry, best.rz, 0.5 * best.d2, best.j, best.code);
-    if (c < 0) return 3;
-    if (c > 0) r2 = v3_rmax2(C, nt);
-    have_last = true;
-    last = best;
+  } else {  // too many candidates for the LDS: the next one is a wave minimum over a rescan
+    bool have_last = false;
+    V3Cand last{};
+    while (true) {
+      bool have = false;
+      V3Cand best{};
+      v3_lane_candidates(B, D, V, s, i, xi, yi, zi, K, [&](const V3Cand& c) {
+        if ((!have_last || v3_less(last, c)) && (!have || v3_less(c, best))) {
+          have = true;
+          best = c;
+        }
+      });
+      wave_min_cand(have, best);
+      if (!have || 0.25 * best.d2 > r2) break;
+      const int c = v3_clip(C, np, nt, best.rx, best.ry, best.rz, 0.5 * best.d2, best.j, best.code);
+      if (c < 0) return 3;
+      if (c > 0) r2 = v3_rmax2(C, nt);
+      have_last = true;
+      last = best;
+    }
   }
+  // completeness: every unvisited generator lies farther than K bin widths
   double minw = 1e300;
   minw = K * w0 < minw ? K * w0 : minw;
   minw = K * w1 < minw ? K * w1 : minw;
@@ -373,150 +538,159 @@ __device__ int v3_build(const Batch& B, const SimDesc& D, const V2Bufs& V, int s
   return 0;
 }
 
-// Cell3::finish on the clipped cell: per plane its fan chained into a polygon (lane per plane), the face
-// areas and centroids (lane per plane), the cell's volume and centroid summed over the planes and their
-// fans in order (lane 0).  Returns 0, 1 (periodic cell touching the initial region: retry with a larger
-// K) or 3 (a fan beyond the LDS bound); on 0 the faces and the geometry are written.
+// Cell3::finish on the clipped cell.  Returns 0, 1 (a periodic cell touching the initial region: retry
+// with a larger K) or 3 (a face of more than HFV_V3_FAN vertices); on 0 the faces and the geometry are
+// written.
 __device__ int v3_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, SimEngine& E, int s, int64_t gi, int32_t i,
                          double xi, double yi, double zi, V3Cell& C, int np, int nt, double r2) {
   const int lane = __lane_id();
+  const int p = lane;
+  V3Fin& F = C.u.f;  // over the candidates, read for the last time before the completeness test
+  wave_sync();
+  // plane p's fan: its vertices rotated to (p, u, v), in vertex order; chained into a polygon
+  int m = 0;
   bool over = false;
-  for (int p = lane; p < np; p += 64) {
+  if (p < np) {
     int f = 0;
-    for (int t = 0; t < nt; ++t) {
-      const int a = C.ta[t], b = C.tb[t], c = C.tc[t];
-      int u = -1, v = -1;
-      if (a == p) {
-        u = b;
-        v = c;
-      } else if (b == p) {
-        u = c;
-        v = a;
-      } else if (c == p) {
-        u = a;
-        v = b;
+    for (int t0 = 0; t0 < nt; t0 += 8) {  // (8 loads in flight)
+      uint32_t w[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) w[e] = t0 + e < nt ? C.tp[t0 + e] : 0xFFFFFFFFu;  // matches no plane id
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const int a = w[e] & 0xFF, b = (w[e] >> 8) & 0xFF, c = w[e] >> 16;
+        const bool ia = a == p, ib = b == p, ic = c == p;
+        const int u = ia ? b : ib ? c : ic ? a : -1, v = ia ? c : ib ? a : b;
+        if (u >= 0) {
+          if (f < HFV_V3_FAN) {
+            F.fu[p][f] = (uint8_t)u;
+            F.fv[p][f] = (uint8_t)v;
+            F.ft[p][f] = (uint8_t)(t0 + e);
+          }
+          ++f;
+        }
       }
-      if (u < 0) continue;
-      if (f < HFV_V3_FAN) {
-        C.fu[p][f] = (uint8_t)u;
-        C.fv[p][f] = (uint8_t)v;
-        C.ft[p][f] = (uint8_t)t;
-      }
-      ++f;
     }
-    if (f > HFV_V3_FAN) over = true;
-    C.fn[p] = (uint8_t)(f > HFV_V3_FAN ? 0 : f);
-    // chain: from (u, v) the next fan entry around p is the first one whose u is this v
-    int m = 0;
-    if (f >= 3 && f <= HFV_V3_FAN) {
+    over = f > HFV_V3_FAN;
+    if (f >= 3 && !over) {
+      // from (u, v) the next entry around p is the first one whose u is this v (8 entries a load)
+      const uint2* fu8 = reinterpret_cast<const uint2*>(&F.fu[p][0]);
       int cur = 0;
       for (int step = 0; step < f; ++step) {
-        C.poly[p][m++] = C.ft[p][cur];
-        const int want = C.fv[p][cur];
+        F.poly[p][m++] = F.ft[p][cur];
+        const uint32_t want = F.fv[p][cur];
         int nxt = -1;
-        for (int q = 0; q < f; ++q)
-          if (C.fu[p][q] == want) {
-            nxt = q;
-            break;
+        for (int q0 = 0; q0 < f; q0 += 8) {
+          const uint2 w = fu8[q0 >> 3];
+#pragma unroll
+          for (int e = 0; e < 8; ++e) {
+            const uint32_t u = ((e < 4 ? w.x : w.y) >> (8 * (e & 3))) & 0xFFu;
+            nxt = ((nxt < 0) & (q0 + e < f) & (u == want)) ? q0 + e : nxt;
           }
+        }
         if (nxt < 0) break;
         cur = nxt;
         if (cur == 0) break;
       }
     }
-    C.pn[p] = (uint8_t)(m >= 3 ? m : 0);
-    // the face: area vector and centroid of its triangle fan
-    double area = 0., cx = 0., cy = 0., cz = 0.;
-    if (m >= 3) {
-      const int t0 = C.poly[p][0];
-      const double p0x = C.tx[t0], p0y = C.ty[t0], p0z = C.tz[t0];
-      double avx = 0., avy = 0., avz = 0., acx = 0., acy = 0., acz = 0., asum = 0.;
-      for (int k = 1; k + 1 < m; ++k) {
-        const int ta_ = C.poly[p][k], tb_ = C.poly[p][k + 1];
-        const double ax = C.tx[ta_] - p0x, ay = C.ty[ta_] - p0y, az = C.tz[ta_] - p0z;
-        const double bx = C.tx[tb_] - p0x, by = C.ty[tb_] - p0y, bz = C.tz[tb_] - p0z;
-        const double crx = ay * bz - az * by, cry = az * bx - ax * bz, crz = ax * by - ay * bx;
-        const double a = 0.5 * sqrt(v3dot(crx, cry, crz, crx, cry, crz));
-        avx = avx + 0.5 * crx;
-        avy = avy + 0.5 * cry;
-        avz = avz + 0.5 * crz;
-        asum += a;
-        const double s3 = a / 3.0;
-        acx = acx + s3 * ((p0x + C.tx[ta_]) + C.tx[tb_]);
-        acy = acy + s3 * ((p0y + C.ty[ta_]) + C.ty[tb_]);
-        acz = acz + s3 * ((p0z + C.tz[ta_]) + C.tz[tb_]);
-      }
-      area = sqrt(v3dot(avx, avy, avz, avx, avy, avz));
-      if (asum > 0) {
-        const double inv = 1.0 / asum;
-        cx = inv * acx;
-        cy = inv * acy;
-        cz = inv * acz;
-      } else {
-        cx = p0x;
-        cy = p0y;
-        cz = p0z;
-      }
-    }
-    C.farea[p] = area;
-    C.fcx[p] = cx;
-    C.fcy[p] = cy;
-    C.fcz[p] = cz;
+    if (m < 3) m = 0;
   }
   if (__ballot(over)) return 3;
-  wave_sync();
-  // a periodic cell must not touch the initial region (a wall face of positive area)
-  bool wall = false;
-  for (int p = lane; p < np; p += 64) wall = wall || (C.pnb[p] < 0 && C.farea[p] > 0.);
-  if (D.periodic && __ballot(wall)) return 1;
-  if (lane != 0) return 0;
-  // lane 0: the cell's volume and centroid over the planes and their fans in order, then the faces
-  double vol = 0., cenx = 0., ceny = 0., cenz = 0.;
-  int cnt = 0;
-  for (int p = 0; p < np; ++p) {
-    const int m = C.pn[p];
-    if (m < 3) continue;
-    const int t0 = C.poly[p][0];
-    const double p0x = C.tx[t0], p0y = C.ty[t0], p0z = C.tz[t0];
-    for (int k = 1; k + 1 < m; ++k) {
-      const int ta_ = C.poly[p][k], tb_ = C.poly[p][k + 1];
-      const double ax = C.tx[ta_], ay = C.ty[ta_], az = C.tz[ta_], bx = C.tx[tb_], by = C.ty[tb_], bz = C.tz[tb_];
-      const double crx = ay * bz - az * by, cry = az * bx - ax * bz, crz = ax * by - ay * bx;
-      const double tv = fabs(v3dot(p0x, p0y, p0z, crx, cry, crz)) / 6.0;
-      vol += tv;
-      const double s4 = tv / 4.0;
-      cenx = cenx + s4 * ((p0x + ax) + bx);
-      ceny = ceny + s4 * ((p0y + ay) + by);
-      cenz = cenz + s4 * ((p0z + az) + bz);
-    }
-    if (C.farea[p] > 0. && v2_emits(B, D.cell_base, C.pnb[p], i)) ++cnt;
+  // the plane's fan triangles: their place among the cell's (planes in order), by a wave prefix sum
+  const int nq = m >= 3 ? m - 2 : 0;
+  int incl = nq;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
   }
+  const int ntri = __builtin_amdgcn_readlane(incl, 63);
+  if (ntri > HFV_V3_NTRI) return 3;  // (2 V - 4 on a closed cell)
+  const int q0 = incl - nq;
+  // the face's area vector and centroid; the cell's tetrahedron terms (vertex fan with the generator)
+  double area = 0., fcx = 0., fcy = 0., fcz = 0.;
+  if (m >= 3) {
+    const int t0 = F.poly[p][0];
+    const double p0x = C.tx[t0], p0y = C.ty[t0], p0z = C.tz[t0];
+    double avx = 0., avy = 0., avz = 0., acx = 0., acy = 0., acz = 0., asum = 0.;
+    for (int k = 1; k + 1 < m; ++k) {
+      const int ta = F.poly[p][k], tb = F.poly[p][k + 1];
+      const double Ax = C.tx[ta], Ay = C.ty[ta], Az = C.tz[ta], Bx = C.tx[tb], By = C.ty[tb], Bz = C.tz[tb];
+      const double ax = Ax - p0x, ay = Ay - p0y, az = Az - p0z, bx = Bx - p0x, by = By - p0y, bz = Bz - p0z;
+      const double crx = ay * bz - az * by, cry = az * bx - ax * bz, crz = ax * by - ay * bx;
+      const double a = 0.5 * sqrt(v3dot(crx, cry, crz, crx, cry, crz));
+      avx = avx + 0.5 * crx;
+      avy = avy + 0.5 * cry;
+      avz = avz + 0.5 * crz;
+      asum += a;
+      const double s3 = a / 3.0;
+      acx = acx + s3 * ((p0x + Ax) + Bx);
+      acy = acy + s3 * ((p0y + Ay) + By);
+      acz = acz + s3 * ((p0z + Az) + Bz);
+      const double ux = Ay * Bz - Az * By, uy = Az * Bx - Ax * Bz, uz = Ax * By - Ay * Bx;
+      const double tv = fabs(v3dot(p0x, p0y, p0z, ux, uy, uz)) / 6.0;
+      const double s4 = tv / 4.0;
+      const int qi = q0 + k - 1;
+      F.qv[qi] = tv;
+      F.qx[qi] = s4 * ((p0x + Ax) + Bx);
+      F.qy[qi] = s4 * ((p0y + Ay) + By);
+      F.qz[qi] = s4 * ((p0z + Az) + Bz);
+    }
+    area = sqrt(v3dot(avx, avy, avz, avx, avy, avz));
+    if (asum > 0) {
+      const double inv = 1.0 / asum;
+      fcx = inv * acx;
+      fcy = inv * acy;
+      fcz = inv * acz;
+    } else {
+      fcx = p0x;
+      fcy = p0y;
+      fcz = p0z;
+    }
+  }
+  const int32_t pnb = p < np ? C.pnb[p] : 0;
+  // a periodic cell must not touch the initial region (a wall face of positive area)
+  if (D.periodic && __ballot(p < np && pnb < 0 && area > 0.)) return 1;
   const int64_t cb = D.cell_base, N = B.C.n;
-  const int32_t start = atomicAdd(&E.nf_tmp, cnt);
+  const bool emit = p < np && area > 0. && v2_emits(B, cb, pnb, i);
+  const uint64_t me = __ballot(emit);
+  const int cnt = __popcll(me);
+  int start = 0;
+  if (lane == 0) start = atomicAdd(&E.nf_tmp, cnt);
+  start = __shfl(start, 0, 64);
   if ((int64_t)start + cnt > D.face_cap) {
-    V.femit[cb + s + i] = 0;
-    atomicCAS(&B.cnt->err, 0, E_MESH);
-    atomicCAS(&E.err, 0, E_MESH);
+    if (lane == 0) {
+      V.femit[cb + s + i] = 0;
+      atomicCAS(&B.cnt->err, 0, E_MESH);
+      atomicCAS(&E.err, 0, E_MESH);
+    }
     return 0;
   }
-  V.vst[gi] = start;
-  V.femit[cb + s + i] = cnt;
-  V2Face* out = V.vf + D.face_base + start;
-  int q = 0;
-  for (int p = 0; p < np; ++p) {
-    if (!(C.farea[p] > 0.) || !v2_emits(B, cb, C.pnb[p], i)) continue;
+  if (emit) {
     const double inv = 1.0 / sqrt(v3dot(C.px[p], C.py[p], C.pz[p], C.px[p], C.py[p], C.pz[p]));
     V2Face f;
-    f.area = C.farea[p];
-    f.cx = C.fcx[p];
-    f.cy = C.fcy[p];
-    f.cz = C.fcz[p];
+    f.area = area;
+    f.cx = fcx;
+    f.cy = fcy;
+    f.cz = fcz;
     f.nx = inv * C.px[p];
     f.ny = inv * C.py[p];
     f.nz = inv * C.pz[p];
-    f.nb = C.pnb[p] < 0 ? -1 : C.pnb[p];
+    f.nb = pnb < 0 ? -1 : pnb;
     f.code = C.pcode[p];
-    out[q++] = f;
+    V.vf[D.face_base + start + __popcll(me & ((1ull << lane) - 1ull))] = f;
+  }
+  wave_sync();
+  if (lane != 0) return 0;
+  V.vst[gi] = start;
+  V.femit[cb + s + i] = cnt;
+  // the cell's volume and centroid: the tetrahedron terms added in the host's order
+  double vol = 0., cenx = 0., ceny = 0., cenz = 0.;
+  for (int q = 0; q < ntri; ++q) {
+    vol += F.qv[q];
+    cenx = cenx + F.qx[q];
+    ceny = ceny + F.qy[q];
+    cenz = cenz + F.qz[q];
   }
   if (B.C.act[gi]) {  // Particle::update_geometry (part.rs:151-158)
     B.C.vol[gi] = vol;
@@ -529,9 +703,7 @@ __device__ int v3_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, SimE
   return 0;
 }
 
-// grid: HFV_BLOCK one-wavefront blocks per cell chunk, one cell each.  WIDE: only the cells that the first
-// pass marked.
-template <bool WIDE>
+// grid: HFV_BLOCK one-wavefront blocks per cell chunk, one cell each
 __global__ void __launch_bounds__(64) k_v3_cells(Batch B, V2Bufs V) {
   __shared__ V3Cell C;
   const BlockJob J = B.cjobs[blockIdx.x / HFV_BLOCK];
@@ -545,19 +717,15 @@ __global__ void __launch_bounds__(64) k_v3_cells(Batch B, V2Bufs V) {
   if (!B.C.act[gi]) return;  // unbuilt in a partial build (femit stays 0)
   const int64_t N = B.C.n, cb = D.cell_base;
   const int32_t i = (int32_t)(gi - cb);
-  if (WIDE && V.femit[cb + s + i] != HFV_V2_WIDE) return;
   const double xi = B.C.loc[gi], yi = B.C.loc[N + gi], zi = B.C.loc[2 * N + gi];
   const int kmax = max(6, max(D.vnb[0], max(D.vnb[1], D.vnb[2])));
   int np = 0, nt = 0, r = 1;
   double r2 = 0.;
   for (int K = 2; K <= kmax && r == 1; ++K) {
-    r = v3_build<!WIDE>(B, D, V, s, i, xi, yi, zi, K, C, np, nt, r2);
+    r = v3_build(B, D, V, s, i, xi, yi, zi, K, C, np, nt, r2);
     if (r == 0) r = v3_finish(B, V, D, E, s, gi, i, xi, yi, zi, C, np, nt, r2);
   }
-  if (__lane_id() != 0) return;
-  if (r == 2 && !WIDE) {
-    V.femit[cb + s + i] = HFV_V2_WIDE;
-  } else if (r != 0) {
+  if (r != 0 && __lane_id() == 0) {
     V.femit[cb + s + i] = 0;
     atomicCAS(&B.cnt->err, 0, E_MESH);
     atomicCAS(&E.err, 0, E_MESH);
