@@ -13,9 +13,9 @@
 //
 // One wavefront (a block of its own) builds one cell, everything in LDS: the block's candidates are
 // gathered once per K and ranked by their key (each lane ranks every 64th candidate against all), then
-// taken in rank order; a clip tests two vertices per lane, lists the removed ones with two ballots, makes
-// the new vertices one boundary edge per lane and compacts the kept ones through registers; the largest
-// vertex radius is a DPP reduction.  The finish builds each plane's fan and polygon on its own lane,
+// taken in rank order; a clip reads each vertex once (two per lane), lists the removed ones with two
+// ballots, compacts the kept ones from registers and makes the new ones one boundary edge per lane; the
+// largest vertex radius is a DPP reduction over the lanes' kept and new vertices.  The finish builds each plane's fan and polygon on its own lane,
 // with the face sums and the cell's tetrahedron terms, which one lane then adds in the host's order.
 // Blocks of more than HFV_V3_CAP candidates take the next one as a wave minimum over a rescan of the bins.
 #pragma once
@@ -35,13 +35,11 @@ struct __align__(16) V3Key {
   int32_t j, k;
 };
 
-// while clipping: the clip scratch (the removed vertices' indices and plane ids, the new vertices) and the
-// block's candidates
+// while clipping: the clip scratch (the removed vertices' plane ids and positions) and the block's
+// candidates
 struct V3Build {
-  uint8_t rem[HFV_V3_MAXT];
   uint32_t remp[HFV_V3_MAXT];
   double sx[HFV_V3_MAXT], sy[HFV_V3_MAXT], sz[HFV_V3_MAXT];
-  uint32_t sp[HFV_V3_MAXT];
   V3Key key[HFV_V3_CAP];
   double rx[HFV_V3_CAP], ry[HFV_V3_CAP], rz[HFV_V3_CAP];
   uint16_t ord[HFV_V3_CAP];  // candidate of each rank
@@ -133,14 +131,31 @@ __device__ __forceinline__ double v3_rmax2(const V3Cell& C, int nt) {
   return wave_max(r);
 }
 
-// Cell3::clip by plane (n, d, nb, code): 0 no cut, 1 cut, -1 more than the LDS bounds hold.
-__device__ int v3_clip(V3Cell& C, int& np, int& nt, double nx, double ny, double nz, double d, int32_t nb,
-                       int32_t code) {
+// Cell3::clip by plane (n, d, nb, code): 0 no cut, 1 cut (r2 the new Cell3::rmax2), -1 more than the LDS
+// bounds hold.  Every vertex is read once, at the start: the kept ones are compacted from registers, the
+// removed ones' plane ids and positions go to the scratch, and the new ones are written in place.
+__device__ int v3_clip(V3Cell& C, int& np, int& nt, double& r2, double nx, double ny, double nz, double d,
+                       int32_t nb, int32_t code) {
   const int lane = __lane_id();
   V3Build& S = C.u.b;
-  // the removed vertices (ascending), as a 128-bit mask
-  const bool o0 = lane < nt && v3dot(nx, ny, nz, C.tx[lane], C.ty[lane], C.tz[lane]) > d;
-  const bool o1 = lane + 64 < nt && v3dot(nx, ny, nz, C.tx[lane + 64], C.ty[lane + 64], C.tz[lane + 64]) > d;
+  // this lane's vertices (lane, lane + 64) and whether the plane removes them
+  const bool h0 = lane < nt, h1 = lane + 64 < nt;
+  double x0 = 0., y0 = 0., z0 = 0., x1 = 0., y1 = 0., z1 = 0.;
+  uint32_t p0 = 0, p1 = 0;
+  if (h0) {
+    x0 = C.tx[lane];
+    y0 = C.ty[lane];
+    z0 = C.tz[lane];
+    p0 = C.tp[lane];
+  }
+  if (h1) {
+    x1 = C.tx[lane + 64];
+    y1 = C.ty[lane + 64];
+    z1 = C.tz[lane + 64];
+    p1 = C.tp[lane + 64];
+  }
+  const bool o0 = h0 && v3dot(nx, ny, nz, x0, y0, z0) > d;
+  const bool o1 = h1 && v3dot(nx, ny, nz, x1, y1, z1) > d;
   const uint64_t m0 = __ballot(o0), m1 = __ballot(o1);
   const int nrem = __popcll(m0) + __popcll(m1);
   if (nrem == 0 || nrem == nt) return 0;
@@ -148,28 +163,53 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double nx, double ny, double
   const uint64_t below = (1ull << lane) - 1ull;
   const int r0 = __popcll(m0 & below), r1 = __popcll(m0) + __popcll(m1 & below);
   if (o0) {
-    S.rem[r0] = (uint8_t)lane;
-    S.remp[r0] = C.tp[lane];
+    S.remp[r0] = p0;
+    S.sx[r0] = x0;
+    S.sy[r0] = y0;
+    S.sz[r0] = z0;
   }
   if (o1) {
-    S.rem[r1] = (uint8_t)(lane + 64);
-    S.remp[r1] = C.tp[lane + 64];
+    S.remp[r1] = p1;
+    S.sx[r1] = x1;
+    S.sy[r1] = y1;
+    S.sz[r1] = z1;
   }
   wave_sync();
+  // the kept vertices, in order
+  const int nkeep = nt - nrem;
+  double q = 0.;  // this lane's largest squared vertex radius
+  if (h0 && !o0) {
+    const int t = lane - r0;
+    C.tx[t] = x0;
+    C.ty[t] = y0;
+    C.tz[t] = z0;
+    C.tp[t] = p0;
+    q = v3dot(x0, y0, z0, x0, y0, z0);
+  }
+  if (h1 && !o1) {
+    const int t = lane + 64 - r1;
+    C.tx[t] = x1;
+    C.ty[t] = y1;
+    C.tz[t] = z1;
+    C.tp[t] = p1;
+    const double q1 = v3dot(x1, y1, z1, x1, y1, z1);
+    q = q < q1 ? q1 : q;
+  }
   const int pid = np;
-  // boundary edges of the removed region, in (removed vertex, edge) order: one new vertex each
+  // boundary edges of the removed region, in (removed vertex, edge) order: one new vertex each, after
+  // the kept ones
   int nnew = 0;
   for (int q0 = 0; q0 < 3 * nrem; q0 += 64) {
-    const int q = q0 + lane;
+    const int e3 = q0 + lane;
     bool bnd = false;
     int x = 0, y = 0;
     double ox = 0., oy = 0., oz = 0.;
-    if (q < 3 * nrem) {
-      const int rq = q / 3, e = q - 3 * rq;
+    if (e3 < 3 * nrem) {
+      const int rq = e3 / 3, e = e3 - 3 * rq;
       const uint32_t w = S.remp[rq];
-      const int p0 = w & 0xFF, p1 = (w >> 8) & 0xFF, p2 = w >> 16;
-      x = e == 0 ? p0 : e == 1 ? p1 : p2;
-      y = e == 0 ? p1 : e == 1 ? p2 : p0;
+      const int a0 = w & 0xFF, a1 = (w >> 8) & 0xFF, a2 = w >> 16;
+      x = e == 0 ? a0 : e == 1 ? a1 : a2;
+      y = e == 0 ? a1 : e == 1 ? a2 : a0;
       // internal: another removed vertex has the edge (y, x) (loads batched, no early exit)
       bool internal = false;
       for (int k = 0; k < nrem; k += 4) {
@@ -184,66 +224,26 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double nx, double ny, double
       }
       if (!internal) {
         bnd = true;
-        if (!v3_intersect(C, x, y, nx, ny, nz, d, ox, oy, oz)) {
-          const int ri = S.rem[rq];
-          ox = C.tx[ri];
-          oy = C.ty[ri];
-          oz = C.tz[ri];
+        if (!v3_intersect(C, x, y, nx, ny, nz, d, ox, oy, oz)) {  // the removed vertex's position
+          ox = S.sx[rq];
+          oy = S.sy[rq];
+          oz = S.sz[rq];
         }
       }
     }
     const uint64_t mb = __ballot(bnd);
-    if (bnd) {
-      const int slot = nnew + __popcll(mb & below);
-      if (slot < HFV_V3_MAXT) {
-        S.sx[slot] = ox;
-        S.sy[slot] = oy;
-        S.sz[slot] = oz;
-        S.sp[slot] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)pid << 16);
-      }
+    const int t = nkeep + nnew + __popcll(mb & below);
+    if (bnd && t < HFV_V3_MAXT) {
+      C.tx[t] = ox;
+      C.ty[t] = oy;
+      C.tz[t] = oz;
+      C.tp[t] = (uint32_t)x | ((uint32_t)y << 8) | ((uint32_t)pid << 16);
+      const double qn = v3dot(ox, oy, oz, ox, oy, oz);
+      q = q < qn ? qn : q;
     }
     nnew += __popcll(mb);
   }
-  const int nkeep = nt - nrem;
   if (nkeep + nnew > HFV_V3_MAXT) return -1;
-  // compact the kept vertices in order (staged through registers), then append the new ones
-  double kx0 = 0., ky0 = 0., kz0 = 0., kx1 = 0., ky1 = 0., kz1 = 0.;
-  uint32_t kp0 = 0, kp1 = 0;
-  const bool keep0 = lane < nt && !o0, keep1 = lane + 64 < nt && !o1;
-  if (keep0) {
-    kx0 = C.tx[lane];
-    ky0 = C.ty[lane];
-    kz0 = C.tz[lane];
-    kp0 = C.tp[lane];
-  }
-  if (keep1) {
-    kx1 = C.tx[lane + 64];
-    ky1 = C.ty[lane + 64];
-    kz1 = C.tz[lane + 64];
-    kp1 = C.tp[lane + 64];
-  }
-  wave_sync();
-  if (keep0) {
-    const int t = lane - r0;
-    C.tx[t] = kx0;
-    C.ty[t] = ky0;
-    C.tz[t] = kz0;
-    C.tp[t] = kp0;
-  }
-  if (keep1) {
-    const int t = lane + 64 - r1;
-    C.tx[t] = kx1;
-    C.ty[t] = ky1;
-    C.tz[t] = kz1;
-    C.tp[t] = kp1;
-  }
-  for (int k = lane; k < nnew; k += 64) {
-    const int t = nkeep + k;
-    C.tx[t] = S.sx[k];
-    C.ty[t] = S.sy[k];
-    C.tz[t] = S.sz[k];
-    C.tp[t] = S.sp[k];
-  }
   if (lane == 0) {
     C.px[pid] = nx;
     C.py[pid] = ny;
@@ -252,6 +252,7 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double nx, double ny, double
     C.pnb[pid] = nb;
     C.pcode[pid] = code;
   }
+  r2 = wave_max(q);
   wave_sync();
   np = pid + 1;
   nt = nkeep + nnew;
@@ -504,9 +505,7 @@ __device__ int v3_build(const Batch& B, const SimDesc& D, const V2Bufs& V, int s
       const int a = __builtin_amdgcn_readfirstlane(Q.ord[q]);
       const V3Key key = Q.key[a];
       if (0.25 * key.d2 > r2) break;  // security radius: the bisector lies beyond every vertex
-      const int c = v3_clip(C, np, nt, Q.rx[a], Q.ry[a], Q.rz[a], 0.5 * key.d2, key.j, v3_code_of(key.k));
-      if (c < 0) return 3;
-      if (c > 0) r2 = v3_rmax2(C, nt);
+      if (v3_clip(C, np, nt, r2, Q.rx[a], Q.ry[a], Q.rz[a], 0.5 * key.d2, key.j, v3_code_of(key.k)) < 0) return 3;
     }
   } else {  // too many candidates for the LDS: the next one is a wave minimum over a rescan
     bool have_last = false;
@@ -522,9 +521,7 @@ __device__ int v3_build(const Batch& B, const SimDesc& D, const V2Bufs& V, int s
       });
       wave_min_cand(have, best);
       if (!have || 0.25 * best.d2 > r2) break;
-      const int c = v3_clip(C, np, nt, best.rx, best.ry, best.rz, 0.5 * best.d2, best.j, best.code);
-      if (c < 0) return 3;
-      if (c > 0) r2 = v3_rmax2(C, nt);
+      if (v3_clip(C, np, nt, r2, best.rx, best.ry, best.rz, 0.5 * best.d2, best.j, best.code) < 0) return 3;
       have_last = true;
       last = best;
     }
