@@ -17,7 +17,9 @@
 // ballots, compacts the kept ones from registers and makes the new ones one boundary edge per lane; the
 // largest vertex radius is a DPP reduction over the lanes' kept and new vertices.  The finish builds each plane's fan and polygon on its own lane,
 // with the face sums and the cell's tetrahedron terms, which one lane then adds in the host's order.
-// Blocks of more than HFV_V3_CAP candidates take the next one as a wave minimum over a rescan of the bins.
+// Blocks of more than HFV_V3_CAP candidates are taken in bands of squared distance that fit (chosen from an
+// LDS histogram); only a band that cannot be narrowed enough takes each next candidate as a wave minimum
+// over a rescan of the bins.
 #pragma once
 
 namespace hfv {
@@ -43,6 +45,8 @@ struct V3Build {
   V3Key key[HFV_V3_CAP];
   double rx[HFV_V3_CAP], ry[HFV_V3_CAP], rz[HFV_V3_CAP];
   uint16_t ord[HFV_V3_CAP];  // candidate of each rank
+  uint32_t hist[64];         // squared-distance histogram of a block too large for one pass
+  uint32_t nband;            // candidates of the current band
 };
 
 // at the finish
@@ -336,37 +340,11 @@ __device__ __forceinline__ void v3_lane_candidates(const Batch& B, const SimDesc
   });
 }
 
-// The block's candidates into LDS, ranked by (squared distance, index, image code): their count, or -1
-// beyond HFV_V3_CAP.
-__device__ int v3_gather(const Batch& B, const SimDesc& D, const V2Bufs& V, int s, int32_t i, double xi, double yi,
-                         double zi, int K, V3Cell& C) {
+// Ranks the n candidates in LDS by (squared distance, index, image code): ord[rank] = candidate.
+__device__ void v3_rank(V3Cell& C, int n) {
   const int lane = __lane_id();
-  const int32_t* gst = V.gcnt + D.cell_base + s;
-  int cnt = 0;
-  v3_lane_bins(D, xi, yi, zi, K, [&](int key, int, int, int, bool own) { cnt += gst[key + 1] - gst[key] - (own ? 1 : 0); });
-  int incl = cnt;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    const int t = __shfl_up(incl, o, 64);
-    if (lane >= o) incl += t;
-  }
-  const int n = __builtin_amdgcn_readlane(incl, 63);
-  if (n > HFV_V3_CAP) return -1;
   V3Build& Q = C.u.b;
-  int at = incl - cnt;
-  v3_lane_candidates(B, D, V, s, i, xi, yi, zi, K, [&](const V3Cand& c) {
-    if (at >= n) return;  // (the counts agree: the generator's own entry is the one skipped)
-    V3Key key;
-    key.d2 = c.d2;
-    key.j = c.j;
-    key.k = c.k;
-    Q.key[at] = key;
-    Q.rx[at] = c.rx;
-    Q.ry[at] = c.ry;
-    Q.rz[at] = c.rz;
-    ++at;
-  });
-  wave_sync();
+  n = __builtin_amdgcn_readfirstlane(n);
   // rank of each of this lane's candidates: the number of keys below it (keys are distinct).  A
   // non-negative double orders as its bits, so keys compare as integers.
   const int nown = __builtin_amdgcn_readfirstlane((n + 63) >> 6);
@@ -434,6 +412,106 @@ __device__ int v3_gather(const Batch& B, const SimDesc& D, const V2Bufs& V, int 
     if (h < nown && a < n) Q.ord[rk[h]] = (uint16_t)a;
   }
   wave_sync();
+}
+
+// The block's candidates into LDS, ranked by (squared distance, index, image code): their count, or -1
+// beyond HFV_V3_CAP.
+__device__ int v3_gather(const Batch& B, const SimDesc& D, const V2Bufs& V, int s, int32_t i, double xi, double yi,
+                         double zi, int K, V3Cell& C) {
+  const int lane = __lane_id();
+  const int32_t* gst = V.gcnt + D.cell_base + s;
+  int cnt = 0;
+  v3_lane_bins(D, xi, yi, zi, K, [&](int key, int, int, int, bool own) { cnt += gst[key + 1] - gst[key] - (own ? 1 : 0); });
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(incl, o, 64);
+    if (lane >= o) incl += t;
+  }
+  const int n = __builtin_amdgcn_readlane(incl, 63);
+  if (n > HFV_V3_CAP) return -1;
+  V3Build& Q = C.u.b;
+  int at = incl - cnt;
+  v3_lane_candidates(B, D, V, s, i, xi, yi, zi, K, [&](const V3Cand& c) {
+    if (at >= n) return;  // (the counts agree: the generator's own entry is the one skipped)
+    V3Key key;
+    key.d2 = c.d2;
+    key.j = c.j;
+    key.k = c.k;
+    Q.key[at] = key;
+    Q.rx[at] = c.rx;
+    Q.ry[at] = c.ry;
+    Q.rz[at] = c.rz;
+    ++at;
+  });
+  wave_sync();
+  v3_rank(C, n);
+  return n;
+}
+
+// The bin of squared distance d2 among 64 of width step from lo (the last one open), by exact comparisons
+// with the edges lo + k step, so that a band [lo, lo + (b + 1) step) holds exactly bins 0..b.
+__device__ __forceinline__ int v3_band_bin(double d2, double lo, double step) {
+  int k = (int)fmin(fmax((d2 - lo) / step, 0.), 63.);
+  while (k < 63 && d2 >= lo + (double)(k + 1) * step) ++k;
+  while (k > 0 && d2 < lo + (double)k * step) --k;
+  return k;
+}
+
+// A block of more than HFV_V3_CAP candidates, taken in bands of squared distance: the candidates of the
+// next band [lo, hi) (hi chosen from a histogram so that at most HFV_V3_CAP fall in it; +inf for the
+// last) gathered into LDS and ranked.  Returns their count, or -1 where even a narrow band near lo holds
+// more than HFV_V3_CAP.
+__device__ int v3_band(const Batch& B, const SimDesc& D, const V2Bufs& V, int s, int32_t i, double xi, double yi,
+                       double zi, int K, V3Cell& C, double lo, double& hi) {
+  const int lane = __lane_id();
+  V3Build& Q = C.u.b;
+  double dmax = 0.;  // beyond every candidate of the block
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const double r = (K + 1) * D.vw[a];
+    dmax += r * r;
+  }
+  if (!(dmax > lo)) dmax = 2. * lo + 1.;
+  int b = -1;
+  for (int zoom = 0; zoom < 4 && b < 0; ++zoom) {
+    const double step = (dmax - lo) / 64.;
+    Q.hist[lane] = 0u;
+    wave_sync();
+    v3_lane_candidates(B, D, V, s, i, xi, yi, zi, K, [&](const V3Cand& c) {
+      if (c.d2 >= lo) atomicAdd(&Q.hist[v3_band_bin(c.d2, lo, step)], 1u);
+    });
+    wave_sync();
+    int incl = (int)Q.hist[lane];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int t = __shfl_up(incl, o, 64);
+      if (lane >= o) incl += t;
+    }
+    b = __popcll(__ballot(incl <= HFV_V3_CAP)) - 1;  // the longest prefix of bins that fits
+    if (b >= 0) hi = b == 63 ? (double)INFINITY : lo + (double)(b + 1) * step;
+    if (b >= 0 && !(hi > lo)) b = -1;  // (a band too narrow for the doubles)
+    dmax = lo + step;  // else: the first bin alone holds too many; split it
+  }
+  if (b < 0) return -1;
+  if (lane == 0) Q.nband = 0u;
+  wave_sync();
+  v3_lane_candidates(B, D, V, s, i, xi, yi, zi, K, [&](const V3Cand& c) {
+    if (!(c.d2 >= lo && c.d2 < hi)) return;
+    const uint32_t at = atomicAdd(&Q.nband, 1u);  // (any order: the rank orders them)
+    if (at >= HFV_V3_CAP) return;                 // (cannot happen: the histogram counted them)
+    V3Key key;
+    key.d2 = c.d2;
+    key.j = c.j;
+    key.k = c.k;
+    Q.key[at] = key;
+    Q.rx[at] = c.rx;
+    Q.ry[at] = c.ry;
+    Q.rz[at] = c.rz;
+  });
+  wave_sync();
+  const int n = __builtin_amdgcn_readfirstlane((int)min(Q.nband, (uint32_t)HFV_V3_CAP));
+  v3_rank(C, n);
   return n;
 }
 
@@ -498,23 +576,35 @@ __device__ int v3_build(const Batch& B, const SimDesc& D, const V2Bufs& V, int s
   np = 6;
   nt = 8;
   r2 = v3_rmax2(C, nt);
-  const int n = v3_gather(B, D, V, s, i, xi, yi, zi, K, C);
-  if (n >= 0) {
-    const V3Build& Q = C.u.b;
+  // the candidates in rank order until the security radius holds: the whole block at once, or in bands
+  // of squared distance (each band's candidates all beyond the previous band's)
+  const V3Build& Q = C.u.b;
+  int n = v3_gather(B, D, V, s, i, xi, yi, zi, K, C);
+  double blo = 0., bhi = (double)INFINITY;
+  if (n < 0) n = v3_band(B, D, V, s, i, xi, yi, zi, K, C, blo, bhi);
+  while (n >= 0) {
+    bool stop = false;
     for (int q = 0; q < n; ++q) {
       const int a = __builtin_amdgcn_readfirstlane(Q.ord[q]);
       const V3Key key = Q.key[a];
-      if (0.25 * key.d2 > r2) break;  // security radius: the bisector lies beyond every vertex
+      if (0.25 * key.d2 > r2) {  // security radius: the bisector lies beyond every vertex
+        stop = true;
+        break;
+      }
       if (v3_clip(C, np, nt, r2, Q.rx[a], Q.ry[a], Q.rz[a], 0.5 * key.d2, key.j, v3_code_of(key.k)) < 0) return 3;
     }
-  } else {  // too many candidates for the LDS: the next one is a wave minimum over a rescan
+    if (stop || bhi == (double)INFINITY) break;
+    blo = bhi;
+    n = v3_band(B, D, V, s, i, xi, yi, zi, K, C, blo, bhi);
+  }
+  if (n < 0) {  // a band too dense for the LDS: from blo on, the next one is a wave minimum over a rescan
     bool have_last = false;
     V3Cand last{};
     while (true) {
       bool have = false;
       V3Cand best{};
       v3_lane_candidates(B, D, V, s, i, xi, yi, zi, K, [&](const V3Cand& c) {
-        if ((!have_last || v3_less(last, c)) && (!have || v3_less(c, best))) {
+        if (c.d2 >= blo && (!have_last || v3_less(last, c)) && (!have || v3_less(c, best))) {
           have = true;
           best = c;
         }

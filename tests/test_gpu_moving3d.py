@@ -82,6 +82,22 @@ def test_device_rebuild3d_random_generators():
     assert_same_mesh(dev.get_mesh(), host_mesh_of(dev, prob))
 
 
+@pytest.mark.parametrize("boundary", [fvs.PERIODIC, fvs.REFLECTIVE])
+def test_device_rebuild3d_clustered_generators(boundary):
+    """Half the generators in a cube of 1/125 of the box (60x denser than the rest): blocks of several hundred
+    candidates, taken in bands of squared distance."""
+    rng = np.random.default_rng(12)
+    prob = moving_kh(boundary, sync_all=True, side=10)
+    loc = np.concatenate([rng.random((prob.n // 2, 3)), 0.4 + 0.2 * rng.random((prob.n - prob.n // 2, 3))])
+    prob = dataclasses.replace(prob, loc=loc,
+                               mesh=VoronoiMesh(3, prob.cfg.box, boundary == fvs.PERIODIC, loc))
+    dev = ics.build_device_sim(prob)
+    say(f"[rebuild3d clustered] n={prob.n} boundary={boundary}")
+    dev.run(1)
+    assert dev.engine_info()["error_code"] == 0
+    assert_same_mesh(dev.get_mesh(), host_mesh_of(dev, prob))
+
+
 @pytest.mark.parametrize("case", ["periodic_fluid", "reflective_fluid", "periodic_steer", "periodic_sync_all"])
 def test_moving3d_full_run_parity(ora, case):
     """hfv_run (drift, device rebuild of the active cells, step) against the oracle's run with host
