@@ -214,6 +214,7 @@ struct hfv_ctx {
   int solver = -1;  // uniform solver of the loaded simulations, or -1
   bool any_moving1d = false;
   bool any_vmesh = false;   // a 2D or 3D moving mesh rebuilt on the device
+  bool any_vmesh2 = false;  // ... a 2D one
   bool any_vmesh3 = false;  // ... a 3D one
   bool any_moving = false;
   hipGraph_t graph{};
@@ -587,6 +588,7 @@ void recompute_uniforms(hfv_ctx* c) {
   int sol = -2;
   c->any_moving1d = false;
   c->any_vmesh = false;
+  c->any_vmesh2 = false;
   c->any_vmesh3 = false;
   c->any_moving = false;
   int prec = -1;
@@ -597,6 +599,7 @@ void recompute_uniforms(hfv_ctx* c) {
     else if (sol != p.solver) sol = -1;
     if (p.dim == 1 && p.motion != HFV_FIXED) c->any_moving1d = true;
     if (p.dim >= 2 && p.motion != HFV_FIXED) c->any_vmesh = true;
+    if (p.dim == 2 && p.motion != HFV_FIXED) c->any_vmesh2 = true;
     if (p.dim == 3 && p.motion != HFV_FIXED) c->any_vmesh3 = true;
     if (p.motion != HFV_FIXED) c->any_moving = true;
     if (prec < 0) prec = p.precision;
@@ -617,16 +620,16 @@ void recompute_uniforms(hfv_ctx* c) {
 struct LaunchSet {
   int solver;
   int precision;
-  bool moving1d, vmesh, vmesh3, moving, st, dyn, sgrad, dgrad;
+  bool moving1d, vmesh, vmesh2, vmesh3, moving, st, dyn, sgrad, dgrad;
   bool operator==(const LaunchSet& o) const {
     return solver == o.solver && precision == o.precision && moving1d == o.moving1d && vmesh == o.vmesh &&
-           vmesh3 == o.vmesh3 &&
+           vmesh2 == o.vmesh2 && vmesh3 == o.vmesh3 &&
            moving == o.moving &&
            st == o.st && dyn == o.dyn && sgrad == o.sgrad && dgrad == o.dgrad;
   }
 };
 LaunchSet launch_set(const hfv_ctx* c) {
-  return LaunchSet{c->solver, c->precision, c->any_moving1d, c->any_vmesh, c->any_vmesh3, c->any_moving, c->any_static,
+  return LaunchSet{c->solver, c->precision, c->any_moving1d, c->any_vmesh, c->any_vmesh2, c->any_vmesh3, c->any_moving, c->any_static,
                    c->any_dynamic, c->any_static_grad, c->any_dynamic_grad};
 }
 
@@ -1068,8 +1071,10 @@ void enqueue_rebuild(hfv_ctx* c, const Batch& B) {
     hipLaunchKernelGGL(k_v2_scan_mid<0>, gs, gsb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_scan_down<0>, gc, gb, 0, st, B, V);
     hipLaunchKernelGGL(k_v2_place, gc, gb, 0, st, B, V);
-    hipLaunchKernelGGL(k_v2_cells<false>, dim3((unsigned)(c->ncg * HFV_V2_SUB)), gb, 0, st, B, V);
-    hipLaunchKernelGGL(k_v2_cells<true>, dim3((unsigned)(c->ncg * HFV_V2_SUB)), gb, 0, st, B, V);
+    if (c->any_vmesh2) {
+      hipLaunchKernelGGL(k_v2_cells<false>, dim3((unsigned)(c->ncg * HFV_V2_SUB)), gb, 0, st, B, V);
+      hipLaunchKernelGGL(k_v2_cells<true>, dim3((unsigned)(c->ncg * HFV_V2_SUB)), gb, 0, st, B, V);
+    }
     if (c->any_vmesh3) {
       const dim3 g3((unsigned)(c->ncg * BLOCK)), w3(64);
       hipLaunchKernelGGL(k_v3_cells, g3, w3, 0, st, B, V);

@@ -712,7 +712,10 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_csr_fill(Batch B, V2Bufs V) {
   }
 }
 
+#define HFV_CSR_SORT_LDS 24  // a cell's CSR entries sorted in LDS (more: in place in memory)
+
 __global__ void __launch_bounds__(HFV_BLOCK) k_v2_csr_sort(Batch B, V2Bufs V) {
+  __shared__ uint64_t ent[HFV_CSR_SORT_LDS][HFV_BLOCK];  // [slot][thread]: (CSR value << 32) | neighbour
   const BlockJob J = B.cjobs[blockIdx.x];
   if (J.sim < 0) return;
   const int s = J.sim;
@@ -720,11 +723,33 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_v2_csr_sort(Batch B, V2Bufs V) {
   SimEngine& E = B.eng[s];
   if (!D.vmesh || !E.stepping || E.err) return;
   const int64_t gi = (int64_t)J.lo + threadIdx.x;
-  if (gi >= J.hi) return;
+  if (gi >= J.hi) return;  // (no block barrier follows)
   const int32_t e0 = B.F.off[gi], e1 = B.F.off[gi + 1];
   int32_t* c = B.F.conn;
   int32_t* nb = B.F.nbr;
-  for (int32_t k = e0 + 1; k < e1; ++k) {  // insertion sort of the cell's entries by face index (then side)
+  const int m = e1 - e0;
+  if (m <= HFV_CSR_SORT_LDS) {
+    // insertion sort of the cell's entries by face index (then side): CSR values are distinct and
+    // non-negative, so the packed words order as they do
+    uint64_t* v = &ent[0][threadIdx.x];
+    for (int k = 0; k < m; ++k) v[k * HFV_BLOCK] = ((uint64_t)(uint32_t)c[e0 + k] << 32) | (uint32_t)nb[e0 + k];
+    for (int k = 1; k < m; ++k) {
+      const uint64_t x = v[k * HFV_BLOCK];
+      int j = k - 1;
+      while (j >= 0 && v[j * HFV_BLOCK] > x) {
+        v[(j + 1) * HFV_BLOCK] = v[j * HFV_BLOCK];
+        --j;
+      }
+      v[(j + 1) * HFV_BLOCK] = x;
+    }
+    for (int k = 0; k < m; ++k) {
+      const uint64_t x = v[k * HFV_BLOCK];
+      c[e0 + k] = (int32_t)(x >> 32);
+      nb[e0 + k] = (int32_t)(uint32_t)x;
+    }
+    return;
+  }
+  for (int32_t k = e0 + 1; k < e1; ++k) {  // insertion sort in place
     const int32_t x = c[k], y = nb[k];
     int32_t j = k - 1;
     while (j >= e0 && c[j] > x) {
