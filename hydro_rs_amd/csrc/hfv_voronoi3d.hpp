@@ -37,11 +37,11 @@ struct __align__(16) V3Key {
   int32_t j, k;
 };
 
-// while clipping: the clip scratch (the removed vertices' plane ids and positions) and the block's
+// while clipping: the clip scratch (the removed vertices' indices and plane ids) and the block's
 // candidates
 struct V3Build {
   uint32_t remp[HFV_V3_MAXT];
-  double sx[HFV_V3_MAXT], sy[HFV_V3_MAXT], sz[HFV_V3_MAXT];
+  uint8_t remi[HFV_V3_MAXT];
   V3Key key[HFV_V3_CAP];
   double rx[HFV_V3_CAP], ry[HFV_V3_CAP], rz[HFV_V3_CAP];
   uint16_t ord[HFV_V3_CAP];  // candidate of each rank
@@ -49,13 +49,21 @@ struct V3Build {
   uint32_t nband;            // candidates of the current band
 };
 
-// at the finish
-struct V3Fin {
-  // per plane its fan (u, v, vertex) in vertex order, then its polygon
+// at the finish: per plane its polygon; its fan (u, v, vertex) in vertex order until the polygons are
+// chained, then in the same place the cell's tetrahedron terms (volume, centroid numerators) in plane and
+// fan order
+struct V3Fans {
   uint8_t fu[HFV_V3_MAXP][HFV_V3_FAN], fv[HFV_V3_MAXP][HFV_V3_FAN], ft[HFV_V3_MAXP][HFV_V3_FAN];
-  uint8_t poly[HFV_V3_MAXP][HFV_V3_FAN];
-  // the cell's tetrahedron terms (volume, centroid numerators) in plane and fan order
+};
+struct V3Terms {
   double qv[HFV_V3_NTRI], qx[HFV_V3_NTRI], qy[HFV_V3_NTRI], qz[HFV_V3_NTRI];
+};
+struct V3Fin {
+  uint8_t poly[HFV_V3_MAXP][HFV_V3_FAN];
+  union {
+    V3Fans a;
+    V3Terms b;
+  } u;
 };
 
 struct V3Cell {
@@ -168,15 +176,11 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double& r2, double nx, doubl
   const int r0 = __popcll(m0 & below), r1 = __popcll(m0) + __popcll(m1 & below);
   if (o0) {
     S.remp[r0] = p0;
-    S.sx[r0] = x0;
-    S.sy[r0] = y0;
-    S.sz[r0] = z0;
+    S.remi[r0] = (uint8_t)lane;
   }
   if (o1) {
     S.remp[r1] = p1;
-    S.sx[r1] = x1;
-    S.sy[r1] = y1;
-    S.sz[r1] = z1;
+    S.remi[r1] = (uint8_t)(lane + 64);
   }
   wave_sync();
   // the kept vertices, in order
@@ -205,11 +209,12 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double& r2, double nx, doubl
   int nnew = 0;
   for (int q0 = 0; q0 < 3 * nrem; q0 += 64) {
     const int e3 = q0 + lane;
-    bool bnd = false;
-    int x = 0, y = 0;
+    bool bnd = false, fb = false;
+    int x = 0, y = 0, rq = 0;
     double ox = 0., oy = 0., oz = 0.;
     if (e3 < 3 * nrem) {
-      const int rq = e3 / 3, e = e3 - 3 * rq;
+      rq = e3 / 3;
+      const int e = e3 - 3 * rq;
       const uint32_t w = S.remp[rq];
       const int a0 = w & 0xFF, a1 = (w >> 8) & 0xFF, a2 = w >> 16;
       x = e == 0 ? a0 : e == 1 ? a1 : a2;
@@ -228,11 +233,18 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double& r2, double nx, doubl
       }
       if (!internal) {
         bnd = true;
-        if (!v3_intersect(C, x, y, nx, ny, nz, d, ox, oy, oz)) {  // the removed vertex's position
-          ox = S.sx[rq];
-          oy = S.sy[rq];
-          oz = S.sz[rq];
-        }
+        fb = !v3_intersect(C, x, y, nx, ny, nz, d, ox, oy, oz);
+      }
+    }
+    if (__ballot(fb)) {  // (rare) no intersection: the removed vertex's position, from the lane that read it
+      const int ri = fb ? S.remi[rq] : 0;
+      const int src = ri & 63;
+      const double ax = __shfl(x0, src, 64), ay = __shfl(y0, src, 64), az = __shfl(z0, src, 64);
+      const double bx = __shfl(x1, src, 64), by = __shfl(y1, src, 64), bz = __shfl(z1, src, 64);
+      if (fb) {
+        ox = ri < 64 ? ax : bx;
+        oy = ri < 64 ? ay : by;
+        oz = ri < 64 ? az : bz;
       }
     }
     const uint64_t mb = __ballot(bnd);
@@ -650,9 +662,9 @@ __device__ int v3_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, SimE
         const int u = ia ? b : ib ? c : ic ? a : -1, v = ia ? c : ib ? a : b;
         if (u >= 0) {
           if (f < HFV_V3_FAN) {
-            F.fu[p][f] = (uint8_t)u;
-            F.fv[p][f] = (uint8_t)v;
-            F.ft[p][f] = (uint8_t)(t0 + e);
+            F.u.a.fu[p][f] = (uint8_t)u;
+            F.u.a.fv[p][f] = (uint8_t)v;
+            F.u.a.ft[p][f] = (uint8_t)(t0 + e);
           }
           ++f;
         }
@@ -661,11 +673,11 @@ __device__ int v3_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, SimE
     over = f > HFV_V3_FAN;
     if (f >= 3 && !over) {
       // from (u, v) the next entry around p is the first one whose u is this v (8 entries a load)
-      const uint2* fu8 = reinterpret_cast<const uint2*>(&F.fu[p][0]);
+      const uint2* fu8 = reinterpret_cast<const uint2*>(&F.u.a.fu[p][0]);
       int cur = 0;
       for (int step = 0; step < f; ++step) {
-        F.poly[p][m++] = F.ft[p][cur];
-        const uint32_t want = F.fv[p][cur];
+        F.poly[p][m++] = F.u.a.ft[p][cur];
+        const uint32_t want = F.u.a.fv[p][cur];
         int nxt = -1;
         for (int q0 = 0; q0 < f; q0 += 8) {
           const uint2 w = fu8[q0 >> 3];
@@ -694,6 +706,7 @@ __device__ int v3_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, SimE
   const int ntri = __builtin_amdgcn_readlane(incl, 63);
   if (ntri > HFV_V3_NTRI) return 3;  // (2 V - 4 on a closed cell)
   const int q0 = incl - nq;
+  wave_sync();  // the fans are read no more: their place takes the terms
   // the face's area vector and centroid; the cell's tetrahedron terms (vertex fan with the generator)
   double area = 0., fcx = 0., fcy = 0., fcz = 0.;
   if (m >= 3) {
@@ -718,10 +731,10 @@ __device__ int v3_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, SimE
       const double tv = fabs(v3dot(p0x, p0y, p0z, ux, uy, uz)) / 6.0;
       const double s4 = tv / 4.0;
       const int qi = q0 + k - 1;
-      F.qv[qi] = tv;
-      F.qx[qi] = s4 * ((p0x + Ax) + Bx);
-      F.qy[qi] = s4 * ((p0y + Ay) + By);
-      F.qz[qi] = s4 * ((p0z + Az) + Bz);
+      F.u.b.qv[qi] = tv;
+      F.u.b.qx[qi] = s4 * ((p0x + Ax) + Bx);
+      F.u.b.qy[qi] = s4 * ((p0y + Ay) + By);
+      F.u.b.qz[qi] = s4 * ((p0z + Az) + Bz);
     }
     area = sqrt(v3dot(avx, avy, avz, avx, avy, avz));
     if (asum > 0) {
@@ -774,10 +787,10 @@ __device__ int v3_finish(const Batch& B, const V2Bufs& V, const SimDesc& D, SimE
   // the cell's volume and centroid: the tetrahedron terms added in the host's order
   double vol = 0., cenx = 0., ceny = 0., cenz = 0.;
   for (int q = 0; q < ntri; ++q) {
-    vol += F.qv[q];
-    cenx = cenx + F.qx[q];
-    ceny = ceny + F.qy[q];
-    cenz = cenz + F.qz[q];
+    vol += F.u.b.qv[q];
+    cenx = cenx + F.u.b.qx[q];
+    ceny = ceny + F.u.b.qy[q];
+    cenz = cenz + F.u.b.qz[q];
   }
   if (B.C.act[gi]) {  // Particle::update_geometry (part.rs:151-158)
     B.C.vol[gi] = vol;

@@ -83,6 +83,27 @@ def test_device_rebuild3d_random_generators():
 
 
 @pytest.mark.parametrize("boundary", [fvs.PERIODIC, fvs.REFLECTIVE])
+def test_device_rebuild3d_exact_lattice(boundary):
+    """Generators on an exact cubic lattice, gas at rest (the generators stay put): equal distances break
+    ties by index and image code, and bisectors parallel to the walls and to each other meet in no point
+    (the clip's fallback to the removed vertex's position)."""
+    prob = moving_kh(boundary, sync_all=True, side=6)
+    side = 6
+    g = (np.arange(side) + 0.5) / side
+    loc = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    mesh = VoronoiMesh(3, prob.cfg.box, boundary == fvs.PERIODIC, loc)
+    prob = dataclasses.replace(prob, loc=loc, mesh=mesh, vel=np.zeros_like(prob.vel),
+                               mass=np.full(prob.n, 1.0 / prob.n), u_int=np.full(prob.n, 1.5))
+    dev = ics.build_device_sim(prob)
+    say(f"[rebuild3d lattice] n={prob.n} boundary={boundary}")
+    dev.run(2)
+    assert dev.engine_info()["error_code"] == 0
+    got = dev.get_mesh()
+    assert_same_mesh(got, host_mesh_of(dev, prob))
+    assert np.array_equal(dev.get_state(["loc"])["loc"].reshape(3, prob.n).T, loc)  # at rest
+
+
+@pytest.mark.parametrize("boundary", [fvs.PERIODIC, fvs.REFLECTIVE])
 def test_device_rebuild3d_clustered_generators(boundary):
     """Half the generators in a cube of 1/125 of the box (60x denser than the rest): blocks of several hundred
     candidates, taken in bands of squared distance."""
