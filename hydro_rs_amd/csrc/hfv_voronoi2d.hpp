@@ -28,6 +28,9 @@
 namespace hfv {
 
 #define HFV_SCAN_BLOCK 1024
+// A clip removes the vertices beyond a plane by more than this fraction of its offset (the host
+// tessellator's threshold): vertices on it test as beyond by a rounding error on exact lattices.
+#define HFV_CLIP_TOL 1e-12
 
 // A face in the provisional order: area (length in 2D), centroid and unit normal relative to the emitting
 // generator, the neighbour's local index (< 0: a wall) and periodic image code.
@@ -394,7 +397,7 @@ __device__ __forceinline__ int team_clip(V2TeamPoly<TL>& P, int& m, double a, do
 #pragma unroll
   for (int t = 0; t < TL; t += HFV_V2_TEAM) {
     const int k = t + lane;
-    const bool o = k < m && (a * P.vx[k] + b * P.vy[k]) > dd;
+    const bool o = k < m && (a * P.vx[k] + b * P.vy[k]) > dd + HFV_CLIP_TOL * dd;
     om |= (uint64_t)team_ballot(o, tiw) << t;
   }
   const int nout = __popcll(om);

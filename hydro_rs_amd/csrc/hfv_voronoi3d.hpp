@@ -166,8 +166,9 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double& r2, double nx, doubl
     z1 = C.tz[lane + 64];
     p1 = C.tp[lane + 64];
   }
-  const bool o0 = h0 && v3dot(nx, ny, nz, x0, y0, z0) > d;
-  const bool o1 = h1 && v3dot(nx, ny, nz, x1, y1, z1) > d;
+  const double dth = d + HFV_CLIP_TOL * d;  // (beyond by more than a rounding error)
+  const bool o0 = h0 && v3dot(nx, ny, nz, x0, y0, z0) > dth;
+  const bool o1 = h1 && v3dot(nx, ny, nz, x1, y1, z1) > dth;
   const uint64_t m0 = __ballot(o0), m1 = __ballot(o1);
   const int nrem = __popcll(m0) + __popcll(m1);
   if (nrem == 0 || nrem == nt) return 0;
@@ -236,15 +237,40 @@ __device__ int v3_clip(V3Cell& C, int& np, int& nt, double& r2, double nx, doubl
         fb = !v3_intersect(C, x, y, nx, ny, nz, d, ox, oy, oz);
       }
     }
-    if (__ballot(fb)) {  // (rare) no intersection: the removed vertex's position, from the lane that read it
-      const int ri = fb ? S.remi[rq] : 0;
-      const int src = ri & 63;
-      const double ax = __shfl(x0, src, 64), ay = __shfl(y0, src, 64), az = __shfl(z0, src, 64);
-      const double bx = __shfl(x1, src, 64), by = __shfl(y1, src, 64), bz = __shfl(z1, src, 64);
-      if (fb) {
-        ox = ri < 64 ? ax : bx;
-        oy = ri < 64 ? ay : by;
-        oz = ri < 64 ? az : bz;
+    // (rare) the three planes meet in no point: where the edge (x, y) from the removed vertex to the vertex
+    // across it (the first with the edge (y, x), a kept one) crosses the plane; the removed vertex's
+    // position where that is no crossing.  Each such lane in turn, the others searching their vertices.
+    for (uint64_t todo = __ballot(fb); todo; todo &= todo - 1) {
+      const int L = __builtin_ctzll(todo);
+      const int xq = __builtin_amdgcn_readlane(x, L), yq = __builtin_amdgcn_readlane(y, L);
+      const int ri = __builtin_amdgcn_readlane(fb ? (int)S.remi[rq] : 0, L);
+      auto has = [&](uint32_t w) {
+        const int u0 = w & 0xFF, u1 = (w >> 8) & 0xFF, u2 = w >> 16;
+        return ((u0 == yq) & (u1 == xq)) | ((u1 == yq) & (u2 == xq)) | ((u2 == yq) & (u0 == xq));
+      };
+      const uint64_t k0 = __ballot(h0 && has(p0)), k1 = __ballot(h1 && has(p1));
+      const int rs = ri & 63;  // the removed vertex, from the lane that read it
+      const double tx = ri < 64 ? __shfl(x0, rs, 64) : __shfl(x1, rs, 64);
+      const double ty = ri < 64 ? __shfl(y0, rs, 64) : __shfl(y1, rs, 64);
+      const double tz = ri < 64 ? __shfl(z0, rs, 64) : __shfl(z1, rs, 64);
+      double cx = tx, cy = ty, cz = tz;
+      if (k0 | k1) {
+        const int us = k0 ? __builtin_ctzll(k0) : __builtin_ctzll(k1);
+        const double ux = k0 ? __shfl(x0, us, 64) : __shfl(x1, us, 64);
+        const double uy = k0 ? __shfl(y0, us, 64) : __shfl(y1, us, 64);
+        const double uz = k0 ? __shfl(z0, us, 64) : __shfl(z1, us, 64);
+        const double a = v3dot(nx, ny, nz, ux, uy, uz) - d, b = v3dot(nx, ny, nz, tx, ty, tz) - d;
+        if (a <= 0.0 && b > a) {
+          const double sc = a / (a - b);
+          cx = ux + sc * (tx - ux);
+          cy = uy + sc * (ty - uy);
+          cz = uz + sc * (tz - uz);
+        }
+      }
+      if (lane == L) {
+        ox = cx;
+        oy = cy;
+        oz = cz;
       }
     }
     const uint64_t mb = __ballot(bnd);

@@ -37,6 +37,11 @@ inline double dot(V3 a, V3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 inline V3 cross(V3 a, V3 b) { return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
 inline double norm(V3 a) { return std::sqrt(dot(a, a)); }
 
+// A clip removes the vertices beyond a plane by more than this fraction of its offset d: vertices that lie
+// on it (exact lattices: several bisectors through one vertex) test as beyond by a rounding error, and
+// cutting them off would leave slivers of zero width.  The device rebuild uses the same threshold.
+constexpr double kClipTol = 1e-12;
+
 // A half-space n.y <= d in coordinates relative to the generator.
 struct Plane {
   V3 n;
@@ -115,11 +120,28 @@ struct Cell3 {
     return r;
   }
 
+  // Where three planes meet in no point (their normals coplanar, as where more than three bisectors pass
+  // through one vertex of an exact lattice): the point where the edge (x, y) from the removed vertex t to
+  // the kept vertex across it (the triangle with the edge (y, x)) crosses P; t's own position if there is
+  // none.
+  V3 edge_crossing(const Plane& P, const Tri& t, int x, int y) const {
+    for (const Tri& u : tris) {
+      bool has = false;
+      for (int k = 0; k < 3; ++k) has = has || (u.p[k] == y && u.p[(k + 1) % 3] == x);
+      if (!has) continue;
+      const double a = dot(P.n, u.pos) - P.d, b = dot(P.n, t.pos) - P.d;
+      if (!(a <= 0.0) || !(b > a)) return t.pos;  // (not a kept vertex against a removed one)
+      const double s = a / (a - b);
+      return u.pos + (s * (t.pos - u.pos));
+    }
+    return t.pos;
+  }
+
   // Returns false if the plane does not cut the cell.
   bool clip(const Plane& P) {
     removed.clear();
     for (int i = 0; i < (int)tris.size(); ++i)
-      if (dot(P.n, tris[i].pos) > P.d) removed.push_back(i);
+      if (dot(P.n, tris[i].pos) > P.d + kClipTol * P.d) removed.push_back(i);
     if (removed.empty()) return false;
     if (removed.size() == tris.size()) return false;  // cannot happen for a bisector of a neighbour
     int pid = (int)planes.size();
@@ -145,7 +167,7 @@ struct Cell3 {
           nt.p[0] = x;
           nt.p[1] = y;
           nt.p[2] = pid;
-          if (!intersect3(planes[x], planes[y], P, &nt.pos)) nt.pos = t.pos;
+          if (!intersect3(planes[x], planes[y], P, &nt.pos)) nt.pos = edge_crossing(P, t, x, y);
           scratch.push_back(nt);
         }
       }
@@ -270,7 +292,7 @@ struct Cell2 {
     std::vector<char> out(m);
     int nout = 0;
     for (size_t k = 0; k < m; ++k) {
-      out[k] = (P.n.x * verts[k].x + P.n.y * verts[k].y) > P.d;
+      out[k] = (P.n.x * verts[k].x + P.n.y * verts[k].y) > P.d + kClipTol * P.d;
       nout += out[k];
     }
     if (nout == 0 || nout == (int)m) return false;

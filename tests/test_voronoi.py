@@ -81,18 +81,42 @@ def test_linear_field_divergence_theorem():
 
 
 @pytest.mark.parametrize("periodic", [True, False])
-def test_partial_build_is_the_full_build_around_the_masked_cells(periodic):
+def test_3d_exact_lattice_known_answers(periodic):
+    """Generators on an exact cubic lattice: every cell is the cube around its generator (volume w^3,
+    centroid at the generator), with six faces of area w^2 and axis normals; 3 N faces on a periodic box
+    (each shared), 3 N + 3 side^2 with walls."""
+    side = 5
+    w = 1.0 / side
+    g = (np.arange(side) + 0.5) * w
+    loc = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    m = VoronoiMesh(3, (1.0, 1.0, 1.0), periodic, loc)
+    n = side ** 3
+    assert m.n_faces == (3 * n if periodic else 3 * n + 3 * side * side)
+    assert np.allclose(m.volume, w ** 3, rtol=1e-12, atol=0)
+    assert np.allclose(m.cell_centroid, loc, rtol=0, atol=1e-14)
+    assert np.allclose(m.area, w * w, rtol=1e-12, atol=0)
+    assert np.allclose(np.abs(m.normal).max(axis=1), 1.0) and np.allclose(np.abs(m.normal).sum(axis=1), 1.0)
+    assert np.all(m.count == 6)
+    # each interior face sits halfway between its generators (through the periodic image where shifted)
+    inner = m.right >= 0
+    mid = 0.5 * (loc[m.left[inner]] + loc[m.right[inner]] + m.shift[inner])
+    assert np.allclose(m.centroid[inner], mid, rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_partial_build_is_the_full_build_around_the_masked_cells(periodic, dim):
     """Voronoi::build_partial (space.rs:236-245): the masked cells' geometry equals the full build's, the
     faces are exactly the full build's faces with a masked cell (each from a masked cell's polygon: within
     1e-12 of the full build's, and bit for bit when both cells are masked), every cell's CSR lists exactly
     its faces, and unmasked cells are unbuilt."""
     rng = np.random.default_rng(11)
     box = np.array([1.0, 1.0, 1.0])
-    loc = jittered_grid(2, 20, box, 0.3, rng)
+    loc = jittered_grid(dim, 20 if dim == 2 else 7, box, 0.3, rng)
     n = loc.shape[0]
     mask = rng.uniform(size=n) < 0.3
-    full = VoronoiMesh(2, box, periodic, loc)
-    part = VoronoiMesh(2, box, periodic, loc, mask=mask)
+    full = VoronoiMesh(dim, box, periodic, loc)
+    part = VoronoiMesh(dim, box, periodic, loc, mask=mask)
     for k in ("volume", "search_radius"):
         assert np.array_equal(getattr(part, k)[mask], getattr(full, k)[mask]), k
         assert np.all(getattr(part, k)[~mask] == 0.0), k
