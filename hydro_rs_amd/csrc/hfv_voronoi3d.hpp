@@ -6,20 +6,21 @@
 // operations in the same order: a convex polyhedron stored by its dual, every vertex a triangle of three
 // plane ids (the initial box's 8 corners oriented by det(n_a, n_b, n_c) > 0), clipped by the bisectors of
 // the generators of the (2K+1)^3 surrounding bins in (squared distance, index, image code) order until the
-// security radius holds.  A clip removes the vertices beyond the plane, adds one vertex per boundary edge
-// of the removed region (in the order of the removed vertices and their edges), keeps the others in
-// order and appends the new ones.  The faces are the planes' vertex fans chained into polygons; area
-// vector, centroid and the cell's volume and centroid are the host's triangle-fan sums in its order.
+// security radius holds.  A clip removes the vertices beyond the plane (by more than HFV_CLIP_TOL of its
+// offset), adds one vertex per boundary edge of the removed region (in the order of the removed vertices
+// and their edges; where the three planes meet in no point, the edge's crossing of the plane), keeps the
+// others in order and appends the new ones.  The faces are the planes' vertex fans chained into polygons;
+// area vector, centroid and the cell's volume and centroid are the host's triangle-fan sums in its order.
 //
 // One wavefront (a block of its own) builds one cell, everything in LDS: the block's candidates are
 // gathered once per K and ranked by their key (each lane ranks every 64th candidate against all), then
 // taken in rank order; a clip reads each vertex once (two per lane), lists the removed ones with two
 // ballots, compacts the kept ones from registers and makes the new ones one boundary edge per lane; the
-// largest vertex radius is a DPP reduction over the lanes' kept and new vertices.  The finish builds each plane's fan and polygon on its own lane,
-// with the face sums and the cell's tetrahedron terms, which one lane then adds in the host's order.
-// Blocks of more than HFV_V3_CAP candidates are taken in bands of squared distance that fit (chosen from an
-// LDS histogram); only a band that cannot be narrowed enough takes each next candidate as a wave minimum
-// over a rescan of the bins.
+// largest vertex radius is a DPP reduction over the lanes' kept and new vertices.  The finish builds each
+// plane's fan and polygon on its own lane, with the face sums and the cell's tetrahedron terms, which one
+// lane then adds in the host's order.  Blocks of more than HFV_V3_CAP candidates are taken in bands of
+// squared distance that fit (chosen from an LDS histogram); only a band that cannot be narrowed enough
+// takes each next candidate as a wave minimum over a rescan of the bins.
 #pragma once
 
 namespace hfv {
