@@ -546,6 +546,12 @@ __device__ __forceinline__ V3<double> decode_shift(uint8_t code, const double* b
 #ifndef HFV_GRAD_CHUNK
 #define HFV_GRAD_CHUNK 4
 #endif
+// CSR entries of a cell whose loads (CSR value, face flag, record) k_apply issues together: 4 against a
+// three-stage pipeline, KH 128^3 accumulation -6% (FP64) / -27% (FP32 flux); 6 and 8 gained 0.3% more on
+// KH at occupancy 3 and cost the ensemble 2%
+#ifndef HFV_APPLY_CHUNK
+#define HFV_APPLY_CHUNK 4
+#endif
 #ifndef HFV_GRAD_WAVES
 #define HFV_GRAD_WAVES 0
 #endif
@@ -1583,42 +1589,39 @@ __global__ void __launch_bounds__(HFV_BLOCK) k_apply(Batch B, int mode) {
         aor = r67.x;
       }
     };
-    // Three-stage pipeline over the cell's CSR entries: entry e+3's CSR value, entry e+2's face flag
-    // and entry e+1's record (when its face is current) are loaded while entry e is applied, so the
-    // per-entry chain of dependent loads overlaps with the work.  Entries are applied in CSR order.
-    int32_t c0 = 0, c1 = 0, c2 = 0;  // CSR values of entries e, e+1, e+2
-    uint8_t u0 = 0, u1 = 0;          // face flags of entries e, e+1
-    St<double> r0 = vacuum<double>();  // record of entry e
-    double v0 = 0., o0 = 0.;
-    if (k0 < k1) c0 = F.conn[k0];
-    if (k0 + 1 < k1) c1 = F.conn[k0 + 1];
-    if (k0 + 2 < k1) c2 = F.conn[k0 + 2];
-    if (k0 < k1) u0 = F.cur[(uint32_t)(c0 >> 1)];
-    if (k0 + 1 < k1) u1 = F.cur[(uint32_t)(c1 >> 1)];
-    if (k0 < k1 && u0) load_rec((uint32_t)(c0 >> 1), r0, v0, o0);
-    for (int32_t e = k0; e < k1; ++e) {
-      const int32_t c = c0;
-      const uint8_t cu = u0;
-      const St<double> rec = r0;
-      const double vmx = v0, aor = o0;
-      c0 = c1;
-      u0 = u1;
-      if (e + 1 < k1 && u0) load_rec((uint32_t)(c0 >> 1), r0, v0, o0);
-      c1 = c2;
-      if (e + 2 < k1) u1 = F.cur[(uint32_t)(c1 >> 1)];
-      if (e + 3 < k1) c2 = F.conn[e + 3];
-      if (!cu) continue;
-      if (!have) {
-        acc = ld5(C.facc, n, i);
-        have = true;
+    // Entries in chunks: a chunk's CSR values, then its faces' flags, then the records of its current
+    // faces, all in flight together before the chunk's entries are applied in CSR order.
+    for (int32_t kb = k0; kb < k1; kb += HFV_APPLY_CHUNK) {
+      int32_t cv[HFV_APPLY_CHUNK];
+#pragma unroll
+      for (int u = 0; u < HFV_APPLY_CHUNK; ++u) cv[u] = kb + u < k1 ? F.conn[kb + u] : 0;
+      uint32_t cf[HFV_APPLY_CHUNK];
+#pragma unroll
+      for (int u = 0; u < HFV_APPLY_CHUNK; ++u) cf[u] = kb + u < k1 ? (uint32_t)F.cur[(uint32_t)(cv[u] >> 1)] : 0u;
+      St<double> rc[HFV_APPLY_CHUNK];
+      double vm[HFV_APPLY_CHUNK], ao[HFV_APPLY_CHUNK];
+#pragma unroll
+      for (int u = 0; u < HFV_APPLY_CHUNK; ++u) {
+        rc[u] = vacuum<double>();
+        vm[u] = 0.;
+        ao[u] = 0.;
+        if (cf[u]) load_rec((uint32_t)(cv[u] >> 1), rc[u], vm[u], ao[u]);
       }
-      if ((c & 1) == 0)
-        acc = acc - rec;
-      else
-        acc = acc + rec;
-      if (a) {
-        msv = hmax(vmx, msv);
-        maor = hmax(maor, aor);
+#pragma unroll
+      for (int u = 0; u < HFV_APPLY_CHUNK; ++u) {
+        if (!cf[u]) continue;
+        if (!have) {
+          acc = ld5(C.facc, n, i);
+          have = true;
+        }
+        if ((cv[u] & 1) == 0)
+          acc = acc - rc[u];
+        else
+          acc = acc + rc[u];
+        if (a) {
+          msv = hmax(vm[u], msv);
+          maor = hmax(maor, ao[u]);
+        }
       }
     }
     if (have) {
@@ -1801,38 +1804,47 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
       __builtin_amdgcn_sched_barrier(0);
     };
     if (STATIC) {
-      // the entry's neighbour reference (LDS stage slot when staged, else its cell; -1 at a wall) and
-      // geometry line, loaded one entry ahead
-      int32_t r_n = -1;
-      double4 a_n = make_double4(0., 0., 0., 0.);
-      auto fetch = [&](int32_t k) {
-        if (gst) {
-          const uint32_t li = F.nloc[k];
-          r_n = li == 0xFFFFu ? -1 : (int32_t)li;
-        } else {
-          r_n = F.nbr[k];
+      // entries in chunks of HFV_GRAD_CHUNK: a chunk's neighbour references (LDS stage slot when staged,
+      // else its cell; -1 at a wall) and geometry lines in flight together, then its entries in order
+      for (int32_t kb = k0; kb < k1; kb += HFV_GRAD_CHUNK) {
+        int32_t rr[HFV_GRAD_CHUNK];
+        double4 ag[HFV_GRAD_CHUNK];
+#pragma unroll
+        for (int u = 0; u < HFV_GRAD_CHUNK; ++u) {
+          const int32_t k = kb + u;
+          if (k < k1) {
+            if (gst) {
+              const uint32_t li = F.nloc[k];
+              rr[u] = li == 0xFFFFu ? -1 : (int32_t)li;
+            } else {
+              rr[u] = F.nbr[k];
+            }
+            ag[u] = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k))[0];
+          } else {
+            rr[u] = -1;
+            ag[u] = make_double4(0., 0., 0., 0.);
+          }
         }
-        a_n = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k))[0];
-      };
-      if (k0 < k1) fetch(k0);
-      for (int32_t k = k0; k < k1; ++k) {
-        const int32_t other = r_n;
-        const uint32_t li = (uint32_t)other;
-        const double4 a = a_n;
-        if (k + 1 < k1) fetch(k + 1);
-        const double w = a.x;
-        const V3<double> ds = mk3(a.y, a.z, a.w);
-        St<double> op;
-        if (other >= 0) {
-          V3<double> oc;
-          if (gst) load_pc((const double*)(gst + 2 * li), 0, op, oc);
-          else load_pc(C.pc, (uint32_t)other, op, oc);
-        } else {
-          op = prim;  // Space::get_boundary_part (space.rs:213-229)
-          if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, ld3(F.nrm, nf, (uint32_t)(F.conn[k] >> 1)));
-          else if (cfg.boundary == BND_VACUUM) op = vacuum<double>();
+#pragma unroll
+        for (int u = 0; u < HFV_GRAD_CHUNK; ++u) {
+          const int32_t k = kb + u;
+          if (k >= k1) break;
+          const int32_t other = rr[u];
+          const double4 a = ag[u];
+          const double w = a.x;
+          const V3<double> ds = mk3(a.y, a.z, a.w);
+          St<double> op;
+          if (other >= 0) {
+            V3<double> oc;
+            if (gst) load_pc((const double*)(gst + 2 * (uint32_t)other), 0, op, oc);
+            else load_pc(C.pc, (uint32_t)other, op, oc);
+          } else {
+            op = prim;  // Space::get_boundary_part (space.rs:213-229)
+            if (cfg.boundary == BND_REFLECTIVE) op = reflect(prim, ld3(F.nrm, nf, (uint32_t)(F.conn[k] >> 1)));
+            else if (cfg.boundary == BND_VACUUM) op = vacuum<double>();
+          }
+          accumulate(op, ds, w);
         }
-        accumulate(op, ds, w);
       }
     } else {
       // Entries in chunks of HFV_GRAD_CHUNK: a chunk's CSR values and neighbour references, then its faces'
@@ -1939,12 +1951,16 @@ __device__ __forceinline__ void grad_cell(const Batch& B, const Cfg& cfg, const 
       }
     };
     if (STATIC) {
-      double4 b_n = make_double4(0., 0., 0., 0.);
-      if (k0 < k1) b_n = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)k0))[1];
-      for (int32_t k = k0; k < k1; ++k) {
-        const double4 b = b_n;
-        if (k + 1 < k1) b_n = ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)(k + 1)))[1];
-        extrema(mk3(b.x, b.y, b.z));
+      for (int32_t kb = k0; kb < k1; kb += HFV_GRAD_CHUNK) {  // chunks of lines in flight together
+        double4 bg[HFV_GRAD_CHUNK];
+#pragma unroll
+        for (int u = 0; u < HFV_GRAD_CHUNK; ++u)
+          bg[u] = kb + u < k1 ? ((const double4*)(F.eg + 8 * (uint64_t)(uint32_t)(kb + u)))[1] : make_double4(0., 0., 0., 0.);
+#pragma unroll
+        for (int u = 0; u < HFV_GRAD_CHUNK; ++u) {
+          if (kb + u >= k1) break;
+          extrema(mk3(bg[u].x, bg[u].y, bg[u].z));
+        }
       }
     } else {
       // chunked as the estimate sweep: a chunk's CSR values, then its faces' shift codes and centroids
