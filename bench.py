@@ -232,29 +232,30 @@ def traffic_for(label, kernel_name):
     return None
 
 
-def flux_kernel_name(fp32, solver, static_geometry=True):
+def flux_kernel_name(fp32, solver, static_geometry=True, fpt=1):
+    """The flux kernel's name as the profiler reports it (fpt: face chunks per block, the launch shape)."""
     return (f"void hfv::k_flux<{'float' if fp32 else 'double'}, {solver}, "
-            f"{'true' if static_geometry else 'false'}>(hfv::Batch)")
+            f"{'true' if static_geometry else 'false'}, {fpt or 1}>(hfv::Batch)")
 
 
 SLOT_KERNELS = {"apply_c2p": ["hfv::k_apply(hfv::Batch, int)"], "mask_predict": ["hfv::k_mask(hfv::Batch, int)"],
-                "wakeup": ["hfv::k_wakeup(hfv::Batch)"], "limit_kick": ["hfv::k_limit(hfv::Batch, int)"]}
+                "wakeup": ["hfv::k_wakeup(hfv::Batch, int)"], "limit_kick": ["hfv::k_limit(hfv::Batch, int)"]}
 
 
-def slot_kernels(slot, fp32, solver, tuning):
+def slot_kernels(slot, fp32, solver, tuning, fpt=1):
     """Kernel names that a timing slot's launches consist of (static geometry; the gradient variants
     the autotuner chose)."""
     if slot == "flux":
-        return [flux_kernel_name(fp32, solver, True)]
+        return [flux_kernel_name(fp32, solver, True, fpt)]
     if slot == "grad_timestep":
         chosen = {c["static_grad"] for c in (tuning or [])} or {0}
         return [f"void hfv::k_grad<{'true' if g else 'false'}>(hfv::Batch, int, int)" for g in sorted(chosen)]
     return SLOT_KERNELS.get(slot, [])
 
 
-def attach_traffic(rl, label, fp32, solver, tuning):
+def attach_traffic(rl, label, fp32, solver, tuning, fpt=1):
     """roofline["traffic"]: HBM bytes per launch of the dominant slot's kernels from profiles/."""
-    names = slot_kernels(rl["kernel"], fp32, solver, tuning)
+    names = slot_kernels(rl["kernel"], fp32, solver, tuning, fpt)
     vals = [traffic_for(label, k) for k in names]
     rl["kernel_name"] = " + ".join(names)
     rl["traffic"] = sum(vals) if vals and all(v is not None for v in vals) else None
@@ -349,7 +350,8 @@ def bench_ensemble(args):
            "n_all": n_all, "members_done": sum(1 for i in i1 if i["done"]),
            "members_failed": [s for s, i in enumerate(i1) if i["error_code"]],
            "roofline": roofline(kt, launches, cu, ff, n_all, fp32=True)}
-    attach_traffic(res["roofline"], "ensemble512", True, fvs.HLLC, tuning)
+    attach_traffic(res["roofline"], "ensemble512", True, fvs.HLLC, tuning,
+                   b.launch_shape()["flux_faces_per_thread"])
     if args.no_gate:  # a profiling run: the timed window only
         res.update({"fp64_batched_value": None, "l1_gate": None, "plain_fp64": None, "speedup_vs_plain_fp64": None,
                     "l1_vs_oracle": None})
@@ -636,13 +638,14 @@ def bench_moving(args, dim):
     loc_full = dfull.get_state(["loc"])["loc"].reshape(3, prob.n).T.copy()
     dfull.close()
     full_us = sfull["kernel_times"]["mesh_rebuild"]["ms"] / max(sfull["kernel_times"]["mesh_rebuild"]["launches"], 1) * 1e3
-    t = time.perf_counter()
-    reps = 3 if dim == 2 else 1
-    for _ in range(reps):
+    host = []
+    for _ in range(3):  # best of 3: a single call can catch the host busy
+        t = time.perf_counter()
         VoronoiMesh(dim, prob.cfg.box, True, loc_full, n_threads=args.threads)
-    host_ms = 1e3 * (time.perf_counter() - t) / reps
+        host.append(time.perf_counter() - t)
+    host_ms = 1e3 * min(host)
     out["rebuild"] = {"partial_device_us": out["fp64"]["per_kernel_us"].get("mesh_rebuild"),
-                      "partial_faces": int(faces), "full_device_us": full_us, "full_host_tessellator_ms": host_ms,
+                      "partial_faces": int(faces), "full_device_us": full_us, "full_host_tessellator_ms": host_ms, "host_timing": "best of 3 calls",
                       "host_threads": args.threads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count())),
                       "full_speedup_vs_host": 1e3 * host_ms / full_us if full_us else None,
                       "device_time_from": "HIP events around the rebuild's kernels (eager launches)"}
@@ -686,7 +689,8 @@ def bench_kh3d(args):
                           tune=args.kh_tune, tuning_from=tf, tune_rounds=args.kh_tune_rounds)
         rl = roofline(s["kernel_times"], s["steps"], s["cell_updates"], s["face_fluxes"], prob.n, fp32=fp32,
                       periodic=True)
-        attach_traffic(rl, f"kh3d_{args.kh_side}_{'fp32' if fp32 else 'fp64'}", fp32, fvs.AIRS, s["autotune"])
+        attach_traffic(rl, f"kh3d_{args.kh_side}_{'fp32' if fp32 else 'fp64'}", fp32, fvs.AIRS, s["autotune"],
+                       s["launch_shape"])
         n_act = s["cell_updates"] / max(s["steps"], 1)
         f_act = s["face_fluxes"] / max(s["steps"], 1)
         sec_step = s["seconds"] / max(s["steps"], 1)
