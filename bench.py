@@ -18,7 +18,7 @@ Printed JSON line: the contract keys plus
   l1_vs_oracle           L1(rho) of sampled FP32-flux members against the FP64 oracle advanced to the
                          member's step count (at equal time)
   l1_gate                L1(rho) of every FP32-flux member against the FP64 batch on the device
-  roofline               dominant kernel of the timed window: the survey's algorithmic bytes of that
+  roofline               dominant kernel of the timed window: the step byte model's algorithmic bytes of that
                          kernel per launch (ALGO below) / its mean duration from HIP events on the
                          context's stream, measured on a replica context replaying the same window
   cpu_baseline           the oracle (C restatement of the reference path, OpenMP at the reference's
@@ -44,7 +44,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 METRIC = "cell-updates/s vs plain FP64 GPU path; p50/p99 job latency at L1 err<1e-5"
 
-# Algorithmic HBM bytes of one OptimalOrderRunner::step on the FP64 SoA (the survey's contract,
+# Algorithmic HBM bytes of one OptimalOrderRunner::step on the FP64 SoA (the step byte model,
 #   B_step = 192 N_all + 1247 N_act + 248 F_act (+8 F_act periodic)),
 # attributed to the kernels that do each part of the step:
 #   mask_predict   192 / cell      predict of every cell: rho, P, 9 gradient components, v_rel; extrap r/w
@@ -84,7 +84,7 @@ PERIODIC_SHIFT = 8  # bytes per active face of a periodic mesh (the flux phase's
 
 
 def step_bytes(n_all, n_act, f_act, fp32=False, periodic=False):
-    """B_step of the survey's contract at the context's storage widths."""
+    """B_step of the step byte model at the context's storage widths."""
     t = ALGO_FP32 if fp32 else ALGO
     return sum(f(n_act, f_act, n_all) for f in t.values()) + (PERIODIC_SHIFT * f_act if periodic else 0)
 
@@ -111,7 +111,7 @@ def roofline(kt, launches_timed, cell_updates, face_fluxes, n_all, fp32=False, p
         "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "avg_launch_us": round(avg_s * 1e6, 3),
         "algorithmic_bytes_per_launch": int(algo), "launches": launches,
         "mean_active_cells": round(n_act, 1), "mean_active_faces": round(f_act, 1), "cells": int(n_all),
-        "byte_model": "survey B_step per kernel (bench.py ALGO%s)" % ("_FP32" if fp32 else ""),
+        "byte_model": "step byte model B_step per kernel (bench.py ALGO%s)" % ("_FP32" if fp32 else ""),
         "per_kernel_ms": {k: round(v["ms"], 3) for k, v in kt.items() if v["launches"]},
     }
 

@@ -75,7 +75,7 @@ def test_trace_summary_window_column(tmp_path, profiles):
     assert line.rstrip().endswith("| 3.50 |")  # (3000 + 4000) / 2 ns = 3.5 us
 
 
-def test_byte_model_is_the_survey_step():
+def test_byte_model_is_the_step_model():
     """The per-kernel attribution sums to B_step = 192 N + 1247 N_act + 248 F_act (+8 F_act periodic) in
     FP64 contexts, and the FP32-flux widths remove exactly the stored-FP32 fields: gradients 120 -> 60 (read
     by predict 72 -> 36 and the flux gather, written by the gradient phase), extrapolations 40 -> 20 (predict
