@@ -142,6 +142,28 @@ def test_moving3d_full_run_parity(ora, case):
     assert np.max(moved) > 1e-3
 
 
+@pytest.mark.parametrize("boundary", [fvs.PERIODIC, fvs.REFLECTIVE])
+def test_moving3d_random_generators_full_run_parity(ora, boundary):
+    """A whole run from uniform random generators (irregular cells, blocks beyond the LDS candidate list)
+    with steered motion, against the oracle's run."""
+    rng = np.random.default_rng(21)
+    prob = moving_kh(boundary, motion=fvs.STEER, side=8, t_end=0.03)
+    loc = rng.random((prob.n, 3))
+    prob = dataclasses.replace(prob, loc=loc,
+                               mesh=VoronoiMesh(3, prob.cfg.box, boundary == fvs.PERIODIC, loc))
+    osim = oracle_sim(prob)
+    n_ref = osim.advance(1 << 40)
+    ref = osim.get_fields()
+    dev = ics.build_device_sim(prob)
+    n_dev = dev.run(1 << 40)
+    assert dev.engine_info()["done"] and dev.engine_info()["error_code"] == 0
+    errs = compare_states(dev.get_state(), ref, prob.n)
+    say(f"[moving3d random {boundary}] steps ref={n_ref} dev={n_dev} errs={ {k: f'{v:.1e}' for k, v in errs.items()} }")
+    assert n_dev == n_ref
+    assert errs.pop("timebin_mismatch") == 0
+    assert max(errs.values()) <= 1e-9, errs
+
+
 def test_moving3d_fp32_flux_l1(ora):
     """The FP32-flux context on the moving 3D mesh (16^3 generators): L1(rho) against the FP64 oracle below
     1e-5 (measured 7e-7).  Generators move with the fluid velocity at the generator, which takes the
