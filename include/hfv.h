@@ -189,7 +189,10 @@ int hfv_create(const hfv_params* params, hfv_ctx** out);
 int hfv_destroy(hfv_ctx* ctx);
 /* Space::volume_calculation output (src/space.rs:236-263): copies the mesh to HBM. update_mask
  * (nullable, [N]) selects the cells whose volume/centroid/search radius are replaced
- * (Particle::update_geometry, src/part.rs:151-158); others only get new face offsets/counts. */
+ * (Particle::update_geometry, src/part.rs:151-158); others only get new face offsets/counts.  The first
+ * call gives every simulation its initial mesh; a simulation with moving generators then rebuilds its
+ * mesh on the device at every step (hfv_step, hfv_run), so a mesh set between its steps is replaced by
+ * the next step's rebuild. */
 int hfv_set_mesh(hfv_ctx* ctx, const hfv_mesh_arrays* mesh, const uint8_t* update_mask);
 /* Particle::from_ic + first_init (src/part.rs:200-217, src/space.rs:571-594): mesh must be set. */
 int hfv_init_from_ic(hfv_ctx* ctx, int64_t n, const double* loc, const double* mass, const double* vel,
