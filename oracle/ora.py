@@ -458,6 +458,7 @@ def face_velocity(loc_l, loc_r, shift, v_l, v_r, fc):
 
 
 def set_threads(n):
+    """OpenMP threads for the oracle (n > 0), or the process's default (n <= 0); returns the count in effect."""
     return lib().ora_sim_set_threads(n)
 
 

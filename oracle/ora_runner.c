@@ -1185,11 +1185,15 @@ void ora_sim_free(ora_sim* s) {
   free(s);
 }
 
+/* OpenMP threads of the oracle's parallel loops (and of every later OpenMP region in the process):
+ * n > 0 threads, n <= 0 the default the process started with.  Returns the count now in effect. */
 int ora_sim_set_threads(int n) {
 #ifdef _OPENMP
   extern void omp_set_num_threads(int);
   extern int omp_get_max_threads(void);
-  if (n > 0) omp_set_num_threads(n);
+  static int initial = 0;
+  if (initial == 0) initial = omp_get_max_threads();
+  omp_set_num_threads(n > 0 ? n : initial);
   return omp_get_max_threads();
 #else
   (void)n;
