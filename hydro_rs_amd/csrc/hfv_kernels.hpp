@@ -534,9 +534,18 @@ __device__ __forceinline__ V3<double> decode_shift(uint8_t code, const double* b
   } while (0)
 #endif
 // Minimum resident blocks per CU (of 4 waves, one per SIMD) the compiler must fit the flux / gradient
-// kernels into (0: its choice).  5 for the flux kernel: the FP32 variants then fit 5 waves per SIMD
-// (96 VGPRs, a few spilled words on cold paths), which measured 12% faster on the 3D case; the FP64
-// variants keep 4 (the target cannot be met without heavy spilling, and the compiler keeps 4).
+// kernels into (0: its choice).  Static-geometry FP32 flux: 6, where the one-chunk variants fit 6 waves
+// per SIMD (76-78 VGPRs, no spills): 3D FP32 flux -10% against 5 waves (96 VGPRs), which was itself 12%
+// faster than the compiler's choice; 7 for its AIRS variants (72 VGPRs, 12 spilled on cold paths): 3D
+// FP32 AIRS flux -6% more, where the 1D HLLC flux of the ensemble lost 0.4% at 7.  The others: 5, which
+// the per-step-geometry variants meet (93-96 VGPRs; at 6 they spill more); the staged FP64 ones keep 4
+// (their LDS allows no more).
+#ifndef HFV_FLUX_WAVES_F32
+#define HFV_FLUX_WAVES_F32 6
+#endif
+#ifndef HFV_FLUX_WAVES_F32_AIRS
+#define HFV_FLUX_WAVES_F32_AIRS 7
+#endif
 #ifndef HFV_FLUX_WAVES
 #define HFV_FLUX_WAVES 5
 #endif
@@ -1471,7 +1480,9 @@ __device__ __forceinline__ void flux_chunk(const Batch& B, const BlockJob J) {
 // block b takes the table's positions b, b + grid, .. (the chunks of a block keep the XCD their positions
 // were placed on whenever the grid is a multiple of 8).
 template <class T, int SOLVER, bool STATIC, int FPT>
-__global__ void __launch_bounds__(HFV_BLOCK, HFV_FLUX_WAVES) k_flux(Batch B) {
+__global__ void __launch_bounds__(HFV_BLOCK, !STATIC || sizeof(T) != 4 ? HFV_FLUX_WAVES
+                                             : SOLVER == SOLVER_AIRS ? HFV_FLUX_WAVES_F32_AIRS : HFV_FLUX_WAVES_F32)
+    k_flux(Batch B) {
 #pragma unroll
   for (int q = 0; q < FPT; ++q) {
     flux_chunk<T, SOLVER, STATIC>(B, B.fjobs[blockIdx.x + q * gridDim.x]);
