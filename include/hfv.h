@@ -257,8 +257,9 @@ int hfv_gravity_accelerations(hfv_ctx* ctx, double* a_out);
 int hfv_step(hfv_ctx* ctx, const hfv_timestep_info* info, uint64_t* ti_next);
 /* Engine::run (src/engine.rs:152-192): advance until max_steps Step sync points ran or t_end.  The
  * device loop drifts generators and rebuilds moving meshes.  A device rebuild that fails (more faces or
- * CSR entries than the context holds; a 2D cell of more than 48 edges; a 3D cell of more than 64 faces or
- * 98 vertices, or a face of more than 24 vertices) stops the simulation with HFV_ERR_MESH. */
+ * CSR entries than the context holds; a 2D cell of more than 48 edges; a 3D cell whose clipping takes more
+ * than 64 planes (the box's 6 it starts from and every bisector that cuts it) or 98 vertices, or a face of
+ * more than 24 vertices) stops the simulation with HFV_ERR_MESH. */
 int hfv_run(hfv_ctx* ctx, int64_t max_steps, int64_t* steps_done);
 int hfv_get_engine_info(hfv_ctx* ctx, hfv_engine_info* out);
 /* Space::status sums (src/space.rs:1248-1280): sum of conserved + pending fluxes per component,

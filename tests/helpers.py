@@ -105,3 +105,34 @@ def synced(prob, k):
     dev.set_mesh(mesh_from_oracle(osim, f, c.dim, c.box, c.boundary == fvs.PERIODIC))
     dev.set_state(f)
     return osim, dev, inf
+
+
+def star_problem(base, ring, boundary, seed=5):
+    """A moving-mesh problem at rest whose first generator (the box centre) has `ring` others on a sphere
+    (3D, Fibonacci points) or circle (2D) of radius 0.08 around it, in a jittered 5^dim lattice (points
+    within 0.15 of the centre dropped): the centre cell has exactly `ring` faces, one per ring point.
+    base: a Problem of the same dimension whose configuration is kept (motion, solver, ...)."""
+    import dataclasses
+
+    dim = base.cfg.dim
+    rng = np.random.default_rng(seed)
+    g = (np.arange(5) + 0.5) / 5
+    bg = np.stack(np.meshgrid(*[g] * dim, indexing="ij"), -1).reshape(-1, dim)
+    bg = bg + rng.uniform(-0.02, 0.02, bg.shape)
+    c = np.full(dim, 0.5)
+    bg = bg[np.linalg.norm(bg - c, axis=1) > 0.15]
+    if dim == 3:
+        k = np.arange(ring) + 0.5
+        th, ph = np.arccos(1 - 2 * k / ring), np.pi * (1 + 5 ** 0.5) * k
+        rim = np.stack([np.sin(th) * np.cos(ph), np.sin(th) * np.sin(ph), np.cos(th)], -1)
+    else:
+        a = 2 * np.pi * (np.arange(ring) + 0.25) / ring
+        rim = np.stack([np.cos(a), np.sin(a)], -1)
+    pts = np.concatenate([c[None], c + 0.08 * rim, bg])
+    n = len(pts)
+    loc = np.zeros((n, 3))
+    loc[:, :dim] = pts
+    cfg = dataclasses.replace(base.cfg, boundary=boundary)
+    mesh = VoronoiMesh(dim, cfg.box, boundary == fvs.PERIODIC, loc)
+    return dataclasses.replace(base, cfg=cfg, loc=loc, mass=np.full(n, 1.0 / n), vel=np.zeros((n, 3)),
+                               u_int=np.full(n, 1.5), mesh=mesh)

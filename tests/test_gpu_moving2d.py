@@ -13,7 +13,7 @@ import pytest
 
 from hydro_rs_amd import fvs, ics
 from hydro_rs_amd.mesh import VoronoiMesh
-from tests.helpers import compare_states, oracle_sim
+from tests.helpers import compare_states, oracle_sim, star_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -61,6 +61,37 @@ def test_device_rebuild_equals_host_tessellator(boundary):
             assert a.shape == b.shape and np.array_equal(a, b), (k, np.max(np.abs(a - b)) if a.shape == b.shape else a.shape)
     # the generators moved: the mesh is not the initial one
     assert not np.array_equal(got["area"][: prob.mesh.n_faces], prob.mesh.area[: got["n_faces"]])
+
+
+@pytest.mark.parametrize("boundary", [fvs.PERIODIC, fvs.REFLECTIVE])
+def test_device_rebuild_cell_of_40_edges(boundary):
+    """A generator ringed by 40 others on a small circle: its 40-edge cell outgrows the first pass's 32
+    edges and is built by the second pass (48 edges), as the host tessellates it."""
+    prob = star_problem(moving_sedov(fvs.PERIODIC, side=8, sync_all=True), 40, boundary)
+    assert prob.mesh.as_dict()["count"][0] == 40
+    dev = ics.build_device_sim(prob)
+    say(f"[rebuild star 40] n={prob.n} boundary={boundary}")
+    dev.run(1)
+    assert dev.engine_info()["error_code"] == 0
+    got, want = dev.get_mesh(), host_mesh_of(dev, prob)
+    assert got["n_faces"] == want["n_faces"], (got["n_faces"], want["n_faces"])
+    for k in ("area", "normal", "centroid", "shift", "left", "right", "offset", "count", "conn", "volume",
+              "cell_centroid", "search_radius"):
+        a, b = np.asarray(got[k]), np.asarray(want[k])
+        assert a.shape == b.shape and np.array_equal(a, b), k
+
+
+def test_device_rebuild_cell_beyond_capacity_stops_with_mesh_error():
+    """A 60-edge cell (more than the 48 a 2D cell holds): the rebuild stops the simulation with
+    HFV_ERR_MESH (-22), as hfv_run documents."""
+    prob = star_problem(moving_sedov(fvs.PERIODIC, side=8, sync_all=True), 60, fvs.PERIODIC)
+    assert prob.mesh.as_dict()["count"][0] == 60
+    dev = ics.build_device_sim(prob)
+    say(f"[rebuild star 60] n={prob.n}")
+    with pytest.raises(fvs.HfvError) as ex:
+        dev.run(1)
+    assert ex.value.code == -22
+    assert dev.engine_info()["error_code"] == -22
 
 
 @pytest.mark.parametrize("case", ["periodic_fluid", "reflective_fluid", "periodic_steer", "periodic_sync_all"])

@@ -14,7 +14,7 @@ import pytest
 
 from hydro_rs_amd import fvs, ics
 from hydro_rs_amd.mesh import VoronoiMesh
-from tests.helpers import compare_states, oracle_sim
+from tests.helpers import compare_states, oracle_sim, star_problem
 
 pytestmark = pytest.mark.gpu
 
@@ -117,6 +117,32 @@ def test_device_rebuild3d_clustered_generators(boundary):
     dev.run(1)
     assert dev.engine_info()["error_code"] == 0
     assert_same_mesh(dev.get_mesh(), host_mesh_of(dev, prob))
+
+
+@pytest.mark.parametrize("boundary", [fvs.PERIODIC, fvs.REFLECTIVE])
+def test_device_rebuild3d_cell_of_40_faces(boundary):
+    """A generator ringed by 40 others on a small sphere: its cell has 40 faces (46 of the 64 planes a cell
+    holds, with the box's 6 it starts from), and the rebuild gives the host tessellation."""
+    prob = star_problem(moving_kh(fvs.PERIODIC, sync_all=True, side=6), 40, boundary)
+    assert prob.mesh.as_dict()["count"][0] == 40
+    dev = ics.build_device_sim(prob)
+    say(f"[rebuild3d star 40] n={prob.n} boundary={boundary}")
+    dev.run(1)
+    assert dev.engine_info()["error_code"] == 0
+    assert_same_mesh(dev.get_mesh(), host_mesh_of(dev, prob))
+
+
+def test_device_rebuild3d_cell_beyond_capacity_stops_with_mesh_error():
+    """A cell of 91 faces needs more planes than a 3D cell holds (64, the box's 6 included): the rebuild
+    stops the simulation with HFV_ERR_MESH (-22), as hfv_run documents, and emits no partial cell."""
+    prob = star_problem(moving_kh(fvs.PERIODIC, sync_all=True, side=6), 91, fvs.PERIODIC)
+    assert prob.mesh.as_dict()["count"][0] == 91
+    dev = ics.build_device_sim(prob)
+    say(f"[rebuild3d star 91] n={prob.n}")
+    with pytest.raises(fvs.HfvError) as ex:
+        dev.run(1)
+    assert ex.value.code == -22
+    assert dev.engine_info()["error_code"] == -22
 
 
 @pytest.mark.parametrize("case", ["periodic_fluid", "reflective_fluid", "periodic_steer", "periodic_sync_all"])
