@@ -136,3 +136,26 @@ def star_problem(base, ring, boundary, seed=5):
     mesh = VoronoiMesh(dim, cfg.box, boundary == fvs.PERIODIC, loc)
     return dataclasses.replace(base, cfg=cfg, loc=loc, mass=np.full(n, 1.0 / n), vel=np.zeros((n, 3)),
                                u_int=np.full(n, 1.5), mesh=mesh)
+
+
+def advected_wave(dim, side, vel, t_end, seed=7, precision=fvs.FP64):
+    """A density wave rho = 1 + 0.3 prod_a sin(2 pi x_a) carried by a uniform flow `vel` at uniform pressure
+    through the unit periodic box, generators moving with the fluid (HLLC, gamma 1.4): over t_end a
+    fraction vel_a * t_end of the generators drifts through each box face (box_wrap, part.rs:449-455)."""
+    from hydro_rs_amd import ics
+
+    rng = np.random.default_rng(seed)
+    loc = ics.jittered_grid(dim, side, (1.0, 1.0, 1.0), 0.1, rng)
+    n = loc.shape[0]
+    rho = 1.0 + 0.3 * np.prod(np.sin(2 * np.pi * loc[:, :dim]), axis=1)
+    v = np.zeros((n, 3))
+    v[:, :dim] = vel
+    cfg = fvs.Config(dim=dim, box=(1.0, 1.0, 1.0), gamma=1.4, cfl=0.3, solver=fvs.HLLC, boundary=fvs.PERIODIC,
+                     motion=fvs.FLUID, t_end=t_end, precision=precision)
+    return ics._finish(f"advected_wave_{dim}d", cfg, loc, rho, v, np.ones(n))
+
+
+def wrapped_fraction(loc0, loc1, vel):
+    """Per axis with a flow, the fraction of generators that drifted through the box face downstream."""
+    return [float(np.mean(loc1[:, a] < loc0[:, a]) if u > 0 else np.mean(loc1[:, a] > loc0[:, a]))
+            for a, u in enumerate(vel) if u != 0]
