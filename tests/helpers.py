@@ -159,3 +159,44 @@ def wrapped_fraction(loc0, loc1, vel):
     """Per axis with a flow, the fraction of generators that drifted through the box face downstream."""
     return [float(np.mean(loc1[:, a] < loc0[:, a]) if u > 0 else np.mean(loc1[:, a] > loc0[:, a]))
             for a, u in enumerate(vel) if u != 0]
+
+
+def wall_box(dim, side, precision=fvs.FP64):
+    """Gas at rest pressure and density (1, 1) in the unit box with reflective walls, flowing out along
+    every axis at 0.5 (v_a = 0.5 sign(x_a - 0.5)), generators moving with it: a side^dim lattice (inner
+    coordinates jittered by 0.05 dx) whose outer layers sit 0.05 dx from the walls, so that they drift
+    through them within a few steps and are reflected (Particle::box_reflect, part.rs:457-469)."""
+    from hydro_rs_amd import ics
+
+    dx = 1.0 / side
+    g = (np.arange(side) + 0.5) * dx
+    g[0], g[-1] = 0.05 * dx, 1.0 - 0.05 * dx
+    rng = np.random.default_rng(9)
+    pts = np.stack(np.meshgrid(*[g] * dim, indexing="ij"), -1).reshape(-1, dim)
+    inner = (pts > 0.06 * dx) & (pts < 1 - 0.06 * dx)
+    pts = np.where(inner, pts + rng.uniform(-0.05, 0.05, pts.shape) * dx, pts)
+    n = len(pts)
+    loc = np.zeros((n, 3))
+    loc[:, :dim] = pts
+    vel = np.zeros((n, 3))
+    vel[:, :dim] = 0.5 * np.sign(pts - 0.5)
+    cfg = fvs.Config(dim=dim, box=(1.0, 1.0, 1.0), gamma=1.4, cfl=0.3, solver=fvs.HLLC, boundary=fvs.REFLECTIVE,
+                     motion=fvs.FLUID, t_end=0.05, precision=precision)
+    return ics._finish(f"wall_box_{dim}d", cfg, loc, np.ones(n), vel, np.ones(n))
+
+
+def reflections(osim, steps, dim):
+    """Generator coordinates reflected at a wall during `steps` oracle steps (they moved against the
+    velocity they drifted with)."""
+    n = osim.n
+    count = 0
+    prev = osim.get_fields()
+    for _ in range(steps):
+        osim.advance(1)
+        f = osim.get_fields()
+        x0 = np.asarray(prev["loc"]).reshape(3, -1)[:dim, :n]
+        x1 = np.asarray(f["loc"]).reshape(3, -1)[:dim, :n]
+        v = np.asarray(prev["v"]).reshape(3, -1)[:dim, :n]
+        count += int(np.sum((v > 0) & (x1 < x0)) + np.sum((v < 0) & (x1 > x0)))
+        prev = f
+    return count

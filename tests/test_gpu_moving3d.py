@@ -14,7 +14,8 @@ import pytest
 
 from hydro_rs_amd import fvs, ics
 from hydro_rs_amd.mesh import VoronoiMesh
-from tests.helpers import advected_wave, compare_states, oracle_sim, star_problem, wrapped_fraction
+from tests.helpers import (advected_wave, compare_states, oracle_sim, reflections, star_problem, wall_box,
+                           wrapped_fraction)
 
 pytestmark = pytest.mark.gpu
 
@@ -243,3 +244,25 @@ def test_moving3d_periodic_wrap_full_run_parity(ora):
     assert errs.pop("timebin_mismatch") == 0
     assert max(errs.values()) <= 1e-9, errs
     assert min(wrapped) >= 1.0 / 6.0 - 1e-12
+
+
+def test_moving3d_wall_reflection_parity(ora):
+    """Generators drifting out through every reflective wall are reflected with their primitives, gradients
+    and kept rate (k_drift; box_reflect, reflect_quantities, reflect_gradients) as in the oracle: 7 steps
+    (364 reflected coordinates) within the whole-run tolerance, then the reference's negative-pressure
+    panic at the same step with the same code."""
+    prob = wall_box(3, 6)
+    osim = oracle_sim(prob)
+    assert reflections(osim, 7, 3) == 364
+    dev = ics.build_device_sim(prob)
+    assert dev.run(7) == 7
+    errs = compare_states(dev.get_state(), osim.get_fields(), prob.n)
+    say(f"[wall 3d] 7 steps: errs={ {k: f'{v:.1e}' for k, v in errs.items()} }")
+    assert errs.pop("timebin_mismatch") == 0
+    assert max(errs.values()) <= 1e-9, errs
+    with pytest.raises(RuntimeError) as oe:
+        osim.advance(1 << 40)
+    with pytest.raises(fvs.HfvError) as de:
+        dev.run(1 << 40)
+    assert de.value.code == int(str(oe.value).split()[-1]) == -4
+    assert dev.engine_info()["step_count"] == osim.info()["step_count"]
