@@ -190,3 +190,25 @@ def test_moving2d_wall_reflection_parity(ora):
         dev.run(1 << 40)
     assert de.value.code == int(str(oe.value).split()[-1]) == -4
     assert dev.engine_info()["step_count"] == osim.info()["step_count"]
+
+
+@pytest.mark.parametrize("case", ["wall", "wrap"])
+def test_moving2d_drift_through_box_fp32(ora, case):
+    """The FP32-flux context on the same drifts (reflective walls for 9 steps, periodic wraps to the end):
+    it sums each generator's deferred predicts with the rate it had before a reflection and extrapolates the
+    reflected state afterwards; L1(rho) against the FP64 oracle < 1e-5."""
+    if case == "wall":
+        prob, steps = wall_box(2, 10, precision=fvs.FP32_FLUX), 9
+    else:
+        prob, steps = advected_wave(2, 12, (1.0, 0.6), 0.3, precision=fvs.FP32_FLUX), 1 << 40
+    ref = oracle_sim(dataclasses.replace(prob, cfg=dataclasses.replace(prob.cfg, precision=fvs.FP64)))
+    dev = ics.build_device_sim(prob)
+    n_dev = dev.run(steps)
+    ref.advance(steps)
+    assert dev.engine_info()["error_code"] == 0
+    if case == "wall":  # the same step of the same timeline (the wraps run to t_end)
+        assert dev.engine_info()["ti_current"] == ref.info()["ti_current"]
+    rho = ref.get_fields()["prim"][: prob.n]
+    l1 = float(np.mean(np.abs(dev.get_state(["prim"])["prim"][: prob.n] - rho)))
+    say(f"[2d {case} fp32] {n_dev} steps, L1(rho) vs oracle {l1:.2e}")
+    assert l1 < 1e-5
