@@ -522,17 +522,21 @@ __device__ __forceinline__ V3<double> decode_shift(uint8_t code, const double* b
 // registers) at once.
 #define HFV_STAGE() asm volatile("" ::: "memory")
 // The flux kernel's stage barriers can be compiled out (HFV_FLUX_STAGES 0) to let the scheduler overlap
-// the stages' gathers at the cost of registers.
+// the stages' gathers at the cost of registers.  Static-geometry FP32 HLLC faces run without them (the
+// 1D/2D ensemble's flux -1.1%; the 3D AIRS flux was 1-2% slower without them, FP64 and FP32)
 #ifndef HFV_FLUX_STAGES
 #define HFV_FLUX_STAGES 1
 #endif
-#if HFV_FLUX_STAGES
-#define HFV_FSTAGE() HFV_STAGE()
-#else
-#define HFV_FSTAGE() \
-  do {               \
+template <class T, int SOLVER, bool STATIC>
+constexpr bool flux_stages() {
+  return HFV_FLUX_STAGES && !(STATIC && sizeof(T) == 4 && SOLVER == SOLVER_HLLC);
+}
+#define HFV_FSTAGE()                                  \
+  do {                                                \
+    if constexpr (flux_stages<T, SOLVER, STATIC>()) { \
+      HFV_STAGE();                                    \
+    }                                                 \
   } while (0)
-#endif
 // Minimum resident blocks per CU (of 4 waves, one per SIMD) the compiler must fit the flux / gradient
 // kernels into (0: its choice).  Static-geometry FP32 flux: 6, where the one-chunk variants fit 6 waves
 // per SIMD (76-78 VGPRs, no spills): 3D FP32 flux -10% against 5 waves (96 VGPRs), which was itself 12%
