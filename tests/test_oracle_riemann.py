@@ -219,3 +219,26 @@ def test_consistency_equal_states(ora, kind):
         exp = np.array([rho * vn, *(rho * vn * v + p * n), (E + p) * vn])
         np.testing.assert_allclose(f, exp, rtol=1e-10, atol=1e-12)
 
+
+
+@pytest.mark.parametrize("kind", ["EXACT", "HLLC"])
+def test_isolated_contact_is_upwinded(ora, kind):
+    """An isolated contact or shear layer (equal pressure and normal velocity, different densities and
+    transverse velocities): the exact solver and HLLC (whose middle wave is the contact) both return the
+    physical Euler flux of the upwind state, in closed form; the upwind side is the left one for u_n > 0."""
+    rng = np.random.default_rng(11)
+    for _ in range(20):
+        n = rng.normal(size=3)
+        n /= np.linalg.norm(n)
+        un = rng.choice([-1.0, 1.0]) * rng.uniform(0.05, 1.0)
+        p = rng.uniform(0.1, 3.0)
+        t1, t2 = rng.normal(size=3), rng.normal(size=3)
+        vl = un * n + (t1 - (t1 @ n) * n)  # transverse parts differ, normal parts equal
+        vr = un * n + (t2 - (t2 @ n) * n)
+        wl, wr = prim(rng.uniform(0.1, 3.0), vl, p), prim(rng.uniform(0.1, 3.0), vr, p)
+        f = ora.solve_for_flux(getattr(ora, kind), 1.4, wl, wr, np.zeros(3), n)
+        w = wl if un > 0 else wr
+        rho, v = w[0], w[1:4]
+        E = p / 0.4 + 0.5 * rho * v @ v
+        exp = np.array([rho * un, *(rho * un * v + p * n), (E + p) * un])
+        np.testing.assert_allclose(f, exp, rtol=1e-9, atol=1e-12)
