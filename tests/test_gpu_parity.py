@@ -57,6 +57,11 @@ CASES = {
     # pressure ratio 100 at the interface: AIRS hands those faces to the exact-solver fallback pass
     "tube1d_airs": lambda: ics.riemann_tube_1d(300, 1.0, 10.0, 0.0, 0.125, 0.1, 0.0, 1.4, solver=fvs.AIRS,
                                                t_end=0.05),
+    # the remaining solvers of the trait (tsrs.rs, trrs.rs, linear_advection.rs)
+    "tube1d_tsrs": lambda: ics.riemann_tube_1d(300, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, solver=fvs.TSRS, t_end=0.05),
+    "tube1d_trrs": lambda: ics.riemann_tube_1d(300, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, solver=fvs.TRRS, t_end=0.05),
+    "tube1d_linadv": lambda: _with(ics.riemann_tube_1d(300, 1.0, 1.0, 0.0, 0.125, 0.1, 0.0, 1.4, t_end=0.05),
+                                   solver=fvs.LINEAR_ADVECTION, linadv_velocity=(0.7, 0.0, 0.0)),
 }
 
 
@@ -178,7 +183,9 @@ def test_step_parity(ora, case, k):
                                         ("tube1d_isothermal", 1 << 40), ("tube1d_n3", 1 << 40),
                                         ("sedov2d_side4", 1 << 40), ("kh3d_side3", 1 << 40),
                                         ("sedov2d", 400), ("tube2d_reflective_exact", 1 << 40),
-                                        ("tube2d_vacuum_hllc", 1 << 40), ("blast3d_reflective", 1 << 40)])
+                                        ("tube2d_vacuum_hllc", 1 << 40), ("blast3d_reflective", 1 << 40),
+                                        ("tube1d_tsrs", 1 << 40), ("tube1d_trrs", 1 << 40),
+                                        ("tube1d_linadv", 1 << 40)])
 def test_full_run_parity(ora, case, steps):
     say(f"[run] case={case}")
     prob = CASES[case]()

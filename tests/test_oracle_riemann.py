@@ -242,3 +242,68 @@ def test_isolated_contact_is_upwinded(ora, kind):
         E = p / 0.4 + 0.5 * rho * v @ v
         exp = np.array([rho * un, *(rho * un * v + p * n), (E + p) * un])
         np.testing.assert_allclose(f, exp, rtol=1e-9, atol=1e-12)
+
+
+def _two_wave_case(rng, eps, apart):
+    """Nearly equal states whose normal velocities are -eps/+eps (apart: the flows separate, two
+    rarefactions) or +eps/-eps (they collide, two shocks)."""
+    rho, p = rng.uniform(0.5, 2.0), rng.uniform(0.5, 2.0)
+    s = -1.0 if apart else 1.0
+    return prim(rho, s * eps, p), prim(rho * (1.0 + 0.3 * eps), -s * eps, p * (1.0 + 0.2 * eps))
+
+
+def _star_pressure(ora, wl, wr, g=1.4):
+    a_l = ora.sound_speed(g, wl[4], 1.0 / wl[0])
+    a_r = ora.sound_speed(g, wr[4], 1.0 / wr[0])
+    e, star = ora.exact_star(g, wl, wr, wl[1], wr[1], a_l, a_r)
+    assert e == 0
+    return star[3]
+
+
+def test_trrs_is_exact_for_two_rarefactions(ora):
+    """When both waves are rarefactions the two-rarefaction estimate is the exact star pressure (trrs.rs:
+    the rarefaction relations solved for p* in closed form): its flux equals the exact solver's, on
+    random separating flows and on Toro's test 2 (p* = 0.001894)."""
+    rng = np.random.default_rng(5)
+    cases = [_two_wave_case(rng, eps, True) for eps in rng.uniform(0.05, 0.8, 20)]
+    cases.append((prim(1.0, -2.0, 0.4), prim(1.0, 2.0, 0.4)))
+    for wl, wr in cases:
+        assert _star_pressure(ora, wl, wr) < min(wl[4], wr[4])  # two rarefactions indeed
+        fe = ora.solve_for_flux(ora.EXACT, 1.4, wl, wr, np.zeros(3), X)
+        ft = ora.solve_for_flux(ora.TRRS, 1.4, wl, wr, np.zeros(3), X)
+        np.testing.assert_allclose(ft, fe, rtol=1e-12, atol=1e-14 * np.max(np.abs(fe)))
+
+
+def test_tsrs_converges_to_exact_at_third_order(ora):
+    """For colliding flows (two shocks) the two-shock estimate (tsrs.rs) differs from the exact solution
+    by O(eps^3) in the wave strength eps (shock curves evaluated at the PVRS guess, which is itself
+    second-order accurate): each tenfold weaker pair of shocks brings the flux at least 500x closer to
+    the exact solver's.  A wrong term would leave an O(eps) or O(1) difference."""
+    rng = np.random.default_rng(6)
+    for _ in range(6):
+        seed = int(rng.integers(1 << 30))
+        d = []
+        for eps in (1e-1, 1e-2, 1e-3):
+            wl, wr = _two_wave_case(np.random.default_rng(seed), eps, False)
+            assert _star_pressure(ora, wl, wr) > max(wl[4], wr[4])  # two shocks indeed
+            fe = ora.solve_for_flux(ora.EXACT, 1.4, wl, wr, np.zeros(3), X)
+            ft = ora.solve_for_flux(ora.TSRS, 1.4, wl, wr, np.zeros(3), X)
+            d.append(np.max(np.abs(ft - fe)) / np.max(np.abs(fe)))
+        assert d[0] / d[1] > 500 and d[1] / d[2] > 500, d
+
+
+def test_linear_advection_upwinds_the_primitive_fields(ora):
+    """linear_advection.rs:17-33: v = (velocity - interface velocity) . n; the flux is v times the upwind
+    state's (density, velocity, pressure) placed in the conserved slots as they are (left for v > 0, right
+    otherwise, v = 0 included).  Cases where a flipped interface velocity or swapped side gives another
+    answer."""
+    wl, wr = prim(1.3, (0.2, -0.1, 0.4), 0.7), prim(0.6, (-0.5, 0.3, 0.1), 1.9)
+    n = np.array([0.6, 0.8, 0.0])
+    adv = np.array([1.0, 0.5, -2.0])
+    for iv, side in (((0.0, 0.0, 0.0), "L"), ((2.5, 0.0, 0.0), "R"), ((0.0, 1.5, 0.0), "R"), ((1.0, 0.5, 7.0), "R"),
+                     ((-1.0, 0.0, 0.0), "L")):
+        iv = np.asarray(iv)
+        v = (adv - iv) @ n
+        assert (v > 0) == (side == "L")
+        f = ora.solve_for_flux(ora.LINADV, 1.4, wl, wr, iv, n, velocity=adv)
+        np.testing.assert_array_equal(f, v * (wl if side == "L" else wr))
