@@ -306,4 +306,6 @@ def test_linear_advection_upwinds_the_primitive_fields(ora):
         v = (adv - iv) @ n
         assert (v > 0) == (side == "L")
         f = ora.solve_for_flux(ora.LINADV, 1.4, wl, wr, iv, n, velocity=adv)
-        np.testing.assert_array_equal(f, v * (wl if side == "L" else wr))
+        exp = v * (wl if side == "L" else wr)
+        np.testing.assert_array_equal(f, exp)
+        np.testing.assert_array_equal(np.signbit(f), np.signbit(exp))  # at v = 0: the signed zeros of the side
